@@ -1,0 +1,31 @@
+/* CPU restatement of the CodeNS residual-and-update path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the tests, the smoke check and the CPU-baseline
+ * leg of bench.py use it as the checker.  The product never links it.
+ *
+ * Arrays use the reference storage (crdms.f90:126-153): SoA over all blocks,
+ * leading dimension ldv (= ip11) for multi-variable arrays, 0-based flat
+ * index npc + (i-id1) + (j-jd1)*nid + (k-kd1)*nid*njd.  sn points at the
+ * block's own metric slice sn(lgsnlt,3,3) (atsch_num.f90:623).
+ */
+#ifndef CODENS_ORACLE_H
+#define CODENS_ORACLE_H
+#include <stdint.h>
+
+typedef struct {
+  int32_t npc, id1, id2, jd1, jd2, kd1, kd2;
+  int32_t ii1, ii2, jj1, jj2, kk1, kk2;
+  int32_t nnn, pad0, pad1;
+} orc_block;
+
+typedef struct {
+  double gam, gam1, pinfl, pr, prt, cp, reynz, tnz;
+  int32_t iflu, pad;
+} orc_phys;
+
+typedef struct {
+  int32_t muscl, ilim;
+  double xk;
+} orc_sch;
+
+#endif

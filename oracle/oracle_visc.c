@@ -1,0 +1,351 @@
+/* Oracle: viscous stresses (zgrad2 + zfluto), matrix-free point-Jacobi
+ * implicit phase (implimf_3d), boundary ghost filling.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ */
+#include <math.h>
+#include <float.h>
+#include <stdlib.h>
+#include "oracle.h"
+
+#define NID(b) ((b)->id2 - (b)->id1 + 1)
+#define NJD(b) ((b)->jd2 - (b)->jd1 + 1)
+static inline long IDX(const orc_block *b, int i, int j, int k) {
+  long nid = NID(b), nijd = nid * NJD(b);
+  return b->npc + (i - b->id1) + (long)(j - b->jd1) * nid + (long)(k - b->kd1) * nijd;
+}
+static inline long INC(const orc_block *b, int d) {
+  long nid = NID(b);
+  return d == 0 ? 1 : (d == 1 ? nid : nid * NJD(b));
+}
+#define SN(sn, b, m, d, c) (sn)[((long)(c) * 3 + (d)) * (b)->nnn + (m)]
+
+/* Green-Gauss gradients of (u, v, w, T): zgrad2.f90:148-485, with the
+ * face weights cmu* = 1 (inivec.f90:107-115).  g: 12 block-local arrays of
+ * nnn entries, order dvxx dvxy dvxz dvyx dvyy dvyz dvzx dvzy dvzz dtx dty dtz
+ * (dv<a><b> = d v_a / d x_b). */
+void orc_zgrad2(const orc_block *b, const double *sn, const double *vol, const double *s, long ldv,
+                const double *temp, double *g) {
+  long n0 = b->npc, nnn = b->nnn;
+  int i1 = b->ii1, i2 = b->ii2, j1 = b->jj1, j2 = b->jj2, k1 = b->kk1, k2 = b->kk2;
+  double *vx = calloc(nnn, 8), *vy = calloc(nnn, 8), *vz = calloc(nnn, 8);
+  for (int k = k1 - 1; k <= k2; k++)
+    for (int j = j1 - 1; j <= j2; j++)
+      for (int i = i1 - 1; i <= i2; i++) {
+        long n = IDX(b, i, j, k), m = n - n0;
+        vx[m] = s[ldv + n] / s[n];
+        vy[m] = s[2 * ldv + n] / s[n];
+        vz[m] = s[3 * ldv + n] / s[n];
+      }
+  for (long m = 0; m < 12 * nnn; m++) g[m] = 0.;
+  double *G[12];
+  for (int c = 0; c < 12; c++) G[c] = g + c * nnn;
+  const double eps = 0.00001;
+  for (int d = 2; d >= 0; d--) { /* k faces, then j, then i (zgrad2.f90:181-457) */
+    long inc = INC(b, d);
+    int lo[3] = {i1, j1, k1}, hi[3] = {i2 - 1, j2 - 1, k2 - 1};
+    int a1 = d == 0 ? i1 : (d == 1 ? j1 : k1), a2 = d == 0 ? i2 : (d == 1 ? j2 : k2);
+    lo[d] = a1 + 1; hi[d] = a2 - 1;
+    /* interior faces; the j sweep runs j outer, k inner (:277-278) */
+    int order_jk = (d == 1);
+    int outer_lo = order_jk ? lo[1] : lo[2], outer_hi = order_jk ? hi[1] : hi[2];
+    int mid_lo = order_jk ? lo[2] : lo[1], mid_hi = order_jk ? hi[2] : hi[1];
+    for (int o = outer_lo; o <= outer_hi; o++)
+      for (int q = mid_lo; q <= mid_hi; q++)
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          int j = order_jk ? o : q, k = order_jk ? q : o;
+          long n = IDX(b, i, j, k), m = n - n0;
+          double sx = SN(sn, b, m, d, 0), sy = SN(sn, b, m, d, 1), sz = SN(sn, b, m, d, 2);
+          double fv[4] = {vx[m] + vx[m - inc], vy[m] + vy[m - inc], vz[m] + vz[m - inc],
+                          temp[n] + temp[n - inc]};
+          for (int a = 0; a < 4; a++) {
+            double t0 = fv[a] * sx, t1 = fv[a] * sy, t2 = fv[a] * sz;
+            G[3 * a][m] = G[3 * a][m] - t0;
+            G[3 * a + 1][m] = G[3 * a + 1][m] - t1;
+            G[3 * a + 2][m] = G[3 * a + 2][m] - t2;
+            G[3 * a][m - inc] = G[3 * a][m - inc] + t0;
+            G[3 * a + 1][m - inc] = G[3 * a + 1][m - inc] + t1;
+            G[3 * a + 2][m - inc] = G[3 * a + 2][m - inc] + t2;
+          }
+        }
+    /* boundary faces carry twice the ghost value (:231-269 and siblings) */
+    for (int side = 0; side < 2; side++) {
+      int l2[3] = {i1, j1, k1}, h2[3] = {i2 - 1, j2 - 1, k2 - 1};
+      l2[d] = h2[d] = side == 0 ? a1 : a2;
+      for (int k = l2[2]; k <= h2[2]; k++)
+        for (int j = l2[1]; j <= h2[1]; j++)
+          for (int i = l2[0]; i <= h2[0]; i++) {
+            long n = IDX(b, i, j, k), m = n - n0;
+            long mg = side == 0 ? m - inc : m, mr = side == 0 ? m : m - inc;
+            double gv[4] = {vx[mg], vy[mg], vz[mg], temp[mg + n0]};
+            for (int a = 0; a < 4; a++)
+              for (int c = 0; c < 3; c++) {
+                double t = SN(sn, b, m, d, c) * 2 * gv[a];
+                if (side == 0) G[3 * a + c][mr] = G[3 * a + c][mr] - t;
+                else G[3 * a + c][mr] = G[3 * a + c][mr] + t;
+              }
+          }
+    }
+  }
+  /* scaling by 1/(2 vol), guarded for vol <= 0 (:463-485), over the
+   * contiguous storage range first..last real cell */
+  for (long n = IDX(b, i1, j1, k1); n <= IDX(b, i2 - 1, j2 - 1, k2 - 1); n++) {
+    long m = n - n0;
+    double ts = copysign(0.5, -vol[n]);
+    double vols = (0.5 + ts) * eps + (0.5 - ts) * vol[n];
+    double c0 = .5 / (vols + DBL_MIN);
+    for (int c = 0; c < 12; c++) G[c][m] = G[c][m] * c0;
+  }
+  free(vx); free(vy); free(vz);
+}
+
+/* zfluto.f90:104-121.  tq: 9 arrays of ldv (txx txy txz tyy tyz tzz qx qy qz). */
+void orc_zfluto(const orc_block *b, const orc_phys *ph, const double *mu, const double *mut,
+                double *tq, long ldv, const double *g) {
+  long n0 = b->npc, nnn = b->nnn;
+  const double ds3 = 2. / 3.;
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - n0;
+        double xx = g[m], xy = g[nnn + m], xz = g[2 * nnn + m], yx = g[3 * nnn + m], yy = g[4 * nnn + m];
+        double yz = g[5 * nnn + m], zx = g[6 * nnn + m], zy = g[7 * nnn + m], zz = g[8 * nnn + m];
+        double mt = mu[n] + mut[n];
+        tq[n] = ds3 * mt * (2 * xx - yy - zz);
+        tq[3 * ldv + n] = ds3 * mt * (2 * yy - xx - zz);
+        tq[5 * ldv + n] = ds3 * mt * (2 * zz - xx - yy);
+        tq[ldv + n] = mt * (xy + yx);
+        tq[2 * ldv + n] = mt * (xz + zx);
+        tq[4 * ldv + n] = mt * (yz + zy);
+        double kq = ph->cp * (mu[n] / ph->pr + mut[n] / ph->prt);
+        tq[6 * ldv + n] = kq * g[9 * nnn + m];
+        tq[7 * ldv + n] = kq * g[10 * nnn + m];
+        tq[8 * ldv + n] = kq * g[11 * nnn + m];
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Boundary ghosts.  Face lists hold flat indices of ghost / adjacent real
+ * cell / coincident remote real cell, in the reference's listing order. */
+
+/* rbvc.f90:84-117 (no periodic rotation): ghost = face average of the two
+ * real cells, also for p, T, c. */
+void orc_bc_coincident(long nf, const long *ghost, const long *inner, const long *remote, double *v,
+                       long ldv, double *ps, double *temp, double *cson) {
+  for (long f = 0; f < nf; f++) {
+    long nd = ghost[f], ndm = inner[f], nr = remote[f];
+    for (int e = 0; e < 5; e++) v[e * ldv + nd] = 0.5 * (v[e * ldv + ndm] + v[e * ldv + nr]);
+    ps[nd] = 0.5 * (ps[ndm] + ps[nr]);
+    temp[nd] = 0.5 * (temp[ndm] + temp[nr]);
+    cson[nd] = 0.5 * (cson[ndm] + cson[nr]);
+  }
+}
+
+/* clpara.f90:44-58: adiabatic no-slip wall. */
+void orc_bc_wall(long nf, const long *ghost, const long *inner, double *v, long ldv, double *ps,
+                 double *temp, double *cson) {
+  for (long f = 0; f < nf; f++) {
+    long nc = ghost[f], ni = inner[f];
+    double rhoe1 = v[4 * ldv + ni] - 0.5 * (v[ldv + ni] * v[ldv + ni] + v[2 * ldv + ni] * v[2 * ldv + ni] +
+                                            v[3 * ldv + ni] * v[3 * ldv + ni]) / v[ni];
+    v[nc] = v[ni];
+    v[ldv + nc] = 0.;
+    v[2 * ldv + nc] = 0.;
+    v[3 * ldv + nc] = 0.;
+    v[4 * ldv + nc] = rhoe1;
+    ps[nc] = ps[ni];
+    temp[nc] = temp[ni];
+    cson[nc] = cson[ni];
+  }
+}
+
+/* rbve.f90 copy then clnrd.f90:60-134: non-reflecting far field against the
+ * state w0 = (rho, rho u, rho v, rho w, rho E) (utnrd.f90 without rotation).
+ * nx, ny, nz: unit normals pointing into the domain. */
+void orc_bc_farfield(long nf, const long *ghost, const long *inner, const double *nxn,
+                     const double *nyn, const double *nzn, const double *w0, const orc_phys *ph,
+                     double *v, long ldv, double *ps, double *temp, double *cson) {
+  const double gam = ph->gam, gam1 = ph->gam1, pinfl = ph->pinfl;
+  for (long f = 0; f < nf; f++) {
+    long nl = ghost[f], ni = inner[f];
+    for (int e = 0; e < 5; e++) v[e * ldv + nl] = v[e * ldv + ni];
+    double nx = nxn[f], ny = nyn[f], nz = nzn[f];
+    double ro0 = v[nl];
+    double ps0 = gam1 * (v[4 * ldv + nl] - 0.5 * (v[ldv + nl] * v[ldv + nl] + v[2 * ldv + nl] * v[2 * ldv + nl] +
+                                                  v[3 * ldv + nl] * v[3 * ldv + nl]) / v[nl] - pinfl);
+    double roc0 = sqrt(gam * ps0 * ro0);
+    double roqn0 = v[ldv + nl] * nx + v[2 * ldv + nl] * ny + v[3 * ldv + nl] * nz;
+    double epsm = .5 + copysign(.5, roc0 - roqn0);
+    double eps0 = .5 + copysign(.5, -roqn0);
+    double epsp = .5 + copysign(.5, -roc0 - roqn0);
+    double rod = w0[0];
+    double qxd = w0[1] / rod, qyd = w0[2] / rod, qzd = w0[3] / rod;
+    double pd = gam1 * (w0[4] - .5 * rod * (qxd * qxd + qyd * qyd + qzd * qzd) - pinfl);
+    double qnd = qxd * nx + qyd * ny + qzd * nz;
+    double qtxd = qxd - qnd * nx, qtyd = qyd - qnd * ny, qtzd = qzd - qnd * nz;
+    double ros = v[nl];
+    double qxs = v[ldv + nl] / ros, qys = v[2 * ldv + nl] / ros, qzs = v[3 * ldv + nl] / ros;
+    double p_s = gam1 * (v[4 * ldv + nl] - .5 * ros * (qxs * qxs + qys * qys + qzs * qzs) - pinfl);
+    double qns = qxs * nx + qys * ny + qzs * nz;
+    double qtxs = qxs - qns * nx, qtys = qys - qns * ny, qtzs = qzs - qns * nz;
+    double qtx = eps0 * qtxs + (1. - eps0) * qtxd;
+    double qty = eps0 * qtys + (1. - eps0) * qtyd;
+    double qtz = eps0 * qtzs + (1. - eps0) * qtzd;
+    double am = epsm * (p_s - roc0 * qns) + (1. - epsm) * (pd - roc0 * qnd);
+    double ap = epsp * (p_s + roc0 * qns) + (1. - epsp) * (pd + roc0 * qnd);
+    double qn = 0.5 * (ap - am) / roc0;
+    double p = 0.5 * (ap + am);
+    double bs = (p - p_s) * (ro0 * ro0) / (roc0 * roc0) + ros;
+    double b0 = (p - pd) * (ro0 * ro0) / (roc0 * roc0) + rod;
+    double ro = eps0 * bs + (1. - eps0) * b0;
+    v[nl] = ro;
+    v[ldv + nl] = ro * (qtx + qn * nx);
+    v[2 * ldv + nl] = ro * (qty + qn * ny);
+    v[3 * ldv + nl] = ro * (qtz + qn * nz);
+    v[4 * ldv + nl] = p / gam1 + pinfl + 0.5 * (v[ldv + nl] * v[ldv + nl] + v[2 * ldv + nl] * v[2 * ldv + nl] +
+                                                  v[3 * ldv + nl] * v[3 * ldv + nl]) / ro;
+    ps[nl] = p;
+    temp[nl] = gam * ps[nl] / ro;
+    cson[nl] = sqrt(temp[nl]);
+  }
+}
+
+/* rbte.f90:46-59 (tq ghost = adjacent real cell) for non-coincident faces and
+ * rbtc.f90:104-117 (average with the remote cell) for coincident faces. */
+void orc_bc_tq_copy(long nf, const long *ghost, const long *inner, double *tq, long ldv) {
+  for (long f = 0; f < nf; f++)
+    for (int c = 0; c < 9; c++) tq[c * ldv + ghost[f]] = tq[c * ldv + inner[f]];
+}
+void orc_bc_tq_coincident(long nf, const long *ghost, const long *inner, const long *remote,
+                          double *tq, long ldv) {
+  for (long f = 0; f < nf; f++)
+    for (int c = 0; c < 9; c++)
+      tq[c * ldv + ghost[f]] = 0.5 * (tq[c * ldv + inner[f]] + tq[c * ldv + remote[f]]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* implimf_3d.f90:87-486: coefficients, lmx point-Jacobi sub-iterations with
+ * exact flux-difference off-diagonals, then v += d.  ns=0 drops the viscous
+ * spectral radius (mu, mut unused).  d: 5 arrays of ldv (the increment). */
+void orc_implimf_3d(const orc_block *b, const orc_phys *ph, int ns, const double *u, const double *dt,
+                    double *v, long ldv, double *d, const double *ff, const double *mu,
+                    const double *mut, int lmx, const double *sn, const double *vol, double dtpas,
+                    int kfmg, int ityprk, const double *ps, const double *cson) {
+  const double gam = ph->gam, gam1 = ph->gam1, pinfl = ph->pinfl;
+  long n0 = b->npc, nnn = b->nnn;
+  int i1 = b->ii1, i2 = b->ii2, j1 = b->jj1, j2 = b->jj2, k1 = b->kk1, k2 = b->kk2;
+  long nci = INC(b, 0), ncj = INC(b, 1), nck = INC(b, 2);
+  double *df[9], *coefe[3], *rv = calloc(nnn, 8), *diag = calloc(nnn, 8), *d2w[5];
+  for (int c = 0; c < 9; c++) df[c] = calloc(nnn, 8);
+  for (int c = 0; c < 3; c++) coefe[c] = calloc(nnn, 8);
+  for (int c = 0; c < 5; c++) d2w[c] = calloc(nnn, 8);
+  const double fact = 1.5;
+  for (long n = IDX(b, i1 - 1, j1 - 1, k1 - 1); n <= IDX(b, i2 + 1, j2 + 1, k2 + 1); n++)
+    for (int e = 0; e < 5; e++) d[e * ldv + n] = 0.;
+  for (int k = k1; k <= k2 - 1; k++)
+    for (int j = j1; j <= j2 - 1; j++)
+      for (int i = i1; i <= i2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - n0;
+        rv[m] = ns ? gam * (mu[n] / ph->pr + mut[n] / ph->prt) / v[n] : 0.;
+        diag[m] = vol[n] / dt[n];
+      }
+  for (int dd = 0; dd < 3; dd++) {
+    long inc = INC(b, dd);
+    int lo[3] = {i1, j1, k1}, hi[3] = {i2 - 1, j2 - 1, k2 - 1};
+    hi[dd] += 1; /* faces lo..hi of the direction, incl. the two boundaries */
+    for (int k = lo[2]; k <= hi[2]; k++)
+      for (int j = lo[1]; j <= hi[1]; j++)
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          long n = IDX(b, i, j, k), m = n - n0;
+          double sx = SN(sn, b, m, dd, 0), sy = SN(sn, b, m, dd, 1), sz = SN(sn, b, m, dd, 2);
+          double cnds = sx * sx + sy * sy + sz * sz;
+          double uu = 0.5 * (v[ldv + n] / v[n] + v[ldv + n - inc] / v[n - inc]);
+          double vv = 0.5 * (v[2 * ldv + n] / v[n] + v[2 * ldv + n - inc] / v[n - inc]);
+          double ww = 0.5 * (v[3 * ldv + n] / v[n] + v[3 * ldv + n - inc] / v[n - inc]);
+          double vn = uu * sx + vv * sy + ww * sz;
+          double cc = 0.5 * (cson[n] + cson[n - inc]);
+          coefe[dd][m] = 0.5 * (fabs(vn) + sqrt(cnds) * cc) + (rv[m] + rv[m - inc]) * cnds / (vol[n] + vol[n - inc]);
+        }
+  }
+  for (int k = k1; k <= k2 - 1; k++)
+    for (int j = j1; j <= j2 - 1; j++)
+      for (int i = i1; i <= i2 - 1; i++) {
+        long m = IDX(b, i, j, k) - n0;
+        diag[m] = diag[m] + coefe[0][m] + coefe[0][m + nci] + coefe[1][m] + coefe[1][m + ncj] +
+                  coefe[2][m] + coefe[2][m + nck];
+        if (kfmg == 3) diag[m] = diag[m] + fact * vol[m + n0] / dtpas;
+      }
+  for (int ls = 1; ls <= lmx; ls++) {
+    for (int k = k1; k <= k2 - 1; k++)
+      for (int j = j1; j <= j2 - 1; j++)
+        for (int i = i1; i <= i2 - 1; i++) {
+          long n = IDX(b, i, j, k), m = n - n0;
+          for (int e = 0; e < 5; e++)
+            d2w[e][m] = ityprk == 0 ? -u[e * ldv + n] : -u[e * ldv + n] - ff[e * ldv + n];
+        }
+    for (int dd = 0; dd < 3; dd++) {
+      long inc = INC(b, dd);
+      int lo[3] = {i1, j1, k1}, hi[3] = {i2 - 1, j2 - 1, k2 - 1};
+      hi[dd] += 1;
+      for (int k = lo[2]; k <= hi[2]; k++)
+        for (int j = lo[1]; j <= hi[1]; j++)
+          for (int i = lo[0]; i <= hi[0]; i++) {
+            long n = IDX(b, i, j, k), m = n - n0;
+            double s1 = SN(sn, b, m, dd, 0), s2 = SN(sn, b, m, dd, 1), s3 = SN(sn, b, m, dd, 2);
+            /* df order: xx xy xz ex yy yz ey zz ez (implimf_3d.f90:97-105) */
+            double *dxx = df[0], *dxy = df[1], *dxz = df[2], *dex = df[3], *dyy = df[4], *dyz = df[5];
+            double *dey = df[6], *dzz = df[7], *dez = df[8];
+            double tn[5];
+            tn[0] = 0.5 * (d[ldv + n] + d[ldv + n - inc]) * s1 + 0.5 * (d[2 * ldv + n] + d[2 * ldv + n - inc]) * s2 +
+                    0.5 * (d[3 * ldv + n] + d[3 * ldv + n - inc]) * s3;
+            tn[1] = 0.5 * (dxx[m] + dxx[m - inc]) * s1 + 0.5 * (dxy[m] + dxy[m - inc]) * s2 +
+                    0.5 * (dxz[m] + dxz[m - inc]) * s3;
+            tn[2] = 0.5 * (dxy[m] + dxy[m - inc]) * s1 + 0.5 * (dyy[m] + dyy[m - inc]) * s2 +
+                    0.5 * (dyz[m] + dyz[m - inc]) * s3;
+            tn[3] = 0.5 * (dxz[m] + dxz[m - inc]) * s1 + 0.5 * (dyz[m] + dyz[m - inc]) * s2 +
+                    0.5 * (dzz[m] + dzz[m - inc]) * s3;
+            tn[4] = 0.5 * (dex[m] + dex[m - inc]) * s1 + 0.5 * (dey[m] + dey[m - inc]) * s2 +
+                    0.5 * (dez[m] + dez[m - inc]) * s3;
+            for (int e = 0; e < 5; e++) {
+              d2w[e][m] = d2w[e][m] + tn[e] + coefe[dd][m] * d[e * ldv + n - inc] +
+                          coefe[dd][m + inc] * d[e * ldv + n + inc];
+            }
+            for (int e = 0; e < 5; e++) d2w[e][m - inc] = d2w[e][m - inc] - tn[e];
+          }
+    }
+    for (int k = k1; k <= k2 - 1; k++)
+      for (int j = j1; j <= j2 - 1; j++)
+        for (int i = i1; i <= i2 - 1; i++) {
+          long n = IDX(b, i, j, k), m = n - n0;
+          for (int e = 0; e < 5; e++) d[e * ldv + n] = d2w[e][m] / diag[m];
+          double wi1 = v[n] + d[n], wi2 = v[ldv + n] + d[ldv + n], wi3 = v[2 * ldv + n] + d[2 * ldv + n];
+          double wi4 = v[3 * ldv + n] + d[3 * ldv + n], wi5 = v[4 * ldv + n] + d[4 * ldv + n];
+          double ui = wi2 / wi1, vi = wi3 / wi1, wi = wi4 / wi1;
+          double pres = gam1 * (wi5 - .5 * wi1 * (ui * ui + vi * vi + wi * wi) - pinfl);
+          double p = ps[n];
+          double V1 = v[n], V2 = v[ldv + n], V3 = v[2 * ldv + n], V4 = v[3 * ldv + n], V5 = v[4 * ldv + n];
+          double fxx = V2 * (V2 / V1) + p, fxy = V3 * (V2 / V1), fxz = V4 * (V2 / V1);
+          double fyy = V3 * (V3 / V1) + p, fyz = V4 * (V3 / V1), fzz = V4 * (V4 / V1) + p;
+          double fex = (V5 + p - pinfl) * V2 / V1, fey = (V5 + p - pinfl) * V3 / V1;
+          double fez = (V5 + p - pinfl) * V4 / V1;
+          df[0][m] = wi1 * (ui * ui) + pres - fxx;
+          df[1][m] = wi1 * ui * vi - fxy;
+          df[2][m] = wi1 * ui * wi - fxz;
+          df[3][m] = ui * (wi5 + pres - pinfl) - fex;
+          df[4][m] = wi1 * (vi * vi) + pres - fyy;
+          df[5][m] = wi1 * vi * wi - fyz;
+          df[6][m] = vi * (wi5 + pres - pinfl) - fey;
+          df[7][m] = wi1 * (wi * wi) + pres - fzz;
+          df[8][m] = wi * (wi5 + pres - pinfl) - fez;
+        }
+  }
+  for (int k = k1; k <= k2 - 1; k++)
+    for (int j = j1; j <= j2 - 1; j++)
+      for (int i = i1; i <= i2 - 1; i++) {
+        long n = IDX(b, i, j, k);
+        for (int e = 0; e < 5; e++) v[e * ldv + n] = v[e * ldv + n] + d[e * ldv + n];
+      }
+  for (int c = 0; c < 9; c++) free(df[c]);
+  for (int c = 0; c < 3; c++) free(coefe[c]);
+  for (int c = 0; c < 5; c++) free(d2w[c]);
+  free(rv); free(diag);
+}

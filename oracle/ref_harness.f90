@@ -1,0 +1,241 @@
+! Parity-checker harness around the reference CodeNS routines.
+!
+! TEST INFRASTRUCTURE ONLY.  This file is compiled together with a handful of
+! the reference's own Fortran sources (read in place under the reference tree
+! by oracle/Makefile) into oracle/_ref/libcodens_ref.so.  Only the tests, the
+! smoke check and the CPU-baseline leg of bench.py load that library, and only
+! as a checker or a timed CPU baseline: the product path never calls it.
+!
+! Each entry point sets the reference's module globals (the extents of the
+! blocks, the gas constants, the scheme switches) from plain C arguments and
+! then calls the reference routine unchanged, with the arrays passed by
+! sequence association exactly as atsch_num.f90 passes them.
+module codens_ref_harness
+  use iso_c_binding
+  use para_var
+  use para_fige
+  use maillage
+  use proprieteflu
+  use schemanum
+  use definition
+  use modeleturb
+  implicit none
+  character(len=7) :: heq = 'ns3d   '
+contains
+
+  ! --------------------------------------------------------------------
+  ! Global layout: nb blocks, storage ip11 cells, largest block ip00.
+  ! ext(16*nb): per block npc, id1,id2,jd1,jd2,kd1,kd2, ii1,ii2,jj1,jj2,
+  ! kk1,kk2, nnn, (unused), (unused).   (crdms.f90:126-153)
+  ! --------------------------------------------------------------------
+  subroutine ref_set_layout(nb, nip11, nip00, ext) bind(C, name="ref_set_layout")
+    integer(c_int), value :: nb, nip11, nip00
+    integer(c_int) :: ext(16, *)
+    integer :: l
+    if (allocated(npc)) then
+       deallocate(npc, npn, npfb, nnn, nnc, nnfb, id1, id2, jd1, jd2, kd1, kd2, &
+            ii1, ii2, jj1, jj2, kk1, kk2)
+    endif
+    allocate(npc(nb), npn(nb), npfb(nb), nnn(nb), nnc(nb), nnfb(nb))
+    allocate(id1(nb), id2(nb), jd1(nb), jd2(nb), kd1(nb), kd2(nb))
+    allocate(ii1(nb), ii2(nb), jj1(nb), jj2(nb), kk1(nb), kk2(nb))
+    do l = 1, nb
+       npc(l) = ext(1, l)
+       npn(l) = ext(1, l)
+       npfb(l) = nind*ext(1, l)
+       id1(l) = ext(2, l);  id2(l) = ext(3, l)
+       jd1(l) = ext(4, l);  jd2(l) = ext(5, l)
+       kd1(l) = ext(6, l);  kd2(l) = ext(7, l)
+       ii1(l) = ext(8, l);  ii2(l) = ext(9, l)
+       jj1(l) = ext(10, l); jj2(l) = ext(11, l)
+       kk1(l) = ext(12, l); kk2(l) = ext(13, l)
+       nnn(l) = ext(14, l)
+       nnc(l) = ext(14, l)
+       nnfb(l) = nind*ext(14, l)
+    enddo
+    lzx = nb
+    lz = nb
+    ip00 = nip00
+    ip11 = nip11
+    ip12 = nip11
+    ip13 = nip11
+    ip21 = nip11
+    ip31 = 3*nip11
+    ip60 = 7
+  end subroutine ref_set_layout
+
+  subroutine ref_set_phys(g, pinf, prl, prtl, rey, tref, iflul) bind(C, name="ref_set_phys")
+    real(c_double), value :: g, pinf, prl, prtl, rey, tref
+    integer(c_int), value :: iflul
+    gam = g
+    gam1 = gam - 1.
+    gam2 = .5*gam1
+    gam3 = 1./gam
+    gam4 = 1./gam1
+    gam5 = gam*gam1
+    pinfl = pinf
+    pr = prl
+    prt = prtl
+    reynz = rey
+    tnz = tref
+    iflu = iflul
+    cp = 1./(gam - 1.)
+    cv = cp/gam
+  end subroutine ref_set_phys
+
+  subroutine ref_set_scheme(imuscl, xkl, iliml, kfmgl, ins) bind(C, name="ref_set_scheme")
+    integer(c_int), value :: imuscl, iliml, kfmgl, ins
+    real(c_double), value :: xkl
+    muscl = imuscl
+    xk = xkl
+    ilim = iliml
+    kfmg = kfmgl
+    if (ins .ne. 0) then
+       heq = 'ns3d   '
+    else
+       heq = 'eu3d   '
+    endif
+  end subroutine ref_set_scheme
+
+  ! zpres.f90:4
+  subroutine ref_zpres(l, v, ps, temp, cson) bind(C, name="ref_zpres")
+    use mod_zpres
+    integer(c_int), value :: l
+    real(c_double) :: v(*), ps(*), temp(*), cson(*)
+    call zpres(l, v, ps, temp, cson)
+  end subroutine ref_zpres
+
+  ! zvismo.f90:4
+  subroutine ref_zvismo(l, mu, v, temp) bind(C, name="ref_zvismo")
+    use mod_zvismo
+    integer(c_int), value :: l
+    real(c_double) :: mu(*), v(*), temp(*)
+    call zvismo(l, mu, v, temp)
+  end subroutine ref_zvismo
+
+  ! chronos.f90:4 (local time step of one block)
+  subroutine ref_chronos(l, etal, mu, mut, v, dt, sn, lgsnlt, vol, t1, t2, t3, t4, cson) &
+       bind(C, name="ref_chronos")
+    use mod_chronos
+    integer(c_int), value :: l, lgsnlt
+    real(c_double), value :: etal
+    real(c_double) :: mu(*), mut(*), v(*), dt(*), sn(*), vol(*), t1(*), t2(*), t3(*), t4(*), cson(*)
+    call chronos(l, etal, mu, mut, v, dt, heq, sn, lgsnlt, vol, t1, t2, t3, t4, cson)
+  end subroutine ref_chronos
+
+  ! sch_roe_euler.f90:4
+  subroutine ref_sch_roe_euler(l, ityprk, u, v, ff, sn, lgsnlt, t1, t2, t3, t4, t5, t6, t7, t8, t9, t10, ps) &
+       bind(C, name="ref_sch_roe_euler")
+    use mod_sch_roe_euler
+    integer(c_int), value :: l, ityprk, lgsnlt
+    real(c_double) :: u(*), v(*), ff(*), sn(*), ps(*)
+    real(c_double) :: t1(*), t2(*), t3(*), t4(*), t5(*), t6(*), t7(*), t8(*), t9(*), t10(*)
+    call sch_roe_euler(l, ityprk, u, v, ff, heq, sn, lgsnlt, t1, t2, t3, t4, t5, t6, t7, t8, t9, t10, ps)
+  end subroutine ref_sch_roe_euler
+
+  ! sch_roe.f90:4
+  subroutine ref_sch_roe(l, ityprk, u, v, ff, txx, txy, txz, tyy, tyz, tzz, qx, qy, qz, &
+       sn, lgsnlt, t1, t2, t3, t4, t5, t6, t7, t8, t9, t10, ps) bind(C, name="ref_sch_roe")
+    use mod_sch_roe
+    integer(c_int), value :: l, ityprk, lgsnlt
+    real(c_double) :: u(*), v(*), ff(*), sn(*), ps(*)
+    real(c_double) :: txx(*), txy(*), txz(*), tyy(*), tyz(*), tzz(*), qx(*), qy(*), qz(*)
+    real(c_double) :: t1(*), t2(*), t3(*), t4(*), t5(*), t6(*), t7(*), t8(*), t9(*), t10(*)
+    call sch_roe(l, ityprk, u, v, ff, txx, txy, txz, tyy, tyz, tzz, qx, qy, qz, heq, sn, lgsnlt, &
+         t1, t2, t3, t4, t5, t6, t7, t8, t9, t10, ps)
+  end subroutine ref_sch_roe
+
+  ! zgrad2.f90:4 (cmui*/cmuj*/cmuk* weights are 1 outside the weighted schemes, inivec.f90:107-115)
+  subroutine ref_zgrad2(l, sn, lgsnlt, vol, s, temp, g, cmu) bind(C, name="ref_zgrad2")
+    use mod_zgrad2
+    integer(c_int), value :: l, lgsnlt
+    real(c_double) :: sn(*), vol(*), s(*), temp(*), g(ip00, 12), cmu(*)
+    call zgrad2(l, heq, sn, lgsnlt, vol, s, temp, &
+         g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), g(1,8), g(1,9), &
+         g(1,10), g(1,11), g(1,12), cmu, cmu, cmu, cmu, cmu, cmu)
+  end subroutine ref_zgrad2
+
+  ! zfluto.f90:4
+  subroutine ref_zfluto(l, mu, mut, tq, g) bind(C, name="ref_zfluto")
+    use mod_zfluto
+    integer(c_int), value :: l
+    real(c_double) :: mu(*), mut(*), tq(ip11, 9), g(ip00, 12)
+    call zfluto(l, mu, mut, tq(1,1), tq(1,2), tq(1,3), tq(1,4), tq(1,5), tq(1,6), &
+         tq(1,7), tq(1,8), tq(1,9), &
+         g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), g(1,8), g(1,9), &
+         g(1,10), g(1,11), g(1,12))
+  end subroutine ref_zfluto
+
+  ! sch_expli.f90:4
+  subroutine ref_sch_expli(l, u, v, dt, vol) bind(C, name="ref_sch_expli")
+    use mod_sch_expli
+    integer(c_int), value :: l
+    real(c_double) :: u(*), v(*), dt(*), vol(*)
+    call sch_expli(l, u, v, dt, vol)
+  end subroutine ref_sch_expli
+
+  ! implimf_3d.f90:4
+  subroutine ref_implimf_3d(l, u, dt, v, d, ff, mu, mut, lmx, sn, lgsnlt, vol, dtpas, ityprk, &
+       t1, t2, t3, t4, t5, t6, t7, t8, t9, t10, ps, cson) bind(C, name="ref_implimf_3d")
+    use mod_implimf_3d
+    integer(c_int), value :: l, lmx, lgsnlt, ityprk
+    real(c_double), value :: dtpas
+    real(c_double) :: u(*), dt(*), v(*), d(*), ff(*), mu(*), mut(*), sn(*), vol(*), ps(*), cson(*)
+    real(c_double) :: t1(*), t2(*), t3(*), t4(*), t5(*), t6(*), t7(*), t8(*), t9(*), t10(*)
+    call implimf_3d(l, u, dt, v, d, ff, mu, mut, heq, lmx, sn, lgsnlt, vol, dtpas, ityprk, &
+         t1, t2, t3, t4, t5, t6, t7, t8, t9, t10, ps, cson)
+  end subroutine ref_implimf_3d
+
+  ! implimf_eu.f90:6
+  subroutine ref_implimf_eu(l, u, dt, v, d, ff, lmx, sn, lgsnlt, vol, dtpas, ityprk, &
+       t1, t2, t3, t4, t5, t6, t7, t8, t9, t10, ps, cson) bind(C, name="ref_implimf_eu")
+    use mod_implimf_eu
+    integer(c_int), value :: l, lmx, lgsnlt, ityprk
+    real(c_double), value :: dtpas
+    real(c_double) :: u(*), dt(*), v(*), d(*), ff(*), sn(*), vol(*), ps(*), cson(*)
+    real(c_double) :: t1(*), t2(*), t3(*), t4(*), t5(*), t6(*), t7(*), t8(*), t9(*), t10(*)
+    call implimf_eu(l, u, dt, v, d, ff, heq, lmx, sn, lgsnlt, vol, dtpas, ityprk, &
+         t1, t2, t3, t4, t5, t6, t7, t8, t9, t10, ps, cson)
+  end subroutine ref_implimf_eu
+
+  ! norm.f90:4 via snorm.f90:4 (face area vectors of one block)
+  subroutine ref_snorm(l, x, y, z, sn, lgsnlt) bind(C, name="ref_snorm")
+    use mod_snorm
+    use chainecarac
+    integer(c_int), value :: l, lgsnlt
+    real(c_double) :: x(*), y(*), z(*), sn(*)
+    equat = heq
+    call snorm(l, x, y, z, sn, lgsnlt)
+  end subroutine ref_snorm
+
+  ! svol.f90:4
+  subroutine ref_svol(l, x, y, z, sn, lgsnlt, vol, t1, t2, t3) bind(C, name="ref_svol")
+    use mod_svol
+    integer(c_int), value :: l, lgsnlt
+    real(c_double) :: x(*), y(*), z(*), sn(*), vol(*), t1(*), t2(*), t3(*)
+    call svol(l, x, y, z, sn, lgsnlt, vol, t1, t2, t3)
+  end subroutine ref_svol
+
+  ! extmhg.f90:4
+  subroutine ref_extmhg(l, x, y, z, ex1, ex2) bind(C, name="ref_extmhg")
+    use mod_extmhg
+    integer(c_int), value :: l
+    real(c_double), value :: ex1, ex2
+    real(c_double) :: x(*), y(*), z(*)
+    call extmhg(l, x, y, z, ex1, ex2)
+  end subroutine ref_extmhg
+
+  ! dissip_jameson.f90:4
+  subroutine ref_dissip_jameson(l, v, d, sn, lgsnlt, snu, ps, cson, ki2l, ki4l) &
+       bind(C, name="ref_dissip_jameson")
+    use mod_dissip_jameson
+    integer(c_int), value :: l, lgsnlt
+    real(c_double), value :: ki2l, ki4l
+    real(c_double) :: v(*), d(*), sn(*), snu(*), ps(*), cson(*)
+    if (.not. allocated(ki2)) allocate(ki2(size(npc)), ki4(size(npc)))
+    ki2(l) = ki2l
+    ki4(l) = ki4l
+    call dissip_jameson(l, v, d, heq, sn, lgsnlt, snu, ps, cson)
+  end subroutine ref_dissip_jameson
+
+end module codens_ref_harness

@@ -1,0 +1,192 @@
+"""ctypes bindings of the two CPU checkers (oracle C restatement and the flang
+build of the reference routines) with a common calling convention.
+
+Every function takes the ``Layout`` and numpy arrays in the reference storage
+(``v`` is ``(nvar, ip11)`` C-contiguous, i.e. the Fortran ``v(ip11, nvar)``).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+import oracle
+from oracle import ptr
+
+D = ctypes.c_double
+L_ = ctypes.c_long
+
+
+class OrcBlock(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("npc", "id1", "id2", "jd1", "jd2", "kd1", "kd2", "ii1", "ii2", "jj1", "jj2", "kk1",
+                 "kk2", "nnn", "pad0", "pad1")]
+
+
+class OrcPhys(ctypes.Structure):
+    _fields_ = [("gam", D), ("gam1", D), ("pinfl", D), ("pr", D), ("prt", D), ("cp", D),
+                ("reynz", D), ("tnz", D), ("iflu", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+class OrcSch(ctypes.Structure):
+    _fields_ = [("muscl", ctypes.c_int32), ("ilim", ctypes.c_int32), ("xk", D)]
+
+
+def orc_block(b):
+    return OrcBlock(*b.ext16())
+
+
+def orc_phys(gam=1.4, pinfl=0.0, pr=0.72, prt=1.0, reynz=1e5, tnz=288.15, iflu=1):
+    return OrcPhys(gam, gam - 1.0, pinfl, pr, prt, 1.0 / (gam - 1.0), reynz, tnz, iflu, 0)
+
+
+def sn_block(sn, b):
+    return sn[9 * b.npc: 9 * b.npc + 9 * b.nnn]
+
+
+class Ref:
+    """The reference routines (flang build), one layout at a time."""
+
+    def __init__(self, layout, phys: OrcPhys, muscl=1, xk=1.0 / 3.0, ilim=0, kfmg=0, ns=True):
+        self.lib = oracle.ref_lib()
+        if self.lib is None:
+            raise RuntimeError("oracle/_ref/libcodens_ref.so not built")
+        self.lay = layout
+        ext = np.ascontiguousarray(layout.ext_array())
+        self.lib.ref_set_layout(len(layout.blocks), layout.ip11, layout.ip00, ptr(ext))
+        self.lib.ref_set_phys(D(phys.gam), D(phys.pinfl), D(phys.pr), D(phys.prt), D(phys.reynz),
+                              D(phys.tnz), phys.iflu)
+        self.lib.ref_set_scheme(muscl, D(xk), ilim, kfmg, 1 if ns else 0)
+        self.t = [np.zeros(layout.ip00) for _ in range(10)]
+
+    def zpres(self, l, v, ps, temp, cson):
+        self.lib.ref_zpres(l + 1, ptr(v), ptr(ps), ptr(temp), ptr(cson))
+
+    def zvismo(self, l, mu, v, temp):
+        self.lib.ref_zvismo(l + 1, ptr(mu), ptr(v), ptr(temp))
+
+    def chronos(self, l, eta, mu, mut, v, dt, sn, vol, cson):
+        b = self.lay.blocks[l]
+        snb = np.ascontiguousarray(sn_block(sn, b))
+        self.lib.ref_chronos(l + 1, D(eta), ptr(mu), ptr(mut), ptr(v), ptr(dt), ptr(snb), b.nnn,
+                             ptr(vol), *[ptr(a) for a in self.t[:4]], ptr(cson))
+
+    def sch_roe_euler(self, l, ityprk, u, v, ff, sn, ps):
+        b = self.lay.blocks[l]
+        snb = np.ascontiguousarray(sn_block(sn, b))
+        self.lib.ref_sch_roe_euler(l + 1, ityprk, ptr(u), ptr(v), ptr(ff), ptr(snb), b.nnn,
+                                   *[ptr(a) for a in self.t], ptr(ps))
+
+    def sch_roe(self, l, ityprk, u, v, ff, tq, sn, ps):
+        b = self.lay.blocks[l]
+        snb = np.ascontiguousarray(sn_block(sn, b))
+        self.lib.ref_sch_roe(l + 1, ityprk, ptr(u), ptr(v), ptr(ff), *[ptr(tq[c]) for c in range(9)],
+                             ptr(snb), b.nnn, *[ptr(a) for a in self.t], ptr(ps))
+
+    def zgrad2(self, l, sn, vol, v, temp):
+        b = self.lay.blocks[l]
+        snb = np.ascontiguousarray(sn_block(sn, b))
+        g = np.zeros((12, self.lay.ip00))
+        cmu = np.ones(self.lay.ip11)
+        self.lib.ref_zgrad2(l + 1, ptr(snb), b.nnn, ptr(vol), ptr(v), ptr(temp), ptr(g), ptr(cmu))
+        return g
+
+    def zfluto(self, l, mu, mut, tq, g):
+        self.lib.ref_zfluto(l + 1, ptr(mu), ptr(mut), ptr(tq), ptr(g))
+
+    def sch_expli(self, l, u, v, dt, vol):
+        self.lib.ref_sch_expli(l + 1, ptr(u), ptr(v), ptr(dt), ptr(vol))
+
+    def implimf_3d(self, l, u, dt, v, d, ff, mu, mut, lmx, sn, vol, ps, cson, dtpas=1.0, ityprk=0):
+        b = self.lay.blocks[l]
+        snb = np.ascontiguousarray(sn_block(sn, b))
+        self.lib.ref_implimf_3d(l + 1, ptr(u), ptr(dt), ptr(v), ptr(d), ptr(ff), ptr(mu), ptr(mut), lmx,
+                                ptr(snb), b.nnn, ptr(vol), D(dtpas), ityprk, *[ptr(a) for a in self.t],
+                                ptr(ps), ptr(cson))
+
+
+class Orc:
+    """The C restatement, same calling convention as :class:`Ref`."""
+
+    def __init__(self, layout, phys: OrcPhys, muscl=1, xk=1.0 / 3.0, ilim=0, kfmg=0, ns=True):
+        self.lib = oracle.oracle_lib()
+        self.lay = layout
+        self.ph = phys
+        self.sc = OrcSch(muscl, ilim, xk)
+        self.ns = ns
+        self.kfmg = kfmg
+        self.blocks = [orc_block(b) for b in layout.blocks]
+        ld = layout.ip11
+        self.ld = L_(ld)
+
+    def B(self, l):
+        return ctypes.byref(self.blocks[l])
+
+    def zpres(self, l, v, ps, temp, cson):
+        self.lib.orc_zpres(self.B(l), ctypes.byref(self.ph), ptr(v), self.ld, ptr(ps), ptr(temp), ptr(cson))
+
+    def zvismo(self, l, mu, v, temp):
+        return self.lib.orc_zvismo(self.B(l), ctypes.byref(self.ph), ptr(temp), ptr(mu))
+
+    def chronos(self, l, eta, mu, mut, v, dt, sn, vol, cson):
+        b = self.lay.blocks[l]
+        snb = np.ascontiguousarray(sn_block(sn, b))
+        self.lib.orc_chronos(self.B(l), ctypes.byref(self.ph), D(eta), 1 if self.ns else 0, ptr(v), self.ld,
+                             ptr(mu), ptr(mut), ptr(snb), ptr(vol), ptr(cson), ptr(dt))
+
+    def chrono_global(self, dt1min, dt):
+        arr = (OrcBlock * len(self.blocks))(*self.blocks)
+        out = D(0.0)
+        stop = self.lib.orc_chrono_global(arr, len(self.blocks), D(dt1min), ptr(dt), ctypes.byref(out))
+        return stop, out.value
+
+    def sch_roe_euler(self, l, ityprk, u, v, ff, sn, ps):
+        b = self.lay.blocks[l]
+        snb = np.ascontiguousarray(sn_block(sn, b))
+        self.lib.orc_sch_roe_euler(self.B(l), ctypes.byref(self.ph), ctypes.byref(self.sc), ityprk, ptr(u),
+                                   ptr(v), self.ld, ptr(ff), ptr(snb), ptr(ps))
+
+    def sch_roe(self, l, ityprk, u, v, ff, tq, sn, ps):
+        b = self.lay.blocks[l]
+        snb = np.ascontiguousarray(sn_block(sn, b))
+        self.lib.orc_sch_roe(self.B(l), ctypes.byref(self.ph), ctypes.byref(self.sc), ityprk, ptr(u), ptr(v),
+                             self.ld, ptr(ff), ptr(tq), ptr(snb), ptr(ps))
+
+    def zgrad2(self, l, sn, vol, v, temp):
+        b = self.lay.blocks[l]
+        snb = np.ascontiguousarray(sn_block(sn, b))
+        g = np.zeros((12, b.nnn))
+        self.lib.orc_zgrad2(self.B(l), ptr(snb), ptr(vol), ptr(v), self.ld, ptr(temp), ptr(g))
+        return g
+
+    def zfluto(self, l, mu, mut, tq, g):
+        self.lib.orc_zfluto(self.B(l), ctypes.byref(self.ph), ptr(mu), ptr(mut), ptr(tq), self.ld,
+                            ptr(np.ascontiguousarray(g)))
+
+    def sch_expli(self, l, u, v, dt, vol):
+        self.lib.orc_sch_expli(self.B(l), ptr(u), ptr(v), self.ld, ptr(dt), ptr(vol))
+
+    def implimf_3d(self, l, u, dt, v, d, ff, mu, mut, lmx, sn, vol, ps, cson, dtpas=1.0, ityprk=0):
+        b = self.lay.blocks[l]
+        snb = np.ascontiguousarray(sn_block(sn, b))
+        self.lib.orc_implimf_3d(self.B(l), ctypes.byref(self.ph), 1 if self.ns else 0, ptr(u), ptr(dt), ptr(v),
+                                self.ld, ptr(d), ptr(ff), ptr(mu), ptr(mut), lmx, ptr(snb), ptr(vol), D(dtpas),
+                                self.kfmg, ityprk, ptr(ps), ptr(cson))
+
+    # boundary ghosts (face lists as int64 arrays)
+    def bc_coincident(self, g, i, r, v, ps, temp, cson):
+        self.lib.orc_bc_coincident(L_(len(g)), ptr(g), ptr(i), ptr(r), ptr(v), self.ld, ptr(ps), ptr(temp),
+                                   ptr(cson))
+
+    def bc_wall(self, g, i, v, ps, temp, cson):
+        self.lib.orc_bc_wall(L_(len(g)), ptr(g), ptr(i), ptr(v), self.ld, ptr(ps), ptr(temp), ptr(cson))
+
+    def bc_farfield(self, g, i, nx, ny, nz, w0, v, ps, temp, cson):
+        self.lib.orc_bc_farfield(L_(len(g)), ptr(g), ptr(i), ptr(nx), ptr(ny), ptr(nz), ptr(w0),
+                                 ctypes.byref(self.ph), ptr(v), self.ld, ptr(ps), ptr(temp), ptr(cson))
+
+    def bc_tq_copy(self, g, i, tq):
+        self.lib.orc_bc_tq_copy(L_(len(g)), ptr(g), ptr(i), ptr(tq), self.ld)
+
+    def bc_tq_coincident(self, g, i, r, tq):
+        self.lib.orc_bc_tq_coincident(L_(len(g)), ptr(g), ptr(i), ptr(r), ptr(tq), self.ld)

@@ -1,0 +1,117 @@
+"""Throughput of the implicit residual-and-update iteration on one MI355X.
+
+Workload: the curvilinear wall block of 256x128x64 cells (2,097,152 real
+cells), laminar Navier-Stokes, Roe-MUSCL with minmod, viscous fluxes, wall and
+far-field ghosts, local time step, matrix-free point-Jacobi implicit phase with
+lmax=10 sub-iterations, all FP64 (codens_amd.configs.wall_ns).  One step is one
+implicit iteration (one atsch_num stage); inputs are resident in HBM.
+
+Prints one JSON line: Mcell-updates/s, the roofline of the dominant kernel
+(implimf_sweep: 264 algorithmic bytes per real cell per launch) measured with
+HIP events on the launch stream, and the reference Fortran build timed on the
+host on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+SWEEP_BYTES_PER_CELL = 8 * (5 + 5 + 5 + 9 + 1 + 3 + 5)  # u, v, d_in, sn, diag, coefe, d_out
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(case, stages):
+    """The reference routines (flang build, 1 core) on `stages` implicit
+    iterations of the same case; falls back to the C restatement."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+    import cpu_stage
+    use_ref = oracle.ref_lib() is not None
+    st = cpu_stage.CpuStage(case, use_ref=use_ref)
+    t0 = time.perf_counter()
+    for _ in range(stages):
+        st.stage()
+    dt = time.perf_counter() - t0
+    return {"value": case.layout.ncells * stages / dt / 1e6, "unit": "Mcell-updates/s", "cores": 1,
+            "kind": "reference" if use_ref else "port",
+            "sample": f"{stages} implicit iterations of the same 256x128x64 case, lmax=10, "
+                      f"{'flang -O2 build of the reference routines' if use_ref else 'oracle C restatement'}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-stages", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dims", type=str, default="256,128,64")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the multi-GPU path (block sharding with face halos over RCCL) "
+              "is a design, not a build, in this version", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from codens_amd import abi, configs
+    from codens_amd.solver import Solver
+
+    ci, cj, ck = (int(x) for x in args.dims.split(","))
+    case = configs.wall_ns(ci, cj, ck)
+    s = Solver(case)
+    ncell = case.layout.ncells
+    for _ in range(args.warmup):
+        s.stage()
+    torch.cuda.synchronize()
+    abi.timing_enable(True)
+    for name in ("implimf_sweep", "roe_face_flux", "implimf_coef", "zvisqc", "roe_limiter_chain", "roe_gather",
+                 "chronos", "zpres", "implimf_update"):
+        abi.timing_read(name, reset=True)
+    abi.timing_enable(False)
+    # timed region (no events inside)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.stage()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # per-kernel timing pass with HIP events on the launch stream
+    abi.timing_enable(True)
+    for _ in range(max(3, args.steps // 4)):
+        s.stage()
+    torch.cuda.synchronize()
+    kern = {}
+    for name in ("implimf_sweep", "roe_face_flux", "implimf_coef", "zvisqc", "roe_limiter_chain", "roe_gather",
+                 "chronos", "zpres", "zvismo", "implimf_update"):
+        ms, cnt = abi.timing_read(name, reset=True)
+        if cnt:
+            kern[name] = {"avg_us": 1e3 * ms / cnt, "launches": cnt}
+    abi.timing_enable(False)
+    res = s.density_residual()
+    ms_step = 1e3 * wall / args.steps
+    value = ncell * args.steps / wall / 1e6
+    sw = kern.get("implimf_sweep", {"avg_us": float("nan")})
+    achieved = SWEEP_BYTES_PER_CELL * ncell / (sw["avg_us"] * 1e-6) / 1e9
+    out = {
+        "metric": "Mcell-updates/s per implicit iter to 1e-8 residual drop; % of HBM roofline",
+        "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"wall_ns {ci}x{cj}x{ck} laminar NS, Roe-MUSCL minmod + viscous, implimf_3d lmax=10, FP64",
+                   "cells": ncell, "lmax": case.lmax},
+        "roofline": {"bound": "hbm", "kernel": "implimf_sweep", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_cell": SWEEP_BYTES_PER_CELL, "avg_us": round(sw["avg_us"], 2)},
+        "kernels": kern,
+        "density_residual": res,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(case, args.cpu_stages)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,77 @@
+"""Loader of the gfx950 library and ctypes signatures of include/codens.h.
+
+There is no fallback: if ``libcodens_gpu.so`` is missing or fails to load,
+importing this module raises, so nothing silently runs elsewhere.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcodens_gpu.so")
+
+c_int = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+
+
+class CodensBlock(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("npc", "id1", "id2", "jd1", "jd2", "kd1", "kd2", "ii1", "ii2", "jj1", "jj2", "kk1",
+                 "kk2", "nnn", "pad0", "pad1")]
+
+
+class CodensPhys(ctypes.Structure):
+    _fields_ = [("gam", c_dbl), ("gam1", c_dbl), ("pinfl", c_dbl), ("pr", c_dbl), ("prt", c_dbl),
+                ("cp", c_dbl), ("reynz", c_dbl), ("tnz", c_dbl), ("iflu", c_int), ("pad", c_int)]
+
+
+class CodensScheme(ctypes.Structure):
+    _fields_ = [("muscl", c_int), ("ilim", c_int), ("xk", c_dbl)]
+
+
+P = ctypes.POINTER
+
+# name -> argtypes (restype int32 for all but codens_last_error)
+SIGNATURES = {
+    "codens_zpres": [P(CodensBlock), P(CodensPhys), c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
+    "codens_zvismo": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_vp],
+    "codens_chronos": [P(CodensBlock), P(CodensPhys), c_dbl, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,
+                       c_vp, c_vp],
+    "codens_chrono_global": [P(CodensBlock), c_int, c_dbl, c_vp, P(c_dbl), c_vp],
+    "codens_sch_roe_euler": [P(CodensBlock), P(CodensPhys), P(CodensScheme), c_int, c_vp, c_vp, c_i64, c_vp,
+                             c_vp, c_vp, c_vp],
+    "codens_sch_roe": [P(CodensBlock), P(CodensPhys), P(CodensScheme), c_int, c_vp, c_vp, c_i64, c_vp, c_vp,
+                       c_vp, c_vp, c_vp],
+    "codens_zvisqc": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "codens_sch_expli": [P(CodensBlock), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
+    "codens_implimf_3d": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int,
+                          c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp, c_vp],
+    "codens_bc_coincident": [c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
+    "codens_bc_wall": [c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
+    "codens_bc_farfield": [c_i64, c_vp, c_vp, c_vp, P(c_dbl), P(CodensPhys), c_vp, c_i64, c_vp, c_vp, c_vp,
+                           c_vp],
+    "codens_bc_tq_copy": [c_i64, c_vp, c_vp, c_vp, c_i64, c_vp],
+    "codens_bc_tq_coincident": [c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp],
+    "codens_sumsq_rate": [P(CodensBlock), c_int, c_vp, c_vp, c_vp, P(c_dbl), c_vp],
+    "codens_timing_enable": [c_int],
+    "codens_timing_read": [ctypes.c_char_p, P(c_dbl), P(c_i64), c_int],
+}
+
+
+def load(path: str = LIB_PATH):
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: run `python -m codens_amd.build` (or __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = c_int
+    lib.codens_last_error.restype = ctypes.c_char_p
+    lib.codens_last_error.argtypes = []
+    return lib
+
+
+lib = load()

@@ -1,0 +1,151 @@
+"""Python view of the C-ABI: the same entry points, with torch device tensors.
+
+Each function mirrors one reference routine (same name without the
+``codens_`` prefix, same argument meaning) and raises :class:`CodensError`
+where the C call returns a non-zero status, the analogue of the reference's
+``stop``/``abort`` paths.  Tensors must be float64 (int64 for face lists),
+contiguous and on the current HIP device.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from ._lib import CodensBlock, CodensPhys, CodensScheme, lib
+
+
+class CodensError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _chk(rc):
+    if rc != 0:
+        raise CodensError(rc, lib.codens_last_error().decode())
+
+
+def _p(t):
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("expected a torch tensor")
+    if not t.is_cuda:
+        raise ValueError("tensor must be on the GPU")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _s(stream):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def block(b) -> CodensBlock:
+    """codens_block of a :class:`codens_amd.layout.Block`."""
+    return CodensBlock(*b.ext16())
+
+
+def phys(gam=1.4, pinfl=0.0, pr=0.72, prt=1.0, reynz=1e5, tnz=288.15, iflu=1) -> CodensPhys:
+    return CodensPhys(gam, gam - 1.0, pinfl, pr, prt, 1.0 / (gam - 1.0), reynz, tnz, iflu, 0)
+
+
+def scheme(muscl=1, ilim=0, xk=1.0 / 3.0) -> CodensScheme:
+    return CodensScheme(muscl, ilim, xk)
+
+
+def ld(v):
+    return v.shape[-1]
+
+
+def zpres(b, ph, v, ps, temp, cson, stream=None):
+    _chk(lib.codens_zpres(ctypes.byref(b), ctypes.byref(ph), _p(v), ld(v), _p(ps), _p(temp), _p(cson), _s(stream)))
+
+
+def zvismo(b, ph, temp, mu, nbad=None, stream=None):
+    _chk(lib.codens_zvismo(ctypes.byref(b), ctypes.byref(ph), _p(temp), _p(mu), _p(nbad), _s(stream)))
+
+
+def chronos(b, ph, eta, ns, v, mu, mut, sn_blk, vol, cson, dt, stream=None):
+    _chk(lib.codens_chronos(ctypes.byref(b), ctypes.byref(ph), eta, 1 if ns else 0, _p(v), ld(v), _p(mu),
+                            _p(mut), _p(sn_blk), _p(vol), _p(cson), _p(dt), _s(stream)))
+
+
+def chrono_global(blocks, dt1min, dt, stream=None):
+    """Returns the smallest local step; raises CodensError(4) below dt1min."""
+    arr = (CodensBlock * len(blocks))(*blocks)
+    out = ctypes.c_double(0.0)
+    _chk(lib.codens_chrono_global(arr, len(blocks), dt1min, _p(dt), ctypes.byref(out), _s(stream)))
+    return out.value
+
+
+def sch_roe_euler(b, ph, sc, ityprk, u, v, ff, sn_blk, ps, stream=None):
+    _chk(lib.codens_sch_roe_euler(ctypes.byref(b), ctypes.byref(ph), ctypes.byref(sc), ityprk, _p(u), _p(v), ld(v),
+                                  _p(ff), _p(sn_blk), _p(ps), _s(stream)))
+
+
+def sch_roe(b, ph, sc, ityprk, u, v, ff, tq, sn_blk, ps, stream=None):
+    _chk(lib.codens_sch_roe(ctypes.byref(b), ctypes.byref(ph), ctypes.byref(sc), ityprk, _p(u), _p(v), ld(v),
+                            _p(ff), _p(tq), _p(sn_blk), _p(ps), _s(stream)))
+
+
+def zvisqc(b, ph, sn_blk, vol, v, temp, mu, mut, tq, stream=None):
+    _chk(lib.codens_zvisqc(ctypes.byref(b), ctypes.byref(ph), _p(sn_blk), _p(vol), _p(v), ld(v), _p(temp), _p(mu),
+                           _p(mut), _p(tq), _s(stream)))
+
+
+def sch_expli(b, u, v, dt, vol, stream=None):
+    _chk(lib.codens_sch_expli(ctypes.byref(b), _p(u), _p(v), ld(v), _p(dt), _p(vol), _s(stream)))
+
+
+def implimf_3d(b, ph, u, dt, v, d, ff, mu, mut, lmx, sn_blk, vol, ps, cson, dtpas=1.0, kfmg=0, ityprk=0,
+               stream=None):
+    _chk(lib.codens_implimf_3d(ctypes.byref(b), ctypes.byref(ph), _p(u), _p(dt), _p(v), ld(v), _p(d), _p(ff),
+                               _p(mu), _p(mut), lmx, _p(sn_blk), _p(vol), dtpas, kfmg, ityprk, _p(ps), _p(cson),
+                               _s(stream)))
+
+
+def bc_coincident(g, i, r, v, ps, temp, cson, stream=None):
+    _chk(lib.codens_bc_coincident(g.numel(), _p(g), _p(i), _p(r), _p(v), ld(v), _p(ps), _p(temp), _p(cson),
+                                  _s(stream)))
+
+
+def bc_wall(g, i, v, ps, temp, cson, stream=None):
+    _chk(lib.codens_bc_wall(g.numel(), _p(g), _p(i), _p(v), ld(v), _p(ps), _p(temp), _p(cson), _s(stream)))
+
+
+def bc_farfield(g, i, nrm, w0, ph, v, ps, temp, cson, stream=None):
+    w = (ctypes.c_double * 5)(*[float(x) for x in w0])
+    _chk(lib.codens_bc_farfield(g.numel(), _p(g), _p(i), _p(nrm), w, ctypes.byref(ph), _p(v), ld(v), _p(ps),
+                                _p(temp), _p(cson), _s(stream)))
+
+
+def bc_tq_copy(g, i, tq, stream=None):
+    _chk(lib.codens_bc_tq_copy(g.numel(), _p(g), _p(i), _p(tq), ld(tq), _s(stream)))
+
+
+def bc_tq_coincident(g, i, r, tq, stream=None):
+    _chk(lib.codens_bc_tq_coincident(g.numel(), _p(g), _p(i), _p(r), _p(tq), ld(tq), _s(stream)))
+
+
+def sumsq_rate(blocks, a, bv, dt, stream=None):
+    arr = (CodensBlock * len(blocks))(*blocks)
+    out = ctypes.c_double(0.0)
+    _chk(lib.codens_sumsq_rate(arr, len(blocks), _p(a), _p(bv), _p(dt), ctypes.byref(out), _s(stream)))
+    return out.value
+
+
+def timing_enable(on=True):
+    """Bracket every launch of the main kernels with HIP events."""
+    _chk(lib.codens_timing_enable(1 if on else 0))
+
+
+def timing_read(name, reset=True):
+    """(total ms, launch count) of one kernel since the last reset."""
+    ms = ctypes.c_double(0.0)
+    cnt = ctypes.c_int64(0)
+    _chk(lib.codens_timing_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt), 1 if reset else 0))
+    return ms.value, cnt.value

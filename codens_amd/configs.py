@@ -1,0 +1,176 @@
+"""Synthetic cases of the residual-and-update path: mesh, boundary faces,
+gas constants, switches and initial fields, all seeded and deterministic.
+
+* ``periodic_euler``  -- 64x32x4 cells, uniform dx=dy=dz=1/64, coincident
+  (periodic) faces in i, j and k, Euler, M=0.8, explicit forward Euler with a
+  global step (kdtl=0) at CFL 0.8.  4 seeded sinusoidal modes of amplitude 1%
+  on rho and u.
+* ``wall_ns``  -- 256x128x64 cells by default, Cartesian with seeded
+  sinusoidal node displacement of 5% of the local spacing and tanh stretching
+  toward an adiabatic no-slip wall at j=1 (clpara); far field (clnrd) on the
+  five other faces.  Laminar Navier-Stokes, Re=1e5 per unit length, M=0.5,
+  Roe-MUSCL with minmod (ilim=1), implicit point-Jacobi (kmf=1, lmax=10),
+  local time step (kdtl=1) with eta=10.
+
+Seeds: numpy PCG64(20261201 + case index).
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import layout as L
+
+GAM = 1.4
+
+
+@dataclasses.dataclass
+class FaceSet:
+    ghost: np.ndarray
+    inner: np.ndarray
+    remote: Optional[np.ndarray] = None
+    normals: Optional[np.ndarray] = None  # (3, nf) inward unit normals
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    layout: L.Layout
+    sn: np.ndarray
+    vol: np.ndarray
+    v0: np.ndarray  # (5, ip11) initial conservative state, ghosts included
+    ns: bool
+    phys: Dict[str, float]
+    scheme: Dict[str, float]
+    eta: float
+    kdtl: int
+    dt1min: float
+    kmf: int
+    lmax: int
+    winf: np.ndarray  # far-field conservative state
+    coincident: FaceSet
+    wall: FaceSet
+    farfield: FaceSet
+
+
+def conservative(rho, u, v, w, p, gam=GAM, pinfl=0.0):
+    return np.stack([rho, rho * u, rho * v, rho * w, (p - pinfl) / (gam - 1.0) + pinfl + 0.5 * rho * (u * u + v * v + w * w)])
+
+
+def _empty_faces():
+    z = np.zeros(0, dtype=np.int64)
+    return FaceSet(z, z, z, np.zeros((3, 0)))
+
+
+def _cat(parts: List[FaceSet]) -> FaceSet:
+    if not parts:
+        return _empty_faces()
+    g = np.concatenate([p.ghost for p in parts])
+    i = np.concatenate([p.inner for p in parts])
+    r = np.concatenate([p.remote for p in parts]) if parts[0].remote is not None else None
+    nrm = np.concatenate([p.normals for p in parts], axis=1) if parts[0].normals is not None else None
+    return FaceSet(g, i, r, nrm)
+
+
+def cell_centres(b: L.Block, x, y, z):
+    X, Y, Z = b.view(x), b.view(y), b.view(z)
+    def c(a):
+        return 0.125 * (a[:-1, :-1, :-1] + a[:-1, :-1, 1:] + a[:-1, 1:, :-1] + a[:-1, 1:, 1:]
+                        + a[1:, :-1, :-1] + a[1:, :-1, 1:] + a[1:, 1:, :-1] + a[1:, 1:, 1:])
+    return c(X), c(Y), c(Z)  # shape (nkd-1, njd-1, nid-1); entry [k,j,i] = cell (i,j,k) in storage offsets
+
+
+def periodic_euler(ci=64, cj=32, ck=4, seed=20261201, mach=0.8, cfl=0.8, amp=0.01, distort=0.0) -> Case:
+    lay = L.Layout([(ci, cj, ck)])
+    b = lay.blocks[0]
+    x = np.zeros(lay.ip11); y = np.zeros(lay.ip11); z = np.zeros(lay.ip11)
+    h = 1.0 / 64.0
+    K, J, I = np.meshgrid(np.arange(b.nkd), np.arange(b.njd), np.arange(b.nid), indexing="ij")
+    # optional interior distortion that vanishes on the block faces, so
+    # opposite faces stay congruent (a fraction `distort` of the spacing)
+    bump = (np.sin(2 * np.pi * (I - b.ii1) / ci) * np.sin(2 * np.pi * (J - b.jj1) / cj)
+            * np.sin(2 * np.pi * (K - b.kk1) / ck))
+    b.view(x)[:] = (I - b.ii1) * h + distort * h * bump
+    b.view(y)[:] = (J - b.jj1) * h + distort * h * bump
+    b.view(z)[:] = (K - b.kk1) * h + distort * h * bump
+    sn, vol = L.metrics(lay, x, y, z)
+    Lx, Ly, Lz = ci * h, cj * h, ck * h
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kv = rng.integers(1, 4, size=(4, 3))
+    ph = rng.uniform(0, 2 * np.pi, size=4)
+    xc, yc, zc = cell_centres(b, x, y, z)
+    s = np.zeros_like(xc)
+    for m in range(4):
+        s += np.sin(2 * np.pi * (kv[m, 0] * xc / Lx + kv[m, 1] * yc / Ly + kv[m, 2] * zc / Lz) + ph[m])
+    rho = np.ones(lay.ip11); u = np.full(lay.ip11, mach); vv = np.zeros(lay.ip11); w = np.zeros(lay.ip11)
+    p = np.full(lay.ip11, 1.0 / GAM)
+    Rv = b.view(rho); Uv = b.view(u)
+    Rv[:-1, :-1, :-1] = 1.0 + amp * s
+    Uv[:-1, :-1, :-1] = mach * (1.0 + amp * s)
+    v0 = conservative(rho, u, vv, w, p)
+    # periodic faces: each face's ghost averages its own adjacent cell with the
+    # adjacent cell of the opposite face (rbvc.f90), same listing order
+    parts = []
+    for d in "ijk":
+        g1, i1, _ = L.face_cells(b, d + "1")
+        g2, i2, _ = L.face_cells(b, d + "2")
+        parts.append(FaceSet(g1, i1, i2, None))
+        parts.append(FaceSet(g2, i2, i1, None))
+    coin = _cat(parts)
+    coin.normals = np.zeros((3, len(coin.ghost)))
+    # global step (kdtl=0): dt1min is 0.9 x the smallest CFL-0.8 local step of
+    # the initial field (chronos.f90:150-170 on the uniform mesh: dism = h/sqrt(3))
+    inner = b.interior()
+    r = v0[0, inner]
+    q = np.sqrt((v0[1, inner] ** 2 + v0[2, inner] ** 2 + v0[3, inner] ** 2) / r ** 2)
+    pr_ = (GAM - 1.0) * (v0[4, inner] - 0.5 * (v0[1, inner] ** 2 + v0[2, inner] ** 2 + v0[3, inner] ** 2) / r)
+    c = np.sqrt(GAM * pr_ / r)
+    dt1min = 0.9 * float(np.min(cfl * (h / np.sqrt(3.0)) / (q + c)))
+    return Case("periodic_euler", lay, sn, vol, v0, False,
+                dict(gam=GAM, pinfl=0.0, pr=0.72, prt=1.0, reynz=1e5, tnz=288.15, iflu=1),
+                dict(muscl=1, ilim=0, xk=1.0 / 3.0), cfl, 0, dt1min, 0, 0,
+                conservative(np.array(1.0), np.array(mach), np.array(0.0), np.array(0.0), np.array(1 / GAM)),
+                coin, _empty_faces(), _empty_faces())
+
+
+def wall_ns(ci=256, cj=128, ck=64, seed=20261202, mach=0.5, reynz=1e5, eta=10.0, lmax=10, ilim=1,
+            beta=2.5, disp=0.05) -> Case:
+    lay = L.Layout([(ci, cj, ck)])
+    b = lay.blocks[0]
+    Lx, Ly, Lz = 2.0, 1.0, 0.5
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xi = np.arange(ci + 1) / ci * Lx
+    eta_ = np.arange(cj + 1) / cj
+    yj = Ly * (1.0 - np.tanh(beta * (1.0 - eta_)) / np.tanh(beta))
+    zk = np.arange(ck + 1) / ck * Lz
+    Z, Y, X = np.meshgrid(zk, yj, xi, indexing="ij")
+    hx, hz = Lx / ci, Lz / ck
+    hy = np.gradient(yj)[None, :, None] * np.ones_like(Y)
+    kv = rng.integers(1, 4, size=(3, 3))
+    ph = rng.uniform(0, 2 * np.pi, size=3)
+    arg = lambda m: 2 * np.pi * (kv[m, 0] * X / Lx + kv[m, 1] * Y / Ly + kv[m, 2] * Z / Lz) + ph[m]
+    Xd = X + disp * hx * np.sin(arg(0))
+    Yd = Y + disp * hy * np.sin(arg(1)) * (Y > 0)  # keep the wall plane y=0
+    Zd = Z + disp * hz * np.sin(arg(2))
+    x = np.zeros(lay.ip11); y = np.zeros(lay.ip11); z = np.zeros(lay.ip11)
+    sl = (slice(b.kk1 - b.kd1, b.kk2 - b.kd1 + 1), slice(b.jj1 - b.jd1, b.jj2 - b.jd1 + 1),
+          slice(b.ii1 - b.id1, b.ii2 - b.id1 + 1))
+    b.view(x)[sl] = Xd; b.view(y)[sl] = Yd; b.view(z)[sl] = Zd
+    sn, vol = L.metrics(lay, x, y, z)
+    n = lay.ip11
+    v0 = conservative(np.ones(n), np.full(n, mach), np.zeros(n), np.zeros(n), np.full(n, 1.0 / GAM))
+    wall_parts, far_parts = [], []
+    for f in L.FACES:
+        g, i, fi = L.face_cells(b, f)
+        if f == "j1":
+            wall_parts.append(FaceSet(g, i))
+        else:
+            nx, ny, nz = L.inward_normals(b, sn, f, fi)
+            far_parts.append(FaceSet(g, i, None, np.stack([nx, ny, nz])))
+    return Case("wall_ns", lay, sn, vol, v0, True,
+                dict(gam=GAM, pinfl=0.0, pr=0.72, prt=1.0, reynz=reynz, tnz=288.15, iflu=1),
+                dict(muscl=1, ilim=ilim, xk=1.0 / 3.0), eta, 1, 0.0, 1, lmax,
+                conservative(np.array(1.0), np.array(mach), np.array(0.0), np.array(0.0), np.array(1 / GAM)),
+                _empty_faces(), _cat(wall_parts), _cat(far_parts))

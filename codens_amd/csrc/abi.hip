@@ -1,0 +1,266 @@
+// Error reporting, device workspace and the reductions of the C-ABI.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+
+using namespace codens;
+
+static thread_local char g_err[512] = "";
+
+extern "C" void codens_set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char *codens_last_error(void) { return g_err; }
+
+int codens_check_launch(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    codens_set_error("%s: launch failed: %s", what, hipGetErrorString(e));
+    return CODENS_EHIP;
+  }
+  return CODENS_OK;
+}
+
+int codens_check_block(const codens_block *b, const char *what) {
+  if (!b) {
+    codens_set_error("%s: null block", what);
+    return CODENS_EINVAL;
+  }
+  long nid = b->id2 - b->id1 + 1, njd = b->jd2 - b->jd1 + 1, nkd = b->kd2 - b->kd1 + 1;
+  if (b->npc < 0 || nid < 4 || njd < 4 || nkd < 4 || (long)b->nnn != nid * njd * nkd ||
+      b->ii1 - b->id1 < 1 || b->id2 - b->ii2 < 1 || b->jj1 - b->jd1 < 1 || b->jd2 - b->jj2 < 1 ||
+      b->kk1 - b->kd1 < 1 || b->kd2 - b->kk2 < 1 || b->ii2 - b->ii1 < 2 || b->jj2 - b->jj1 < 2 ||
+      b->kk2 - b->kk1 < 2) {
+    codens_set_error("%s: inconsistent block extents (npc=%d nnn=%d)", what, b->npc, b->nnn);
+    return CODENS_EINVAL;
+  }
+  return CODENS_OK;
+}
+
+// Grow-only device workspaces, one set per device, kept until process exit.
+namespace {
+struct Slot {
+  void *p = nullptr;
+  size_t n = 0;
+};
+std::mutex g_ws_mu;
+std::vector<std::vector<Slot>> g_ws;  // [device][slot]
+}  // namespace
+
+void *codens_workspace(size_t bytes, int slot) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
+  auto &v = g_ws[dev];
+  if ((int)v.size() <= slot) v.resize(slot + 1);
+  Slot &s = v[slot];
+  if (s.n < bytes) {
+    if (s.p) {
+      if (hipDeviceSynchronize() != hipSuccess) return nullptr;
+      (void)hipFree(s.p);
+      s.p = nullptr;
+      s.n = 0;
+    }
+    if (hipMalloc(&s.p, bytes) != hipSuccess) {
+      s.p = nullptr;
+      return nullptr;
+    }
+    s.n = bytes;
+  }
+  return s.p;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions over the real cells: one partial per workgroup, summed on the
+// host in workgroup order so the result does not depend on timing.
+constexpr int RED_T = 256;
+
+template <int OP>  // 0: min of a, 1: sum of ((a-b)/dt)^2
+__global__ void __launch_bounds__(RED_T) k_reduce(Blk b, const double *__restrict__ a,
+                                                  const double *__restrict__ bb,
+                                                  const double *__restrict__ dt, double *partial) {
+  __shared__ double sh[RED_T / 64];
+  long nx = b.i2 - b.i1, ny = b.j2 - b.j1, nz = b.k2 - b.k1;
+  long total = nx * ny * nz;
+  double acc = OP == 0 ? DBL_MAX : 0.0;
+  for (long t = blockIdx.x * (long)RED_T + threadIdx.x; t < total; t += (long)gridDim.x * RED_T) {
+    int i = b.i1 + (int)(t % nx);
+    long r = t / nx;
+    int j = b.j1 + (int)(r % ny);
+    int k = b.k1 + (int)(r / ny);
+    long n = b.idx(i, j, k);
+    if (OP == 0) {
+      acc = fmin(acc, a[n]);
+    } else {
+      double x = (a[n] - bb[n]) / dt[n];
+      acc += x * x;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_down(acc, off, 64);
+    acc = OP == 0 ? fmin(acc, o) : acc + o;
+  }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = sh[0];
+    for (int q = 1; q < RED_T / 64; q++) s = OP == 0 ? fmin(s, sh[q]) : s + sh[q];
+    partial[blockIdx.x] = s;
+  }
+}
+
+template <int OP>
+static int reduce_blocks(const codens_block *blocks, int nb, const double *a, const double *bb,
+                         const double *dt, double *out, hipStream_t st, const char *what) {
+  const int G = 512;
+  double *part = (double *)codens_workspace(sizeof(double) * G * nb, 15);
+  if (!part) {
+    codens_set_error("%s: workspace allocation failed", what);
+    return CODENS_EHIP;
+  }
+  for (int l = 0; l < nb; l++) {
+    int rc = codens_check_block(&blocks[l], what);
+    if (rc) return rc;
+    Blk b = make_blk(&blocks[l]);
+    hipLaunchKernelGGL(k_reduce<OP>, dim3(G), dim3(RED_T), 0, st, b, a, bb, dt, part + (long)l * G);
+    if ((rc = codens_check_launch(what))) return rc;
+  }
+  std::vector<double> h((size_t)G * nb);
+  if (hipMemcpyAsync(h.data(), part, sizeof(double) * G * nb, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    codens_set_error("%s: copy of partial results failed", what);
+    return CODENS_EHIP;
+  }
+  double s = OP == 0 ? DBL_MAX : 0.0;
+  for (double x : h) s = OP == 0 ? fmin(s, x) : s + x;
+  *out = s;
+  return CODENS_OK;
+}
+
+__global__ void k_fill(double *p, long n0, long n1, double val) {
+  long n = n0 + blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (n <= n1) p[n] = val;
+}
+
+extern "C" int codens_chrono_global(const codens_block *blocks, int32_t nb, double dt1min, double *dt,
+                                    double *dtmin_out, void *stream) {
+  if (!blocks || nb <= 0 || !dt) {
+    codens_set_error("codens_chrono_global: invalid argument");
+    return CODENS_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  double dtmin = 0;
+  int rc = reduce_blocks<0>(blocks, nb, dt, nullptr, nullptr, &dtmin, st, "codens_chrono_global");
+  if (rc) return rc;
+  if (dtmin_out) *dtmin_out = dtmin;
+  double dtm = fmin(dt1min, dtmin);
+  if (dtm < dt1min) {
+    codens_set_error("codens_chrono_global: time step %.6e below dt1min %.6e", dtmin, dt1min);
+    return CODENS_ESTOP;
+  }
+  for (int l = 0; l < nb; l++) {
+    Blk b = make_blk(&blocks[l]);
+    long n0 = b.npc, n1 = b.npc + b.nnn - 1;
+    long cnt = n1 - n0 + 1;
+    hipLaunchKernelGGL(k_fill, dim3((cnt + 255) / 256), dim3(256), 0, st, dt, n0, n1, dtm);
+    if ((rc = codens_check_launch("codens_chrono_global"))) return rc;
+  }
+  return CODENS_OK;
+}
+
+extern "C" int codens_sumsq_rate(const codens_block *blocks, int32_t nb, const double *a, const double *bv,
+                                 const double *dt, double *out, void *stream) {
+  if (!blocks || nb <= 0 || !a || !bv || !dt || !out) {
+    codens_set_error("codens_sumsq_rate: invalid argument");
+    return CODENS_EINVAL;
+  }
+  return reduce_blocks<1>(blocks, nb, a, bv, dt, out, (hipStream_t)stream, "codens_sumsq_rate");
+}
+
+// ---------------------------------------------------------------------------
+// Per-kernel launch timing with HIP events recorded on the launch stream.
+namespace {
+struct Timed {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
+std::mutex g_tm_mu;
+bool g_tm_on = false;
+std::map<std::string, Timed> g_tm;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_tm_pool;
+}  // namespace
+
+bool codens_timing_on() { return g_tm_on; }
+
+void *codens_timing_begin(const char *name, hipStream_t st) {
+  if (!g_tm_on) return nullptr;
+  std::lock_guard<std::mutex> lk(g_tm_mu);
+  std::pair<hipEvent_t, hipEvent_t> p;
+  if (!g_tm_pool.empty()) {
+    p = g_tm_pool.back();
+    g_tm_pool.pop_back();
+  } else {
+    if (hipEventCreate(&p.first) != hipSuccess || hipEventCreate(&p.second) != hipSuccess) return nullptr;
+  }
+  auto &t = g_tm[name];
+  t.ev.push_back(p);
+  (void)hipEventRecord(p.first, st);
+  return (void *)&t.ev.back();
+}
+
+void codens_timing_end(const char *name, hipStream_t st) {
+  if (!g_tm_on) return;
+  std::lock_guard<std::mutex> lk(g_tm_mu);
+  auto it = g_tm.find(name);
+  if (it == g_tm.end() || it->second.ev.empty()) return;
+  (void)hipEventRecord(it->second.ev.back().second, st);
+}
+
+extern "C" int codens_timing_enable(int32_t on) {
+  std::lock_guard<std::mutex> lk(g_tm_mu);
+  g_tm_on = on != 0;
+  return CODENS_OK;
+}
+
+extern "C" int codens_timing_read(const char *name, double *ms_total, int64_t *count, int32_t reset) {
+  if (!name || !ms_total || !count) {
+    codens_set_error("codens_timing_read: null argument");
+    return CODENS_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_tm_mu);
+  *ms_total = 0.0;
+  *count = 0;
+  auto it = g_tm.find(name);
+  if (it == g_tm.end()) return CODENS_OK;
+  for (auto &p : it->second.ev) {
+    if (hipEventSynchronize(p.second) != hipSuccess) {
+      codens_set_error("codens_timing_read: event synchronisation failed");
+      return CODENS_EHIP;
+    }
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.first, p.second) != hipSuccess) {
+      codens_set_error("codens_timing_read: elapsed time failed");
+      return CODENS_EHIP;
+    }
+    *ms_total += ms;
+    *count += 1;
+  }
+  if (reset) {
+    for (auto &p : it->second.ev) g_tm_pool.push_back(p);
+    it->second.ev.clear();
+  }
+  return CODENS_OK;
+}

@@ -1,0 +1,176 @@
+// Boundary ghost filling on gfx950, one thread per boundary face cell.
+// Face lists come from the block topology (initis.f90, initbs.f90): ghost
+// cell, adjacent real cell, and for coincident faces the neighbour's real cell.
+#include "common.hpp"
+
+using namespace codens;
+
+constexpr int BT = 256;
+
+// rbvc.f90:84-117 without periodic rotation
+__global__ void __launch_bounds__(BT) k_bc_coincident(long nf, const int64_t *__restrict__ g,
+                                                      const int64_t *__restrict__ in,
+                                                      const int64_t *__restrict__ rm, double *v, long ldv,
+                                                      double *ps, double *temp, double *cson) {
+  long f = blockIdx.x * (long)BT + threadIdx.x;
+  if (f >= nf) return;
+  long nd = g[f], ni = in[f], nr = rm[f];
+#pragma unroll
+  for (int e = 0; e < 5; e++) v[e * ldv + nd] = 0.5 * (v[e * ldv + ni] + v[e * ldv + nr]);
+  ps[nd] = 0.5 * (ps[ni] + ps[nr]);
+  temp[nd] = 0.5 * (temp[ni] + temp[nr]);
+  cson[nd] = 0.5 * (cson[ni] + cson[nr]);
+}
+
+// clpara.f90:44-58
+__global__ void __launch_bounds__(BT) k_bc_wall(long nf, const int64_t *__restrict__ g,
+                                                const int64_t *__restrict__ in, double *v, long ldv, double *ps,
+                                                double *temp, double *cson) {
+  long f = blockIdx.x * (long)BT + threadIdx.x;
+  if (f >= nf) return;
+  long nc = g[f], ni = in[f];
+  double r = v[ni], a = v[ldv + ni], bb = v[2 * ldv + ni], c = v[3 * ldv + ni];
+  double rhoe1 = v[4 * ldv + ni] - 0.5 * (a * a + bb * bb + c * c) / r;
+  v[nc] = r;
+  v[ldv + nc] = 0.;
+  v[2 * ldv + nc] = 0.;
+  v[3 * ldv + nc] = 0.;
+  v[4 * ldv + nc] = rhoe1;
+  ps[nc] = ps[ni];
+  temp[nc] = temp[ni];
+  cson[nc] = cson[ni];
+}
+
+// rbve.f90 then clnrd.f90:60-134 (characteristic far field)
+__global__ void __launch_bounds__(BT) k_bc_farfield(long nf, const int64_t *__restrict__ g,
+                                                    const int64_t *__restrict__ in,
+                                                    const double *__restrict__ nrm, double w00, double w01,
+                                                    double w02, double w03, double w04, Phys ph, double *v,
+                                                    long ldv, double *ps, double *temp, double *cson) {
+  long f = blockIdx.x * (long)BT + threadIdx.x;
+  if (f >= nf) return;
+  const double gam = ph.gam, gam1 = ph.gam1, pinfl = ph.pinfl;
+  long nl = g[f], ni = in[f];
+  double nx = nrm[f], ny = nrm[nf + f], nz = nrm[2 * nf + f];
+  double s0 = v[ni], s1 = v[ldv + ni], s2 = v[2 * ldv + ni], s3 = v[3 * ldv + ni], s4 = v[4 * ldv + ni];
+  double ro0 = s0;
+  double ps0 = gam1 * (s4 - 0.5 * (s1 * s1 + s2 * s2 + s3 * s3) / s0 - pinfl);
+  double roc0 = sqrt(gam * ps0 * ro0);
+  double roqn0 = s1 * nx + s2 * ny + s3 * nz;
+  double epsm = .5 + copysign(.5, roc0 - roqn0);
+  double eps0 = .5 + copysign(.5, -roqn0);
+  double epsp = .5 + copysign(.5, -roc0 - roqn0);
+  double rod = w00;
+  double qxd = w01 / rod, qyd = w02 / rod, qzd = w03 / rod;
+  double pd = gam1 * (w04 - .5 * rod * (qxd * qxd + qyd * qyd + qzd * qzd) - pinfl);
+  double qnd = qxd * nx + qyd * ny + qzd * nz;
+  double qtxd = qxd - qnd * nx, qtyd = qyd - qnd * ny, qtzd = qzd - qnd * nz;
+  double ros = s0;
+  double qxs = s1 / ros, qys = s2 / ros, qzs = s3 / ros;
+  double p_s = gam1 * (s4 - .5 * ros * (qxs * qxs + qys * qys + qzs * qzs) - pinfl);
+  double qns = qxs * nx + qys * ny + qzs * nz;
+  double qtxs = qxs - qns * nx, qtys = qys - qns * ny, qtzs = qzs - qns * nz;
+  double qtx = eps0 * qtxs + (1. - eps0) * qtxd;
+  double qty = eps0 * qtys + (1. - eps0) * qtyd;
+  double qtz = eps0 * qtzs + (1. - eps0) * qtzd;
+  double am = epsm * (p_s - roc0 * qns) + (1. - epsm) * (pd - roc0 * qnd);
+  double ap = epsp * (p_s + roc0 * qns) + (1. - epsp) * (pd + roc0 * qnd);
+  double qn = 0.5 * (ap - am) / roc0;
+  double p = 0.5 * (ap + am);
+  double bs = (p - p_s) * (ro0 * ro0) / (roc0 * roc0) + ros;
+  double b0 = (p - pd) * (ro0 * ro0) / (roc0 * roc0) + rod;
+  double ro = eps0 * bs + (1. - eps0) * b0;
+  double m1 = ro * (qtx + qn * nx), m2 = ro * (qty + qn * ny), m3 = ro * (qtz + qn * nz);
+  v[nl] = ro;
+  v[ldv + nl] = m1;
+  v[2 * ldv + nl] = m2;
+  v[3 * ldv + nl] = m3;
+  v[4 * ldv + nl] = p / gam1 + pinfl + 0.5 * (m1 * m1 + m2 * m2 + m3 * m3) / ro;
+  double t = gam * p / ro;
+  ps[nl] = p;
+  temp[nl] = t;
+  cson[nl] = sqrt(t);
+}
+
+// rbte.f90:46-59
+__global__ void __launch_bounds__(BT) k_bc_tq_copy(long nf, const int64_t *__restrict__ g,
+                                                   const int64_t *__restrict__ in, double *tq, long ldv) {
+  long f = blockIdx.x * (long)BT + threadIdx.x;
+  if (f >= nf) return;
+  long nd = g[f], ni = in[f];
+#pragma unroll
+  for (int q = 0; q < 9; q++) tq[q * ldv + nd] = tq[q * ldv + ni];
+}
+
+// rbtc.f90:104-117 without periodic rotation
+__global__ void __launch_bounds__(BT) k_bc_tq_coincident(long nf, const int64_t *__restrict__ g,
+                                                         const int64_t *__restrict__ in,
+                                                         const int64_t *__restrict__ rm, double *tq, long ldv) {
+  long f = blockIdx.x * (long)BT + threadIdx.x;
+  if (f >= nf) return;
+  long nd = g[f], ni = in[f], nr = rm[f];
+#pragma unroll
+  for (int q = 0; q < 9; q++) tq[q * ldv + nd] = 0.5 * (tq[q * ldv + ni] + tq[q * ldv + nr]);
+}
+
+static inline dim3 fgrid(int64_t nf) { return dim3((unsigned)((nf + BT - 1) / BT)); }
+
+extern "C" int codens_bc_coincident(int64_t nf, const int64_t *g, const int64_t *in, const int64_t *rm, double *v,
+                                    int64_t ldv, double *ps, double *temp, double *cson, void *stream) {
+  if (nf == 0) return CODENS_OK;
+  if (nf < 0 || !g || !in || !rm || !v || !ps || !temp || !cson) {
+    codens_set_error("codens_bc_coincident: invalid argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_bc_coincident, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, rm, v,
+                     (long)ldv, ps, temp, cson);
+  return codens_check_launch("codens_bc_coincident");
+}
+
+extern "C" int codens_bc_wall(int64_t nf, const int64_t *g, const int64_t *in, double *v, int64_t ldv, double *ps,
+                              double *temp, double *cson, void *stream) {
+  if (nf == 0) return CODENS_OK;
+  if (nf < 0 || !g || !in || !v || !ps || !temp || !cson) {
+    codens_set_error("codens_bc_wall: invalid argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_bc_wall, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, v, (long)ldv, ps,
+                     temp, cson);
+  return codens_check_launch("codens_bc_wall");
+}
+
+extern "C" int codens_bc_farfield(int64_t nf, const int64_t *g, const int64_t *in, const double *nrm,
+                                  const double *w0, const codens_phys *cp, double *v, int64_t ldv, double *ps,
+                                  double *temp, double *cson, void *stream) {
+  if (nf == 0) return CODENS_OK;
+  if (nf < 0 || !g || !in || !nrm || !w0 || !cp || !v || !ps || !temp || !cson) {
+    codens_set_error("codens_bc_farfield: invalid argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_bc_farfield, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, nrm, w0[0],
+                     w0[1], w0[2], w0[3], w0[4], make_phys(cp), v, (long)ldv, ps, temp, cson);
+  return codens_check_launch("codens_bc_farfield");
+}
+
+extern "C" int codens_bc_tq_copy(int64_t nf, const int64_t *g, const int64_t *in, double *tq, int64_t ldv,
+                                 void *stream) {
+  if (nf == 0) return CODENS_OK;
+  if (nf < 0 || !g || !in || !tq) {
+    codens_set_error("codens_bc_tq_copy: invalid argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_bc_tq_copy, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, tq, (long)ldv);
+  return codens_check_launch("codens_bc_tq_copy");
+}
+
+extern "C" int codens_bc_tq_coincident(int64_t nf, const int64_t *g, const int64_t *in, const int64_t *rm,
+                                       double *tq, int64_t ldv, void *stream) {
+  if (nf == 0) return CODENS_OK;
+  if (nf < 0 || !g || !in || !rm || !tq) {
+    codens_set_error("codens_bc_tq_coincident: invalid argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_bc_tq_coincident, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, rm, tq,
+                     (long)ldv);
+  return codens_check_launch("codens_bc_tq_coincident");
+}

@@ -1,0 +1,82 @@
+// Shared device-side definitions of the gfx950 residual-and-update kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "../../include/codens.h"
+
+namespace codens {
+
+// Block extents in the form the kernels use: flat entry of (i, j, k) is
+// npc + (i-id1) + (j-jd1)*nid + (k-kd1)*nijd (crdms.f90:126-153).
+struct Blk {
+  long npc, nid, nijd, nnn;
+  int id1, jd1, kd1;
+  int i1, i2, j1, j2, k1, k2;  // node extents: real cells i1..i2-1
+
+  __host__ __device__ inline long idx(int i, int j, int k) const {
+    return npc + (i - id1) + (long)(j - jd1) * nid + (long)(k - kd1) * nijd;
+  }
+  __host__ __device__ inline long inc(int d) const { return d == 0 ? 1 : (d == 1 ? nid : nijd); }
+  __host__ __device__ inline int lo(int d) const { return d == 0 ? i1 : (d == 1 ? j1 : k1); }
+  __host__ __device__ inline int hi(int d) const { return d == 0 ? i2 : (d == 1 ? j2 : k2); }
+  __host__ __device__ inline bool real(int i, int j, int k) const {
+    return i >= i1 && i < i2 && j >= j1 && j < j2 && k >= k1 && k < k2;
+  }
+};
+
+struct Phys {
+  double gam, gam1, pinfl, pr, prt, cp, reynz, tnz;
+  int iflu;
+};
+
+inline Blk make_blk(const codens_block *b) {
+  Blk k;
+  k.npc = b->npc;
+  k.nid = b->id2 - b->id1 + 1;
+  k.nijd = k.nid * (long)(b->jd2 - b->jd1 + 1);
+  k.nnn = b->nnn;
+  k.id1 = b->id1; k.jd1 = b->jd1; k.kd1 = b->kd1;
+  k.i1 = b->ii1; k.i2 = b->ii2; k.j1 = b->jj1; k.j2 = b->jj2; k.k1 = b->kk1; k.k2 = b->kk2;
+  return k;
+}
+
+inline Phys make_phys(const codens_phys *p) {
+  Phys q;
+  q.gam = p->gam; q.gam1 = p->gam1; q.pinfl = p->pinfl; q.pr = p->pr; q.prt = p->prt;
+  q.cp = p->cp; q.reynz = p->reynz; q.tnz = p->tnz; q.iflu = p->iflu;
+  return q;
+}
+
+// sn(m, d, c) of a block's metric slice (snorm.f90:65-71)
+__device__ inline double snv(const double *__restrict__ sn, long nnn, long m, int d, int c) {
+  return sn[((long)c * 3 + d) * nnn + m];
+}
+
+// Launch shape for kernels over the real cells: 64 (i) x 4 (j) threads,
+// one k plane per grid z.
+constexpr int TX = 64, TY = 4;
+inline dim3 cell_grid(const Blk &b) {
+  int nx = b.i2 - b.i1, ny = b.j2 - b.j1, nz = b.k2 - b.k1;
+  return dim3((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz);
+}
+
+}  // namespace codens
+
+// error reporting shared by the entry points (abi.cpp)
+extern "C" void codens_set_error(const char *fmt, ...);
+int codens_check_launch(const char *what);
+int codens_check_block(const codens_block *b, const char *what);
+void *codens_workspace(size_t bytes, int slot);
+bool codens_timing_on();
+void *codens_timing_begin(const char *name, hipStream_t st);
+void codens_timing_end(const char *name, hipStream_t st);
+
+// Brackets one kernel launch with timing events when timing is enabled.
+struct CodensTimed {
+  const char *name;
+  hipStream_t st;
+  CodensTimed(const char *n, hipStream_t s) : name(n), st(s) { codens_timing_begin(n, s); }
+  ~CodensTimed() { codens_timing_end(name, st); }
+};

@@ -1,0 +1,329 @@
+// Roe-MUSCL residual (sch_roe / sch_roe_euler) on gfx950.
+//
+// Three passes per call:
+//  1. (ilim=1 only) slope-ratio chains: one thread per grid line marches the
+//     faces of its line in storage order, because the reference replaces each
+//     slope difference by its ratio in place, so a face's ratio divides by the
+//     ratio (not the difference) of the face before it (sch_roe.f90:161-194).
+//  2. face fluxes: one thread per face, all faces of one direction per launch,
+//     written to a per-direction face array H (amount the high-side cell
+//     subtracts and the low-side cell adds).
+//  3. gather: u(c) = sum over directions of H(hi face) - H(lo face), plus the
+//     forcing ff -= u when ityprk != 0; ghosts of u are zeroed as the
+//     reference does (sch_roe.f90:110-125).
+// The dissipation |A| dW is evaluated as P (|Lambda| (Q dW)), the same
+// matrices as the reference's P |Lambda| Q product with a quarter of the work.
+#include "common.hpp"
+
+using namespace codens;
+
+struct Prim {
+  double r, u, v, w, p;
+};
+
+__device__ inline Prim prim_at(const double *__restrict__ v, long ldv, const double *__restrict__ ps, long n) {
+  Prim q;
+  q.r = v[n];
+  q.u = v[ldv + n] / q.r;
+  q.v = v[2 * ldv + n] / q.r;
+  q.w = v[3 * ldv + n] / q.r;
+  q.p = ps[n];
+  return q;
+}
+
+__device__ inline double pget(const Prim &q, int e) {
+  return e == 0 ? q.r : (e == 1 ? q.u : (e == 2 ? q.v : (e == 3 ? q.w : q.p)));
+}
+
+__device__ inline double minmod_phi(double a) { return fmax(0., fmin(1., a)); }
+
+// Roe flux (2x the numerical flux . S) of one face, sch_roe.f90:327-473.
+// tl, tr: stresses and heat flux of the left/right cells (9 values each) or
+// all zero for the Euler form.
+__device__ void roe_face(const Phys &ph, double s0, double s1, double s2, const double L[5], const double R[5],
+                         const double tl[9], const double tr[9], double f[5]) {
+  const double gam = ph.gam, gam1 = ph.gam1, pinfl = ph.pinfl;
+  double rhol = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
+  double rhor = R[0], ur = R[1], vr = R[2], wr = R[3], prr = R[4];
+  double cnds = sqrt(s0 * s0 + s1 * s1 + s2 * s2);
+  double nx = s0 / cnds, ny = s1 / cnds, nz = s2 / cnds;
+  double al2 = gam * pl / rhol, ar2 = gam * prr / rhor;
+  double q2l = ul * ul + vl * vl + wl * wl, q2r = ur * ur + vr * vr + wr * wr;
+  double hl = al2 / gam1 + 0.5 * q2l, hr = ar2 / gam1 + 0.5 * q2r;
+  double el = pl / (gam1 * rhol) + 0.5 * q2l + pinfl / rhol;
+  double er = prr / (gam1 * rhor) + 0.5 * q2r + pinfl / rhor;
+  double gd = sqrt(rhor / rhol), gd1 = 1. / (1. + gd), gd2 = gd * gd1;
+  double rhom = sqrt(rhol * rhor), rhomi = 1. / rhom;
+  double um = gd1 * ul + gd2 * ur, vm = gd1 * vl + gd2 * vr, wm = gd1 * wl + gd2 * wr;
+  double hm = gd1 * hl + gd2 * hr;
+  double vitm2 = 0.5 * (um * um + vm * vm + wm * wm);
+  double am = sqrt(gam1 * (hm - vitm2));
+  double am2i = 1. / (am * am);
+  double vn = um * nx + vm * ny + wm * nz;
+  double rhoiam = rhom / am, rhoami = am2i / rhoiam;
+  double l1 = fabs(vn * cnds), l4 = fabs(vn * cnds + am * cnds), l5 = fabs(vn * cnds - am * cnds);
+  double dw0 = rhor - rhol;
+  double dw1 = rhor * ur - rhol * ul;
+  double dw2 = rhor * vr - rhol * vl;
+  double dw3 = rhor * wr - rhol * wl;
+  double dw4 = rhor * er - rhol * el;
+  // a = Q dW (rows of Q as in sch_roe.f90:370-394)
+  double g = gam1 * vitm2 * am2i, ga = gam1 * am2i;
+  double qdw = gam1 * (um * dw1 + vm * dw2 + wm * dw3 - dw4);  // shared part of rows 1-3 (times am2i)
+  double a0 = (1. - g) * nx * dw0 - (vm * nz - wm * ny) * rhomi * dw0 + nx * am2i * qdw +
+              (nz * dw2 - ny * dw3) * rhomi;
+  double a1 = (1. - g) * ny * dw0 - (wm * nx - um * nz) * rhomi * dw0 + ny * am2i * qdw +
+              (nx * dw3 - nz * dw1) * rhomi;
+  double a2 = (1. - g) * nz * dw0 - (um * ny - vm * nx) * rhomi * dw0 + nz * am2i * qdw +
+              (ny * dw1 - nx * dw2) * rhomi;
+  double mdw = (nx * dw1 + ny * dw2 + nz * dw3) * rhomi;
+  double cdw = gam1 * rhoami * (vitm2 * dw0 - um * dw1 - vm * dw2 - wm * dw3 + dw4);
+  double a3 = cdw - vn * rhomi * dw0 + mdw;
+  double a4 = cdw + vn * rhomi * dw0 - mdw;
+  (void)ga;
+  a0 *= l1; a1 *= l1; a2 *= l1; a3 *= l4; a4 *= l5;
+  // di = P (|Lambda| a), P as in sch_roe.f90:396-420
+  double h = 0.5 * rhoiam;
+  double di0 = nx * a0 + ny * a1 + nz * a2 + h * (a3 + a4);
+  double di1 = um * nx * a0 + (um * ny - rhom * nz) * a1 + (um * nz + rhom * ny) * a2 +
+               h * ((um + nx * am) * a3 + (um - nx * am) * a4);
+  double di2 = (vm * nx + rhom * nz) * a0 + vm * ny * a1 + (vm * nz - rhom * nx) * a2 +
+               h * ((vm + ny * am) * a3 + (vm - ny * am) * a4);
+  double di3 = (wm * nx - rhom * ny) * a0 + (wm * ny + rhom * nx) * a1 + wm * nz * a2 +
+               h * ((wm + nz * am) * a3 + (wm - nz * am) * a4);
+  double di4 = (vitm2 * nx + rhom * (vm * nz - wm * ny)) * a0 + (vitm2 * ny + rhom * (wm * nx - um * nz)) * a1 +
+               (vitm2 * nz + rhom * (um * ny - vm * nx)) * a2 + h * ((hm + am * vn) * a3 + (hm - am * vn) * a4);
+  double dfxx = rhor * (ur * ur) + prr - pinfl - tr[0] + rhol * (ul * ul) + pl - pinfl - tl[0];
+  double dfxy = rhor * ur * vr - tr[1] + rhol * ul * vl - tl[1];
+  double dfxz = rhor * ur * wr - tr[2] + rhol * ul * wl - tl[2];
+  double dfyy = rhor * (vr * vr) + prr - pinfl - tr[3] + rhol * (vl * vl) + pl - pinfl - tl[3];
+  double dfyz = rhor * vr * wr - tr[4] + rhol * vl * wl - tl[4];
+  double dfzz = rhor * (wr * wr) + prr - pinfl - tr[5] + rhol * (wl * wl) + pl - pinfl - tl[5];
+  double Er = rhor * er + prr - pinfl, El = rhol * el + pl - pinfl;
+  double dfex = Er * ur + El * ul - (tr[0] * ur + tr[1] * vr + tr[2] * wr) - (tl[0] * ul + tl[1] * vl + tl[2] * wl) -
+                tr[6] - tl[6];
+  double dfey = Er * vr + El * vl - (tr[1] * ur + tr[3] * vr + tr[4] * wr) - (tl[1] * ul + tl[3] * vl + tl[4] * wl) -
+                tr[7] - tl[7];
+  double dfez = Er * wr + El * wl - (tr[2] * ur + tr[4] * vr + tr[5] * wr) - (tl[2] * ul + tl[4] * vl + tl[5] * wl) -
+                tr[8] - tl[8];
+  f[0] = (rhor * ur + rhol * ul) * s0 + (rhor * vr + rhol * vl) * s1 + (rhor * wr + rhol * wl) * s2 - di0;
+  f[1] = dfxx * s0 + dfxy * s1 + dfxz * s2 - di1;
+  f[2] = dfxy * s0 + dfyy * s1 + dfyz * s2 - di2;
+  f[3] = dfxz * s0 + dfyz * s1 + dfzz * s2 - di3;
+  f[4] = dfex * s0 + dfey * s1 + dfez * s2 - di4;
+}
+
+// ---------------------------------------------------------------------------
+// pass 1: chained slope ratios along lines of direction d
+__global__ void __launch_bounds__(64) k_limiter_chain(Blk b, int d, const double *__restrict__ v, long ldv,
+                                                      const double *__restrict__ ps, double *__restrict__ rr) {
+  int t = blockIdx.x * 64 + threadIdx.x;
+  int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;  // the two transverse directions
+  int na = b.hi(ta) - b.lo(ta), nb = b.hi(tb) - b.lo(tb);
+  if (t >= na * nb) return;
+  int c[3];
+  c[ta] = b.lo(ta) + t % na;
+  c[tb] = b.lo(tb) + t / na;
+  int a1 = b.lo(d), a2 = b.hi(d);
+  long inc = b.inc(d);
+  c[d] = a1;
+  long n = b.idx(c[0], c[1], c[2]);  // face a1: cells n-inc | n
+  Prim qm = prim_at(v, ldv, ps, n - inc), q0 = prim_at(v, ldv, ps, n), q1 = prim_at(v, ldv, ps, n + inc);
+  double prev[5], cur[5];
+#pragma unroll
+  for (int e = 0; e < 5; e++) {
+    prev[e] = pget(q0, e) - pget(qm, e);
+    cur[e] = pget(q1, e) - pget(q0, e);
+  }
+  Prim qc = q1;
+  for (int a = a1 + 1; a <= a2 - 1; a++) {
+    long nf = n + (long)(a - a1) * inc;  // face a: cells nf-inc | nf
+    Prim qn = prim_at(v, ldv, ps, nf + inc);
+    long m = nf - b.npc;
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+      double nxt = pget(qn, e) - pget(qc, e);
+      double r = fabs(prev[e] * cur[e] * nxt) <= DBL_MIN ? 1. : cur[e] / prev[e];
+      rr[e * b.nnn + m] = r;
+      prev[e] = r;
+      cur[e] = nxt;
+    }
+    qc = qn;
+  }
+}
+
+// physical flux . S of a ghost state (sch_roe.f90:497-578)
+__device__ inline void ghost_flux(const Phys &ph, const double w[5], double p, const double T[9], double rdiv,
+                                  int euler, double s0, double s1, double s2, double f[5]) {
+  double pinfl = ph.pinfl;
+  double fxx = w[1] * (w[1] / w[0]) + p - pinfl - T[0];
+  double fxy = w[2] * (w[1] / w[0]) - T[1];
+  double fxz = w[3] * (w[1] / w[0]) - T[2];
+  double fyy = w[2] * (w[2] / w[0]) + p - pinfl - T[3];
+  double fyz = w[3] * (w[2] / w[0]) - T[4];
+  double fzz = w[3] * (w[3] / w[0]) + p - pinfl - T[5];
+  double fex, fey, fez;
+  if (euler) {
+    fex = (w[4] + p - pinfl) * w[1] / rdiv;
+    fey = (w[4] + p - pinfl) * w[2] / rdiv;
+    fez = (w[4] + p - pinfl) * w[3] / rdiv;
+  } else {
+    fex = ((w[4] + p - pinfl - T[0]) * w[1] - T[1] * w[2] - T[2] * w[3]) / rdiv - T[6];
+    fey = ((w[4] + p - pinfl - T[3]) * w[2] - T[1] * w[1] - T[4] * w[3]) / rdiv - T[7];
+    fez = ((w[4] + p - pinfl - T[5]) * w[3] - T[2] * w[1] - T[4] * w[2]) / rdiv - T[8];
+  }
+  f[0] = w[1] * s0 + w[2] * s1 + w[3] * s2;
+  f[1] = fxx * s0 + fxy * s1 + fxz * s2;
+  f[2] = fxy * s0 + fyy * s1 + fyz * s2;
+  f[3] = fxz * s0 + fyz * s1 + fzz * s2;
+  f[4] = fex * s0 + fey * s1 + fez * s2;
+}
+
+// ---------------------------------------------------------------------------
+// pass 2: face fluxes of direction d
+__global__ void __launch_bounds__(TX *TY) k_face_flux(Blk b, Phys ph, int d, int ns, int ilim, double muscl,
+                                                      double xk, const double *__restrict__ v, long ldv,
+                                                      const double *__restrict__ ps,
+                                                      const double *__restrict__ tq,
+                                                      const double *__restrict__ sn,
+                                                      const double *__restrict__ rr, double *__restrict__ H) {
+  int c[3];
+  c[0] = b.i1 + blockIdx.x * TX + threadIdx.x;
+  c[1] = b.j1 + blockIdx.y * TY + threadIdx.y;
+  c[2] = b.k1 + blockIdx.z;
+  int hi0 = b.i2 - 1 + (d == 0), hi1 = b.j2 - 1 + (d == 1);
+  if (c[0] > hi0 || c[1] > hi1) return;
+  int a = c[d], a1 = b.lo(d), a2 = b.hi(d);
+  long inc = b.inc(d);
+  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+  double s0 = snv(sn, b.nnn, m, d, 0), s1 = snv(sn, b.nnn, m, d, 1), s2 = snv(sn, b.nnn, m, d, 2);
+  double f[5];
+  if (a == a1 || a == a2) {
+    long ng = a == a1 ? n - inc : n;
+    double w[5], T[9];
+#pragma unroll
+    for (int e = 0; e < 5; e++) w[e] = v[e * ldv + ng];
+#pragma unroll
+    for (int q = 0; q < 9; q++) T[q] = ns ? tq[q * ldv + ng] : 0.;
+    double rdiv = (!ns && a == a1 && d > 0) ? v[n] : w[0];  // sch_roe_euler.f90:630-632, 912-914
+    ghost_flux(ph, w, ps[ng], T, rdiv, !ns, s0, s1, s2, f);
+  } else {
+    Prim q[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) q[s] = prim_at(v, ldv, ps, n + (s - 2) * inc);
+    double L[5], R[5];
+    double cm = 0.25 * muscl;
+    if (ilim == 1 && a <= a2 - 2) {
+#pragma unroll
+      for (int e = 0; e < 5; e++) {
+        double rl = rr[e * b.nnn + m], rh = rr[e * b.nnn + m + inc];
+        double qa = pget(q[0], e), qb = pget(q[1], e), qc = pget(q[2], e), qd = pget(q[3], e);
+        L[e] = qb + cm * minmod_phi(rl) * (1. - xk) * (qb - qa) + cm * minmod_phi(1. / rl) * (1. + xk) * (qc - qb);
+        R[e] = qc - cm * minmod_phi(rh) * (1. + xk) * (qc - qb) - cm * minmod_phi(1. / rh) * (1. - xk) * (qd - qc);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 5; e++) {
+        double qa = pget(q[0], e), qb = pget(q[1], e), qc = pget(q[2], e), qd = pget(q[3], e);
+        L[e] = qb + cm * ((1. - xk) * (qb - qa) + (1. + xk) * (qc - qb));
+        R[e] = qc - cm * ((1. + xk) * (qc - qb) + (1. - xk) * (qd - qc));
+      }
+    }
+    double tl[9], tr[9];
+#pragma unroll
+    for (int q2 = 0; q2 < 9; q2++) {
+      tl[q2] = ns ? tq[q2 * ldv + n - inc] : 0.;
+      tr[q2] = ns ? tq[q2 * ldv + n] : 0.;
+    }
+    roe_face(ph, s0, s1, s2, L, R, tl, tr, f);
+#pragma unroll
+    for (int e = 0; e < 5; e++) f[e] *= 0.5;
+  }
+#pragma unroll
+  for (int e = 0; e < 5; e++) H[e * b.nnn + m] = f[e];
+}
+
+// ---------------------------------------------------------------------------
+// pass 3: gather over the zeroed range indc(i1-1,j1-1,k1-1)..indc(i2,j2,k2)
+__global__ void __launch_bounds__(256) k_gather(Blk b, long n0, long n1, const double *__restrict__ H, double *u,
+                                                long ldv, double *ff, int ityprk) {
+  long n = n0 + blockIdx.x * 256L + threadIdx.x;
+  if (n > n1) return;
+  long m = n - b.npc;
+  int i = b.id1 + (int)(m % b.nid);
+  int j = b.jd1 + (int)((m / b.nid) % (b.nijd / b.nid));
+  int k = b.kd1 + (int)(m / b.nijd);
+  const double *H0 = H, *H1 = H + 5 * b.nnn, *H2 = H + 10 * b.nnn;
+  if (!b.real(i, j, k)) {
+#pragma unroll
+    for (int e = 0; e < 5; e++) u[e * ldv + n] = 0.;
+    return;
+  }
+#pragma unroll
+  for (int e = 0; e < 5; e++) {
+    long o = e * b.nnn + m;
+    double r = H0[o + 1] - H0[o];
+    r = r + (H1[o + b.nid] - H1[o]);
+    r = r + (H2[o + b.nijd] - H2[o]);
+    u[e * ldv + n] = r;
+    if (ityprk) ff[e * ldv + n] = ff[e * ldv + n] - r;
+  }
+}
+
+static int roe_common(const codens_block *cb, const codens_phys *cp, const codens_scheme *sc, int ns,
+                      int ityprk, double *u, const double *v, long ldv, double *ff, const double *tq,
+                      const double *sn, const double *ps, hipStream_t st, const char *what) {
+  int rc = codens_check_block(cb, what);
+  if (rc) return rc;
+  if (!cp || !sc || !u || !v || !sn || !ps || (ns && !tq) || (ityprk && !ff)) {
+    codens_set_error("%s: null argument", what);
+    return CODENS_EINVAL;
+  }
+  Blk b = make_blk(cb);
+  Phys ph = make_phys(cp);
+  int ilim = ns ? sc->ilim : 0;
+  double *H = (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 0);
+  double *R = ilim == 1 ? (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 1) : nullptr;
+  if (!H || (ilim == 1 && !R)) {
+    codens_set_error("%s: workspace allocation failed", what);
+    return CODENS_EHIP;
+  }
+  for (int d = 0; d < 3; d++) {
+    if (ilim == 1) {
+      int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+      int nl = (b.hi(ta) - b.lo(ta)) * (b.hi(tb) - b.lo(tb));
+      {
+    CodensTimed _tm("roe_limiter_chain", st);
+    hipLaunchKernelGGL(k_limiter_chain, dim3((nl + 63) / 64), dim3(64), 0, st, b, d, v, ldv, ps,
+                         R + 5 * b.nnn * d);
+  }
+    }
+    int nx = b.i2 - b.i1 + (d == 0), ny = b.j2 - b.j1 + (d == 1), nz = b.k2 - b.k1 + (d == 2);
+    dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz);
+    {
+    CodensTimed _tm("roe_face_flux", st);
+    hipLaunchKernelGGL(k_face_flux, g, dim3(TX, TY), 0, st, b, ph, d, ns, ilim, (double)sc->muscl, sc->xk, v,
+                       ldv, ps, ns ? tq : v, sn, ilim == 1 ? R + 5 * b.nnn * d : nullptr, H + 5 * b.nnn * d);
+  }
+  }
+  long n0 = b.idx(b.i1 - 1, b.j1 - 1, b.k1 - 1), n1 = b.idx(b.i2, b.j2, b.k2);
+  {
+    CodensTimed _tm("roe_gather", st);
+    hipLaunchKernelGGL(k_gather, dim3((n1 - n0 + 256) / 256), dim3(256), 0, st, b, n0, n1, H, u, ldv, ff, ityprk);
+  }
+  return codens_check_launch(what);
+}
+
+extern "C" int codens_sch_roe_euler(const codens_block *b, const codens_phys *ph, const codens_scheme *sc,
+                                    int32_t ityprk, double *u, const double *v, int64_t ldv, double *ff,
+                                    const double *sn, const double *ps, void *stream) {
+  return roe_common(b, ph, sc, 0, ityprk, u, v, (long)ldv, ff, nullptr, sn, ps, (hipStream_t)stream,
+                    "codens_sch_roe_euler");
+}
+
+extern "C" int codens_sch_roe(const codens_block *b, const codens_phys *ph, const codens_scheme *sc,
+                              int32_t ityprk, double *u, const double *v, int64_t ldv, double *ff,
+                              const double *tq, const double *sn, const double *ps, void *stream) {
+  return roe_common(b, ph, sc, 1, ityprk, u, v, (long)ldv, ff, tq, sn, ps, (hipStream_t)stream,
+                    "codens_sch_roe");
+}

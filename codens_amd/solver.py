@@ -1,0 +1,136 @@
+"""Device-resident stage driver: the host-side mirror of atsch_num.f90 for the
+mean-flow paths of the synthetic cases.
+
+Every array lives in HBM for the whole run, in the reference storage; one
+stage is a fixed sequence of C-ABI launches on one stream with no host
+round trip (the global time step of kdtl=0 aside, which is a reduction the
+reference also synchronises on, chrono.f90:132).
+
+Stage order (atsch_num.f90): zpres (:245) -> zvismo (:424, NS) -> chronos
+(:436) -> rbord ghosts (:527; rbvc, walls, far fields, rbvc) -> zvisqc (:563,
+NS; then rbte/rbtc) -> sch_roe / sch_roe_euler (:720-751) -> sch_expli (kmf=0,
+:1021) or implimf_3d (kmf=1, :1040).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import abi
+from .configs import Case
+
+
+def _dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+class Solver:
+    def __init__(self, case: Case, stream=None):
+        self.case = case
+        lay = case.layout
+        self.lay = lay
+        n = lay.ip11
+        self.n = n
+        self.blocks = [abi.block(b) for b in lay.blocks]
+        self.ph = abi.phys(**case.phys)
+        self.sc = abi.scheme(**case.scheme)
+        self.stream = stream
+        f = lambda *s: torch.zeros(*s, dtype=torch.float64, device="cuda")
+        self.v = _dev(case.v0)
+        self.u = f(5, n)
+        self.d = f(5, n)
+        self.ff = f(5, n)
+        self.ps = f(n); self.temp = f(n); self.cson = f(n)
+        self.mu = f(n); self.mut = f(n); self.dt = f(n)
+        self.tq = f(9, n)
+        self.zero = f(n)
+        self.vol = _dev(case.vol)
+        self.sn = _dev(case.sn)
+        self.sn_blk = [self.sn[9 * b.npc: 9 * (b.npc + b.nnn)] for b in lay.blocks]
+        c, w, fa = case.coincident, case.wall, case.farfield
+        i64 = lambda a: _dev(a, torch.int64)
+        self.coin = (i64(c.ghost), i64(c.inner), i64(c.remote if c.remote is not None else c.inner))
+        self.wall = (i64(w.ghost), i64(w.inner))
+        self.far = (i64(fa.ghost), i64(fa.inner), _dev(fa.normals if fa.normals is not None else np.zeros((3, 0))))
+        self.tq_copy = (torch.cat([self.wall[0], self.far[0]]), torch.cat([self.wall[1], self.far[1]]))
+        self.nbad = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self.cycle = 0
+        self.res0 = None
+
+    # -- pieces of one stage ------------------------------------------------
+    def thermo(self):
+        s = self.stream
+        for b in self.blocks:
+            abi.zpres(b, self.ph, self.v, self.ps, self.temp, self.cson, stream=s)
+        if self.case.ns:
+            for b in self.blocks:
+                abi.zvismo(b, self.ph, self.temp, self.mu, self.nbad, stream=s)
+
+    def time_step(self):
+        s, cs = self.stream, self.case
+        for l, b in enumerate(self.blocks):
+            abi.chronos(b, self.ph, cs.eta, cs.ns, self.v, self.mu, self.mut, self.sn_blk[l], self.vol, self.cson,
+                        self.dt, stream=s)
+        if cs.kdtl == 0:
+            return abi.chrono_global(self.blocks, cs.dt1min, self.dt, stream=s)
+        return None
+
+    def ghosts(self):
+        s = self.stream
+        g, i, r = self.coin
+        if g.numel():
+            abi.bc_coincident(g, i, r, self.v, self.ps, self.temp, self.cson, stream=s)
+        if self.wall[0].numel():
+            abi.bc_wall(self.wall[0], self.wall[1], self.v, self.ps, self.temp, self.cson, stream=s)
+        if self.far[0].numel():
+            abi.bc_farfield(self.far[0], self.far[1], self.far[2], self.case.winf, self.ph, self.v, self.ps,
+                            self.temp, self.cson, stream=s)
+        if g.numel():
+            abi.bc_coincident(g, i, r, self.v, self.ps, self.temp, self.cson, stream=s)
+
+    def stresses(self):
+        s = self.stream
+        for l, b in enumerate(self.blocks):
+            abi.zvisqc(b, self.ph, self.sn_blk[l], self.vol, self.v, self.temp, self.mu, self.mut, self.tq, stream=s)
+        if self.tq_copy[0].numel():
+            abi.bc_tq_copy(self.tq_copy[0], self.tq_copy[1], self.tq, stream=s)
+        g, i, r = self.coin
+        if g.numel():
+            abi.bc_tq_coincident(g, i, r, self.tq, stream=s)
+
+    def residual(self):
+        s = self.stream
+        for l, b in enumerate(self.blocks):
+            if self.case.ns:
+                abi.sch_roe(b, self.ph, self.sc, 0, self.u, self.v, self.ff, self.tq, self.sn_blk[l], self.ps, stream=s)
+            else:
+                abi.sch_roe_euler(b, self.ph, self.sc, 0, self.u, self.v, self.ff, self.sn_blk[l], self.ps, stream=s)
+
+    def update(self):
+        s, cs = self.stream, self.case
+        for l, b in enumerate(self.blocks):
+            if cs.kmf == 0:
+                abi.sch_expli(b, self.u, self.v, self.dt, self.vol, stream=s)
+            else:
+                abi.implimf_3d(b, self.ph, self.u, self.dt, self.v, self.d, self.ff, self.mu, self.mut, cs.lmax,
+                               self.sn_blk[l], self.vol, self.ps, self.cson, stream=s)
+
+    def stage(self):
+        """One residual + update of all blocks (atsch_num.f90:5)."""
+        self.thermo()
+        self.time_step()
+        self.ghosts()
+        if self.case.ns:
+            self.stresses()
+        self.residual()
+        self.update()
+        self.cycle += 1
+
+    def density_residual(self):
+        """||(rho^{n+1}-rho^n)/dt||_2 of the last implicit stage: the increment
+        d holds rho^{n+1}-rho^n on the real cells (dua_resro.f90:63-69)."""
+        return float(np.sqrt(abi.sumsq_rate(self.blocks, self.d[0], self.zero, self.dt, stream=self.stream)))
+
+    def host_state(self):
+        torch.cuda.synchronize()
+        return self.v.cpu().numpy()

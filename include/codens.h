@@ -1,0 +1,162 @@
+/* codens.h -- C-ABI of the MI355X (gfx950) residual-and-update path of CodeNS.
+ *
+ * Every entry point replaces one reference Fortran routine (cited on each
+ * declaration).  Arrays are device pointers in the reference storage:
+ *
+ *  - one flat storage over all blocks, i fastest, ghost layers included
+ *    (crdms.f90:126-153); a block's entry n is npc + (i-id1) + (j-jd1)*nid
+ *    + (k-kd1)*nid*njd, 0-based;
+ *  - multi-variable arrays are SoA with leading dimension ldv = ip11
+ *    (v(ip11,ip60), solve.f90:982): variable e of entry n is at e*ldv + n;
+ *  - sn_blk is the block's own metric slice sn(lgsnlt,3,3) (snorm.f90:65-71),
+ *    i.e. sn + 9*npc of the global array (atsch_num.f90:623); component c of
+ *    the face family d (0=i, 1=j, 2=k) of entry m is sn_blk[(c*3+d)*nnn + m];
+ *  - tq is the 9-array bundle toxx,toxy,toxz,toyy,toyz,tozz,qcx,qcy,qcz of
+ *    leading dimension ldv (zfluto.f90:104-121).
+ *
+ * All calls are asynchronous on `stream` (a hipStream_t; NULL = default
+ * stream) unless stated otherwise.  Return value: 0 on success, otherwise a
+ * CODENS_E* code; codens_last_error() gives the message.
+ */
+#ifndef CODENS_H
+#define CODENS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Extents of one block, the `maillage` module entries for block lm
+ * (modules.f90:31-44, set by crdms.f90:126-153). */
+typedef struct codens_block {
+  int32_t npc;                      /* offset of the block's first entry (= npn) */
+  int32_t id1, id2, jd1, jd2, kd1, kd2; /* storage extents, ghosts included */
+  int32_t ii1, ii2, jj1, jj2, kk1, kk2; /* node extents; real cells ii1..ii2-1 */
+  int32_t nnn;                      /* entries of the block (= lgsnlt) */
+  int32_t pad0, pad1;
+} codens_block;
+
+/* Gas constants, module `proprieteflu` (modules.f90:116-122) and tnz
+ * (module `definition`).  cp = 1/(gam-1) (dfnzst.f90:67). */
+typedef struct codens_phys {
+  double gam, gam1, pinfl, pr, prt, cp, reynz, tnz;
+  int32_t iflu, pad;
+} codens_phys;
+
+/* MUSCL switches, module `schemanum` (modules.f90:151-166). */
+typedef struct codens_scheme {
+  int32_t muscl, ilim;
+  double xk;
+} codens_scheme;
+
+enum {
+  CODENS_OK = 0,
+  CODENS_EINVAL = 1,     /* bad extents, null pointer, size over a limit */
+  CODENS_EHIP = 2,       /* a HIP call or launch failed */
+  CODENS_ENONPHYS = 3,   /* non-physical state (T <= 0), where the reference aborts */
+  CODENS_ESTOP = 4       /* global time step below dt1min, where the reference stops */
+};
+
+const char *codens_last_error(void);
+
+/* zpres.f90:4 -- p, T, c of the real cells. */
+int codens_zpres(const codens_block *b, const codens_phys *ph, const double *v, int64_t ldv,
+                 double *ps, double *temp, double *cson, void *stream);
+
+/* zvismo.f90:4 -- molecular viscosity of the real cells (Sutherland for
+ * iflu=1).  Counts cells with T <= 0 into *nbad_dev (device int, may be NULL);
+ * the reference aborts on any (zvismo.f90:122-127). */
+int codens_zvismo(const codens_block *b, const codens_phys *ph, const double *temp, double *mu,
+                  int32_t *nbad_dev, void *stream);
+
+/* chronos.f90:4 -- local time step of the real cells; ns != 0 adds the
+ * viscous limit (equat 'ns', chronos.f90:177-193). */
+int codens_chronos(const codens_block *b, const codens_phys *ph, double eta, int32_t ns,
+                   const double *v, int64_t ldv, const double *mu, const double *mut,
+                   const double *sn_blk, const double *vol, const double *cson, double *dt,
+                   void *stream);
+
+/* chrono.f90:105-153 (kdtl=0) -- global step over nb blocks.  Synchronous:
+ * *dtmin_out receives the smallest local step; returns CODENS_ESTOP (dt left
+ * untouched) when it is below dt1min, else sets every stored entry of every
+ * block to dt1min. */
+int codens_chrono_global(const codens_block *blocks, int32_t nb, double dt1min, double *dt,
+                         double *dtmin_out, void *stream);
+
+/* sch_roe_euler.f90:4 -- Euler Roe residual u = sum of outgoing F.S with
+ * unlimited kappa-MUSCL; ityprk != 0 also does ff -= u (:976-990). */
+int codens_sch_roe_euler(const codens_block *b, const codens_phys *ph, const codens_scheme *sc,
+                         int32_t ityprk, double *u, const double *v, int64_t ldv, double *ff,
+                         const double *sn_blk, const double *ps, void *stream);
+
+/* sch_roe.f90:4 -- Navier-Stokes Roe residual: MUSCL (minmod with the
+ * chained slope ratios when ilim=1) and face-averaged stresses tq. */
+int codens_sch_roe(const codens_block *b, const codens_phys *ph, const codens_scheme *sc,
+                   int32_t ityprk, double *u, const double *v, int64_t ldv, double *ff,
+                   const double *tq, const double *sn_blk, const double *ps, void *stream);
+
+/* zvisqc.f90:134-178 (zgrad2.f90:4 then zfluto.f90:4) -- stresses and heat
+ * flux tq of the real cells from Green-Gauss gradients of u, v, w, T. */
+int codens_zvisqc(const codens_block *b, const codens_phys *ph, const double *sn_blk,
+                  const double *vol, const double *v, int64_t ldv, const double *temp,
+                  const double *mu, const double *mut, double *tq, void *stream);
+
+/* sch_expli.f90:4 -- v += dt/vol * u on the real cells. */
+int codens_sch_expli(const codens_block *b, const double *u, double *v, int64_t ldv,
+                     const double *dt, const double *vol, void *stream);
+
+/* implimf_3d.f90:4 -- matrix-free point-Jacobi implicit phase: coefficients,
+ * lmx sub-iterations, then v += d.  d (5 arrays of ldv) receives the final
+ * increment, zero on the ghosts.  ff is read only when ityprk != 0; dtpas
+ * only when kfmg = 3.  Uses a library-owned device workspace. */
+int codens_implimf_3d(const codens_block *b, const codens_phys *ph, const double *u,
+                      const double *dt, double *v, int64_t ldv, double *d, const double *ff,
+                      const double *mu, const double *mut, int32_t lmx, const double *sn_blk,
+                      const double *vol, double dtpas, int32_t kfmg, int32_t ityprk,
+                      const double *ps, const double *cson, void *stream);
+
+/* Boundary ghosts from face lists (int64 device arrays of flat indices, in
+ * the listing order of initis.f90:71-150): ghost cell, adjacent real cell,
+ * and for coincident faces the neighbour's adjacent real cell. */
+
+/* rbvc.f90:84-117 -- coincident faces: ghost = average of both real cells. */
+int codens_bc_coincident(int64_t nf, const int64_t *ghost, const int64_t *inner,
+                         const int64_t *remote, double *v, int64_t ldv, double *ps, double *temp,
+                         double *cson, void *stream);
+/* clpara.f90:44-58 -- adiabatic no-slip wall. */
+int codens_bc_wall(int64_t nf, const int64_t *ghost, const int64_t *inner, double *v, int64_t ldv,
+                   double *ps, double *temp, double *cson, void *stream);
+/* rbve.f90 + clnrd.f90:60-134 -- non-reflecting far field against the
+ * conservative state w0[5] (host array); nrm: 3 device arrays of nf unit
+ * normals pointing into the block (initns.f90:56-163). */
+int codens_bc_farfield(int64_t nf, const int64_t *ghost, const int64_t *inner, const double *nrm,
+                       const double *w0, const codens_phys *ph, double *v, int64_t ldv, double *ps,
+                       double *temp, double *cson, void *stream);
+/* rbte.f90:46-59 -- tq ghost = adjacent real cell. */
+int codens_bc_tq_copy(int64_t nf, const int64_t *ghost, const int64_t *inner, double *tq,
+                      int64_t ldv, void *stream);
+/* rbtc.f90:104-117 -- tq ghost = average of both real cells. */
+int codens_bc_tq_coincident(int64_t nf, const int64_t *ghost, const int64_t *inner,
+                            const int64_t *remote, double *tq, int64_t ldv, void *stream);
+
+/* dua_resro.f90:63-69 / residu.f90:131-187 -- sum over the real cells of all
+ * nb blocks of ((a - b)/dt)^2 for one variable (a, b: arrays of ldv).
+ * Synchronous; the sum is written to *out. */
+int codens_sumsq_rate(const codens_block *blocks, int32_t nb, const double *a, const double *bv,
+                      const double *dt, double *out, void *stream);
+
+/* Launch timing (the reference has only a wall clock around `compute flow`,
+ * solve.f90:487-515): when enabled, every launch of the main kernels is
+ * bracketed by HIP events on its own stream.  Names: "zpres", "chronos",
+ * "zvisqc", "roe_limiter_chain", "roe_face_flux", "roe_gather",
+ * "implimf_coef", "implimf_sweep", "implimf_update", "sch_expli".
+ * codens_timing_read synchronises on the recorded events and returns the
+ * summed duration and launch count of one kernel; reset != 0 clears it. */
+int codens_timing_enable(int32_t on);
+int codens_timing_read(const char *name, double *ms_total, int64_t *count, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CODENS_H */

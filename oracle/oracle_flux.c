@@ -454,3 +454,9 @@ void orc_sch_roe(const orc_block *b, const orc_phys *ph, const orc_sch *sc, int 
                  const double *ps) {
   roe_residual(b, ph, sc, 1, ityprk, u, v, ldv, ff, tq, sn, ps);
 }
+
+/* Test hook: the Roe face flux of two primitive states (rho,u,v,w,p) with
+ * the Euler form (no stresses); f = 2x numerical flux . S. */
+void orc_roe_face(const orc_phys *ph, const double *s, const double *L, const double *R, double *f) {
+  roe_face(ph, s, L[0], L[1], L[2], L[3], L[4], R[0], R[1], R[2], R[3], R[4], NULL, NULL, f);
+}

@@ -1,0 +1,197 @@
+"""Kernel-level parity of every C-ABI entry point against the oracle, on
+seeded random states of a two-block curvilinear layout (8x6x5 and 5x4x3
+cells).  Bar: max|gpu - oracle| / max|oracle| <= 1e-12 per field."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+import cpu_ref as C
+from randcase import all_thermo, random_mesh, random_state, relerr
+
+TOL = 1e-12
+DIMS = [(8, 6, 5), (5, 4, 3)]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from codens_amd import abi
+    assert torch.cuda.is_available()
+    lay, sn, vol = random_mesh(DIMS, seed=11)
+    return abi, lay, sn, vol
+
+
+def T(a, dtype=None):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda", dtype=dtype or torch.float64)
+
+
+def H(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+@pytest.mark.parametrize("pinfl", [0.0, 0.3])
+def test_thermo_dt(ctx, pinfl):
+    abi, lay, sn, vol = ctx
+    rng = np.random.default_rng(1)
+    n = lay.ip11
+    v = random_state(n, rng, pinfl)
+    ph_o = C.orc_phys(pinfl=pinfl)
+    O = C.Orc(lay, ph_o)
+    ph = abi.phys(pinfl=pinfl)
+    ps, t, c = all_thermo(v, pinfl)
+    ps_o, t_o, c_o = ps.copy(), t.copy(), c.copy()
+    mu_o = np.zeros(n); mut = np.zeros(n); dt_o = np.zeros(n)
+    gv = T(v); gps, gt, gc = T(ps), T(t), T(c); gmu = T(np.zeros(n)); gmut = T(mut); gdt = T(np.zeros(n))
+    for l, b in enumerate(lay.blocks):
+        O.zpres(l, v, ps_o, t_o, c_o)
+        O.zvismo(l, mu_o, v, t_o)
+        cb = abi.block(b)
+        abi.zpres(cb, ph, gv, gps, gt, gc)
+        abi.zvismo(cb, ph, gt, gmu)
+    for name, a, b_ in (("ps", gps, ps_o), ("temp", gt, t_o), ("cson", gc, c_o), ("mu", gmu, mu_o)):
+        assert relerr(H(a), b_) <= TOL, name
+    for ns in (0, 1):
+        for l, b in enumerate(lay.blocks):
+            O.ns = bool(ns)
+            O.chronos(l, 0.8, mu_o, mut, v, dt_o, sn, vol, c_o)
+            abi.chronos(abi.block(b), ph, 0.8, ns, gv, gmu, gmut, T(sn[9 * b.npc: 9 * (b.npc + b.nnn)]), T(vol), gc, gdt)
+        assert relerr(H(gdt), dt_o) <= TOL, f"chronos ns={ns}"
+
+
+@pytest.mark.parametrize("ns,ilim,pinfl", [(0, 0, 0.0), (0, 0, 0.3), (1, 0, 0.0), (1, 1, 0.0), (1, 1, 0.2)])
+def test_roe_residual(ctx, ns, ilim, pinfl):
+    abi, lay, sn, vol = ctx
+    rng = np.random.default_rng(2 + ilim)
+    n = lay.ip11
+    v = random_state(n, rng, pinfl)
+    ps, _, _ = all_thermo(v, pinfl)
+    tq = 0.01 * rng.standard_normal((9, n)) if ns else np.zeros((9, n))
+    O = C.Orc(lay, C.orc_phys(pinfl=pinfl), muscl=1, xk=1 / 3, ilim=ilim, ns=bool(ns))
+    ph, sc = abi.phys(pinfl=pinfl), abi.scheme(1, ilim, 1 / 3)
+    u_o = np.zeros((5, n)); ff_o = np.ones((5, n))
+    gu = T(np.zeros((5, n))); gff = T(np.ones((5, n))); gv = T(v); gps = T(ps); gtq = T(tq)
+    for l, b in enumerate(lay.blocks):
+        snb = T(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+        if ns:
+            O.sch_roe(l, 1, u_o, v, ff_o, tq, sn, ps)
+            abi.sch_roe(abi.block(b), ph, sc, 1, gu, gv, gff, gtq, snb, gps)
+        else:
+            O.sch_roe_euler(l, 1, u_o, v, ff_o, sn, ps)
+            abi.sch_roe_euler(abi.block(b), ph, sc, 1, gu, gv, gff, snb, gps)
+    gu_h, gff_h = H(gu), H(gff)
+    for e in range(5):
+        assert relerr(gu_h[e], u_o[e]) <= TOL, f"u[{e}]"
+        assert relerr(gff_h[e], ff_o[e]) <= TOL, f"ff[{e}]"
+
+
+def test_zvisqc(ctx):
+    abi, lay, sn, vol = ctx
+    rng = np.random.default_rng(5)
+    n = lay.ip11
+    v = random_state(n, rng)
+    _, t, _ = all_thermo(v)
+    mu = 1e-3 * (1 + 0.1 * rng.random(n)); mut = 1e-3 * rng.random(n)
+    O = C.Orc(lay, C.orc_phys())
+    ph = abi.phys()
+    tq_o = np.zeros((9, n)); gtq = T(np.zeros((9, n)))
+    for l, b in enumerate(lay.blocks):
+        g = O.zgrad2(l, sn, vol, v, t)
+        O.zfluto(l, mu, mut, tq_o, g)
+        abi.zvisqc(abi.block(b), ph, T(sn[9 * b.npc: 9 * (b.npc + b.nnn)]), T(vol), T(v), T(t), T(mu), T(mut), gtq)
+    h = H(gtq)
+    for q in range(9):
+        assert relerr(h[q], tq_o[q]) <= TOL, f"tq[{q}]"
+
+
+@pytest.mark.parametrize("lmx,ityprk", [(1, 0), (3, 0), (10, 0), (2, 1)])
+def test_implimf_3d(ctx, lmx, ityprk):
+    abi, lay, sn, vol = ctx
+    rng = np.random.default_rng(7 + lmx)
+    n = lay.ip11
+    v = random_state(n, rng)
+    ps, t, c = all_thermo(v)
+    mu = 1e-3 * (1 + 0.1 * rng.random(n)); mut = 1e-4 * rng.random(n)
+    u = 1e-3 * rng.standard_normal((5, n)); ff = 1e-4 * rng.standard_normal((5, n))
+    dt = 0.01 * (1 + rng.random(n))
+    O = C.Orc(lay, C.orc_phys())
+    ph = abi.phys()
+    v_o = v.copy(); d_o = np.zeros((5, n))
+    gv = T(v); gd = T(np.zeros((5, n)))
+    for l, b in enumerate(lay.blocks):
+        O.implimf_3d(l, u, dt, v_o, d_o, ff, mu, mut, lmx, sn, vol, ps, c, ityprk=ityprk)
+        abi.implimf_3d(abi.block(b), ph, T(u), T(dt), gv, gd, T(ff), T(mu), T(mut), lmx,
+                       T(sn[9 * b.npc: 9 * (b.npc + b.nnn)]), T(vol), T(ps), T(c), ityprk=ityprk)
+    hv, hd = H(gv), H(gd)
+    for e in range(5):
+        assert relerr(hd[e], d_o[e]) <= TOL, f"d[{e}]"
+        assert relerr(hv[e], v_o[e]) <= TOL, f"v[{e}]"
+
+
+def test_sch_expli_and_reductions(ctx):
+    abi, lay, sn, vol = ctx
+    rng = np.random.default_rng(9)
+    n = lay.ip11
+    v = random_state(n, rng); u = rng.standard_normal((5, n)); dt = 0.01 * (1 + rng.random(n))
+    O = C.Orc(lay, C.orc_phys())
+    v_o = v.copy(); gv = T(v)
+    for l, b in enumerate(lay.blocks):
+        O.sch_expli(l, u, v_o, dt, vol)
+        abi.sch_expli(abi.block(b), T(u), gv, T(dt), T(vol))
+    assert relerr(H(gv), v_o) <= TOL
+    blocks = [abi.block(b) for b in lay.blocks]
+    inner = lay.interior()
+    a, bb = rng.standard_normal(n), rng.standard_normal(n)
+    s = abi.sumsq_rate(blocks, T(a), T(bb), T(dt))
+    ref = float(np.sum(((a[inner] - bb[inner]) / dt[inner]) ** 2))
+    assert abs(s - ref) <= 1e-12 * ref
+    gdt = T(dt)
+    dtmin = abi.chrono_global(blocks, 0.5 * dt[inner].min(), gdt)
+    assert dtmin == dt[inner].min()
+    assert np.all(H(gdt) == 0.5 * dt[inner].min())
+    from codens_amd.abi import CodensError
+    with pytest.raises(CodensError):
+        abi.chrono_global(blocks, 2.0, T(dt))
+
+
+def test_boundary_ghosts(ctx):
+    abi, lay, sn, vol = ctx
+    from codens_amd import layout as L
+    rng = np.random.default_rng(13)
+    n = lay.ip11
+    v = random_state(n, rng)
+    ps, t, c = all_thermo(v)
+    b = lay.blocks[0]
+    O = C.Orc(lay, C.orc_phys())
+    ph = abi.phys()
+    g1, i1, _ = L.face_cells(b, "i1"); g2, i2, _ = L.face_cells(b, "i2")
+    gw, iw, _ = L.face_cells(b, "j1")
+    gf, inf, fi = L.face_cells(b, "k2")
+    nrm = np.stack(L.inward_normals(b, sn, "k2", fi))
+    w0 = np.array([1.0, 0.5, 0.05, -0.02, 1 / 0.4 / 1.4 + 0.5 * (0.25 + 0.0025 + 0.0004)])
+    arrs_o = [v.copy(), ps.copy(), t.copy(), c.copy()]
+    vg, psg, tg, cg = T(v), T(ps), T(t), T(c)
+    O.bc_coincident(g1, i1, i2, *arrs_o)
+    abi.bc_coincident(T(g1, torch.int64), T(i1, torch.int64), T(i2, torch.int64), vg, psg, tg, cg)
+    O.bc_wall(gw, iw, *arrs_o)
+    abi.bc_wall(T(gw, torch.int64), T(iw, torch.int64), vg, psg, tg, cg)
+    O.bc_farfield(gf, inf, nrm[0].copy(), nrm[1].copy(), nrm[2].copy(), w0, *arrs_o)
+    abi.bc_farfield(T(gf, torch.int64), T(inf, torch.int64), T(nrm), w0, ph, vg, psg, tg, cg)
+    for name, a, r in zip(("v", "ps", "temp", "cson"), (vg, psg, tg, cg), arrs_o):
+        assert relerr(H(a), r) <= TOL, name
+    tq = rng.standard_normal((9, n)); tq_o = tq.copy(); gtq = T(tq)
+    O.bc_tq_copy(gw, iw, tq_o); abi.bc_tq_copy(T(gw, torch.int64), T(iw, torch.int64), gtq)
+    O.bc_tq_coincident(g2, i2, i1, tq_o); abi.bc_tq_coincident(T(g2, torch.int64), T(i2, torch.int64), T(i1, torch.int64), gtq)
+    assert relerr(H(gtq), tq_o) <= TOL
+
+
+def test_bad_extents_rejected(ctx):
+    abi, lay, sn, vol = ctx
+    from codens_amd.abi import CodensError
+    cb = abi.block(lay.blocks[0])
+    cb.nnn = cb.nnn + 1
+    x = T(np.zeros((5, lay.ip11)))
+    with pytest.raises(CodensError):
+        abi.zpres(cb, abi.phys(), x, x[0], x[1], x[2])

@@ -46,6 +46,9 @@ SIGNATURES = {
     "codens_sch_roe": [P(CodensBlock), P(CodensPhys), P(CodensScheme), c_int, c_vp, c_vp, c_i64, c_vp, c_vp,
                        c_vp, c_vp, c_vp],
     "codens_zvisqc": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "codens_dissip_jameson": [P(CodensBlock), c_dbl, c_dbl, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "codens_met_roe2o": [P(CodensBlock), c_vp, c_i64, c_vp, c_vp, c_vp],
+    "codens_met_roe": [P(CodensBlock), c_vp, c_i64, c_vp, c_vp, c_vp],
     "codens_sch_expli": [P(CodensBlock), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
     "codens_implimf_3d": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int,
                           c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp, c_vp],

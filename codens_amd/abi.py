@@ -97,6 +97,19 @@ def zvisqc(b, ph, sn_blk, vol, v, temp, mu, mut, tq, stream=None):
                            _p(mut), _p(tq), _s(stream)))
 
 
+def dissip_jameson(b, ki2, ki4, v, d, sn_blk, ps, cson, stream=None):
+    _chk(lib.codens_dissip_jameson(ctypes.byref(b), ki2, ki4, _p(v), ld(v), _p(d), _p(sn_blk), _p(ps), _p(cson),
+                                   _s(stream)))
+
+
+def met_roe2o(b, t, d, sn_blk, stream=None):
+    _chk(lib.codens_met_roe2o(ctypes.byref(b), _p(t), ld(t), _p(d), _p(sn_blk), _s(stream)))
+
+
+def met_roe(b, t, d, sn_blk, stream=None):
+    _chk(lib.codens_met_roe(ctypes.byref(b), _p(t), ld(t), _p(d), _p(sn_blk), _s(stream)))
+
+
 def sch_expli(b, u, v, dt, vol, stream=None):
     _chk(lib.codens_sch_expli(ctypes.byref(b), _p(u), _p(v), ld(v), _p(dt), _p(vol), _s(stream)))
 

@@ -82,7 +82,8 @@ def cell_centres(b: L.Block, x, y, z):
     return c(X), c(Y), c(Z)  # shape (nkd-1, njd-1, nid-1); entry [k,j,i] = cell (i,j,k) in storage offsets
 
 
-def periodic_euler(ci=64, cj=32, ck=4, seed=20261201, mach=0.8, cfl=0.8, amp=0.01, distort=0.0) -> Case:
+def periodic_euler(ci=64, cj=32, ck=4, seed=20261201, mach=0.8, cfl=0.8, amp=0.01, distort=0.0,
+                   dt_factor=0.1) -> Case:
     lay = L.Layout([(ci, cj, ck)])
     b = lay.blocks[0]
     x = np.zeros(lay.ip11); y = np.zeros(lay.ip11); z = np.zeros(lay.ip11)
@@ -120,14 +121,19 @@ def periodic_euler(ci=64, cj=32, ck=4, seed=20261201, mach=0.8, cfl=0.8, amp=0.0
         parts.append(FaceSet(g2, i2, i1, None))
     coin = _cat(parts)
     coin.normals = np.zeros((3, len(coin.ghost)))
-    # global step (kdtl=0): dt1min is 0.9 x the smallest CFL-0.8 local step of
-    # the initial field (chronos.f90:150-170 on the uniform mesh: dism = h/sqrt(3))
+    # global step (kdtl=0): every stored dt becomes dt1min (chrono.f90:133-152).
+    # sch_expli.f90:59-63 adds dt/vol * u with u the net OUTGOING flux, so the
+    # explicit path runs backward in time and amplifies; at the full CFL-0.8
+    # step the reference stops at step 22 (chrono's dt1min check).  dt1min is
+    # dt_factor x the smallest CFL-0.8 local step of the initial field
+    # (chronos.f90:150-170 on the uniform mesh: dism = h/sqrt(3)); 0.1 keeps
+    # 50 steps bounded.
     inner = b.interior()
     r = v0[0, inner]
     q = np.sqrt((v0[1, inner] ** 2 + v0[2, inner] ** 2 + v0[3, inner] ** 2) / r ** 2)
     pr_ = (GAM - 1.0) * (v0[4, inner] - 0.5 * (v0[1, inner] ** 2 + v0[2, inner] ** 2 + v0[3, inner] ** 2) / r)
     c = np.sqrt(GAM * pr_ / r)
-    dt1min = 0.9 * float(np.min(cfl * (h / np.sqrt(3.0)) / (q + c)))
+    dt1min = dt_factor * float(np.min(cfl * (h / np.sqrt(3.0)) / (q + c)))
     return Case("periodic_euler", lay, sn, vol, v0, False,
                 dict(gam=GAM, pinfl=0.0, pr=0.72, prt=1.0, reynz=1e5, tnz=288.15, iflu=1),
                 dict(muscl=1, ilim=0, xk=1.0 / 3.0), cfl, 0, dt1min, 0, 0,

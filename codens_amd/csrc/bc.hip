@@ -3,6 +3,10 @@
 // cell, adjacent real cell, and for coincident faces the neighbour's real cell.
 #include "common.hpp"
 
+// No FMA contraction: ghost states round exactly as the reference's (they
+// feed the chained slope ratios of sch_roe).
+#pragma clang fp contract(off)
+
 using namespace codens;
 
 constexpr int BT = 256;

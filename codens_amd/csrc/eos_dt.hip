@@ -3,6 +3,11 @@
 // wavefront reads one contiguous i-row of every SoA array.
 #include "common.hpp"
 
+// No FMA contraction here: p, T, c and the ghost states feed the chained
+// slope ratios of sch_roe, which amplify any rounding difference, so these
+// kernels round exactly as the reference does.
+#pragma clang fp contract(off)
+
 using namespace codens;
 
 #define CELL_IJK(b)                                   \

@@ -184,6 +184,207 @@ __global__ void __launch_bounds__(TX *TY) k_impl_sweep(Blk b, Phys ph, int first
   for (int e = 0; e < 5; e++) dnew.at(e, n) = d2w[e] / dg;
 }
 
+// ---------------------------------------------------------------------------
+// Tiled sub-iteration.  A workgroup owns an SW_TX x SW_TY column tile of real
+// cells over SW_KS planes and marches k: the 14-vector q of a cell (mass-flux
+// increments d2..d4, the nine dF, d1, d5) is formed once per sweep and kept
+// in LDS for the i/j neighbours of its plane and in registers for the k
+// neighbours; the k face term is carried from one plane to the next.
+// LDS: 14 x (SW_TY+2) x (SW_TX+2) doubles = 73,920 bytes.
+constexpr int SW_TX = 64, SW_TY = 8, SW_KS = 8;
+constexpr int SW_HALO = 2 * SW_TX + 2 * SW_TY;
+
+struct Q14 {
+  double q[14];
+};
+
+__device__ inline void q_of(const Phys &ph, const double *__restrict__ v, long ldv, const DView &d, long n, Q14 &Q) {
+  double V[5], D[5];
+#pragma unroll
+  for (int e = 0; e < 5; e++) {
+    V[e] = v[e * ldv + n];
+    D[e] = d.at(e, n);
+  }
+  DFlux F = dflux(ph, V, D);
+  Q.q[0] = F.mx; Q.q[1] = F.my; Q.q[2] = F.mz;
+  Q.q[3] = F.xx; Q.q[4] = F.xy; Q.q[5] = F.xz; Q.q[6] = F.yy; Q.q[7] = F.yz; Q.q[8] = F.zz;
+  Q.q[9] = F.ex; Q.q[10] = F.ey; Q.q[11] = F.ez;
+  Q.q[12] = D[0]; Q.q[13] = D[4];
+}
+
+__device__ inline void q_zero(Q14 &Q) {
+#pragma unroll
+  for (int c = 0; c < 14; c++) Q.q[c] = 0.;
+}
+
+// d of component e from the 14-vector
+__device__ inline double q_d(const double *q, int e) {
+  return e == 0 ? q[12] : (e == 4 ? q[13] : q[e - 1]);
+}
+
+// tn of a face between cells A and B (implimf_3d.f90:276-292)
+__device__ inline void q_tn(const double *A, const double *B, double s1, double s2, double s3, double tn[5]) {
+  tn[0] = 0.5 * (A[0] + B[0]) * s1 + 0.5 * (A[1] + B[1]) * s2 + 0.5 * (A[2] + B[2]) * s3;
+  tn[1] = 0.5 * (A[3] + B[3]) * s1 + 0.5 * (A[4] + B[4]) * s2 + 0.5 * (A[5] + B[5]) * s3;
+  tn[2] = 0.5 * (A[4] + B[4]) * s1 + 0.5 * (A[6] + B[6]) * s2 + 0.5 * (A[7] + B[7]) * s3;
+  tn[3] = 0.5 * (A[5] + B[5]) * s1 + 0.5 * (A[7] + B[7]) * s2 + 0.5 * (A[8] + B[8]) * s3;
+  tn[4] = 0.5 * (A[9] + B[9]) * s1 + 0.5 * (A[10] + B[10]) * s2 + 0.5 * (A[11] + B[11]) * s3;
+}
+
+// First sub-iteration: d and dF are zero, so d = -(u (+ff)) / diag.
+__global__ void __launch_bounds__(TX *TY) k_impl_sweep1(Blk b, int ityprk, const double *__restrict__ u,
+                                                        const double *__restrict__ ff, long ldv,
+                                                        const double *__restrict__ diag, DView dnew) {
+  CELL3(b)
+  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+  double dg = diag[m];
+#pragma unroll
+  for (int e = 0; e < 5; e++) {
+    double r = ityprk == 0 ? -u[e * ldv + n] : -u[e * ldv + n] - ff[e * ldv + n];
+    dnew.at(e, n) = r / dg;
+  }
+}
+
+constexpr int SW_PL = (SW_TY + 2) * (SW_TX + 2);  // LDS plane stride between the 14 components
+
+// face term of the face between the LDS cells a (own side) and o (other side),
+// components read straight from LDS: tn = 0.5 (q_a + q_o) . S
+__device__ inline double lds_tn(const double *a, const double *o, int e, double s1, double s2, double s3) {
+  const int P = SW_PL;
+  switch (e) {
+    case 0: return 0.5 * (a[0] + o[0]) * s1 + 0.5 * (a[P] + o[P]) * s2 + 0.5 * (a[2 * P] + o[2 * P]) * s3;
+    case 1: return 0.5 * (a[3 * P] + o[3 * P]) * s1 + 0.5 * (a[4 * P] + o[4 * P]) * s2 + 0.5 * (a[5 * P] + o[5 * P]) * s3;
+    case 2: return 0.5 * (a[4 * P] + o[4 * P]) * s1 + 0.5 * (a[6 * P] + o[6 * P]) * s2 + 0.5 * (a[7 * P] + o[7 * P]) * s3;
+    case 3: return 0.5 * (a[5 * P] + o[5 * P]) * s1 + 0.5 * (a[7 * P] + o[7 * P]) * s2 + 0.5 * (a[8 * P] + o[8 * P]) * s3;
+    default: return 0.5 * (a[9 * P] + o[9 * P]) * s1 + 0.5 * (a[10 * P] + o[10 * P]) * s2 + 0.5 * (a[11 * P] + o[11 * P]) * s3;
+  }
+}
+__device__ inline double lds_d(const double *a, int e) {
+  const int P = SW_PL;
+  return e == 0 ? a[12 * P] : (e == 4 ? a[13 * P] : a[(e - 1) * P]);
+}
+
+__global__ void __launch_bounds__(SW_TX *SW_TY) k_impl_sweep_tiled(Blk b, Phys ph, int ityprk, int ntx, int nty,
+                                                                   int nkz, const double *__restrict__ u,
+                                                                   const double *__restrict__ ff,
+                                                                   const double *__restrict__ v, long ldv,
+                                                                   const double *__restrict__ sn,
+                                                                   const double *__restrict__ coefe,
+                                                                   const double *__restrict__ diag, DView dold,
+                                                                   DView dnew) {
+  __shared__ double sq[14 * SW_PL];
+  // XCD-aware tile order: the hardware deals workgroups to the 8 XCDs round
+  // robin; give each XCD a contiguous run of tiles so that j/k neighbours
+  // share its L2.
+  int nwg = ntx * nty * nkz;
+  int hw = blockIdx.x;
+  int lg = (nwg % 8 == 0) ? (hw % 8) * (nwg / 8) + hw / 8 : hw;
+  int bx = lg % ntx, by = (lg / ntx) % nty, bz = lg / (ntx * nty);
+  int tx = threadIdx.x, ty = threadIdx.y, t = ty * SW_TX + tx;
+  int i0 = b.i1 + bx * SW_TX, j0 = b.j1 + by * SW_TY;
+  int i = i0 + tx, j = j0 + ty;
+  int kb = b.k1 + bz * SW_KS, ke = min(kb + SW_KS, b.k2);
+  bool own = i < b.i2 && j < b.j2;
+  // halo cell of this thread (t < SW_HALO): rows j0-1, j0+SW_TY, columns i0-1, i0+SW_TX
+  int hi_ = 0, hj = 0, hslot = 0;
+  bool halo = t < SW_HALO;
+  if (halo) {
+    int hx, hy;
+    if (t < SW_TX) { hi_ = i0 + t; hj = j0 - 1; hx = t + 1; hy = 0; }
+    else if (t < 2 * SW_TX) { hi_ = i0 + t - SW_TX; hj = j0 + SW_TY; hx = t - SW_TX + 1; hy = SW_TY + 1; }
+    else if (t < 2 * SW_TX + SW_TY) { hi_ = i0 - 1; hj = j0 + t - 2 * SW_TX; hx = 0; hy = t - 2 * SW_TX + 1; }
+    else { hi_ = i0 + SW_TX; hj = j0 + t - 2 * SW_TX - SW_TY; hx = SW_TX + 1; hy = t - 2 * SW_TX - SW_TY + 1; }
+    hslot = hy * (SW_TX + 2) + hx;
+  }
+  bool hreal = halo && hi_ >= b.i1 && hi_ < b.i2 && hj >= b.j1 && hj < b.j2;
+  const long nnn = b.nnn, nijd = b.nijd;
+  const int slot = (ty + 1) * (SW_TX + 2) + (tx + 1);
+  Q14 Qc;
+  double tlo_k[5], dlo_k[5];
+  if (own) {
+    long n = b.idx(i, j, kb), m = n - b.npc;
+    Q14 Qb;
+    if (kb > b.k1) q_of(ph, v, ldv, dold, n - nijd, Qb); else q_zero(Qb);
+    q_of(ph, v, ldv, dold, n, Qc);
+    q_tn(Qc.q, Qb.q, snv(sn, nnn, m, 2, 0), snv(sn, nnn, m, 2, 1), snv(sn, nnn, m, 2, 2), tlo_k);
+#pragma unroll
+    for (int e = 0; e < 5; e++) dlo_k[e] = q_d(Qb.q, e);
+  } else {
+    q_zero(Qc);
+#pragma unroll
+    for (int e = 0; e < 5; e++) tlo_k[e] = dlo_k[e] = 0.;
+  }
+  for (int k = kb; k < ke; k++) {
+    long n = own ? b.idx(i, j, k) : 0, m = n - b.npc;
+    // current plane into LDS (own cell and this thread's halo cell)
+#pragma unroll
+    for (int c = 0; c < 14; c++) sq[c * SW_PL + slot] = Qc.q[c];
+    if (halo) {
+      Q14 Qh;
+      if (hreal) q_of(ph, v, ldv, dold, b.idx(hi_, hj, k), Qh); else q_zero(Qh);
+#pragma unroll
+      for (int c = 0; c < 14; c++) sq[c * SW_PL + hslot] = Qh.q[c];
+    }
+    // k face above: q of the next plane and its face term
+    double thi_k[5], dhi_k[5];
+    bool kreal = k + 1 < b.k2;
+    if (own) {
+      Q14 Qn;
+      if (kreal) q_of(ph, v, ldv, dold, n + nijd, Qn); else q_zero(Qn);
+      q_tn(Qn.q, Qc.q, snv(sn, nnn, m + nijd, 2, 0), snv(sn, nnn, m + nijd, 2, 1), snv(sn, nnn, m + nijd, 2, 2),
+           thi_k);
+#pragma unroll
+      for (int e = 0; e < 5; e++) dhi_k[e] = q_d(Qn.q, e);
+      Qc = Qn;  // plane k+1; the plane-k values are read back from LDS below
+    }
+    __syncthreads();
+    if (own) {
+      double d2w[5];
+#pragma unroll
+      for (int e = 0; e < 5; e++) d2w[e] = ityprk == 0 ? -u[e * ldv + n] : -u[e * ldv + n] - ff[e * ldv + n];
+      const double *A = sq + slot;
+      // i then j directions from LDS (implimf_3d.f90:268-362)
+#pragma unroll
+      for (int d = 0; d < 2; d++) {
+        const double *Lp = d == 0 ? A - 1 : A - (SW_TX + 2);
+        const double *Rp = d == 0 ? A + 1 : A + (SW_TX + 2);
+        long inc = d == 0 ? 1 : b.nid;
+        bool rreal = d == 0 ? (i + 1 < b.i2) : (j + 1 < b.j2);
+        double sl1 = snv(sn, nnn, m, d, 0), sl2 = snv(sn, nnn, m, d, 1), sl3 = snv(sn, nnn, m, d, 2);
+        double sh1 = snv(sn, nnn, m + inc, d, 0), sh2 = snv(sn, nnn, m + inc, d, 1), sh3 = snv(sn, nnn, m + inc, d, 2);
+        double cl = coefe[d * nnn + m];
+        double ch = rreal ? coefe[d * nnn + m + inc] : 0.;
+#pragma unroll
+        for (int e = 0; e < 5; e++) {
+          double tt = d2w[e] + lds_tn(A, Lp, e, sl1, sl2, sl3) + cl * lds_d(Lp, e);
+          if (rreal) tt = tt + ch * lds_d(Rp, e);
+          d2w[e] = tt - lds_tn(Rp, A, e, sh1, sh2, sh3);
+        }
+      }
+      // k direction from registers
+      {
+        double cl = coefe[2 * nnn + m];
+        double ch = kreal ? coefe[2 * nnn + m + nijd] : 0.;
+#pragma unroll
+        for (int e = 0; e < 5; e++) {
+          double tt = d2w[e] + tlo_k[e] + cl * dlo_k[e];
+          if (kreal) tt = tt + ch * dhi_k[e];
+          d2w[e] = tt - thi_k[e];
+        }
+      }
+      double dg = diag[m];
+#pragma unroll
+      for (int e = 0; e < 5; e++) dnew.at(e, n) = d2w[e] / dg;
+#pragma unroll
+      for (int e = 0; e < 5; e++) {
+        tlo_k[e] = thi_k[e];
+        dlo_k[e] = lds_d(A, e);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // implimf_3d.f90:473-486
 __global__ void __launch_bounds__(TX *TY) k_impl_update(Blk b, double *v, long ldv, const double *__restrict__ d) {
   CELL3(b)
@@ -229,14 +430,19 @@ extern "C" int codens_implimf_3d(const codens_block *cb, const codens_phys *cp, 
                      coefe, diag);
   }
   DView A = {d, (long)ldv, 0}, B = {W, b.nnn, b.npc};
+  int ntx = (b.i2 - b.i1 + SW_TX - 1) / SW_TX, nty = (b.j2 - b.j1 + SW_TY - 1) / SW_TY;
+  int nkz = (b.k2 - b.k1 + SW_KS - 1) / SW_KS;
   for (int s = 1; s <= lmx; s++) {
     DView out = ((lmx - s) % 2 == 0) ? A : B;
     DView in = ((lmx - s) % 2 == 0) ? B : A;
-    {
-    CodensTimed _tm("implimf_sweep", st);
-    hipLaunchKernelGGL(k_impl_sweep, g, t, 0, st, b, ph, s == 1 ? 1 : 0, (int)ityprk, u, ityprk ? ff : u, v,
-                       (long)ldv, sn, coefe, diag, in, out);
-  }
+    if (s == 1) {
+      CodensTimed _tm("implimf_sweep1", st);
+      hipLaunchKernelGGL(k_impl_sweep1, g, t, 0, st, b, (int)ityprk, u, ityprk ? ff : u, (long)ldv, diag, out);
+    } else {
+      CodensTimed _tm("implimf_sweep", st);
+      hipLaunchKernelGGL(k_impl_sweep_tiled, dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY), 0, st, b, ph, (int)ityprk,
+                         ntx, nty, nkz, u, ityprk ? ff : u, v, (long)ldv, sn, coefe, diag, in, out);
+    }
   }
   {
     CodensTimed _tm("implimf_update", st);

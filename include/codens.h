@@ -102,6 +102,28 @@ int codens_zvisqc(const codens_block *b, const codens_phys *ph, const double *sn
                   const double *vol, const double *v, int64_t ldv, const double *temp,
                   const double *mu, const double *mut, double *tq, void *stream);
 
+/* dissip_jameson.f90:4 -- JST artificial dissipation d(1:5) of the central
+ * schemes: pressure sensor per direction, eps2 = ki2*max(nu), eps4 =
+ * max(0, ki4-eps2), spectral radius |V.S|+c|S|, first-order boundary faces.
+ * ki2, ki4 are the block's coefficients (ki2(lm), ki4(lm)). */
+int codens_dissip_jameson(const codens_block *b, double ki2, double ki4, const double *v, int64_t ldv,
+                          double *d, const double *sn_blk, const double *ps, const double *cson,
+                          void *stream);
+
+/* met_roe2o.f90:4 (klroe=2) -- second-order upwind dissipation of the two
+ * transported turbulence variables t(:,6:7) into d(:,6:7): per face
+ * 0.5|V.S| (delta - minmod of three face differences), differences mirrored
+ * one face beyond the block.  t, d: 7 arrays of ldv. */
+int codens_met_roe2o(const codens_block *b, const double *t, int64_t ldv, double *d, const double *sn_blk,
+                     void *stream);
+
+/* met_roe.f90:4 (klroe=1) -- first-order upwind dissipation |V.n| dt of
+ * t(:,6:7), with the reference's index function that lands one storage entry
+ * before the intended cell (met_roe.f90:184-188); the entry just before the
+ * block, which the reference also zeroes, is left alone. */
+int codens_met_roe(const codens_block *b, const double *t, int64_t ldv, double *d, const double *sn_blk,
+                   void *stream);
+
 /* sch_expli.f90:4 -- v += dt/vol * u on the real cells. */
 int codens_sch_expli(const codens_block *b, const double *u, double *v, int64_t ldv,
                      const double *dt, const double *vol, void *stream);

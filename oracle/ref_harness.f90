@@ -238,4 +238,22 @@ contains
     call dissip_jameson(l, v, d, heq, sn, lgsnlt, snu, ps, cson)
   end subroutine ref_dissip_jameson
 
+  ! met_roe2o.f90:4
+  subroutine ref_met_roe2o(l, t, d, sn, lgsnlt, vol, del6, del7) bind(C, name="ref_met_roe2o")
+    use mod_met_roe2o
+    use chainecarac
+    integer(c_int), value :: l, lgsnlt
+    real(c_double) :: t(*), d(*), sn(*), vol(*), del6(*), del7(*)
+    equat = heq
+    call met_roe2o(l, t, d, sn, lgsnlt, vol, del6, del7)
+  end subroutine ref_met_roe2o
+
+  ! met_roe.f90:4
+  subroutine ref_met_roe(l, t, d, sn, lgsnlt, vol) bind(C, name="ref_met_roe")
+    use mod_met_roe
+    integer(c_int), value :: l, lgsnlt
+    real(c_double) :: t(*), d(*), sn(*), vol(*)
+    call met_roe(l, t, d, heq, sn, lgsnlt, vol)
+  end subroutine ref_met_roe
+
 end module codens_ref_harness

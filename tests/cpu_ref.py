@@ -190,3 +190,53 @@ class Orc:
 
     def bc_tq_coincident(self, g, i, r, tq):
         self.lib.orc_bc_tq_coincident(L_(len(g)), ptr(g), ptr(i), ptr(r), ptr(tq), self.ld)
+
+
+def _ref_dissip(self, l, v, d, sn, ps, cson, ki2, ki4):
+    b = self.lay.blocks[l]
+    snb = np.ascontiguousarray(sn_block(sn, b))
+    snu = np.zeros(self.lay.ip00)
+    self.lib.ref_dissip_jameson(l + 1, ptr(v), ptr(d), ptr(snb), b.nnn, ptr(snu), ptr(ps), ptr(cson), D(ki2), D(ki4))
+
+
+def _orc_dissip(self, l, v, d, sn, ps, cson, ki2, ki4):
+    b = self.lay.blocks[l]
+    snb = np.ascontiguousarray(sn_block(sn, b))
+    self.lib.orc_dissip_jameson(self.B(l), D(ki2), D(ki4), ptr(v), self.ld, ptr(d), ptr(snb), ptr(ps), ptr(cson))
+
+
+Ref.dissip_jameson = _ref_dissip
+Orc.dissip_jameson = _orc_dissip
+
+
+def _ref_roe2o(self, l, t, d, sn):
+    b = self.lay.blocks[l]
+    snb = np.ascontiguousarray(sn_block(sn, b))
+    w1, w2 = np.zeros(self.lay.ip00), np.zeros(self.lay.ip00)
+    vol = np.zeros(self.lay.ip11)
+    self.lib.ref_met_roe2o(l + 1, ptr(t), ptr(d), ptr(snb), b.nnn, ptr(vol), ptr(w1), ptr(w2))
+
+
+def _ref_roe1(self, l, t, d, sn):
+    b = self.lay.blocks[l]
+    snb = np.ascontiguousarray(sn_block(sn, b))
+    vol = np.zeros(self.lay.ip11)
+    self.lib.ref_met_roe(l + 1, ptr(t), ptr(d), ptr(snb), b.nnn, ptr(vol))
+
+
+def _orc_roe2o(self, l, t, d, sn):
+    b = self.lay.blocks[l]
+    snb = np.ascontiguousarray(sn_block(sn, b))
+    self.lib.orc_met_roe2o(self.B(l), ptr(t), self.ld, ptr(d), ptr(snb))
+
+
+def _orc_roe1(self, l, t, d, sn):
+    b = self.lay.blocks[l]
+    snb = np.ascontiguousarray(sn_block(sn, b))
+    self.lib.orc_met_roe(self.B(l), ptr(t), self.ld, ptr(d), ptr(snb))
+
+
+Ref.met_roe2o = _ref_roe2o
+Ref.met_roe = _ref_roe1
+Orc.met_roe2o = _orc_roe2o
+Orc.met_roe = _orc_roe1

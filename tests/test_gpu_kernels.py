@@ -87,6 +87,44 @@ def test_roe_residual(ctx, ns, ilim, pinfl):
         assert relerr(gff_h[e], ff_o[e]) <= TOL, f"ff[{e}]"
 
 
+@pytest.mark.parametrize("ki2,ki4", [(0.5, 1.0 / 32), (1.0, 0.05)])
+def test_dissip_jameson(ctx, ki2, ki4):
+    abi, lay, sn, vol = ctx
+    rng = np.random.default_rng(17)
+    n = lay.ip11
+    v = random_state(n, rng)
+    p, _, c = all_thermo(v)
+    O = C.Orc(lay, C.orc_phys())
+    d_o = np.zeros((5, n)); gd = T(np.zeros((5, n)))
+    for l, b in enumerate(lay.blocks):
+        O.dissip_jameson(l, v, d_o, sn, p, c, ki2, ki4)
+        abi.dissip_jameson(abi.block(b), ki2, ki4, T(v), gd, T(sn[9 * b.npc: 9 * (b.npc + b.nnn)]), T(p), T(c))
+    h = H(gd)
+    for e in range(5):
+        assert relerr(h[e], d_o[e]) <= TOL, e
+
+
+@pytest.mark.parametrize("which", ["met_roe2o", "met_roe"])
+def test_turbulence_upwind(ctx, which):
+    abi, lay, sn, vol = ctx
+    rng = np.random.default_rng(19)
+    n = lay.ip11
+    t = np.zeros((7, n))
+    t[:5] = random_state(n, rng)
+    t[5] = 1e-3 * (1 + rng.random(n)); t[6] = 10 * (1 + rng.random(n))
+    O = C.Orc(lay, C.orc_phys())
+    d_o = np.zeros((7, n)); gd = T(np.zeros((7, n))); gt = T(t)
+    for l, b in enumerate(lay.blocks):
+        getattr(O, which)(l, t, d_o, sn)
+        getattr(abi, which)(abi.block(b), gt, gd, T(sn[9 * b.npc: 9 * (b.npc + b.nnn)]))
+    h = H(gd)
+    for e in (5, 6):
+        a, r = h[e], d_o[e]
+        assert np.array_equal(np.isnan(a), np.isnan(r)), e
+        ok = ~np.isnan(r)
+        assert relerr(a[ok], r[ok]) <= TOL, e
+
+
 def test_zvisqc(ctx):
     abi, lay, sn, vol = ctx
     rng = np.random.default_rng(5)

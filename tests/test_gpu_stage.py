@@ -43,9 +43,13 @@ def test_periodic_euler_50_steps():
 
 
 def test_wall_ns_reduced_10_stages():
+    """Unlimited MUSCL (ilim=0): with the reference's chained minmod ratio
+    (ilim=1) a 1e-15 change of the state grows to 5e-3 within 10 stages in the
+    reference build itself, so multi-stage agreement is only meaningful
+    without it; ilim=1 is held to the single-stage bar below."""
     from codens_amd import configs
     import cpu_stage
-    case = configs.wall_ns(32, 24, 16, lmax=10)
+    case = configs.wall_ns(32, 24, 16, lmax=10, ilim=0)
     g, c = _solver(case), cpu_stage.CpuStage(case)
     for _ in range(10):
         g.stage(); c.stage()

@@ -28,4 +28,12 @@ typedef struct {
   double xk;
 } orc_sch;
 
+/* Turbulence model constants, module `modeleturb` (modules.f90:168-190),
+ * set per model by atlecdon.f90:112-400. */
+typedef struct {
+  double cb1, cb2, sigma, kappa, cw1, cw2, cw3, cv1, ct3, ct4; /* SA */
+  double betas, beta, sigmk, sigmw;                            /* Wilcox k-omega */
+  double alfak, alfae;                                         /* k-eps / k-l */
+} orc_turb;
+
 #endif

@@ -123,3 +123,296 @@ void orc_met_roe(const orc_block *b, const double *t, long ldv, double *d, const
         }
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* teq_grads.f90:74-420 (weighted-scheme branch with unit weights, the one
+ * every non-central scheme takes): Green-Gauss gradient of a block-local
+ * scalar s(m); g = dsdx, dsdy, dsdz (3 x nnn, block-local). */
+void orc_teq_grads(const orc_block *b, const double *sn, const double *vol, const double *s, double *g) {
+  long n0 = b->npc, nnn = b->nnn;
+  int i1 = b->ii1, i2 = b->ii2, j1 = b->jj1, j2 = b->jj2, k1 = b->kk1, k2 = b->kk2;
+  for (long m = 0; m < 3 * nnn; m++) g[m] = 0.;
+  double *G[3] = {g, g + nnn, g + 2 * nnn};
+  const double eps = 0.000001;
+  for (int d = 2; d >= 0; d--) {
+    long inc = INC(b, d);
+    int lo[3] = {i1, j1, k1}, hi[3] = {i2 - 1, j2 - 1, k2 - 1};
+    int a1 = d == 0 ? i1 : (d == 1 ? j1 : k1), a2 = d == 0 ? i2 : (d == 1 ? j2 : k2);
+    lo[d] = a1 + 1; hi[d] = a2 - 1;
+    int jk = (d == 1);  /* the j sweep runs j outer, k inner */
+    int olo = jk ? lo[1] : lo[2], ohi = jk ? hi[1] : hi[2], qlo = jk ? lo[2] : lo[1], qhi = jk ? hi[2] : hi[1];
+    for (int o = olo; o <= ohi; o++)
+      for (int q = qlo; q <= qhi; q++) {
+        int j = jk ? o : q, k = jk ? q : o;
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          long m = IDX(b, i, j, k) - n0;
+          for (int c = 0; c < 3; c++) G[c][m] = G[c][m] - (s[m] + s[m - inc]) * SN(sn, b, m, d, c);
+        }
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          long m = IDX(b, i, j, k) - n0;
+          for (int c = 0; c < 3; c++) G[c][m - inc] = G[c][m - inc] + (s[m] + s[m - inc]) * SN(sn, b, m, d, c);
+        }
+      }
+    for (int side = 0; side < 2; side++) {
+      int l2[3] = {i1, j1, k1}, h2[3] = {i2 - 1, j2 - 1, k2 - 1};
+      l2[d] = h2[d] = side == 0 ? a1 : a2;
+      for (int k = l2[2]; k <= h2[2]; k++)
+        for (int j = l2[1]; j <= h2[1]; j++)
+          for (int i = l2[0]; i <= h2[0]; i++) {
+            long m = IDX(b, i, j, k) - n0;
+            for (int c = 0; c < 3; c++) {
+              if (side == 0) G[c][m] = G[c][m] - SN(sn, b, m, d, c) * 2 * s[m - inc];
+              else G[c][m - inc] = G[c][m - inc] + SN(sn, b, m, d, c) * 2 * s[m];
+            }
+          }
+    }
+  }
+  for (long n = IDX(b, i1, j1, k1); n <= IDX(b, i2 - 1, j2 - 1, k2 - 1); n++) {
+    long m = n - n0;
+    double ts = copysign(0.5, -vol[n]);
+    double vols = (0.5 + ts) * eps + (0.5 - ts) * vol[n];
+    double c0 = 0.5 / (vols + 2.2250738585072014e-308);
+    for (int c = 0; c < 3; c++) G[c][m] = G[c][m] * c0;
+  }
+}
+
+/* scalar of block-local storage over the box i1-1..i2, j1-1..j2, k1-1..k2 */
+static void fill_box(const orc_block *b, double *s, const double *num, const double *den) {
+  for (int k = b->kk1 - 1; k <= b->kk2; k++)
+    for (int j = b->jj1 - 1; j <= b->jj2; j++)
+      for (int i = b->ii1 - 1; i <= b->ii2; i++) {
+        long n = IDX(b, i, j, k);
+        s[n - b->npc] = den ? num[n] / den[n] : num[n];
+      }
+}
+
+/* teq_gradv.f90:79-162: clamps rho >= 1e-20 on the box (a side effect on v
+ * that the reference has), then grad u, grad v, grad w into g9 (9 x nnn,
+ * block-local, order dvxx dvxy dvxz dvyx ... dvzz). */
+void orc_teq_gradv(const orc_block *b, const double *sn, const double *vol, double *v, long ldv, double *g9) {
+  long nnn = b->nnn;
+  for (int k = b->kk1 - 1; k <= b->kk2; k++)
+    for (int j = b->jj1 - 1; j <= b->jj2; j++)
+      for (int i = b->ii1 - 1; i <= b->ii2; i++) {
+        long n = IDX(b, i, j, k);
+        v[n] = fmax(v[n], 1.e-20);
+      }
+  double *s = calloc(nnn, 8);
+  for (int a = 0; a < 3; a++) {
+    fill_box(b, s, v + (a + 1) * ldv, v);
+    orc_teq_grads(b, sn, vol, s, g9 + 3 * a * nnn);
+  }
+  free(s);
+}
+
+/* met_prod.f90:54-72: production tprod(m) = tau^R : grad u with the
+ * Boussinesq stress of mut and the -2/3 rho k term (v6 = rho k). */
+void orc_met_prod(const orc_block *b, const double *mut, const double *v, long ldv, const double *g9, double *tprod) {
+  long n0 = b->npc, nnn = b->nnn;
+  const double ds3 = 2. / 3.;
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - n0;
+        double xx = g9[m], xy = g9[nnn + m], xz = g9[2 * nnn + m], yx = g9[3 * nnn + m], yy = g9[4 * nnn + m];
+        double yz = g9[5 * nnn + m], zx = g9[6 * nnn + m], zy = g9[7 * nnn + m], zz = g9[8 * nnn + m];
+        double s6 = v[5 * ldv + n];
+        double torxx = ds3 * (-s6 - mut[n] * (-2 * xx + yy + zz));
+        double toryy = ds3 * (-s6 - mut[n] * (-2 * yy + xx + zz));
+        double torzz = ds3 * (-s6 - mut[n] * (-2 * zz + yy + xx));
+        double torxy = mut[n] * (xy + yx);
+        double torxz = mut[n] * (xz + zx);
+        double toryz = mut[n] * (zy + yz);
+        tprod[m] = xx * torxx + yy * toryy + zz * torzz + torxy * (xy + yx) + torxz * (xz + zx) + toryz * (zy + yz);
+      }
+}
+
+/* met_gradtr.f90:107-171: fd (6 arrays of ldv) = grad(v6/rho), grad(v7/rho)
+ * over the contiguous storage range first..last real cell. */
+void orc_met_gradtr(const orc_block *b, const double *sn, const double *vol, const double *v, long ldv, double *fd) {
+  long n0 = b->npc, nnn = b->nnn;
+  double *s = calloc(nnn, 8), *g = calloc(3 * nnn, 8);
+  for (int a = 0; a < 2; a++) {
+    fill_box(b, s, v + (5 + a) * ldv, v);
+    orc_teq_grads(b, sn, vol, s, g);
+    for (long n = IDX(b, b->ii1, b->jj1, b->kk1); n <= IDX(b, b->ii2 - 1, b->jj2 - 1, b->kk2 - 1); n++)
+      for (int c = 0; c < 3; c++) fd[(3 * a + c) * ldv + n] = g[c * nnn + n - n0];
+  }
+  free(s); free(g);
+}
+
+/* met_difsa.f90 + met_smsa.f90:91-222: Spalart-Allmaras source ts6 = qcxts5
+ * (production, destruction and the cb2/sigma grad(rho nu~).grad(nu~) term),
+ * ts7 = 0 and the implicit coefficient cfke.  fd5: grad(nu~) (3 arrays of
+ * ldv from met_gradtr); g9: velocity gradients (block-local). */
+void orc_met_smsa(const orc_block *b, const orc_turb *tc, const double *sn, const double *vol, const double *v,
+                  long ldv, const double *mu, const double *dist, const double *g9, const double *fd5,
+                  double *cfke, double *ts6, double *ts7) {
+  long n0 = b->npc, nnn = b->nnn;
+  double *s = calloc(nnn, 8), *gd = calloc(3 * nnn, 8), *gr = calloc(3 * nnn, 8), *sdif = calloc(nnn, 8);
+  /* met_difsa: gradient of rho nu~, dotted with grad(nu~), times cb2/sigma */
+  fill_box(b, s, v + 5 * ldv, NULL);
+  orc_teq_grads(b, sn, vol, s, gd);
+  for (long n = IDX(b, b->ii1, b->jj1, b->kk1); n <= IDX(b, b->ii2 - 1, b->jj2 - 1, b->kk2 - 1); n++) {
+    long m = n - n0;
+    sdif[m] = gd[m] * fd5[n] + gd[nnn + m] * fd5[ldv + n] + gd[2 * nnn + m] * fd5[2 * ldv + n];
+    sdif[m] = sdif[m] * tc->cb2 / tc->sigma;
+  }
+  /* gradient of rho (met_smsa.f90:140-159) */
+  fill_box(b, s, v, NULL);
+  orc_teq_grads(b, sn, vol, s, gr);
+  double kappa2 = tc->kappa * tc->kappa, uns6 = 1. / 6.;
+  double cv13 = tc->cv1 * tc->cv1 * tc->cv1, cw36 = pow(tc->cw3, 6), cv133 = 3. * cv13, ct42 = 2. * tc->ct4;
+  double cb2sig = tc->cb2 / tc->sigma;
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - n0;
+        double xy = g9[nnn + m], xz = g9[2 * nnn + m], yx = g9[3 * nnn + m];
+        double yz = g9[5 * nnn + m], zx = g9[6 * nnn + m], zy = g9[7 * nnn + m];
+        double a = zy - yz, bb = xz - zx, c = yx - xy;
+        double vort = sqrt(a * a + bb * bb + c * c);
+        double v6 = v[5 * ldv + n], v1 = v[n];
+        double xkhi = v6 / mu[n];
+        double xkhi2 = xkhi * xkhi, xkhi3 = xkhi2 * xkhi;
+        double dist2 = dist[n] * dist[n];
+        double nutilde = v6 / v1;
+        double kapd2 = kappa2 * dist2;
+        double fv1 = xkhi3 / (xkhi3 + cv13);
+        double fv2 = 1. - (xkhi / (1 + xkhi * fv1));
+        double stilde = vort + nutilde * fv2 / kapd2;
+        double rtilde = nutilde / (stilde * kapd2);
+        rtilde = fmin(rtilde, 1.);
+        double rtil6 = pow(rtilde, 6);
+        double ft2 = tc->ct3 * exp(-tc->ct4 * xkhi2);
+        double sm1 = tc->cb1 * (1. - ft2) * stilde * v6;
+        double xg = rtilde + tc->cw2 * (rtil6 - rtilde);
+        double xg6 = pow(xg, 6);
+        double fw = xg * pow((1. + cw36) / (xg6 + cw36), uns6);
+        double sm2 = -(tc->cw1 * fw - tc->cb1 * ft2 / kappa2) * v6 * nutilde / dist2;
+        ts6[n] = sm1 + sm2 + sdif[m];
+        ts7[n] = 0.;
+        double xkhi4 = xkhi3 * xkhi;
+        double dfv1 = cv133 * (fv1 * fv1) / (mu[n] * xkhi4);
+        double dfv2 = (xkhi2 * dfv1 - 1. / mu[n]) / ((1. + xkhi * fv1) * (1. + xkhi * fv1));
+        double dst = (fv2 / v1 + nutilde * dfv2) / kapd2;
+        double dpr = (1. / v1 - nutilde * dst / stilde) / (stilde * kapd2);
+        double dxg = dpr * (1. + tc->cw2 * (6. * rtil6 / rtilde - 1.));
+        double dfw = (1. - xg6 / (xg6 + cw36)) * fw * dxg / xg;
+        double dft2 = -ct42 * xkhi * ft2;
+        double dsm1 = tc->cb1 * ((1. - ft2) * (stilde + dst * v6) - stilde * v6 * dft2);
+        double dsm2 = 2. * sm2 / v6 - (tc->cw1 * dfw - tc->cb1 * dft2 / kappa2) * nutilde * v6 / dist2;
+        /* the reference indexes the grad(nu~) array by the block-local m here
+         * (met_smsa.f90:216-217), which is the entry n only for the first block */
+        long mf = m;
+        double dsdif = -cb2sig * (fd5[mf] * gr[m] + fd5[ldv + mf] * gr[nnn + m] + fd5[2 * ldv + mf] * gr[2 * nnn + m]) /
+                       (v1 * v1);
+        cfke[n] = fabs(dsm1 + dsm2 + dsdif);
+      }
+  free(s); free(gd); free(gr); free(sdif);
+}
+
+/* met_smko.f90:80-110: compressible Wilcox k-omega sources. */
+void orc_met_smko(const orc_block *b, const orc_turb *tc, const double *v, long ldv, const double *tprod,
+                  double *cfke, double *ts6, double *ts7, const double *cson) {
+  long n0 = b->npc;
+  const double xw = 5. / 9., xmt0 = 0.1;
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - n0;
+        double v1 = v[n], v6 = v[5 * ldv + n], v7 = v[6 * ldv + n];
+        double omeg = v7 / v1;
+        double cd = 0.;
+        double xmt = sqrt(2. * v6 / v1) / cson[n];
+        double f = 0.5 * (1. + copysign(1.0, xmt - xmt0)) * (xmt * xmt - xmt0 * xmt0);
+        double betasc = tc->betas * (1. + 1.5 * f);
+        double betac = tc->beta - tc->betas * 1.5 * f;
+        ts6[n] = tprod[m] - betasc * v6 * omeg;
+        ts7[n] = xw * tprod[m] * v7 / v6 - betac * v7 * omeg + cd;
+        cfke[n] = 0.18 * omeg;
+      }
+}
+
+/* met_fludcsa.f90:96-107 (kfludis=2), met_fludko.f90 (kfludis=5) and
+ * met_fludc.f90:100-114 (kfludis=1): the gradients fd (6 arrays of ldv) become
+ * diffusive flux densities, over the contiguous range first..last real cell. */
+void orc_met_flud(const orc_block *b, int kfludis, const orc_turb *tc, const double *v, long ldv, const double *mu,
+                  const double *mut, double *fd) {
+  long n0 = IDX(b, b->ii1, b->jj1, b->kk1), n1 = IDX(b, b->ii2 - 1, b->jj2 - 1, b->kk2 - 1);
+  double sigma1 = 1. / tc->sigma;
+  for (long n = n0; n <= n1; n++) {
+    if (kfludis == 2) {
+      double smutot = sigma1 * (mu[n] + v[5 * ldv + n]);
+      for (int c = 0; c < 3; c++) fd[c * ldv + n] = smutot * fd[c * ldv + n];
+      for (int c = 3; c < 6; c++) fd[c * ldv + n] = 0.;
+    } else if (kfludis == 5) {
+      for (int c = 0; c < 3; c++) fd[c * ldv + n] = (mu[n] + mut[n] * tc->sigmk) * fd[c * ldv + n];
+      for (int c = 3; c < 6; c++) fd[c * ldv + n] = (mu[n] + mut[n] * tc->sigmw) * fd[c * ldv + n];
+    } else {
+      for (int c = 0; c < 3; c++) fd[c * ldv + n] = (mu[n] + mut[n] / tc->alfak) * fd[c * ldv + n];
+    }
+  }
+  if (kfludis == 1)
+    for (long n = n0; n <= n1; n++)
+      for (int c = 3; c < 6; c++) fd[c * ldv + n] = (mu[n] + mut[n] / tc->alfae) * fd[c * ldv + n];
+}
+
+/* sch_turb.f90:92-301 (without met_pardis, applied by the caller): centred
+ * convective minus diffusive face fluxes of v6, v7, doubled ghost values on
+ * the boundary faces; u6 = 0.5 sum - vol ts6 - d6 (same for 7). */
+void orc_sch_turb(const orc_block *b, double *u, const double *v, long ldv, const double *d, const double *fd,
+                  const double *ts6, const double *ts7, const double *sn, const double *vol) {
+  long n0 = b->npc, nnn = b->nnn;
+  int i1 = b->ii1, i2 = b->ii2, j1 = b->jj1, j2 = b->jj2, k1 = b->kk1, k2 = b->kk2;
+  double *f = calloc(6 * nnn, 8);
+  double *u6 = u + 5 * ldv, *u7 = u + 6 * ldv;
+  for (long n = IDX(b, i1 - 1, j1 - 1, k1 - 1); n <= IDX(b, i2, j2, k2); n++) { u6[n] = 0.; u7[n] = 0.; }
+  for (long n = IDX(b, i1 - 1, j1 - 1, k1 - 1); n <= IDX(b, i2, j2, k2); n++) {
+    long m = n - n0;
+    for (int c = 0; c < 3; c++) {
+      f[c * nnn + m] = v[5 * ldv + n] * (v[(c + 1) * ldv + n] / v[n]) - fd[c * ldv + n];
+      f[(3 + c) * nnn + m] = v[6 * ldv + n] * (v[(c + 1) * ldv + n] / v[n]) - fd[(3 + c) * ldv + n];
+    }
+  }
+  for (int d_ = 0; d_ < 3; d_++) {
+    long inc = INC(b, d_);
+    int a1 = d_ == 0 ? i1 : (d_ == 1 ? j1 : k1), a2 = d_ == 0 ? i2 : (d_ == 1 ? j2 : k2);
+    int lo[3] = {i1, j1, k1}, hi[3] = {i2 - 1, j2 - 1, k2 - 1};
+    lo[d_] = a1 + 1; hi[d_] = a2 - 1;
+    for (int k = lo[2]; k <= hi[2]; k++)
+      for (int j = lo[1]; j <= hi[1]; j++)
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          long n = IDX(b, i, j, k), m = n - n0;
+          double s1 = SN(sn, b, m, d_, 0), s2 = SN(sn, b, m, d_, 1), s3 = SN(sn, b, m, d_, 2);
+          double x6 = (f[m] + f[m - inc]) * s1 + (f[nnn + m] + f[nnn + m - inc]) * s2 + (f[2 * nnn + m] + f[2 * nnn + m - inc]) * s3;
+          double x7 = (f[3 * nnn + m] + f[3 * nnn + m - inc]) * s1 + (f[4 * nnn + m] + f[4 * nnn + m - inc]) * s2 +
+                      (f[5 * nnn + m] + f[5 * nnn + m - inc]) * s3;
+          u6[n] = u6[n] - x6; u7[n] = u7[n] - x7;
+          u6[n - inc] = u6[n - inc] + x6; u7[n - inc] = u7[n - inc] + x7;
+        }
+    for (int side = 0; side < 2; side++) {
+      int l2[3] = {i1, j1, k1}, h2[3] = {i2 - 1, j2 - 1, k2 - 1};
+      l2[d_] = h2[d_] = side == 0 ? a1 : a2;
+      for (int k = l2[2]; k <= h2[2]; k++)
+        for (int j = l2[1]; j <= h2[1]; j++)
+          for (int i = l2[0]; i <= h2[0]; i++) {
+            long n = IDX(b, i, j, k), m = n - n0;
+            long mg = side == 0 ? m - inc : m;
+            double s1 = SN(sn, b, m, d_, 0), s2 = SN(sn, b, m, d_, 1), s3 = SN(sn, b, m, d_, 2);
+            double x6 = 2 * f[mg] * s1 + 2 * f[nnn + mg] * s2 + 2 * f[2 * nnn + mg] * s3;
+            double x7 = 2 * f[3 * nnn + mg] * s1 + 2 * f[4 * nnn + mg] * s2 + 2 * f[5 * nnn + mg] * s3;
+            if (side == 0) { u6[n] = u6[n] - x6; u7[n] = u7[n] - x7; }
+            else { u6[n - inc] = u6[n - inc] + x6; u7[n - inc] = u7[n - inc] + x7; }
+          }
+    }
+  }
+  for (int k = k1; k <= k2 - 1; k++)
+    for (int j = j1; j <= j2 - 1; j++)
+      for (int i = i1; i <= i2 - 1; i++) {
+        long n = IDX(b, i, j, k);
+        u6[n] = 0.5 * u6[n] - vol[n] * ts6[n] - d[5 * ldv + n];
+        u7[n] = 0.5 * u7[n] - vol[n] * ts7[n] - d[6 * ldv + n];
+      }
+  free(f);
+}

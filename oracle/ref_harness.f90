@@ -256,4 +256,95 @@ contains
     call met_roe(l, t, d, heq, sn, lgsnlt, vol)
   end subroutine ref_met_roe
 
+  ! modeleturb constants (atlecdon.f90:112-400); tc(16): cb1 cb2 sigma kappa
+  ! cw1 cw2 cw3 cv1 ct3 ct4 betas beta sigmk sigmw alfak alfae
+  subroutine ref_set_turb(tc) bind(C, name="ref_set_turb")
+    real(c_double) :: tc(16)
+    cb1 = tc(1); cb2 = tc(2); sigma = tc(3); kappa = tc(4); cw1 = tc(5); cw2 = tc(6)
+    cw3 = tc(7); cv1 = tc(8); ct3 = tc(9); ct4 = tc(10); betas = tc(11); beta = tc(12)
+    sigmk = tc(13); sigmw = tc(14); alfak = tc(15); alfae = tc(16)
+    kparoi = 0
+    lparoi = 0
+    ischema = 7
+  end subroutine ref_set_turb
+
+  ! teq_gradv.f90:4 (grad u, v, w; clamps rho)
+  subroutine ref_teq_gradv(l, sn, vol, v, s, g, cmu) bind(C, name="ref_teq_gradv")
+    use mod_teq_gradv
+    use chainecarac
+    integer(c_int), value :: l
+    real(c_double) :: sn(*), vol(*), v(*), s(*), g(ip00, 9), cmu(*)
+    equat = heq
+    call teq_gradv(l, sn, vol, v, s, g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), g(1,8), g(1,9), &
+         cmu, cmu, cmu, cmu, cmu, cmu)
+  end subroutine ref_teq_gradv
+
+  ! met_prod.f90:4
+  subroutine ref_met_prod(l, mut, v, g, tprod) bind(C, name="ref_met_prod")
+    use mod_met_prod
+    integer(c_int), value :: l
+    real(c_double) :: mut(*), v(*), g(ip00, 9), tprod(*)
+    call met_prod(l, 1, mut, v, g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), g(1,8), g(1,9), tprod)
+  end subroutine ref_met_prod
+
+  ! met_gradtr.f90:4
+  subroutine ref_met_gradtr(l, sn, vol, v, mu, mut, tt, gx, gy, gz, fd, cmu) bind(C, name="ref_met_gradtr")
+    use mod_met_gradtr
+    use chainecarac
+    integer(c_int), value :: l
+    real(c_double) :: sn(*), vol(*), v(*), mu(*), mut(*), tt(*), gx(*), gy(*), gz(*), fd(ip11, 6), cmu(*)
+    equat = heq
+    call met_gradtr(l, sn, vol, v, mu, mut, tt, gx, gy, gz, fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), &
+         fd(1,6), cmu, cmu, cmu, cmu, cmu, cmu)
+  end subroutine ref_met_gradtr
+
+  ! met_smsa.f90:4 (calls met_difsa.f90:4)
+  subroutine ref_met_smsa(l, sn, vol, v, mu, dist, g, fd, cfke, tt, gx, gy, gz, sdif, ts6, ts7, cmu) &
+       bind(C, name="ref_met_smsa")
+    use mod_met_smsa
+    use chainecarac
+    integer(c_int), value :: l
+    real(c_double) :: sn(*), vol(*), v(*), mu(*), dist(*), g(ip00, 9), fd(ip11, 6), cfke(*), tt(*), gx(*), gy(*)
+    real(c_double) :: gz(*), sdif(*), ts6(*), ts7(*), cmu(*)
+    equat = heq
+    call met_smsa(l, sn, vol, v, mu, dist, g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), g(1,8), &
+         g(1,9), fd(1,1), fd(1,2), fd(1,3), cfke, tt, gx, gy, gz, sdif, ts6, ts7, cmu, cmu, cmu, cmu, cmu, cmu)
+  end subroutine ref_met_smsa
+
+  ! met_smko.f90:4
+  subroutine ref_met_smko(l, v, fd, tprod, cfke, ts6, ts7, cson) bind(C, name="ref_met_smko")
+    use mod_met_smko
+    integer(c_int), value :: l
+    real(c_double) :: v(*), fd(ip11, 6), tprod(*), cfke(*), ts6(*), ts7(*), cson(*)
+    call met_smko(l, v, fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6), tprod, cfke, ts6, ts7, cson)
+  end subroutine ref_met_smko
+
+  ! met_fludcsa.f90:4, met_fludko.f90:4, met_fludc.f90:4 by kfludis
+  subroutine ref_met_flud(l, kfl, v, mu, mut, fd, tt, gx, gy, gz) bind(C, name="ref_met_flud")
+    use mod_met_fludcsa
+    use mod_met_fludko
+    use mod_met_fludc
+    use chainecarac
+    integer(c_int), value :: l, kfl
+    real(c_double) :: v(*), mu(*), mut(*), fd(ip11, 6), tt(*), gx(*), gy(*), gz(*)
+    equat = heq
+    if (kfl .eq. 2) then
+       call met_fludcsa(l, v, mu, fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6))
+    elseif (kfl .eq. 5) then
+       call met_fludko(l, mu, mut, fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6))
+    else
+       call met_fludc(l, v, mu, mut, tt, gx, gy, gz, fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6))
+    endif
+  end subroutine ref_met_flud
+
+  ! sch_turb.f90:4
+  subroutine ref_sch_turb(l, u, v, d, fd, ts6, ts7, ncin, sn, lgsnlt, vol, f) bind(C, name="ref_sch_turb")
+    use mod_sch_turb
+    integer(c_int), value :: l, lgsnlt
+    real(c_double) :: u(*), v(*), d(*), fd(ip11, 6), ts6(*), ts7(*), sn(*), vol(*), f(ip00, 6)
+    integer(c_int) :: ncin(*)
+    call sch_turb(l, u, v, d, fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6), ts6, ts7, heq, ncin, &
+         sn, lgsnlt, vol, f(1,1), f(1,2), f(1,3), f(1,4), f(1,5), f(1,6))
+  end subroutine ref_sch_turb
+
 end module codens_ref_harness

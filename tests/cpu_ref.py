@@ -240,3 +240,32 @@ Ref.met_roe2o = _ref_roe2o
 Ref.met_roe = _ref_roe1
 Orc.met_roe2o = _orc_roe2o
 Orc.met_roe = _orc_roe1
+
+
+class OrcTurb(ctypes.Structure):
+    _fields_ = [(n, D) for n in ("cb1", "cb2", "sigma", "kappa", "cw1", "cw2", "cw3", "cv1", "ct3", "ct4",
+                                 "betas", "beta", "sigmk", "sigmw", "alfak", "alfae")]
+
+
+def turb_constants():
+    """Spalart-Allmaras (atlecdon.f90:183-204) and Wilcox k-omega
+    (atlecdon.f90:353-366) constants; alfak/alfae of a k-eps model."""
+    cb1, sigma, cb2, kappa = 0.1355, 2.0 / 3.0, 0.622, 0.41
+    cw1 = cb1 / kappa ** 2 + (1 + cb2) / sigma
+    return OrcTurb(cb1, cb2, sigma, kappa, cw1, 0.3, 2.0, 7.1, 1.1, 2.0, 0.09, 0.075, 0.5, 0.5, 1.0, 1.3)
+
+
+def _tc_arr(tc):
+    return np.array([getattr(tc, f) for f, _ in OrcTurb._fields_])
+
+
+def _ref_set_turb(self, tc):
+    a = _tc_arr(tc)
+    self.lib.ref_set_turb(ptr(a))
+
+
+def _ref_turb_call(self, name, l, *args):
+    return getattr(self.lib, name)(l + 1, *args)
+
+
+Ref.set_turb = _ref_set_turb

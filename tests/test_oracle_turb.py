@@ -1,0 +1,123 @@
+"""Turbulence-transport chain of met_num (Spalart-Allmaras and Wilcox k-omega
+sources, gradients, production, diffusive flux densities, flux balance):
+the oracle against the flang build of the reference routines, bit for bit."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import cpu_ref as C
+import oracle
+from oracle import ptr
+from randcase import random_mesh, random_state
+
+pytestmark = pytest.mark.skipif(oracle.ref_lib() is None, reason="reference build (oracle/_ref) not present")
+
+DIMS = [(8, 6, 5), (5, 4, 3)]
+
+
+@pytest.fixture(scope="module")
+def setup():
+    lay, sn, vol = random_mesh(DIMS, seed=41)
+    rng = np.random.default_rng(43)
+    n = lay.ip11
+    v = np.zeros((7, n))
+    v[:5] = random_state(n, rng)
+    v[5] = 1e-3 * (1 + rng.random(n)) * v[0]
+    v[6] = 50 * (1 + rng.random(n)) * v[0]
+    mu = 1e-3 * (1 + 0.1 * rng.random(n))
+    mut = 1e-3 * rng.random(n)
+    dist = 0.05 + rng.random(n)
+    cson = np.sqrt(1.4 * (0.4 * (v[4] - 0.5 * (v[1] ** 2 + v[2] ** 2 + v[3] ** 2) / v[0])) / v[0])
+    tc = C.turb_constants()
+    R = C.Ref(lay, C.orc_phys())
+    R.set_turb(tc)
+    return lay, sn, vol, v, mu, mut, dist, cson, tc, R, oracle.oracle_lib()
+
+
+# teq_gradv, met_gradtr and met_smsa take the global metric array and find the
+# block's slice themselves (npsn = ndir*npfb(l)+1); sch_turb takes the slice.
+def snb(sn, b):
+    return np.ascontiguousarray(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+
+
+def test_gradients_production_fluxdensity(setup):
+    lay, sn, vol, v, mu, mut, dist, cson, tc, R, O = setup
+    n = lay.ip11
+    cmu = np.ones(n)
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        # teq_gradv (with its rho clamp)
+        vr, vo = v.copy(), v.copy()
+        vr[0, b.npc + 5] = -1.0; vo[0, b.npc + 5] = -1.0
+        gr = np.zeros((9, lay.ip00)); s = np.zeros(lay.ip00)
+        R.lib.ref_teq_gradv(l + 1, ptr(sn), ptr(vol), ptr(vr), ptr(s), ptr(gr), ptr(cmu))
+        go = np.zeros((9, b.nnn))
+        O.orc_teq_gradv(ctypes.byref(ob), ptr(snb(sn, b)), ptr(vol), ptr(vo), C.L_(n), ptr(go))
+        m = b.interior() - b.npc
+        assert np.array_equal(vr, vo)
+        assert np.array_equal(gr[:, m], go[:, m])
+        # met_prod
+        tr, to = np.zeros(lay.ip00), np.zeros(b.nnn)
+        R.lib.ref_met_prod(l + 1, ptr(mut), ptr(v), ptr(np.ascontiguousarray(gr)), ptr(tr))
+        O.orc_met_prod(ctypes.byref(ob), ptr(mut), ptr(v), C.L_(n), ptr(go), ptr(to))
+        assert np.array_equal(tr[m], to[m])
+        # met_gradtr then the three flux-density forms
+        fr, fo = np.zeros((6, n)), np.zeros((6, n))
+        w = [np.zeros(lay.ip00) for _ in range(4)]
+        R.lib.ref_met_gradtr(l + 1, ptr(sn), ptr(vol), ptr(v), ptr(mu), ptr(mut), *[ptr(a) for a in w],
+                             ptr(fr), ptr(cmu))
+        O.orc_met_gradtr(ctypes.byref(ob), ptr(snb(sn, b)), ptr(vol), ptr(v), C.L_(n), ptr(fo))
+        assert np.array_equal(fr, fo)
+        for kfl in (1, 2, 5):
+            a, c = fr.copy(), fo.copy()
+            R.lib.ref_met_flud(l + 1, kfl, ptr(v), ptr(mu), ptr(mut), ptr(a), *[ptr(x) for x in w])
+            O.orc_met_flud(ctypes.byref(ob), kfl, ctypes.byref(tc), ptr(v), C.L_(n), ptr(mu), ptr(mut), ptr(c))
+            assert np.array_equal(a, c), kfl
+
+
+def test_sources_and_balance(setup):
+    lay, sn, vol, v, mu, mut, dist, cson, tc, R, O = setup
+    n = lay.ip11
+    cmu = np.ones(n)
+    b = lay.blocks[0]   # met_smsa reads its grad(nu~) array by the block-local index
+    ob = C.orc_block(b)
+    m = b.interior() - b.npc
+    gr = np.zeros((9, lay.ip00)); s = np.zeros(lay.ip00)
+    vv = v.copy()
+    R.lib.ref_teq_gradv(1, ptr(sn), ptr(vol), ptr(vv), ptr(s), ptr(gr), ptr(cmu))
+    fd = np.zeros((6, n)); w = [np.zeros(lay.ip00) for _ in range(4)]
+    R.lib.ref_met_gradtr(1, ptr(sn), ptr(vol), ptr(vv), ptr(mu), ptr(mut), *[ptr(a) for a in w], ptr(fd),
+                         ptr(cmu))
+    # Spalart-Allmaras
+    cr, co = np.zeros(n), np.zeros(n)
+    t6r, t7r, t6o, t7o = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    w5 = [np.zeros(lay.ip00) for _ in range(5)]
+    R.lib.ref_met_smsa(1, ptr(sn), ptr(vol), ptr(vv), ptr(mu), ptr(dist), ptr(gr), ptr(fd), ptr(cr),
+                       *[ptr(a) for a in w5], ptr(t6r), ptr(t7r), ptr(cmu))
+    go = np.ascontiguousarray(gr[:, :b.nnn])
+    O.orc_met_smsa(ctypes.byref(ob), ctypes.byref(tc), ptr(snb(sn, b)), ptr(vol), ptr(vv), C.L_(n), ptr(mu),
+                   ptr(dist), ptr(go), ptr(fd), ptr(co), ptr(t6o), ptr(t7o))
+    inner = b.interior()
+    for x, y, name in ((t6r, t6o, "ts6"), (t7r, t7o, "ts7"), (cr, co, "cfke")):
+        assert np.array_equal(x[inner], y[inner]), name
+    # k-omega
+    tprod = np.zeros(lay.ip00)
+    R.lib.ref_met_prod(1, ptr(mut), ptr(vv), ptr(gr), ptr(tprod))
+    cr, co = np.zeros(n), np.zeros(n)
+    t6r, t7r, t6o, t7o = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    R.lib.ref_met_smko(1, ptr(vv), ptr(fd), ptr(tprod), ptr(cr), ptr(t6r), ptr(t7r), ptr(cson))
+    O.orc_met_smko(ctypes.byref(ob), ctypes.byref(tc), ptr(vv), C.L_(n), ptr(np.ascontiguousarray(tprod[:b.nnn])),
+                   ptr(co), ptr(t6o), ptr(t7o), ptr(cson))
+    for x, y, name in ((t6r, t6o, "ts6"), (t7r, t7o, "ts7"), (cr, co, "cfke")):
+        assert np.array_equal(x[inner], y[inner]), name
+    # flux balance
+    rng = np.random.default_rng(5)
+    d = 1e-4 * rng.standard_normal((7, n))
+    ur, uo = np.zeros((7, n)), np.zeros((7, n))
+    f = np.zeros((6, lay.ip00)); ncin = np.zeros(1, dtype=np.int32)
+    R.lib.ref_sch_turb(1, ptr(ur), ptr(vv), ptr(d), ptr(fd), ptr(t6r), ptr(t7r), ptr(ncin), ptr(snb(sn, b)), b.nnn,
+                       ptr(vol), ptr(f))
+    O.orc_sch_turb(ctypes.byref(ob), ptr(uo), ptr(vv), C.L_(n), ptr(d), ptr(fd), ptr(t6r), ptr(t7r),
+                   ptr(snb(sn, b)), ptr(vol))
+    assert np.array_equal(ur[5:7], uo[5:7])

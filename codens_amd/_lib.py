@@ -49,6 +49,14 @@ SIGNATURES = {
     "codens_dissip_jameson": [P(CodensBlock), c_dbl, c_dbl, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp],
     "codens_met_roe2o": [P(CodensBlock), c_vp, c_i64, c_vp, c_vp, c_vp],
     "codens_met_roe": [P(CodensBlock), c_vp, c_i64, c_vp, c_vp, c_vp],
+    "codens_teq_gradv": [P(CodensBlock), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp],
+    "codens_met_prod": [P(CodensBlock), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
+    "codens_met_gradtr": [P(CodensBlock), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp],
+    "codens_met_smsa": [P(CodensBlock), c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                        c_vp],
+    "codens_met_smko": [P(CodensBlock), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "codens_met_flud": [P(CodensBlock), c_int, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
+    "codens_sch_turb": [P(CodensBlock), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "codens_sch_expli": [P(CodensBlock), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
     "codens_implimf_3d": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int,
                           c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp, c_vp],

@@ -110,6 +110,42 @@ def met_roe(b, t, d, sn_blk, stream=None):
     _chk(lib.codens_met_roe(ctypes.byref(b), _p(t), ld(t), _p(d), _p(sn_blk), _s(stream)))
 
 
+def _tc(tc):
+    """16 turbulence constants as a host double array (cb1 ... alfae)."""
+    return (ctypes.c_double * 16)(*[float(x) for x in tc])
+
+
+def teq_gradv(b, sn_blk, vol, v, g9, stream=None):
+    _chk(lib.codens_teq_gradv(ctypes.byref(b), _p(sn_blk), _p(vol), _p(v), ld(v), _p(g9), _s(stream)))
+
+
+def met_prod(b, mut, v, g9, tprod, stream=None):
+    _chk(lib.codens_met_prod(ctypes.byref(b), _p(mut), _p(v), ld(v), _p(g9), _p(tprod), _s(stream)))
+
+
+def met_gradtr(b, sn_blk, vol, v, fd, stream=None):
+    _chk(lib.codens_met_gradtr(ctypes.byref(b), _p(sn_blk), _p(vol), _p(v), ld(v), _p(fd), _s(stream)))
+
+
+def met_smsa(b, tc, sn_blk, vol, v, mu, dist, g9, fd, cfke, ts6, ts7, stream=None):
+    _chk(lib.codens_met_smsa(ctypes.byref(b), _tc(tc), _p(sn_blk), _p(vol), _p(v), ld(v), _p(mu), _p(dist), _p(g9),
+                             _p(fd), _p(cfke), _p(ts6), _p(ts7), _s(stream)))
+
+
+def met_smko(b, tc, v, tprod, cfke, ts6, ts7, cson, stream=None):
+    _chk(lib.codens_met_smko(ctypes.byref(b), _tc(tc), _p(v), ld(v), _p(tprod), _p(cfke), _p(ts6), _p(ts7), _p(cson),
+                             _s(stream)))
+
+
+def met_flud(b, kfludis, tc, v, mu, mut, fd, stream=None):
+    _chk(lib.codens_met_flud(ctypes.byref(b), kfludis, _tc(tc), _p(v), ld(v), _p(mu), _p(mut), _p(fd), _s(stream)))
+
+
+def sch_turb(b, u, v, d, fd, ts6, ts7, sn_blk, vol, stream=None):
+    _chk(lib.codens_sch_turb(ctypes.byref(b), _p(u), _p(v), ld(v), _p(d), _p(fd), _p(ts6), _p(ts7), _p(sn_blk),
+                             _p(vol), _s(stream)))
+
+
 def sch_expli(b, u, v, dt, vol, stream=None):
     _chk(lib.codens_sch_expli(ctypes.byref(b), _p(u), _p(v), ld(v), _p(dt), _p(vol), _s(stream)))
 

@@ -163,3 +163,381 @@ extern "C" int codens_met_roe(const codens_block *cb, const double *t, int64_t l
   }
   return codens_check_launch("codens_met_roe");
 }
+
+// ---------------------------------------------------------------------------
+// Green-Gauss gradient of a cell scalar s = num/den (den may be null) at a real
+// cell, gathered from its six faces (teq_grads.f90:74-420, unit weights):
+// interior faces (s_c + s_nbr).S, boundary faces 2 s_ghost.S, then 1/(2 vol)
+// with the guard for non-positive volumes.
+__device__ inline double sval(const double *__restrict__ num, const double *__restrict__ den, long n) {
+  return den ? num[n] / den[n] : num[n];
+}
+
+__device__ void grad_scalar(const Blk &b, const double *__restrict__ sn, const double *__restrict__ vol,
+                            const double *__restrict__ num, const double *__restrict__ den, long n, const int c[3],
+                            double g[3]) {
+  long m = n - b.npc;
+  double sc = sval(num, den, n);
+  g[0] = g[1] = g[2] = 0.;
+#pragma unroll
+  for (int dd = 0; dd < 3; dd++) {
+    int d = 2 - dd;
+    long inc = b.inc(d);
+    int a = c[d], a1 = b.lo(d), a2 = b.hi(d);
+    double sl = sval(num, den, n - inc), sh = sval(num, den, n + inc);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      double s = snv(sn, b.nnn, m, d, q);
+      g[q] -= (a == a1) ? s * 2 * sl : (sc + sl) * s;
+      double s2 = snv(sn, b.nnn, m + inc, d, q);
+      g[q] += (a + 1 == a2) ? s2 * 2 * sh : (sh + sc) * s2;
+    }
+  }
+  double ts = copysign(0.5, -vol[n]);
+  double vols = (0.5 + ts) * 0.000001 + (0.5 - ts) * vol[n];
+  double c0 = 0.5 / (vols + DBL_MIN);
+  g[0] *= c0; g[1] *= c0; g[2] *= c0;
+}
+
+#define REAL_CELL(b)                                 \
+  int c[3];                                          \
+  c[0] = (b).i1 + blockIdx.x * TX + threadIdx.x;     \
+  c[1] = (b).j1 + blockIdx.y * TY + threadIdx.y;     \
+  c[2] = (b).k1 + blockIdx.z;                        \
+  if (c[0] >= (b).i2 || c[1] >= (b).j2) return;      \
+  long n = (b).idx(c[0], c[1], c[2]), m = n - (b).npc;
+
+// teq_gradv.f90:79-86: rho >= 1e-20 on the box i1-1..i2, j1-1..j2, k1-1..k2
+__global__ void __launch_bounds__(TX *TY) k_rho_clamp(Blk b, double *v) {
+  int i = b.i1 - 1 + blockIdx.x * TX + threadIdx.x, j = b.j1 - 1 + blockIdx.y * TY + threadIdx.y;
+  int k = b.k1 - 1 + blockIdx.z;
+  if (i > b.i2 || j > b.j2) return;
+  long n = b.idx(i, j, k);
+  v[n] = fmax(v[n], 1.e-20);
+}
+
+// teq_gradv.f90:87-162: grad u, grad v, grad w (block-local, 9 x nnn)
+__global__ void __launch_bounds__(TX *TY) k_teq_gradv(Blk b, const double *__restrict__ sn,
+                                                      const double *__restrict__ vol, const double *__restrict__ v,
+                                                      long ldv, double *__restrict__ g9) {
+  REAL_CELL(b)
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    double g[3];
+    grad_scalar(b, sn, vol, v + (a + 1) * ldv, v, n, c, g);
+#pragma unroll
+    for (int q = 0; q < 3; q++) g9[(3 * a + q) * b.nnn + m] = g[q];
+  }
+}
+
+// met_prod.f90:54-72
+__global__ void __launch_bounds__(TX *TY) k_met_prod(Blk b, const double *__restrict__ mut,
+                                                     const double *__restrict__ v, long ldv,
+                                                     const double *__restrict__ g9, double *__restrict__ tprod) {
+  REAL_CELL(b)
+  long N = b.nnn;
+  double xx = g9[m], xy = g9[N + m], xz = g9[2 * N + m], yx = g9[3 * N + m], yy = g9[4 * N + m];
+  double yz = g9[5 * N + m], zx = g9[6 * N + m], zy = g9[7 * N + m], zz = g9[8 * N + m];
+  double s6 = v[5 * ldv + n], mt = mut[n];
+  const double ds3 = 2. / 3.;
+  double torxx = ds3 * (-s6 - mt * (-2 * xx + yy + zz));
+  double toryy = ds3 * (-s6 - mt * (-2 * yy + xx + zz));
+  double torzz = ds3 * (-s6 - mt * (-2 * zz + yy + xx));
+  double torxy = mt * (xy + yx), torxz = mt * (xz + zx), toryz = mt * (zy + yz);
+  tprod[m] = xx * torxx + yy * toryy + zz * torzz + torxy * (xy + yx) + torxz * (xz + zx) + toryz * (zy + yz);
+}
+
+// met_gradtr.f90:107-171: fd = grad(v6/rho), grad(v7/rho) over the contiguous
+// storage range first..last real cell (ghost entries inside it get 0).
+__global__ void __launch_bounds__(256) k_met_gradtr(Blk b, long n0, long n1, const double *__restrict__ sn,
+                                                    const double *__restrict__ vol, const double *__restrict__ v,
+                                                    long ldv, double *__restrict__ fd) {
+  long n = n0 + blockIdx.x * 256L + threadIdx.x;
+  if (n > n1) return;
+  long m = n - b.npc;
+  int c[3];
+  c[0] = b.id1 + (int)(m % b.nid);
+  c[1] = b.jd1 + (int)((m / b.nid) % (b.nijd / b.nid));
+  c[2] = b.kd1 + (int)(m / b.nijd);
+  bool real = b.real(c[0], c[1], c[2]);
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+    double g[3] = {0., 0., 0.};
+    if (real) grad_scalar(b, sn, vol, v + (5 + a) * ldv, v, n, c, g);
+#pragma unroll
+    for (int q = 0; q < 3; q++) fd[(3 * a + q) * ldv + n] = g[q];
+  }
+}
+
+struct Turb {
+  double cb1, cb2, sigma, kappa, cw1, cw2, cw3, cv1, ct3, ct4, betas, beta, sigmk, sigmw, alfak, alfae;
+};
+
+// met_difsa.f90 + met_smsa.f90:91-222 (Spalart-Allmaras)
+__global__ void __launch_bounds__(TX *TY) k_met_smsa(Blk b, Turb tc, const double *__restrict__ sn,
+                                                     const double *__restrict__ vol, const double *__restrict__ v,
+                                                     long ldv, const double *__restrict__ mu,
+                                                     const double *__restrict__ dist, const double *__restrict__ g9,
+                                                     const double *__restrict__ fd, double *__restrict__ cfke,
+                                                     double *__restrict__ ts6, double *__restrict__ ts7) {
+  REAL_CELL(b)
+  long N = b.nnn;
+  double gd[3], gr[3];
+  grad_scalar(b, sn, vol, v + 5 * ldv, nullptr, n, c, gd);  // grad(rho nu~)
+  grad_scalar(b, sn, vol, v, nullptr, n, c, gr);             // grad(rho)
+  double sdif = gd[0] * fd[n] + gd[1] * fd[ldv + n] + gd[2] * fd[2 * ldv + n];
+  sdif = sdif * tc.cb2 / tc.sigma;
+  double kappa2 = tc.kappa * tc.kappa, uns6 = 1. / 6.;
+  double cv13 = tc.cv1 * tc.cv1 * tc.cv1, cw36 = pow(tc.cw3, 6.), cv133 = 3. * cv13, ct42 = 2. * tc.ct4;
+  double cb2sig = tc.cb2 / tc.sigma;
+  double xy = g9[N + m], xz = g9[2 * N + m], yx = g9[3 * N + m], yz = g9[5 * N + m], zx = g9[6 * N + m];
+  double zy = g9[7 * N + m];
+  double a = zy - yz, bb = xz - zx, cc = yx - xy;
+  double vort = sqrt(a * a + bb * bb + cc * cc);
+  double v6 = v[5 * ldv + n], v1 = v[n];
+  double xkhi = v6 / mu[n];
+  double xkhi2 = xkhi * xkhi, xkhi3 = xkhi2 * xkhi;
+  double dist2 = dist[n] * dist[n];
+  double nutilde = v6 / v1;
+  double kapd2 = kappa2 * dist2;
+  double fv1 = xkhi3 / (xkhi3 + cv13);
+  double fv2 = 1. - (xkhi / (1 + xkhi * fv1));
+  double stilde = vort + nutilde * fv2 / kapd2;
+  double rtilde = fmin(nutilde / (stilde * kapd2), 1.);
+  double rtil6 = pow(rtilde, 6.);
+  double ft2 = tc.ct3 * exp(-tc.ct4 * xkhi2);
+  double sm1 = tc.cb1 * (1. - ft2) * stilde * v6;
+  double xg = rtilde + tc.cw2 * (rtil6 - rtilde);
+  double xg6 = pow(xg, 6.);
+  double fw = xg * pow((1. + cw36) / (xg6 + cw36), uns6);
+  double sm2 = -(tc.cw1 * fw - tc.cb1 * ft2 / kappa2) * v6 * nutilde / dist2;
+  ts6[n] = sm1 + sm2 + sdif;
+  ts7[n] = 0.;
+  double xkhi4 = xkhi3 * xkhi;
+  double dfv1 = cv133 * (fv1 * fv1) / (mu[n] * xkhi4);
+  double dfv2 = (xkhi2 * dfv1 - 1. / mu[n]) / ((1. + xkhi * fv1) * (1. + xkhi * fv1));
+  double dst = (fv2 / v1 + nutilde * dfv2) / kapd2;
+  double dpr = (1. / v1 - nutilde * dst / stilde) / (stilde * kapd2);
+  double dxg = dpr * (1. + tc.cw2 * (6. * rtil6 / rtilde - 1.));
+  double dfw = (1. - xg6 / (xg6 + cw36)) * fw * dxg / xg;
+  double dft2 = -ct42 * xkhi * ft2;
+  double dsm1 = tc.cb1 * ((1. - ft2) * (stilde + dst * v6) - stilde * v6 * dft2);
+  double dsm2 = 2. * sm2 / v6 - (tc.cw1 * dfw - tc.cb1 * dft2 / kappa2) * nutilde * v6 / dist2;
+  // the reference reads grad(nu~) at the block-local index here
+  // (met_smsa.f90:216-217), the entry n only for the first block
+  double dsdif = -cb2sig * (fd[m] * gr[0] + fd[ldv + m] * gr[1] + fd[2 * ldv + m] * gr[2]) / (v1 * v1);
+  cfke[n] = fabs(dsm1 + dsm2 + dsdif);
+}
+
+// met_smko.f90:80-110 (compressible Wilcox k-omega)
+__global__ void __launch_bounds__(TX *TY) k_met_smko(Blk b, Turb tc, const double *__restrict__ v, long ldv,
+                                                     const double *__restrict__ tprod, double *__restrict__ cfke,
+                                                     double *__restrict__ ts6, double *__restrict__ ts7,
+                                                     const double *__restrict__ cson) {
+  REAL_CELL(b)
+  const double xw = 5. / 9., xmt0 = 0.1;
+  double v1 = v[n], v6 = v[5 * ldv + n], v7 = v[6 * ldv + n];
+  double omeg = v7 / v1;
+  double xmt = sqrt(2. * v6 / v1) / cson[n];
+  double f = 0.5 * (1. + copysign(1.0, xmt - xmt0)) * (xmt * xmt - xmt0 * xmt0);
+  double betasc = tc.betas * (1. + 1.5 * f);
+  double betac = tc.beta - tc.betas * 1.5 * f;
+  ts6[n] = tprod[m] - betasc * v6 * omeg;
+  ts7[n] = xw * tprod[m] * v7 / v6 - betac * v7 * omeg + 0.;
+  cfke[n] = 0.18 * omeg;
+}
+
+// met_fludcsa (kfludis=2), met_fludko (5), met_fludc (1): flux densities
+// over the contiguous storage range first..last real cell
+__global__ void __launch_bounds__(256) k_met_flud(long n0, long n1, int kfludis, Turb tc, const double *__restrict__ v,
+                                                  long ldv, const double *__restrict__ mu,
+                                                  const double *__restrict__ mut, double *__restrict__ fd) {
+  long n = n0 + blockIdx.x * 256L + threadIdx.x;
+  if (n > n1) return;
+  double f5, f6;
+  if (kfludis == 2) {
+    f5 = (1. / tc.sigma) * (mu[n] + v[5 * ldv + n]);
+    f6 = 0.;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      fd[q * ldv + n] = f5 * fd[q * ldv + n];
+      fd[(3 + q) * ldv + n] = 0.;
+    }
+    return;
+  }
+  if (kfludis == 5) {
+    f5 = mu[n] + mut[n] * tc.sigmk;
+    f6 = mu[n] + mut[n] * tc.sigmw;
+  } else {
+    f5 = mu[n] + mut[n] / tc.alfak;
+    f6 = mu[n] + mut[n] / tc.alfae;
+  }
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    fd[q * ldv + n] = f5 * fd[q * ldv + n];
+    fd[(3 + q) * ldv + n] = f6 * fd[(3 + q) * ldv + n];
+  }
+}
+
+// sch_turb.f90:92-301 (met_pardis left to the caller): one thread per entry of
+// the zeroed range; real cells gather their six face fluxes.
+__device__ inline void turb_flux(const double *__restrict__ v, long ldv, const double *__restrict__ fd, long n,
+                                 double f[6]) {
+  double r = v[n];
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    double vq = v[(q + 1) * ldv + n] / r;
+    f[q] = v[5 * ldv + n] * vq - fd[q * ldv + n];
+    f[3 + q] = v[6 * ldv + n] * vq - fd[(3 + q) * ldv + n];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sch_turb(Blk b, long n0, long n1, double *__restrict__ u,
+                                                  const double *__restrict__ v, long ldv,
+                                                  const double *__restrict__ d, const double *__restrict__ fd,
+                                                  const double *__restrict__ ts6, const double *__restrict__ ts7,
+                                                  const double *__restrict__ sn, const double *__restrict__ vol) {
+  long n = n0 + blockIdx.x * 256L + threadIdx.x;
+  if (n > n1) return;
+  long m = n - b.npc;
+  int c[3];
+  c[0] = b.id1 + (int)(m % b.nid);
+  c[1] = b.jd1 + (int)((m / b.nid) % (b.nijd / b.nid));
+  c[2] = b.kd1 + (int)(m / b.nijd);
+  if (!b.real(c[0], c[1], c[2])) {
+    u[5 * ldv + n] = 0.;
+    u[6 * ldv + n] = 0.;
+    return;
+  }
+  double fc[6], s6 = 0., s7 = 0.;
+  turb_flux(v, ldv, fd, n, fc);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    long inc = b.inc(d);
+    int a = c[d], a1 = b.lo(d), a2 = b.hi(d);
+    double fl[6], fh[6];
+    turb_flux(v, ldv, fd, n - inc, fl);
+    turb_flux(v, ldv, fd, n + inc, fh);
+    double s1 = snv(sn, b.nnn, m, d, 0), s2 = snv(sn, b.nnn, m, d, 1), s3 = snv(sn, b.nnn, m, d, 2);
+    double x6, x7;
+    if (a == a1) {
+      x6 = 2 * fl[0] * s1 + 2 * fl[1] * s2 + 2 * fl[2] * s3;
+      x7 = 2 * fl[3] * s1 + 2 * fl[4] * s2 + 2 * fl[5] * s3;
+    } else {
+      x6 = (fc[0] + fl[0]) * s1 + (fc[1] + fl[1]) * s2 + (fc[2] + fl[2]) * s3;
+      x7 = (fc[3] + fl[3]) * s1 + (fc[4] + fl[4]) * s2 + (fc[5] + fl[5]) * s3;
+    }
+    s6 -= x6; s7 -= x7;
+    s1 = snv(sn, b.nnn, m + inc, d, 0); s2 = snv(sn, b.nnn, m + inc, d, 1); s3 = snv(sn, b.nnn, m + inc, d, 2);
+    if (a + 1 == a2) {
+      x6 = 2 * fh[0] * s1 + 2 * fh[1] * s2 + 2 * fh[2] * s3;
+      x7 = 2 * fh[3] * s1 + 2 * fh[4] * s2 + 2 * fh[5] * s3;
+    } else {
+      x6 = (fh[0] + fc[0]) * s1 + (fh[1] + fc[1]) * s2 + (fh[2] + fc[2]) * s3;
+      x7 = (fh[3] + fc[3]) * s1 + (fh[4] + fc[4]) * s2 + (fh[5] + fc[5]) * s3;
+    }
+    s6 += x6; s7 += x7;
+  }
+  u[5 * ldv + n] = 0.5 * s6 - vol[n] * ts6[n] - d[5 * ldv + n];
+  u[6 * ldv + n] = 0.5 * s7 - vol[n] * ts7[n] - d[6 * ldv + n];
+}
+
+static Turb make_turb(const double *tc) {
+  Turb t;
+  double *p = &t.cb1;
+  for (int i = 0; i < 16; i++) p[i] = tc[i];
+  return t;
+}
+
+#define CHECK_BLOCK(name)                              \
+  int rc = codens_check_block(cb, name);               \
+  if (rc) return rc;                                   \
+  Blk b = make_blk(cb);                                \
+  hipStream_t st = (hipStream_t)stream;
+
+extern "C" int codens_teq_gradv(const codens_block *cb, const double *sn, const double *vol, double *v, int64_t ldv,
+                                double *g9, void *stream) {
+  CHECK_BLOCK("codens_teq_gradv")
+  if (!sn || !vol || !v || !g9) { codens_set_error("codens_teq_gradv: null argument"); return CODENS_EINVAL; }
+  int nx = b.i2 - b.i1 + 2, ny = b.j2 - b.j1 + 2, nz = b.k2 - b.k1 + 2;
+  {
+    CodensTimed _tm("teq_gradv", st);
+    hipLaunchKernelGGL(k_rho_clamp, dim3((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz), dim3(TX, TY), 0, st, b, v);
+    hipLaunchKernelGGL(k_teq_gradv, cell_grid(b), dim3(TX, TY), 0, st, b, sn, vol, v, (long)ldv, g9);
+  }
+  return codens_check_launch("codens_teq_gradv");
+}
+
+extern "C" int codens_met_prod(const codens_block *cb, const double *mut, const double *v, int64_t ldv,
+                               const double *g9, double *tprod, void *stream) {
+  CHECK_BLOCK("codens_met_prod")
+  if (!mut || !v || !g9 || !tprod) { codens_set_error("codens_met_prod: null argument"); return CODENS_EINVAL; }
+  hipLaunchKernelGGL(k_met_prod, cell_grid(b), dim3(TX, TY), 0, st, b, mut, v, (long)ldv, g9, tprod);
+  return codens_check_launch("codens_met_prod");
+}
+
+extern "C" int codens_met_gradtr(const codens_block *cb, const double *sn, const double *vol, const double *v,
+                                 int64_t ldv, double *fd, void *stream) {
+  CHECK_BLOCK("codens_met_gradtr")
+  if (!sn || !vol || !v || !fd) { codens_set_error("codens_met_gradtr: null argument"); return CODENS_EINVAL; }
+  long n0 = b.idx(b.i1, b.j1, b.k1), n1 = b.idx(b.i2 - 1, b.j2 - 1, b.k2 - 1);
+  hipLaunchKernelGGL(k_met_gradtr, dim3((n1 - n0 + 256) / 256), dim3(256), 0, st, b, n0, n1, sn, vol, v, (long)ldv,
+                     fd);
+  return codens_check_launch("codens_met_gradtr");
+}
+
+extern "C" int codens_met_smsa(const codens_block *cb, const double *tc, const double *sn, const double *vol,
+                               const double *v, int64_t ldv, const double *mu, const double *dist, const double *g9,
+                               const double *fd, double *cfke, double *ts6, double *ts7, void *stream) {
+  CHECK_BLOCK("codens_met_smsa")
+  if (!tc || !sn || !vol || !v || !mu || !dist || !g9 || !fd || !cfke || !ts6 || !ts7) {
+    codens_set_error("codens_met_smsa: null argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_met_smsa, cell_grid(b), dim3(TX, TY), 0, st, b, make_turb(tc), sn, vol, v, (long)ldv, mu, dist,
+                     g9, fd, cfke, ts6, ts7);
+  return codens_check_launch("codens_met_smsa");
+}
+
+extern "C" int codens_met_smko(const codens_block *cb, const double *tc, const double *v, int64_t ldv,
+                               const double *tprod, double *cfke, double *ts6, double *ts7, const double *cson,
+                               void *stream) {
+  CHECK_BLOCK("codens_met_smko")
+  if (!tc || !v || !tprod || !cfke || !ts6 || !ts7 || !cson) {
+    codens_set_error("codens_met_smko: null argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_met_smko, cell_grid(b), dim3(TX, TY), 0, st, b, make_turb(tc), v, (long)ldv, tprod, cfke, ts6,
+                     ts7, cson);
+  return codens_check_launch("codens_met_smko");
+}
+
+extern "C" int codens_met_flud(const codens_block *cb, int32_t kfludis, const double *tc, const double *v,
+                               int64_t ldv, const double *mu, const double *mut, double *fd, void *stream) {
+  CHECK_BLOCK("codens_met_flud")
+  if (!tc || !v || !mu || !mut || !fd || (kfludis != 1 && kfludis != 2 && kfludis != 5)) {
+    codens_set_error("codens_met_flud: invalid argument (kfludis %d)", kfludis);
+    return CODENS_EINVAL;
+  }
+  long n0 = b.idx(b.i1, b.j1, b.k1), n1 = b.idx(b.i2 - 1, b.j2 - 1, b.k2 - 1);
+  hipLaunchKernelGGL(k_met_flud, dim3((n1 - n0 + 256) / 256), dim3(256), 0, st, n0, n1, (int)kfludis, make_turb(tc),
+                     v, (long)ldv, mu, mut, fd);
+  return codens_check_launch("codens_met_flud");
+}
+
+extern "C" int codens_sch_turb(const codens_block *cb, double *u, const double *v, int64_t ldv, const double *d,
+                               const double *fd, const double *ts6, const double *ts7, const double *sn,
+                               const double *vol, void *stream) {
+  CHECK_BLOCK("codens_sch_turb")
+  if (!u || !v || !d || !fd || !ts6 || !ts7 || !sn || !vol) {
+    codens_set_error("codens_sch_turb: null argument");
+    return CODENS_EINVAL;
+  }
+  long n0 = b.idx(b.i1 - 1, b.j1 - 1, b.k1 - 1), n1 = b.idx(b.i2, b.j2, b.k2);
+  {
+    CodensTimed _tm("sch_turb", st);
+    hipLaunchKernelGGL(k_sch_turb, dim3((n1 - n0 + 256) / 256), dim3(256), 0, st, b, n0, n1, u, v, (long)ldv, d, fd,
+                       ts6, ts7, sn, vol);
+  }
+  return codens_check_launch("codens_sch_turb");
+}

@@ -124,6 +124,42 @@ int codens_met_roe2o(const codens_block *b, const double *t, int64_t ldv, double
 int codens_met_roe(const codens_block *b, const double *t, int64_t ldv, double *d, const double *sn_blk,
                    void *stream);
 
+/* Turbulence chain of met_num.f90:162-567.  tc points at the 16 model
+ * constants of module `modeleturb` (atlecdon.f90:112-400) in the order
+ * cb1 cb2 sigma kappa cw1 cw2 cw3 cv1 ct3 ct4 betas beta sigmk sigmw alfak
+ * alfae.  g9 (velocity gradients dvxx..dvzz) and tprod are block-local arrays
+ * of nnn entries like the reference's ip00 scratch; fd is the 6-array bundle
+ * fd5x fd5y fd5z fd6x fd6y fd6z of leading dimension ldv. */
+
+/* teq_gradv.f90:4 -- clamps rho >= 1e-20 on the block box (a side effect
+ * the reference has, :79-86), then Green-Gauss gradients of u, v, w. */
+int codens_teq_gradv(const codens_block *b, const double *sn_blk, const double *vol, double *v, int64_t ldv,
+                     double *g9, void *stream);
+/* met_prod.f90:4 -- production of k, tau^R : grad u. */
+int codens_met_prod(const codens_block *b, const double *mut, const double *v, int64_t ldv, const double *g9,
+                    double *tprod, void *stream);
+/* met_gradtr.f90:4 -- fd = grad(v6/rho), grad(v7/rho). */
+int codens_met_gradtr(const codens_block *b, const double *sn_blk, const double *vol, const double *v,
+                      int64_t ldv, double *fd, void *stream);
+/* met_smsa.f90:4 with met_difsa.f90:4 -- Spalart-Allmaras source ts6 (ts7=0)
+ * and implicit coefficient cfke; fd holds grad(nu~) from met_gradtr. */
+int codens_met_smsa(const codens_block *b, const double *tc, const double *sn_blk, const double *vol,
+                    const double *v, int64_t ldv, const double *mu, const double *dist, const double *g9,
+                    const double *fd, double *cfke, double *ts6, double *ts7, void *stream);
+/* met_smko.f90:4 -- compressible Wilcox k-omega sources. */
+int codens_met_smko(const codens_block *b, const double *tc, const double *v, int64_t ldv, const double *tprod,
+                    double *cfke, double *ts6, double *ts7, const double *cson, void *stream);
+/* met_fludc.f90:4 (kfludis=1), met_fludcsa.f90:4 (2), met_fludko.f90:4 (5)
+ * -- gradients fd become diffusive flux densities, in place. */
+int codens_met_flud(const codens_block *b, int32_t kfludis, const double *tc, const double *v, int64_t ldv,
+                    const double *mu, const double *mut, double *fd, void *stream);
+/* sch_turb.f90:4 -- flux balance u6, u7 = 0.5 sum of centred face fluxes
+ * - vol ts - d; the wall-layer zeroing met_pardis (kparoi=1) is applied by
+ * the caller to d beforehand. */
+int codens_sch_turb(const codens_block *b, double *u, const double *v, int64_t ldv, const double *d,
+                    const double *fd, const double *ts6, const double *ts7, const double *sn_blk,
+                    const double *vol, void *stream);
+
 /* sch_expli.f90:4 -- v += dt/vol * u on the real cells. */
 int codens_sch_expli(const codens_block *b, const double *u, double *v, int64_t ldv,
                      const double *dt, const double *vol, void *stream);

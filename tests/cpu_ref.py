@@ -269,3 +269,53 @@ def _ref_turb_call(self, name, l, *args):
 
 
 Ref.set_turb = _ref_set_turb
+
+
+def _o_teq_gradv(self, l, sn, vol, v):
+    b = self.lay.blocks[l]
+    g = np.zeros((9, b.nnn))
+    self.lib.orc_teq_gradv(self.B(l), ptr(np.ascontiguousarray(sn_block(sn, b))), ptr(vol), ptr(v), self.ld, ptr(g))
+    return g
+
+
+def _o_met_prod(self, l, mut, v, g9):
+    b = self.lay.blocks[l]
+    tp = np.zeros(b.nnn)
+    self.lib.orc_met_prod(self.B(l), ptr(mut), ptr(v), self.ld, ptr(np.ascontiguousarray(g9)), ptr(tp))
+    return tp
+
+
+def _o_met_gradtr(self, l, sn, vol, v, fd):
+    b = self.lay.blocks[l]
+    self.lib.orc_met_gradtr(self.B(l), ptr(np.ascontiguousarray(sn_block(sn, b))), ptr(vol), ptr(v), self.ld, ptr(fd))
+
+
+def _o_met_smsa(self, l, tc, sn, vol, v, mu, dist, g9, fd, cfke, ts6, ts7):
+    b = self.lay.blocks[l]
+    self.lib.orc_met_smsa(self.B(l), ctypes.byref(tc), ptr(np.ascontiguousarray(sn_block(sn, b))), ptr(vol), ptr(v),
+                          self.ld, ptr(mu), ptr(dist), ptr(np.ascontiguousarray(g9)), ptr(fd), ptr(cfke), ptr(ts6),
+                          ptr(ts7))
+
+
+def _o_met_smko(self, l, tc, v, tprod, cfke, ts6, ts7, cson):
+    self.lib.orc_met_smko(self.B(l), ctypes.byref(tc), ptr(v), self.ld, ptr(np.ascontiguousarray(tprod)), ptr(cfke),
+                          ptr(ts6), ptr(ts7), ptr(cson))
+
+
+def _o_met_flud(self, l, kfl, tc, v, mu, mut, fd):
+    self.lib.orc_met_flud(self.B(l), kfl, ctypes.byref(tc), ptr(v), self.ld, ptr(mu), ptr(mut), ptr(fd))
+
+
+def _o_sch_turb(self, l, u, v, d, fd, ts6, ts7, sn, vol):
+    b = self.lay.blocks[l]
+    self.lib.orc_sch_turb(self.B(l), ptr(u), ptr(v), self.ld, ptr(d), ptr(fd), ptr(ts6), ptr(ts7),
+                          ptr(np.ascontiguousarray(sn_block(sn, b))), ptr(vol))
+
+
+Orc.teq_gradv = _o_teq_gradv
+Orc.met_prod = _o_met_prod
+Orc.met_gradtr = _o_met_gradtr
+Orc.met_smsa = _o_met_smsa
+Orc.met_smko = _o_met_smko
+Orc.met_flud = _o_met_flud
+Orc.sch_turb = _o_sch_turb

@@ -57,10 +57,19 @@ def test_wall_ns_reduced_10_stages():
 
 
 def test_wall_ns_full_one_stage():
+    """Full 256x128x64 block, minmod limiter: one stage from a developed state
+    (20 device stages from free stream, copied to the oracle), residual within
+    1e-12 and updated state within 1e-10.  From the free stream itself the
+    residual is a cancellation of O(1) fluxes down to round-off, so its
+    relative error only measures rounding."""
     from codens_amd import configs
     import cpu_stage
     case = configs.wall_ns()
-    g, c = _solver(case), cpu_stage.CpuStage(case)
+    g = _solver(case)
+    for _ in range(20):
+        g.stage()
+    c = cpu_stage.CpuStage(case)
+    c.v[:] = g.host_state()
     g.stage(); c.stage()
     torch.cuda.synchronize()
     _cmp(case, g.u.cpu().numpy(), c.u, 1e-12)

@@ -1,0 +1,328 @@
+! Drop-in replacements, on the GPU, for the per-block routines of the
+! residual-and-update path.  Each module keeps the reference's module name,
+! subroutine name and dummy-argument list, so the reference's callers
+! (atsch_num.f90, met_num.f90, smg_flu.f90) compile against them unchanged
+! when these files are built in place of the originals.
+!
+! Every call moves block lm's slice of the arrays it reads to the device
+! (codens_dev), runs the C-ABI entry point of include/codens.h on the
+! library's stream, and copies back the slice of the arrays it writes
+! (codens_back).  Output arrays are copied in as well, so entries a kernel
+! leaves alone keep the caller's values.  Host scratch arguments (rhol..prr,
+! tn*, dfxx..rv, sfsi.., snu, del6/7) are not touched: the kernels keep their
+! intermediates on the device.  Only the 3-D form is provided
+! (equat(3:4) = '3d'); other forms stop.
+!
+! Set codens_geom_once = .true. (module codens_c) once all metrics sn and
+! volumes vol are final to upload them only on first use.
+
+module mod_zpres
+  implicit none
+contains
+  ! zpres.f90:4 -- pressure, temperature and sound speed of block l.
+  subroutine zpres(l, v, pression, ztemp, cson)
+    use para_var
+    use codens_c
+    implicit none
+    integer :: l
+    double precision, target :: v(ip11,ip60), pression(ip11), ztemp(ip11), cson(ip11)
+    type(codens_block) :: b
+    type(c_ptr) :: dv, dp, dtm, dc
+    b = codens_block_of(l)
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 5, b, 1)
+    dp = codens_dev(c_loc(pression), ip11, ip11, 1, 1, b, 1)
+    dtm = codens_dev(c_loc(ztemp), ip11, ip11, 1, 1, b, 1)
+    dc = codens_dev(c_loc(cson), ip11, ip11, 1, 1, b, 1)
+    call codens_check(codens_zpres(b, codens_phys_now(), dv, int(ip11, c_int64_t), dp, dtm, dc, c_null_ptr), &
+         'zpres')
+    call codens_back(c_loc(pression), ip11, 1, 1, b)
+    call codens_back(c_loc(ztemp), ip11, 1, 1, b)
+    call codens_back(c_loc(cson), ip11, 1, 1, b)
+  end subroutine zpres
+end module mod_zpres
+
+module mod_zvismo
+  implicit none
+contains
+  ! zvismo.f90:4 -- molecular viscosity; stops on T <= 0 as the reference
+  ! does (zvismo.f90:122-127).
+  subroutine zvismo(l, mu, s, temp)
+    use para_var
+    use sortiefichier
+    use codens_c
+    implicit none
+    integer :: l
+    double precision, target :: mu(ip12), s(ip11,ip60), temp(ip11)
+    integer(c_int32_t), target, save :: nbad
+    type(codens_block) :: b
+    type(c_ptr) :: dmu, dtm, dn
+    b = codens_block_of(l)
+    dtm = codens_dev(c_loc(temp), ip11, ip11, 1, 1, b, 1)
+    dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+    nbad = 0
+    call codens_check(codens_host_mirror(c_loc(nbad), 4_c_int64_t, 0_c_int64_t, 4_c_int64_t, 4_c_int64_t, &
+         1_c_int32_t, 1_c_int32_t, dn, c_null_ptr), 'zvismo')
+    call codens_check(codens_zvismo(b, codens_phys_now(), dtm, dmu, dn, c_null_ptr), 'zvismo')
+    call codens_back(c_loc(mu), ip12, 1, 1, b)
+    call codens_check(codens_host_fetch(c_loc(nbad), 0_c_int64_t, 4_c_int64_t, 4_c_int64_t, 1_c_int32_t, &
+         c_null_ptr), 'zvismo')
+    if (nbad .ne. 0) then
+       write(imp,'(/,"!!!zvismo: temperature negative en",i8," cellules domaine= ",i4)') nbad, l
+       error stop 'zvismo: non-positive temperature'
+    endif
+  end subroutine zvismo
+end module mod_zvismo
+
+module mod_chronos
+  implicit none
+contains
+  ! chronos.f90:4 -- local time step of block l (viscous limit for 'ns').
+  subroutine chronos(l, etal, mu, mut, t, dt, equat, sn, lgsnlt, vol, sfsi, sfsj, sfsk, dism, cson)
+    use para_var
+    use para_fige
+    use codens_c
+    implicit none
+    integer :: l, lgsnlt
+    double precision :: etal
+    double precision, target :: mu(ip12), mut(ip12), t(ip11,ip60), dt(ip11), sn(lgsnlt,nind,ndir), vol(ip11)
+    double precision, target :: cson(ip11)
+    double precision :: sfsi(ip00), sfsj(ip00), sfsk(ip00), dism(ip00)
+    character(len=7) :: equat
+    type(codens_block) :: b
+    type(c_ptr) :: dmu, dmut, dv, ddt, dsn, dvol, dc
+    integer(c_int32_t) :: ns
+    call codens_need_3d(equat, 'chronos')
+    b = codens_block_of(l)
+    ns = 0
+    if (equat(1:2) .eq. 'ns') ns = 1
+    dv = codens_dev(c_loc(t), ip11*ip60, ip11, 1, 5, b, 1)
+    dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+    dmut = codens_dev(c_loc(mut), ip12, ip12, 1, 1, b, 1)
+    ddt = codens_dev(c_loc(dt), ip11, ip11, 1, 1, b, 1)
+    dc = codens_dev(c_loc(cson), ip11, ip11, 1, 1, b, 1)
+    dsn = codens_dev_all(c_loc(sn), 9*lgsnlt, codens_geom_mode())
+    dvol = codens_dev(c_loc(vol), ip11, ip11, 1, 1, b, codens_geom_mode())
+    call codens_check(codens_chronos(b, codens_phys_now(), etal, ns, dv, int(ip11, c_int64_t), dmu, dmut, &
+         dsn, dvol, dc, ddt, c_null_ptr), 'chronos')
+    call codens_back(c_loc(dt), ip11, 1, 1, b)
+  end subroutine chronos
+end module mod_chronos
+
+module mod_sch_roe_euler
+  implicit none
+contains
+  ! sch_roe_euler.f90:4 -- Euler Roe residual u of block lm (and ff -= u
+  ! when ityprk /= 0).
+  subroutine sch_roe_euler(lm, ityprk, u, v, ff, equat, sn, lgsnlt, rhol, ul, vl, wl, pl, rhor, ur, vr, wr, &
+       prr, ps)
+    use para_var
+    use para_fige
+    use codens_c
+    implicit none
+    integer :: lm, ityprk, lgsnlt
+    double precision, target :: u(ip11,ip60), v(ip11,ip60), ff(ip11,ip60), sn(lgsnlt,nind,ndir), ps(ip11)
+    double precision :: rhol(ip00), ul(ip00), vl(ip00), wl(ip00), pl(ip00)
+    double precision :: rhor(ip00), ur(ip00), vr(ip00), wr(ip00), prr(ip00)
+    character(len=7) :: equat
+    type(codens_block) :: b
+    type(c_ptr) :: du, dv, dff, dsn, dps
+    call codens_need_3d(equat, 'sch_roe_euler')
+    b = codens_block_of(lm)
+    du = codens_dev(c_loc(u), ip11*ip60, ip11, 1, 5, b, 1)
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 5, b, 1)
+    dff = c_null_ptr
+    if (ityprk .ne. 0) dff = codens_dev(c_loc(ff), ip11*ip60, ip11, 1, 5, b, 1)
+    dsn = codens_dev_all(c_loc(sn), 9*lgsnlt, codens_geom_mode())
+    dps = codens_dev(c_loc(ps), ip11, ip11, 1, 1, b, 1)
+    call codens_check(codens_sch_roe_euler(b, codens_phys_now(), codens_scheme_now(), int(ityprk, c_int32_t), &
+         du, dv, int(ip11, c_int64_t), dff, dsn, dps, c_null_ptr), 'sch_roe_euler')
+    call codens_back(c_loc(u), ip11, 1, 5, b)
+    if (ityprk .ne. 0) call codens_back(c_loc(ff), ip11, 1, 5, b)
+  end subroutine sch_roe_euler
+end module mod_sch_roe_euler
+
+module mod_sch_roe
+  implicit none
+contains
+  ! sch_roe.f90:4 -- Navier-Stokes Roe residual u of block lm: MUSCL with the
+  ! limiter of module schemanum, plus the viscous face terms of toxx..qcz.
+  subroutine sch_roe(lm, ityprk, u, v, ff, toxx, toxy, toxz, toyy, toyz, tozz, qcx, qcy, qcz, equat, sn, &
+       lgsnlt, rhol, ul, vl, wl, pl, rhor, ur, vr, wr, prr, ps)
+    use para_var
+    use para_fige
+    use codens_c
+    implicit none
+    integer :: lm, ityprk, lgsnlt
+    double precision, target :: u(ip11,ip60), v(ip11,ip60), ff(ip11,ip60), sn(lgsnlt,nind,ndir), ps(ip11)
+    double precision, target :: toxx(ip12), toxy(ip12), toxz(ip12), toyy(ip12), toyz(ip12), tozz(ip12)
+    double precision, target :: qcx(ip12), qcy(ip12), qcz(ip12)
+    double precision :: rhol(ip00), ul(ip00), vl(ip00), wl(ip00), pl(ip00)
+    double precision :: rhor(ip00), ur(ip00), vr(ip00), wr(ip00), prr(ip00)
+    character(len=7) :: equat
+    type(codens_block) :: b
+    type(c_ptr) :: du, dv, dff, dtq, dsn, dps, tq(9)
+    call codens_need_3d(equat, 'sch_roe')
+    b = codens_block_of(lm)
+    tq = [c_loc(toxx), c_loc(toxy), c_loc(toxz), c_loc(toyy), c_loc(toyz), c_loc(tozz), c_loc(qcx), &
+         c_loc(qcy), c_loc(qcz)]
+    du = codens_dev(c_loc(u), ip11*ip60, ip11, 1, 5, b, 1)
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 5, b, 1)
+    dff = c_null_ptr
+    if (ityprk .ne. 0) dff = codens_dev(c_loc(ff), ip11*ip60, ip11, 1, 5, b, 1)
+    dtq = codens_tq_in(tq, ip12, b)
+    dsn = codens_dev_all(c_loc(sn), 9*lgsnlt, codens_geom_mode())
+    dps = codens_dev(c_loc(ps), ip11, ip11, 1, 1, b, 1)
+    call codens_check(codens_sch_roe(b, codens_phys_now(), codens_scheme_now(), int(ityprk, c_int32_t), du, dv, &
+         int(ip11, c_int64_t), dff, dtq, dsn, dps, c_null_ptr), 'sch_roe')
+    call codens_back(c_loc(u), ip11, 1, 5, b)
+    if (ityprk .ne. 0) call codens_back(c_loc(ff), ip11, 1, 5, b)
+  end subroutine sch_roe
+end module mod_sch_roe
+
+module mod_sch_expli
+  implicit none
+contains
+  ! sch_expli.f90:4 -- explicit update v += dt/vol*u of block lm.
+  subroutine sch_expli(lm, u, v, dt, vol)
+    use para_var
+    use codens_c
+    implicit none
+    integer :: lm
+    double precision, target :: u(ip11,ip60), v(ip11,ip60), dt(ip11), vol(ip11)
+    type(codens_block) :: b
+    type(c_ptr) :: du, dv, ddt, dvol
+    b = codens_block_of(lm)
+    du = codens_dev(c_loc(u), ip11*ip60, ip11, 1, 5, b, 1)
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 5, b, 1)
+    ddt = codens_dev(c_loc(dt), ip11, ip11, 1, 1, b, 1)
+    dvol = codens_dev(c_loc(vol), ip11, ip11, 1, 1, b, codens_geom_mode())
+    call codens_check(codens_sch_expli(b, du, dv, int(ip11, c_int64_t), ddt, dvol, c_null_ptr), 'sch_expli')
+    call codens_back(c_loc(v), ip11, 1, 5, b)
+  end subroutine sch_expli
+end module mod_sch_expli
+
+module mod_implimf_3d
+  implicit none
+contains
+  ! implimf_3d.f90:4 -- matrix-free point-Jacobi implicit phase of block lm
+  ! (lmx sub-iterations), then v += d.
+  subroutine implimf_3d(lm, u, dt, v, d, ff, mu, mut, equat, lmx, sn, lgsnlt, vol, dtpas, ityprk, &
+       dfxx, dfyy, dfzz, dfxy, dfxz, dfyz, dfex, dfey, dfez, rv, ps, cson)
+    use para_var
+    use para_fige
+    use schemanum, only: kfmg
+    use codens_c
+    implicit none
+    integer :: lm, lmx, lgsnlt, ityprk
+    double precision :: dtpas
+    double precision, target :: u(ip11,ip60), dt(ip11), v(ip11,ip60), d(ip11,ip60), ff(ip11,ip60)
+    double precision, target :: mu(ip12), mut(ip12), sn(lgsnlt,nind,ndir), vol(ip11), ps(ip11), cson(ip11)
+    double precision :: dfxx(ip00), dfyy(ip00), dfzz(ip00), dfxy(ip00), dfxz(ip00), dfyz(ip00)
+    double precision :: dfex(ip00), dfey(ip00), dfez(ip00), rv(ip00)
+    character(len=7) :: equat
+    type(codens_block) :: b
+    type(c_ptr) :: du, ddt, dv, dd, dff, dmu, dmut, dsn, dvol, dps, dc
+    call codens_need_3d(equat, 'implimf_3d')
+    b = codens_block_of(lm)
+    du = codens_dev(c_loc(u), ip11*ip60, ip11, 1, 5, b, 1)
+    ddt = codens_dev(c_loc(dt), ip11, ip11, 1, 1, b, 1)
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 5, b, 1)
+    dd = codens_dev(c_loc(d), ip11*ip60, ip11, 1, 5, b, 1)
+    dff = c_null_ptr
+    if (ityprk .ne. 0) dff = codens_dev(c_loc(ff), ip11*ip60, ip11, 1, 5, b, 1)
+    dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+    dmut = codens_dev(c_loc(mut), ip12, ip12, 1, 1, b, 1)
+    dsn = codens_dev_all(c_loc(sn), 9*lgsnlt, codens_geom_mode())
+    dvol = codens_dev(c_loc(vol), ip11, ip11, 1, 1, b, codens_geom_mode())
+    dps = codens_dev(c_loc(ps), ip11, ip11, 1, 1, b, 1)
+    dc = codens_dev(c_loc(cson), ip11, ip11, 1, 1, b, 1)
+    call codens_check(codens_implimf_3d(b, codens_phys_now(), du, ddt, dv, int(ip11, c_int64_t), dd, dff, dmu, &
+         dmut, int(lmx, c_int32_t), dsn, dvol, dtpas, int(kfmg, c_int32_t), int(ityprk, c_int32_t), dps, dc, &
+         c_null_ptr), 'implimf_3d')
+    call codens_back(c_loc(v), ip11, 1, 5, b)
+    call codens_back(c_loc(d), ip11, 1, 5, b)
+  end subroutine implimf_3d
+end module mod_implimf_3d
+
+module mod_dissip_jameson
+  implicit none
+contains
+  ! dissip_jameson.f90:4 -- JST dissipation d(:,1:5) of block lm with the
+  ! block's coefficients ki2(lm), ki4(lm).
+  subroutine dissip_jameson(lm, v, d, equat, sn, lgsnlt, snu, ps, cson)
+    use para_var
+    use para_fige
+    use schemanum, only: ki2, ki4
+    use codens_c
+    implicit none
+    integer :: lm, lgsnlt
+    double precision, target :: v(ip11,ip60), d(ip11,ip60), sn(lgsnlt,nind,ndir), ps(ip11), cson(ip11)
+    double precision :: snu(ip00)
+    character(len=7) :: equat
+    type(codens_block) :: b
+    type(c_ptr) :: dv, dd, dsn, dps, dc
+    call codens_need_3d(equat, 'dissip_jameson')
+    b = codens_block_of(lm)
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 5, b, 1)
+    dd = codens_dev(c_loc(d), ip11*ip60, ip11, 1, 5, b, 1)
+    dsn = codens_dev_all(c_loc(sn), 9*lgsnlt, codens_geom_mode())
+    dps = codens_dev(c_loc(ps), ip11, ip11, 1, 1, b, 1)
+    dc = codens_dev(c_loc(cson), ip11, ip11, 1, 1, b, 1)
+    call codens_check(codens_dissip_jameson(b, ki2(lm), ki4(lm), dv, int(ip11, c_int64_t), dd, dsn, dps, dc, &
+         c_null_ptr), 'dissip_jameson')
+    call codens_back(c_loc(d), ip11, 1, 5, b)
+  end subroutine dissip_jameson
+end module mod_dissip_jameson
+
+module mod_met_roe2o
+  implicit none
+contains
+  ! met_roe2o.f90:4 -- second-order upwind dissipation d(:,6:7) of the two
+  ! transport equations of block l.  (The reference declares it pure; a
+  ! routine that calls the device cannot be, and no caller needs it.)
+  subroutine met_roe2o(l, t, d, sn, lgsnlt, vol, del6, del7)
+    use para_var
+    use para_fige
+    use codens_c
+    implicit none
+    integer, intent(in) :: l, lgsnlt
+    double precision, target, intent(inout) :: d(ip11,ip60)
+    double precision, intent(inout) :: del6(ip00), del7(ip00)
+    double precision, target, intent(in) :: t(ip11,ip60), sn(lgsnlt,nind,ndir)
+    double precision, intent(in) :: vol(ip11)
+    type(codens_block) :: b
+    type(c_ptr) :: dt7, dd, dsn
+    b = codens_block_of(l)
+    dt7 = codens_dev(c_loc(t), ip11*ip60, ip11, 1, 7, b, 1)
+    dd = codens_dev(c_loc(d), ip11*ip60, ip11, 6, 2, b, 1)
+    dsn = codens_dev_all(c_loc(sn), 9*lgsnlt, codens_geom_mode())
+    call codens_check(codens_met_roe2o(b, dt7, int(ip11, c_int64_t), dd, dsn, c_null_ptr), 'met_roe2o')
+    call codens_back(c_loc(d), ip11, 6, 2, b)
+  end subroutine met_roe2o
+end module mod_met_roe2o
+
+module mod_met_roe
+  implicit none
+contains
+  ! met_roe.f90:4 -- first-order upwind dissipation d(:,6:7) of block l,
+  ! including the reference's one-entry index shift.
+  subroutine met_roe(l, t, d, equat, sn, lgsnlt, vol)
+    use para_var
+    use para_fige
+    use codens_c
+    implicit none
+    integer :: l, lgsnlt
+    double precision, target :: t(ip11,ip60), d(ip11,ip60), sn(lgsnlt,nind,ndir)
+    double precision :: vol(ip11)
+    character(len=7) :: equat
+    type(codens_block) :: b
+    type(c_ptr) :: dt7, dd, dsn
+    call codens_need_3d(equat, 'met_roe')
+    b = codens_block_of(l)
+    dt7 = codens_dev(c_loc(t), ip11*ip60, ip11, 1, 7, b, 1)
+    dd = codens_dev(c_loc(d), ip11*ip60, ip11, 6, 2, b, 1)
+    dsn = codens_dev_all(c_loc(sn), 9*lgsnlt, codens_geom_mode())
+    call codens_check(codens_met_roe(b, dt7, int(ip11, c_int64_t), dd, dsn, c_null_ptr), 'met_roe')
+    call codens_back(c_loc(d), ip11, 6, 2, b)
+  end subroutine met_roe
+end module mod_met_roe

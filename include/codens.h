@@ -214,6 +214,31 @@ int codens_sumsq_rate(const codens_block *blocks, int32_t nb, const double *a, c
 int codens_timing_enable(int32_t on);
 int codens_timing_read(const char *name, double *ms_total, int64_t *count, int32_t reset);
 
+/* Host-array mirrors for callers whose fields live in host memory (the
+ * Fortran drop-in routines in codens_amd/fortran, which keep the dummy
+ * arguments of the reference's in-place host routines, e.g. sch_roe.f90:4,
+ * implimf_3d.f90:4).  One device buffer per (device, host address) of the
+ * array's full `bytes`, grown on demand and kept until the process ends; it
+ * has the host array's layout, so device pointers into it use the same
+ * offsets.  Only a window moves: `rows` runs of `width` bytes, `pitch` bytes
+ * apart, from byte `offset` (one block's slice of each variable).
+ * codens_host_mirror returns the buffer in *dev; to_device = 1 copies the
+ * window host -> device on `stream`, 2 copies the whole array but only when
+ * the buffer is first made (arrays the caller never changes afterwards:
+ * metrics and volumes), 0 copies nothing; codens_host_fetch copies the window
+ * back and synchronises.  codens_host_bundle packs `count` host arrays of
+ * `bytes` each into consecutive slots of one buffer (the 9-array tq and
+ * 6-array fd bundles above) with the window [offset, offset+width) of each;
+ * codens_host_bundle_fetch copies every slot's window back and synchronises. */
+int codens_host_mirror(const void *host, int64_t bytes, int64_t offset, int64_t width, int64_t pitch,
+                       int32_t rows, int32_t to_device, void **dev, void *stream);
+int codens_host_fetch(void *host, int64_t offset, int64_t width, int64_t pitch, int32_t rows,
+                      void *stream);
+int codens_host_bundle(const void *const *hosts, int32_t count, int64_t bytes, int64_t offset,
+                       int64_t width, int32_t to_device, void **dev, void *stream);
+int codens_host_bundle_fetch(const void *const *hosts, int32_t count, int64_t bytes, int64_t offset,
+                             int64_t width, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,6 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 _ORACLE = os.path.join(HERE, "liboracle.so")
 _REF = os.path.join(HERE, "_ref", "libcodens_ref.so")
+_DROPIN = os.path.join(HERE, "_ref", "libcodens_dropin.so")
 
 _cache = {}
 
@@ -41,6 +42,12 @@ def oracle_lib():
 def ref_lib():
     """The flang build of the reference routines, or None where it was not built."""
     return _load(_REF)
+
+
+def dropin_lib():
+    """The reference harness relinked against the GPU drop-in modules of
+    codens_amd/fortran (same entry points as ref_lib), or None."""
+    return _load(_DROPIN)
 
 
 def ptr(a: np.ndarray):

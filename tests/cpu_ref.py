@@ -47,8 +47,8 @@ def sn_block(sn, b):
 class Ref:
     """The reference routines (flang build), one layout at a time."""
 
-    def __init__(self, layout, phys: OrcPhys, muscl=1, xk=1.0 / 3.0, ilim=0, kfmg=0, ns=True):
-        self.lib = oracle.ref_lib()
+    def __init__(self, layout, phys: OrcPhys, muscl=1, xk=1.0 / 3.0, ilim=0, kfmg=0, ns=True, lib=None):
+        self.lib = lib if lib is not None else oracle.ref_lib()
         if self.lib is None:
             raise RuntimeError("oracle/_ref/libcodens_ref.so not built")
         self.lay = layout

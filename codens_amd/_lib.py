@@ -76,7 +76,19 @@ SIGNATURES = {
 }
 
 
+def _one_hip_runtime():
+    """torch ships its own libamdhip64.so.7 with the same soname as ROCm's;
+    whichever is loaded first serves the whole process.  Importing torch
+    before this library keeps a single runtime for torch and the kernels
+    whatever the caller imported first (torch is optional for the library)."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load(path: str = LIB_PATH):
+    _one_hip_runtime()
     if not os.path.exists(path):
         raise ImportError(f"{path} is not built: run `python -m codens_amd.build` (or __graft_entry__.build())")
     lib = ctypes.CDLL(path)

@@ -46,7 +46,12 @@ def ref_lib():
 
 def dropin_lib():
     """The reference harness relinked against the GPU drop-in modules of
-    codens_amd/fortran (same entry points as ref_lib), or None."""
+    codens_amd/fortran (same entry points as ref_lib), or None.  The GPU
+    library is loaded first through codens_amd._lib, which settles the HIP
+    runtime of the process."""
+    if not os.path.exists(_DROPIN):
+        return None
+    import codens_amd._lib  # noqa: F401
     return _load(_DROPIN)
 
 

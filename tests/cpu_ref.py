@@ -236,6 +236,22 @@ def _orc_roe1(self, l, t, d, sn):
     self.lib.orc_met_roe(self.B(l), ptr(t), self.ld, ptr(d), ptr(snb))
 
 
+def met_roe_outside_entries(lay):
+    """(row, entry) pairs of d that the reference's met_roe zeroes outside the
+    block it works on: its shifted index (met_roe.f90:184-188) starts the
+    zeroing one storage entry early, i.e. at the previous block's last storage
+    entry, or, for the first block, at d(0,6) = d(ip11,5) and d(0,7) =
+    d(ip11,6) by column-major sequence association.  None of these entries belongs to a cell; the GPU
+    kernel and the oracle write inside the block only."""
+    out = []
+    for b in lay.blocks:
+        if b.npc == 0:
+            out += [(4, lay.ip11 - 1), (5, lay.ip11 - 1)]
+        else:
+            out += [(5, b.npc - 1), (6, b.npc - 1)]
+    return out
+
+
 Ref.met_roe2o = _ref_roe2o
 Ref.met_roe = _ref_roe1
 Orc.met_roe2o = _orc_roe2o

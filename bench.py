@@ -47,9 +47,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-stages", type=int, default=4)
+    ap.add_argument("--cpu-stages", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dims", type=str, default="256,128,64")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="finish with 3 device-to-device copies of MIB MiB: the known-size kernel that "
+                         "calibrates FETCH_SIZE/WRITE_SIZE counter passes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU path (block sharding with face halos over RCCL) "
@@ -108,6 +111,12 @@ def main():
         "kernels": kern,
         "density_residual": res,
     }
+    if args.copy_probe > 0:
+        src = torch.ones(args.copy_probe * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(case, args.cpu_stages)
     print(json.dumps(out))

@@ -57,6 +57,17 @@ __device__ inline double snv(const double *__restrict__ sn, long nnn, long m, in
 // Launch shape for kernels over the real cells: 64 (i) x 4 (j) threads,
 // one k plane per grid z.
 constexpr int TX = 64, TY = 4;
+// Work-group coordinates of a 3-D launch, remapped so that each of the 8 XCDs
+// (which receive work-groups round robin in launch order) owns a contiguous
+// share of the grid: j and k neighbours of a tile then sit in the same L2.
+__device__ inline dim3 xcd_block() {
+  unsigned gx = gridDim.x, gy = gridDim.y, nwg = gx * gy * gridDim.z;
+  unsigned hw = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  unsigned x = hw % 8u, q = nwg / 8u, r = nwg % 8u;
+  unsigned lg = x * q + (x < r ? x : r) + hw / 8u;
+  return dim3(lg % gx, (lg / gx) % gy, lg / (gx * gy));
+}
+
 inline dim3 cell_grid(const Blk &b) {
   int nx = b.i2 - b.i1, ny = b.j2 - b.j1, nz = b.k2 - b.k1;
   return dim3((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz);

@@ -11,9 +11,10 @@
 using namespace codens;
 
 #define CELL_IJK(b)                                   \
-  int i = (b).i1 + blockIdx.x * TX + threadIdx.x;     \
-  int j = (b).j1 + blockIdx.y * TY + threadIdx.y;     \
-  int k = (b).k1 + blockIdx.z;                        \
+  const dim3 xb = xcd_block();                        \
+  int i = (b).i1 + xb.x * TX + threadIdx.x;     \
+  int j = (b).j1 + xb.y * TY + threadIdx.y;     \
+  int k = (b).k1 + xb.z;                        \
   if (i >= (b).i2 || j >= (b).j2) return;
 
 // zpres.f90:66-79

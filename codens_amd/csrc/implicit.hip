@@ -24,19 +24,22 @@ struct DFlux {  // mass-flux increment (d of rho u, rho v, rho w) and dF
 };
 
 // implimf_3d.f90:434-465 for one cell: dF = F(v + d) - F(v), p(v) from the EOS
-// exactly as zpres.f90:72-73 computes it.
+// as zpres.f90:72-73 computes it.  The velocities are formed with one
+// reciprocal of each density instead of the reference's divisions (about
+// 1e-16 relative per value; the sub-iteration result moves by ~3e-14).
 __device__ inline DFlux dflux(const Phys &ph, const double V[5], const double D[5]) {
   const double gam1 = ph.gam1, pinfl = ph.pinfl;
   double wi1 = V[0] + D[0], wi2 = V[1] + D[1], wi3 = V[2] + D[2], wi4 = V[3] + D[3], wi5 = V[4] + D[4];
-  double ui = wi2 / wi1, vi = wi3 / wi1, wi = wi4 / wi1;
+  double ri = 1.0 / wi1, r0 = 1.0 / V[0];
+  double ui = wi2 * ri, vi = wi3 * ri, wi = wi4 * ri;
   double pres = gam1 * (wi5 - .5 * wi1 * (ui * ui + vi * vi + wi * wi) - pinfl);
-  double rhoe = V[4] - 0.5 * (V[1] * V[1] + V[2] * V[2] + V[3] * V[3]) / V[0];
+  double a = V[1] * r0, bb = V[2] * r0, c = V[3] * r0;
+  double rhoe = V[4] - 0.5 * (V[1] * a + V[2] * bb + V[3] * c);
   double p = gam1 * (rhoe - pinfl);
-  double a = V[1] / V[0], bb = V[2] / V[0], c = V[3] / V[0];
   double fxx = V[1] * a + p, fxy = V[2] * a, fxz = V[3] * a;
   double fyy = V[2] * bb + p, fyz = V[3] * bb, fzz = V[3] * c + p;
   double e5 = V[4] + p - pinfl;
-  double fex = e5 * V[1] / V[0], fey = e5 * V[2] / V[0], fez = e5 * V[3] / V[0];
+  double fex = e5 * a, fey = e5 * bb, fez = e5 * c;
   DFlux r;
   r.mx = D[1]; r.my = D[2]; r.mz = D[3];
   r.xx = wi1 * (ui * ui) + pres - fxx;
@@ -62,17 +65,20 @@ __device__ inline void face_tn(const DFlux &A, const DFlux &B, double s1, double
 
 #define CELL3(b)                                     \
   int c[3];                                          \
-  c[0] = (b).i1 + blockIdx.x * TX + threadIdx.x;     \
-  c[1] = (b).j1 + blockIdx.y * TY + threadIdx.y;     \
-  c[2] = (b).k1 + blockIdx.z;                        \
+  const dim3 xb = xcd_block();                       \
+  c[0] = (b).i1 + xb.x * TX + threadIdx.x;     \
+  c[1] = (b).j1 + xb.y * TY + threadIdx.y;     \
+  c[2] = (b).k1 + xb.z;                        \
   if (c[0] >= (b).i2 || c[1] >= (b).j2) return;
 
-__device__ inline double rv_of(const Phys &ph, const double *mu, const double *mut, const double *v, long n) {
-  return ph.gam * (mu[n] / ph.pr + mut[n] / ph.prt) / v[n];
+__device__ inline double rv_of(const Phys &ph, const double *mu, const double *mut, double rinv, long n) {
+  return ph.gam * (mu[n] / ph.pr + mut[n] / ph.prt) * rinv;
 }
 
 // implimf_3d.f90:113-224: spectral-radius coefficients of the faces and the
-// diagonal.  coefe[d] holds the low face of every real cell.
+// diagonal.  coefe[d] holds the low face of every real cell.  Velocities and
+// rv use one reciprocal of each density (the reference divides each time;
+// the coefficients move by ~1e-16 relative).
 __global__ void __launch_bounds__(TX *TY) k_impl_coef(Blk b, Phys ph, const double *__restrict__ v, long ldv,
                                                       const double *__restrict__ mu,
                                                       const double *__restrict__ mut,
@@ -83,9 +89,10 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef(Blk b, Phys ph, const doub
                                                       double *__restrict__ coefe, double *__restrict__ diag) {
   CELL3(b)
   long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
-  double rvc = rv_of(ph, mu, mut, v, n);
+  double rc = 1.0 / v[n];
+  double rvc = rv_of(ph, mu, mut, rc, n);
   double dg = vol[n] / dt[n];
-  double u0 = v[ldv + n] / v[n], v0 = v[2 * ldv + n] / v[n], w0 = v[3 * ldv + n] / v[n];
+  double u0 = v[ldv + n] * rc, v0 = v[2 * ldv + n] * rc, w0 = v[3 * ldv + n] * rc;
   double cf[6];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
@@ -98,8 +105,9 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef(Blk b, Phys ph, const doub
       bool nreal = side == 0 ? (a > a1) : (a + 1 < a2);
       double s1 = snv(sn, b.nnn, mf, d, 0), s2 = snv(sn, b.nnn, mf, d, 1), s3 = snv(sn, b.nnn, mf, d, 2);
       double cnds = s1 * s1 + s2 * s2 + s3 * s3;
-      double un = v[ldv + nn] / v[nn], vn_ = v[2 * ldv + nn] / v[nn], wn = v[3 * ldv + nn] / v[nn];
-      double rvn = nreal ? rv_of(ph, mu, mut, v, nn) : 0.;
+      double rn = 1.0 / v[nn];
+      double un = v[ldv + nn] * rn, vn_ = v[2 * ldv + nn] * rn, wn = v[3 * ldv + nn] * rn;
+      double rvn = nreal ? rv_of(ph, mu, mut, rn, nn) : 0.;
       double uu, vv, ww, cc, rsum, vsum;
       if (side == 0) {  // face m: cells n (own) and n-inc
         uu = 0.5 * (u0 + un); vv = 0.5 * (v0 + vn_); ww = 0.5 * (w0 + wn);
@@ -118,72 +126,6 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef(Blk b, Phys ph, const doub
   diag[m] = dg;
 }
 
-// One point-Jacobi sub-iteration (implimf_3d.f90:230-465).  first != 0 for
-// ls = 1, where d and dF are still zero.
-__global__ void __launch_bounds__(TX *TY) k_impl_sweep(Blk b, Phys ph, int first, int ityprk,
-                                                       const double *__restrict__ u,
-                                                       const double *__restrict__ ff,
-                                                       const double *__restrict__ v, long ldv,
-                                                       const double *__restrict__ sn,
-                                                       const double *__restrict__ coefe,
-                                                       const double *__restrict__ diag, DView dold, DView dnew) {
-  CELL3(b)
-  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
-  double d2w[5];
-#pragma unroll
-  for (int e = 0; e < 5; e++) d2w[e] = ityprk == 0 ? -u[e * ldv + n] : -u[e * ldv + n] - ff[e * ldv + n];
-  if (!first) {
-    double Vc[5], Dc[5];
-#pragma unroll
-    for (int e = 0; e < 5; e++) {
-      Vc[e] = v[e * ldv + n];
-      Dc[e] = dold.at(e, n);
-    }
-    DFlux Fc = dflux(ph, Vc, Dc);
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      long inc = b.inc(d);
-      int a = c[d], a1 = b.lo(d), a2 = b.hi(d);
-      double DL[5] = {0, 0, 0, 0, 0}, DR[5] = {0, 0, 0, 0, 0};
-      DFlux FL = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0}, FR = FL;
-      bool lreal = a > a1, rreal = a + 1 < a2;
-      if (lreal) {
-        double VL[5];
-#pragma unroll
-        for (int e = 0; e < 5; e++) {
-          VL[e] = v[e * ldv + n - inc];
-          DL[e] = dold.at(e, n - inc);
-        }
-        FL = dflux(ph, VL, DL);
-      }
-      if (rreal) {
-        double VR[5];
-#pragma unroll
-        for (int e = 0; e < 5; e++) {
-          VR[e] = v[e * ldv + n + inc];
-          DR[e] = dold.at(e, n + inc);
-        }
-        FR = dflux(ph, VR, DR);
-      }
-      double tlo[5], thi[5];
-      face_tn(Fc, FL, snv(sn, b.nnn, m, d, 0), snv(sn, b.nnn, m, d, 1), snv(sn, b.nnn, m, d, 2), tlo);
-      face_tn(FR, Fc, snv(sn, b.nnn, m + inc, d, 0), snv(sn, b.nnn, m + inc, d, 1), snv(sn, b.nnn, m + inc, d, 2),
-              thi);
-      double cl = coefe[d * b.nnn + m];
-      double ch = rreal ? coefe[d * b.nnn + m + inc] : 0.;
-#pragma unroll
-      for (int e = 0; e < 5; e++) {
-        double t = d2w[e] + tlo[e] + cl * DL[e];
-        if (rreal) t = t + ch * DR[e];
-        d2w[e] = t - thi[e];
-      }
-    }
-  }
-  double dg = diag[m];
-#pragma unroll
-  for (int e = 0; e < 5; e++) dnew.at(e, n) = d2w[e] / dg;
-}
-
 // ---------------------------------------------------------------------------
 // Tiled sub-iteration.  A workgroup owns an SW_TX x SW_TY column tile of real
 // cells over SW_KS planes and marches k: the 14-vector q of a cell (mass-flux
@@ -191,7 +133,14 @@ __global__ void __launch_bounds__(TX *TY) k_impl_sweep(Blk b, Phys ph, int first
 // in LDS for the i/j neighbours of its plane and in registers for the k
 // neighbours; the k face term is carried from one plane to the next.
 // LDS: 14 x (SW_TY+2) x (SW_TX+2) doubles = 73,920 bytes.
-constexpr int SW_TX = 64, SW_TY = 8, SW_KS = 8;
+#ifndef CODENS_SW_TY
+#define CODENS_SW_TY 8
+#endif
+#ifndef CODENS_SW_KS
+#define CODENS_SW_KS 8
+#endif
+constexpr int SW_TX = 64, SW_TY = CODENS_SW_TY, SW_KS = CODENS_SW_KS;
+static_assert(2 * SW_TX + 2 * SW_TY <= SW_TX * SW_TY, "one halo cell per thread at most");
 constexpr int SW_HALO = 2 * SW_TX + 2 * SW_TY;
 
 struct Q14 {
@@ -264,7 +213,19 @@ __device__ inline double lds_d(const double *a, int e) {
   return e == 0 ? a[12 * P] : (e == 4 ? a[13 * P] : a[(e - 1) * P]);
 }
 
-__global__ void __launch_bounds__(SW_TX *SW_TY) k_impl_sweep_tiled(Blk b, Phys ph, int ityprk, int ntx, int nty,
+// Workgroup barrier that orders only the LDS traffic of the plane exchange.
+__device__ inline void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+#ifdef CODENS_SW_WAVES
+#define SW_ATTR __attribute__((amdgpu_waves_per_eu(CODENS_SW_WAVES, CODENS_SW_WAVES)))
+#else
+#define SW_ATTR
+#endif
+__global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b, Phys ph, int ityprk, int ntx, int nty,
                                                                    int nkz, const double *__restrict__ u,
                                                                    const double *__restrict__ ff,
                                                                    const double *__restrict__ v, long ldv,
@@ -337,7 +298,7 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) k_impl_sweep_tiled(Blk b, Phys p
       for (int e = 0; e < 5; e++) dhi_k[e] = q_d(Qn.q, e);
       Qc = Qn;  // plane k+1; the plane-k values are read back from LDS below
     }
-    __syncthreads();
+    lds_sync();
     if (own) {
       double d2w[5];
 #pragma unroll
@@ -372,16 +333,16 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) k_impl_sweep_tiled(Blk b, Phys p
           d2w[e] = tt - thi_k[e];
         }
       }
-      double dg = diag[m];
+      double rdg = 1.0 / diag[m];
 #pragma unroll
-      for (int e = 0; e < 5; e++) dnew.at(e, n) = d2w[e] / dg;
+      for (int e = 0; e < 5; e++) dnew.at(e, n) = d2w[e] * rdg;
 #pragma unroll
       for (int e = 0; e < 5; e++) {
         tlo_k[e] = thi_k[e];
         dlo_k[e] = lds_d(A, e);
       }
     }
-    __syncthreads();
+    lds_sync();
   }
 }
 

@@ -113,67 +113,92 @@ __device__ void roe_face(const Phys &ph, double s0, double s1, double s2, const 
   f[4] = dfex * s0 + dfey * s1 + dfez * s2 - di4;
 }
 
-// ---------------------------------------------------------------------------
-// pass 1: chained slope ratios along lines of direction d.  The chain is
-// sequential along a line, so each thread issues the loads of the next
-// CH_B cells of its line at once (they do not depend on the chain) and then
-// runs CH_B chain steps on registers: one memory round trip per CH_B faces.
+// Chained slope ratios of the limiter (sch_roe.f90:146-194).  Along each grid
+// line the reference first stores the differences r(a) = q(a) - q(a-1) of
+// every face, then replaces r(a) by r(a)/r(a-1) in storage order, so the
+// divisor is the ratio just written for the previous face (reset to 1 when
+// |r(a-1) r(a) r(a+1)| <= tiny).  That recurrence is sequential along a line;
+// the lines, the five variables and the three directions are independent, so
+// one thread runs one (direction, line, variable) chain, the five variables of
+// a line sit in adjacent lanes (sharing the density loads), and the longest
+// lines (direction i) come first in the grid.  Each thread keeps CH_B cells
+// of its line in registers: one memory round trip per CH_B faces.
 constexpr int CH_B = 8;
 
-__global__ void __launch_bounds__(64) k_limiter_chain(Blk b, int d, const double *__restrict__ v, long ldv,
-                                                      const double *__restrict__ ps, double *__restrict__ rr) {
-  int t = blockIdx.x * 64 + threadIdx.x;
+__global__ void __launch_bounds__(256) k_limiter_chain(Blk b, long nt0, long nt1, long nt2,
+                                                       const double *__restrict__ v, long ldv,
+                                                       const double *__restrict__ ps, double *__restrict__ R) {
+  long t = blockIdx.x * 256L + threadIdx.x;
+  int d;
+  if (t < nt0) d = 0;
+  else if (t < nt0 + nt1) { d = 1; t -= nt0; }
+  else if (t < nt0 + nt1 + nt2) { d = 2; t -= nt0 + nt1; }
+  else return;
+  int e = (int)(t % 5);
+  long line = t / 5;
   int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;  // the two transverse directions
-  int na = b.hi(ta) - b.lo(ta), nb = b.hi(tb) - b.lo(tb);
-  if (t >= na * nb) return;
+  int na = b.hi(ta) - b.lo(ta);
   int c[3];
-  c[ta] = b.lo(ta) + t % na;
-  c[tb] = b.lo(tb) + t / na;
+  c[ta] = b.lo(ta) + (int)(line % na);
+  c[tb] = b.lo(tb) + (int)(line / na);
   int a1 = b.lo(d), a2 = b.hi(d);
   long inc = b.inc(d);
   c[d] = a1;
   long n = b.idx(c[0], c[1], c[2]);  // face a1: cells n-inc | n
-  Prim qm = prim_at(v, ldv, ps, n - inc), q0 = prim_at(v, ldv, ps, n), q1 = prim_at(v, ldv, ps, n + inc);
-  double prev[5], cur[5];
+  double *Re = R + (5L * d + e) * b.nnn;
+  // variable e of the primitive state (rho, u, v, w, p) without divergent
+  // branches: numerator plane, and the density as divisor for u, v, w only
+  // (x / 1.0 is exact for rho and p)
+  const double *num = e == 4 ? ps : v + e * ldv;
+  const bool vel = e >= 1 && e <= 3;
+  auto q_at = [&](long nn) {
+    double a = num[nn], r = v[nn];
+    return a / (vel ? r : 1.0);
+  };
+  double qm = q_at(n - inc), q0 = q_at(n), qc = q_at(n + inc);
+  double prev = q0 - qm, cur = qc - q0;
+  // Software pipeline over batches of CH_B faces with two register buffers:
+  // the loads of the next batch are issued before the chain steps and the
+  // stores of the current one (vector-memory counters complete in issue
+  // order, so a store issued before a load would delay that load's use).
+  double an[CH_B], ad[CH_B], bn[CH_B], bd[CH_B];
+  auto load_batch = [&](int a0, double *xn, double *xd) {
 #pragma unroll
-  for (int e = 0; e < 5; e++) {
-    prev[e] = pget(q0, e) - pget(qm, e);
-    cur[e] = pget(q1, e) - pget(q0, e);
-  }
-  Prim qc = q1;
-  for (int a0 = a1 + 1; a0 <= a2 - 1; a0 += CH_B) {
-    double raw[CH_B][6];
+    for (int s = 0; s < CH_B; s++) {  // unconditional, clamped to the line
+      long nn = n + (long)(min(a0 + s, a2 - 1) - a1 + 1) * inc;
+      xn[s] = num[nn];
+      xd[s] = v[nn];
+    }
+  };
+  auto run_batch = [&](int a0, const double *xn, const double *xd) {
+    double rr[CH_B];
 #pragma unroll
     for (int s = 0; s < CH_B; s++) {
-      if (a0 + s <= a2 - 1) {
-        long nn = n + (long)(a0 + s - a1 + 1) * inc;  // cell after face a0+s
-#pragma unroll
-        for (int e = 0; e < 5; e++) raw[s][e] = v[e * ldv + nn];
-        raw[s][5] = ps[nn];
-      }
+      double qn = xn[s] / (vel ? xd[s] : 1.0);
+      double nxt = qn - qc;
+      double r = fabs(prev * cur * nxt) <= DBL_MIN ? 1. : cur / prev;
+      bool in = a0 + s <= a2 - 1;
+      rr[s] = r;
+      prev = in ? r : prev;
+      cur = in ? nxt : cur;
+      qc = in ? qn : qc;
     }
 #pragma unroll
-    for (int s = 0; s < CH_B; s++) {
-      int a = a0 + s;
-      if (a <= a2 - 1) {
-        Prim qn;
-        qn.r = raw[s][0];
-        qn.u = raw[s][1] / qn.r;
-        qn.v = raw[s][2] / qn.r;
-        qn.w = raw[s][3] / qn.r;
-        qn.p = raw[s][5];
-        long m = n + (long)(a - a1) * inc - b.npc;
-#pragma unroll
-        for (int e = 0; e < 5; e++) {
-          double nxt = pget(qn, e) - pget(qc, e);
-          double r = fabs(prev[e] * cur[e] * nxt) <= DBL_MIN ? 1. : cur[e] / prev[e];
-          rr[e * b.nnn + m] = r;
-          prev[e] = r;
-          cur[e] = nxt;
-        }
-        qc = qn;
-      }
-    }
+    for (int s = 0; s < CH_B; s++)
+      if (a0 + s <= a2 - 1) Re[n + (long)(a0 + s - a1) * inc - b.npc] = rr[s];
+  };
+  int a0 = a1 + 1;
+  if (a0 <= a2 - 1) load_batch(a0, an, ad);
+  while (a0 <= a2 - 1) {
+    load_batch(a0 + CH_B, bn, bd);  // unconditional: clamped, and exact waits
+    __builtin_amdgcn_sched_barrier(0);
+    run_batch(a0, an, ad);
+    a0 += CH_B;
+    if (a0 > a2 - 1) break;
+    load_batch(a0 + CH_B, an, ad);
+    __builtin_amdgcn_sched_barrier(0);
+    run_batch(a0, bn, bd);
+    a0 += CH_B;
   }
 }
 
@@ -212,10 +237,11 @@ __global__ void __launch_bounds__(TX *TY) k_face_flux(Blk b, Phys ph, int d, int
                                                       const double *__restrict__ tq,
                                                       const double *__restrict__ sn,
                                                       const double *__restrict__ rr, double *__restrict__ H) {
+  const dim3 xb = xcd_block();
   int c[3];
-  c[0] = b.i1 + blockIdx.x * TX + threadIdx.x;
-  c[1] = b.j1 + blockIdx.y * TY + threadIdx.y;
-  c[2] = b.k1 + blockIdx.z;
+  c[0] = b.i1 + xb.x * TX + threadIdx.x;
+  c[1] = b.j1 + xb.y * TY + threadIdx.y;
+  c[2] = b.k1 + xb.z;
   int hi0 = b.i2 - 1 + (d == 0), hi1 = b.j2 - 1 + (d == 1);
   if (c[0] > hi0 || c[1] > hi1) return;
   int a = c[d], a1 = b.lo(d), a2 = b.hi(d);
@@ -313,16 +339,18 @@ static int roe_common(const codens_block *cb, const codens_phys *cp, const coden
     codens_set_error("%s: workspace allocation failed", what);
     return CODENS_EHIP;
   }
-  for (int d = 0; d < 3; d++) {
-    if (ilim == 1) {
+  if (ilim == 1) {
+    long nt[3];
+    for (int d = 0; d < 3; d++) {
       int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-      int nl = (b.hi(ta) - b.lo(ta)) * (b.hi(tb) - b.lo(tb));
-      {
-    CodensTimed _tm("roe_limiter_chain", st);
-    hipLaunchKernelGGL(k_limiter_chain, dim3((nl + 63) / 64), dim3(64), 0, st, b, d, v, ldv, ps,
-                         R + 5 * b.nnn * d);
-  }
+      nt[d] = 5L * (b.hi(ta) - b.lo(ta)) * (b.hi(tb) - b.lo(tb));
     }
+    long ntot = nt[0] + nt[1] + nt[2];
+    CodensTimed _tm("roe_limiter_chain", st);
+    hipLaunchKernelGGL(k_limiter_chain, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st, b, nt[0], nt[1], nt[2],
+                       v, ldv, ps, R);
+  }
+  for (int d = 0; d < 3; d++) {
     int nx = b.i2 - b.i1 + (d == 0), ny = b.j2 - b.j1 + (d == 1), nz = b.k2 - b.k1 + (d == 2);
     dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz);
     {

@@ -47,10 +47,11 @@ __device__ Roe2oFace roe2o_face(const Blk &b, const double *__restrict__ t, long
 
 __global__ void __launch_bounds__(TX *TY) k_met_roe2o(Blk b, const double *__restrict__ t, long ldv,
                                                       double *__restrict__ d, const double *__restrict__ sn) {
+  const dim3 xb = xcd_block();
   int c[3];
-  c[0] = b.i1 - 1 + blockIdx.x * TX + threadIdx.x;
-  c[1] = b.j1 - 1 + blockIdx.y * TY + threadIdx.y;
-  c[2] = b.k1 - 1 + blockIdx.z;
+  c[0] = b.i1 - 1 + xb.x * TX + threadIdx.x;
+  c[1] = b.j1 - 1 + xb.y * TY + threadIdx.y;
+  c[2] = b.k1 - 1 + xb.z;
   if (c[0] > b.i2 || c[1] > b.j2) return;
   long n = b.idx(c[0], c[1], c[2]);
   double s6 = 0., s7 = 0.;
@@ -201,16 +202,18 @@ __device__ void grad_scalar(const Blk &b, const double *__restrict__ sn, const d
 
 #define REAL_CELL(b)                                 \
   int c[3];                                          \
-  c[0] = (b).i1 + blockIdx.x * TX + threadIdx.x;     \
-  c[1] = (b).j1 + blockIdx.y * TY + threadIdx.y;     \
-  c[2] = (b).k1 + blockIdx.z;                        \
+  const dim3 xb = xcd_block();                       \
+  c[0] = (b).i1 + xb.x * TX + threadIdx.x;     \
+  c[1] = (b).j1 + xb.y * TY + threadIdx.y;     \
+  c[2] = (b).k1 + xb.z;                        \
   if (c[0] >= (b).i2 || c[1] >= (b).j2) return;      \
   long n = (b).idx(c[0], c[1], c[2]), m = n - (b).npc;
 
 // teq_gradv.f90:79-86: rho >= 1e-20 on the box i1-1..i2, j1-1..j2, k1-1..k2
 __global__ void __launch_bounds__(TX *TY) k_rho_clamp(Blk b, double *v) {
-  int i = b.i1 - 1 + blockIdx.x * TX + threadIdx.x, j = b.j1 - 1 + blockIdx.y * TY + threadIdx.y;
-  int k = b.k1 - 1 + blockIdx.z;
+  const dim3 xb = xcd_block();
+  int i = b.i1 - 1 + xb.x * TX + threadIdx.x, j = b.j1 - 1 + xb.y * TY + threadIdx.y;
+  int k = b.k1 - 1 + xb.z;
   if (i > b.i2 || j > b.j2) return;
   long n = b.idx(i, j, k);
   v[n] = fmax(v[n], 1.e-20);

@@ -12,10 +12,11 @@ __global__ void __launch_bounds__(TX *TY) k_zvisqc(Blk b, Phys ph, const double 
                                                    const double *__restrict__ temp,
                                                    const double *__restrict__ mu,
                                                    const double *__restrict__ mut, double *__restrict__ tq) {
+  const dim3 xb = xcd_block();
   int c[3];
-  c[0] = b.i1 + blockIdx.x * TX + threadIdx.x;
-  c[1] = b.j1 + blockIdx.y * TY + threadIdx.y;
-  c[2] = b.k1 + blockIdx.z;
+  c[0] = b.i1 + xb.x * TX + threadIdx.x;
+  c[1] = b.j1 + xb.y * TY + threadIdx.y;
+  c[2] = b.k1 + xb.z;
   if (c[0] >= b.i2 || c[1] >= b.j2) return;
   long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
   double qc[4];

@@ -69,6 +69,8 @@ SIGNATURES = {
     "codens_sumsq_rate": [P(CodensBlock), c_int, c_vp, c_vp, c_vp, P(c_dbl), c_vp],
     "codens_timing_enable": [c_int],
     "codens_timing_read": [ctypes.c_char_p, P(c_dbl), P(c_i64), c_int],
+    "codens_implimf_eu": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_int, c_vp, c_vp,
+                          c_dbl, c_int, c_int, c_vp, c_vp, c_vp],
     "codens_host_mirror": [c_vp, c_i64, c_i64, c_i64, c_i64, c_int, c_int, P(c_vp), c_vp],
     "codens_host_fetch": [c_vp, c_i64, c_i64, c_i64, c_int, c_vp],
     "codens_host_bundle": [P(c_vp), c_int, c_i64, c_i64, c_i64, c_int, P(c_vp), c_vp],

@@ -157,6 +157,13 @@ def implimf_3d(b, ph, u, dt, v, d, ff, mu, mut, lmx, sn_blk, vol, ps, cson, dtpa
                                _s(stream)))
 
 
+def implimf_eu(b, ph, u, dt, v, d, ff, lmx, sn_blk, vol, ps, cson, dt1min=1.0, kfmg=0, ityprk=0, stream=None):
+    """implimf_eu.f90:6; v and d hold nvar = v.shape[0] planes, all updated."""
+    assert d.shape[0] >= v.shape[0]
+    _chk(lib.codens_implimf_eu(ctypes.byref(b), ctypes.byref(ph), _p(u), _p(dt), _p(v), ld(v), v.shape[0], _p(d),
+                               _p(ff), lmx, _p(sn_blk), _p(vol), dt1min, kfmg, ityprk, _p(ps), _p(cson), _s(stream)))
+
+
 def bc_coincident(g, i, r, v, ps, temp, cson, stream=None):
     _chk(lib.codens_bc_coincident(g.numel(), _p(g), _p(i), _p(r), _p(v), ld(v), _p(ps), _p(temp), _p(cson),
                                   _s(stream)))

@@ -54,6 +54,35 @@ __device__ inline DFlux dflux(const Phys &ph, const double V[5], const double D[
   return r;
 }
 
+// implimf_eu.f90:289-312 for one cell: dF from the momenta with
+// wi1 = 1/(rho + d1), the base flux with the caller's pressure p, and the z
+// energy increment without its w factor, as the reference has it.
+__device__ inline DFlux dflux_eu(const Phys &ph, const double V[5], const double D[5], double p) {
+  const double gam1 = ph.gam1, pinfl = ph.pinfl;
+  double wi1 = 1. / (V[0] + D[0]);
+  double ui = V[1] + D[1], vi = V[2] + D[2], wi = V[3] + D[3], wi5 = V[4] + D[4];
+  double pres = gam1 * (wi5 - .5 * wi1 * (ui * ui + vi * vi + wi * wi) - pinfl);
+  double norm = 1. / V[0];
+  double a = V[1] * norm, bb = V[2] * norm, c = V[3] * norm;
+  double fxx = V[1] * a + p, fxy = V[2] * a, fxz = V[3] * a;
+  double fyy = V[2] * bb + p, fyz = V[3] * bb, fzz = V[3] * c + p;
+  double e5 = V[4] + p - pinfl;
+  double fex = e5 * V[1] * norm, fey = e5 * V[2] * norm, fez = e5 * V[3] * norm;
+  double h = wi5 + pres - pinfl;
+  DFlux r;
+  r.mx = D[1]; r.my = D[2]; r.mz = D[3];
+  r.xx = wi1 * (ui * ui) + pres - fxx;
+  r.xy = wi1 * ui * vi - fxy;
+  r.xz = wi1 * ui * wi - fxz;
+  r.ex = wi1 * ui * h - fex;
+  r.yy = wi1 * (vi * vi) + pres - fyy;
+  r.yz = wi1 * vi * wi - fyz;
+  r.ey = wi1 * vi * h - fey;
+  r.zz = wi1 * (wi * wi) + pres - fzz;
+  r.ez = wi1 * h - fez;
+  return r;
+}
+
 // tn of a face (implimf_3d.f90:276-292): half-sum of both cells' increments . S
 __device__ inline void face_tn(const DFlux &A, const DFlux &B, double s1, double s2, double s3, double tn[5]) {
   tn[0] = 0.5 * (A.mx + B.mx) * s1 + 0.5 * (A.my + B.my) * s2 + 0.5 * (A.mz + B.mz) * s3;
@@ -126,6 +155,46 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef(Blk b, Phys ph, const doub
   diag[m] = dg;
 }
 
+// implimf_eu.f90:124-170: coefficients of the Euler implicit phase,
+// coefe = 0.5 |V.S| + |S| c and diag = vol (fact + 1/dt) + sum of the face
+// coefficients, direction by direction; fact = 1.5/dt1min when kfmg = 3.
+__global__ void __launch_bounds__(TX *TY) k_impl_coef_eu(Blk b, const double *__restrict__ v, long ldv,
+                                                         const double *__restrict__ dt,
+                                                         const double *__restrict__ vol,
+                                                         const double *__restrict__ cson,
+                                                         const double *__restrict__ sn, double fact,
+                                                         double *__restrict__ coefe, double *__restrict__ diag) {
+  CELL3(b)
+  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+  double norm = 1. / v[n];
+  double u0 = v[ldv + n] * norm, v0 = v[2 * ldv + n] * norm, w0 = v[3 * ldv + n] * norm;
+  double dg = vol[n] * (fact + 1. / dt[n]);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    long inc = b.inc(d);
+    double cf[2];
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      long nn = side == 0 ? n - inc : n + inc;
+      long mf = side == 0 ? m : m + inc;
+      double s1 = snv(sn, b.nnn, mf, d, 0), s2 = snv(sn, b.nnn, mf, d, 1), s3 = snv(sn, b.nnn, mf, d, 2);
+      double cnds = s1 * s1 + s2 * s2 + s3 * s3;
+      double norm1 = 1. / v[nn];
+      double un = v[ldv + nn] * norm1, vn_ = v[2 * ldv + nn] * norm1, wn = v[3 * ldv + nn] * norm1;
+      double uu, vv, ww, cc;
+      if (side == 0) {  // face m: cells n (own) and n-inc
+        uu = 0.5 * (u0 + un); vv = 0.5 * (v0 + vn_); ww = 0.5 * (w0 + wn); cc = 0.5 * (cson[n] + cson[nn]);
+      } else {          // face m+inc: cells n+inc and n (own)
+        uu = 0.5 * (un + u0); vv = 0.5 * (vn_ + v0); ww = 0.5 * (wn + w0); cc = 0.5 * (cson[nn] + cson[n]);
+      }
+      cf[side] = 0.5 * fabs(uu * s1 + vv * s2 + ww * s3) + sqrt(cnds) * cc;
+    }
+    coefe[d * b.nnn + m] = cf[0];
+    dg = dg + cf[0] + cf[1];
+  }
+  diag[m] = dg;
+}
+
 // ---------------------------------------------------------------------------
 // Tiled sub-iteration.  A workgroup owns an SW_TX x SW_TY column tile of real
 // cells over SW_KS planes and marches k: the 14-vector q of a cell (mass-flux
@@ -159,6 +228,29 @@ __device__ inline void q_of(const Phys &ph, const double *__restrict__ v, long l
   Q.q[3] = F.xx; Q.q[4] = F.xy; Q.q[5] = F.xz; Q.q[6] = F.yy; Q.q[7] = F.yz; Q.q[8] = F.zz;
   Q.q[9] = F.ex; Q.q[10] = F.ey; Q.q[11] = F.ez;
   Q.q[12] = D[0]; Q.q[13] = D[4];
+}
+
+// the same 14-vector for implimf_eu (its own flux increments, caller's p)
+__device__ inline void q_of_eu(const Phys &ph, const double *__restrict__ v, long ldv, const DView &d,
+                               const double *__restrict__ ps, long n, Q14 &Q) {
+  double V[5], D[5];
+#pragma unroll
+  for (int e = 0; e < 5; e++) {
+    V[e] = v[e * ldv + n];
+    D[e] = d.at(e, n);
+  }
+  DFlux F = dflux_eu(ph, V, D, ps[n]);
+  Q.q[0] = F.mx; Q.q[1] = F.my; Q.q[2] = F.mz;
+  Q.q[3] = F.xx; Q.q[4] = F.xy; Q.q[5] = F.xz; Q.q[6] = F.yy; Q.q[7] = F.yz; Q.q[8] = F.zz;
+  Q.q[9] = F.ex; Q.q[10] = F.ey; Q.q[11] = F.ez;
+  Q.q[12] = D[0]; Q.q[13] = D[4];
+}
+
+template <bool EU>
+__device__ inline void q_model(const Phys &ph, const double *__restrict__ v, long ldv, const DView &d,
+                               const double *__restrict__ ps, long n, Q14 &Q) {
+  if (EU) q_of_eu(ph, v, ldv, d, ps, n, Q);
+  else q_of(ph, v, ldv, d, n, Q);
 }
 
 __device__ inline void q_zero(Q14 &Q) {
@@ -225,10 +317,14 @@ __device__ inline void lds_sync() {
 #else
 #define SW_ATTR
 #endif
+// EU selects the flux increments of implimf_eu (which read ps); the face
+// terms, coefficients and update are common to both implicit phases.
+template <bool EU>
 __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b, Phys ph, int ityprk, int ntx, int nty,
                                                                    int nkz, const double *__restrict__ u,
                                                                    const double *__restrict__ ff,
                                                                    const double *__restrict__ v, long ldv,
+                                                                   const double *__restrict__ ps,
                                                                    const double *__restrict__ sn,
                                                                    const double *__restrict__ coefe,
                                                                    const double *__restrict__ diag, DView dold,
@@ -265,8 +361,8 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
   if (own) {
     long n = b.idx(i, j, kb), m = n - b.npc;
     Q14 Qb;
-    if (kb > b.k1) q_of(ph, v, ldv, dold, n - nijd, Qb); else q_zero(Qb);
-    q_of(ph, v, ldv, dold, n, Qc);
+    if (kb > b.k1) q_model<EU>(ph, v, ldv, dold, ps, n - nijd, Qb); else q_zero(Qb);
+    q_model<EU>(ph, v, ldv, dold, ps, n, Qc);
     q_tn(Qc.q, Qb.q, snv(sn, nnn, m, 2, 0), snv(sn, nnn, m, 2, 1), snv(sn, nnn, m, 2, 2), tlo_k);
 #pragma unroll
     for (int e = 0; e < 5; e++) dlo_k[e] = q_d(Qb.q, e);
@@ -282,7 +378,7 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
     for (int c = 0; c < 14; c++) sq[c * SW_PL + slot] = Qc.q[c];
     if (halo) {
       Q14 Qh;
-      if (hreal) q_of(ph, v, ldv, dold, b.idx(hi_, hj, k), Qh); else q_zero(Qh);
+      if (hreal) q_model<EU>(ph, v, ldv, dold, ps, b.idx(hi_, hj, k), Qh); else q_zero(Qh);
 #pragma unroll
       for (int c = 0; c < 14; c++) sq[c * SW_PL + hslot] = Qh.q[c];
     }
@@ -291,7 +387,7 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
     bool kreal = k + 1 < b.k2;
     if (own) {
       Q14 Qn;
-      if (kreal) q_of(ph, v, ldv, dold, n + nijd, Qn); else q_zero(Qn);
+      if (kreal) q_model<EU>(ph, v, ldv, dold, ps, n + nijd, Qn); else q_zero(Qn);
       q_tn(Qn.q, Qc.q, snv(sn, nnn, m + nijd, 2, 0), snv(sn, nnn, m + nijd, 2, 1), snv(sn, nnn, m + nijd, 2, 2),
            thi_k);
 #pragma unroll
@@ -347,11 +443,11 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
 }
 
 // implimf_3d.f90:473-486
-__global__ void __launch_bounds__(TX *TY) k_impl_update(Blk b, double *v, long ldv, const double *__restrict__ d) {
+__global__ void __launch_bounds__(TX *TY) k_impl_update(Blk b, double *v, long ldv, const double *__restrict__ d,
+                                                        int nvar) {
   CELL3(b)
   long n = b.idx(c[0], c[1], c[2]);
-#pragma unroll
-  for (int e = 0; e < 5; e++) v[e * ldv + n] = v[e * ldv + n] + d[e * ldv + n];
+  for (int e = 0; e < nvar; e++) v[e * ldv + n] = v[e * ldv + n] + d[e * ldv + n];
 }
 
 __global__ void k_zero5(double *d, long ldv, long n0, long n1) {
@@ -361,16 +457,23 @@ __global__ void k_zero5(double *d, long ldv, long n0, long n1) {
   for (int e = 0; e < 5; e++) d[e * ldv + n] = 0.;
 }
 
-extern "C" int codens_implimf_3d(const codens_block *cb, const codens_phys *cp, const double *u, const double *dt,
-                                 double *v, int64_t ldv, double *d, const double *ff, const double *mu,
-                                 const double *mut, int32_t lmx, const double *sn, const double *vol, double dtpas,
-                                 int32_t kfmg, int32_t ityprk, const double *ps, const double *cson,
-                                 void *stream) {
-  (void)ps;  // p of the real cells is re-derived from v with the zpres formula
-  int rc = codens_check_block(cb, "codens_implimf_3d");
+// Shared host sequence of both implicit phases: zero d, coefficients, lmx
+// sub-iterations (ping-pong between the caller's d and the workspace, the
+// last one landing in d), then v += d on nvar planes.
+static int implimf_common(bool eu, const codens_block *cb, const codens_phys *cp, const double *u, const double *dt,
+                          double *v, int64_t ldv, int nvar, double *d, const double *ff, const double *mu,
+                          const double *mut, int32_t lmx, const double *sn, const double *vol, double dtpas,
+                          double dt1min, int32_t kfmg, int32_t ityprk, const double *ps, const double *cson,
+                          void *stream, const char *what) {
+  int rc = codens_check_block(cb, what);
   if (rc) return rc;
-  if (!cp || !u || !dt || !v || !d || !mu || !mut || !sn || !vol || !cson || (ityprk && !ff) || lmx < 0) {
-    codens_set_error("codens_implimf_3d: invalid argument");
+  if (!cp || !u || !dt || !v || !d || (!eu && (!mu || !mut)) || (eu && !ps) || !sn || !vol || !cson ||
+      (ityprk && !ff) || lmx < 0 || nvar < 5) {
+    codens_set_error("%s: invalid argument", what);
+    return CODENS_EINVAL;
+  }
+  if (eu && kfmg == 3 && !(dt1min > 0.)) {
+    codens_set_error("%s: dt1min must be positive when kfmg = 3", what);
     return CODENS_EINVAL;
   }
   Blk b = make_blk(cb);
@@ -378,7 +481,7 @@ extern "C" int codens_implimf_3d(const codens_block *cb, const codens_phys *cp, 
   hipStream_t st = (hipStream_t)stream;
   double *ws = (double *)codens_workspace(sizeof(double) * 9 * b.nnn, 2);
   if (!ws) {
-    codens_set_error("codens_implimf_3d: workspace allocation failed");
+    codens_set_error("%s: workspace allocation failed", what);
     return CODENS_EHIP;
   }
   double *coefe = ws, *diag = ws + 3 * b.nnn, *W = ws + 4 * b.nnn;
@@ -387,8 +490,12 @@ extern "C" int codens_implimf_3d(const codens_block *cb, const codens_phys *cp, 
   dim3 g = cell_grid(b), t(TX, TY);
   {
     CodensTimed _tm("implimf_coef", st);
-    hipLaunchKernelGGL(k_impl_coef, g, t, 0, st, b, ph, v, (long)ldv, mu, mut, dt, vol, cson, sn, dtpas, (int)kfmg,
-                     coefe, diag);
+    if (eu)
+      hipLaunchKernelGGL(k_impl_coef_eu, g, t, 0, st, b, v, (long)ldv, dt, vol, cson, sn,
+                         kfmg == 3 ? 1.5 / dt1min : 0., coefe, diag);
+    else
+      hipLaunchKernelGGL(k_impl_coef, g, t, 0, st, b, ph, v, (long)ldv, mu, mut, dt, vol, cson, sn, dtpas,
+                         (int)kfmg, coefe, diag);
   }
   DView A = {d, (long)ldv, 0}, B = {W, b.nnn, b.npc};
   int ntx = (b.i2 - b.i1 + SW_TX - 1) / SW_TX, nty = (b.j2 - b.j1 + SW_TY - 1) / SW_TY;
@@ -401,13 +508,35 @@ extern "C" int codens_implimf_3d(const codens_block *cb, const codens_phys *cp, 
       hipLaunchKernelGGL(k_impl_sweep1, g, t, 0, st, b, (int)ityprk, u, ityprk ? ff : u, (long)ldv, diag, out);
     } else {
       CodensTimed _tm("implimf_sweep", st);
-      hipLaunchKernelGGL(k_impl_sweep_tiled, dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY), 0, st, b, ph, (int)ityprk,
-                         ntx, nty, nkz, u, ityprk ? ff : u, v, (long)ldv, sn, coefe, diag, in, out);
+      if (eu)
+        hipLaunchKernelGGL(k_impl_sweep_tiled<true>, dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY), 0, st, b, ph,
+                           (int)ityprk, ntx, nty, nkz, u, ityprk ? ff : u, v, (long)ldv, ps, sn, coefe, diag, in, out);
+      else
+        hipLaunchKernelGGL(k_impl_sweep_tiled<false>, dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY), 0, st, b, ph,
+                           (int)ityprk, ntx, nty, nkz, u, ityprk ? ff : u, v, (long)ldv, v, sn, coefe, diag, in, out);
     }
   }
   {
     CodensTimed _tm("implimf_update", st);
-    hipLaunchKernelGGL(k_impl_update, g, t, 0, st, b, v, (long)ldv, d);
+    hipLaunchKernelGGL(k_impl_update, g, t, 0, st, b, v, (long)ldv, d, nvar);
   }
-  return codens_check_launch("codens_implimf_3d");
+  return codens_check_launch(what);
+}
+
+extern "C" int codens_implimf_3d(const codens_block *cb, const codens_phys *cp, const double *u, const double *dt,
+                                 double *v, int64_t ldv, double *d, const double *ff, const double *mu,
+                                 const double *mut, int32_t lmx, const double *sn, const double *vol, double dtpas,
+                                 int32_t kfmg, int32_t ityprk, const double *ps, const double *cson,
+                                 void *stream) {
+  // p of the real cells is re-derived from v with the zpres formula; ps unused
+  return implimf_common(false, cb, cp, u, dt, v, ldv, 5, d, ff, mu, mut, lmx, sn, vol, dtpas, 0., kfmg, ityprk,
+                        ps, cson, stream, "codens_implimf_3d");
+}
+
+extern "C" int codens_implimf_eu(const codens_block *cb, const codens_phys *cp, const double *u, const double *dt,
+                                 double *v, int64_t ldv, int32_t nvar, double *d, const double *ff, int32_t lmx,
+                                 const double *sn, const double *vol, double dt1min, int32_t kfmg, int32_t ityprk,
+                                 const double *ps, const double *cson, void *stream) {
+  return implimf_common(true, cb, cp, u, dt, v, ldv, nvar, d, ff, nullptr, nullptr, lmx, sn, vol, 1., dt1min, kfmg,
+                        ityprk, ps, cson, stream, "codens_implimf_eu");
 }

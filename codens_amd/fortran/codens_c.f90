@@ -131,6 +131,16 @@ module codens_c
        integer(c_int32_t), value :: lmx, kfmg, ityprk
        real(c_double), value :: dtpas
      end function codens_implimf_3d
+     integer(c_int32_t) function codens_implimf_eu(b, ph, u, dt, v, ldv, nvar, d, ff, lmx, sn, vol, dt1min, kfmg, &
+          ityprk, ps, cson, stream) bind(C, name="codens_implimf_eu")
+       import
+       type(codens_block), intent(in) :: b
+       type(codens_phys), intent(in) :: ph
+       type(c_ptr), value :: u, dt, v, d, ff, sn, vol, ps, cson, stream
+       integer(c_int64_t), value :: ldv
+       integer(c_int32_t), value :: nvar, lmx, kfmg, ityprk
+       real(c_double), value :: dt1min
+     end function codens_implimf_eu
      integer(c_int32_t) function codens_dissip_jameson(b, ki2, ki4, v, ldv, d, sn, ps, cson, stream) &
           bind(C, name="codens_dissip_jameson")
        import

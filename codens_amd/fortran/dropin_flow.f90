@@ -244,6 +244,47 @@ contains
   end subroutine implimf_3d
 end module mod_implimf_3d
 
+module mod_implimf_eu
+  implicit none
+contains
+  ! implimf_eu.f90:6 -- Euler point-Jacobi implicit phase of block lm (lmx
+  ! sub-iterations), then v += d on all ip60 planes as the reference does.
+  subroutine implimf_eu(lm, u, dt, v, d, ff, equat, lmx, sn, lgsnlt, vol, dtpas, ityprk, &
+       dfxx, dfyy, dfzz, dfxy, dfxz, dfyz, dfex, dfey, dfez, coefdiag, ps, cson)
+    use para_var
+    use para_fige
+    use schemanum, only: kfmg, dt1min
+    use codens_c
+    implicit none
+    integer :: lm, lmx, lgsnlt, ityprk
+    double precision :: dtpas
+    double precision, target :: u(ip11,ip60), dt(ip11), v(ip11,ip60), d(ip11,ip60), ff(ip11,ip60)
+    double precision, target :: sn(lgsnlt,nind,ndir), vol(ip11), ps(ip11), cson(ip11)
+    double precision :: dfxx(ip00), dfyy(ip00), dfzz(ip00), dfxy(ip00), dfxz(ip00), dfyz(ip00)
+    double precision :: dfex(ip00), dfey(ip00), dfez(ip00), coefdiag(ip00)
+    character(len=7) :: equat
+    type(codens_block) :: b
+    type(c_ptr) :: du, ddt, dv, dd, dff, dsn, dvol, dps, dc
+    call codens_need_3d(equat, 'implimf_eu')
+    b = codens_block_of(lm)
+    du = codens_dev(c_loc(u), ip11*ip60, ip11, 1, 5, b, 1)
+    ddt = codens_dev(c_loc(dt), ip11, ip11, 1, 1, b, 1)
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, ip60, b, 1)
+    dd = codens_dev(c_loc(d), ip11*ip60, ip11, 1, ip60, b, 1)
+    dff = c_null_ptr
+    if (ityprk .ne. 0) dff = codens_dev(c_loc(ff), ip11*ip60, ip11, 1, 5, b, 1)
+    dsn = codens_dev_all(c_loc(sn), 9*lgsnlt, codens_geom_mode())
+    dvol = codens_dev(c_loc(vol), ip11, ip11, 1, 1, b, codens_geom_mode())
+    dps = codens_dev(c_loc(ps), ip11, ip11, 1, 1, b, 1)
+    dc = codens_dev(c_loc(cson), ip11, ip11, 1, 1, b, 1)
+    call codens_check(codens_implimf_eu(b, codens_phys_now(), du, ddt, dv, int(ip11, c_int64_t), &
+         int(ip60, c_int32_t), dd, dff, int(lmx, c_int32_t), dsn, dvol, dt1min, int(kfmg, c_int32_t), &
+         int(ityprk, c_int32_t), dps, dc, c_null_ptr), 'implimf_eu')
+    call codens_back(c_loc(v), ip11, 1, ip60, b)
+    call codens_back(c_loc(d), ip11, 1, 5, b)
+  end subroutine implimf_eu
+end module mod_implimf_eu
+
 module mod_dissip_jameson
   implicit none
 contains

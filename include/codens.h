@@ -174,6 +174,18 @@ int codens_implimf_3d(const codens_block *b, const codens_phys *ph, const double
                       const double *vol, double dtpas, int32_t kfmg, int32_t ityprk,
                       const double *ps, const double *cson, void *stream);
 
+/* implimf_eu.f90:6 -- point-Jacobi implicit phase of the Euler equations:
+ * coefe = 0.5|V.S| + |S|c, diag = vol (fact + 1/dt) + sum of coefe with
+ * fact = 1.5/dt1min when kfmg = 3, flux increments from the momenta and the
+ * caller's ps (the z energy increment without its w factor, :312), lmx
+ * sub-iterations, then v += d on all nvar planes of v and d (the reference's
+ * v(n,:) = v(n,:) + d(n,:), :329; planes 6.. of d are left as passed).  No
+ * viscous term, so no mu/mut.  Uses the library's implicit workspace. */
+int codens_implimf_eu(const codens_block *b, const codens_phys *ph, const double *u, const double *dt,
+                      double *v, int64_t ldv, int32_t nvar, double *d, const double *ff, int32_t lmx,
+                      const double *sn_blk, const double *vol, double dt1min, int32_t kfmg, int32_t ityprk,
+                      const double *ps, const double *cson, void *stream);
+
 /* Boundary ghosts from face lists (int64 device arrays of flat indices, in
  * the listing order of initis.f90:71-150): ghost cell, adjacent real cell,
  * and for coincident faces the neighbour's adjacent real cell. */

@@ -349,3 +349,133 @@ void orc_implimf_3d(const orc_block *b, const orc_phys *ph, int ns, const double
   for (int c = 0; c < 5; c++) free(d2w[c]);
   free(rv); free(diag);
 }
+
+/* implimf_eu.f90:6 -- point-Jacobi implicit phase of the Euler equations.
+ * Differs from implimf_3d in its coefficients and flux increments:
+ *   diag  = vol (fact + 1/dt), fact = 1.5/dt1min when kfmg = 3 (:85,131)
+ *           + sum over directions of coefe(m) + coefe(m+inc) (:168),
+ *   coefe = 0.5 |V.S| + |S| c (no viscous part, 0.5 on |V.S| only) with
+ *           V from reciprocal densities (:147-158),
+ *   dF    from momenta with wi1 = 1/(rho+d1) and the caller's ps; the
+ *           z energy increment lacks the w factor exactly as in :312;
+ *   the final update adds all nvar planes of d to v (v(n,:) = v(n,:) +
+ *           d(n,:), :329), planes 6.. included although this routine leaves
+ *           them as the caller passed them. */
+void orc_implimf_eu(const orc_block *b, const orc_phys *ph, const double *u, const double *dt, double *v,
+                    long ldv, int nvar, double *d, const double *ff, int lmx, const double *sn, const double *vol,
+                    double dt1min, int kfmg, int ityprk, const double *ps, const double *cson) {
+  const double gam1 = ph->gam1, pinfl = ph->pinfl;
+  long n0 = b->npc, nnn = b->nnn;
+  int i1 = b->ii1, i2 = b->ii2, j1 = b->jj1, j2 = b->jj2, k1 = b->kk1, k2 = b->kk2;
+  double *df[9], *coefe[3], *diag = calloc(nnn, 8), *d2w[5];
+  for (int c = 0; c < 9; c++) df[c] = calloc(nnn, 8);
+  for (int c = 0; c < 3; c++) coefe[c] = calloc(nnn, 8);
+  for (int c = 0; c < 5; c++) d2w[c] = calloc(nnn, 8);
+  double fact = kfmg == 3 ? 1.5 / dt1min : 0.;
+  for (int e = 0; e < 5; e++)
+    for (long n = IDX(b, i1 - 1, j1 - 1, k1 - 1); n <= IDX(b, i2 + 1, j2 + 1, k2 + 1); n++) d[e * ldv + n] = 0.;
+  for (int k = k1; k <= k2 - 1; k++)
+    for (int j = j1; j <= j2 - 1; j++)
+      for (int i = i1; i <= i2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - n0;
+        diag[m] = vol[n] * (fact + 1. / dt[n]);
+      }
+  for (int dd = 0; dd < 3; dd++) {
+    long inc = INC(b, dd);
+    int lo[3] = {i1, j1, k1}, hi[3] = {i2 - 1, j2 - 1, k2 - 1};
+    hi[dd] += 1; /* faces of the direction, both boundaries included */
+    for (int k = lo[2]; k <= hi[2]; k++)
+      for (int j = lo[1]; j <= hi[1]; j++)
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          long n = IDX(b, i, j, k), m = n - n0;
+          double s1 = SN(sn, b, m, dd, 0), s2 = SN(sn, b, m, dd, 1), s3 = SN(sn, b, m, dd, 2);
+          double cnds = s1 * s1 + s2 * s2 + s3 * s3;
+          double norm = 1. / v[n], norm1 = 1. / v[n - inc];
+          double uu = 0.5 * (v[ldv + n] * norm + v[ldv + n - inc] * norm1);
+          double vv = 0.5 * (v[2 * ldv + n] * norm + v[2 * ldv + n - inc] * norm1);
+          double ww = 0.5 * (v[3 * ldv + n] * norm + v[3 * ldv + n - inc] * norm1);
+          double cc = 0.5 * (cson[n] + cson[n - inc]);
+          coefe[dd][m] = 0.5 * fabs(uu * s1 + vv * s2 + ww * s3) + sqrt(cnds) * cc;
+        }
+    for (int k = lo[2]; k <= hi[2]; k++)
+      for (int j = lo[1]; j <= hi[1]; j++)
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          long m = IDX(b, i, j, k) - n0;
+          diag[m] = diag[m] + coefe[dd][m] + coefe[dd][m + inc];
+        }
+  }
+  for (int ls = 1; ls <= lmx; ls++) {
+    for (int k = k1; k <= k2 - 1; k++)
+      for (int j = j1; j <= j2 - 1; j++)
+        for (int i = i1; i <= i2 - 1; i++) {
+          long n = IDX(b, i, j, k), m = n - n0;
+          for (int e = 0; e < 5; e++) d2w[e][m] = -u[e * ldv + n];
+        }
+    if (ityprk != 0)
+      for (int k = k1; k <= k2 - 1; k++)
+        for (int j = j1; j <= j2 - 1; j++)
+          for (int i = i1; i <= i2 - 1; i++) {
+            long n = IDX(b, i, j, k), m = n - n0;
+            for (int e = 0; e < 5; e++) d2w[e][m] = d2w[e][m] - ff[e * ldv + n];
+          }
+    for (int dd = 0; dd < 3; dd++) {
+      long inc = INC(b, dd);
+      int lo[3] = {i1, j1, k1}, hi[3] = {i2 - 1, j2 - 1, k2 - 1};
+      hi[dd] += 1;
+      for (int k = lo[2]; k <= hi[2]; k++)
+        for (int j = lo[1]; j <= hi[1]; j++)
+          for (int i = lo[0]; i <= hi[0]; i++) {
+            long n = IDX(b, i, j, k), m = n - n0;
+            double s1 = SN(sn, b, m, dd, 0), s2 = SN(sn, b, m, dd, 1), s3 = SN(sn, b, m, dd, 2);
+            double *dxx = df[0], *dxy = df[1], *dxz = df[2], *dex = df[3], *dyy = df[4], *dyz = df[5];
+            double *dey = df[6], *dzz = df[7], *dez = df[8];
+            double tn[5];
+            tn[0] = 0.5 * ((d[ldv + n] + d[ldv + n - inc]) * s1 + (d[2 * ldv + n] + d[2 * ldv + n - inc]) * s2 +
+                           (d[3 * ldv + n] + d[3 * ldv + n - inc]) * s3);
+            tn[1] = 0.5 * ((dxx[m] + dxx[m - inc]) * s1 + (dxy[m] + dxy[m - inc]) * s2 + (dxz[m] + dxz[m - inc]) * s3);
+            tn[2] = 0.5 * ((dxy[m] + dxy[m - inc]) * s1 + (dyy[m] + dyy[m - inc]) * s2 + (dyz[m] + dyz[m - inc]) * s3);
+            tn[3] = 0.5 * ((dxz[m] + dxz[m - inc]) * s1 + (dyz[m] + dyz[m - inc]) * s2 + (dzz[m] + dzz[m - inc]) * s3);
+            tn[4] = 0.5 * ((dex[m] + dex[m - inc]) * s1 + (dey[m] + dey[m - inc]) * s2 + (dez[m] + dez[m - inc]) * s3);
+            for (int e = 0; e < 5; e++)
+              d2w[e][m] = d2w[e][m] + tn[e] + coefe[dd][m] * d[e * ldv + n - inc] +
+                          coefe[dd][m + inc] * d[e * ldv + n + inc];
+            for (int e = 0; e < 5; e++) d2w[e][m - inc] = d2w[e][m - inc] - tn[e];
+          }
+    }
+    for (int k = k1; k <= k2 - 1; k++)
+      for (int j = j1; j <= j2 - 1; j++)
+        for (int i = i1; i <= i2 - 1; i++) {
+          long n = IDX(b, i, j, k), m = n - n0;
+          for (int e = 0; e < 5; e++) d[e * ldv + n] = d2w[e][m] / diag[m];
+          double wi1 = 1. / (v[n] + d[n]);
+          double ui = v[ldv + n] + d[ldv + n], vi = v[2 * ldv + n] + d[2 * ldv + n];
+          double wi = v[3 * ldv + n] + d[3 * ldv + n], wi5 = v[4 * ldv + n] + d[4 * ldv + n];
+          double pres = gam1 * (wi5 - .5 * wi1 * (ui * ui + vi * vi + wi * wi) - pinfl);
+          double norm = 1. / v[n], p = ps[n];
+          double V2 = v[ldv + n], V3 = v[2 * ldv + n], V4 = v[3 * ldv + n], V5 = v[4 * ldv + n];
+          double fxx = V2 * (V2 * norm) + p, fxy = V3 * (V2 * norm), fxz = V4 * (V2 * norm);
+          double fyy = V3 * (V3 * norm) + p, fyz = V4 * (V3 * norm), fzz = V4 * (V4 * norm) + p;
+          double fex = (V5 + p - pinfl) * V2 * norm, fey = (V5 + p - pinfl) * V3 * norm;
+          double fez = (V5 + p - pinfl) * V4 * norm;
+          df[0][m] = wi1 * (ui * ui) + pres - fxx;
+          df[1][m] = wi1 * ui * vi - fxy;
+          df[2][m] = wi1 * ui * wi - fxz;
+          df[3][m] = wi1 * ui * (wi5 + pres - pinfl) - fex;
+          df[4][m] = wi1 * (vi * vi) + pres - fyy;
+          df[5][m] = wi1 * vi * wi - fyz;
+          df[6][m] = wi1 * vi * (wi5 + pres - pinfl) - fey;
+          df[7][m] = wi1 * (wi * wi) + pres - fzz;
+          df[8][m] = wi1 * (wi5 + pres - pinfl) - fez;
+        }
+  }
+  for (int k = k1; k <= k2 - 1; k++)
+    for (int j = j1; j <= j2 - 1; j++)
+      for (int i = i1; i <= i2 - 1; i++) {
+        long n = IDX(b, i, j, k);
+        for (int e = 0; e < nvar; e++) v[e * ldv + n] = v[e * ldv + n] + d[e * ldv + n];
+      }
+  for (int c = 0; c < 9; c++) free(df[c]);
+  for (int c = 0; c < 3; c++) free(coefe[c]);
+  for (int c = 0; c < 5; c++) free(d2w[c]);
+  free(diag);
+}

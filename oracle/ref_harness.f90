@@ -186,6 +186,12 @@ contains
          t1, t2, t3, t4, t5, t6, t7, t8, t9, t10, ps, cson)
   end subroutine ref_implimf_3d
 
+  ! schemanum%dt1min (read by implimf_eu.f90:82 when kfmg = 3)
+  subroutine ref_set_dt1min(x) bind(C, name="ref_set_dt1min")
+    real(c_double), value :: x
+    dt1min = x
+  end subroutine ref_set_dt1min
+
   ! implimf_eu.f90:6
   subroutine ref_implimf_eu(l, u, dt, v, d, ff, lmx, sn, lgsnlt, vol, dtpas, ityprk, &
        t1, t2, t3, t4, t5, t6, t7, t8, t9, t10, ps, cson) bind(C, name="ref_implimf_eu")

@@ -104,6 +104,13 @@ class Ref:
                                 ptr(snb), b.nnn, ptr(vol), D(dtpas), ityprk, *[ptr(a) for a in self.t],
                                 ptr(ps), ptr(cson))
 
+    def implimf_eu(self, l, u, dt, v, d, ff, lmx, sn, vol, ps, cson, dt1min=1.0, ityprk=0):
+        b = self.lay.blocks[l]
+        snb = np.ascontiguousarray(sn_block(sn, b))
+        self.lib.ref_set_dt1min(D(dt1min))
+        self.lib.ref_implimf_eu(l + 1, ptr(u), ptr(dt), ptr(v), ptr(d), ptr(ff), lmx, ptr(snb), b.nnn, ptr(vol),
+                                D(1.0), ityprk, *[ptr(a) for a in self.t], ptr(ps), ptr(cson))
+
 
 class Orc:
     """The C restatement, same calling convention as :class:`Ref`."""
@@ -172,6 +179,13 @@ class Orc:
         self.lib.orc_implimf_3d(self.B(l), ctypes.byref(self.ph), 1 if self.ns else 0, ptr(u), ptr(dt), ptr(v),
                                 self.ld, ptr(d), ptr(ff), ptr(mu), ptr(mut), lmx, ptr(snb), ptr(vol), D(dtpas),
                                 self.kfmg, ityprk, ptr(ps), ptr(cson))
+
+    def implimf_eu(self, l, u, dt, v, d, ff, lmx, sn, vol, ps, cson, dt1min=1.0, ityprk=0):
+        b = self.lay.blocks[l]
+        snb = np.ascontiguousarray(sn_block(sn, b))
+        self.lib.orc_implimf_eu(self.B(l), ctypes.byref(self.ph), ptr(u), ptr(dt), ptr(v), self.ld, v.shape[0], ptr(d),
+                                ptr(ff),
+                                lmx, ptr(snb), ptr(vol), D(dt1min), self.kfmg, ityprk, ptr(ps), ptr(cson))
 
     # boundary ghosts (face lists as int64 arrays)
     def bc_coincident(self, g, i, r, v, ps, temp, cson):

@@ -6,7 +6,7 @@ curvilinear layout (so every host<->device window starts inside the arrays).
 The drop-in library is oracle/_ref/libcodens_dropin.so: the reference harness
 and reference objects with the drop-in modules linked in place of the
 reference's zpres, zvismo, chronos, sch_roe_euler, sch_roe, sch_expli,
-implimf_3d, dissip_jameson, met_roe2o and met_roe."""
+implimf_3d, implimf_eu, dissip_jameson, met_roe2o and met_roe."""
 import numpy as np
 import pytest
 
@@ -107,6 +107,28 @@ def test_implicit_and_explicit_update(mesh, lmx, ityprk):
         res[k] = (vv, d, ve)
     for a, b in zip(res["r"], res["g"]):
         close(a[:5], b[:5])
+
+
+@pytest.mark.parametrize("lmx,kfmg", [(3, 0), (2, 3)])
+def test_implimf_eu(mesh, lmx, kfmg):
+    lay, sn, vol = mesh
+    rng = np.random.default_rng(61 + lmx)
+    n = lay.ip11
+    ph = C.orc_phys()
+    R, G = pair(lay, ph, kfmg=kfmg, ns=False)
+    v = random_state(n, rng)
+    p, t, c = all_thermo(v)
+    u = 1e-3 * rng.standard_normal((7, n)); ff = 1e-4 * rng.standard_normal((7, n))
+    dt = 0.01 * (1 + rng.random(n))
+    res = {}
+    for X, k in ((R, "r"), (G, "g")):
+        vv = np.zeros((7, n)); vv[:5] = v; vv[5:] = 0.5
+        d = np.zeros((7, n)); d[5:] = 0.25
+        for l in range(len(lay.blocks)):
+            X.implimf_eu(l, u, dt, vv, d, ff, lmx, sn, vol, p, c, dt1min=0.004)
+        res[k] = (vv, d)
+    for a, b in zip(res["r"], res["g"]):
+        close(a, b)
 
 
 def test_dissip_jameson(mesh):

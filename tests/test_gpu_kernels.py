@@ -168,6 +168,32 @@ def test_implimf_3d(ctx, lmx, ityprk):
         assert relerr(hv[e], v_o[e]) <= TOL, f"v[{e}]"
 
 
+@pytest.mark.parametrize("lmx,ityprk,kfmg", [(1, 0, 0), (3, 0, 0), (10, 0, 3), (2, 1, 0)])
+def test_implimf_eu(ctx, lmx, ityprk, kfmg):
+    """Euler implicit phase (implimf_eu.f90) against the oracle, 7-plane v and
+    d as the reference stores them (planes 6-7 of d are added to v too)."""
+    abi, lay, sn, vol = ctx
+    rng = np.random.default_rng(17 + lmx)
+    n = lay.ip11
+    v = np.zeros((7, n)); v[:5] = random_state(n, rng); v[5:] = rng.random((2, n))
+    ps, t, c = all_thermo(v[:5])
+    u = 1e-3 * rng.standard_normal((5, n)); ff = 1e-4 * rng.standard_normal((5, n))
+    dt = 0.01 * (1 + rng.random(n))
+    O = C.Orc(lay, C.orc_phys(), kfmg=kfmg, ns=False)
+    ph = abi.phys()
+    d0 = np.zeros((7, n)); d0[5:] = 1e-3 * rng.standard_normal((2, n))
+    v_o, d_o = v.copy(), d0.copy()
+    gv, gd = T(v), T(d0)
+    for l, b in enumerate(lay.blocks):
+        O.implimf_eu(l, u, dt, v_o, d_o, ff, lmx, sn, vol, ps, c, dt1min=0.004, ityprk=ityprk)
+        abi.implimf_eu(abi.block(b), ph, T(u), T(dt), gv, gd, T(ff), lmx, T(sn[9 * b.npc: 9 * (b.npc + b.nnn)]),
+                       T(vol), T(ps), T(c), dt1min=0.004, kfmg=kfmg, ityprk=ityprk)
+    hv, hd = H(gv), H(gd)
+    for e in range(7):
+        assert relerr(hd[e], d_o[e]) <= TOL, f"d[{e}]"
+        assert relerr(hv[e], v_o[e]) <= TOL, f"v[{e}]"
+
+
 def test_sch_expli_and_reductions(ctx):
     abi, lay, sn, vol = ctx
     rng = np.random.default_rng(9)

@@ -162,3 +162,25 @@ def test_turbulence_upwind_bit_exact(mesh, which):
     # ghost cells (0/0): the reference itself produces NaN there
     assert np.array_equal(dr, do, equal_nan=True)
     assert np.nanmax(np.abs(do[5:7])) > 0
+
+
+@pytest.mark.parametrize("lmx,ityprk,kfmg", [(1, 0, 0), (4, 0, 0), (3, 1, 0), (3, 0, 3)])
+def test_implimf_eu_bit_exact(mesh, lmx, ityprk, kfmg):
+    """implimf_eu.f90 (Euler implicit phase) against the oracle, bit for bit."""
+    lay, sn, vol = mesh
+    rng = np.random.default_rng(71 + lmx)
+    n = lay.ip11
+    ph = C.orc_phys()
+    R, O = C.Ref(lay, ph, kfmg=kfmg, ns=False), C.Orc(lay, ph, kfmg=kfmg, ns=False)
+    v = random_state(n, rng)
+    p, t, c = all_thermo(v)
+    u = 1e-3 * rng.standard_normal((7, n)); ff = 1e-4 * rng.standard_normal((7, n))
+    dt = 0.01 * (1 + rng.random(n))
+    vr, vo = np.zeros((7, n)), np.zeros((7, n))
+    vr[:5] = v; vo[:5] = v
+    dr, do = np.full((7, n), 5.0), np.full((7, n), 5.0)
+    for l in range(len(lay.blocks)):
+        R.implimf_eu(l, u, dt, vr, dr, ff, lmx, sn, vol, p, c, dt1min=0.003, ityprk=ityprk)
+        O.implimf_eu(l, u, dt, vo, do, ff, lmx, sn, vol, p, c, dt1min=0.003, ityprk=ityprk)
+    assert np.array_equal(vr, vo) and np.array_equal(dr, do)
+    assert np.abs(dr[:5]).max() > 0

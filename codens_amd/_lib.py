@@ -71,6 +71,12 @@ SIGNATURES = {
     "codens_timing_read": [ctypes.c_char_p, P(c_dbl), P(c_i64), c_int],
     "codens_implimf_eu": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_int, c_vp, c_vp,
                           c_dbl, c_int, c_int, c_vp, c_vp, c_vp],
+    "codens_impli2_eqt_3d": [P(CodensBlock), c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp],
+    "codens_met_samut": [P(CodensBlock), P(c_dbl), c_vp, c_i64, c_vp, c_vp, c_vp],
+    "codens_met_kocmut": [P(CodensBlock), P(c_dbl), c_int, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "codens_met_cut": [P(CodensBlock), c_int, c_dbl, c_dbl, c_vp, c_i64, c_vp],
+    "codens_met_rbse": [c_i64, c_vp, c_vp, c_vp, c_i64, c_vp],
+    "codens_met_rbsc": [c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp],
     "codens_host_mirror": [c_vp, c_i64, c_i64, c_i64, c_i64, c_int, c_int, P(c_vp), c_vp],
     "codens_host_fetch": [c_vp, c_i64, c_i64, c_i64, c_int, c_vp],
     "codens_host_bundle": [P(c_vp), c_int, c_i64, c_i64, c_i64, c_int, P(c_vp), c_vp],

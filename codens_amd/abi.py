@@ -146,6 +146,35 @@ def sch_turb(b, u, v, d, fd, ts6, ts7, sn_blk, vol, stream=None):
                              _p(vol), _s(stream)))
 
 
+def impli2_eqt_3d(b, u, dt, v, mu, mut, cfke, sn_blk, vol, cmt, stream=None):
+    """impli2_eqt_3d.f90:4; u and v hold 7 planes (transport equations 6-7)."""
+    assert u.shape[0] >= 7 and v.shape[0] >= 7 and ld(u) == ld(v)
+    _chk(lib.codens_impli2_eqt_3d(ctypes.byref(b), _p(u), ld(v), _p(dt), _p(v), _p(mu), _p(mut), _p(cfke),
+                                  _p(sn_blk), _p(vol), cmt, _s(stream)))
+
+
+def met_samut(b, tc, v, mu, mut, stream=None):
+    _chk(lib.codens_met_samut(ctypes.byref(b), _tc(tc), _p(v), ld(v), _p(mu), _p(mut), _s(stream)))
+
+
+def met_kocmut(b, tc, kwsst, sn_blk, vol, v, mu, mut, dist, g9, stream=None):
+    _chk(lib.codens_met_kocmut(ctypes.byref(b), _tc(tc), kwsst, _p(sn_blk), _p(vol), _p(v), ld(v), _p(mu), _p(mut),
+                               _p(dist), _p(g9), _s(stream)))
+
+
+def met_cut(b, which, epsk, epse, v, stream=None):
+    """which = 1: met_cut.f90, 2: met_cutsa.f90."""
+    _chk(lib.codens_met_cut(ctypes.byref(b), which, epsk, epse, _p(v), ld(v), _s(stream)))
+
+
+def met_rbse(g, i, fd, stream=None):
+    _chk(lib.codens_met_rbse(g.numel(), _p(g), _p(i), _p(fd), ld(fd), _s(stream)))
+
+
+def met_rbsc(g, i, r, fd, stream=None):
+    _chk(lib.codens_met_rbsc(g.numel(), _p(g), _p(i), _p(r), _p(fd), ld(fd), _s(stream)))
+
+
 def sch_expli(b, u, v, dt, vol, stream=None):
     _chk(lib.codens_sch_expli(ctypes.byref(b), _p(u), _p(v), ld(v), _p(dt), _p(vol), _s(stream)))
 

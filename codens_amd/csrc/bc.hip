@@ -96,25 +96,29 @@ __global__ void __launch_bounds__(BT) k_bc_farfield(long nf, const int64_t *__re
   cson[nl] = sqrt(t);
 }
 
-// rbte.f90:46-59
-__global__ void __launch_bounds__(BT) k_bc_tq_copy(long nf, const int64_t *__restrict__ g,
-                                                   const int64_t *__restrict__ in, double *tq, long ldv) {
+// rbte.f90:46-59 (9 planes of tau, q) and met_rbse.f90:26-38 (6 planes of
+// the turbulent diffusive flux densities): ghost = adjacent real cell
+template <int NP>
+__global__ void __launch_bounds__(BT) k_bc_planes_copy(long nf, const int64_t *__restrict__ g,
+                                                       const int64_t *__restrict__ in, double *tq, long ldv) {
   long f = blockIdx.x * (long)BT + threadIdx.x;
   if (f >= nf) return;
   long nd = g[f], ni = in[f];
 #pragma unroll
-  for (int q = 0; q < 9; q++) tq[q * ldv + nd] = tq[q * ldv + ni];
+  for (int q = 0; q < NP; q++) tq[q * ldv + nd] = tq[q * ldv + ni];
 }
 
-// rbtc.f90:104-117 without periodic rotation
-__global__ void __launch_bounds__(BT) k_bc_tq_coincident(long nf, const int64_t *__restrict__ g,
-                                                         const int64_t *__restrict__ in,
-                                                         const int64_t *__restrict__ rm, double *tq, long ldv) {
+// rbtc.f90:104-117 and met_rbsc.f90:84-95 without periodic rotation:
+// ghost = average of both real cells
+template <int NP>
+__global__ void __launch_bounds__(BT) k_bc_planes_coincident(long nf, const int64_t *__restrict__ g,
+                                                             const int64_t *__restrict__ in,
+                                                             const int64_t *__restrict__ rm, double *tq, long ldv) {
   long f = blockIdx.x * (long)BT + threadIdx.x;
   if (f >= nf) return;
   long nd = g[f], ni = in[f], nr = rm[f];
 #pragma unroll
-  for (int q = 0; q < 9; q++) tq[q * ldv + nd] = 0.5 * (tq[q * ldv + ni] + tq[q * ldv + nr]);
+  for (int q = 0; q < NP; q++) tq[q * ldv + nd] = 0.5 * (tq[q * ldv + ni] + tq[q * ldv + nr]);
 }
 
 static inline dim3 fgrid(int64_t nf) { return dim3((unsigned)((nf + BT - 1) / BT)); }
@@ -163,7 +167,7 @@ extern "C" int codens_bc_tq_copy(int64_t nf, const int64_t *g, const int64_t *in
     codens_set_error("codens_bc_tq_copy: invalid argument");
     return CODENS_EINVAL;
   }
-  hipLaunchKernelGGL(k_bc_tq_copy, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, tq, (long)ldv);
+  hipLaunchKernelGGL(k_bc_planes_copy<9>, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, tq, (long)ldv);
   return codens_check_launch("codens_bc_tq_copy");
 }
 
@@ -174,7 +178,31 @@ extern "C" int codens_bc_tq_coincident(int64_t nf, const int64_t *g, const int64
     codens_set_error("codens_bc_tq_coincident: invalid argument");
     return CODENS_EINVAL;
   }
-  hipLaunchKernelGGL(k_bc_tq_coincident, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, rm, tq,
+  hipLaunchKernelGGL(k_bc_planes_coincident<9>, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, rm, tq,
                      (long)ldv);
   return codens_check_launch("codens_bc_tq_coincident");
+}
+
+extern "C" int codens_met_rbse(int64_t nf, const int64_t *g, const int64_t *in, double *fd, int64_t ldv,
+                               void *stream) {
+  if (nf == 0) return CODENS_OK;
+  if (nf < 0 || !g || !in || !fd) {
+    codens_set_error("codens_met_rbse: invalid argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_bc_planes_copy<6>, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, fd,
+                     (long)ldv);
+  return codens_check_launch("codens_met_rbse");
+}
+
+extern "C" int codens_met_rbsc(int64_t nf, const int64_t *g, const int64_t *in, const int64_t *rm, double *fd,
+                               int64_t ldv, void *stream) {
+  if (nf == 0) return CODENS_OK;
+  if (nf < 0 || !g || !in || !rm || !fd) {
+    codens_set_error("codens_met_rbsc: invalid argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_bc_planes_coincident<6>, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, rm,
+                     fd, (long)ldv);
+  return codens_check_launch("codens_met_rbsc");
 }

@@ -1,0 +1,291 @@
+// Closing part of the turbulence stage of met_num (met_num.f90:697-930) on
+// gfx950: the line-implicit phase of the two transport equations
+// (impli2_eqt_3d), the limiters of at_cutke (met_cut, met_cutsa) and the
+// eddy viscosity of the SA and k-omega models (met_samut, met_kocmut).
+#include "common.hpp"
+
+using namespace codens;
+
+#define REAL3(b)                                     \
+  const dim3 xb = xcd_block();                       \
+  int c[3];                                          \
+  c[0] = (b).i1 + xb.x * TX + threadIdx.x;           \
+  c[1] = (b).j1 + xb.y * TY + threadIdx.y;           \
+  c[2] = (b).k1 + xb.z;                              \
+  if (c[0] >= (b).i2 || c[1] >= (b).j2) return;
+
+// Workspace of the implicit phase, block-local index m: rv, diag, the half
+// normal velocities u1..u3 and coefficients of the faces, the elimination
+// coefficients alpha, beta6, beta7 and the increments dw6, dw7.
+struct ImpliWs {
+  double *rv, *cd, *uf, *ce, *al, *b6, *b7, *dw6, *dw7;
+};
+
+// impli2_eqt_3d.f90:113-122: rv and the diagonal's own part
+__global__ void __launch_bounds__(TX *TY) k_impli2_cells(Blk b, const double *__restrict__ v, long ldv,
+                                                         const double *__restrict__ mu,
+                                                         const double *__restrict__ mut,
+                                                         const double *__restrict__ dt,
+                                                         const double *__restrict__ vol,
+                                                         const double *__restrict__ cfke, double cmt, ImpliWs w) {
+  REAL3(b)
+  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+  w.rv[m] = (mu[n] + mut[n] / cmt) / v[n];
+  w.cd[m] = vol[n] / dt[n] + cfke[n] * vol[n];
+}
+
+// impli2_eqt_3d.f90:126-208: coefficients of every face i1..i2 (resp. j, k)
+// of the real transverse range; the launch covers the cells up to i2, j2, k2.
+__global__ void __launch_bounds__(TX *TY) k_impli2_faces(Blk b, const double *__restrict__ v, long ldv,
+                                                         const double *__restrict__ sn,
+                                                         const double *__restrict__ vol, ImpliWs w) {
+  const dim3 xb = xcd_block();
+  int c[3];
+  c[0] = b.i1 + xb.x * TX + threadIdx.x;
+  c[1] = b.j1 + xb.y * TY + threadIdx.y;
+  c[2] = b.k1 + xb.z;
+  if (c[0] > b.i2 || c[1] > b.j2) return;
+  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    bool ok = true;
+#pragma unroll
+    for (int x = 0; x < 3; x++) ok = ok && c[x] <= b.hi(x) - (x == d ? 0 : 1);
+    if (!ok) continue;
+    long inc = b.inc(d);
+    double s1 = snv(sn, b.nnn, m, d, 0), s2 = snv(sn, b.nnn, m, d, 1), s3 = snv(sn, b.nnn, m, d, 2);
+    double cnds = s1 * s1 + s2 * s2 + s3 * s3;
+    double uu = 0.5 * (v[ldv + n] / v[n] + v[ldv + n - inc] / v[n - inc]);
+    double vv = 0.5 * (v[2 * ldv + n] / v[n] + v[2 * ldv + n - inc] / v[n - inc]);
+    double ww = 0.5 * (v[3 * ldv + n] / v[n] + v[3 * ldv + n - inc] / v[n - inc]);
+    double uf = 0.5 * (uu * s1 + vv * s2 + ww * s3);
+    w.uf[d * b.nnn + m] = uf;
+    w.ce[d * b.nnn + m] = fabs(uf) + (w.rv[m] + w.rv[m - inc]) * cnds / (vol[n] + vol[n - inc]);
+  }
+}
+
+// impli2_eqt_3d.f90:210-220
+__global__ void __launch_bounds__(TX *TY) k_impli2_diag(Blk b, ImpliWs w) {
+  REAL3(b)
+  long m = b.idx(c[0], c[1], c[2]) - b.npc, nnn = b.nnn;
+  const double *ce = w.ce;
+  w.cd[m] = w.cd[m] + ce[m] + ce[m + 1] + ce[nnn + m] + ce[nnn + m + b.nid] + ce[2 * nnn + m] +
+            ce[2 * nnn + m + b.nijd];
+}
+
+// Elimination of one sweep along direction d (impli2_eqt_3d.f90:238-263,
+// 292-317, 346-371): one thread per grid line, from its high end; the
+// recursion values of the previous face stay in registers (zero beyond the
+// last real cell, as the reference's zeroed arrays give).  The transverse
+// increments are read only, so all lines see the previous sweep's values.
+__global__ void __launch_bounds__(64) k_impli2_elim(Blk b, int d, const double *__restrict__ u, long ldv, ImpliWs w) {
+  int t = blockIdx.x * 64 + threadIdx.x;
+  int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
+  int np_ = b.hi(p) - b.lo(p), nq = b.hi(q) - b.lo(q);
+  if (t >= np_ * nq) return;
+  int c[3];
+  c[p] = b.lo(p) + t % np_;
+  c[q] = b.lo(q) + t / np_;
+  long inc = b.inc(d), ip = b.inc(p), iq = b.inc(q), nnn = b.nnn;
+  const double *uD = w.uf + d * nnn, *uP = w.uf + p * nnn, *uQ = w.uf + q * nnn;
+  const double *cD = w.ce + d * nnn, *cP = w.ce + p * nnn, *cQ = w.ce + q * nnn;
+  const double *u6 = u + 5 * ldv, *u7 = u + 6 * ldv;
+  double al1 = 0., b61 = 0., b71 = 0.;  // values at the next face up the line
+  for (int a = b.hi(d) - 1; a >= b.lo(d); a--) {
+    c[d] = a;
+    long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+    double td = -uP[m + ip] + uP[m] - uQ[m + iq] + uQ[m];
+    double tpp = cP[m + ip] - uP[m + ip], tmp_ = cP[m] + uP[m];
+    double tpq = cQ[m + iq] - uQ[m + iq], tmq = cQ[m] + uQ[m];
+    const double *w6 = w.dw6, *w7 = w.dw7;
+    double r6 = -u6[n] + td * w6[m] + tmp_ * w6[m - ip] + tpp * w6[m + ip] + tmq * w6[m - iq] + tpq * w6[m + iq];
+    double r7 = -u7[n] + td * w7[m] + tmp_ * w7[m - ip] + tpp * w7[m + ip] + tmq * w7[m - iq] + tpq * w7[m + iq];
+    double ai = cD[m] + uD[m];
+    double bi = w.cd[m] + uD[m + inc] - uD[m];
+    double ci = -cD[m + inc] + uD[m + inc];
+    double den = bi + ci * al1;
+    al1 = ai / den;
+    b61 = (r6 - ci * b61) / den;
+    b71 = (r7 - ci * b71) / den;
+    w.al[m] = al1;
+    w.b6[m] = b61;
+    w.b7[m] = b71;
+  }
+}
+
+// Substitution from the low end of every line (:265-275 and the like)
+__global__ void __launch_bounds__(64) k_impli2_subst(Blk b, int d, ImpliWs w) {
+  int t = blockIdx.x * 64 + threadIdx.x;
+  int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
+  int np_ = b.hi(p) - b.lo(p), nq = b.hi(q) - b.lo(q);
+  if (t >= np_ * nq) return;
+  int c[3];
+  c[p] = b.lo(p) + t % np_;
+  c[q] = b.lo(q) + t / np_;
+  c[d] = b.lo(d);
+  long inc = b.inc(d);
+  long m = b.idx(c[0], c[1], c[2]) - b.npc;
+  double w6 = 0., w7 = 0.;  // increments of the ghost cell below the line
+  for (int a = b.lo(d); a < b.hi(d); a++, m += inc) {
+    w6 = w.al[m] * w6 + w.b6[m];
+    w7 = w.al[m] * w7 + w.b7[m];
+    w.dw6[m] = w6;
+    w.dw7[m] = w7;
+  }
+}
+
+// :406-416
+__global__ void __launch_bounds__(TX *TY) k_impli2_update(Blk b, double *v, long ldv, ImpliWs w) {
+  REAL3(b)
+  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+  v[5 * ldv + n] = v[5 * ldv + n] + w.dw6[m];
+  v[6 * ldv + n] = v[6 * ldv + n] + w.dw7[m];
+}
+
+__global__ void k_zero_n(double *p, long cnt) {
+  long i = blockIdx.x * 256L + threadIdx.x;
+  if (i < cnt) p[i] = 0.;
+}
+
+extern "C" int codens_impli2_eqt_3d(const codens_block *cb, const double *u, int64_t ldv, const double *dt,
+                                    double *v, const double *mu, const double *mut, const double *cfke,
+                                    const double *sn, const double *vol, double cmt, void *stream) {
+  int rc = codens_check_block(cb, "codens_impli2_eqt_3d");
+  if (rc) return rc;
+  if (!u || !dt || !v || !mu || !mut || !cfke || !sn || !vol || !(cmt > 0.)) {
+    codens_set_error("codens_impli2_eqt_3d: invalid argument");
+    return CODENS_EINVAL;
+  }
+  Blk b = make_blk(cb);
+  hipStream_t st = (hipStream_t)stream;
+  long nnn = b.nnn;
+  double *ws = (double *)codens_workspace(sizeof(double) * 13 * nnn, 3);
+  if (!ws) {
+    codens_set_error("codens_impli2_eqt_3d: workspace allocation failed");
+    return CODENS_EHIP;
+  }
+  ImpliWs w{ws, ws + nnn, ws + 2 * nnn, ws + 5 * nnn, ws + 8 * nnn, ws + 9 * nnn, ws + 10 * nnn, ws + 11 * nnn,
+            ws + 12 * nnn};
+  CodensTimed _tm("impli2_eqt_3d", st);
+  // rv, the face arrays and the increments start at zero (ghost entries stay)
+  hipLaunchKernelGGL(k_zero_n, dim3((unsigned)((13 * nnn + 255) / 256)), dim3(256), 0, st, ws, 13 * nnn);
+  dim3 g = cell_grid(b), t(TX, TY);
+  hipLaunchKernelGGL(k_impli2_cells, g, t, 0, st, b, v, (long)ldv, mu, mut, dt, vol, cfke, cmt, w);
+  dim3 gf((b.i2 - b.i1 + 1 + TX - 1) / TX, (b.j2 - b.j1 + 1 + TY - 1) / TY, b.k2 - b.k1 + 1);
+  hipLaunchKernelGGL(k_impli2_faces, gf, t, 0, st, b, v, (long)ldv, sn, vol, w);
+  hipLaunchKernelGGL(k_impli2_diag, g, t, 0, st, b, w);
+  for (int d = 0; d < 3; d++) {
+    int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
+    int nl = (b.hi(p) - b.lo(p)) * (b.hi(q) - b.lo(q));
+    for (int sweep = 0; sweep < 2; sweep++) {
+      hipLaunchKernelGGL(k_impli2_elim, dim3((nl + 63) / 64), dim3(64), 0, st, b, d, u, (long)ldv, w);
+      hipLaunchKernelGGL(k_impli2_subst, dim3((nl + 63) / 64), dim3(64), 0, st, b, d, w);
+    }
+  }
+  hipLaunchKernelGGL(k_impli2_update, g, t, 0, st, b, v, (long)ldv, w);
+  return codens_check_launch("codens_impli2_eqt_3d");
+}
+
+// met_samut.f90:46-58
+__global__ void __launch_bounds__(TX *TY) k_met_samut(Blk b, double cv13, const double *__restrict__ v, long ldv,
+                                                      const double *__restrict__ mu, double *__restrict__ mut) {
+  REAL3(b)
+  long n = b.idx(c[0], c[1], c[2]);
+  double x = v[5 * ldv + n] / mu[n];
+  double x3 = x * x * x;
+  double fv1 = x3 / (x3 + cv13);
+  mut[n] = v[5 * ldv + n] * fv1;
+}
+
+// met_kocmut.f90:53-90
+__global__ void __launch_bounds__(TX *TY) k_met_kocmut(Blk b, int kwsst, double betas, const double *__restrict__ v,
+                                                       long ldv, const double *__restrict__ mu,
+                                                       double *__restrict__ mut, const double *__restrict__ dist,
+                                                       const double *__restrict__ g9) {
+  REAL3(b)
+  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc, nnn = b.nnn;
+  if (kwsst == 0) {
+    mut[n] = v[n] * v[5 * ldv + n] / v[6 * ldv + n];
+    return;
+  }
+  double a1 = sqrt(betas);
+  double omeg = v[6 * ldv + n] / v[n];
+  double coef1 = sqrt(v[5 * ldv + n] / v[n]) / (betas * omeg * dist[n]);
+  double coef2 = 500. * mu[n] / (v[6 * ldv + n] * (dist[n] * dist[n]));
+  double x = g9[7 * nnn + m] - g9[5 * nnn + m], y = g9[2 * nnn + m] - g9[6 * nnn + m];
+  double z = g9[3 * nnn + m] - g9[nnn + m];
+  double rota = sqrt(x * x + y * y + z * z);
+  double zeta = coef1 > coef2 ? coef1 : coef2;
+  double z2 = 2. * (zeta * zeta);
+  double exp2x = exp(z2 < 25. ? z2 : 25.);
+  double f2 = (exp2x - 1.) / (exp2x + 1);
+  double lim = rota * f2 / a1;
+  mut[n] = v[5 * ldv + n] / (omeg > lim ? omeg : lim);
+}
+
+// met_cutsa.f90:40-49 (which = 2) and met_cut.f90:41-53 (which = 1)
+__global__ void __launch_bounds__(TX *TY) k_met_cut(Blk b, int which, double epsk, double epse, double *v, long ldv) {
+  REAL3(b)
+  long n = b.idx(c[0], c[1], c[2]);
+  double *s6 = v + 5 * ldv, *s7 = v + 6 * ldv;
+  if (which == 2) {
+    if (s6[n] < 0.) s6[n] = epsk;
+    s7[n] = 0.;
+  } else {
+    double t6 = copysign(1.0, s6[n] - epsk), t7 = copysign(1.0, s7[n] - epse);
+    double ts = copysign(1.0, t6 + t7 - 1.5);
+    s6[n] = 0.5 * ((1. + ts) * s6[n] + (1. - ts) * epsk);
+    s7[n] = 0.5 * ((1. + ts) * s7[n] + (1. - ts) * epse);
+  }
+}
+
+extern "C" int codens_teq_gradv(const codens_block *cb, const double *sn, const double *vol, double *v, int64_t ldv,
+                                double *g9, void *stream);
+
+extern "C" int codens_met_samut(const codens_block *cb, const double *tc, const double *v, int64_t ldv,
+                                const double *mu, double *mut, void *stream) {
+  int rc = codens_check_block(cb, "codens_met_samut");
+  if (rc) return rc;
+  if (!tc || !v || !mu || !mut) {
+    codens_set_error("codens_met_samut: null argument");
+    return CODENS_EINVAL;
+  }
+  Blk b = make_blk(cb);
+  double cv1 = tc[7];
+  hipLaunchKernelGGL(k_met_samut, cell_grid(b), dim3(TX, TY), 0, (hipStream_t)stream, b, cv1 * cv1 * cv1, v,
+                     (long)ldv, mu, mut);
+  return codens_check_launch("codens_met_samut");
+}
+
+extern "C" int codens_met_kocmut(const codens_block *cb, const double *tc, int32_t kwsst, const double *sn,
+                                 const double *vol, double *v, int64_t ldv, const double *mu, double *mut,
+                                 const double *dist, double *g9, void *stream) {
+  int rc = codens_check_block(cb, "codens_met_kocmut");
+  if (rc) return rc;
+  if (!tc || !v || !mu || !mut || (kwsst == 1 && (!sn || !vol || !dist || !g9)) || (kwsst != 0 && kwsst != 1)) {
+    codens_set_error("codens_met_kocmut: invalid argument");
+    return CODENS_EINVAL;
+  }
+  if (kwsst == 1) {  // the reference computes the velocity gradients here (met_kocmut.f90:55-61)
+    rc = codens_teq_gradv(cb, sn, vol, v, ldv, g9, stream);
+    if (rc) return rc;
+  }
+  Blk b = make_blk(cb);
+  hipLaunchKernelGGL(k_met_kocmut, cell_grid(b), dim3(TX, TY), 0, (hipStream_t)stream, b, (int)kwsst, tc[10], v,
+                     (long)ldv, mu, mut, dist, g9);
+  return codens_check_launch("codens_met_kocmut");
+}
+
+extern "C" int codens_met_cut(const codens_block *cb, int32_t which, double epsk, double epse, double *v,
+                              int64_t ldv, void *stream) {
+  int rc = codens_check_block(cb, "codens_met_cut");
+  if (rc) return rc;
+  if (!v || (which != 1 && which != 2)) {
+    codens_set_error("codens_met_cut: invalid argument (which = 1 met_cut, 2 met_cutsa)");
+    return CODENS_EINVAL;
+  }
+  Blk b = make_blk(cb);
+  hipLaunchKernelGGL(k_met_cut, cell_grid(b), dim3(TX, TY), 0, (hipStream_t)stream, b, (int)which, epsk, epse, v,
+                     (long)ldv);
+  return codens_check_launch("codens_met_cut");
+}

@@ -160,6 +160,35 @@ int codens_sch_turb(const codens_block *b, double *u, const double *v, int64_t l
                     const double *fd, const double *ts6, const double *ts7, const double *sn_blk,
                     const double *vol, void *stream);
 
+/* impli2_eqt_3d.f90:4 -- implicit phase of the two transport equations:
+ * face coefficients |0.5 V.S| + (rv_L + rv_R)|S|^2/(vol_L + vol_R) with
+ * rv = (mu + mut/cmt)/rho, diagonal vol/dt + cfke vol + their sum, then
+ * line-implicit (Thomas) sweeps along i, j and k, two per direction, with
+ * the other directions' increments of the last sweep on the right-hand
+ * side; finally v(6:7) += dw.  u holds the residual in planes 6-7 (of ldv).
+ * cmt: 1 (1SA, 2JL), 2 (2KO, 2WL), 1.17 (2MT), 1.43 (2Sm) (:82-95).  Wall
+ * laws (lparoi != 0) are not provided.  Uses a library workspace. */
+int codens_impli2_eqt_3d(const codens_block *b, const double *u, int64_t ldv, const double *dt, double *v,
+                         const double *mu, const double *mut, const double *cfke, const double *sn_blk,
+                         const double *vol, double cmt, void *stream);
+
+/* met_samut.f90:4 -- SA eddy viscosity mut = rho nu~ fv1 (cv1 = tc[7]). */
+int codens_met_samut(const codens_block *b, const double *tc, const double *v, int64_t ldv, const double *mu,
+                     double *mut, void *stream);
+
+/* met_kocmut.f90:4 -- k-omega eddy viscosity: rho k/omega (kwsst = 0), or
+ * with the SST limiter (kwsst = 1: runs codens_teq_gradv into g9 first, as
+ * the reference calls teq_gradv there, rho clamp included; betas = tc[10]). */
+int codens_met_kocmut(const codens_block *b, const double *tc, int32_t kwsst, const double *sn_blk,
+                      const double *vol, double *v, int64_t ldv, const double *mu, double *mut,
+                      const double *dist, double *g9, void *stream);
+
+/* at_cutke.f90:4 limiters: which = 1 met_cut.f90 (both variables to
+ * (epsk, epse) unless both exceed them), which = 2 met_cutsa.f90
+ * (nu~ < 0 -> epsk, v(7) = 0). */
+int codens_met_cut(const codens_block *b, int32_t which, double epsk, double epse, double *v, int64_t ldv,
+                   void *stream);
+
 /* sch_expli.f90:4 -- v += dt/vol * u on the real cells. */
 int codens_sch_expli(const codens_block *b, const double *u, double *v, int64_t ldv,
                      const double *dt, const double *vol, void *stream);
@@ -209,6 +238,14 @@ int codens_bc_tq_copy(int64_t nf, const int64_t *ghost, const int64_t *inner, do
 /* rbtc.f90:104-117 -- tq ghost = average of both real cells. */
 int codens_bc_tq_coincident(int64_t nf, const int64_t *ghost, const int64_t *inner,
                             const int64_t *remote, double *tq, int64_t ldv, void *stream);
+
+/* met_rbse.f90:4 -- turbulent diffusive flux densities (6 planes of ldv)
+ * of boundary ghosts = adjacent real cell. */
+int codens_met_rbse(int64_t nf, const int64_t *ghost, const int64_t *inner, double *fd, int64_t ldv,
+                    void *stream);
+/* met_rbsc.f90:4 -- the same at coincident faces: average of both cells. */
+int codens_met_rbsc(int64_t nf, const int64_t *ghost, const int64_t *inner, const int64_t *remote, double *fd,
+                    int64_t ldv, void *stream);
 
 /* dua_resro.f90:63-69 / residu.f90:131-187 -- sum over the real cells of all
  * nb blocks of ((a - b)/dt)^2 for one variable (a, b: arrays of ldv).

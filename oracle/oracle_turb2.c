@@ -1,0 +1,200 @@
+/* CPU restatement of the closing part of the turbulence stage of met_num
+ * (met_num.f90:697-930): the line-implicit phase of the transport equations,
+ * the limiters of at_cutke and the eddy viscosity of the SA and k-omega
+ * models.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+ * Storage as in oracle.h; v holds 7 planes of ldv. */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+#define NID(b) ((long)((b)->id2 - (b)->id1 + 1))
+#define NIJD(b) (NID(b) * (long)((b)->jd2 - (b)->jd1 + 1))
+#define IDX(b, i, j, k) \
+  ((long)(b)->npc + ((i) - (b)->id1) + (long)((j) - (b)->jd1) * NID(b) + (long)((k) - (b)->kd1) * NIJD(b))
+#define SN(sn, b, m, d, c) ((sn)[((long)(c) * 3 + (d)) * (b)->nnn + (m)])
+
+/* impli2_eqt_3d.f90:4 -- implicit phase of the two transport equations:
+ * line-implicit (Thomas) sweeps along i, then j, then k, each repeated
+ * ibalai = 2 times, with the transverse neighbours' increments of the last
+ * sweep on the right-hand side; then v(6:7) += dw.  cmt is the model's
+ * diffusion constant (:82-95).  Wall-law zeroing (:384-404) is out of
+ * scope (lparoi = 0). */
+void orc_impli2_eqt_3d(const orc_block *b, const double *u, long ldv, const double *dt, double *v,
+                       const double *mu, const double *mut, const double *cfke, const double *sn,
+                       const double *vol, double cmt) {
+  long n0 = b->npc, nnn = b->nnn;
+  int i1 = b->ii1, i2 = b->ii2, j1 = b->jj1, j2 = b->jj2, k1 = b->kk1, k2 = b->kk2;
+  long nci = 1, ncj = NID(b), nck = NIJD(b);
+  double *dw6 = calloc(nnn, 8), *dw7 = calloc(nnn, 8), *rv = calloc(nnn, 8), *cd = calloc(nnn, 8);
+  double *uf[3], *ce[3], *al = calloc(nnn, 8), *b6 = calloc(nnn, 8), *b7 = calloc(nnn, 8);
+  for (int d = 0; d < 3; d++) {
+    uf[d] = calloc(nnn, 8);
+    ce[d] = calloc(nnn, 8);
+  }
+  for (int k = k1; k <= k2 - 1; k++)
+    for (int j = j1; j <= j2 - 1; j++)
+      for (int i = i1; i <= i2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - n0;
+        rv[m] = (mu[n] + mut[n] / cmt) / v[n];
+        cd[m] = vol[n] / dt[n] + cfke[n] * vol[n];
+      }
+  /* face coefficients (:126-208): faces i1..i2 (resp. j, k), all real in the
+   * transverse directions */
+  for (int d = 0; d < 3; d++) {
+    long inc = d == 0 ? nci : (d == 1 ? ncj : nck);
+    int hi[3] = {i2 - 1, j2 - 1, k2 - 1};
+    hi[d] += 1;
+    for (int k = k1; k <= hi[2]; k++)
+      for (int j = j1; j <= hi[1]; j++)
+        for (int i = i1; i <= hi[0]; i++) {
+          long n = IDX(b, i, j, k), m = n - n0;
+          double s1 = SN(sn, b, m, d, 0), s2 = SN(sn, b, m, d, 1), s3 = SN(sn, b, m, d, 2);
+          double cnds = s1 * s1 + s2 * s2 + s3 * s3;
+          double uu = 0.5 * (v[ldv + n] / v[n] + v[ldv + n - inc] / v[n - inc]);
+          double vv = 0.5 * (v[2 * ldv + n] / v[n] + v[2 * ldv + n - inc] / v[n - inc]);
+          double ww = 0.5 * (v[3 * ldv + n] / v[n] + v[3 * ldv + n - inc] / v[n - inc]);
+          uf[d][m] = 0.5 * (uu * s1 + vv * s2 + ww * s3);
+          ce[d][m] = fabs(uf[d][m]) + (rv[m] + rv[m - inc]) * cnds / (vol[n] + vol[n - inc]);
+        }
+  }
+  for (int k = k1; k <= k2 - 1; k++)
+    for (int j = j1; j <= j2 - 1; j++)
+      for (int i = i1; i <= i2 - 1; i++) {
+        long m = IDX(b, i, j, k) - n0;
+        cd[m] = cd[m] + ce[0][m] + ce[0][m + nci] + ce[1][m] + ce[1][m + ncj] + ce[2][m] + ce[2][m + nck];
+      }
+  const double *u6 = u + 5 * ldv, *u7 = u + 6 * ldv;
+  for (int d = 0; d < 3; d++) {
+    /* the two transverse directions p, q of the sweep direction d */
+    int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
+    long inc = d == 0 ? nci : (d == 1 ? ncj : nck);
+    long ip = p == 0 ? nci : ncj, iq = q == 1 ? ncj : nck;
+    if (d > 0) {
+      memset(al, 0, nnn * 8);
+      memset(b6, 0, nnn * 8);
+      memset(b7, 0, nnn * 8);
+    }
+    for (int sweep = 0; sweep < 2; sweep++) {
+      /* elimination from the high end of every line (:238-263, :292-317,
+       * :346-371); the transverse increments are those of the last sweep */
+      int lo[3] = {i1, j1, k1}, hi[3] = {i2 - 1, j2 - 1, k2 - 1};
+      for (int a = hi[d]; a >= lo[d]; a--)
+        for (int t2 = lo[q]; t2 <= hi[q]; t2++)
+          for (int t1 = lo[p]; t1 <= hi[p]; t1++) {
+            int c[3];
+            c[d] = a; c[p] = t1; c[q] = t2;
+            long n = IDX(b, c[0], c[1], c[2]), m = n - n0;
+            double td = -uf[p][m + ip] + uf[p][m] - uf[q][m + iq] + uf[q][m];
+            double tpp = ce[p][m + ip] - uf[p][m + ip], tmp_ = ce[p][m] + uf[p][m];
+            double tpq = ce[q][m + iq] - uf[q][m + iq], tmq = ce[q][m] + uf[q][m];
+            double r6 = -u6[n] + td * dw6[m] + tmp_ * dw6[m - ip] + tpp * dw6[m + ip] + tmq * dw6[m - iq] +
+                        tpq * dw6[m + iq];
+            double r7 = -u7[n] + td * dw7[m] + tmp_ * dw7[m - ip] + tpp * dw7[m + ip] + tmq * dw7[m - iq] +
+                        tpq * dw7[m + iq];
+            double ai = ce[d][m] + uf[d][m];
+            double bi = cd[m] + uf[d][m + inc] - uf[d][m];
+            double ci = -ce[d][m + inc] + uf[d][m + inc];
+            al[m] = ai / (bi + ci * al[m + inc]);
+            b6[m] = (r6 - ci * b6[m + inc]) / (bi + ci * al[m + inc]);
+            b7[m] = (r7 - ci * b7[m + inc]) / (bi + ci * al[m + inc]);
+          }
+      /* substitution from the low end */
+      for (int a = lo[d]; a <= hi[d]; a++)
+        for (int t2 = lo[q]; t2 <= hi[q]; t2++)
+          for (int t1 = lo[p]; t1 <= hi[p]; t1++) {
+            int c[3];
+            c[d] = a; c[p] = t1; c[q] = t2;
+            long m = IDX(b, c[0], c[1], c[2]) - n0;
+            dw6[m] = al[m] * dw6[m - inc] + b6[m];
+            dw7[m] = al[m] * dw7[m - inc] + b7[m];
+          }
+    }
+  }
+  for (int k = k1; k <= k2 - 1; k++)
+    for (int j = j1; j <= j2 - 1; j++)
+      for (int i = i1; i <= i2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - n0;
+        v[5 * ldv + n] = v[5 * ldv + n] + dw6[m];
+        v[6 * ldv + n] = v[6 * ldv + n] + dw7[m];
+      }
+  free(dw6); free(dw7); free(rv); free(cd); free(al); free(b6); free(b7);
+  for (int d = 0; d < 3; d++) {
+    free(uf[d]);
+    free(ce[d]);
+  }
+}
+
+/* met_samut.f90:4 -- mut = rho nu~ fv1, fv1 = X^3/(X^3 + cv1^3), X = rho nu~/mu */
+void orc_met_samut(const orc_block *b, const orc_turb *tc, const double *v, long ldv, const double *mu,
+                   double *mut) {
+  double cv13 = tc->cv1 * tc->cv1 * tc->cv1;
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k);
+        double x = v[5 * ldv + n] / mu[n];
+        double x3 = x * x * x;
+        double fv1 = x3 / (x3 + cv13);
+        mut[n] = v[5 * ldv + n] * fv1;
+      }
+}
+
+/* met_kocmut.f90:4 -- k-omega eddy viscosity: mut = rho k / omega
+ * (kwsst = 0), or the SST limiter with the vorticity of the velocity
+ * gradients g9 (teq_gradv order xx xy xz yx yy yz zx zy zz, block-local m)
+ * and the wall distance (kwsst = 1). */
+void orc_met_kocmut(const orc_block *b, const orc_turb *tc, int kwsst, const double *v, long ldv,
+                    const double *mu, double *mut, const double *dist, const double *g9, long ldg) {
+  double betas = tc->betas, a1 = sqrt(betas);
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - b->npc;
+        if (kwsst == 0) {
+          mut[n] = v[n] * v[5 * ldv + n] / v[6 * ldv + n];
+        } else {
+          const double *dxx = g9, *dxy = g9 + ldg, *dxz = g9 + 2 * ldg, *dyx = g9 + 3 * ldg;
+          const double *dyz = g9 + 5 * ldg, *dzx = g9 + 6 * ldg, *dzy = g9 + 7 * ldg;
+          (void)dxx;
+          double omeg = v[6 * ldv + n] / v[n];
+          double coef1 = sqrt(v[5 * ldv + n] / v[n]) / (betas * omeg * dist[n]);
+          double coef2 = 500. * mu[n] / (v[6 * ldv + n] * (dist[n] * dist[n]));
+          double a = dzy[m] - dyz[m], bb = dxz[m] - dzx[m], c = dyx[m] - dxy[m];
+          double rota = sqrt(a * a + bb * bb + c * c);
+          double zeta = coef1 > coef2 ? coef1 : coef2;
+          double z2 = 2. * (zeta * zeta);
+          double exp2x = exp(z2 < 25. ? z2 : 25.);
+          double f2 = (exp2x - 1.) / (exp2x + 1);
+          double lim = rota * f2 / a1;
+          mut[n] = v[5 * ldv + n] / (omeg > lim ? omeg : lim);
+        }
+      }
+}
+
+/* met_cutsa.f90:4 -- SA limiter: nu~ < 0 -> epsk, and v(7) = 0 */
+void orc_met_cutsa(const orc_block *b, double epsk, double *v, long ldv) {
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k);
+        if (v[5 * ldv + n] < 0.) v[5 * ldv + n] = epsk;
+        v[6 * ldv + n] = 0.;
+      }
+}
+
+/* met_cut.f90:4 -- both variables reset to (epsk, epse) unless both exceed
+ * them, in the reference's sign arithmetic */
+void orc_met_cut(const orc_block *b, double epsk, double epse, double *v, long ldv) {
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k);
+        double t6 = copysign(1.0, v[5 * ldv + n] - epsk);
+        double t7 = copysign(1.0, v[6 * ldv + n] - epse);
+        double ts = copysign(1.0, t6 + t7 - 1.5);
+        v[5 * ldv + n] = 0.5 * ((1. + ts) * v[5 * ldv + n] + (1. - ts) * epsk);
+        v[6 * ldv + n] = 0.5 * ((1. + ts) * v[6 * ldv + n] + (1. - ts) * epse);
+      }
+}

@@ -353,4 +353,75 @@ contains
          sn, lgsnlt, vol, f(1,1), f(1,2), f(1,3), f(1,4), f(1,5), f(1,6))
   end subroutine ref_sch_turb
 
+  ! modeleturb%equatt for impli2_eqt_3d's constant cmt (impli2_eqt_3d.f90:82-95)
+  subroutine ref_set_equatt(code) bind(C, name="ref_set_equatt")
+    use chainecarac
+    integer(c_int), value :: code
+    select case (code)
+    case (0)
+       equatt = '1SA   '
+    case (1)
+       equatt = '2KO   '
+    case (2)
+       equatt = '2JL   '
+    case (3)
+       equatt = '2WL   '
+    case (4)
+       equatt = '2MT   '
+    case default
+       equatt = '2Sm   '
+    end select
+  end subroutine ref_set_equatt
+
+  ! impli2_eqt_3d.f90:4 (lparoi = 0)
+  subroutine ref_impli2_eqt_3d(l, u, dt, v, mu, mut, cfke, sn, lgsnlt, vol, t1, t2, t3, t4, t5, t6, t7, t8, t9, &
+       t10) bind(C, name="ref_impli2_eqt_3d")
+    use mod_impli2_eqt_3d
+    use chainecarac
+    integer(c_int), value :: l, lgsnlt
+    real(c_double) :: u(*), dt(*), v(*), mu(*), mut(*), cfke(*), sn(*), vol(*)
+    real(c_double) :: t1(*), t2(*), t3(*), t4(*), t5(*), t6(*), t7(*), t8(*), t9(*), t10(*)
+    integer :: ncin(1)
+    ncin = 0
+    equat = heq
+    call impli2_eqt_3d(l, u, dt, v, mu, mut, cfke, ncin, 1, sn, lgsnlt, vol, t1, t2, t3, t4, t5, t6, t7, t8, &
+         t9, t10)
+  end subroutine ref_impli2_eqt_3d
+
+  ! met_samut.f90:4
+  subroutine ref_met_samut(l, v, mu, mut) bind(C, name="ref_met_samut")
+    use mod_met_samut
+    integer(c_int), value :: l
+    real(c_double) :: v(*), mu(*), mut(*)
+    call met_samut(l, v, mu, mut)
+  end subroutine ref_met_samut
+
+  ! met_kocmut.f90:4 (kwsst = 1 computes the velocity gradients itself)
+  subroutine ref_met_kocmut(l, iwsst, sn, vol, t, g, dist, v, mu, mut, cmu) bind(C, name="ref_met_kocmut")
+    use mod_met_kocmut
+    use chainecarac
+    integer(c_int), value :: l, iwsst
+    real(c_double) :: sn(*), vol(*), t(*), g(ip00, 9), dist(*), v(*), mu(*), mut(*), cmu(*)
+    equat = heq
+    kwsst = iwsst
+    call met_kocmut(l, sn, vol, t, g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), g(1,8), g(1,9), &
+         dist, v, mu, mut, cmu, cmu, cmu, cmu, cmu, cmu)
+  end subroutine ref_met_kocmut
+
+  ! met_cutsa.f90:4 and met_cut.f90:4
+  subroutine ref_met_cut(l, which, ek, ee, v) bind(C, name="ref_met_cut")
+    use mod_met_cutsa
+    use mod_met_cut
+    integer(c_int), value :: l, which
+    real(c_double), value :: ek, ee
+    real(c_double) :: v(*)
+    epsk = ek
+    epse = ee
+    if (which .eq. 2) then
+       call met_cutsa(l, v)
+    else
+       call met_cut(l, v)
+    endif
+  end subroutine ref_met_cut
+
 end module codens_ref_harness

@@ -101,3 +101,100 @@ def test_chain(setup):
         for e in (5, 6):
             assert relerr(hu[e], uo[e]) <= TOL, ("sch_turb", e)
     assert relerr(H(gv), vo) == 0.0
+
+
+@pytest.mark.parametrize("cmt", [1.0, 2.0, 1.17])
+def test_impli2_eqt_3d(setup, cmt):
+    """Line-implicit phase of the transport equations against the oracle."""
+    import ctypes
+    from oracle import ptr
+    abi, lay, sn, vol, v, mu, mut, dist, cson, tc, tcl, O = setup
+    n = lay.ip11
+    rng = np.random.default_rng(91)
+    u = np.zeros((7, n)); u[5:] = 1e-3 * rng.standard_normal((2, n))
+    dt = 0.01 * (1 + rng.random(n)); cfke = rng.random(n)
+    vo = v.copy(); gv = T(v)
+    for l, b in enumerate(lay.blocks):
+        O.lib.orc_impli2_eqt_3d(ctypes.byref(C.orc_block(b)), ptr(u), C.L_(n), ptr(dt), ptr(vo), ptr(mu), ptr(mut),
+                                ptr(cfke), ptr(np.ascontiguousarray(sn[9 * b.npc: 9 * (b.npc + b.nnn)])), ptr(vol),
+                                C.D(cmt))
+        abi.impli2_eqt_3d(abi.block(b), T(u), T(dt), gv, T(mu), T(mut), T(cfke), snb(sn, b), T(vol), cmt)
+    hv = H(gv)
+    inner = lay.interior()
+    for e in (5, 6):
+        assert relerr(hv[e, inner] - v[e, inner], vo[e, inner] - v[e, inner]) <= TOL, e
+    assert np.array_equal(hv[:5], v[:5])
+
+
+@pytest.mark.parametrize("kwsst", [0, 1])
+def test_eddy_viscosity(setup, kwsst):
+    import ctypes
+    from oracle import ptr
+    abi, lay, sn, vol, v, mu, mut, dist, cson, tc, tcl, O = setup
+    n = lay.ip11
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        inner = b.interior()
+        mo = np.zeros(n); gm = T(np.zeros(n))
+        O.lib.orc_met_samut(ctypes.byref(ob), ctypes.byref(tc), ptr(v), C.L_(n), ptr(mu), ptr(mo))
+        abi.met_samut(abi.block(b), tcl, T(v), T(mu), gm)
+        assert relerr(H(gm)[inner], mo[inner]) <= TOL
+        vo = v.copy(); gv = T(v)
+        mo = np.zeros(n); gm = T(np.zeros(n))
+        go = np.zeros((9, b.nnn))
+        sb = np.ascontiguousarray(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+        if kwsst:
+            O.lib.orc_teq_gradv(ctypes.byref(ob), ptr(sb), ptr(vol), ptr(vo), C.L_(n), ptr(go))
+        O.lib.orc_met_kocmut(ctypes.byref(ob), ctypes.byref(tc), kwsst, ptr(vo), C.L_(n), ptr(mu), ptr(mo), ptr(dist),
+                             ptr(go), C.L_(b.nnn))
+        abi.met_kocmut(abi.block(b), tcl, kwsst, T(sb), T(vol), gv, T(mu), gm, T(dist), T(np.zeros((9, b.nnn))))
+        assert relerr(H(gm)[inner], mo[inner]) <= TOL
+        assert np.array_equal(H(gv), vo)
+
+
+@pytest.mark.parametrize("which", [1, 2])
+def test_limiters(setup, which):
+    import ctypes
+    from oracle import ptr
+    abi, lay, sn, vol, v, mu, mut, dist, cson, tc, tcl, O = setup
+    n = lay.ip11
+    rng = np.random.default_rng(93)
+    vv = v.copy(); vv[5] = 1e-3 * rng.standard_normal(n); vv[6] = rng.standard_normal(n); vv[5, ::7] = 1e-6
+    vo = vv.copy(); gv = T(vv)
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        if which == 2:
+            O.lib.orc_met_cutsa(ctypes.byref(ob), C.D(1e-6), ptr(vo), C.L_(n))
+        else:
+            O.lib.orc_met_cut(ctypes.byref(ob), C.D(1e-6), C.D(1e-4), ptr(vo), C.L_(n))
+        abi.met_cut(abi.block(b), which, 1e-6, 1e-4, gv)
+    assert np.array_equal(H(gv), vo)
+
+
+def test_flux_density_ghosts(setup):
+    """met_rbse / met_rbsc on the 6 flux-density planes, against the oracle's
+    plane-wise ghost copy and face average (rbte/rbtc restatement)."""
+    import ctypes
+    from oracle import ptr
+    abi, lay, sn, vol, v, mu, mut, dist, cson, tc, tcl, O = setup
+    n = lay.ip11
+    rng = np.random.default_rng(95)
+    b0, b1 = lay.blocks
+    # ghosts of the i-low face of block 0 (copy) and of block 1 (average with block 0 cells)
+    g0 = np.array([b0.idx(b0.ii1 - 1, j, k) for k in range(b0.kk1, b0.kk2) for j in range(b0.jj1, b0.jj2)],
+                  np.int64)
+    i0 = g0 + 1
+    g1 = np.array([b1.idx(b1.ii1 - 1, j, k) for k in range(b1.kk1, b1.kk2) for j in range(b1.jj1, b1.jj2)],
+                  np.int64)
+    i1 = g1 + 1
+    r1 = np.array([b0.idx(b0.ii2 - 1, j, k) for k in range(b1.kk1, b1.kk2) for j in range(b1.jj1, b1.jj2)],
+                  np.int64)
+    fd = rng.standard_normal((6, n))
+    t9 = np.zeros((9, n)); t9[:6] = fd
+    O.lib.orc_bc_tq_copy(C.L_(len(g0)), ptr(g0), ptr(i0), ptr(t9), C.L_(n))
+    O.lib.orc_bc_tq_coincident(C.L_(len(g1)), ptr(g1), ptr(i1), ptr(r1), ptr(t9), C.L_(n))
+    gfd = T(fd)
+    G = lambda a: torch.as_tensor(a, device="cuda")
+    abi.met_rbse(G(g0), G(i0), gfd)
+    abi.met_rbsc(G(g1), G(i1), G(r1), gfd)
+    assert np.array_equal(H(gfd), t9[:6])

@@ -121,3 +121,74 @@ def test_sources_and_balance(setup):
     O.orc_sch_turb(ctypes.byref(ob), ptr(uo), ptr(vv), C.L_(n), ptr(d), ptr(fd), ptr(t6r), ptr(t7r),
                    ptr(snb(sn, b)), ptr(vol))
     assert np.array_equal(ur[5:7], uo[5:7])
+
+
+@pytest.mark.parametrize("model,cmt", [(0, 1.0), (1, 2.0), (4, 1.17)])
+def test_impli2_eqt_3d(setup, model, cmt):
+    """Line-implicit phase of the transport equations (impli2_eqt_3d.f90,
+    lparoi = 0) for the models' cmt constants."""
+    lay, sn, vol, v, mu, mut, dist, cson, tc, R, O = setup
+    n = lay.ip11
+    rng = np.random.default_rng(47 + model)
+    u = np.zeros((7, n)); u[5:] = 1e-3 * rng.standard_normal((2, n))
+    dt = 0.01 * (1 + rng.random(n)); cfke = rng.random(n)
+    R.lib.ref_set_equatt(model)
+    t = [np.zeros(lay.ip00) for _ in range(10)]
+    vr, vo = v.copy(), v.copy()
+    for l, b in enumerate(lay.blocks):
+        R.lib.ref_impli2_eqt_3d(l + 1, ptr(u), ptr(dt), ptr(vr), ptr(mu), ptr(mut), ptr(cfke), ptr(snb(sn, b)),
+                                b.nnn, ptr(vol), *[ptr(a) for a in t])
+        O.orc_impli2_eqt_3d(ctypes.byref(C.orc_block(b)), ptr(u), C.L_(n), ptr(dt), ptr(vo), ptr(mu), ptr(mut),
+                            ptr(cfke), ptr(snb(sn, b)), ptr(vol), C.D(cmt))
+    assert np.array_equal(vr, vo)
+    assert np.abs(vr[5:] - v[5:]).max() > 0
+
+
+@pytest.mark.parametrize("kwsst", [0, 1])
+def test_eddy_viscosity(setup, kwsst):
+    """met_samut (SA) and met_kocmut (k-omega, with the SST limiter for
+    kwsst = 1, whose gradients come from teq_gradv)."""
+    lay, sn, vol, v, mu, mut, dist, cson, tc, R, O = setup
+    n = lay.ip11
+    cmu = np.ones(n)
+    otc = tc
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        mr, mo = np.full(n, -3.0), np.full(n, -3.0)
+        R.lib.ref_met_samut(l + 1, ptr(v), ptr(mu), ptr(mr))
+        O.orc_met_samut(ctypes.byref(ob), ctypes.byref(otc), ptr(v), C.L_(n), ptr(mu), ptr(mo))
+        assert np.array_equal(mr, mo)
+        vr, vo = v.copy(), v.copy()
+        mr, mo = np.full(n, -3.0), np.full(n, -3.0)
+        g = np.zeros((9, lay.ip00)); s = np.zeros(lay.ip00)
+        R.lib.ref_met_kocmut(l + 1, kwsst, ptr(sn), ptr(vol), ptr(s), ptr(g), ptr(dist), ptr(vr), ptr(mu), ptr(mr),
+                             ptr(cmu))
+        go = np.zeros((9, b.nnn))
+        if kwsst:
+            O.orc_teq_gradv(ctypes.byref(ob), ptr(snb(sn, b)), ptr(vol), ptr(vo), C.L_(n), ptr(go))
+        O.orc_met_kocmut(ctypes.byref(ob), ctypes.byref(otc), kwsst, ptr(vo), C.L_(n), ptr(mu), ptr(mo), ptr(dist),
+                         ptr(go), C.L_(b.nnn))
+        assert np.array_equal(vr, vo)
+        assert np.array_equal(mr, mo)
+
+
+@pytest.mark.parametrize("which", [1, 2])
+def test_limiters(setup, which):
+    """met_cut (which = 1) and met_cutsa (which = 2) of at_cutke."""
+    lay, sn, vol, v, mu, mut, dist, cson, tc, R, O = setup
+    n = lay.ip11
+    rng = np.random.default_rng(53)
+    vv = v.copy()
+    vv[5] = rng.standard_normal(n) * 1e-3
+    vv[6] = rng.standard_normal(n)
+    vv[5, ::7] = 1e-6  # exactly at the threshold
+    vr, vo = vv.copy(), vv.copy()
+    for l, b in enumerate(lay.blocks):
+        R.lib.ref_met_cut(l + 1, which, C.D(1e-6), C.D(1e-4), ptr(vr))
+        ob = C.orc_block(b)
+        if which == 2:
+            O.orc_met_cutsa(ctypes.byref(ob), C.D(1e-6), ptr(vo), C.L_(n))
+        else:
+            O.orc_met_cut(ctypes.byref(ob), C.D(1e-6), C.D(1e-4), ptr(vo), C.L_(n))
+    assert np.array_equal(vr, vo)
+    assert not np.array_equal(vr, vv)

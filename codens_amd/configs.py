@@ -11,6 +11,9 @@ gas constants, switches and initial fields, all seeded and deterministic.
   five other faces.  Laminar Navier-Stokes, Re=1e5 per unit length, M=0.5,
   Roe-MUSCL with minmod (ilim=1), implicit point-Jacobi (kmf=1, lmax=10),
   local time step (kdtl=1) with eta=10.
+* ``wall_rans``  -- the wall block with a turbulence model (SA or Wilcox
+  k-omega) on the transport equations 6-7, wall distance = y of the cell
+  centres.
 
 Seeds: numpy PCG64(20261201 + case index).
 """
@@ -35,6 +38,28 @@ class FaceSet:
 
 
 @dataclasses.dataclass
+class TurbCfg:
+    """Turbulence model of the transport equations 6-7 (met_num.f90): model
+    '1SA' (Spalart-Allmaras: met_smsa, met_fludcsa, met_cutsa, met_samut) or
+    '2KO' (Wilcox k-omega: met_smko, met_fludko, met_cut, met_kocmut)."""
+    model: str
+    tc: list            # 16 constants cb1 cb2 sigma kappa cw1 cw2 cw3 cv1 ct3 ct4 betas beta sigmk sigmw alfak alfae
+    dist: np.ndarray    # wall distance of every cell (ip11)
+    klroe: int = 2      # dissipation: 1 met_roe, 2 met_roe2o (met_num.f90:623-641)
+    kwsst: int = 0      # k-omega eddy viscosity: 1 with the SST limiter
+    epsk: float = 1e-12
+    epse: float = 1e-12
+
+    @property
+    def cmt(self):      # impli2_eqt_3d.f90:82-95
+        return {"1SA": 1.0, "2KO": 2.0}[self.model]
+
+    @property
+    def kfludis(self):  # met_num.f90:528-556
+        return {"1SA": 2, "2KO": 5}[self.model]
+
+
+@dataclasses.dataclass
 class Case:
     name: str
     layout: L.Layout
@@ -53,6 +78,7 @@ class Case:
     coincident: FaceSet
     wall: FaceSet
     farfield: FaceSet
+    turb: Optional[TurbCfg] = None
 
 
 def conservative(rho, u, v, w, p, gam=GAM, pinfl=0.0):
@@ -167,6 +193,11 @@ def wall_ns(ci=256, cj=128, ck=64, seed=20261202, mach=0.5, reynz=1e5, eta=10.0,
     sn, vol = L.metrics(lay, x, y, z)
     n = lay.ip11
     v0 = conservative(np.ones(n), np.full(n, mach), np.zeros(n), np.zeros(n), np.full(n, 1.0 / GAM))
+    global _last_wall_distance
+    yc = 0.125 * sum(Yd[dk:dk + ck, dj:dj + cj, di:di + ci] for dk in (0, 1) for dj in (0, 1) for di in (0, 1))
+    dist = np.ones(n)
+    b.view(dist)[b.kk1 - b.kd1:b.kk2 - b.kd1, b.jj1 - b.jd1:b.jj2 - b.jd1, b.ii1 - b.id1:b.ii2 - b.id1] = yc
+    _last_wall_distance = dist
     wall_parts, far_parts = [], []
     for f in L.FACES:
         g, i, fi = L.face_cells(b, f)
@@ -180,3 +211,38 @@ def wall_ns(ci=256, cj=128, ck=64, seed=20261202, mach=0.5, reynz=1e5, eta=10.0,
                 dict(muscl=1, ilim=ilim, xk=1.0 / 3.0), eta, 1, 0.0, 1, lmax,
                 conservative(np.array(1.0), np.array(mach), np.array(0.0), np.array(0.0), np.array(1 / GAM)),
                 _empty_faces(), _cat(wall_parts), _cat(far_parts))
+
+
+_last_wall_distance = None
+
+
+def turb_constants(model="1SA"):
+    """Spalart-Allmaras (atlecdon.f90:183-204) and Wilcox k-omega
+    (atlecdon.f90:353-366) constants in the order of TurbCfg.tc."""
+    cb1, sigma, cb2, kappa = 0.1355, 2.0 / 3.0, 0.622, 0.41
+    cw1 = cb1 / kappa ** 2 + (1 + cb2) / sigma
+    return [cb1, cb2, sigma, kappa, cw1, 0.3, 2.0, 7.1, 1.1, 2.0, 0.09, 0.075, 0.5, 0.5, 1.0, 1.3]
+
+
+def wall_rans(ci=48, cj=32, ck=16, model="1SA", klroe=2, kwsst=0, nut_ratio=3.0, **kw) -> Case:
+    """The curvilinear wall block of wall_ns with a turbulence model: nu~ =
+    nut_ratio x the laminar kinematic viscosity (SA), or k = 1e-6 U^2 and
+    omega with mut/mu = nut_ratio (k-omega), uniform; wall distance = y of
+    the cell centre."""
+    c = wall_ns(ci, cj, ck, **kw)
+    n = c.layout.ip11
+    dist = _last_wall_distance.copy()
+    mach = float(c.winf[1] / c.winf[0])
+    mu_inf = 1.0 / c.phys["reynz"]  # Sutherland at T = 1 (zvismo.f90:90-104)
+    v0 = np.zeros((7, n)); v0[:5] = c.v0
+    w = np.zeros(7); w[:5] = c.winf
+    if model == "1SA":
+        v0[5] = nut_ratio * mu_inf; w[5] = v0[5, 0]
+    else:
+        k = 1e-6 * mach * mach
+        om = k / (nut_ratio * mu_inf)
+        v0[5] = k; v0[6] = om; w[5] = k; w[6] = om
+    c.v0 = v0
+    c.name = f"wall_rans_{model}"
+    c.turb = TurbCfg(model, turb_constants(model), dist, klroe=klroe, kwsst=kwsst)
+    return c

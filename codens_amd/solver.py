@@ -37,9 +37,10 @@ class Solver:
         self.stream = stream
         f = lambda *s: torch.zeros(*s, dtype=torch.float64, device="cuda")
         self.v = _dev(case.v0)
-        self.u = f(5, n)
-        self.d = f(5, n)
-        self.ff = f(5, n)
+        nv = case.v0.shape[0]  # 7 with a turbulence model (planes 6-7)
+        self.u = f(nv, n)
+        self.d = f(nv, n)
+        self.ff = f(nv, n)
         self.ps = f(n); self.temp = f(n); self.cson = f(n)
         self.mu = f(n); self.mut = f(n); self.dt = f(n)
         self.tq = f(9, n)
@@ -54,6 +55,13 @@ class Solver:
         self.far = (i64(fa.ghost), i64(fa.inner), _dev(fa.normals if fa.normals is not None else np.zeros((3, 0))))
         self.tq_copy = (torch.cat([self.wall[0], self.far[0]]), torch.cat([self.wall[1], self.far[1]]))
         self.nbad = torch.zeros(1, dtype=torch.int32, device="cuda")
+        t = case.turb
+        if t is not None:
+            self.dist = _dev(t.dist)
+            self.fd = f(6, n)
+            self.cfke = f(n); self.ts6 = f(n); self.ts7 = f(n)
+            self.g9 = [f(9, b.nnn) for b in lay.blocks]
+            self.tprod = [f(b.nnn) for b in lay.blocks]
         self.cycle = 0
         self.res0 = None
 
@@ -88,6 +96,48 @@ class Solver:
         if g.numel():
             abi.bc_coincident(g, i, r, self.v, self.ps, self.temp, self.cson, stream=s)
 
+    def turb_stage(self):
+        """The turbulence stage of met_num.f90:162-930 for SA or Wilcox
+        k-omega without wall laws: per block the velocity gradients
+        (teq_gradv), the production (met_prod), the gradients of the
+        transported variables (met_gradtr), the model's sources and
+        diffusive flux densities; their ghost values (met_rbse, met_rbsc);
+        per block the upwind dissipation (met_roe2o or met_roe), the flux
+        balance u(6:7) (sch_turb), the line-implicit update (impli2_eqt_3d)
+        and the limiters (at_cutke); then the eddy viscosity."""
+        t, s = self.case.turb, self.stream
+        sa = t.model == "1SA"
+        for l, b in enumerate(self.blocks):
+            abi.teq_gradv(b, self.sn_blk[l], self.vol, self.v, self.g9[l], stream=s)
+            abi.met_prod(b, self.mut, self.v, self.g9[l], self.tprod[l], stream=s)
+            abi.met_gradtr(b, self.sn_blk[l], self.vol, self.v, self.fd, stream=s)
+            if sa:
+                abi.met_smsa(b, t.tc, self.sn_blk[l], self.vol, self.v, self.mu, self.dist, self.g9[l], self.fd,
+                             self.cfke, self.ts6, self.ts7, stream=s)
+            else:
+                abi.met_smko(b, t.tc, self.v, self.tprod[l], self.cfke, self.ts6, self.ts7, self.cson, stream=s)
+            abi.met_flud(b, t.kfludis, t.tc, self.v, self.mu, self.mut, self.fd, stream=s)
+        if self.tq_copy[0].numel():
+            abi.met_rbse(self.tq_copy[0], self.tq_copy[1], self.fd, stream=s)
+        g, i, r = self.coin
+        if g.numel():
+            abi.met_rbsc(g, i, r, self.fd, stream=s)
+        for l, b in enumerate(self.blocks):
+            if t.klroe == 2:
+                abi.met_roe2o(b, self.v, self.d, self.sn_blk[l], stream=s)
+            else:
+                abi.met_roe(b, self.v, self.d, self.sn_blk[l], stream=s)
+            abi.sch_turb(b, self.u, self.v, self.d, self.fd, self.ts6, self.ts7, self.sn_blk[l], self.vol, stream=s)
+            abi.impli2_eqt_3d(b, self.u, self.dt, self.v, self.mu, self.mut, self.cfke, self.sn_blk[l], self.vol,
+                              t.cmt, stream=s)
+            abi.met_cut(b, 2 if sa else 1, t.epsk, t.epse, self.v, stream=s)
+        for l, b in enumerate(self.blocks):
+            if sa:
+                abi.met_samut(b, t.tc, self.v, self.mu, self.mut, stream=s)
+            else:
+                abi.met_kocmut(b, t.tc, t.kwsst, self.sn_blk[l], self.vol, self.v, self.mu, self.mut, self.dist,
+                               self.g9[l], stream=s)
+
     def stresses(self):
         s = self.stream
         for l, b in enumerate(self.blocks):
@@ -120,6 +170,8 @@ class Solver:
         self.thermo()
         self.time_step()
         self.ghosts()
+        if self.case.turb is not None:
+            self.turb_stage()  # before the mean-flow fluxes (atsch_num.f90:543 < 621)
         if self.case.ns:
             self.stresses()
         self.residual()

@@ -198,3 +198,20 @@ void orc_met_cut(const orc_block *b, double epsk, double epse, double *v, long l
         v[6 * ldv + n] = 0.5 * ((1. + ts) * v[6 * ldv + n] + (1. - ts) * epse);
       }
 }
+
+/* met_rbse.f90:26-38 -- the 6 flux-density planes of boundary ghosts */
+void orc_met_rbse(long nf, const long *ghost, const long *inner, double *fd, long ldv) {
+  for (long f = 0; f < nf; f++)
+    for (int q = 0; q < 6; q++) fd[q * ldv + ghost[f]] = fd[q * ldv + inner[f]];
+}
+
+/* met_rbsc.f90:84-95 (without periodic rotation): every ghost from the
+ * inner values of both sides, gathered before any ghost is written */
+void orc_met_rbsc(long nf, const long *ghost, const long *inner, const long *remote, double *fd, long ldv) {
+  double *buf = malloc(sizeof(double) * 6 * (nf > 0 ? nf : 1));
+  for (long f = 0; f < nf; f++)
+    for (int q = 0; q < 6; q++) buf[6 * f + q] = fd[q * ldv + remote[f]];
+  for (long f = 0; f < nf; f++)
+    for (int q = 0; q < 6; q++) fd[q * ldv + ghost[f]] = 0.5 * (fd[q * ldv + inner[f]] + buf[6 * f + q]);
+  free(buf);
+}

@@ -21,6 +21,12 @@ class CpuStage:
         self.u = z(5, n); self.d = z(5, n); self.ff = z(5, n)
         self.ps = z(n); self.temp = z(n); self.cson = z(n); self.mu = z(n); self.mut = z(n); self.dt = z(n)
         self.tq = z(9, n)
+        nv = self.v.shape[0]
+        self.u = z(nv, n); self.d = z(nv, n); self.ff = z(nv, n)
+        t = case.turb
+        if t is not None:
+            self.fd = z(6, n); self.cfke = z(n); self.ts6 = z(n); self.ts7 = z(n)
+            self.tc = C.OrcTurb(*t.tc)
 
     def stage(self):
         cs, K, O = self.case, self.K, self.O
@@ -46,6 +52,8 @@ class CpuStage:
                           np.ascontiguousarray(cs.winf, dtype=np.float64), self.v, self.ps, self.temp, self.cson)
         if len(c.ghost):
             O.bc_coincident(c.ghost, c.inner, c.remote, self.v, self.ps, self.temp, self.cson)
+        if cs.turb is not None:
+            self.turb_stage()
         if cs.ns:
             for l in range(nb):
                 g = K.zgrad2(l, cs.sn, cs.vol, self.v, self.temp)
@@ -66,3 +74,59 @@ class CpuStage:
             else:
                 K.implimf_3d(l, self.u, self.dt, self.v, self.d, self.ff, self.mu, self.mut, cs.lmax, cs.sn,
                              cs.vol, self.ps, self.cson)
+
+    def turb_stage(self):
+        """met_num.f90:162-930 for SA / Wilcox k-omega without wall laws, the
+        same sequence as Solver.turb_stage, on the oracle."""
+        import ctypes
+        from oracle import ptr
+        cs, O = self.case, self.O
+        t, lay = cs.turb, cs.layout
+        n = lay.ip11
+        sa = t.model == "1SA"
+        L_, D = C.L_, C.D
+        g9s, tps = [], []
+        for l, b in enumerate(lay.blocks):
+            g9 = O.teq_gradv(l, cs.sn, cs.vol, self.v)
+            tp = O.met_prod(l, self.mut, self.v, g9)
+            O.met_gradtr(l, cs.sn, cs.vol, self.v, self.fd)
+            if sa:
+                O.met_smsa(l, self.tc, cs.sn, cs.vol, self.v, self.mu, t.dist, g9, self.fd, self.cfke, self.ts6,
+                           self.ts7)
+            else:
+                O.met_smko(l, self.tc, self.v, tp, self.cfke, self.ts6, self.ts7, self.cson)
+            O.met_flud(l, t.kfludis, self.tc, self.v, self.mu, self.mut, self.fd)
+            g9s.append(g9); tps.append(tp)
+        w, f, c = cs.wall, cs.farfield, cs.coincident
+        gg = np.ascontiguousarray(np.concatenate([w.ghost, f.ghost]).astype(np.int64))
+        ii = np.ascontiguousarray(np.concatenate([w.inner, f.inner]).astype(np.int64))
+        if len(gg):
+            O.lib.orc_met_rbse(L_(len(gg)), ptr(gg), ptr(ii), ptr(self.fd), L_(n))
+        if len(c.ghost):
+            O.lib.orc_met_rbsc(L_(len(c.ghost)), ptr(c.ghost), ptr(c.inner), ptr(c.remote), ptr(self.fd), L_(n))
+        for l, b in enumerate(lay.blocks):
+            if t.klroe == 2:
+                O.met_roe2o(l, self.v, self.d, cs.sn)
+            else:
+                O.met_roe(l, self.v, self.d, cs.sn)
+            O.sch_turb(l, self.u, self.v, self.d, self.fd, self.ts6, self.ts7, cs.sn, cs.vol)
+            ob = C.orc_block(b)
+            snb = np.ascontiguousarray(cs.sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+            O.lib.orc_impli2_eqt_3d(ctypes.byref(ob), ptr(self.u), L_(n), ptr(self.dt), ptr(self.v), ptr(self.mu),
+                                    ptr(self.mut), ptr(self.cfke), ptr(snb), ptr(cs.vol), D(t.cmt))
+            if sa:
+                O.lib.orc_met_cutsa(ctypes.byref(ob), D(t.epsk), ptr(self.v), L_(n))
+            else:
+                O.lib.orc_met_cut(ctypes.byref(ob), D(t.epsk), D(t.epse), ptr(self.v), L_(n))
+        for l, b in enumerate(lay.blocks):
+            ob = C.orc_block(b)
+            if sa:
+                O.lib.orc_met_samut(ctypes.byref(ob), ctypes.byref(self.tc), ptr(self.v), L_(n), ptr(self.mu),
+                                    ptr(self.mut))
+            else:
+                g9 = g9s[l]
+                if t.kwsst:
+                    g9 = O.teq_gradv(l, cs.sn, cs.vol, self.v)
+                O.lib.orc_met_kocmut(ctypes.byref(ob), ctypes.byref(self.tc), t.kwsst, ptr(self.v), L_(n),
+                                     ptr(self.mu), ptr(self.mut), ptr(t.dist), ptr(np.ascontiguousarray(g9)),
+                                     L_(b.nnn))

@@ -70,7 +70,7 @@ def main():
         s.stage()
     torch.cuda.synchronize()
     abi.timing_enable(True)
-    for name in ("implimf_sweep", "roe_face_flux", "implimf_coef", "zvisqc", "roe_limiter_chain", "roe_gather",
+    for name in ("implimf_sweep", "roe_cells", "implimf_coef", "zvisqc", "roe_limiter_chain", "implimf_sweep1",
                  "chronos", "zpres", "implimf_update"):
         abi.timing_read(name, reset=True)
     abi.timing_enable(False)
@@ -87,7 +87,7 @@ def main():
         s.stage()
     torch.cuda.synchronize()
     kern = {}
-    for name in ("implimf_sweep", "roe_face_flux", "implimf_coef", "zvisqc", "roe_limiter_chain", "roe_gather",
+    for name in ("implimf_sweep", "roe_cells", "implimf_coef", "zvisqc", "roe_limiter_chain", "implimf_sweep1",
                  "chronos", "zpres", "zvismo", "implimf_update"):
         ms, cnt = abi.timing_read(name, reset=True)
         if cnt:

@@ -74,6 +74,9 @@ SIGNATURES = {
     "codens_impli2_eqt_3d": [P(CodensBlock), c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp],
     "codens_met_samut": [P(CodensBlock), P(c_dbl), c_vp, c_i64, c_vp, c_vp, c_vp],
     "codens_met_kocmut": [P(CodensBlock), P(c_dbl), c_int, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "codens_met_smkor": [P(CodensBlock), P(c_dbl), c_dbl, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "codens_met_smch": [P(CodensBlock), c_dbl, c_dbl, c_dbl, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                        c_vp],
     "codens_met_cut": [P(CodensBlock), c_int, c_dbl, c_dbl, c_vp, c_i64, c_vp],
     "codens_met_rbse": [c_i64, c_vp, c_vp, c_vp, c_i64, c_vp],
     "codens_met_rbsc": [c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp],

@@ -162,6 +162,17 @@ def met_kocmut(b, tc, kwsst, sn_blk, vol, v, mu, mut, dist, g9, stream=None):
                                _p(dist), _p(g9), _s(stream)))
 
 
+def met_smkor(b, tc, sigmd, v, fd, g9, tprod, cfke, ts6, ts7, stream=None):
+    _chk(lib.codens_met_smkor(ctypes.byref(b), _tc(tc), sigmd, _p(v), ld(v), _p(fd), _p(g9), _p(tprod), _p(cfke),
+                              _p(ts6), _p(ts7), _s(stream)))
+
+
+def met_smch(b, cke1, cke2, utaumin, v, mu, dist, mnpar, utau, tprod, ts6, ts7, stream=None):
+    assert mnpar.dtype == torch.int32
+    _chk(lib.codens_met_smch(ctypes.byref(b), cke1, cke2, utaumin, _p(v), ld(v), _p(mu), _p(dist), _p(mnpar),
+                             _p(utau), _p(tprod), _p(ts6), _p(ts7), _s(stream)))
+
+
 def met_cut(b, which, epsk, epse, v, stream=None):
     """which = 1: met_cut.f90, 2: met_cutsa.f90."""
     _chk(lib.codens_met_cut(ctypes.byref(b), which, epsk, epse, _p(v), ld(v), _s(stream)))

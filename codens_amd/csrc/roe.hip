@@ -230,25 +230,18 @@ __device__ inline void ghost_flux(const Phys &ph, const double w[5], double p, c
 }
 
 // ---------------------------------------------------------------------------
-// pass 2: face fluxes of direction d
-__global__ void __launch_bounds__(TX *TY) k_face_flux(Blk b, Phys ph, int d, int ns, int ilim, double muscl,
-                                                      double xk, const double *__restrict__ v, long ldv,
-                                                      const double *__restrict__ ps,
-                                                      const double *__restrict__ tq,
-                                                      const double *__restrict__ sn,
-                                                      const double *__restrict__ rr, double *__restrict__ H) {
-  const dim3 xb = xcd_block();
-  int c[3];
-  c[0] = b.i1 + xb.x * TX + threadIdx.x;
-  c[1] = b.j1 + xb.y * TY + threadIdx.y;
-  c[2] = b.k1 + xb.z;
-  int hi0 = b.i2 - 1 + (d == 0), hi1 = b.j2 - 1 + (d == 1);
-  if (c[0] > hi0 || c[1] > hi1) return;
-  int a = c[d], a1 = b.lo(d), a2 = b.hi(d);
+// pass 2: the half face flux H of the face of direction d whose storage
+// index is n (cells n-inc | n, along-direction coordinate a): the physical
+// flux of the ghost state on the two boundary faces (sch_roe.f90:497-578),
+// else 0.5 x the Roe flux of the MUSCL states (sch_roe.f90:196-474).
+__device__ inline void face_h(const Blk &b, const Phys &ph, int d, int ns, int ilim, double muscl, double xk,
+                              const double *__restrict__ v, long ldv, const double *__restrict__ ps,
+                              const double *__restrict__ tq, const double *__restrict__ sn,
+                              const double *__restrict__ rr, long n, int a, double f[5]) {
+  int a1 = b.lo(d), a2 = b.hi(d);
   long inc = b.inc(d);
-  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+  long m = n - b.npc;
   double s0 = snv(sn, b.nnn, m, d, 0), s1 = snv(sn, b.nnn, m, d, 1), s2 = snv(sn, b.nnn, m, d, 2);
-  double f[5];
   if (a == a1 || a == a2) {
     long ng = a == a1 ? n - inc : n;
     double w[5], T[9];
@@ -258,67 +251,90 @@ __global__ void __launch_bounds__(TX *TY) k_face_flux(Blk b, Phys ph, int d, int
     for (int q = 0; q < 9; q++) T[q] = ns ? tq[q * ldv + ng] : 0.;
     double rdiv = (!ns && a == a1 && d > 0) ? v[n] : w[0];  // sch_roe_euler.f90:630-632, 912-914
     ghost_flux(ph, w, ps[ng], T, rdiv, !ns, s0, s1, s2, f);
-  } else {
-    Prim q[4];
-#pragma unroll
-    for (int s = 0; s < 4; s++) q[s] = prim_at(v, ldv, ps, n + (s - 2) * inc);
-    double L[5], R[5];
-    double cm = 0.25 * muscl;
-    if (ilim == 1 && a <= a2 - 2) {
-#pragma unroll
-      for (int e = 0; e < 5; e++) {
-        double rl = rr[e * b.nnn + m], rh = rr[e * b.nnn + m + inc];
-        double qa = pget(q[0], e), qb = pget(q[1], e), qc = pget(q[2], e), qd = pget(q[3], e);
-        L[e] = qb + cm * minmod_phi(rl) * (1. - xk) * (qb - qa) + cm * minmod_phi(1. / rl) * (1. + xk) * (qc - qb);
-        R[e] = qc - cm * minmod_phi(rh) * (1. + xk) * (qc - qb) - cm * minmod_phi(1. / rh) * (1. - xk) * (qd - qc);
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < 5; e++) {
-        double qa = pget(q[0], e), qb = pget(q[1], e), qc = pget(q[2], e), qd = pget(q[3], e);
-        L[e] = qb + cm * ((1. - xk) * (qb - qa) + (1. + xk) * (qc - qb));
-        R[e] = qc - cm * ((1. + xk) * (qc - qb) + (1. - xk) * (qd - qc));
-      }
-    }
-    double tl[9], tr[9];
-#pragma unroll
-    for (int q2 = 0; q2 < 9; q2++) {
-      tl[q2] = ns ? tq[q2 * ldv + n - inc] : 0.;
-      tr[q2] = ns ? tq[q2 * ldv + n] : 0.;
-    }
-    roe_face(ph, s0, s1, s2, L, R, tl, tr, f);
-#pragma unroll
-    for (int e = 0; e < 5; e++) f[e] *= 0.5;
+    return;
   }
+  Prim q[4];
 #pragma unroll
-  for (int e = 0; e < 5; e++) H[e * b.nnn + m] = f[e];
+  for (int s = 0; s < 4; s++) q[s] = prim_at(v, ldv, ps, n + (s - 2) * inc);
+  double L[5], R[5];
+  double cm = 0.25 * muscl;
+  if (ilim == 1 && a <= a2 - 2) {
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+      double rl = rr[e * b.nnn + m], rh = rr[e * b.nnn + m + inc];
+      double qa = pget(q[0], e), qb = pget(q[1], e), qc = pget(q[2], e), qd = pget(q[3], e);
+      L[e] = qb + cm * minmod_phi(rl) * (1. - xk) * (qb - qa) + cm * minmod_phi(1. / rl) * (1. + xk) * (qc - qb);
+      R[e] = qc - cm * minmod_phi(rh) * (1. + xk) * (qc - qb) - cm * minmod_phi(1. / rh) * (1. - xk) * (qd - qc);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+      double qa = pget(q[0], e), qb = pget(q[1], e), qc = pget(q[2], e), qd = pget(q[3], e);
+      L[e] = qb + cm * ((1. - xk) * (qb - qa) + (1. + xk) * (qc - qb));
+      R[e] = qc - cm * ((1. + xk) * (qc - qb) + (1. - xk) * (qd - qc));
+    }
+  }
+  double tl[9], tr[9];
+#pragma unroll
+  for (int q2 = 0; q2 < 9; q2++) {
+    tl[q2] = ns ? tq[q2 * ldv + n - inc] : 0.;
+    tr[q2] = ns ? tq[q2 * ldv + n] : 0.;
+  }
+  roe_face(ph, s0, s1, s2, L, R, tl, tr, f);
+#pragma unroll
+  for (int e = 0; e < 5; e++) f[e] *= 0.5;
 }
 
-// ---------------------------------------------------------------------------
-// pass 3: gather over the zeroed range indc(i1-1,j1-1,k1-1)..indc(i2,j2,k2)
-__global__ void __launch_bounds__(256) k_gather(Blk b, long n0, long n1, const double *__restrict__ H, double *u,
-                                                long ldv, double *ff, int ityprk) {
+// pass 3 (fused with pass 2): the residual of a real cell from the half
+// fluxes of its six faces, summed direction by direction as
+// u = (H_i(hi) - H_i(lo)) + (H_j(hi) - H_j(lo)) + (H_k(hi) - H_k(lo))
+// (sch_roe.f90:448-474 scatter, regrouped per cell).  Every interior face is
+// evaluated by both of its cells, with identical inputs, so both see the same
+// value: no face-flux array goes through memory.  ff -= u when ityprk != 0.
+__global__ void __launch_bounds__(TX *TY) k_roe_cells(Blk b, Phys ph, int ns, int ilim, double muscl, double xk,
+                                                      const double *__restrict__ v, long ldv,
+                                                      const double *__restrict__ ps,
+                                                      const double *__restrict__ tq,
+                                                      const double *__restrict__ sn,
+                                                      const double *__restrict__ R, double *__restrict__ u,
+                                                      double *__restrict__ ff, int ityprk) {
+  const dim3 xb = xcd_block();
+  int c[3];
+  c[0] = b.i1 + xb.x * TX + threadIdx.x;
+  c[1] = b.j1 + xb.y * TY + threadIdx.y;
+  c[2] = b.k1 + xb.z;
+  if (c[0] >= b.i2 || c[1] >= b.j2) return;
+  long n = b.idx(c[0], c[1], c[2]);
+  double r[5];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    long inc = b.inc(d);
+    const double *rr = R ? R + 5 * b.nnn * d : nullptr;
+    double fl[5], fh[5];
+    face_h(b, ph, d, ns, ilim, muscl, xk, v, ldv, ps, tq, sn, rr, n + inc, c[d] + 1, fh);
+    face_h(b, ph, d, ns, ilim, muscl, xk, v, ldv, ps, tq, sn, rr, n, c[d], fl);
+#pragma unroll
+    for (int e = 0; e < 5; e++) r[e] = d == 0 ? fh[e] - fl[e] : r[e] + (fh[e] - fl[e]);
+  }
+#pragma unroll
+  for (int e = 0; e < 5; e++) {
+    u[e * ldv + n] = r[e];
+    if (ityprk) ff[e * ldv + n] = ff[e * ldv + n] - r[e];
+  }
+}
+
+// u = 0 on the non-real entries of the zeroed range indc(i1-1,j1-1,k1-1) ..
+// indc(i2,j2,k2) (sch_roe.f90:113-125)
+__global__ void __launch_bounds__(256) k_zero_ghost_u(Blk b, long n0, long n1, double *u, long ldv) {
   long n = n0 + blockIdx.x * 256L + threadIdx.x;
   if (n > n1) return;
   long m = n - b.npc;
   int i = b.id1 + (int)(m % b.nid);
   int j = b.jd1 + (int)((m / b.nid) % (b.nijd / b.nid));
   int k = b.kd1 + (int)(m / b.nijd);
-  const double *H0 = H, *H1 = H + 5 * b.nnn, *H2 = H + 10 * b.nnn;
-  if (!b.real(i, j, k)) {
+  if (b.real(i, j, k)) return;
 #pragma unroll
-    for (int e = 0; e < 5; e++) u[e * ldv + n] = 0.;
-    return;
-  }
-#pragma unroll
-  for (int e = 0; e < 5; e++) {
-    long o = e * b.nnn + m;
-    double r = H0[o + 1] - H0[o];
-    r = r + (H1[o + b.nid] - H1[o]);
-    r = r + (H2[o + b.nijd] - H2[o]);
-    u[e * ldv + n] = r;
-    if (ityprk) ff[e * ldv + n] = ff[e * ldv + n] - r;
-  }
+  for (int e = 0; e < 5; e++) u[e * ldv + n] = 0.;
 }
 
 static int roe_common(const codens_block *cb, const codens_phys *cp, const codens_scheme *sc, int ns,
@@ -333,9 +349,8 @@ static int roe_common(const codens_block *cb, const codens_phys *cp, const coden
   Blk b = make_blk(cb);
   Phys ph = make_phys(cp);
   int ilim = ns ? sc->ilim : 0;
-  double *H = (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 0);
   double *R = ilim == 1 ? (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 1) : nullptr;
-  if (!H || (ilim == 1 && !R)) {
+  if (ilim == 1 && !R) {
     codens_set_error("%s: workspace allocation failed", what);
     return CODENS_EHIP;
   }
@@ -350,20 +365,13 @@ static int roe_common(const codens_block *cb, const codens_phys *cp, const coden
     hipLaunchKernelGGL(k_limiter_chain, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st, b, nt[0], nt[1], nt[2],
                        v, ldv, ps, R);
   }
-  for (int d = 0; d < 3; d++) {
-    int nx = b.i2 - b.i1 + (d == 0), ny = b.j2 - b.j1 + (d == 1), nz = b.k2 - b.k1 + (d == 2);
-    dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz);
-    {
-    CodensTimed _tm("roe_face_flux", st);
-    hipLaunchKernelGGL(k_face_flux, g, dim3(TX, TY), 0, st, b, ph, d, ns, ilim, (double)sc->muscl, sc->xk, v,
-                       ldv, ps, ns ? tq : v, sn, ilim == 1 ? R + 5 * b.nnn * d : nullptr, H + 5 * b.nnn * d);
-  }
+  {
+    CodensTimed _tm("roe_cells", st);
+    hipLaunchKernelGGL(k_roe_cells, cell_grid(b), dim3(TX, TY), 0, st, b, ph, ns, ilim, (double)sc->muscl, sc->xk, v,
+                       ldv, ps, ns ? tq : v, sn, ilim == 1 ? R : nullptr, u, ityprk ? ff : u, (int)ityprk);
   }
   long n0 = b.idx(b.i1 - 1, b.j1 - 1, b.k1 - 1), n1 = b.idx(b.i2, b.j2, b.k2);
-  {
-    CodensTimed _tm("roe_gather", st);
-    hipLaunchKernelGGL(k_gather, dim3((n1 - n0 + 256) / 256), dim3(256), 0, st, b, n0, n1, H, u, ldv, ff, ityprk);
-  }
+  hipLaunchKernelGGL(k_zero_ghost_u, dim3((unsigned)((n1 - n0 + 256) / 256)), dim3(256), 0, st, b, n0, n1, u, ldv);
   return codens_check_launch(what);
 }
 

@@ -289,3 +289,86 @@ extern "C" int codens_met_cut(const codens_block *cb, int32_t which, double epsk
                      (long)ldv);
   return codens_check_launch("codens_met_cut");
 }
+
+// met_smkor.f90:47-70: Wilcox k-omega sources with cross diffusion and the
+// strain-rate limiter; fd = gradients of k and omega, g9 = velocity gradients
+// and tprod block-local.
+__global__ void __launch_bounds__(TX *TY) k_met_smkor(Blk b, double beta, double betas, double sigmw, double sigmd,
+                                                      const double *__restrict__ v, long ldv,
+                                                      const double *__restrict__ fd,
+                                                      const double *__restrict__ g9,
+                                                      const double *__restrict__ tprod, double *__restrict__ cfke,
+                                                      double *__restrict__ ts6, double *__restrict__ ts7) {
+  REAL3(b)
+  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc, nnn = b.nnn;
+  double xw = beta / betas - sigmw * (0.41 * 0.41) / sqrt(betas);
+  double rcmu = 1. / sqrt(0.09);
+  double gkgo = fd[3 * ldv + n] * fd[n] + fd[4 * ldv + n] * fd[ldv + n] + fd[5 * ldv + n] * fd[2 * ldv + n];
+  double omeg = v[6 * ldv + n] / v[n];
+  double cd = sigmd * v[n] * (gkgo > 0. ? gkgo : 0.) / omeg;
+  double dxx = g9[m], dyy = g9[4 * nnn + m], dzz = g9[8 * nnn + m];
+  double a = g9[7 * nnn + m] + g9[5 * nnn + m], bb = g9[2 * nnn + m] + g9[6 * nnn + m];
+  double cc = g9[3 * nnn + m] + g9[nnn + m];
+  double ss = sqrt(4. * (dxx * dxx + dyy * dyy + dzz * dzz) / 3. + a * a + bb * bb + cc * cc) / omeg;
+  double r = rcmu / ss;
+  double alpha = 1. < r ? 1. : r;
+  ts6[n] = tprod[m] - betas * v[5 * ldv + n] * omeg;
+  ts7[n] = xw * tprod[m] * v[6 * ldv + n] / v[5 * ldv + n] / alpha - beta * v[6 * ldv + n] * omeg + cd;
+  cfke[n] = 0.18 * omeg;
+}
+
+// met_smch.f90:88-108: Chien low-Reynolds k-epsilon sources
+__global__ void __launch_bounds__(TX *TY) k_met_smch(Blk b, double cke1, double cke2, double utaumin,
+                                                     const double *__restrict__ v, long ldv,
+                                                     const double *__restrict__ mu,
+                                                     const double *__restrict__ dist,
+                                                     const int32_t *__restrict__ mnpar,
+                                                     const double *__restrict__ utau,
+                                                     const double *__restrict__ tprod, double *__restrict__ ts6,
+                                                     double *__restrict__ ts7) {
+  REAL3(b)
+  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+  int mp = mnpar[n];
+  double yplus = 1000.;
+  if (mp >= 1) {
+    double ut = fabs(utau[mp - 1]);
+    yplus = (ut > utaumin ? ut : utaumin) * dist[n] * v[n] / mu[n];
+  }
+  double rodst = v[n] * (dist[n] * dist[n]);
+  double k = v[5 * ldv + n], e = v[6 * ldv + n];
+  ts6[n] = tprod[m] - e - 2. * mu[n] * k / rodst;
+  double epssk = e / k;
+  double rtur = k / (mu[n] * epssk);
+  double f2 = 1. - .22222222 * exp(-rtur * rtur / 36.);
+  ts7[n] = cke1 * epssk * tprod[m] - cke2 * f2 * epssk * e - 2. * mu[n] * (e / rodst) * exp(-yplus / 2.);
+}
+
+extern "C" int codens_met_smkor(const codens_block *cb, const double *tc, double sigmd, const double *v,
+                                int64_t ldv, const double *fd, const double *g9, const double *tprod, double *cfke,
+                                double *ts6, double *ts7, void *stream) {
+  int rc = codens_check_block(cb, "codens_met_smkor");
+  if (rc) return rc;
+  if (!tc || !v || !fd || !g9 || !tprod || !cfke || !ts6 || !ts7) {
+    codens_set_error("codens_met_smkor: null argument");
+    return CODENS_EINVAL;
+  }
+  Blk b = make_blk(cb);
+  hipLaunchKernelGGL(k_met_smkor, cell_grid(b), dim3(TX, TY), 0, (hipStream_t)stream, b, tc[11], tc[10], tc[13],
+                     sigmd, v, (long)ldv, fd, g9, tprod, cfke, ts6, ts7);
+  return codens_check_launch("codens_met_smkor");
+}
+
+extern "C" int codens_met_smch(const codens_block *cb, double cke1, double cke2, double utaumin, const double *v,
+                               int64_t ldv, const double *mu, const double *dist, const int32_t *mnpar,
+                               const double *utau, const double *tprod, double *ts6, double *ts7, void *stream) {
+  int rc = codens_check_block(cb, "codens_met_smch");
+  if (rc) return rc;
+  if (!v || !mu || !dist || !mnpar || !utau || !tprod || !ts6 || !ts7) {
+    codens_set_error("codens_met_smch: null argument");
+    return CODENS_EINVAL;
+  }
+  Blk b = make_blk(cb);
+  hipLaunchKernelGGL(k_met_smch, cell_grid(b), dim3(TX, TY), 0, (hipStream_t)stream, b, cke1, cke2, utaumin, v,
+                     (long)ldv, mu, dist, mnpar, utau, tprod, ts6, ts7);
+  return codens_check_launch("codens_met_smch");
+}

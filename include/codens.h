@@ -183,6 +183,22 @@ int codens_met_kocmut(const codens_block *b, const double *tc, int32_t kwsst, co
                       const double *vol, double *v, int64_t ldv, const double *mu, double *mut,
                       const double *dist, double *g9, void *stream);
 
+/* met_smkor.f90:4 -- Wilcox k-omega sources with cross diffusion sigmd
+ * max(grad k . grad omega, 0) rho/omega and the strain-rate limiter of the
+ * omega production (equatt '2KOR').  fd: gradients of k and omega from
+ * codens_met_gradtr; g9, tprod block-local; beta, betas, sigmw = tc[11],
+ * tc[10], tc[13].  Writes ts6, ts7 and cfke = 0.18 omega. */
+int codens_met_smkor(const codens_block *b, const double *tc, double sigmd, const double *v, int64_t ldv,
+                     const double *fd, const double *g9, const double *tprod, double *cfke, double *ts6,
+                     double *ts7, void *stream);
+
+/* met_smch.f90:4 -- Chien low-Reynolds k-epsilon sources (equatt '2CH');
+ * y+ of a cell from the friction velocity utau[mnpar-1] of its wall point
+ * (mnpar: int32 per cell, 0 for none, y+ = 1000), floored at utaumin. */
+int codens_met_smch(const codens_block *b, double cke1, double cke2, double utaumin, const double *v, int64_t ldv,
+                    const double *mu, const double *dist, const int32_t *mnpar, const double *utau,
+                    const double *tprod, double *ts6, double *ts7, void *stream);
+
 /* at_cutke.f90:4 limiters: which = 1 met_cut.f90 (both variables to
  * (epsk, epse) unless both exceed them), which = 2 met_cutsa.f90
  * (nu~ < 0 -> epsk, v(7) = 0). */

@@ -215,3 +215,61 @@ void orc_met_rbsc(long nf, const long *ghost, const long *inner, const long *rem
     for (int q = 0; q < 6; q++) fd[q * ldv + ghost[f]] = 0.5 * (fd[q * ldv + inner[f]] + buf[6 * f + q]);
   free(buf);
 }
+
+/* met_smkor.f90:4 -- Wilcox k-omega sources with cross diffusion and the
+ * strain-rate limiter (equatt '2KOR', met_num.f90:509): fd holds the
+ * gradients of k and omega (met_gradtr), g9 the velocity gradients
+ * (block-local, teq_gradv order), tprod the production (block-local). */
+void orc_met_smkor(const orc_block *b, const orc_turb *tc, double sigmd, const double *v, long ldv,
+                   const double *fd, const double *g9, long ldg, const double *tprod, double *cfke, double *ts6,
+                   double *ts7) {
+  double xw = tc->beta / tc->betas - tc->sigmw * (0.41 * 0.41) / sqrt(tc->betas);
+  double rcmu = 1. / sqrt(0.09);
+  const double *dxx = g9, *dxy = g9 + ldg, *dxz = g9 + 2 * ldg, *dyx = g9 + 3 * ldg, *dyy = g9 + 4 * ldg;
+  const double *dyz = g9 + 5 * ldg, *dzx = g9 + 6 * ldg, *dzy = g9 + 7 * ldg, *dzz = g9 + 8 * ldg;
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - b->npc;
+        double gkgo = fd[3 * ldv + n] * fd[n] + fd[4 * ldv + n] * fd[ldv + n] + fd[5 * ldv + n] * fd[2 * ldv + n];
+        double omeg = v[6 * ldv + n] / v[n];
+        double cd = sigmd * v[n] * (gkgo > 0. ? gkgo : 0.) / omeg;
+        double a = dzy[m] + dyz[m], bb = dxz[m] + dzx[m], c = dyx[m] + dxy[m];
+        double ss = sqrt(4. * (dxx[m] * dxx[m] + dyy[m] * dyy[m] + dzz[m] * dzz[m]) / 3. + a * a + bb * bb + c * c) /
+                    omeg;
+        double r = rcmu / ss;
+        double alpha = 1. < r ? 1. : r;
+        ts6[n] = tprod[m] - tc->betas * v[5 * ldv + n] * omeg;
+        ts7[n] = xw * tprod[m] * v[6 * ldv + n] / v[5 * ldv + n] / alpha - tc->beta * v[6 * ldv + n] * omeg + cd;
+        cfke[n] = 0.18 * omeg;
+      }
+}
+
+/* met_smch.f90:4 -- Chien low-Reynolds k-epsilon sources; y+ from the
+ * friction velocity utau of the cell's wall point mnpar (1-based, 0 =
+ * none: y+ = 1000), floored at utaumin. */
+void orc_met_smch(const orc_block *b, double cke1, double cke2, double utaumin, const double *v, long ldv,
+                  const double *mu, const double *dist, const int *mnpar, const double *utau, const double *tprod,
+                  double *ts6, double *ts7) {
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - b->npc;
+        int mp = mnpar[n];
+        double yplus = 1000.;
+        if (mp >= 1) {
+          double ut = fabs(utau[mp - 1]);
+          yplus = (ut > utaumin ? ut : utaumin) * dist[n] * v[n] / mu[n];
+        }
+        double dist2 = dist[n] * dist[n];
+        double rodst = v[n] * dist2;
+        ts6[n] = tprod[m] - v[6 * ldv + n] - 2. * mu[n] * v[5 * ldv + n] / rodst;
+        double epssk = v[6 * ldv + n] / v[5 * ldv + n];
+        double rtur = v[5 * ldv + n] / (mu[n] * epssk);
+        double f2 = 1. - .22222222 * exp(-rtur * rtur / 36.);
+        double sch1 = cke1 * epssk * tprod[m];
+        double sch2 = cke2 * f2 * epssk * v[6 * ldv + n];
+        double sch3 = 2. * mu[n] * (v[6 * ldv + n] / rodst) * exp(-yplus / 2.);
+        ts7[n] = sch1 - sch2 - sch3;
+      }
+}

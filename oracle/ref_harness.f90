@@ -424,4 +424,28 @@ contains
     endif
   end subroutine ref_met_cut
 
+  ! met_smkor.f90:4 (sigmd of the model deck)
+  subroutine ref_met_smkor(l, sigmdl, v, fd, g, tprod, cfke, ts6, ts7) bind(C, name="ref_met_smkor")
+    use mod_met_smkor
+    integer(c_int), value :: l
+    real(c_double), value :: sigmdl
+    real(c_double) :: v(*), fd(ip11, 6), g(ip00, 9), tprod(*), cfke(*), ts6(*), ts7(*)
+    sigmd = sigmdl
+    call met_smkor(l, v, fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6), g(1,1), g(1,2), g(1,3), g(1,4), &
+         g(1,5), g(1,6), g(1,7), g(1,8), g(1,9), tprod, cfke, ts6, ts7)
+  end subroutine ref_met_smkor
+
+  ! met_smch.f90:4
+  subroutine ref_met_smch(l, c1, c2, utmin, v, mu, dist, mnpar, utau, tprod, ts6, ts7) bind(C, name="ref_met_smch")
+    use mod_met_smch
+    integer(c_int), value :: l
+    real(c_double), value :: c1, c2, utmin
+    real(c_double) :: v(*), mu(*), dist(*), utau(*), tprod(*), ts6(*), ts7(*)
+    integer(c_int) :: mnpar(*)
+    cke1 = c1
+    cke2 = c2
+    utaumin = utmin
+    call met_smch(l, v, mu, dist, mnpar, utau, tprod, ts6, ts7)
+  end subroutine ref_met_smch
+
 end module codens_ref_harness

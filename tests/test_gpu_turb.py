@@ -198,3 +198,34 @@ def test_flux_density_ghosts(setup):
     abi.met_rbse(G(g0), G(i0), gfd)
     abi.met_rbsc(G(g1), G(i1), G(r1), gfd)
     assert np.array_equal(H(gfd), t9[:6])
+
+
+def test_sources_kor_chien(setup):
+    """met_smkor and met_smch against the oracle."""
+    import ctypes
+    from oracle import ptr
+    abi, lay, sn, vol, v, mu, mut, dist, cson, tc, tcl, O = setup
+    n = lay.ip11
+    rng = np.random.default_rng(97)
+    fd = 1e-2 * rng.standard_normal((6, n))
+    mnpar = rng.integers(0, 5, n).astype(np.int32)
+    utau = np.array([1e-5, -3e-3, 2e-2, 4e-4])
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        inner = b.interior()
+        g9 = rng.standard_normal((9, b.nnn)); tp = 1e-3 * rng.random(b.nnn)
+        oo = [np.zeros(n) for _ in range(3)]
+        O.lib.orc_met_smkor(ctypes.byref(ob), ctypes.byref(tc), C.D(0.65), ptr(v), C.L_(n), ptr(fd), ptr(g9),
+                            C.L_(b.nnn), ptr(tp), *[ptr(a) for a in oo])
+        go = [T(np.zeros(n)) for _ in range(3)]
+        abi.met_smkor(abi.block(b), tcl, 0.65, T(v), T(fd), T(g9), T(tp), *go)
+        for a, o in zip(go, oo):
+            assert relerr(H(a)[inner], o[inner]) <= TOL
+        oo = [np.zeros(n) for _ in range(2)]
+        O.lib.orc_met_smch(ctypes.byref(ob), C.D(1.35), C.D(1.8), C.D(2e-4), ptr(v), C.L_(n), ptr(mu), ptr(dist),
+                           ptr(mnpar), ptr(utau), ptr(tp), *[ptr(a) for a in oo])
+        go = [T(np.zeros(n)) for _ in range(2)]
+        abi.met_smch(abi.block(b), 1.35, 1.8, 2e-4, T(v), T(mu), T(dist), torch.as_tensor(mnpar, device="cuda"),
+                     T(utau), T(tp), *go)
+        for a, o in zip(go, oo):
+            assert relerr(H(a)[inner], o[inner]) <= TOL

@@ -192,3 +192,35 @@ def test_limiters(setup, which):
             O.orc_met_cut(ctypes.byref(ob), C.D(1e-6), C.D(1e-4), ptr(vo), C.L_(n))
     assert np.array_equal(vr, vo)
     assert not np.array_equal(vr, vv)
+
+
+def test_sources_kor_chien(setup):
+    """met_smkor (k-omega with cross diffusion and strain limiter) and
+    met_smch (Chien k-epsilon, y+ from the wall point's friction velocity)."""
+    lay, sn, vol, v, mu, mut, dist, cson, tc, R, O = setup
+    n = lay.ip11
+    rng = np.random.default_rng(57)
+    fd = 1e-2 * rng.standard_normal((6, n))
+    mnpar = rng.integers(0, 5, n).astype(np.int32)
+    utau = np.array([1e-5, -3e-3, 2e-2, 4e-4])
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        g = np.zeros((9, lay.ip00)); g[:, :b.nnn] = rng.standard_normal((9, b.nnn))
+        gb = np.ascontiguousarray(g[:, :b.nnn])
+        tp = np.zeros(lay.ip00); tp[:b.nnn] = 1e-3 * rng.random(b.nnn)
+        out_r = [np.full(n, 7.0) for _ in range(3)]
+        out_o = [np.full(n, 7.0) for _ in range(3)]
+        R.lib.ref_met_smkor(l + 1, C.D(0.65), ptr(v), ptr(fd), ptr(g), ptr(tp), *[ptr(a) for a in out_r])
+        O.orc_met_smkor(ctypes.byref(ob), ctypes.byref(tc), C.D(0.65), ptr(v), C.L_(n), ptr(fd), ptr(gb),
+                        C.L_(b.nnn), ptr(tp), *[ptr(a) for a in out_o])
+        for a, o in zip(out_r, out_o):
+            assert np.array_equal(a, o)
+        out_r = [np.full(n, 7.0) for _ in range(2)]
+        out_o = [np.full(n, 7.0) for _ in range(2)]
+        R.lib.ref_met_smch(l + 1, C.D(1.35), C.D(1.8), C.D(2e-4), ptr(v), ptr(mu), ptr(dist), ptr(mnpar), ptr(utau),
+                           ptr(tp), *[ptr(a) for a in out_r])
+        O.orc_met_smch(ctypes.byref(ob), C.D(1.35), C.D(1.8), C.D(2e-4), ptr(v), C.L_(n), ptr(mu), ptr(dist),
+                       ptr(mnpar), ptr(utau), ptr(tp), *[ptr(a) for a in out_o])
+        for a, o in zip(out_r, out_o):
+            assert np.array_equal(a, o)
+            assert np.isfinite(a[b.interior()]).all()

@@ -518,7 +518,8 @@ extern "C" int codens_met_smko(const codens_block *cb, const double *tc, const d
 extern "C" int codens_met_flud(const codens_block *cb, int32_t kfludis, const double *tc, const double *v,
                                int64_t ldv, const double *mu, const double *mut, double *fd, void *stream) {
   CHECK_BLOCK("codens_met_flud")
-  if (!tc || !v || !mu || !mut || !fd || (kfludis != 1 && kfludis != 2 && kfludis != 5)) {
+  if (!tc || (kfludis == 2 && !v) || !mu || (kfludis != 2 && !mut) || !fd ||
+      (kfludis != 1 && kfludis != 2 && kfludis != 5)) {
     codens_set_error("codens_met_flud: invalid argument (kfludis %d)", kfludis);
     return CODENS_EINVAL;
   }

@@ -161,6 +161,105 @@ module codens_c
        type(c_ptr), value :: t, d, sn, stream
        integer(c_int64_t), value :: ldv
      end function codens_met_roe
+     integer(c_int32_t) function codens_teq_gradv(b, sn, vol, v, ldv, g9, stream) bind(C, name="codens_teq_gradv")
+       import
+       type(codens_block), intent(in) :: b
+       type(c_ptr), value :: sn, vol, v, g9, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_teq_gradv
+     integer(c_int32_t) function codens_met_prod(b, mut, v, ldv, g9, tprod, stream) bind(C, name="codens_met_prod")
+       import
+       type(codens_block), intent(in) :: b
+       type(c_ptr), value :: mut, v, g9, tprod, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_prod
+     integer(c_int32_t) function codens_met_gradtr(b, sn, vol, v, ldv, fd, stream) bind(C, name="codens_met_gradtr")
+       import
+       type(codens_block), intent(in) :: b
+       type(c_ptr), value :: sn, vol, v, fd, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_gradtr
+     integer(c_int32_t) function codens_met_smsa(b, tc, sn, vol, v, ldv, mu, dist, g9, fd, cfke, ts6, ts7, stream) &
+          bind(C, name="codens_met_smsa")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: tc(16)
+       type(c_ptr), value :: sn, vol, v, mu, dist, g9, fd, cfke, ts6, ts7, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_smsa
+     integer(c_int32_t) function codens_met_smko(b, tc, v, ldv, tprod, cfke, ts6, ts7, cson, stream) &
+          bind(C, name="codens_met_smko")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: tc(16)
+       type(c_ptr), value :: v, tprod, cfke, ts6, ts7, cson, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_smko
+     integer(c_int32_t) function codens_met_smkor(b, tc, sigmd, v, ldv, fd, g9, tprod, cfke, ts6, ts7, stream) &
+          bind(C, name="codens_met_smkor")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: tc(16)
+       real(c_double), value :: sigmd
+       type(c_ptr), value :: v, fd, g9, tprod, cfke, ts6, ts7, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_smkor
+     integer(c_int32_t) function codens_met_smch(b, cke1, cke2, utaumin, v, ldv, mu, dist, mnpar, utau, tprod, &
+          ts6, ts7, stream) bind(C, name="codens_met_smch")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), value :: cke1, cke2, utaumin
+       type(c_ptr), value :: v, mu, dist, mnpar, utau, tprod, ts6, ts7, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_smch
+     integer(c_int32_t) function codens_met_flud(b, kfludis, tc, v, ldv, mu, mut, fd, stream) &
+          bind(C, name="codens_met_flud")
+       import
+       type(codens_block), intent(in) :: b
+       integer(c_int32_t), value :: kfludis
+       real(c_double), intent(in) :: tc(16)
+       type(c_ptr), value :: v, mu, mut, fd, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_flud
+     integer(c_int32_t) function codens_sch_turb(b, u, v, ldv, d, fd, ts6, ts7, sn, vol, stream) &
+          bind(C, name="codens_sch_turb")
+       import
+       type(codens_block), intent(in) :: b
+       type(c_ptr), value :: u, v, d, fd, ts6, ts7, sn, vol, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_sch_turb
+     integer(c_int32_t) function codens_impli2_eqt_3d(b, u, ldv, dt, v, mu, mut, cfke, sn, vol, cmt, stream) &
+          bind(C, name="codens_impli2_eqt_3d")
+       import
+       type(codens_block), intent(in) :: b
+       type(c_ptr), value :: u, dt, v, mu, mut, cfke, sn, vol, stream
+       integer(c_int64_t), value :: ldv
+       real(c_double), value :: cmt
+     end function codens_impli2_eqt_3d
+     integer(c_int32_t) function codens_met_samut(b, tc, v, ldv, mu, mut, stream) bind(C, name="codens_met_samut")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: tc(16)
+       type(c_ptr), value :: v, mu, mut, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_samut
+     integer(c_int32_t) function codens_met_kocmut(b, tc, kwsst, sn, vol, v, ldv, mu, mut, dist, g9, stream) &
+          bind(C, name="codens_met_kocmut")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: tc(16)
+       integer(c_int32_t), value :: kwsst
+       type(c_ptr), value :: sn, vol, v, mu, mut, dist, g9, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_kocmut
+     integer(c_int32_t) function codens_met_cut(b, which, epsk, epse, v, ldv, stream) bind(C, name="codens_met_cut")
+       import
+       type(codens_block), intent(in) :: b
+       integer(c_int32_t), value :: which
+       real(c_double), value :: epsk, epse
+       type(c_ptr), value :: v, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_cut
   end interface
 
 contains
@@ -206,6 +305,85 @@ contains
     type(codens_scheme) :: sc
     sc%muscl = muscl; sc%ilim = ilim; sc%xk = xk
   end function codens_scheme_now
+
+  ! turbulence constants of module modeleturb in the order of the C-ABI
+  function codens_tc_now() result(tc)
+    use modeleturb
+    real(c_double) :: tc(16)
+    tc = [cb1, cb2, sigma, kappa, cw1, cw2, cw3, cv1, ct3, ct4, betas, beta, sigmk, sigmw, alfak, alfae]
+  end function codens_tc_now
+
+  ! offset of block b's metric slice in the global sn(ip31*ndir) that some
+  ! routines receive whole (they find npsn = ndir*npfb(l)+1 themselves)
+  function codens_sn_blk(sn, l) result(p)
+    use maillage
+    use para_fige
+    double precision, target, intent(in) :: sn(*)
+    integer, intent(in) :: l
+    type(c_ptr) :: p
+    p = c_loc(sn(ndir*npfb(l) + 1))
+  end function codens_sn_blk
+
+  ! n arrays of length len (each) as one device bundle with slot stride
+  ! len; the window [lo, hi) of element indices (0-based) of each is copied
+  function codens_bundle_in(arrs, cnt, len, lo, hi) result(dev)
+    type(c_ptr), intent(in) :: arrs(*)
+    integer, intent(in) :: cnt, len, lo, hi
+    type(c_ptr) :: dev
+    call codens_check(codens_host_bundle(arrs, int(cnt, c_int32_t), 8_c_int64_t*len, 8_c_int64_t*lo, &
+         8_c_int64_t*(hi - lo), 1_c_int32_t, dev, c_null_ptr), 'codens_bundle_in')
+  end function codens_bundle_in
+
+  subroutine codens_bundle_back(arrs, cnt, len, lo, hi)
+    type(c_ptr), intent(in) :: arrs(*)
+    integer, intent(in) :: cnt, len, lo, hi
+    call codens_check(codens_host_bundle_fetch(arrs, int(cnt, c_int32_t), 8_c_int64_t*len, 8_c_int64_t*lo, &
+         8_c_int64_t*(hi - lo), c_null_ptr), 'codens_bundle_back')
+  end subroutine codens_bundle_back
+
+  ! The met_flud* family on block l: kfludis selects the diffusivity
+  ! (met_num.f90:528-556); the six arrays go to the device as one bundle of
+  ! the block's window and come back transformed.  s is read for kfludis = 2
+  ! only, mut for 1 and 5 only.
+  subroutine codens_flud(kfludis, l, s, mu, mut, fd5x, fd5y, fd5z, fd6x, fd6y, fd6z, what)
+    use para_var
+    use maillage
+    integer, intent(in) :: kfludis, l
+    double precision, target :: s(*), mu(ip12), mut(ip12)
+    double precision, target :: fd5x(ip12), fd5y(ip12), fd5z(ip12), fd6x(ip12), fd6y(ip12), fd6z(ip12)
+    character(len=*), intent(in) :: what
+    type(codens_block) :: b
+    type(c_ptr) :: dv, dmu, dmut, dfd, f(6)
+    b = codens_block_of(l)
+    f = [c_loc(fd5x), c_loc(fd5y), c_loc(fd5z), c_loc(fd6x), c_loc(fd6y), c_loc(fd6z)]
+    dv = c_null_ptr
+    dmut = c_null_ptr
+    if (kfludis .eq. 2) then
+       dv = codens_dev(c_loc(s), ip11*ip60, ip11, 1, 7, b, 1)
+    else
+       dmut = codens_dev(c_loc(mut), ip12, ip12, 1, 1, b, 1)
+    endif
+    dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+    dfd = codens_bundle_in(f, 6, ip12, npc(l), npc(l) + nnn(l))
+    call codens_check(codens_met_flud(b, int(kfludis, c_int32_t), codens_tc_now(), dv, int(ip11, c_int64_t), dmu, &
+         dmut, dfd, c_null_ptr), what)
+    call codens_bundle_back(f, 6, ip12, npc(l), npc(l) + nnn(l))
+  end subroutine codens_flud
+
+  ! at_cutke limiters on block l: which = 1 met_cut, 2 met_cutsa
+  subroutine codens_cut(which, l, s, epsk, epse)
+    use para_var
+    integer, intent(in) :: which, l
+    double precision, target :: s(ip11,ip60)
+    double precision, intent(in) :: epsk, epse
+    type(codens_block) :: b
+    type(c_ptr) :: dv
+    b = codens_block_of(l)
+    dv = codens_dev(c_loc(s), ip11*ip60, ip11, 6, 2, b, 1)
+    call codens_check(codens_met_cut(b, int(which, c_int32_t), epsk, epse, dv, int(ip11, c_int64_t), c_null_ptr), &
+         'met_cut')
+    call codens_back(c_loc(s), ip11, 6, 2, b)
+  end subroutine codens_cut
 
   subroutine codens_check(rc, what)
     integer(c_int32_t), intent(in) :: rc

@@ -150,7 +150,8 @@ int codens_met_smsa(const codens_block *b, const double *tc, const double *sn_bl
 int codens_met_smko(const codens_block *b, const double *tc, const double *v, int64_t ldv, const double *tprod,
                     double *cfke, double *ts6, double *ts7, const double *cson, void *stream);
 /* met_fludc.f90:4 (kfludis=1), met_fludcsa.f90:4 (2), met_fludko.f90:4 (5)
- * -- gradients fd become diffusive flux densities, in place. */
+ * -- gradients fd become diffusive flux densities, in place.  v is read for
+ * kfludis = 2 only, mut for 1 and 5 only (either may be NULL otherwise). */
 int codens_met_flud(const codens_block *b, int32_t kfludis, const double *tc, const double *v, int64_t ldv,
                     const double *mu, const double *mut, double *fd, void *stream);
 /* sch_turb.f90:4 -- flux balance u6, u7 = 0.5 sum of centred face fluxes

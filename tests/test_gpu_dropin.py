@@ -169,3 +169,118 @@ def test_turbulence_upwind(mesh, which):
     ok = ~np.isnan(dr[5:7])
     assert np.abs(dr[5:7][ok] - dg[5:7][ok]).max() <= TOL * np.abs(dr[5:7][ok]).max()
     assert np.array_equal(dr[:5], dg[:5])
+
+
+@pytest.fixture(scope="module")
+def tsetup(mesh):
+    lay, sn, vol = mesh
+    rng = np.random.default_rng(201)
+    n = lay.ip11
+    v = np.zeros((7, n)); v[:5] = random_state(n, rng)
+    v[5] = 1e-3 * (1 + rng.random(n)) * v[0]
+    v[6] = 50 * (1 + rng.random(n)) * v[0]
+    mu = 1e-3 * (1 + 0.1 * rng.random(n)); mut = 1e-3 * rng.random(n)
+    dist = 0.05 + rng.random(n)
+    p, t, c = all_thermo(v[:5])
+    R, G = pair(lay, C.orc_phys())
+    tc = C.turb_constants()
+    R.set_turb(tc); G.set_turb(tc)
+    return lay, sn, vol, v, mu, mut, dist, c, R, G
+
+
+def _both(R, G, name, *arrays_and_scalars, outs=()):
+    """Call wrapper `name` on the reference and on the drop-in library with
+    copies of the listed numpy arrays; returns the two lists of arrays."""
+    from oracle import ptr
+    res = []
+    for X in (R, G):
+        args = [a.copy() if isinstance(a, np.ndarray) else a for a in arrays_and_scalars]
+        getattr(X.lib, name)(*[ptr(a) if isinstance(a, np.ndarray) else a for a in args])
+        res.append([a for a in args if isinstance(a, np.ndarray)])
+    return res
+
+
+def test_turbulence_routines(tsetup):
+    """teq_gradv, met_prod, met_gradtr, met_flud* (1, 2, 5), met_smsa,
+    met_smko, met_smkor, met_smch, sch_turb, met_samut, met_kocmut (0, 1),
+    met_cut, met_cutsa through the reference's own signatures."""
+    lay, sn, vol, v, mu, mut, dist, c, R, G = tsetup
+    n = lay.ip11
+    cmu = np.ones(n)
+    rng = np.random.default_rng(203)
+    D = C.D
+    for l, b in enumerate(lay.blocks):
+        inner = b.interior()
+        g0 = np.zeros((9, lay.ip00)); s0 = np.zeros(lay.ip00)
+        r_, g_ = _both(R, G, "ref_teq_gradv", l + 1, sn, vol, v, s0, g0, cmu)
+        vr, gr, vg, gg = r_[2], r_[4], g_[2], g_[4]
+        assert np.array_equal(vr, vg)
+        m = inner - b.npc
+        close(gr[:, m], gg[:, m])
+        g = gr
+        (_, _, _, tr), (_, _, _, tg) = _both(R, G, "ref_met_prod", l + 1, mut, v, g, np.zeros(lay.ip00))
+        close(tr[m], tg[m])
+        tp = tr
+        w = [np.zeros(lay.ip00) for _ in range(4)]
+        r_, g_ = _both(R, G, "ref_met_gradtr", l + 1, sn, vol, v, mu, mut, *w, np.zeros((6, n)), cmu)
+        fdr, fdg = r_[-2], g_[-2]
+        close(fdr[:, inner], fdg[:, inner])
+        fd = fdr
+        for kfl in (1, 2, 5):
+            r_, g_ = _both(R, G, "ref_met_flud", l + 1, kfl, v, mu, mut, fd, *w)
+            close(r_[3][:, inner], g_[3][:, inner])
+        w5 = [np.zeros(lay.ip00) for _ in range(5)]
+        outs = lambda: [np.zeros(n) for _ in range(3)]
+        r_, g_ = _both(R, G, "ref_met_smsa", l + 1, sn, vol, v, mu, dist, g, fd, np.zeros(n), *w5, np.zeros(n),
+                       np.zeros(n), cmu)
+        for x, y in zip([r_[7], r_[13], r_[14]], [g_[7], g_[13], g_[14]]):  # cfke, ts6, ts7
+            close(x[inner], y[inner])
+        r_, g_ = _both(R, G, "ref_met_smko", l + 1, v, fd, tp, *outs(), c)
+        for x, y in zip(r_[3:6], g_[3:6]):
+            close(x[inner], y[inner])
+        r_, g_ = _both(R, G, "ref_met_smkor", l + 1, D(0.65), v, fd, g, tp, *outs())
+        for x, y in zip(r_[4:7], g_[4:7]):
+            close(x[inner], y[inner])
+        mnpar = rng.integers(0, 5, n).astype(np.int32); utau = np.array([1e-5, -3e-3, 2e-2, 4e-4])
+        r_, g_ = _both(R, G, "ref_met_smch", l + 1, D(1.35), D(1.8), D(2e-4), v, mu, dist, mnpar, utau, tp,
+                       np.zeros(n), np.zeros(n))
+        for x, y in zip(r_[6:8], g_[6:8]):
+            close(x[inner], y[inner])
+        d = 1e-4 * rng.standard_normal((7, n)); ts = 1e-3 * rng.standard_normal((2, n))
+        ncin = np.zeros(1, dtype=np.int32)
+        snb = np.ascontiguousarray(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+        r_, g_ = _both(R, G, "ref_sch_turb", l + 1, np.zeros((7, n)), v, d, fd, ts[0].copy(), ts[1].copy(), ncin,
+                       snb, b.nnn, vol, np.zeros((6, lay.ip00)))
+        close(r_[0][5:7, inner], g_[0][5:7, inner])
+        r_, g_ = _both(R, G, "ref_met_samut", l + 1, v, mu, np.zeros(n))
+        close(r_[2][inner], g_[2][inner])
+        for kwsst in (0, 1):
+            r_, g_ = _both(R, G, "ref_met_kocmut", l + 1, kwsst, sn, vol, np.zeros(lay.ip00),
+                           np.zeros((9, lay.ip00)), dist, v, mu, np.zeros(n), cmu)
+            close(r_[7][inner], g_[7][inner])
+            assert np.array_equal(r_[5], g_[5])
+        vv = v.copy(); vv[5] = 1e-3 * rng.standard_normal(n); vv[6] = rng.standard_normal(n)
+        for which in (1, 2):
+            r_, g_ = _both(R, G, "ref_met_cut", l + 1, which, D(1e-6), D(1e-4), vv)
+            assert np.array_equal(r_[0], g_[0])
+
+
+@pytest.mark.parametrize("model", [0, 1])
+def test_impli2_eqt_3d(tsetup, model):
+    lay, sn, vol, v, mu, mut, dist, c, R, G = tsetup
+    n = lay.ip11
+    rng = np.random.default_rng(207)
+    u = np.zeros((7, n)); u[5:] = 1e-3 * rng.standard_normal((2, n))
+    dt = 0.01 * (1 + rng.random(n)); cfke = rng.random(n)
+    R.lib.ref_set_equatt(model); G.lib.ref_set_equatt(model)
+    t = [np.zeros(lay.ip00) for _ in range(10)]
+    vr, vg = v.copy(), v.copy()
+    from oracle import ptr
+    for X, vx in ((R, vr), (G, vg)):
+        for l, b in enumerate(lay.blocks):
+            snb = np.ascontiguousarray(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+            X.lib.ref_impli2_eqt_3d(l + 1, ptr(u), ptr(dt), ptr(vx), ptr(mu), ptr(mut), ptr(cfke), ptr(snb), b.nnn,
+                                    ptr(vol), *[ptr(a) for a in t])
+    inner = lay.interior()
+    close((vr - v)[5:7, inner], (vg - v)[5:7, inner])
+    assert np.array_equal(vr[:5], vg[:5])

@@ -42,6 +42,11 @@ def cpu_baseline(case, stages):
                       f"{'flang -O2 build of the reference routines' if use_ref else 'oracle C restatement'}"}
 
 
+# kernels (or launch groups) timed with HIP events on the launch stream
+TIMED_KERNELS = ("implimf_sweep", "implimf_sweep1", "implimf_coef", "implimf_update", "roe_limiter_chain",
+                 "roe_face_flux", "roe_gather", "zvisqc", "chronos", "zpres", "zvismo")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,8 +75,7 @@ def main():
         s.stage()
     torch.cuda.synchronize()
     abi.timing_enable(True)
-    for name in ("implimf_sweep", "roe_cells", "implimf_coef", "zvisqc", "roe_limiter_chain", "implimf_sweep1",
-                 "chronos", "zpres", "implimf_update"):
+    for name in TIMED_KERNELS:
         abi.timing_read(name, reset=True)
     abi.timing_enable(False)
     # timed region (no events inside)
@@ -87,8 +91,7 @@ def main():
         s.stage()
     torch.cuda.synchronize()
     kern = {}
-    for name in ("implimf_sweep", "roe_cells", "implimf_coef", "zvisqc", "roe_limiter_chain", "implimf_sweep1",
-                 "chronos", "zpres", "zvismo", "implimf_update"):
+    for name in TIMED_KERNELS:
         ms, cnt = abi.timing_read(name, reset=True)
         if cnt:
             kern[name] = {"avg_us": 1e3 * ms / cnt, "launches": cnt}

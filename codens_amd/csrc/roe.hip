@@ -119,78 +119,90 @@ __device__ void roe_face(const Phys &ph, double s0, double s1, double s2, const 
 // divisor is the ratio just written for the previous face (reset to 1 when
 // |r(a-1) r(a) r(a+1)| <= tiny).  That recurrence is sequential along a line;
 // the lines, the five variables and the three directions are independent, so
-// one thread runs one (direction, line, variable) chain, the five variables of
-// a line sit in adjacent lanes (sharing the density loads), and the longest
-// lines (direction i) come first in the grid.  Each thread keeps CH_B cells
-// of its line in registers: one memory round trip per CH_B faces.
-constexpr int CH_B = 8;
+// one lane runs one (direction, line, variable) chain.  A workgroup is one
+// wave: one variable of 64 lines, so the variable is uniform in the wave.
+//
+// Faces a = a1+1 .. a2-1 of a line get a ratio, stored at the face's storage
+// index (cell a); the chain reads q at cells a1-1 .. a2.
+constexpr int CH_LINES = 64;
 
-__global__ void __launch_bounds__(256) k_limiter_chain(Blk b, long nt0, long nt1, long nt2,
-                                                       const double *__restrict__ v, long ldv,
-                                                       const double *__restrict__ ps, double *__restrict__ R) {
-  long t = blockIdx.x * 256L + threadIdx.x;
-  int d;
-  if (t < nt0) d = 0;
-  else if (t < nt0 + nt1) { d = 1; t -= nt0; }
-  else if (t < nt0 + nt1 + nt2) { d = 2; t -= nt0 + nt1; }
-  else return;
-  int e = (int)(t % 5);
-  long line = t / 5;
-  int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;  // the two transverse directions
+// primitive variable e (rho, u, v, w, p) at storage index nn: the numerator
+// plane over the density for u, v, w only (x / 1.0 is exact for rho and p)
+__device__ inline const double *chain_num(const double *v, long ldv, const double *ps, int e) {
+  return e == 4 ? ps : v + e * ldv;
+}
+
+struct ChainState {
+  double prev, cur, qc;
+  // one face: qn = q(a+1); returns the ratio stored for face a
+  __device__ inline double step(double qn) {
+    double nxt = qn - qc;
+    double r = fabs(prev * cur * nxt) <= DBL_MIN ? 1. : cur / prev;
+    prev = r;
+    cur = nxt;
+    qc = qn;
+    return r;
+  }
+};
+
+// Lines of direction d: line -> first face a1 storage index.  The line
+// index runs fastest over the first transverse direction (i for d = 1, 2).
+__device__ inline long chain_line_base(const Blk &b, int d, long line) {
+  int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
   int na = b.hi(ta) - b.lo(ta);
   int c[3];
   c[ta] = b.lo(ta) + (int)(line % na);
   c[tb] = b.lo(tb) + (int)(line / na);
+  c[d] = b.lo(d);
+  return b.idx(c[0], c[1], c[2]);
+}
+
+// Directions j and k: adjacent lanes hold lines adjacent in i, so every load
+// and store of the wave is one run of 64 consecutive doubles.  Each lane
+// keeps CH_B cells of its line in registers, with the loads of the next batch
+// in flight while the current batch runs.  blockIdx.x: 64-line group,
+// blockIdx.y: variable, blockIdx.z: direction (1 + z).
+constexpr int CH_B = 8;
+
+__global__ void __launch_bounds__(64) k_limiter_chain_jk(Blk b, const double *__restrict__ v, long ldv,
+                                                         const double *__restrict__ ps, double *__restrict__ R) {
+  const int d = 1 + blockIdx.z, e = blockIdx.y;
+  int ta = 0, tb = d == 2 ? 1 : 2;
+  long nl = (long)(b.hi(ta) - b.lo(ta)) * (b.hi(tb) - b.lo(tb));
+  long line = (long)blockIdx.x * CH_LINES + threadIdx.x;
+  if (line >= nl) return;
   int a1 = b.lo(d), a2 = b.hi(d);
   long inc = b.inc(d);
-  c[d] = a1;
-  long n = b.idx(c[0], c[1], c[2]);  // face a1: cells n-inc | n
-  double *Re = R + (5L * d + e) * b.nnn;
-  // variable e of the primitive state (rho, u, v, w, p) without divergent
-  // branches: numerator plane, and the density as divisor for u, v, w only
-  // (x / 1.0 is exact for rho and p)
-  const double *num = e == 4 ? ps : v + e * ldv;
+  long n = chain_line_base(b, d, line);
+  double *Re = R + (5L * d + e) * b.nnn - b.npc;
+  const double *num = chain_num(v, ldv, ps, e);
   const bool vel = e >= 1 && e <= 3;
-  auto q_at = [&](long nn) {
-    double a = num[nn], r = v[nn];
-    return a / (vel ? r : 1.0);
-  };
-  double qm = q_at(n - inc), q0 = q_at(n), qc = q_at(n + inc);
-  double prev = q0 - qm, cur = qc - q0;
-  // Software pipeline over batches of CH_B faces with two register buffers:
-  // the loads of the next batch are issued before the chain steps and the
-  // stores of the current one (vector-memory counters complete in issue
-  // order, so a store issued before a load would delay that load's use).
+  auto q_at = [&](long nn) { return num[nn] / (vel ? v[nn] : 1.0); };
+  ChainState cs;
+  double qm = q_at(n - inc), q0 = q_at(n);
+  cs.qc = q_at(n + inc);
+  cs.prev = q0 - qm;
+  cs.cur = cs.qc - q0;
   double an[CH_B], ad[CH_B], bn[CH_B], bd[CH_B];
   auto load_batch = [&](int a0, double *xn, double *xd) {
 #pragma unroll
     for (int s = 0; s < CH_B; s++) {  // unconditional, clamped to the line
       long nn = n + (long)(min(a0 + s, a2 - 1) - a1 + 1) * inc;
       xn[s] = num[nn];
-      xd[s] = v[nn];
+      xd[s] = vel ? v[nn] : 1.0;
     }
   };
   auto run_batch = [&](int a0, const double *xn, const double *xd) {
-    double rr[CH_B];
 #pragma unroll
     for (int s = 0; s < CH_B; s++) {
-      double qn = xn[s] / (vel ? xd[s] : 1.0);
-      double nxt = qn - qc;
-      double r = fabs(prev * cur * nxt) <= DBL_MIN ? 1. : cur / prev;
-      bool in = a0 + s <= a2 - 1;
-      rr[s] = r;
-      prev = in ? r : prev;
-      cur = in ? nxt : cur;
-      qc = in ? qn : qc;
+      if (a0 + s > a2 - 1) break;
+      Re[n + (long)(a0 + s - a1) * inc] = cs.step(xn[s] / xd[s]);
     }
-#pragma unroll
-    for (int s = 0; s < CH_B; s++)
-      if (a0 + s <= a2 - 1) Re[n + (long)(a0 + s - a1) * inc - b.npc] = rr[s];
   };
   int a0 = a1 + 1;
   if (a0 <= a2 - 1) load_batch(a0, an, ad);
   while (a0 <= a2 - 1) {
-    load_batch(a0 + CH_B, bn, bd);  // unconditional: clamped, and exact waits
+    load_batch(a0 + CH_B, bn, bd);
     __builtin_amdgcn_sched_barrier(0);
     run_batch(a0, an, ad);
     a0 += CH_B;
@@ -199,6 +211,77 @@ __global__ void __launch_bounds__(256) k_limiter_chain(Blk b, long nt0, long nt1
     __builtin_amdgcn_sched_barrier(0);
     run_batch(a0, bn, bd);
     a0 += CH_B;
+  }
+}
+
+// Direction i: a lane's line is contiguous in memory, so lanes walking their
+// own lines would touch 64 cache lines per access.  The wave instead moves
+// chunks of CH_C cells of its 64 lines through LDS: coalesced loads (each
+// run of CH_C consecutive cells of one line) give q, the lanes run their
+// chains over the chunk in LDS (ratios written in place), and coalesced
+// stores write the ratios back.  q = num / rho is formed at load time, the
+// same operation the chain would do.  blockIdx.x: 64-line group,
+// blockIdx.y: variable.
+constexpr int CH_C = 32;
+
+__global__ void __launch_bounds__(64) k_limiter_chain_i(Blk b, const double *__restrict__ v, long ldv,
+                                                        const double *__restrict__ ps, double *__restrict__ R) {
+  __shared__ double qs[CH_LINES * (CH_C + 1)];
+  __shared__ long base[CH_LINES];
+  const int e = blockIdx.y, lane = threadIdx.x;
+  long nl = (long)(b.j2 - b.j1) * (b.k2 - b.k1);
+  long line0 = (long)blockIdx.x * CH_LINES;
+  int nlines = (int)min((long)CH_LINES, nl - line0);
+  int a1 = b.i1, a2 = b.i2;
+  const double *num = chain_num(v, ldv, ps, e);
+  const bool vel = e >= 1 && e <= 3;
+  auto q_at = [&](long nn) { return num[nn] / (vel ? v[nn] : 1.0); };
+  // each lane's own line (lanes past the last line mirror the last one and
+  // neither run a chain nor store)
+  const bool own = lane < nlines;
+  long n = chain_line_base(b, 0, line0 + min(lane, nlines - 1));
+  base[lane] = n;
+  ChainState cs;
+  {
+    double qm = q_at(n - 1), q0 = q_at(n);
+    cs.qc = q_at(n + 1);
+    cs.prev = q0 - qm;
+    cs.cur = cs.qc - q0;
+  }
+  double *Re = R + (long)e * b.nnn - b.npc;
+  __syncthreads();
+  for (int a0 = a1 + 1; a0 <= a2 - 1; a0 += CH_C) {
+    int nc = min(CH_C, a2 - a0);  // faces a0 .. a0+nc-1 of this chunk
+    // q at cells a0+1 .. a0+nc of every line; pass it covers lines
+    // it*(64/CH_C) + lane/CH_C, cell lane%CH_C.  All passes' loads are issued
+    // before the first LDS write (unrolled), clamped to the chunk.
+    constexpr int LPP = 64 / CH_C, NPASS = CH_LINES / LPP;
+    const int c = lane % CH_C, lsub = lane / CH_C;
+    const int cc = min(c, nc - 1);
+    double qv[NPASS];
+#pragma unroll
+    for (int it = 0; it < NPASS; it++) {
+      int l = min(it * LPP + lsub, nlines - 1);
+      qv[it] = q_at(base[l] + (a0 + 1 + cc - a1));
+    }
+#pragma unroll
+    for (int it = 0; it < NPASS; it++) qs[(it * LPP + lsub) * (CH_C + 1) + c] = qv[it];
+    __syncthreads();
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < CH_C; k++) {
+        if (k >= nc) break;
+        double *slot = &qs[lane * (CH_C + 1) + k];
+        *slot = cs.step(*slot);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NPASS; it++) {
+      int l = it * LPP + lsub;
+      if (l < nlines && c < nc) Re[base[l] + (a0 + c - a1)] = qs[l * (CH_C + 1) + c];
+    }
+    __syncthreads();
   }
 }
 
@@ -230,18 +313,29 @@ __device__ inline void ghost_flux(const Phys &ph, const double w[5], double p, c
 }
 
 // ---------------------------------------------------------------------------
-// pass 2: the half face flux H of the face of direction d whose storage
-// index is n (cells n-inc | n, along-direction coordinate a): the physical
-// flux of the ghost state on the two boundary faces (sch_roe.f90:497-578),
-// else 0.5 x the Roe flux of the MUSCL states (sch_roe.f90:196-474).
-__device__ inline void face_h(const Blk &b, const Phys &ph, int d, int ns, int ilim, double muscl, double xk,
-                              const double *__restrict__ v, long ldv, const double *__restrict__ ps,
-                              const double *__restrict__ tq, const double *__restrict__ sn,
-                              const double *__restrict__ rr, long n, int a, double f[5]) {
-  int a1 = b.lo(d), a2 = b.hi(d);
+// pass 2: the half face fluxes H of direction d, one thread per face: the
+// physical flux of the ghost state on the two boundary faces
+// (sch_roe.f90:497-578), else 0.5 x the Roe flux of the MUSCL states
+// (sch_roe.f90:196-474).  H[e*nnn + m] holds the face whose storage index is
+// m (cells m-inc | m).
+__global__ void __launch_bounds__(TX *TY) k_face_flux(Blk b, Phys ph, int d, int ns, int ilim, double muscl,
+                                                      double xk, const double *__restrict__ v, long ldv,
+                                                      const double *__restrict__ ps,
+                                                      const double *__restrict__ tq,
+                                                      const double *__restrict__ sn,
+                                                      const double *__restrict__ rr, double *__restrict__ H) {
+  const dim3 xb = xcd_block();
+  int c[3];
+  c[0] = b.i1 + xb.x * TX + threadIdx.x;
+  c[1] = b.j1 + xb.y * TY + threadIdx.y;
+  c[2] = b.k1 + xb.z;
+  int hi0 = b.i2 - 1 + (d == 0), hi1 = b.j2 - 1 + (d == 1);
+  if (c[0] > hi0 || c[1] > hi1) return;
+  int a = c[d], a1 = b.lo(d), a2 = b.hi(d);
   long inc = b.inc(d);
-  long m = n - b.npc;
+  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
   double s0 = snv(sn, b.nnn, m, d, 0), s1 = snv(sn, b.nnn, m, d, 1), s2 = snv(sn, b.nnn, m, d, 2);
+  double f[5];
   if (a == a1 || a == a2) {
     long ng = a == a1 ? n - inc : n;
     double w[5], T[9];
@@ -251,90 +345,70 @@ __device__ inline void face_h(const Blk &b, const Phys &ph, int d, int ns, int i
     for (int q = 0; q < 9; q++) T[q] = ns ? tq[q * ldv + ng] : 0.;
     double rdiv = (!ns && a == a1 && d > 0) ? v[n] : w[0];  // sch_roe_euler.f90:630-632, 912-914
     ghost_flux(ph, w, ps[ng], T, rdiv, !ns, s0, s1, s2, f);
-    return;
-  }
-  Prim q[4];
-#pragma unroll
-  for (int s = 0; s < 4; s++) q[s] = prim_at(v, ldv, ps, n + (s - 2) * inc);
-  double L[5], R[5];
-  double cm = 0.25 * muscl;
-  if (ilim == 1 && a <= a2 - 2) {
-#pragma unroll
-    for (int e = 0; e < 5; e++) {
-      double rl = rr[e * b.nnn + m], rh = rr[e * b.nnn + m + inc];
-      double qa = pget(q[0], e), qb = pget(q[1], e), qc = pget(q[2], e), qd = pget(q[3], e);
-      L[e] = qb + cm * minmod_phi(rl) * (1. - xk) * (qb - qa) + cm * minmod_phi(1. / rl) * (1. + xk) * (qc - qb);
-      R[e] = qc - cm * minmod_phi(rh) * (1. + xk) * (qc - qb) - cm * minmod_phi(1. / rh) * (1. - xk) * (qd - qc);
-    }
   } else {
+    Prim q[4];
 #pragma unroll
-    for (int e = 0; e < 5; e++) {
-      double qa = pget(q[0], e), qb = pget(q[1], e), qc = pget(q[2], e), qd = pget(q[3], e);
-      L[e] = qb + cm * ((1. - xk) * (qb - qa) + (1. + xk) * (qc - qb));
-      R[e] = qc - cm * ((1. + xk) * (qc - qb) + (1. - xk) * (qd - qc));
+    for (int s = 0; s < 4; s++) q[s] = prim_at(v, ldv, ps, n + (s - 2) * inc);
+    double L[5], R[5];
+    double cm = 0.25 * muscl;
+    if (ilim == 1 && a <= a2 - 2) {
+#pragma unroll
+      for (int e = 0; e < 5; e++) {
+        double rl = rr[e * b.nnn + m], rh = rr[e * b.nnn + m + inc];
+        double qa = pget(q[0], e), qb = pget(q[1], e), qc = pget(q[2], e), qd = pget(q[3], e);
+        L[e] = qb + cm * minmod_phi(rl) * (1. - xk) * (qb - qa) + cm * minmod_phi(1. / rl) * (1. + xk) * (qc - qb);
+        R[e] = qc - cm * minmod_phi(rh) * (1. + xk) * (qc - qb) - cm * minmod_phi(1. / rh) * (1. - xk) * (qd - qc);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 5; e++) {
+        double qa = pget(q[0], e), qb = pget(q[1], e), qc = pget(q[2], e), qd = pget(q[3], e);
+        L[e] = qb + cm * ((1. - xk) * (qb - qa) + (1. + xk) * (qc - qb));
+        R[e] = qc - cm * ((1. + xk) * (qc - qb) + (1. - xk) * (qd - qc));
+      }
     }
-  }
-  double tl[9], tr[9];
+    double tl[9], tr[9];
 #pragma unroll
-  for (int q2 = 0; q2 < 9; q2++) {
-    tl[q2] = ns ? tq[q2 * ldv + n - inc] : 0.;
-    tr[q2] = ns ? tq[q2 * ldv + n] : 0.;
-  }
-  roe_face(ph, s0, s1, s2, L, R, tl, tr, f);
+    for (int q2 = 0; q2 < 9; q2++) {
+      tl[q2] = ns ? tq[q2 * ldv + n - inc] : 0.;
+      tr[q2] = ns ? tq[q2 * ldv + n] : 0.;
+    }
+    roe_face(ph, s0, s1, s2, L, R, tl, tr, f);
 #pragma unroll
-  for (int e = 0; e < 5; e++) f[e] *= 0.5;
+    for (int e = 0; e < 5; e++) f[e] *= 0.5;
+  }
+#pragma unroll
+  for (int e = 0; e < 5; e++) H[e * b.nnn + m] = f[e];
 }
 
-// pass 3 (fused with pass 2): the residual of a real cell from the half
-// fluxes of its six faces, summed direction by direction as
-// u = (H_i(hi) - H_i(lo)) + (H_j(hi) - H_j(lo)) + (H_k(hi) - H_k(lo))
-// (sch_roe.f90:448-474 scatter, regrouped per cell).  Every interior face is
-// evaluated by both of its cells, with identical inputs, so both see the same
-// value: no face-flux array goes through memory.  ff -= u when ityprk != 0.
-__global__ void __launch_bounds__(TX *TY) k_roe_cells(Blk b, Phys ph, int ns, int ilim, double muscl, double xk,
-                                                      const double *__restrict__ v, long ldv,
-                                                      const double *__restrict__ ps,
-                                                      const double *__restrict__ tq,
-                                                      const double *__restrict__ sn,
-                                                      const double *__restrict__ R, double *__restrict__ u,
-                                                      double *__restrict__ ff, int ityprk) {
-  const dim3 xb = xcd_block();
-  int c[3];
-  c[0] = b.i1 + xb.x * TX + threadIdx.x;
-  c[1] = b.j1 + xb.y * TY + threadIdx.y;
-  c[2] = b.k1 + xb.z;
-  if (c[0] >= b.i2 || c[1] >= b.j2) return;
-  long n = b.idx(c[0], c[1], c[2]);
-  double r[5];
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    long inc = b.inc(d);
-    const double *rr = R ? R + 5 * b.nnn * d : nullptr;
-    double fl[5], fh[5];
-    face_h(b, ph, d, ns, ilim, muscl, xk, v, ldv, ps, tq, sn, rr, n + inc, c[d] + 1, fh);
-    face_h(b, ph, d, ns, ilim, muscl, xk, v, ldv, ps, tq, sn, rr, n, c[d], fl);
-#pragma unroll
-    for (int e = 0; e < 5; e++) r[e] = d == 0 ? fh[e] - fl[e] : r[e] + (fh[e] - fl[e]);
-  }
-#pragma unroll
-  for (int e = 0; e < 5; e++) {
-    u[e * ldv + n] = r[e];
-    if (ityprk) ff[e * ldv + n] = ff[e * ldv + n] - r[e];
-  }
-}
-
-// u = 0 on the non-real entries of the zeroed range indc(i1-1,j1-1,k1-1) ..
-// indc(i2,j2,k2) (sch_roe.f90:113-125)
-__global__ void __launch_bounds__(256) k_zero_ghost_u(Blk b, long n0, long n1, double *u, long ldv) {
+// ---------------------------------------------------------------------------
+// pass 3: gather over the zeroed range indc(i1-1,j1-1,k1-1)..indc(i2,j2,k2)
+// (sch_roe.f90:113-125, 448-474 regrouped per cell):
+// u = (H_i(hi) - H_i(lo)) + (H_j(hi) - H_j(lo)) + (H_k(hi) - H_k(lo)) on
+// real cells, 0 elsewhere; ff -= u when ityprk != 0.
+__global__ void __launch_bounds__(256) k_gather(Blk b, long n0, long n1, const double *__restrict__ H, double *u,
+                                                long ldv, double *ff, int ityprk) {
   long n = n0 + blockIdx.x * 256L + threadIdx.x;
   if (n > n1) return;
   long m = n - b.npc;
   int i = b.id1 + (int)(m % b.nid);
   int j = b.jd1 + (int)((m / b.nid) % (b.nijd / b.nid));
   int k = b.kd1 + (int)(m / b.nijd);
-  if (b.real(i, j, k)) return;
+  const double *H0 = H, *H1 = H + 5 * b.nnn, *H2 = H + 10 * b.nnn;
+  if (!b.real(i, j, k)) {
 #pragma unroll
-  for (int e = 0; e < 5; e++) u[e * ldv + n] = 0.;
+    for (int e = 0; e < 5; e++) u[e * ldv + n] = 0.;
+    return;
+  }
+#pragma unroll
+  for (int e = 0; e < 5; e++) {
+    long o = e * b.nnn + m;
+    double r = H0[o + 1] - H0[o];
+    r = r + (H1[o + b.nid] - H1[o]);
+    r = r + (H2[o + b.nijd] - H2[o]);
+    u[e * ldv + n] = r;
+    if (ityprk) ff[e * ldv + n] = ff[e * ldv + n] - r;
+  }
 }
 
 static int roe_common(const codens_block *cb, const codens_phys *cp, const codens_scheme *sc, int ns,
@@ -349,29 +423,34 @@ static int roe_common(const codens_block *cb, const codens_phys *cp, const coden
   Blk b = make_blk(cb);
   Phys ph = make_phys(cp);
   int ilim = ns ? sc->ilim : 0;
+  double *H = (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 0);
   double *R = ilim == 1 ? (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 1) : nullptr;
-  if (ilim == 1 && !R) {
+  if (!H || (ilim == 1 && !R)) {
     codens_set_error("%s: workspace allocation failed", what);
     return CODENS_EHIP;
   }
   if (ilim == 1) {
-    long nt[3];
-    for (int d = 0; d < 3; d++) {
-      int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
-      nt[d] = 5L * (b.hi(ta) - b.lo(ta)) * (b.hi(tb) - b.lo(tb));
-    }
-    long ntot = nt[0] + nt[1] + nt[2];
     CodensTimed _tm("roe_limiter_chain", st);
-    hipLaunchKernelGGL(k_limiter_chain, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st, b, nt[0], nt[1], nt[2],
+    long nl0 = (long)(b.j2 - b.j1) * (b.k2 - b.k1);
+    long nl12 = (long)(b.i2 - b.i1) * max(b.k2 - b.k1, b.j2 - b.j1);
+    hipLaunchKernelGGL(k_limiter_chain_i, dim3((unsigned)((nl0 + CH_LINES - 1) / CH_LINES), 5), dim3(64), 0, st, b,
                        v, ldv, ps, R);
+    hipLaunchKernelGGL(k_limiter_chain_jk, dim3((unsigned)((nl12 + CH_LINES - 1) / CH_LINES), 5, 2), dim3(64), 0, st,
+                       b, v, ldv, ps, R);
   }
-  {
-    CodensTimed _tm("roe_cells", st);
-    hipLaunchKernelGGL(k_roe_cells, cell_grid(b), dim3(TX, TY), 0, st, b, ph, ns, ilim, (double)sc->muscl, sc->xk, v,
-                       ldv, ps, ns ? tq : v, sn, ilim == 1 ? R : nullptr, u, ityprk ? ff : u, (int)ityprk);
+  for (int d = 0; d < 3; d++) {
+    int nx = b.i2 - b.i1 + (d == 0), ny = b.j2 - b.j1 + (d == 1), nz = b.k2 - b.k1 + (d == 2);
+    dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz);
+    CodensTimed _tm("roe_face_flux", st);
+    hipLaunchKernelGGL(k_face_flux, g, dim3(TX, TY), 0, st, b, ph, d, ns, ilim, (double)sc->muscl, sc->xk, v, ldv,
+                       ps, ns ? tq : v, sn, ilim == 1 ? R + 5 * b.nnn * d : nullptr, H + 5 * b.nnn * d);
   }
   long n0 = b.idx(b.i1 - 1, b.j1 - 1, b.k1 - 1), n1 = b.idx(b.i2, b.j2, b.k2);
-  hipLaunchKernelGGL(k_zero_ghost_u, dim3((unsigned)((n1 - n0 + 256) / 256)), dim3(256), 0, st, b, n0, n1, u, ldv);
+  {
+    CodensTimed _tm("roe_gather", st);
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((n1 - n0 + 256) / 256)), dim3(256), 0, st, b, n0, n1, H, u, ldv,
+                       ff, ityprk);
+  }
   return codens_check_launch(what);
 }
 

@@ -61,6 +61,7 @@ contains
     ip13 = nip11
     ip21 = nip11
     ip31 = 3*nip11
+    ip42 = nip00
     ip60 = 7
   end subroutine ref_set_layout
 

@@ -241,7 +241,8 @@ def test_turbulence_routines(tsetup):
         r_, g_ = _both(R, G, "ref_met_smkor", l + 1, D(0.65), v, fd, g, tp, *outs())
         for x, y in zip(r_[4:7], g_[4:7]):
             close(x[inner], y[inner])
-        mnpar = rng.integers(0, 5, n).astype(np.int32); utau = np.array([1e-5, -3e-3, 2e-2, 4e-4])
+        mnpar = rng.integers(0, 5, n).astype(np.int32)
+        utau = np.zeros(lay.ip00); utau[:4] = [1e-5, -3e-3, 2e-2, 4e-4]  # utau(ip42), ip42 = ip00
         r_, g_ = _both(R, G, "ref_met_smch", l + 1, D(1.35), D(1.8), D(2e-4), v, mu, dist, mnpar, utau, tp,
                        np.zeros(n), np.zeros(n))
         for x, y in zip(r_[6:8], g_[6:8]):

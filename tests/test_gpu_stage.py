@@ -121,12 +121,15 @@ def test_wall_rans_stages(model, klroe, kwsst):
     atsch_num.f90:543-621 orders them) on the wall block with SA or k-omega:
     4 stages against the oracle, state (7 planes) and eddy viscosity within
     1e-10.  Unlimited MUSCL for the mean flow, as in the laminar multi-stage
-    test."""
+    test.  With klroe = 1 the reference's met_roe reads every entry one
+    storage slot early (met_roe.f90:184-188, kept by both sides), which makes
+    this k-omega case grow without bound (k x 4e4 after 2 stages, NaN after 4
+    in both): that case is compared after 2 stages, while still finite."""
     from codens_amd import configs
     import cpu_stage
     case = configs.wall_rans(24, 16, 8, model=model, klroe=klroe, kwsst=kwsst, ilim=0, lmax=4)
     g, c = _solver(case), cpu_stage.CpuStage(case)
-    for _ in range(4):
+    for _ in range(4 if klroe == 2 else 2):
         g.stage(); c.stage()
     vg = g.host_state()
     inner = case.layout.interior()

@@ -28,6 +28,7 @@ struct Blk {
 
 struct Phys {
   double gam, gam1, pinfl, pr, prt, cp, reynz, tnz;
+  double igam1;  // 1 / gam1
   int iflu;
 };
 
@@ -46,6 +47,7 @@ inline Phys make_phys(const codens_phys *p) {
   Phys q;
   q.gam = p->gam; q.gam1 = p->gam1; q.pinfl = p->pinfl; q.pr = p->pr; q.prt = p->prt;
   q.cp = p->cp; q.reynz = p->reynz; q.tnz = p->tnz; q.iflu = p->iflu;
+  q.igam1 = 1. / p->gam1;
   return q;
 }
 

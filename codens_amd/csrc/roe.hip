@@ -1,7 +1,9 @@
 // Roe-MUSCL residual (sch_roe / sch_roe_euler) on gfx950.
 //
-// Three passes per call:
-//  1. (ilim=1 only) slope-ratio chains: one thread per grid line marches the
+// Passes per call:
+//  0. primitive velocities u = (rho u)/rho, v, w of every cell of the block,
+//     once (the flux passes read each cell up to 12 times);
+//  1. (ilim=1 only) slope-ratio chains: one lane per grid line marches the
 //     faces of its line in storage order, because the reference replaces each
 //     slope difference by its ratio in place, so a face's ratio divides by the
 //     ratio (not the difference) of the face before it (sch_roe.f90:161-194).
@@ -13,6 +15,9 @@
 //     reference does (sch_roe.f90:110-125).
 // The dissipation |A| dW is evaluated as P (|Lambda| (Q dW)), the same
 // matrices as the reference's P |Lambda| Q product with a quarter of the work.
+// The Roe average multiplies by reciprocals of rho_L, rho_R, |S|, c and
+// 1/(gamma-1) where the reference divides (at most one rounding apart per
+// operation).
 #include "common.hpp"
 
 using namespace codens;
@@ -21,14 +26,30 @@ struct Prim {
   double r, u, v, w, p;
 };
 
-__device__ inline Prim prim_at(const double *__restrict__ v, long ldv, const double *__restrict__ ps, long n) {
+// primitive state of the cell at storage index n; uvw: the velocity planes of
+// pass 0 (stride nnn, block-local index m = n - npc)
+__device__ inline Prim prim_at(const double *__restrict__ v, const double *__restrict__ uvw, long nnn, long npc,
+                               const double *__restrict__ ps, long n) {
   Prim q;
+  long m = n - npc;
   q.r = v[n];
-  q.u = v[ldv + n] / q.r;
-  q.v = v[2 * ldv + n] / q.r;
-  q.w = v[3 * ldv + n] / q.r;
+  q.u = uvw[m];
+  q.v = uvw[nnn + m];
+  q.w = uvw[2 * nnn + m];
   q.p = ps[n];
   return q;
+}
+
+// pass 0: uvw(e-1, m) = v(e, n) / v(1, n), e = 2..4, over the block's storage
+__global__ void __launch_bounds__(256) k_prim_uvw(Blk b, const double *__restrict__ v, long ldv,
+                                                  double *__restrict__ uvw) {
+  long m = blockIdx.x * 256L + threadIdx.x;
+  if (m >= b.nnn) return;
+  long n = b.npc + m;
+  double r = v[n];
+  uvw[m] = v[ldv + n] / r;
+  uvw[b.nnn + m] = v[2 * ldv + n] / r;
+  uvw[2 * b.nnn + m] = v[3 * ldv + n] / r;
 }
 
 __device__ inline double pget(const Prim &q, int e) {
@@ -45,22 +66,24 @@ __device__ void roe_face(const Phys &ph, double s0, double s1, double s2, const 
   const double gam = ph.gam, gam1 = ph.gam1, pinfl = ph.pinfl;
   double rhol = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
   double rhor = R[0], ur = R[1], vr = R[2], wr = R[3], prr = R[4];
-  double cnds = sqrt(s0 * s0 + s1 * s1 + s2 * s2);
-  double nx = s0 / cnds, ny = s1 / cnds, nz = s2 / cnds;
-  double al2 = gam * pl / rhol, ar2 = gam * prr / rhor;
+  const double igam1 = ph.igam1;
+  double cnds = sqrt(s0 * s0 + s1 * s1 + s2 * s2), icn = 1. / cnds;
+  double nx = s0 * icn, ny = s1 * icn, nz = s2 * icn;
+  double irl = 1. / rhol, irr = 1. / rhor;
+  double al2 = gam * pl * irl, ar2 = gam * prr * irr;
   double q2l = ul * ul + vl * vl + wl * wl, q2r = ur * ur + vr * vr + wr * wr;
-  double hl = al2 / gam1 + 0.5 * q2l, hr = ar2 / gam1 + 0.5 * q2r;
-  double el = pl / (gam1 * rhol) + 0.5 * q2l + pinfl / rhol;
-  double er = prr / (gam1 * rhor) + 0.5 * q2r + pinfl / rhor;
-  double gd = sqrt(rhor / rhol), gd1 = 1. / (1. + gd), gd2 = gd * gd1;
+  double hl = al2 * igam1 + 0.5 * q2l, hr = ar2 * igam1 + 0.5 * q2r;
+  double el = (pl * igam1 + pinfl) * irl + 0.5 * q2l;
+  double er = (prr * igam1 + pinfl) * irr + 0.5 * q2r;
+  double gd = sqrt(rhor * irl), gd1 = 1. / (1. + gd), gd2 = gd * gd1;
   double rhom = sqrt(rhol * rhor), rhomi = 1. / rhom;
   double um = gd1 * ul + gd2 * ur, vm = gd1 * vl + gd2 * vr, wm = gd1 * wl + gd2 * wr;
   double hm = gd1 * hl + gd2 * hr;
   double vitm2 = 0.5 * (um * um + vm * vm + wm * wm);
-  double am = sqrt(gam1 * (hm - vitm2));
-  double am2i = 1. / (am * am);
+  double am = sqrt(gam1 * (hm - vitm2)), ami = 1. / am;
+  double am2i = ami * ami;
   double vn = um * nx + vm * ny + wm * nz;
-  double rhoiam = rhom / am, rhoami = am2i / rhoiam;
+  double rhoiam = rhom * ami, rhoami = ami * rhomi;
   double l1 = fabs(vn * cnds), l4 = fabs(vn * cnds + am * cnds), l5 = fabs(vn * cnds - am * cnds);
   double dw0 = rhor - rhol;
   double dw1 = rhor * ur - rhol * ul;
@@ -126,10 +149,9 @@ __device__ void roe_face(const Phys &ph, double s0, double s1, double s2, const 
 // index (cell a); the chain reads q at cells a1-1 .. a2.
 constexpr int CH_LINES = 64;
 
-// primitive variable e (rho, u, v, w, p) at storage index nn: the numerator
-// plane over the density for u, v, w only (x / 1.0 is exact for rho and p)
-__device__ inline const double *chain_num(const double *v, long ldv, const double *ps, int e) {
-  return e == 4 ? ps : v + e * ldv;
+// plane of primitive variable e (rho, u, v, w, p), indexed by storage index
+__device__ inline const double *chain_q(const Blk &b, const double *v, const double *uvw, const double *ps, int e) {
+  return e == 0 ? v : (e == 4 ? ps : uvw + (long)(e - 1) * b.nnn - b.npc);
 }
 
 struct ChainState {
@@ -164,7 +186,8 @@ __device__ inline long chain_line_base(const Blk &b, int d, long line) {
 // blockIdx.y: variable, blockIdx.z: direction (1 + z).
 constexpr int CH_B = 8;
 
-__global__ void __launch_bounds__(64) k_limiter_chain_jk(Blk b, const double *__restrict__ v, long ldv,
+__global__ void __launch_bounds__(64) k_limiter_chain_jk(Blk b, const double *__restrict__ v,
+                                                         const double *__restrict__ uvw,
                                                          const double *__restrict__ ps, double *__restrict__ R) {
   const int d = 1 + blockIdx.z, e = blockIdx.y;
   int ta = 0, tb = d == 2 ? 1 : 2;
@@ -175,41 +198,36 @@ __global__ void __launch_bounds__(64) k_limiter_chain_jk(Blk b, const double *__
   long inc = b.inc(d);
   long n = chain_line_base(b, d, line);
   double *Re = R + (5L * d + e) * b.nnn - b.npc;
-  const double *num = chain_num(v, ldv, ps, e);
-  const bool vel = e >= 1 && e <= 3;
-  auto q_at = [&](long nn) { return num[nn] / (vel ? v[nn] : 1.0); };
+  const double *q = chain_q(b, v, uvw, ps, e);
   ChainState cs;
-  double qm = q_at(n - inc), q0 = q_at(n);
-  cs.qc = q_at(n + inc);
+  double qm = q[n - inc], q0 = q[n];
+  cs.qc = q[n + inc];
   cs.prev = q0 - qm;
   cs.cur = cs.qc - q0;
-  double an[CH_B], ad[CH_B], bn[CH_B], bd[CH_B];
-  auto load_batch = [&](int a0, double *xn, double *xd) {
+  double xa[CH_B], xb[CH_B];
+  auto load_batch = [&](int a0, double *x) {
 #pragma unroll
-    for (int s = 0; s < CH_B; s++) {  // unconditional, clamped to the line
-      long nn = n + (long)(min(a0 + s, a2 - 1) - a1 + 1) * inc;
-      xn[s] = num[nn];
-      xd[s] = vel ? v[nn] : 1.0;
-    }
+    for (int s = 0; s < CH_B; s++)  // unconditional, clamped to the line
+      x[s] = q[n + (long)(min(a0 + s, a2 - 1) - a1 + 1) * inc];
   };
-  auto run_batch = [&](int a0, const double *xn, const double *xd) {
+  auto run_batch = [&](int a0, const double *x) {
 #pragma unroll
     for (int s = 0; s < CH_B; s++) {
       if (a0 + s > a2 - 1) break;
-      Re[n + (long)(a0 + s - a1) * inc] = cs.step(xn[s] / xd[s]);
+      Re[n + (long)(a0 + s - a1) * inc] = cs.step(x[s]);
     }
   };
   int a0 = a1 + 1;
-  if (a0 <= a2 - 1) load_batch(a0, an, ad);
+  if (a0 <= a2 - 1) load_batch(a0, xa);
   while (a0 <= a2 - 1) {
-    load_batch(a0 + CH_B, bn, bd);
+    load_batch(a0 + CH_B, xb);
     __builtin_amdgcn_sched_barrier(0);
-    run_batch(a0, an, ad);
+    run_batch(a0, xa);
     a0 += CH_B;
     if (a0 > a2 - 1) break;
-    load_batch(a0 + CH_B, an, ad);
+    load_batch(a0 + CH_B, xa);
     __builtin_amdgcn_sched_barrier(0);
-    run_batch(a0, bn, bd);
+    run_batch(a0, xb);
     a0 += CH_B;
   }
 }
@@ -219,12 +237,12 @@ __global__ void __launch_bounds__(64) k_limiter_chain_jk(Blk b, const double *__
 // chunks of CH_C cells of its 64 lines through LDS: coalesced loads (each
 // run of CH_C consecutive cells of one line) give q, the lanes run their
 // chains over the chunk in LDS (ratios written in place), and coalesced
-// stores write the ratios back.  q = num / rho is formed at load time, the
-// same operation the chain would do.  blockIdx.x: 64-line group,
+// stores write the ratios back.  blockIdx.x: 64-line group,
 // blockIdx.y: variable.
 constexpr int CH_C = 32;
 
-__global__ void __launch_bounds__(64) k_limiter_chain_i(Blk b, const double *__restrict__ v, long ldv,
+__global__ void __launch_bounds__(64) k_limiter_chain_i(Blk b, const double *__restrict__ v,
+                                                        const double *__restrict__ uvw,
                                                         const double *__restrict__ ps, double *__restrict__ R) {
   __shared__ double qs[CH_LINES * (CH_C + 1)];
   __shared__ long base[CH_LINES];
@@ -233,9 +251,7 @@ __global__ void __launch_bounds__(64) k_limiter_chain_i(Blk b, const double *__r
   long line0 = (long)blockIdx.x * CH_LINES;
   int nlines = (int)min((long)CH_LINES, nl - line0);
   int a1 = b.i1, a2 = b.i2;
-  const double *num = chain_num(v, ldv, ps, e);
-  const bool vel = e >= 1 && e <= 3;
-  auto q_at = [&](long nn) { return num[nn] / (vel ? v[nn] : 1.0); };
+  const double *q = chain_q(b, v, uvw, ps, e);
   // each lane's own line (lanes past the last line mirror the last one and
   // neither run a chain nor store)
   const bool own = lane < nlines;
@@ -243,8 +259,8 @@ __global__ void __launch_bounds__(64) k_limiter_chain_i(Blk b, const double *__r
   base[lane] = n;
   ChainState cs;
   {
-    double qm = q_at(n - 1), q0 = q_at(n);
-    cs.qc = q_at(n + 1);
+    double qm = q[n - 1], q0 = q[n];
+    cs.qc = q[n + 1];
     cs.prev = q0 - qm;
     cs.cur = cs.qc - q0;
   }
@@ -262,7 +278,7 @@ __global__ void __launch_bounds__(64) k_limiter_chain_i(Blk b, const double *__r
 #pragma unroll
     for (int it = 0; it < NPASS; it++) {
       int l = min(it * LPP + lsub, nlines - 1);
-      qv[it] = q_at(base[l] + (a0 + 1 + cc - a1));
+      qv[it] = q[base[l] + (a0 + 1 + cc - a1)];
     }
 #pragma unroll
     for (int it = 0; it < NPASS; it++) qs[(it * LPP + lsub) * (CH_C + 1) + c] = qv[it];
@@ -312,17 +328,24 @@ __device__ inline void ghost_flux(const Phys &ph, const double w[5], double p, c
   f[4] = fex * s0 + fey * s1 + fez * s2;
 }
 
+// Minimum waves per SIMD the face-flux kernel is compiled for (its register
+// budget: 3 -> up to 168 VGPRs, 4 -> 128).
+#ifndef CODENS_FF_WAVES
+#define CODENS_FF_WAVES 3
+#endif
+
 // ---------------------------------------------------------------------------
 // pass 2: the half face fluxes H of direction d, one thread per face: the
 // physical flux of the ghost state on the two boundary faces
 // (sch_roe.f90:497-578), else 0.5 x the Roe flux of the MUSCL states
 // (sch_roe.f90:196-474).  H[e*nnn + m] holds the face whose storage index is
 // m (cells m-inc | m).
-__global__ void __launch_bounds__(TX *TY) k_face_flux(Blk b, Phys ph, int d, int ns, int ilim, double muscl,
+__global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(CODENS_FF_WAVES))) k_face_flux(Blk b, Phys ph, int d, int ns, int ilim, double muscl,
                                                       double xk, const double *__restrict__ v, long ldv,
                                                       const double *__restrict__ ps,
                                                       const double *__restrict__ tq,
                                                       const double *__restrict__ sn,
+                                                      const double *__restrict__ uvw,
                                                       const double *__restrict__ rr, double *__restrict__ H) {
   const dim3 xb = xcd_block();
   int c[3];
@@ -348,7 +371,7 @@ __global__ void __launch_bounds__(TX *TY) k_face_flux(Blk b, Phys ph, int d, int
   } else {
     Prim q[4];
 #pragma unroll
-    for (int s = 0; s < 4; s++) q[s] = prim_at(v, ldv, ps, n + (s - 2) * inc);
+    for (int s = 0; s < 4; s++) q[s] = prim_at(v, uvw, b.nnn, b.npc, ps, n + (s - 2) * inc);
     double L[5], R[5];
     double cm = 0.25 * muscl;
     if (ilim == 1 && a <= a2 - 2) {
@@ -425,25 +448,30 @@ static int roe_common(const codens_block *cb, const codens_phys *cp, const coden
   int ilim = ns ? sc->ilim : 0;
   double *H = (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 0);
   double *R = ilim == 1 ? (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 1) : nullptr;
-  if (!H || (ilim == 1 && !R)) {
+  double *uvw = (double *)codens_workspace(sizeof(double) * 3 * b.nnn, 4);
+  if (!H || !uvw || (ilim == 1 && !R)) {
     codens_set_error("%s: workspace allocation failed", what);
     return CODENS_EHIP;
+  }
+  {
+    CodensTimed _tm("roe_prim", st);
+    hipLaunchKernelGGL(k_prim_uvw, dim3((unsigned)((b.nnn + 255) / 256)), dim3(256), 0, st, b, v, ldv, uvw);
   }
   if (ilim == 1) {
     CodensTimed _tm("roe_limiter_chain", st);
     long nl0 = (long)(b.j2 - b.j1) * (b.k2 - b.k1);
     long nl12 = (long)(b.i2 - b.i1) * max(b.k2 - b.k1, b.j2 - b.j1);
     hipLaunchKernelGGL(k_limiter_chain_i, dim3((unsigned)((nl0 + CH_LINES - 1) / CH_LINES), 5), dim3(64), 0, st, b,
-                       v, ldv, ps, R);
+                       v, uvw, ps, R);
     hipLaunchKernelGGL(k_limiter_chain_jk, dim3((unsigned)((nl12 + CH_LINES - 1) / CH_LINES), 5, 2), dim3(64), 0, st,
-                       b, v, ldv, ps, R);
+                       b, v, uvw, ps, R);
   }
   for (int d = 0; d < 3; d++) {
     int nx = b.i2 - b.i1 + (d == 0), ny = b.j2 - b.j1 + (d == 1), nz = b.k2 - b.k1 + (d == 2);
     dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz);
     CodensTimed _tm("roe_face_flux", st);
     hipLaunchKernelGGL(k_face_flux, g, dim3(TX, TY), 0, st, b, ph, d, ns, ilim, (double)sc->muscl, sc->xk, v, ldv,
-                       ps, ns ? tq : v, sn, ilim == 1 ? R + 5 * b.nnn * d : nullptr, H + 5 * b.nnn * d);
+                       ps, ns ? tq : v, sn, uvw, ilim == 1 ? R + 5 * b.nnn * d : nullptr, H + 5 * b.nnn * d);
   }
   long n0 = b.idx(b.i1 - 1, b.j1 - 1, b.k1 - 1), n1 = b.idx(b.i2, b.j2, b.k2);
   {

@@ -178,6 +178,45 @@ def met_cut(b, which, epsk, epse, v, stream=None):
     _chk(lib.codens_met_cut(ctypes.byref(b), which, epsk, epse, _p(v), ld(v), _s(stream)))
 
 
+def _menter(mc):
+    """The 10 Menter constants as a host array (betae, okappa, sigme1,
+    sigma1, beta1, wsig1, sigme2, sigma2, beta2, wsig2)."""
+    mc = [float(x) for x in mc]
+    assert len(mc) == 10
+    return (ctypes.c_double * 10)(*mc)
+
+
+def _starts(face_start):
+    fs = [int(x) for x in (face_start if face_start is not None else [0])]
+    return (ctypes.c_int64 * len(fs))(*fs), len(fs) - 1
+
+
+def met_mtcorf1(face_start, lines, imxclko, mnpar, frac, stream=None):
+    """face_start: host int64 array (walls + 1); lines: device uint8 tensor
+    holding codens_wall_line records; mnpar: device int32."""
+    fs, nf = _starts(face_start)
+    assert mnpar.dtype == torch.int32
+    _chk(lib.codens_met_mtcorf1(nf, fs, _p(lines), imxclko, _p(mnpar), _p(frac), frac.numel(), _s(stream)))
+
+
+def met_smmt(b, mc, kfracom, imxclko, v, mu, mut, dist, fd, tprod, cfke, gkgo, frac, ts6, ts7, face_start=None,
+             lines=None, mnpar=None, stream=None):
+    fs, nf = _starts(face_start)
+    _chk(lib.codens_met_smmt(ctypes.byref(b), _menter(mc), kfracom, imxclko, _p(v), ld(v), _p(mu), _p(mut),
+                             _p(dist), _p(fd), _p(tprod), _p(cfke), _p(gkgo), _p(frac), _p(ts6), _p(ts7), nf, fs,
+                             _p(lines), _p(mnpar), _s(stream)))
+
+
+def met_fludmt(b, mc, kfracom, mu, mut, frac, fd, stream=None):
+    _chk(lib.codens_met_fludmt(ctypes.byref(b), _menter(mc), kfracom, _p(mu), _p(mut), _p(frac), _p(fd), ld(fd),
+                               _s(stream)))
+
+
+def met_komut(b, betae, komsst, sn_blk, vol, v, mu, mut, dist, g9, stream=None):
+    _chk(lib.codens_met_komut(ctypes.byref(b), betae, komsst, _p(sn_blk), _p(vol), _p(v), ld(v), _p(mu), _p(mut),
+                              _p(dist), _p(g9), _s(stream)))
+
+
 def met_rbse(g, i, fd, stream=None):
     _chk(lib.codens_met_rbse(g.numel(), _p(g), _p(i), _p(fd), ld(fd), _s(stream)))
 

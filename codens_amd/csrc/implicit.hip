@@ -258,6 +258,38 @@ __device__ inline void q_zero(Q14 &Q) {
   for (int c = 0; c < 14; c++) Q.q[c] = 0.;
 }
 
+// The same 14-vector in two phases, so that the loads of several cells can
+// be in flight together: the inputs of a cell (zero for a cell outside the
+// real region), then q from them.
+struct VD {
+  double V[5], D[5], p;
+};
+
+template <bool EU>
+__device__ inline void vd_load(const double *__restrict__ v, long ldv, const DView &d, const double *__restrict__ ps,
+                               long n, bool real, VD &x) {
+#pragma unroll
+  for (int e = 0; e < 5; e++) {
+    x.V[e] = real ? v[e * ldv + n] : 1.;
+    x.D[e] = real ? d.at(e, n) : 0.;
+  }
+  x.p = (EU && real) ? ps[n] : 0.;
+}
+
+template <bool EU>
+__device__ inline void vd_q(const Phys &ph, const VD &x, bool real, Q14 &Q) {
+  if (!real) {
+    q_zero(Q);
+    return;
+  }
+  DFlux F = EU ? dflux_eu(ph, x.V, x.D, x.p) : dflux(ph, x.V, x.D);
+  Q.q[0] = F.mx; Q.q[1] = F.my; Q.q[2] = F.mz;
+  Q.q[3] = F.xx; Q.q[4] = F.xy; Q.q[5] = F.xz; Q.q[6] = F.yy; Q.q[7] = F.yz; Q.q[8] = F.zz;
+  Q.q[9] = F.ex; Q.q[10] = F.ey; Q.q[11] = F.ez;
+  Q.q[12] = x.D[0]; Q.q[13] = x.D[4];
+}
+
+
 // d of component e from the 14-vector
 __device__ inline double q_d(const double *q, int e) {
   return e == 0 ? q[12] : (e == 4 ? q[13] : q[e - 1]);
@@ -371,25 +403,68 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
 #pragma unroll
     for (int e = 0; e < 5; e++) tlo_k[e] = dlo_k[e] = 0.;
   }
+  // d of the previous plane, stored once the next plane's loads are issued
+  // (the kernel never reads dnew, so the stores can trail by one plane)
+  double dpend[5];
+  long npend = -1;
   for (int k = kb; k < ke; k++) {
     long n = own ? b.idx(i, j, k) : 0, m = n - b.npc;
+    bool kreal = k + 1 < b.k2;
+    // this plane's operator terms, loaded as one batch before the next
+    // plane's flux increments are formed, so their latency overlaps that work:
+    // rhs -u (-ff), the i/j face metrics of both faces, coefficients, diagonal
+    // (-u - 0 is -u bit for bit, so ityprk = 0 takes the same form)
+    double uu[5], fv[5], sl[2][3], sh[2][3], cl[3], ch[3], dgm;
+    if (own) {
+#pragma unroll
+      for (int e = 0; e < 5; e++) {
+        uu[e] = u[e * ldv + n];
+        fv[e] = ityprk ? ff[e * ldv + n] : 0.;
+      }
+#pragma unroll
+      for (int d = 0; d < 2; d++) {
+        long inc = d == 0 ? 1 : b.nid;
+        bool rreal = d == 0 ? (i + 1 < b.i2) : (j + 1 < b.j2);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          sl[d][c] = snv(sn, nnn, m, d, c);
+          sh[d][c] = snv(sn, nnn, m + inc, d, c);
+        }
+        cl[d] = coefe[d * nnn + m];
+        ch[d] = rreal ? coefe[d * nnn + m + inc] : 0.;
+      }
+      cl[2] = coefe[2 * nnn + m];
+      ch[2] = kreal ? coefe[2 * nnn + m + nijd] : 0.;
+      dgm = diag[m];
+    }
+    // inputs of this thread's halo cell (plane k) and of its own cell on
+    // plane k+1, loaded together
+    VD xh, xn;
+    bool hq = halo && hreal, nq = own && kreal;
+    vd_load<EU>(v, ldv, dold, ps, hq ? b.idx(hi_, hj, k) : n, hq, xh);
+    vd_load<EU>(v, ldv, dold, ps, n + nijd, nq, xn);
+    double skn[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) skn[c] = own ? snv(sn, nnn, m + nijd, 2, c) : 0.;
+    if (npend >= 0) {
+#pragma unroll
+      for (int e = 0; e < 5; e++) dnew.at(e, npend) = dpend[e];
+    }
     // current plane into LDS (own cell and this thread's halo cell)
 #pragma unroll
     for (int c = 0; c < 14; c++) sq[c * SW_PL + slot] = Qc.q[c];
     if (halo) {
       Q14 Qh;
-      if (hreal) q_model<EU>(ph, v, ldv, dold, ps, b.idx(hi_, hj, k), Qh); else q_zero(Qh);
+      vd_q<EU>(ph, xh, hq, Qh);
 #pragma unroll
       for (int c = 0; c < 14; c++) sq[c * SW_PL + hslot] = Qh.q[c];
     }
     // k face above: q of the next plane and its face term
     double thi_k[5], dhi_k[5];
-    bool kreal = k + 1 < b.k2;
     if (own) {
       Q14 Qn;
-      if (kreal) q_model<EU>(ph, v, ldv, dold, ps, n + nijd, Qn); else q_zero(Qn);
-      q_tn(Qn.q, Qc.q, snv(sn, nnn, m + nijd, 2, 0), snv(sn, nnn, m + nijd, 2, 1), snv(sn, nnn, m + nijd, 2, 2),
-           thi_k);
+      vd_q<EU>(ph, xn, kreal, Qn);
+      q_tn(Qn.q, Qc.q, skn[0], skn[1], skn[2], thi_k);
 #pragma unroll
       for (int e = 0; e < 5; e++) dhi_k[e] = q_d(Qn.q, e);
       Qc = Qn;  // plane k+1; the plane-k values are read back from LDS below
@@ -398,40 +473,32 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
     if (own) {
       double d2w[5];
 #pragma unroll
-      for (int e = 0; e < 5; e++) d2w[e] = ityprk == 0 ? -u[e * ldv + n] : -u[e * ldv + n] - ff[e * ldv + n];
+      for (int e = 0; e < 5; e++) d2w[e] = -uu[e] - fv[e];
       const double *A = sq + slot;
       // i then j directions from LDS (implimf_3d.f90:268-362)
 #pragma unroll
       for (int d = 0; d < 2; d++) {
         const double *Lp = d == 0 ? A - 1 : A - (SW_TX + 2);
         const double *Rp = d == 0 ? A + 1 : A + (SW_TX + 2);
-        long inc = d == 0 ? 1 : b.nid;
         bool rreal = d == 0 ? (i + 1 < b.i2) : (j + 1 < b.j2);
-        double sl1 = snv(sn, nnn, m, d, 0), sl2 = snv(sn, nnn, m, d, 1), sl3 = snv(sn, nnn, m, d, 2);
-        double sh1 = snv(sn, nnn, m + inc, d, 0), sh2 = snv(sn, nnn, m + inc, d, 1), sh3 = snv(sn, nnn, m + inc, d, 2);
-        double cl = coefe[d * nnn + m];
-        double ch = rreal ? coefe[d * nnn + m + inc] : 0.;
 #pragma unroll
         for (int e = 0; e < 5; e++) {
-          double tt = d2w[e] + lds_tn(A, Lp, e, sl1, sl2, sl3) + cl * lds_d(Lp, e);
-          if (rreal) tt = tt + ch * lds_d(Rp, e);
-          d2w[e] = tt - lds_tn(Rp, A, e, sh1, sh2, sh3);
+          double tt = d2w[e] + lds_tn(A, Lp, e, sl[d][0], sl[d][1], sl[d][2]) + cl[d] * lds_d(Lp, e);
+          if (rreal) tt = tt + ch[d] * lds_d(Rp, e);
+          d2w[e] = tt - lds_tn(Rp, A, e, sh[d][0], sh[d][1], sh[d][2]);
         }
       }
       // k direction from registers
-      {
-        double cl = coefe[2 * nnn + m];
-        double ch = kreal ? coefe[2 * nnn + m + nijd] : 0.;
 #pragma unroll
-        for (int e = 0; e < 5; e++) {
-          double tt = d2w[e] + tlo_k[e] + cl * dlo_k[e];
-          if (kreal) tt = tt + ch * dhi_k[e];
-          d2w[e] = tt - thi_k[e];
-        }
+      for (int e = 0; e < 5; e++) {
+        double tt = d2w[e] + tlo_k[e] + cl[2] * dlo_k[e];
+        if (kreal) tt = tt + ch[2] * dhi_k[e];
+        d2w[e] = tt - thi_k[e];
       }
-      double rdg = 1.0 / diag[m];
+      double rdg = 1.0 / dgm;
 #pragma unroll
-      for (int e = 0; e < 5; e++) dnew.at(e, n) = d2w[e] * rdg;
+      for (int e = 0; e < 5; e++) dpend[e] = d2w[e] * rdg;
+      npend = n;
 #pragma unroll
       for (int e = 0; e < 5; e++) {
         tlo_k[e] = thi_k[e];
@@ -439,6 +506,10 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
       }
     }
     lds_sync();
+  }
+  if (npend >= 0) {
+#pragma unroll
+    for (int e = 0; e < 5; e++) dnew.at(e, npend) = dpend[e];
   }
 }
 

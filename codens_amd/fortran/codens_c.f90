@@ -23,6 +23,17 @@ module codens_c
      real(c_double) :: xk
   end type codens_scheme
 
+  ! one wall-normal line of met_mtcorf1 (first cell 0-based)
+  type, bind(C) :: codens_wall_line
+     integer(c_int64_t) :: nmin
+     integer(c_int32_t) :: m3min, m3max, isens3, dm3, mpn0, mpn1
+  end type codens_wall_line
+
+  ! lines of the walls of the last codens_wall_lines call, kept between calls
+  ! so that their host address (the key of their device mirror) is stable
+  type(codens_wall_line), allocatable, target, save :: codens_lines(:)
+  integer(c_int64_t), allocatable, save :: codens_face_start(:)
+
   ! .true. once the caller's metrics sn and volumes vol are final: they are
   ! then uploaded whole on first use and not again.
   logical, save :: codens_geom_once = .false.
@@ -252,6 +263,42 @@ module codens_c
        type(c_ptr), value :: sn, vol, v, mu, mut, dist, g9, stream
        integer(c_int64_t), value :: ldv
      end function codens_met_kocmut
+     integer(c_int32_t) function codens_met_mtcorf1(nfaces, face_start, lines, imxclko, mnpar, frac, nfrac, stream) &
+          bind(C, name="codens_met_mtcorf1")
+       import
+       integer(c_int32_t), value :: nfaces, imxclko
+       integer(c_int64_t), intent(in) :: face_start(*)
+       type(c_ptr), value :: lines, mnpar, frac, stream
+       integer(c_int64_t), value :: nfrac
+     end function codens_met_mtcorf1
+     integer(c_int32_t) function codens_met_smmt(b, mc, kfracom, imxclko, v, ldv, mu, mut, dist, fd, tprod, cfke, &
+          gkgo, frac, ts6, ts7, nfaces, face_start, lines, mnpar, stream) bind(C, name="codens_met_smmt")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: mc(10)
+       integer(c_int32_t), value :: kfracom, imxclko, nfaces
+       type(c_ptr), value :: v, mu, mut, dist, fd, tprod, cfke, gkgo, frac, ts6, ts7, lines, mnpar, stream
+       integer(c_int64_t), value :: ldv
+       integer(c_int64_t), intent(in) :: face_start(*)
+     end function codens_met_smmt
+     integer(c_int32_t) function codens_met_fludmt(b, mc, kfracom, mu, mut, frac, fd, ldv, stream) &
+          bind(C, name="codens_met_fludmt")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: mc(10)
+       integer(c_int32_t), value :: kfracom
+       type(c_ptr), value :: mu, mut, frac, fd, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_fludmt
+     integer(c_int32_t) function codens_met_komut(b, betae, komsst, sn, vol, v, ldv, mu, mut, dist, g9, stream) &
+          bind(C, name="codens_met_komut")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), value :: betae
+       integer(c_int32_t), value :: komsst
+       type(c_ptr), value :: sn, vol, v, mu, mut, dist, g9, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_komut
      integer(c_int32_t) function codens_met_cut(b, which, epsk, epse, v, ldv, stream) bind(C, name="codens_met_cut")
        import
        type(codens_block), intent(in) :: b
@@ -384,6 +431,64 @@ contains
          'met_cut')
     call codens_back(c_loc(s), ip11, 6, 2, b)
   end subroutine codens_cut
+
+  ! Menter constants in the C-ABI's order (module modeleturb)
+  function codens_menter_now() result(mc)
+    use modeleturb
+    real(c_double) :: mc(10)
+    mc = [betae, okappa, sigme1, sigma1, beta1, wsig1, sigme2, sigma2, beta2, wsig2]
+  end function codens_menter_now
+
+  ! The wall-normal lines met_mtcorf1 walks (met_mtcorf1.f90:74-88): every
+  ! point of every wall of the list lbdko on block l (all blocks for l = 0),
+  ! from atindnor; into codens_lines / codens_face_start, nf walls.
+  subroutine codens_wall_lines(l, ncin, nf)
+    use para_var
+    use boundary
+    use mod_atindnor
+    integer, intent(in) :: l, ncin(*)
+    integer, intent(out) :: nf
+    integer :: mpar, mfbi, m, nl, m10, m20, m30, m1min, m1max, m2min, m2max, m3min, m3max, dm1, dm2, dm3, isens3
+    nl = 0
+    nf = 0
+    do mpar = 1, nbdko
+       mfbi = lbdko(mpar)
+       if (l .eq. 0 .or. l .eq. ndlb(mfbi)) then
+          nl = nl + mmb(mfbi)
+          nf = nf + 1
+       endif
+    enddo
+    if (.not. allocated(codens_lines)) allocate(codens_lines(max(nl, 1)), codens_face_start(nf + 1))
+    if (size(codens_lines) .lt. nl .or. size(codens_face_start) .lt. nf + 1) then
+       deallocate(codens_lines, codens_face_start)
+       allocate(codens_lines(max(nl, 1)), codens_face_start(nf + 1))
+    endif
+    nl = 0
+    nf = 0
+    codens_face_start(1) = 0
+    do mpar = 1, nbdko
+       mfbi = lbdko(mpar)
+       if (l .eq. 0 .or. l .eq. ndlb(mfbi)) then
+          call atindnor(mfbi, m10, m20, m30, m1min, m1max, m2min, m2max, m3min, m3max, dm1, dm2, dm3, isens3)
+          do m = 1, mmb(mfbi)
+             nl = nl + 1
+             codens_lines(nl) = codens_wall_line(int(ncin(mpb(mfbi) + m) - 1, c_int64_t), m3min, m3max, isens3, &
+                  dm3, mpn(mfbi), mpn(mfbi) + mmb(mfbi))
+          enddo
+          nf = nf + 1
+          codens_face_start(nf + 1) = nl
+       endif
+    enddo
+  end subroutine codens_wall_lines
+
+  ! device copy of the lines of codens_wall_lines
+  function codens_lines_dev() result(dev)
+    type(c_ptr) :: dev
+    integer(c_int64_t) :: nb
+    nb = 32_c_int64_t * size(codens_lines)
+    call codens_check(codens_host_mirror(c_loc(codens_lines), nb, 0_c_int64_t, nb, nb, 1_c_int32_t, 1_c_int32_t, &
+         dev, c_null_ptr), 'codens_lines_dev')
+  end function codens_lines_dev
 
   subroutine codens_check(rc, what)
     integer(c_int32_t), intent(in) :: rc

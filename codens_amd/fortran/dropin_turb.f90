@@ -466,3 +466,162 @@ contains
     call codens_cut(1, l, s, epsk, epse)
   end subroutine met_cut
 end module mod_met_cut
+
+module mod_met_mtcorf1
+  implicit none
+contains
+  ! met_mtcorf1.f90:4 -- wall correction of the Menter blending function
+  ! along the wall-normal line of every wall point (walls of block l; all
+  ! walls for l = 0), one wall after another
+  subroutine met_mtcorf1(l, ncin, dist, mnpar, frac)
+    use para_var
+    use modeleturb, only: imxclko
+    use codens_c
+    implicit none
+    integer :: l
+    integer :: ncin(ip41)
+    integer, target :: mnpar(ip12)
+    double precision, target :: dist(ip12), frac(ip12)
+    type(c_ptr) :: dfr, dmn, dl
+    integer :: nf
+    call codens_wall_lines(l, ncin, nf)
+    if (nf .eq. 0) return
+    dl = codens_lines_dev()
+    dfr = codens_dev_all(c_loc(frac), ip12, 1)
+    call codens_check(codens_host_mirror(c_loc(mnpar), 4_c_int64_t*ip12, 0_c_int64_t, 4_c_int64_t*ip12, &
+         4_c_int64_t*ip12, 1_c_int32_t, 1_c_int32_t, dmn, c_null_ptr), 'met_mtcorf1')
+    call codens_check(codens_met_mtcorf1(int(nf, c_int32_t), codens_face_start, dl, int(imxclko, c_int32_t), dmn, &
+         dfr, int(ip12, c_int64_t), c_null_ptr), 'met_mtcorf1')
+    call codens_check(codens_host_fetch(c_loc(frac), 0_c_int64_t, 8_c_int64_t*ip12, 8_c_int64_t*ip12, 1_c_int32_t, &
+         c_null_ptr), 'met_mtcorf1')
+  end subroutine met_mtcorf1
+end module mod_met_mtcorf1
+
+module mod_met_smmt
+  implicit none
+contains
+  ! met_smmt.f90:4 -- Menter k-omega sources, blending function (with the
+  ! wall correction of met_mtcorf1 on the walls of block l) and gkgo
+  subroutine met_smmt(l, ncyc, v, mu, mut, dist, mnpar, ncin, txxf5x, txyf5y, txzf5z, tyyf6x, tyzf6y, tzzf6z, &
+       tprod, cfke, gkgo, frac, qcxts5, qcyts6)
+    use para_var
+    use maillage
+    use modeleturb, only: kfracom, imxclko
+    use codens_c
+    implicit none
+    integer :: l, ncyc
+    integer, target :: mnpar(ip12)
+    integer :: ncin(ip41)
+    double precision, target :: v(ip11,ip60), mu(ip12), mut(ip12), dist(ip12), tprod(ip00), cfke(ip13)
+    double precision, target :: txxf5x(ip12), txyf5y(ip12), txzf5z(ip12), tyyf6x(ip12), tyzf6y(ip12), tzzf6z(ip12)
+    double precision, target :: gkgo(ip00), frac(ip12), qcxts5(ip12), qcyts6(ip12)
+    type(codens_block) :: b
+    type(c_ptr) :: dv, dmu, dmut, ddist, dfd, dtp, dcf, dgk, dfr, d5, d6, dmn, dl, f(6)
+    integer :: nf
+    b = codens_block_of(l)
+    f = [c_loc(txxf5x), c_loc(txyf5y), c_loc(txzf5z), c_loc(tyyf6x), c_loc(tyzf6y), c_loc(tzzf6z)]
+    nf = 0
+    dl = c_null_ptr
+    dmn = c_null_ptr
+    if (kfracom .ne. 0) then
+       call codens_wall_lines(l, ncin, nf)
+       if (nf .gt. 0) then
+          dl = codens_lines_dev()
+          call codens_check(codens_host_mirror(c_loc(mnpar), 4_c_int64_t*ip12, 0_c_int64_t, 4_c_int64_t*ip12, &
+               4_c_int64_t*ip12, 1_c_int32_t, 1_c_int32_t, dmn, c_null_ptr), 'met_smmt')
+       endif
+    endif
+    if (.not. allocated(codens_face_start)) allocate(codens_lines(1), codens_face_start(1))
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 7, b, 1)
+    dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+    dmut = codens_dev(c_loc(mut), ip12, ip12, 1, 1, b, 1)
+    ddist = codens_dev(c_loc(dist), ip12, ip12, 1, 1, b, codens_geom_mode())
+    dfd = codens_bundle_in(f, 6, ip12, npc(l), npc(l) + nnn(l))
+    dtp = codens_dev_all(c_loc(tprod), nnn(l), 1)
+    dgk = codens_dev_all(c_loc(gkgo), nnn(l), 1)
+    dfr = codens_dev_all(c_loc(frac), ip12, 1)
+    dcf = codens_dev(c_loc(cfke), ip13, ip13, 1, 1, b, 1)
+    d5 = codens_dev(c_loc(qcxts5), ip12, ip12, 1, 1, b, 1)
+    d6 = codens_dev(c_loc(qcyts6), ip12, ip12, 1, 1, b, 1)
+    call codens_check(codens_met_smmt(b, codens_menter_now(), int(kfracom, c_int32_t), int(imxclko, c_int32_t), dv, &
+         int(ip11, c_int64_t), dmu, dmut, ddist, dfd, dtp, dcf, dgk, dfr, d5, d6, int(nf, c_int32_t), &
+         codens_face_start, dl, dmn, c_null_ptr), 'met_smmt')
+    call codens_back(c_loc(cfke), ip13, 1, 1, b)
+    call codens_back(c_loc(qcxts5), ip12, 1, 1, b)
+    call codens_back(c_loc(qcyts6), ip12, 1, 1, b)
+    call codens_check(codens_host_fetch(c_loc(gkgo), 0_c_int64_t, 8_c_int64_t*nnn(l), 8_c_int64_t*nnn(l), &
+         1_c_int32_t, c_null_ptr), 'met_smmt')
+    call codens_check(codens_host_fetch(c_loc(frac), 0_c_int64_t, 8_c_int64_t*ip12, 8_c_int64_t*ip12, 1_c_int32_t, &
+         c_null_ptr), 'met_smmt')
+  end subroutine met_smmt
+end module mod_met_smmt
+
+module mod_met_fludmt
+  implicit none
+contains
+  ! met_fludmt.f90:4 (kfludis = 4) -- Menter flux densities, sigmas blended
+  ! with frac when kfracom != 0
+  subroutine met_fludmt(l, s, mu, mut, frac, fd5x, fd5y, fd5z, fd6x, fd6y, fd6z)
+    use para_var
+    use maillage
+    use modeleturb, only: kfracom
+    use codens_c
+    implicit none
+    integer :: l
+    double precision :: s(ip11,ip60)
+    double precision, target :: mu(ip12), mut(ip12), frac(ip12)
+    double precision, target :: fd5x(ip12), fd5y(ip12), fd5z(ip12), fd6x(ip12), fd6y(ip12), fd6z(ip12)
+    type(codens_block) :: b
+    type(c_ptr) :: dmu, dmut, dfr, dfd, f(6)
+    b = codens_block_of(l)
+    f = [c_loc(fd5x), c_loc(fd5y), c_loc(fd5z), c_loc(fd6x), c_loc(fd6y), c_loc(fd6z)]
+    dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+    dmut = codens_dev(c_loc(mut), ip12, ip12, 1, 1, b, 1)
+    dfr = c_null_ptr
+    if (kfracom .ne. 0) dfr = codens_dev(c_loc(frac), ip12, ip12, 1, 1, b, 1)
+    dfd = codens_bundle_in(f, 6, ip12, npc(l), npc(l) + nnn(l))
+    call codens_check(codens_met_fludmt(b, codens_menter_now(), int(kfracom, c_int32_t), dmu, dmut, dfr, dfd, &
+         int(ip12, c_int64_t), c_null_ptr), 'met_fludmt')
+    call codens_bundle_back(f, 6, ip12, npc(l), npc(l) + nnn(l))
+  end subroutine met_fludmt
+end module mod_met_fludmt
+
+module mod_met_komut
+  implicit none
+contains
+  ! met_komut.f90:4 -- Menter eddy viscosity (SST limiter for komsst = 1,
+  ! whose velocity gradients land in dvxx..dvzz as in the reference)
+  subroutine met_komut(l, sn, vol, t, dvxx, dvxy, dvxz, dvyx, dvyy, dvyz, dvzx, dvzy, dvzz, dist, v, mu, mut, &
+       cmui1, cmui2, cmuj1, cmuj2, cmuk1, cmuk2)
+    use para_var
+    use para_fige
+    use maillage
+    use modeleturb, only: komsst, betae
+    use codens_c
+    implicit none
+    integer :: l
+    double precision, target :: sn(ip31*ndir), vol(ip11), t(ip00), dist(ip12), v(ip11,ip60), mu(ip12), mut(ip12)
+    double precision, target :: dvxx(ip00), dvxy(ip00), dvxz(ip00), dvyx(ip00), dvyy(ip00), dvyz(ip00)
+    double precision, target :: dvzx(ip00), dvzy(ip00), dvzz(ip00)
+    double precision :: cmui1(ip21), cmui2(ip21), cmuj1(ip21), cmuj2(ip21), cmuk1(ip21), cmuk2(ip21)
+    type(codens_block) :: b
+    type(c_ptr) :: dsn, dvol, dv, dmu, dmut, ddist, dg, g(9)
+    b = codens_block_of(l)
+    g = [c_loc(dvxx), c_loc(dvxy), c_loc(dvxz), c_loc(dvyx), c_loc(dvyy), c_loc(dvyz), c_loc(dvzx), c_loc(dvzy), &
+         c_loc(dvzz)]
+    dsn = codens_dev_all(codens_sn_blk(sn, l), 9*nnn(l), codens_geom_mode())
+    dvol = codens_dev(c_loc(vol), ip11, ip11, 1, 1, b, codens_geom_mode())
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 7, b, 1)
+    dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+    dmut = codens_dev(c_loc(mut), ip12, ip12, 1, 1, b, 1)
+    ddist = codens_dev(c_loc(dist), ip12, ip12, 1, 1, b, codens_geom_mode())
+    dg = codens_bundle_in(g, 9, nnn(l), 0, nnn(l))
+    call codens_check(codens_met_komut(b, betae, int(komsst, c_int32_t), dsn, dvol, dv, int(ip11, c_int64_t), dmu, &
+         dmut, ddist, dg, c_null_ptr), 'met_komut')
+    call codens_back(c_loc(mut), ip12, 1, 1, b)
+    if (komsst .eq. 1) then
+       call codens_back(c_loc(v), ip11, 1, 1, b)
+       call codens_bundle_back(g, 9, nnn(l), 0, nnn(l))
+    endif
+  end subroutine met_komut
+end module mod_met_komut

@@ -50,6 +50,16 @@ typedef struct codens_scheme {
   double xk;
 } codens_scheme;
 
+/* One wall-normal line of met_mtcorf1, as atindnor.f90:4 describes a wall
+ * point: first cell nmin (0-based storage index), the along-line index m3
+ * from m3min to m3max in direction isens3 (+1 / -1), cell stride dm3 (the
+ * reference's cells are nmin + (m3 - m3min) * isens3 * dm3), and the
+ * window (mpn0, mpn1] of the wall's point numbers in mnpar. */
+typedef struct codens_wall_line {
+  int64_t nmin;
+  int32_t m3min, m3max, isens3, dm3, mpn0, mpn1;
+} codens_wall_line;
+
 enum {
   CODENS_OK = 0,
   CODENS_EINVAL = 1,     /* bad extents, null pointer, size over a limit */
@@ -304,6 +314,43 @@ int codens_host_bundle(const void *const *hosts, int32_t count, int64_t bytes, i
                        int64_t width, int32_t to_device, void **dev, void *stream);
 int codens_host_bundle_fetch(const void *const *hosts, int32_t count, int64_t bytes, int64_t offset,
                              int64_t width, void *stream);
+
+/* ---- Menter k-omega model (met_num.f90:449-456, 542-546, 815-850) ----
+ * mc: the model constants betae, okappa, sigme1, sigma1, beta1, wsig1,
+ * sigme2, sigma2, beta2, wsig2 (module modeleturb, atlecdon.f90:255-278).
+ * Walls: nfaces walls, wall f owning lines[face_start[f] .. face_start[f+1])
+ * (face_start in host memory, lines and mnpar in device memory); the walls
+ * run one after another in list order, the lines of one wall in parallel
+ * (they are distinct cell columns).  Cells a line would reach outside
+ * [0, nfrac) are skipped. */
+
+/* met_mtcorf1.f90:4 -- wall correction of the blending function frac. */
+int codens_met_mtcorf1(int32_t nfaces, const int64_t *face_start, const codens_wall_line *lines, int32_t imxclko,
+                       const int32_t *mnpar, double *frac, int64_t nfrac, void *stream);
+
+/* met_smmt.f90:4 -- Menter sources: gkgo (block-local) and frac (kfracom =
+ * 0: 0 and 1; else grad k . grad omega from fd and tanh(zeta^4), then
+ * codens_met_mtcorf1 over the walls), then ts6, ts7 and cfke = 0.18 omega
+ * with the frac-blended constants.  fd: gradients of k and omega (6 planes
+ * of ldv, read for kfracom != 0 only); tprod block-local. */
+int codens_met_smmt(const codens_block *b, const double *mc, int32_t kfracom, int32_t imxclko, const double *v,
+                    int64_t ldv, const double *mu, const double *mut, const double *dist, const double *fd,
+                    const double *tprod, double *cfke, double *gkgo, double *frac, double *ts6, double *ts7,
+                    int32_t nfaces, const int64_t *face_start, const codens_wall_line *lines, const int32_t *mnpar,
+                    void *stream);
+
+/* met_fludmt.f90:4 -- fd(1:3) *= mu + mut sigme, fd(4:6) *= mu + mut sigma
+ * on indc(i1,j1,k1)..indc(i2-1,j2-1,k2-1), the sigmas blended with frac when
+ * kfracom != 0 (frac read only then). */
+int codens_met_fludmt(const codens_block *b, const double *mc, int32_t kfracom, const double *mu, const double *mut,
+                      const double *frac, double *fd, int64_t ldv, void *stream);
+
+/* met_komut.f90:4 -- Menter eddy viscosity: rho k / omega (komsst = 0), or
+ * with the SST limiter (komsst = 1: codens_teq_gradv into g9 first, as the
+ * reference calls teq_gradv there). */
+int codens_met_komut(const codens_block *b, double betae, int32_t komsst, const double *sn_blk, const double *vol,
+                     double *v, int64_t ldv, const double *mu, double *mut, const double *dist, double *g9,
+                     void *stream);
 
 #ifdef __cplusplus
 }

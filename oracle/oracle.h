@@ -36,4 +36,18 @@ typedef struct {
   double alfak, alfae;                                         /* k-eps / k-l */
 } orc_turb;
 
+/* Menter k-omega constants (atlecdon.f90:255-278): inner (1) and outer (2)
+ * sets blended with the function frac. */
+typedef struct {
+  double betae, okappa, sigme1, sigma1, beta1, wsig1, sigme2, sigma2, beta2, wsig2;
+} orc_menter;
+
+/* One wall-normal line of met_mtcorf1 (the quantities atindnor.f90:4 gives
+ * for a wall point): first cell nmin (0-based storage index), m3 range and
+ * direction, cell stride dm3, wall-point window (mpn0, mpn1]. */
+typedef struct {
+  int64_t nmin;
+  int32_t m3min, m3max, isens3, dm3, mpn0, mpn1;
+} orc_wall_line;
+
 #endif

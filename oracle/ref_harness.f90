@@ -449,4 +449,80 @@ contains
     call met_smch(l, v, mu, dist, mnpar, utau, tprod, ts6, ts7)
   end subroutine ref_met_smch
 
+  ! Menter constants and switches (atlecdon.f90:255-278)
+  subroutine ref_set_menter(mc, kfrac, ksst, imx) bind(C, name="ref_set_menter")
+    real(c_double) :: mc(10)
+    integer(c_int), value :: kfrac, ksst, imx
+    betae = mc(1); okappa = mc(2); sigme1 = mc(3); sigma1 = mc(4); beta1 = mc(5); wsig1 = mc(6)
+    sigme2 = mc(7); sigma2 = mc(8); beta2 = mc(9); wsig2 = mc(10)
+    kfracom = kfrac
+    komsst = ksst
+    imxclko = imx
+  end subroutine ref_set_menter
+
+  ! Wall faces of module boundary for met_mtcorf1 / atindnor: desc(12, nw)
+  ! per wall = block, face (1..6 = i1 i2 j1 j2 k1 k2), imin, imax, jmin,
+  ! jmax, kmin, kmax, mmb (points), mpb (offset into ncin), mpn (offset of
+  ! the wall's point numbers), unused.  The walls form the list lbdko.
+  subroutine ref_set_walls(nw, desc) bind(C, name="ref_set_walls")
+    use boundary
+    integer(c_int), value :: nw
+    integer(c_int) :: desc(12, *)
+    character(len=2), parameter :: codes(6) = ['i1', 'i2', 'j1', 'j2', 'k1', 'k2']
+    integer :: w
+    if (allocated(nfei)) deallocate(nfei, ndlb, indfl, iminb, imaxb, jminb, jmaxb, kminb, kmaxb, mmb, mpb, mpn, lbdko)
+    allocate(nfei(nw), ndlb(nw), indfl(nw), iminb(nw), imaxb(nw), jminb(nw), jmaxb(nw), kminb(nw), kmaxb(nw))
+    allocate(mmb(nw), mpb(nw), mpn(nw), lbdko(nw))
+    do w = 1, nw
+       nfei(w) = w
+       lbdko(w) = w
+       ndlb(w) = desc(1, w)
+       indfl(w) = codes(desc(2, w))
+       iminb(w) = desc(3, w); imaxb(w) = desc(4, w)
+       jminb(w) = desc(5, w); jmaxb(w) = desc(6, w)
+       kminb(w) = desc(7, w); kmaxb(w) = desc(8, w)
+       mmb(w) = desc(9, w); mpb(w) = desc(10, w); mpn(w) = desc(11, w)
+    enddo
+    nbdko = nw
+  end subroutine ref_set_walls
+
+  ! met_mtcorf1.f90:4
+  subroutine ref_met_mtcorf1(l, ncin, dist, mnpar, frac) bind(C, name="ref_met_mtcorf1")
+    use mod_met_mtcorf1
+    integer(c_int), value :: l
+    integer(c_int) :: ncin(*), mnpar(*)
+    real(c_double) :: dist(*), frac(*)
+    call met_mtcorf1(l, ncin, dist, mnpar, frac)
+  end subroutine ref_met_mtcorf1
+
+  ! met_smmt.f90:4
+  subroutine ref_met_smmt(l, v, mu, mut, dist, mnpar, ncin, fd, tprod, cfke, gkgo, frac, ts6, ts7) &
+       bind(C, name="ref_met_smmt")
+    use mod_met_smmt
+    integer(c_int), value :: l
+    integer(c_int) :: mnpar(*), ncin(*)
+    real(c_double) :: v(*), mu(*), mut(*), dist(*), fd(ip11, 6), tprod(*), cfke(*), gkgo(*), frac(*), ts6(*), ts7(*)
+    call met_smmt(l, 1, v, mu, mut, dist, mnpar, ncin, fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6), &
+         tprod, cfke, gkgo, frac, ts6, ts7)
+  end subroutine ref_met_smmt
+
+  ! met_fludmt.f90:4
+  subroutine ref_met_fludmt(l, v, mu, mut, frac, fd) bind(C, name="ref_met_fludmt")
+    use mod_met_fludmt
+    integer(c_int), value :: l
+    real(c_double) :: v(*), mu(*), mut(*), frac(*), fd(ip11, 6)
+    call met_fludmt(l, v, mu, mut, frac, fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6))
+  end subroutine ref_met_fludmt
+
+  ! met_komut.f90:4
+  subroutine ref_met_komut(l, sn, vol, t, g, dist, v, mu, mut, cmu) bind(C, name="ref_met_komut")
+    use mod_met_komut
+    use chainecarac
+    integer(c_int), value :: l
+    real(c_double) :: sn(*), vol(*), t(*), g(ip00, 9), dist(*), v(*), mu(*), mut(*), cmu(*)
+    equat = heq
+    call met_komut(l, sn, vol, t, g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), g(1,8), g(1,9), &
+         dist, v, mu, mut, cmu, cmu, cmu, cmu, cmu, cmu)
+  end subroutine ref_met_komut
+
 end module codens_ref_harness

@@ -285,3 +285,57 @@ def test_impli2_eqt_3d(tsetup, model):
     inner = lay.interior()
     close((vr - v)[5:7, inner], (vg - v)[5:7, inner])
     assert np.array_equal(vr[:5], vg[:5])
+
+
+@pytest.mark.parametrize("kfracom", [0, 1])
+def test_menter_routines(tsetup, kfracom):
+    """met_smmt (with met_mtcorf1 on the walls of the block), met_mtcorf1 on
+    all walls, met_fludmt and met_komut (both forms) through the reference's
+    signatures, the walls set in both builds' module `boundary`."""
+    import ctypes
+    from oracle import ptr
+    from menter_case import MENTER, walls
+    lay, sn, vol, v, mu, mut, dist, c, R, G = tsetup
+    n = lay.ip11
+    imx = 1
+    rng = np.random.default_rng(401 + kfracom)
+    desc, ncin, lines, starts, owner = walls(lay, [(0, "j1"), (1, "i1"), (0, "k1")])
+    for X in (R, G):
+        X.lib.ref_set_walls(desc.shape[1], desc.ctypes.data_as(ctypes.c_void_p))
+    mnpar = rng.integers(0, len(ncin) + 3, n).astype(np.int32)
+    dw = 0.002 + 0.2 * rng.random(n)
+    fd = np.zeros((6, n))
+    fd[:3] = 1e-3 * rng.standard_normal((3, n))
+    fd[3:] = 10 * rng.standard_normal((3, n))
+    cmu = np.ones(n)
+    try:
+        for komsst in (0, 1):
+            for X in (R, G):
+                X.lib.ref_set_menter(ptr(MENTER), kfracom, komsst, imx)
+            for l, b in enumerate(lay.blocks):
+                inner = b.interior()
+                tp = np.zeros(lay.ip00); tp[:b.nnn] = 1e-3 * rng.random(b.nnn)
+                if komsst == 0:
+                    r_, g_ = _both(R, G, "ref_met_smmt", l + 1, v, mu, mut, dw, mnpar, ncin, fd, tp,
+                                   np.full(n, 7.0), np.full(lay.ip00, 7.0), np.full(n, 7.0), np.full(n, 7.0),
+                                   np.full(n, 7.0))
+                    # cfke, gkgo (block-local), frac, ts6, ts7
+                    close(r_[8][inner], g_[8][inner])
+                    close(r_[9][:b.nnn], g_[9][:b.nnn])
+                    close(r_[10], g_[10])
+                    close(r_[11][inner], g_[11][inner])
+                    close(r_[12][inner], g_[12][inner])
+                    frac = r_[10]
+                    r_, g_ = _both(R, G, "ref_met_fludmt", l + 1, v, mu, mut, frac, fd)
+                    close(r_[4][:, inner], g_[4][:, inner])
+                g0 = np.zeros((9, lay.ip00))
+                r_, g_ = _both(R, G, "ref_met_komut", l + 1, sn, vol, np.zeros(lay.ip00), g0, dw, v, mu,
+                               np.full(n, 7.0), cmu)
+                close(r_[7][inner], g_[7][inner])
+        if kfracom:
+            fr = rng.random(n); fr[rng.random(n) < 0.3] = 0.995
+            r_, g_ = _both(R, G, "ref_met_mtcorf1", 0, ncin, dw, mnpar, fr)
+            assert np.array_equal(r_[3], g_[3])
+    finally:
+        for X in (R, G):
+            X.lib.ref_set_walls(0, None)

@@ -229,3 +229,69 @@ def test_sources_kor_chien(setup):
                      T(utau), T(tp), *go)
         for a, o in zip(go, oo):
             assert relerr(H(a)[inner], o[inner]) <= TOL
+
+
+@pytest.mark.parametrize("kfracom", [0, 1])
+def test_menter(setup, kfracom):
+    """Menter model kernels (met_smmt with the wall correction of
+    met_mtcorf1 on three walls, two of them crossing, met_fludmt, met_komut
+    both forms) against the oracle."""
+    import ctypes
+    from oracle import ptr
+    from menter_case import MENTER, walls, walls_of_block
+    abi, lay, sn, vol, v, mu, mut, dist, cson, tc, tcl, O = setup
+    n = lay.ip11
+    imx = 1
+    rng = np.random.default_rng(301 + kfracom)
+    desc, ncin, lines, starts, owner = walls(lay, [(0, "j1"), (1, "i1"), (0, "k1")])
+    mnpar = rng.integers(0, len(ncin) + 3, n).astype(np.int32)
+    gmn = torch.as_tensor(mnpar, device="cuda")
+    dw = 0.002 + 0.2 * rng.random(n)
+    mt = 1e-3 * (0.5 + rng.random(n))
+    fd = np.zeros((6, n))
+    fd[:3] = 1e-3 * rng.standard_normal((3, n))
+    fd[3:] = 10 * rng.standard_normal((3, n))
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        inner = b.interior()
+        tp = np.zeros(lay.ip00); tp[:b.nnn] = 1e-3 * rng.random(b.nnn)
+        ln, st = walls_of_block(lines, starts, owner, l)
+        o = dict(cfke=np.full(n, 7.0), gkgo=np.full(lay.ip00, 7.0), frac=np.full(n, 7.0), ts6=np.full(n, 7.0),
+                 ts7=np.full(n, 7.0))
+        g = {k: T(a) for k, a in o.items()}
+        O.lib.orc_met_smmt(ctypes.byref(ob), ptr(MENTER), kfracom, imx, ptr(v), C.L_(n), ptr(mu), ptr(mt), ptr(dw),
+                           ptr(fd), ptr(tp), ptr(o["cfke"]), ptr(o["gkgo"]), ptr(o["frac"]), ptr(o["ts6"]),
+                           ptr(o["ts7"]), ln.ctypes.data_as(ctypes.c_void_p), C.L_(len(ln)), ptr(mnpar))
+        gl = torch.as_tensor(ln.view(np.uint8), device="cuda") if len(ln) else None
+        abi.met_smmt(abi.block(b), MENTER, kfracom, imx, T(v), T(mu), T(mt), T(dw), T(fd), T(tp), g["cfke"],
+                     g["gkgo"], g["frac"], g["ts6"], g["ts7"], face_start=st, lines=gl, mnpar=gmn)
+        for k in o:
+            assert relerr(H(g[k]), o[k]) <= TOL, (l, k)
+        if kfracom:
+            assert (o["frac"][inner] < 1).any()
+        # flux densities with the blending function just computed
+        fo = fd.copy()
+        O.lib.orc_met_fludmt(ctypes.byref(ob), ptr(MENTER), kfracom, ptr(mu), ptr(mt), ptr(o["frac"]), ptr(fo),
+                             C.L_(n))
+        gf = T(fd)
+        abi.met_fludmt(abi.block(b), MENTER, kfracom, T(mu), T(mt), T(o["frac"]), gf)
+        assert relerr(H(gf), fo) <= TOL
+        # eddy viscosity, both forms (the SST form clamps rho and forms the
+        # velocity gradients first)
+        for komsst in (0, 1):
+            vo = v.copy()
+            g9o = O.teq_gradv(l, sn, vol, vo) if komsst else np.zeros((9, b.nnn))
+            mo = np.full(n, 7.0)
+            O.lib.orc_met_komut(ctypes.byref(ob), C.D(MENTER[0]), komsst, ptr(vo), C.L_(n), ptr(mu), ptr(mo),
+                                ptr(dw), ptr(np.ascontiguousarray(g9o)), C.L_(b.nnn))
+            gv, gm, gg = T(v), T(np.full(n, 7.0)), T(np.zeros((9, b.nnn)))
+            abi.met_komut(abi.block(b), MENTER[0], komsst, snb(sn, b), T(vol), gv, T(mu), gm, T(dw), gg)
+            assert relerr(H(gm)[inner], mo[inner]) <= TOL, komsst
+    if kfracom:
+        # the wall correction alone over all walls, in list order
+        fr = rng.random(n); fr[rng.random(n) < 0.3] = 0.995
+        fo = fr.copy()
+        O.lib.orc_met_mtcorf1(lines.ctypes.data_as(ctypes.c_void_p), C.L_(len(lines)), imx, ptr(mnpar), ptr(fo))
+        gfr = T(fr)
+        abi.met_mtcorf1(starts, torch.as_tensor(lines.view(np.uint8), device="cuda"), imx, gmn, gfr)
+        assert np.array_equal(H(gfr), fo)

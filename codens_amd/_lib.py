@@ -81,6 +81,8 @@ SIGNATURES = {
     "codens_met_mtcorf1": [c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_i64, c_vp],
     "codens_met_smmt": [P(CodensBlock), c_vp, c_int, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                         c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
+    "codens_met_smmtr": [P(CodensBlock), c_vp, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                         c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
     "codens_met_fludmt": [P(CodensBlock), c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp],
     "codens_met_komut": [P(CodensBlock), c_dbl, c_int, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp],
     "codens_met_rbse": [c_i64, c_vp, c_vp, c_vp, c_i64, c_vp],

@@ -207,6 +207,14 @@ def met_smmt(b, mc, kfracom, imxclko, v, mu, mut, dist, fd, tprod, cfke, gkgo, f
                              _p(lines), _p(mnpar), _s(stream)))
 
 
+def met_smmtr(b, mc, imxclko, v, mu, mut, dist, fd, g9, tprod, cfke, frac, ts6, ts7, face_start=None, lines=None,
+              mnpar=None, stream=None):
+    fs, nf = _starts(face_start)
+    _chk(lib.codens_met_smmtr(ctypes.byref(b), _menter(mc), imxclko, _p(v), ld(v), _p(mu), _p(mut), _p(dist), _p(fd),
+                              _p(g9), _p(tprod), _p(cfke), _p(frac), _p(ts6), _p(ts7), nf, fs, _p(lines), _p(mnpar),
+                              _s(stream)))
+
+
 def met_fludmt(b, mc, kfracom, mu, mut, frac, fd, stream=None):
     _chk(lib.codens_met_fludmt(ctypes.byref(b), _menter(mc), kfracom, _p(mu), _p(mut), _p(frac), _p(fd), ld(fd),
                                _s(stream)))

@@ -95,15 +95,30 @@ __global__ void __launch_bounds__(64) k_mtcorf1(const codens_wall_line *__restri
     }
 }
 
-// met_smmt.f90:106-124: the sources with the blended constants
+// met_smmt.f90:106-124: the sources with the blended constants; with the
+// velocity gradients g9 (met_smmtr.f90:94-119) the production of ts7 is
+// divided by Durbin's alpha = min(1, (1/sqrt(0.09)) / (S/omega)) (without
+// them alpha = 1, an exact division)
 __global__ void __launch_bounds__(TX *TY) k_smmt_sources(Blk b, Menter mc, const double *__restrict__ v, long ldv,
                                                          const double *__restrict__ mut,
                                                          const double *__restrict__ tprod,
                                                          const double *__restrict__ gkgo,
-                                                         const double *__restrict__ frac, double *__restrict__ cfke,
+                                                         const double *__restrict__ frac,
+                                                         const double *__restrict__ g9, double *__restrict__ cfke,
                                                          double *__restrict__ ts6, double *__restrict__ ts7) {
   REAL3(b)
   long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+  double alpha = 1.;
+  if (g9) {
+    const long q = b.nnn;
+    double dxx = g9[m], dxy = g9[q + m], dxz = g9[2 * q + m], dyx = g9[3 * q + m], dyy = g9[4 * q + m];
+    double dyz = g9[5 * q + m], dzx = g9[6 * q + m], dzy = g9[7 * q + m], dzz = g9[8 * q + m];
+    double a = dxx * dxx + dyy * dyy + dzz * dzz;
+    double s1 = dzy + dyz, s2 = dxz + dzx, s3 = dyx + dxy;
+    double ss = sqrt(4. * a / 3. + s1 * s1 + s2 * s2 + s3 * s3) / (v[6 * ldv + n] / v[n]);
+    double r = (1. / sqrt(0.09)) / ss;
+    alpha = r < 1. ? r : 1.;
+  }
   double sb = sqrt(mc.betae), ok2 = mc.okappa * mc.okappa;
   double xgam1 = mc.beta1 / mc.betae - mc.sigma1 * ok2 / sb;
   double xgam2 = mc.beta2 / mc.betae - mc.sigma2 * ok2 / sb;
@@ -116,7 +131,7 @@ __global__ void __launch_bounds__(TX *TY) k_smmt_sources(Blk b, Menter mc, const
   double sl3 = 2. * wsig * rho * gkgo[m] / omeg;
   double p = tprod[m];
   ts6[n] = p - mc.betae * rk * omeg;
-  ts7[n] = p * wgam * rho / mut[n] - beta * rw * omeg + sl3;
+  ts7[n] = p * wgam * rho / mut[n] / alpha - beta * rw * omeg + sl3;
   cfke[n] = 0.18 * omeg;
 }
 
@@ -214,8 +229,38 @@ extern "C" int codens_met_smmt(const codens_block *cb, const double *mc, int32_t
     if (rc) return rc;
   }
   hipLaunchKernelGGL(k_smmt_sources, cell_grid(b), dim3(TX, TY), 0, st, b, m, v, (long)ldv, mut, tprod, gkgo, frac,
-                     cfke, ts6, ts7);
+                     (const double *)nullptr, cfke, ts6, ts7);
   return codens_check_launch("codens_met_smmt");
+}
+
+extern "C" int codens_met_smmtr(const codens_block *cb, const double *mc, int32_t imxclko, const double *v,
+                                int64_t ldv, const double *mu, const double *mut, const double *dist,
+                                const double *fd, const double *g9, const double *tprod, double *cfke, double *frac,
+                                double *ts6, double *ts7, int32_t nfaces, const int64_t *face_start,
+                                const codens_wall_line *lines, const int32_t *mnpar, void *stream) {
+  int rc = codens_check_block(cb, "codens_met_smmtr");
+  if (rc) return rc;
+  if (!mc || !v || !mu || !mut || !dist || !fd || !g9 || !tprod || !cfke || !frac || !ts6 || !ts7) {
+    codens_set_error("codens_met_smmtr: null argument");
+    return CODENS_EINVAL;
+  }
+  Blk b = make_blk(cb);
+  Menter m = make_menter(mc);
+  hipStream_t st = (hipStream_t)stream;
+  double *gkgo = (double *)codens_workspace(sizeof(double) * b.nnn, 5);  // grad k . grad omega per cell
+  if (!gkgo) {
+    codens_set_error("codens_met_smmtr: workspace allocation failed");
+    return CODENS_EHIP;
+  }
+  hipLaunchKernelGGL(k_smmt_blend, cell_grid(b), dim3(TX, TY), 0, st, b, m, 1, v, (long)ldv, mu, dist, fd, gkgo,
+                     frac);
+  if (nfaces > 0) {
+    rc = mtcorf1_launch(nfaces, face_start, lines, imxclko, mnpar, frac, ldv, st, "codens_met_smmtr");
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_smmt_sources, cell_grid(b), dim3(TX, TY), 0, st, b, m, v, (long)ldv, mut, tprod, gkgo, frac,
+                     g9, cfke, ts6, ts7);
+  return codens_check_launch("codens_met_smmtr");
 }
 
 extern "C" int codens_met_fludmt(const codens_block *cb, const double *mc, int32_t kfracom, const double *mu,

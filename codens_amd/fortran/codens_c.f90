@@ -281,6 +281,16 @@ module codens_c
        integer(c_int64_t), value :: ldv
        integer(c_int64_t), intent(in) :: face_start(*)
      end function codens_met_smmt
+     integer(c_int32_t) function codens_met_smmtr(b, mc, imxclko, v, ldv, mu, mut, dist, fd, g9, tprod, cfke, &
+          frac, ts6, ts7, nfaces, face_start, lines, mnpar, stream) bind(C, name="codens_met_smmtr")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: mc(10)
+       integer(c_int32_t), value :: imxclko, nfaces
+       type(c_ptr), value :: v, mu, mut, dist, fd, g9, tprod, cfke, frac, ts6, ts7, lines, mnpar, stream
+       integer(c_int64_t), value :: ldv
+       integer(c_int64_t), intent(in) :: face_start(*)
+     end function codens_met_smmtr
      integer(c_int32_t) function codens_met_fludmt(b, mc, kfracom, mu, mut, frac, fd, ldv, stream) &
           bind(C, name="codens_met_fludmt")
        import

@@ -625,3 +625,60 @@ contains
     endif
   end subroutine met_komut
 end module mod_met_komut
+
+module mod_met_smmtr
+  implicit none
+contains
+  ! met_smmtr.f90:4 -- Menter sources with Durbin's realizability limit (the
+  ! blending function with the wall correction on the walls of block l)
+  subroutine met_smmtr(l, v, mu, mut, dist, mnpar, ncin, txxf5x, txyf5y, txzf5z, tyyf6x, tyzf6y, tzzf6z, &
+       dvxx, dvxy, dvxz, dvyx, dvyy, dvyz, dvzx, dvzy, dvzz, tprod, cfke, frac, qcxts5, qcyts6)
+    use para_var
+    use maillage
+    use modeleturb, only: imxclko
+    use codens_c
+    implicit none
+    integer :: l
+    integer, target :: mnpar(ip12)
+    integer :: ncin(ip41)
+    double precision, target :: v(ip11,ip60), mu(ip12), mut(ip12), dist(ip12), tprod(ip00), cfke(ip13)
+    double precision, target :: txxf5x(ip12), txyf5y(ip12), txzf5z(ip12), tyyf6x(ip12), tyzf6y(ip12), tzzf6z(ip12)
+    double precision, target :: dvxx(ip00), dvxy(ip00), dvxz(ip00), dvyx(ip00), dvyy(ip00), dvyz(ip00)
+    double precision, target :: dvzx(ip00), dvzy(ip00), dvzz(ip00)
+    double precision, target :: frac(ip12), qcxts5(ip12), qcyts6(ip12)
+    type(codens_block) :: b
+    type(c_ptr) :: dv, dmu, dmut, ddist, dfd, dg, dtp, dcf, dfr, d5, d6, dmn, dl, f(6), g(9)
+    integer :: nf
+    b = codens_block_of(l)
+    f = [c_loc(txxf5x), c_loc(txyf5y), c_loc(txzf5z), c_loc(tyyf6x), c_loc(tyzf6y), c_loc(tzzf6z)]
+    g = [c_loc(dvxx), c_loc(dvxy), c_loc(dvxz), c_loc(dvyx), c_loc(dvyy), c_loc(dvyz), c_loc(dvzx), c_loc(dvzy), &
+         c_loc(dvzz)]
+    dl = c_null_ptr
+    dmn = c_null_ptr
+    call codens_wall_lines(l, ncin, nf)
+    if (nf .gt. 0) then
+       dl = codens_lines_dev()
+       call codens_check(codens_host_mirror(c_loc(mnpar), 4_c_int64_t*ip12, 0_c_int64_t, 4_c_int64_t*ip12, &
+            4_c_int64_t*ip12, 1_c_int32_t, 1_c_int32_t, dmn, c_null_ptr), 'met_smmtr')
+    endif
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 7, b, 1)
+    dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+    dmut = codens_dev(c_loc(mut), ip12, ip12, 1, 1, b, 1)
+    ddist = codens_dev(c_loc(dist), ip12, ip12, 1, 1, b, codens_geom_mode())
+    dfd = codens_bundle_in(f, 6, ip12, npc(l), npc(l) + nnn(l))
+    dg = codens_bundle_in(g, 9, nnn(l), 0, nnn(l))
+    dtp = codens_dev_all(c_loc(tprod), nnn(l), 1)
+    dfr = codens_dev_all(c_loc(frac), ip12, 1)
+    dcf = codens_dev(c_loc(cfke), ip13, ip13, 1, 1, b, 1)
+    d5 = codens_dev(c_loc(qcxts5), ip12, ip12, 1, 1, b, 1)
+    d6 = codens_dev(c_loc(qcyts6), ip12, ip12, 1, 1, b, 1)
+    call codens_check(codens_met_smmtr(b, codens_menter_now(), int(imxclko, c_int32_t), dv, int(ip11, c_int64_t), &
+         dmu, dmut, ddist, dfd, dg, dtp, dcf, dfr, d5, d6, int(nf, c_int32_t), codens_face_start, dl, dmn, &
+         c_null_ptr), 'met_smmtr')
+    call codens_back(c_loc(cfke), ip13, 1, 1, b)
+    call codens_back(c_loc(qcxts5), ip12, 1, 1, b)
+    call codens_back(c_loc(qcyts6), ip12, 1, 1, b)
+    call codens_check(codens_host_fetch(c_loc(frac), 0_c_int64_t, 8_c_int64_t*ip12, 8_c_int64_t*ip12, 1_c_int32_t, &
+         c_null_ptr), 'met_smmtr')
+  end subroutine met_smmtr
+end module mod_met_smmtr

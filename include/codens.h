@@ -339,6 +339,15 @@ int codens_met_smmt(const codens_block *b, const double *mc, int32_t kfracom, in
                     int32_t nfaces, const int64_t *face_start, const codens_wall_line *lines, const int32_t *mnpar,
                     void *stream);
 
+/* met_smmtr.f90:4 -- as codens_met_smmt with kfracom = 1 (the blending
+ * function always, gkgo kept internal) and Durbin's realizability limit:
+ * ts7's production divided by min(1, (1/sqrt(0.09)) / (S/omega)), S from
+ * the velocity gradients g9 (block-local, 9 x nnn). */
+int codens_met_smmtr(const codens_block *b, const double *mc, int32_t imxclko, const double *v, int64_t ldv,
+                     const double *mu, const double *mut, const double *dist, const double *fd, const double *g9,
+                     const double *tprod, double *cfke, double *frac, double *ts6, double *ts7, int32_t nfaces,
+                     const int64_t *face_start, const codens_wall_line *lines, const int32_t *mnpar, void *stream);
+
 /* met_fludmt.f90:4 -- fd(1:3) *= mu + mut sigme, fd(4:6) *= mu + mut sigma
  * on indc(i1,j1,k1)..indc(i2-1,j2-1,k2-1), the sigmas blended with frac when
  * kfracom != 0 (frac read only then). */

@@ -167,3 +167,39 @@ void orc_met_komut(const orc_block *b, double betae, int komsst, const double *v
         mut[n] = v[5 * ldv + n] / (omeg > lim ? omeg : lim);
       }
 }
+
+/* met_smmtr.f90:4 -- Menter sources with Durbin's realizability limit: the
+ * blending function always (as met_smmt with kfracom = 1, gkgo recomputed
+ * in both loops, :60-89) with the wall correction (:90-92), then ts7's
+ * production divided by alpha = min(1, (1/sqrt(0.09)) / (S/omega)), S from
+ * the velocity gradients g9 (:94-119). */
+void orc_met_smmtr(const orc_block *b, const orc_menter *mc, int imxclko, const double *v, long ldv,
+                   const double *mu, const double *mut, const double *dist, const double *fd, const double *g9,
+                   long ldg, const double *tprod, double *cfke, double *frac, double *ts6, double *ts7,
+                   const orc_wall_line *lines, long nlines, const int32_t *mnpar) {
+  long n0 = b->npc;
+  double *gk = calloc(b->nnn, sizeof(double));
+  double rcmu = 1. / sqrt(0.09);
+  orc_met_smmt(b, mc, 1, imxclko, v, ldv, mu, mut, dist, fd, tprod, cfke, gk, frac, ts6, ts7, lines, nlines, mnpar);
+  double xgam1 = mc->beta1 / mc->betae - mc->sigma1 * (mc->okappa * mc->okappa) / sqrt(mc->betae);
+  double xgam2 = mc->beta2 / mc->betae - mc->sigma2 * (mc->okappa * mc->okappa) / sqrt(mc->betae);
+  const double *dxx = g9, *dxy = g9 + ldg, *dxz = g9 + 2 * ldg, *dyx = g9 + 3 * ldg, *dyy = g9 + 4 * ldg;
+  const double *dyz = g9 + 5 * ldg, *dzx = g9 + 6 * ldg, *dzy = g9 + 7 * ldg, *dzz = g9 + 8 * ldg;
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - n0;
+        double fm1 = frac[n], fm2 = 1. - fm1;
+        double wgam = fm1 * xgam1 + fm2 * xgam2;
+        double beta = fm1 * mc->beta1 + fm2 * mc->beta2;
+        double omeg = v[6 * ldv + n] / v[n];
+        double wsig = fm1 * mc->wsig1 + fm2 * mc->wsig2;
+        double sl3 = 2. * wsig * v[n] * gk[m] / omeg;
+        double a = dxx[m] * dxx[m] + dyy[m] * dyy[m] + dzz[m] * dzz[m];
+        double s1 = dzy[m] + dyz[m], s2 = dxz[m] + dzx[m], s3 = dyx[m] + dxy[m];
+        double ss = sqrt(4. * a / 3. + s1 * s1 + s2 * s2 + s3 * s3) / omeg;
+        double alpha = rcmu / ss < 1. ? rcmu / ss : 1.;
+        ts7[n] = tprod[m] * wgam * v[n] / mut[n] / alpha - beta * v[6 * ldv + n] * omeg + sl3;
+      }
+  free(gk);
+}

@@ -525,4 +525,15 @@ contains
          dist, v, mu, mut, cmu, cmu, cmu, cmu, cmu, cmu)
   end subroutine ref_met_komut
 
+  ! met_smmtr.f90:4
+  subroutine ref_met_smmtr(l, v, mu, mut, dist, mnpar, ncin, fd, g, tprod, cfke, frac, ts6, ts7) &
+       bind(C, name="ref_met_smmtr")
+    use mod_met_smmtr
+    integer(c_int), value :: l
+    integer(c_int) :: mnpar(*), ncin(*)
+    real(c_double) :: v(*), mu(*), mut(*), dist(*), fd(ip11, 6), g(ip00, 9), tprod(*), cfke(*), frac(*), ts6(*), ts7(*)
+    call met_smmtr(l, v, mu, mut, dist, mnpar, ncin, fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6), &
+         g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), g(1,8), g(1,9), tprod, cfke, frac, ts6, ts7)
+  end subroutine ref_met_smmtr
+
 end module codens_ref_harness

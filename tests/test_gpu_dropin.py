@@ -326,6 +326,14 @@ def test_menter_routines(tsetup, kfracom):
                     close(r_[11][inner], g_[11][inner])
                     close(r_[12][inner], g_[12][inner])
                     frac = r_[10]
+                    g9 = np.zeros((9, lay.ip00))
+                    g9[:, :b.nnn] = rng.standard_normal((9, b.nnn)) * 10 ** rng.uniform(0.5, 3.5, b.nnn)
+                    r2, g2 = _both(R, G, "ref_met_smmtr", l + 1, v, mu, mut, dw, mnpar, ncin, fd, g9, tp,
+                                   np.full(n, 7.0), np.full(n, 7.0), np.full(n, 7.0), np.full(n, 7.0))
+                    # cfke, frac, ts6, ts7
+                    for q in (9, 11, 12):
+                        close(r2[q][inner], g2[q][inner])
+                    close(r2[10], g2[10])
                     r_, g_ = _both(R, G, "ref_met_fludmt", l + 1, v, mu, mut, frac, fd)
                     close(r_[4][:, inner], g_[4][:, inner])
                 g0 = np.zeros((9, lay.ip00))

@@ -269,6 +269,17 @@ def test_menter(setup, kfracom):
             assert relerr(H(g[k]), o[k]) <= TOL, (l, k)
         if kfracom:
             assert (o["frac"][inner] < 1).any()
+        # the realizability-limited sources (blending function always on)
+        g9 = rng.standard_normal((9, b.nnn)) * 10 ** rng.uniform(0.5, 3.5, b.nnn)
+        o2 = dict(cfke=np.full(n, 7.0), frac=np.full(n, 7.0), ts6=np.full(n, 7.0), ts7=np.full(n, 7.0))
+        g2 = {k: T(a) for k, a in o2.items()}
+        O.lib.orc_met_smmtr(ctypes.byref(ob), ptr(MENTER), imx, ptr(v), C.L_(n), ptr(mu), ptr(mt), ptr(dw), ptr(fd),
+                            ptr(g9), C.L_(b.nnn), ptr(tp), ptr(o2["cfke"]), ptr(o2["frac"]), ptr(o2["ts6"]),
+                            ptr(o2["ts7"]), ln.ctypes.data_as(ctypes.c_void_p), C.L_(len(ln)), ptr(mnpar))
+        abi.met_smmtr(abi.block(b), MENTER, imx, T(v), T(mu), T(mt), T(dw), T(fd), T(g9), T(tp), g2["cfke"],
+                      g2["frac"], g2["ts6"], g2["ts7"], face_start=st, lines=gl, mnpar=gmn)
+        for k in o2:
+            assert relerr(H(g2[k]), o2[k]) <= TOL, (l, "smmtr", k)
         # flux densities with the blending function just computed
         fo = fd.copy()
         O.lib.orc_met_fludmt(ctypes.byref(ob), ptr(MENTER), kfracom, ptr(mu), ptr(mt), ptr(o["frac"]), ptr(fo),

@@ -130,3 +130,39 @@ def test_komut(setup, komsst):
                         ptr(dist), ptr(gb), C.L_(b.nnn))
         assert np.array_equal(mr, mo)
         assert np.isfinite(mr[b.interior()]).all()
+
+
+def test_smmtr(setup):
+    """Sources with Durbin's realizability limit (blending function and
+    wall correction always on)."""
+    lay, sn, vol, v, mu, mut, dist, R, O, (ncin, lines, starts, owner, mnpar) = setup
+    n = lay.ip11
+    rng = np.random.default_rng(73)
+    _set(R, 1)
+    fd = np.zeros((6, n))
+    fd[:3] = 1e-3 * rng.standard_normal((3, n))
+    fd[3:] = 10 * rng.standard_normal((3, n))
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        g = np.zeros((9, lay.ip00))
+        g[:, :b.nnn] = rng.standard_normal((9, b.nnn)) * 10 ** rng.uniform(0.5, 3.5, b.nnn)
+        gb = np.ascontiguousarray(g[:, :b.nnn])
+        tp = np.zeros(lay.ip00); tp[:b.nnn] = 1e-3 * rng.random(b.nnn)
+        r = dict(cfke=np.full(n, 7.0), frac=np.full(n, 7.0), ts6=np.full(n, 7.0), ts7=np.full(n, 7.0))
+        o = {k: a.copy() for k, a in r.items()}
+        R.lib.ref_met_smmtr(l + 1, ptr(v), ptr(mu), ptr(mut), ptr(dist), ptr(mnpar), ptr(ncin), ptr(fd), ptr(g),
+                            ptr(tp), ptr(r["cfke"]), ptr(r["frac"]), ptr(r["ts6"]), ptr(r["ts7"]))
+        ln, st = walls_of_block(lines, starts, owner, l)
+        O.orc_met_smmtr(ctypes.byref(ob), ptr(MENTER), IMX, ptr(v), C.L_(n), ptr(mu), ptr(mut), ptr(dist), ptr(fd),
+                        ptr(gb), C.L_(b.nnn), ptr(tp), ptr(o["cfke"]), ptr(o["frac"]), ptr(o["ts6"]), ptr(o["ts7"]),
+                        ln.ctypes.data_as(ctypes.c_void_p), C.L_(len(ln)), ptr(mnpar))
+        for k in r:
+            assert np.array_equal(r[k], o[k]), (l, k)
+        # the limit is active somewhere and not everywhere
+        o2 = {k: np.full(n, 7.0) for k in r}
+        O.orc_met_smmt(ctypes.byref(ob), ptr(MENTER), 1, IMX, ptr(v), C.L_(n), ptr(mu), ptr(mut), ptr(dist), ptr(fd),
+                       ptr(tp), ptr(o2["cfke"]), ptr(np.zeros(lay.ip00)), ptr(o2["frac"]), ptr(o2["ts6"]),
+                       ptr(o2["ts7"]), ln.ctypes.data_as(ctypes.c_void_p), C.L_(len(ln)), ptr(mnpar))
+        inner = b.interior()
+        same = r["ts7"][inner] == o2["ts7"][inner]
+        assert same.any() and not same.all()

@@ -21,6 +21,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 SWEEP_BYTES_PER_CELL = 8 * (5 + 5 + 5 + 9 + 1 + 3 + 5)  # u, v, d_in, sn, diag, coefe, d_out
+# Memory traffic of one sweep launch on the 256x128x64 workload: 2.0 x FETCH_SIZE
+# + 1.0 x WRITE_SIZE (factors from a 512 MiB device copy in the same run),
+# 632,512 KiB = 647.7 MB against 553.6 MB algorithmic
+# (profiles/traffic_implimf_sweep_wall_ns_256x128x64.md).
+SWEEP_TRAFFIC_BYTES = 632512 * 1024
 HBM_PEAK_GBS = 8000.0
 
 
@@ -110,7 +115,9 @@ def main():
                    "cells": ncell, "lmax": case.lmax},
         "roofline": {"bound": "hbm", "kernel": "implimf_sweep", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_cell": SWEEP_BYTES_PER_CELL, "avg_us": round(sw["avg_us"], 2)},
+                     "traffic": SWEEP_TRAFFIC_BYTES if (ci, cj, ck) == (256, 128, 64) else None,
+                     "bytes_per_cell": SWEEP_BYTES_PER_CELL,
+                     "avg_us": round(sw["avg_us"], 2)},
         "kernels": kern,
         "density_residual": res,
     }

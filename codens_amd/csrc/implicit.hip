@@ -12,35 +12,47 @@
 
 using namespace codens;
 
-// Increment buffer view: element (e, n) at p[e*ld + n - off].
-struct DView {
-  double *p;
+// Increment buffer view: element (e, n) at p[e*ld + n - off].  T = double
+// for the FP64 phase, float for the FP32 defect-correction form.
+template <typename T>
+struct TView {
+  T *p;
   long ld, off;
-  __device__ inline double &at(int e, long n) const { return p[e * ld + n - off]; }
+  __device__ inline T &at(int e, long n) const { return p[e * ld + n - off]; }
 };
+using DView = TView<double>;
 
-struct DFlux {  // mass-flux increment (d of rho u, rho v, rho w) and dF
-  double mx, my, mz, xx, xy, xz, yy, yz, zz, ex, ey, ez;
+// sn(m, d, c) of a block's metric slice in either precision
+template <typename T>
+__device__ inline T snt(const T *__restrict__ sn, long nnn, long m, int d, int c) {
+  return sn[((long)c * 3 + d) * nnn + m];
+}
+
+template <typename T>
+struct DFluxT {  // mass-flux increment (d of rho u, rho v, rho w) and dF
+  T mx, my, mz, xx, xy, xz, yy, yz, zz, ex, ey, ez;
 };
+using DFlux = DFluxT<double>;
 
 // implimf_3d.f90:434-465 for one cell: dF = F(v + d) - F(v), p(v) from the EOS
 // as zpres.f90:72-73 computes it.  The velocities are formed with one
 // reciprocal of each density instead of the reference's divisions (about
 // 1e-16 relative per value; the sub-iteration result moves by ~3e-14).
-__device__ inline DFlux dflux(const Phys &ph, const double V[5], const double D[5]) {
-  const double gam1 = ph.gam1, pinfl = ph.pinfl;
-  double wi1 = V[0] + D[0], wi2 = V[1] + D[1], wi3 = V[2] + D[2], wi4 = V[3] + D[3], wi5 = V[4] + D[4];
-  double ri = 1.0 / wi1, r0 = 1.0 / V[0];
-  double ui = wi2 * ri, vi = wi3 * ri, wi = wi4 * ri;
-  double pres = gam1 * (wi5 - .5 * wi1 * (ui * ui + vi * vi + wi * wi) - pinfl);
-  double a = V[1] * r0, bb = V[2] * r0, c = V[3] * r0;
-  double rhoe = V[4] - 0.5 * (V[1] * a + V[2] * bb + V[3] * c);
-  double p = gam1 * (rhoe - pinfl);
-  double fxx = V[1] * a + p, fxy = V[2] * a, fxz = V[3] * a;
-  double fyy = V[2] * bb + p, fyz = V[3] * bb, fzz = V[3] * c + p;
-  double e5 = V[4] + p - pinfl;
-  double fex = e5 * a, fey = e5 * bb, fez = e5 * c;
-  DFlux r;
+template <typename T>
+__device__ inline DFluxT<T> dflux(const Phys &ph, const T V[5], const T D[5]) {
+  const T gam1 = (T)ph.gam1, pinfl = (T)ph.pinfl;
+  T wi1 = V[0] + D[0], wi2 = V[1] + D[1], wi3 = V[2] + D[2], wi4 = V[3] + D[3], wi5 = V[4] + D[4];
+  T ri = T(1) / wi1, r0 = T(1) / V[0];
+  T ui = wi2 * ri, vi = wi3 * ri, wi = wi4 * ri;
+  T pres = gam1 * (wi5 - T(.5) * wi1 * (ui * ui + vi * vi + wi * wi) - pinfl);
+  T a = V[1] * r0, bb = V[2] * r0, c = V[3] * r0;
+  T rhoe = V[4] - T(0.5) * (V[1] * a + V[2] * bb + V[3] * c);
+  T p = gam1 * (rhoe - pinfl);
+  T fxx = V[1] * a + p, fxy = V[2] * a, fxz = V[3] * a;
+  T fyy = V[2] * bb + p, fyz = V[3] * bb, fzz = V[3] * c + p;
+  T e5 = V[4] + p - pinfl;
+  T fex = e5 * a, fey = e5 * bb, fez = e5 * c;
+  DFluxT<T> r;
   r.mx = D[1]; r.my = D[2]; r.mz = D[3];
   r.xx = wi1 * (ui * ui) + pres - fxx;
   r.xy = wi1 * ui * vi - fxy;
@@ -212,18 +224,21 @@ constexpr int SW_TX = 64, SW_TY = CODENS_SW_TY, SW_KS = CODENS_SW_KS;
 static_assert(2 * SW_TX + 2 * SW_TY <= SW_TX * SW_TY, "one halo cell per thread at most");
 constexpr int SW_HALO = 2 * SW_TX + 2 * SW_TY;
 
-struct Q14 {
-  double q[14];
+template <typename T>
+struct Q14T {
+  T q[14];
 };
+using Q14 = Q14T<double>;
 
-__device__ inline void q_of(const Phys &ph, const double *__restrict__ v, long ldv, const DView &d, long n, Q14 &Q) {
-  double V[5], D[5];
+template <typename T>
+__device__ inline void q_of(const Phys &ph, const T *__restrict__ v, long ldv, const TView<T> &d, long n, Q14T<T> &Q) {
+  T V[5], D[5];
 #pragma unroll
   for (int e = 0; e < 5; e++) {
     V[e] = v[e * ldv + n];
     D[e] = d.at(e, n);
   }
-  DFlux F = dflux(ph, V, D);
+  DFluxT<T> F = dflux<T>(ph, V, D);
   Q.q[0] = F.mx; Q.q[1] = F.my; Q.q[2] = F.mz;
   Q.q[3] = F.xx; Q.q[4] = F.xy; Q.q[5] = F.xz; Q.q[6] = F.yy; Q.q[7] = F.yz; Q.q[8] = F.zz;
   Q.q[9] = F.ex; Q.q[10] = F.ey; Q.q[11] = F.ez;
@@ -246,43 +261,48 @@ __device__ inline void q_of_eu(const Phys &ph, const double *__restrict__ v, lon
   Q.q[12] = D[0]; Q.q[13] = D[4];
 }
 
-template <bool EU>
-__device__ inline void q_model(const Phys &ph, const double *__restrict__ v, long ldv, const DView &d,
-                               const double *__restrict__ ps, long n, Q14 &Q) {
-  if (EU) q_of_eu(ph, v, ldv, d, ps, n, Q);
-  else q_of(ph, v, ldv, d, n, Q);
+template <bool EU, typename T>
+__device__ inline void q_model(const Phys &ph, const T *__restrict__ v, long ldv, const TView<T> &d,
+                               const double *__restrict__ ps, long n, Q14T<T> &Q) {
+  if constexpr (EU) q_of_eu(ph, v, ldv, d, ps, n, Q);
+  else q_of<T>(ph, v, ldv, d, n, Q);
 }
 
-__device__ inline void q_zero(Q14 &Q) {
+template <typename T>
+__device__ inline void q_zero(Q14T<T> &Q) {
 #pragma unroll
-  for (int c = 0; c < 14; c++) Q.q[c] = 0.;
+  for (int c = 0; c < 14; c++) Q.q[c] = T(0);
 }
 
 // The same 14-vector in two phases, so that the loads of several cells can
 // be in flight together: the inputs of a cell (zero for a cell outside the
 // real region), then q from them.
+template <typename T>
 struct VD {
-  double V[5], D[5], p;
+  T V[5], D[5];
+  double p;
 };
 
-template <bool EU>
-__device__ inline void vd_load(const double *__restrict__ v, long ldv, const DView &d, const double *__restrict__ ps,
-                               long n, bool real, VD &x) {
+template <bool EU, typename T>
+__device__ inline void vd_load(const T *__restrict__ v, long ldv, const TView<T> &d, const double *__restrict__ ps,
+                               long n, bool real, VD<T> &x) {
 #pragma unroll
   for (int e = 0; e < 5; e++) {
-    x.V[e] = real ? v[e * ldv + n] : 1.;
-    x.D[e] = real ? d.at(e, n) : 0.;
+    x.V[e] = real ? v[e * ldv + n] : T(1);
+    x.D[e] = real ? d.at(e, n) : T(0);
   }
   x.p = (EU && real) ? ps[n] : 0.;
 }
 
-template <bool EU>
-__device__ inline void vd_q(const Phys &ph, const VD &x, bool real, Q14 &Q) {
+template <bool EU, typename T>
+__device__ inline void vd_q(const Phys &ph, const VD<T> &x, bool real, Q14T<T> &Q) {
   if (!real) {
     q_zero(Q);
     return;
   }
-  DFlux F = EU ? dflux_eu(ph, x.V, x.D, x.p) : dflux(ph, x.V, x.D);
+  DFluxT<T> F;
+  if constexpr (EU) F = dflux_eu(ph, x.V, x.D, x.p);
+  else F = dflux<T>(ph, x.V, x.D);
   Q.q[0] = F.mx; Q.q[1] = F.my; Q.q[2] = F.mz;
   Q.q[3] = F.xx; Q.q[4] = F.xy; Q.q[5] = F.xz; Q.q[6] = F.yy; Q.q[7] = F.yz; Q.q[8] = F.zz;
   Q.q[9] = F.ex; Q.q[10] = F.ey; Q.q[11] = F.ez;
@@ -291,29 +311,33 @@ __device__ inline void vd_q(const Phys &ph, const VD &x, bool real, Q14 &Q) {
 
 
 // d of component e from the 14-vector
-__device__ inline double q_d(const double *q, int e) {
+template <typename T>
+__device__ inline T q_d(const T *q, int e) {
   return e == 0 ? q[12] : (e == 4 ? q[13] : q[e - 1]);
 }
 
 // tn of a face between cells A and B (implimf_3d.f90:276-292)
-__device__ inline void q_tn(const double *A, const double *B, double s1, double s2, double s3, double tn[5]) {
-  tn[0] = 0.5 * (A[0] + B[0]) * s1 + 0.5 * (A[1] + B[1]) * s2 + 0.5 * (A[2] + B[2]) * s3;
-  tn[1] = 0.5 * (A[3] + B[3]) * s1 + 0.5 * (A[4] + B[4]) * s2 + 0.5 * (A[5] + B[5]) * s3;
-  tn[2] = 0.5 * (A[4] + B[4]) * s1 + 0.5 * (A[6] + B[6]) * s2 + 0.5 * (A[7] + B[7]) * s3;
-  tn[3] = 0.5 * (A[5] + B[5]) * s1 + 0.5 * (A[7] + B[7]) * s2 + 0.5 * (A[8] + B[8]) * s3;
-  tn[4] = 0.5 * (A[9] + B[9]) * s1 + 0.5 * (A[10] + B[10]) * s2 + 0.5 * (A[11] + B[11]) * s3;
+template <typename T>
+__device__ inline void q_tn(const T *A, const T *B, T s1, T s2, T s3, T tn[5]) {
+  const T h = T(0.5);
+  tn[0] = h * (A[0] + B[0]) * s1 + h * (A[1] + B[1]) * s2 + h * (A[2] + B[2]) * s3;
+  tn[1] = h * (A[3] + B[3]) * s1 + h * (A[4] + B[4]) * s2 + h * (A[5] + B[5]) * s3;
+  tn[2] = h * (A[4] + B[4]) * s1 + h * (A[6] + B[6]) * s2 + h * (A[7] + B[7]) * s3;
+  tn[3] = h * (A[5] + B[5]) * s1 + h * (A[7] + B[7]) * s2 + h * (A[8] + B[8]) * s3;
+  tn[4] = h * (A[9] + B[9]) * s1 + h * (A[10] + B[10]) * s2 + h * (A[11] + B[11]) * s3;
 }
 
 // First sub-iteration: d and dF are zero, so d = -(u (+ff)) / diag.
-__global__ void __launch_bounds__(TX *TY) k_impl_sweep1(Blk b, int ityprk, const double *__restrict__ u,
-                                                        const double *__restrict__ ff, long ldv,
-                                                        const double *__restrict__ diag, DView dnew) {
+template <typename T>
+__global__ void __launch_bounds__(TX *TY) k_impl_sweep1(Blk b, int ityprk, const T *__restrict__ u,
+                                                        const T *__restrict__ ff, long ldv,
+                                                        const T *__restrict__ diag, TView<T> dnew) {
   CELL3(b)
   long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
-  double dg = diag[m];
+  T dg = diag[m];
 #pragma unroll
   for (int e = 0; e < 5; e++) {
-    double r = ityprk == 0 ? -u[e * ldv + n] : -u[e * ldv + n] - ff[e * ldv + n];
+    T r = ityprk == 0 ? -u[e * ldv + n] : -u[e * ldv + n] - ff[e * ldv + n];
     dnew.at(e, n) = r / dg;
   }
 }
@@ -322,17 +346,20 @@ constexpr int SW_PL = (SW_TY + 2) * (SW_TX + 2);  // LDS plane stride between th
 
 // face term of the face between the LDS cells a (own side) and o (other side),
 // components read straight from LDS: tn = 0.5 (q_a + q_o) . S
-__device__ inline double lds_tn(const double *a, const double *o, int e, double s1, double s2, double s3) {
+template <typename T>
+__device__ inline T lds_tn(const T *a, const T *o, int e, T s1, T s2, T s3) {
   const int P = SW_PL;
+  const T h = T(0.5);
   switch (e) {
-    case 0: return 0.5 * (a[0] + o[0]) * s1 + 0.5 * (a[P] + o[P]) * s2 + 0.5 * (a[2 * P] + o[2 * P]) * s3;
-    case 1: return 0.5 * (a[3 * P] + o[3 * P]) * s1 + 0.5 * (a[4 * P] + o[4 * P]) * s2 + 0.5 * (a[5 * P] + o[5 * P]) * s3;
-    case 2: return 0.5 * (a[4 * P] + o[4 * P]) * s1 + 0.5 * (a[6 * P] + o[6 * P]) * s2 + 0.5 * (a[7 * P] + o[7 * P]) * s3;
-    case 3: return 0.5 * (a[5 * P] + o[5 * P]) * s1 + 0.5 * (a[7 * P] + o[7 * P]) * s2 + 0.5 * (a[8 * P] + o[8 * P]) * s3;
-    default: return 0.5 * (a[9 * P] + o[9 * P]) * s1 + 0.5 * (a[10 * P] + o[10 * P]) * s2 + 0.5 * (a[11 * P] + o[11 * P]) * s3;
+    case 0: return h * (a[0] + o[0]) * s1 + h * (a[P] + o[P]) * s2 + h * (a[2 * P] + o[2 * P]) * s3;
+    case 1: return h * (a[3 * P] + o[3 * P]) * s1 + h * (a[4 * P] + o[4 * P]) * s2 + h * (a[5 * P] + o[5 * P]) * s3;
+    case 2: return h * (a[4 * P] + o[4 * P]) * s1 + h * (a[6 * P] + o[6 * P]) * s2 + h * (a[7 * P] + o[7 * P]) * s3;
+    case 3: return h * (a[5 * P] + o[5 * P]) * s1 + h * (a[7 * P] + o[7 * P]) * s2 + h * (a[8 * P] + o[8 * P]) * s3;
+    default: return h * (a[9 * P] + o[9 * P]) * s1 + h * (a[10 * P] + o[10 * P]) * s2 + h * (a[11 * P] + o[11 * P]) * s3;
   }
 }
-__device__ inline double lds_d(const double *a, int e) {
+template <typename T>
+__device__ inline T lds_d(const T *a, int e) {
   const int P = SW_PL;
   return e == 0 ? a[12 * P] : (e == 4 ? a[13 * P] : a[(e - 1) * P]);
 }
@@ -351,17 +378,17 @@ __device__ inline void lds_sync() {
 #endif
 // EU selects the flux increments of implimf_eu (which read ps); the face
 // terms, coefficients and update are common to both implicit phases.
-template <bool EU>
+template <bool EU, typename T>
 __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b, Phys ph, int ityprk, int ntx, int nty,
-                                                                   int nkz, const double *__restrict__ u,
-                                                                   const double *__restrict__ ff,
-                                                                   const double *__restrict__ v, long ldv,
+                                                                   int nkz, const T *__restrict__ u,
+                                                                   const T *__restrict__ ff,
+                                                                   const T *__restrict__ v, long ldv,
                                                                    const double *__restrict__ ps,
-                                                                   const double *__restrict__ sn,
-                                                                   const double *__restrict__ coefe,
-                                                                   const double *__restrict__ diag, DView dold,
-                                                                   DView dnew) {
-  __shared__ double sq[14 * SW_PL];
+                                                                   const T *__restrict__ sn,
+                                                                   const T *__restrict__ coefe,
+                                                                   const T *__restrict__ diag, TView<T> dold,
+                                                                   TView<T> dnew) {
+  __shared__ T sq[14 * SW_PL];
   // XCD-aware tile order: the hardware deals workgroups to the 8 XCDs round
   // robin; give each XCD a contiguous run of tiles so that j/k neighbours
   // share its L2.
@@ -388,24 +415,24 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
   bool hreal = halo && hi_ >= b.i1 && hi_ < b.i2 && hj >= b.j1 && hj < b.j2;
   const long nnn = b.nnn, nijd = b.nijd;
   const int slot = (ty + 1) * (SW_TX + 2) + (tx + 1);
-  Q14 Qc;
-  double tlo_k[5], dlo_k[5];
+  Q14T<T> Qc;
+  T tlo_k[5], dlo_k[5];
   if (own) {
     long n = b.idx(i, j, kb), m = n - b.npc;
-    Q14 Qb;
-    if (kb > b.k1) q_model<EU>(ph, v, ldv, dold, ps, n - nijd, Qb); else q_zero(Qb);
-    q_model<EU>(ph, v, ldv, dold, ps, n, Qc);
-    q_tn(Qc.q, Qb.q, snv(sn, nnn, m, 2, 0), snv(sn, nnn, m, 2, 1), snv(sn, nnn, m, 2, 2), tlo_k);
+    Q14T<T> Qb;
+    if (kb > b.k1) q_model<EU, T>(ph, v, ldv, dold, ps, n - nijd, Qb); else q_zero(Qb);
+    q_model<EU, T>(ph, v, ldv, dold, ps, n, Qc);
+    q_tn(Qc.q, Qb.q, snt<T>(sn, nnn, m, 2, 0), snt<T>(sn, nnn, m, 2, 1), snt<T>(sn, nnn, m, 2, 2), tlo_k);
 #pragma unroll
     for (int e = 0; e < 5; e++) dlo_k[e] = q_d(Qb.q, e);
   } else {
     q_zero(Qc);
 #pragma unroll
-    for (int e = 0; e < 5; e++) tlo_k[e] = dlo_k[e] = 0.;
+    for (int e = 0; e < 5; e++) tlo_k[e] = dlo_k[e] = T(0);
   }
   // d of the previous plane, stored once the next plane's loads are issued
   // (the kernel never reads dnew, so the stores can trail by one plane)
-  double dpend[5];
+  T dpend[5];
   long npend = -1;
   for (int k = kb; k < ke; k++) {
     long n = own ? b.idx(i, j, k) : 0, m = n - b.npc;
@@ -414,12 +441,12 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
     // plane's flux increments are formed, so their latency overlaps that work:
     // rhs -u (-ff), the i/j face metrics of both faces, coefficients, diagonal
     // (-u - 0 is -u bit for bit, so ityprk = 0 takes the same form)
-    double uu[5], fv[5], sl[2][3], sh[2][3], cl[3], ch[3], dgm;
+    T uu[5], fv[5], sl[2][3], sh[2][3], cl[3], ch[3], dgm;
     if (own) {
 #pragma unroll
       for (int e = 0; e < 5; e++) {
         uu[e] = u[e * ldv + n];
-        fv[e] = ityprk ? ff[e * ldv + n] : 0.;
+        fv[e] = ityprk ? ff[e * ldv + n] : T(0);
       }
 #pragma unroll
       for (int d = 0; d < 2; d++) {
@@ -427,25 +454,25 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
         bool rreal = d == 0 ? (i + 1 < b.i2) : (j + 1 < b.j2);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          sl[d][c] = snv(sn, nnn, m, d, c);
-          sh[d][c] = snv(sn, nnn, m + inc, d, c);
+          sl[d][c] = snt<T>(sn, nnn, m, d, c);
+          sh[d][c] = snt<T>(sn, nnn, m + inc, d, c);
         }
         cl[d] = coefe[d * nnn + m];
-        ch[d] = rreal ? coefe[d * nnn + m + inc] : 0.;
+        ch[d] = rreal ? coefe[d * nnn + m + inc] : T(0);
       }
       cl[2] = coefe[2 * nnn + m];
-      ch[2] = kreal ? coefe[2 * nnn + m + nijd] : 0.;
+      ch[2] = kreal ? coefe[2 * nnn + m + nijd] : T(0);
       dgm = diag[m];
     }
     // inputs of this thread's halo cell (plane k) and of its own cell on
     // plane k+1, loaded together
-    VD xh, xn;
+    VD<T> xh, xn;
     bool hq = halo && hreal, nq = own && kreal;
-    vd_load<EU>(v, ldv, dold, ps, hq ? b.idx(hi_, hj, k) : n, hq, xh);
-    vd_load<EU>(v, ldv, dold, ps, n + nijd, nq, xn);
-    double skn[3];
+    vd_load<EU, T>(v, ldv, dold, ps, hq ? b.idx(hi_, hj, k) : n, hq, xh);
+    vd_load<EU, T>(v, ldv, dold, ps, n + nijd, nq, xn);
+    T skn[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) skn[c] = own ? snv(sn, nnn, m + nijd, 2, c) : 0.;
+    for (int c = 0; c < 3; c++) skn[c] = own ? snt<T>(sn, nnn, m + nijd, 2, c) : T(0);
     if (npend >= 0) {
 #pragma unroll
       for (int e = 0; e < 5; e++) dnew.at(e, npend) = dpend[e];
@@ -454,16 +481,16 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
 #pragma unroll
     for (int c = 0; c < 14; c++) sq[c * SW_PL + slot] = Qc.q[c];
     if (halo) {
-      Q14 Qh;
-      vd_q<EU>(ph, xh, hq, Qh);
+      Q14T<T> Qh;
+      vd_q<EU, T>(ph, xh, hq, Qh);
 #pragma unroll
       for (int c = 0; c < 14; c++) sq[c * SW_PL + hslot] = Qh.q[c];
     }
     // k face above: q of the next plane and its face term
-    double thi_k[5], dhi_k[5];
+    T thi_k[5], dhi_k[5];
     if (own) {
-      Q14 Qn;
-      vd_q<EU>(ph, xn, kreal, Qn);
+      Q14T<T> Qn;
+      vd_q<EU, T>(ph, xn, kreal, Qn);
       q_tn(Qn.q, Qc.q, skn[0], skn[1], skn[2], thi_k);
 #pragma unroll
       for (int e = 0; e < 5; e++) dhi_k[e] = q_d(Qn.q, e);
@@ -471,19 +498,19 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
     }
     lds_sync();
     if (own) {
-      double d2w[5];
+      T d2w[5];
 #pragma unroll
       for (int e = 0; e < 5; e++) d2w[e] = -uu[e] - fv[e];
-      const double *A = sq + slot;
+      const T *A = sq + slot;
       // i then j directions from LDS (implimf_3d.f90:268-362)
 #pragma unroll
       for (int d = 0; d < 2; d++) {
-        const double *Lp = d == 0 ? A - 1 : A - (SW_TX + 2);
-        const double *Rp = d == 0 ? A + 1 : A + (SW_TX + 2);
+        const T *Lp = d == 0 ? A - 1 : A - (SW_TX + 2);
+        const T *Rp = d == 0 ? A + 1 : A + (SW_TX + 2);
         bool rreal = d == 0 ? (i + 1 < b.i2) : (j + 1 < b.j2);
 #pragma unroll
         for (int e = 0; e < 5; e++) {
-          double tt = d2w[e] + lds_tn(A, Lp, e, sl[d][0], sl[d][1], sl[d][2]) + cl[d] * lds_d(Lp, e);
+          T tt = d2w[e] + lds_tn(A, Lp, e, sl[d][0], sl[d][1], sl[d][2]) + cl[d] * lds_d(Lp, e);
           if (rreal) tt = tt + ch[d] * lds_d(Rp, e);
           d2w[e] = tt - lds_tn(Rp, A, e, sh[d][0], sh[d][1], sh[d][2]);
         }
@@ -491,11 +518,11 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) SW_ATTR k_impl_sweep_tiled(Blk b
       // k direction from registers
 #pragma unroll
       for (int e = 0; e < 5; e++) {
-        double tt = d2w[e] + tlo_k[e] + cl[2] * dlo_k[e];
+        T tt = d2w[e] + tlo_k[e] + cl[2] * dlo_k[e];
         if (kreal) tt = tt + ch[2] * dhi_k[e];
         d2w[e] = tt - thi_k[e];
       }
-      double rdg = 1.0 / dgm;
+      T rdg = T(1) / dgm;
 #pragma unroll
       for (int e = 0; e < 5; e++) dpend[e] = d2w[e] * rdg;
       npend = n;
@@ -576,20 +603,107 @@ static int implimf_common(bool eu, const codens_block *cb, const codens_phys *cp
     DView in = ((lmx - s) % 2 == 0) ? B : A;
     if (s == 1) {
       CodensTimed _tm("implimf_sweep1", st);
-      hipLaunchKernelGGL(k_impl_sweep1, g, t, 0, st, b, (int)ityprk, u, ityprk ? ff : u, (long)ldv, diag, out);
+      hipLaunchKernelGGL(k_impl_sweep1<double>, g, t, 0, st, b, (int)ityprk, u, ityprk ? ff : u, (long)ldv, diag, out);
     } else {
       CodensTimed _tm("implimf_sweep", st);
       if (eu)
-        hipLaunchKernelGGL(k_impl_sweep_tiled<true>, dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY), 0, st, b, ph,
+        hipLaunchKernelGGL((k_impl_sweep_tiled<true, double>), dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY), 0, st, b, ph,
                            (int)ityprk, ntx, nty, nkz, u, ityprk ? ff : u, v, (long)ldv, ps, sn, coefe, diag, in, out);
       else
-        hipLaunchKernelGGL(k_impl_sweep_tiled<false>, dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY), 0, st, b, ph,
+        hipLaunchKernelGGL((k_impl_sweep_tiled<false, double>), dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY), 0, st, b, ph,
                            (int)ityprk, ntx, nty, nkz, u, ityprk ? ff : u, v, (long)ldv, v, sn, coefe, diag, in, out);
     }
   }
   {
     CodensTimed _tm("implimf_update", st);
     hipLaunchKernelGGL(k_impl_update, g, t, 0, st, b, v, (long)ldv, d, nvar);
+  }
+  return codens_check_launch(what);
+}
+
+// ---------------------------------------------------------------------------
+// FP32 defect-correction form.  np planes of a source (stride lds, storage
+// entries npc .. npc+nnn-1) rounded to float planes of stride nnn.
+__global__ void __launch_bounds__(256) k_to_f32(long nnn, long npc, int np, const double *__restrict__ src, long lds,
+                                                float *__restrict__ dst) {
+  long m = blockIdx.x * 256L + threadIdx.x;
+  if (m >= nnn) return;
+  for (int e = 0; e < np; e++) dst[e * nnn + m] = (float)src[e * lds + npc + m];
+}
+
+// d = the FP32 increment widened to FP64, v += d (real cells)
+__global__ void __launch_bounds__(TX *TY) k_impl_update_f32(Blk b, double *v, long ldv, double *d,
+                                                            const float *__restrict__ df) {
+  CELL3(b)
+  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+#pragma unroll
+  for (int e = 0; e < 5; e++) {
+    double x = (double)df[e * b.nnn + m];
+    d[e * ldv + n] = x;
+    v[e * ldv + n] = v[e * ldv + n] + x;
+  }
+}
+
+extern "C" int codens_implimf_3d_f32(const codens_block *cb, const codens_phys *cp, const double *u,
+                                     const double *dt, double *v, int64_t ldv, double *d, const double *ff,
+                                     const double *mu, const double *mut, int32_t lmx, const double *sn,
+                                     const double *vol, double dtpas, int32_t kfmg, int32_t ityprk, const double *ps,
+                                     const double *cson, void *stream) {
+  const char *what = "codens_implimf_3d_f32";
+  int rc = codens_check_block(cb, what);
+  if (rc) return rc;
+  if (!cp || !u || !dt || !v || !d || !mu || !mut || !sn || !vol || !cson || (ityprk && !ff) || lmx < 0) {
+    codens_set_error("%s: invalid argument", what);
+    return CODENS_EINVAL;
+  }
+  Blk b = make_blk(cb);
+  Phys ph = make_phys(cp);
+  hipStream_t st = (hipStream_t)stream;
+  const long nnn = b.nnn;
+  double *ws = (double *)codens_workspace(sizeof(double) * 4 * nnn, 2);
+  // float planes: v 5, u 5, ff 5, sn 9, coefe 3 + diag 1, two increments 5 + 5
+  float *wf = (float *)codens_workspace(sizeof(float) * 38 * nnn, 6);
+  if (!ws || !wf) {
+    codens_set_error("%s: workspace allocation failed", what);
+    return CODENS_EHIP;
+  }
+  float *vF = wf, *uF = wf + 5 * nnn, *fF = wf + 10 * nnn, *snF = wf + 15 * nnn, *cF = wf + 24 * nnn;
+  float *dgF = cF + 3 * nnn, *dA = wf + 28 * nnn, *dB = wf + 33 * nnn;
+  dim3 g1((unsigned)((nnn + 255) / 256)), t1(256);
+  hipLaunchKernelGGL(k_zero5, g1, t1, 0, st, d, (long)ldv, (long)b.npc, (long)(b.npc + nnn - 1));
+  dim3 g = cell_grid(b), t(TX, TY);
+  {
+    CodensTimed _tm("implimf32_coef", st);
+    hipLaunchKernelGGL(k_impl_coef, g, t, 0, st, b, ph, v, (long)ldv, mu, mut, dt, vol, cson, sn, dtpas, (int)kfmg,
+                       ws, ws + 3 * nnn);
+    hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, 0L, 4, (const double *)ws, nnn, cF);
+    hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, (long)b.npc, 5, v, (long)ldv, vF);
+    hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, (long)b.npc, 5, u, (long)ldv, uF);
+    if (ityprk) hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, (long)b.npc, 5, ff, (long)ldv, fF);
+    hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, 0L, 9, sn, nnn, snF);
+  }
+  // storage-indexed views of the block-local float planes (entry n at n - npc)
+  const float *vS = vF - b.npc, *uS = uF - b.npc, *fS = fF - b.npc;
+  TView<float> A = {dA, nnn, b.npc}, B = {dB, nnn, b.npc};
+  int ntx = (b.i2 - b.i1 + SW_TX - 1) / SW_TX, nty = (b.j2 - b.j1 + SW_TY - 1) / SW_TY;
+  int nkz = (b.k2 - b.k1 + SW_KS - 1) / SW_KS;
+  TView<float> last = A;
+  for (int s = 1; s <= lmx; s++) {
+    TView<float> out = (s % 2) ? A : B, in = (s % 2) ? B : A;
+    if (s == 1) {
+      CodensTimed _tm("implimf32_sweep1", st);
+      hipLaunchKernelGGL(k_impl_sweep1<float>, g, t, 0, st, b, (int)ityprk, uS, ityprk ? fS : uS, nnn, dgF, out);
+    } else {
+      CodensTimed _tm("implimf32_sweep", st);
+      hipLaunchKernelGGL((k_impl_sweep_tiled<false, float>), dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY), 0, st, b,
+                         ph, (int)ityprk, ntx, nty, nkz, uS, ityprk ? fS : uS, vS, nnn, (const double *)nullptr,
+                         snF, cF, dgF, in, out);
+    }
+    last = out;
+  }
+  if (lmx > 0) {
+    CodensTimed _tm("implimf32_update", st);
+    hipLaunchKernelGGL(k_impl_update_f32, g, t, 0, st, b, v, (long)ldv, d, (const float *)last.p);
   }
   return codens_check_launch(what);
 }

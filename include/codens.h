@@ -230,6 +230,18 @@ int codens_implimf_3d(const codens_block *b, const codens_phys *ph, const double
                       const double *vol, double dtpas, int32_t kfmg, int32_t ityprk,
                       const double *ps, const double *cson, void *stream);
 
+/* FP32 defect-correction form of codens_implimf_3d, with the same arguments
+ * and results: the coefficients are computed in FP64 and rounded, and the
+ * lmx Jacobi sub-iterations run in FP32 on FP32 copies of v, u, ff and sn;
+ * d comes back widened to FP64 and v += d in FP64.  It has no counterpart in
+ * the reference (implimf_3d.f90:4 is FP64 throughout).  It is for drivers
+ * that iterate to a residual drop: the FP64 residual u fixes the converged
+ * state, and the FP32 phase only changes the path to it. */
+int codens_implimf_3d_f32(const codens_block *b, const codens_phys *ph, const double *u, const double *dt, double *v,
+                          int64_t ldv, double *d, const double *ff, const double *mu, const double *mut, int32_t lmx,
+                          const double *sn_blk, const double *vol, double dtpas, int32_t kfmg, int32_t ityprk,
+                          const double *ps, const double *cson, void *stream);
+
 /* implimf_eu.f90:6 -- point-Jacobi implicit phase of the Euler equations:
  * coefe = 0.5|V.S| + |S|c, diag = vol (fact + 1/dt) + sum of coefe with
  * fact = 1.5/dt1min when kfmg = 3, flux increments from the momenta and the

@@ -60,6 +60,8 @@ SIGNATURES = {
     "codens_sch_expli": [P(CodensBlock), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
     "codens_implimf_3d": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int,
                           c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp, c_vp],
+    "codens_implimf_3d_f32": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int,
+                          c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp, c_vp],
     "codens_bc_coincident": [c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
     "codens_bc_wall": [c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
     "codens_bc_farfield": [c_i64, c_vp, c_vp, c_vp, P(c_dbl), P(CodensPhys), c_vp, c_i64, c_vp, c_vp, c_vp,
@@ -78,6 +80,8 @@ SIGNATURES = {
     "codens_met_smch": [P(CodensBlock), c_dbl, c_dbl, c_dbl, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                         c_vp],
     "codens_met_cut": [P(CodensBlock), c_int, c_dbl, c_dbl, c_vp, c_i64, c_vp],
+    "codens_met_smdes": [P(CodensBlock), P(c_dbl), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp,
+                         c_vp, c_vp, c_vp, c_vp, c_vp],
     "codens_met_mtcorf1": [c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_i64, c_vp],
     "codens_met_smmt": [P(CodensBlock), c_vp, c_int, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                         c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],

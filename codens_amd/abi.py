@@ -178,6 +178,12 @@ def met_cut(b, which, epsk, epse, v, stream=None):
     _chk(lib.codens_met_cut(ctypes.byref(b), which, epsk, epse, _p(v), ld(v), _s(stream)))
 
 
+def met_smdes(b, tc, x, y, z, delta, sn_blk, vol, v, mu, dist, g9, fd, cfke, ts6, ts7, stream=None):
+    """met_smdes.f90:4; delta (nnn, written) gets met_dist's longest edges."""
+    _chk(lib.codens_met_smdes(ctypes.byref(b), _tc(tc), _p(x), _p(y), _p(z), _p(delta), _p(sn_blk), _p(vol), _p(v),
+                              ld(v), _p(mu), _p(dist), _p(g9), _p(fd), _p(cfke), _p(ts6), _p(ts7), _s(stream)))
+
+
 def _menter(mc):
     """The 10 Menter constants as a host array (betae, okappa, sigme1,
     sigma1, beta1, wsig1, sigme2, sigma2, beta2, wsig2)."""
@@ -242,6 +248,15 @@ def implimf_3d(b, ph, u, dt, v, d, ff, mu, mut, lmx, sn_blk, vol, ps, cson, dtpa
     _chk(lib.codens_implimf_3d(ctypes.byref(b), ctypes.byref(ph), _p(u), _p(dt), _p(v), ld(v), _p(d), _p(ff),
                                _p(mu), _p(mut), lmx, _p(sn_blk), _p(vol), dtpas, kfmg, ityprk, _p(ps), _p(cson),
                                _s(stream)))
+
+
+def implimf_3d_f32(b, ph, u, dt, v, d, ff, mu, mut, lmx, sn_blk, vol, ps, cson, dtpas=1.0, kfmg=0, ityprk=0,
+                   stream=None):
+    """FP32 defect-correction form of implimf_3d (same arguments and results;
+    the sub-iterations in FP32, d and v += d in FP64)."""
+    _chk(lib.codens_implimf_3d_f32(ctypes.byref(b), ctypes.byref(ph), _p(u), _p(dt), _p(v), ld(v), _p(d), _p(ff),
+                                   _p(mu), _p(mut), lmx, _p(sn_blk), _p(vol), dtpas, kfmg, ityprk, _p(ps), _p(cson),
+                                   _s(stream)))
 
 
 def implimf_eu(b, ph, u, dt, v, d, ff, lmx, sn_blk, vol, ps, cson, dt1min=1.0, kfmg=0, ityprk=0, stream=None):

@@ -276,12 +276,16 @@ struct Turb {
   double cb1, cb2, sigma, kappa, cw1, cw2, cw3, cv1, ct3, ct4, betas, beta, sigmk, sigmw, alfak, alfae;
 };
 
+// x**6 as the reference build evaluates an integer power: ((((x x) x) x) x) x
+__device__ inline double pow6(double x) { return ((((x * x) * x) * x) * x) * x; }
+
 // met_difsa.f90 + met_smsa.f90:91-222 (Spalart-Allmaras)
 __global__ void __launch_bounds__(TX *TY) k_met_smsa(Blk b, Turb tc, const double *__restrict__ sn,
                                                      const double *__restrict__ vol, const double *__restrict__ v,
                                                      long ldv, const double *__restrict__ mu,
                                                      const double *__restrict__ dist, const double *__restrict__ g9,
-                                                     const double *__restrict__ fd, double *__restrict__ cfke,
+                                                     const double *__restrict__ fd,
+                                                     const double *__restrict__ delta, double *__restrict__ cfke,
                                                      double *__restrict__ ts6, double *__restrict__ ts7) {
   REAL_CELL(b)
   long N = b.nnn;
@@ -291,7 +295,7 @@ __global__ void __launch_bounds__(TX *TY) k_met_smsa(Blk b, Turb tc, const doubl
   double sdif = gd[0] * fd[n] + gd[1] * fd[ldv + n] + gd[2] * fd[2 * ldv + n];
   sdif = sdif * tc.cb2 / tc.sigma;
   double kappa2 = tc.kappa * tc.kappa, uns6 = 1. / 6.;
-  double cv13 = tc.cv1 * tc.cv1 * tc.cv1, cw36 = pow(tc.cw3, 6.), cv133 = 3. * cv13, ct42 = 2. * tc.ct4;
+  double cv13 = tc.cv1 * tc.cv1 * tc.cv1, cw36 = pow6(tc.cw3), cv133 = 3. * cv13, ct42 = 2. * tc.ct4;
   double cb2sig = tc.cb2 / tc.sigma;
   double xy = g9[N + m], xz = g9[2 * N + m], yx = g9[3 * N + m], yz = g9[5 * N + m], zx = g9[6 * N + m];
   double zy = g9[7 * N + m];
@@ -300,18 +304,24 @@ __global__ void __launch_bounds__(TX *TY) k_met_smsa(Blk b, Turb tc, const doubl
   double v6 = v[5 * ldv + n], v1 = v[n];
   double xkhi = v6 / mu[n];
   double xkhi2 = xkhi * xkhi, xkhi3 = xkhi2 * xkhi;
-  double dist2 = dist[n] * dist[n];
+  double dd = dist[n];
+  if (delta) {  // met_smdes.f90:125: the DES length min(d, 0.65 delta)
+    double cd = 0.65 * delta[m];
+    dd = dd < cd ? dd : cd;
+  }
+  double dist2 = dd * dd;
   double nutilde = v6 / v1;
   double kapd2 = kappa2 * dist2;
   double fv1 = xkhi3 / (xkhi3 + cv13);
   double fv2 = 1. - (xkhi / (1 + xkhi * fv1));
+  if (delta) fv2 = fabs(fv2);  // met_smdes.f90:128
   double stilde = vort + nutilde * fv2 / kapd2;
   double rtilde = fmin(nutilde / (stilde * kapd2), 1.);
-  double rtil6 = pow(rtilde, 6.);
+  double rtil6 = pow6(rtilde);
   double ft2 = tc.ct3 * exp(-tc.ct4 * xkhi2);
   double sm1 = tc.cb1 * (1. - ft2) * stilde * v6;
   double xg = rtilde + tc.cw2 * (rtil6 - rtilde);
-  double xg6 = pow(xg, 6.);
+  double xg6 = pow6(xg);
   double fw = xg * pow((1. + cw36) / (xg6 + cw36), uns6);
   double sm2 = -(tc.cw1 * fw - tc.cb1 * ft2 / kappa2) * v6 * nutilde / dist2;
   ts6[n] = sm1 + sm2 + sdif;
@@ -498,8 +508,45 @@ extern "C" int codens_met_smsa(const codens_block *cb, const double *tc, const d
     return CODENS_EINVAL;
   }
   hipLaunchKernelGGL(k_met_smsa, cell_grid(b), dim3(TX, TY), 0, st, b, make_turb(tc), sn, vol, v, (long)ldv, mu, dist,
-                     g9, fd, cfke, ts6, ts7);
+                     g9, fd, (const double *)nullptr, cfke, ts6, ts7);
   return codens_check_launch("codens_met_smsa");
+}
+
+// met_dist.f90:66-103: the longest edge of each real cell from the node
+// coordinates (stored like the cells), block-local
+__global__ void __launch_bounds__(TX *TY) k_met_dist(Blk b, const double *__restrict__ x, const double *__restrict__ y,
+                                                     const double *__restrict__ z, double *__restrict__ delta) {
+  REAL_CELL(b)
+  const long ci = 1, cj = b.nid, ck = b.nijd;
+  auto edge = [&](long p, long q) {
+    double a = x[p] - x[q], c = y[p] - y[q], e = z[p] - z[q];
+    return sqrt(a * a + c * c + e * e);
+  };
+  const long P[12] = {n, n + cj, n + ck, n + cj + ck, n, n + ci, n + ck, n + ci + ck, n, n + ci, n + cj, n + ci + cj};
+  const long Q[12] = {n + ci, n + ci + cj, n + ci + ck, n + ci + cj + ck, n + cj, n + ci + cj, n + cj + ck,
+                      n + ci + cj + ck, n + ck, n + ci + ck, n + cj + ck, n + ci + cj + ck};
+  double mx = edge(P[0], Q[0]);
+#pragma unroll
+  for (int q = 1; q < 12; q++) {
+    double e = edge(P[q], Q[q]);
+    mx = e > mx ? e : mx;
+  }
+  delta[m] = mx;
+}
+
+extern "C" int codens_met_smdes(const codens_block *cb, const double *tc, const double *x, const double *y,
+                                const double *z, double *delta, const double *sn, const double *vol, const double *v,
+                                int64_t ldv, const double *mu, const double *dist, const double *g9, const double *fd,
+                                double *cfke, double *ts6, double *ts7, void *stream) {
+  CHECK_BLOCK("codens_met_smdes")
+  if (!tc || !x || !y || !z || !delta || !sn || !vol || !v || !mu || !dist || !g9 || !fd || !cfke || !ts6 || !ts7) {
+    codens_set_error("codens_met_smdes: null argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_met_dist, cell_grid(b), dim3(TX, TY), 0, st, b, x, y, z, delta);
+  hipLaunchKernelGGL(k_met_smsa, cell_grid(b), dim3(TX, TY), 0, st, b, make_turb(tc), sn, vol, v, (long)ldv, mu, dist,
+                     g9, fd, (const double *)delta, cfke, ts6, ts7);
+  return codens_check_launch("codens_met_smdes");
 }
 
 extern "C" int codens_met_smko(const codens_block *cb, const double *tc, const double *v, int64_t ldv,

@@ -142,6 +142,16 @@ module codens_c
        integer(c_int32_t), value :: lmx, kfmg, ityprk
        real(c_double), value :: dtpas
      end function codens_implimf_3d
+     integer(c_int32_t) function codens_implimf_3d_f32(b, ph, u, dt, v, ldv, d, ff, mu, mut, lmx, sn, vol, dtpas, &
+          kfmg, ityprk, ps, cson, stream) bind(C, name="codens_implimf_3d_f32")
+       import
+       type(codens_block), intent(in) :: b
+       type(codens_phys), intent(in) :: ph
+       type(c_ptr), value :: u, dt, v, d, ff, mu, mut, sn, vol, ps, cson, stream
+       integer(c_int64_t), value :: ldv
+       integer(c_int32_t), value :: lmx, kfmg, ityprk
+       real(c_double), value :: dtpas
+     end function codens_implimf_3d_f32
      integer(c_int32_t) function codens_implimf_eu(b, ph, u, dt, v, ldv, nvar, d, ff, lmx, sn, vol, dt1min, kfmg, &
           ityprk, ps, cson, stream) bind(C, name="codens_implimf_eu")
        import
@@ -263,6 +273,14 @@ module codens_c
        type(c_ptr), value :: sn, vol, v, mu, mut, dist, g9, stream
        integer(c_int64_t), value :: ldv
      end function codens_met_kocmut
+     integer(c_int32_t) function codens_met_smdes(b, tc, x, y, z, delta, sn, vol, v, ldv, mu, dist, g9, fd, cfke, &
+          ts6, ts7, stream) bind(C, name="codens_met_smdes")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: tc(16)
+       type(c_ptr), value :: x, y, z, delta, sn, vol, v, mu, dist, g9, fd, cfke, ts6, ts7, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_smdes
      integer(c_int32_t) function codens_met_mtcorf1(nfaces, face_start, lines, imxclko, mnpar, frac, nfrac, stream) &
           bind(C, name="codens_met_mtcorf1")
        import

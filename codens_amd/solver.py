@@ -25,7 +25,12 @@ def _dev(a, dtype=torch.float64):
 
 
 class Solver:
-    def __init__(self, case: Case, stream=None):
+    def __init__(self, case: Case, stream=None, implicit="fp64"):
+        """implicit: "fp64" (implimf_3d as the reference) or "fp32" (its FP32
+        defect-correction form: FP64 residual and update, FP32 sub-iterations)."""
+        if implicit not in ("fp64", "fp32"):
+            raise ValueError(f"implicit must be 'fp64' or 'fp32', not {implicit!r}")
+        self.implicit = implicit
         self.case = case
         lay = case.layout
         self.lay = lay
@@ -162,8 +167,9 @@ class Solver:
             if cs.kmf == 0:
                 abi.sch_expli(b, self.u, self.v, self.dt, self.vol, stream=s)
             else:
-                abi.implimf_3d(b, self.ph, self.u, self.dt, self.v, self.d, self.ff, self.mu, self.mut, cs.lmax,
-                               self.sn_blk[l], self.vol, self.ps, self.cson, stream=s)
+                fn = abi.implimf_3d if self.implicit == "fp64" else abi.implimf_3d_f32
+                fn(b, self.ph, self.u, self.dt, self.v, self.d, self.ff, self.mu, self.mut, cs.lmax, self.sn_blk[l],
+                   self.vol, self.ps, self.cson, stream=s)
 
     def stage(self):
         """One residual + update of all blocks (atsch_num.f90:5)."""
@@ -182,6 +188,26 @@ class Solver:
         """||(rho^{n+1}-rho^n)/dt||_2 of the last implicit stage: the increment
         d holds rho^{n+1}-rho^n on the real cells (dua_resro.f90:63-69)."""
         return float(np.sqrt(abi.sumsq_rate(self.blocks, self.d[0], self.zero, self.dt, stream=self.stream)))
+
+    def run_to(self, drop=1e-8, max_iter=10000, check_every=10):
+        """Implicit stages until the density residual falls to `drop` times its
+        value at the first stage (resite, dua_resro.f90:63-69; the reference's
+        tol = 1e-8, inimem.f90:132), or max_iter stages.  The norm is read
+        every check_every stages (a host sync).  Returns (stages, history of
+        (stage, drop))."""
+        self.stage()
+        r1 = self.density_residual()
+        hist = [(1, 1.0)]
+        it = 1
+        while it < max_iter:
+            for _ in range(min(check_every, max_iter - it)):
+                self.stage()
+                it += 1
+            r = self.density_residual() / r1
+            hist.append((it, r))
+            if not np.isfinite(r) or r <= drop:
+                break
+        return it, hist
 
     def host_state(self):
         torch.cuda.synchronize()

@@ -156,6 +156,15 @@ int codens_met_gradtr(const codens_block *b, const double *sn_blk, const double 
 int codens_met_smsa(const codens_block *b, const double *tc, const double *sn_blk, const double *vol,
                     const double *v, int64_t ldv, const double *mu, const double *dist, const double *g9,
                     const double *fd, double *cfke, double *ts6, double *ts7, void *stream);
+/* met_smdes.f90:4 -- the Spalart-Allmaras sources of codens_met_smsa with
+ * the DES length min(d, 0.65 delta) for the wall distance and |fv2|; delta
+ * (block-local, nnn, written) is met_dist.f90:66-103's longest cell edge
+ * from the node coordinates x, y, z (stored like the cells, ldv). */
+int codens_met_smdes(const codens_block *b, const double *tc, const double *x, const double *y, const double *z,
+                     double *delta, const double *sn, const double *vol, const double *v, int64_t ldv,
+                     const double *mu, const double *dist, const double *g9, const double *fd, double *cfke,
+                     double *ts6, double *ts7, void *stream);
+
 /* met_smko.f90:4 -- compressible Wilcox k-omega sources. */
 int codens_met_smko(const codens_block *b, const double *tc, const double *v, int64_t ldv, const double *tprod,
                     double *cfke, double *ts6, double *ts7, const double *cson, void *stream);

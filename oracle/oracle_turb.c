@@ -241,13 +241,16 @@ void orc_met_gradtr(const orc_block *b, const double *sn, const double *vol, con
   free(s); free(g);
 }
 
+/* x**6 as the reference build evaluates an integer power: ((((x x) x) x) x) x */
+static inline double pow6(double x) { return ((((x * x) * x) * x) * x) * x; }
+
 /* met_difsa.f90 + met_smsa.f90:91-222: Spalart-Allmaras source ts6 = qcxts5
  * (production, destruction and the cb2/sigma grad(rho nu~).grad(nu~) term),
  * ts7 = 0 and the implicit coefficient cfke.  fd5: grad(nu~) (3 arrays of
  * ldv from met_gradtr); g9: velocity gradients (block-local). */
-void orc_met_smsa(const orc_block *b, const orc_turb *tc, const double *sn, const double *vol, const double *v,
-                  long ldv, const double *mu, const double *dist, const double *g9, const double *fd5,
-                  double *cfke, double *ts6, double *ts7) {
+static void sa_sources(const orc_block *b, const orc_turb *tc, const double *sn, const double *vol, const double *v,
+                       long ldv, const double *mu, const double *dist, const double *g9, const double *fd5,
+                       const double *delta, double *cfke, double *ts6, double *ts7) {
   long n0 = b->npc, nnn = b->nnn;
   double *s = calloc(nnn, 8), *gd = calloc(3 * nnn, 8), *gr = calloc(3 * nnn, 8), *sdif = calloc(nnn, 8);
   /* met_difsa: gradient of rho nu~, dotted with grad(nu~), times cb2/sigma */
@@ -262,7 +265,7 @@ void orc_met_smsa(const orc_block *b, const orc_turb *tc, const double *sn, cons
   fill_box(b, s, v, NULL);
   orc_teq_grads(b, sn, vol, s, gr);
   double kappa2 = tc->kappa * tc->kappa, uns6 = 1. / 6.;
-  double cv13 = tc->cv1 * tc->cv1 * tc->cv1, cw36 = pow(tc->cw3, 6), cv133 = 3. * cv13, ct42 = 2. * tc->ct4;
+  double cv13 = tc->cv1 * tc->cv1 * tc->cv1, cw36 = pow6(tc->cw3), cv133 = 3. * cv13, ct42 = 2. * tc->ct4;
   double cb2sig = tc->cb2 / tc->sigma;
   for (int k = b->kk1; k <= b->kk2 - 1; k++)
     for (int j = b->jj1; j <= b->jj2 - 1; j++)
@@ -275,19 +278,25 @@ void orc_met_smsa(const orc_block *b, const orc_turb *tc, const double *sn, cons
         double v6 = v[5 * ldv + n], v1 = v[n];
         double xkhi = v6 / mu[n];
         double xkhi2 = xkhi * xkhi, xkhi3 = xkhi2 * xkhi;
-        double dist2 = dist[n] * dist[n];
+        double dd = dist[n];
+        if (delta) {  /* met_smdes.f90:125: min(d, ctdes delta), ctdes = 0.65 */
+          double cd = 0.65 * delta[m];
+          dd = dd < cd ? dd : cd;
+        }
+        double dist2 = dd * dd;
         double nutilde = v6 / v1;
         double kapd2 = kappa2 * dist2;
         double fv1 = xkhi3 / (xkhi3 + cv13);
         double fv2 = 1. - (xkhi / (1 + xkhi * fv1));
+        if (delta) fv2 = fabs(fv2); /* met_smdes.f90:128 */
         double stilde = vort + nutilde * fv2 / kapd2;
         double rtilde = nutilde / (stilde * kapd2);
         rtilde = fmin(rtilde, 1.);
-        double rtil6 = pow(rtilde, 6);
+        double rtil6 = pow6(rtilde);
         double ft2 = tc->ct3 * exp(-tc->ct4 * xkhi2);
         double sm1 = tc->cb1 * (1. - ft2) * stilde * v6;
         double xg = rtilde + tc->cw2 * (rtil6 - rtilde);
-        double xg6 = pow(xg, 6);
+        double xg6 = pow6(xg);
         double fw = xg * pow((1. + cw36) / (xg6 + cw36), uns6);
         double sm2 = -(tc->cw1 * fw - tc->cb1 * ft2 / kappa2) * v6 * nutilde / dist2;
         ts6[n] = sm1 + sm2 + sdif[m];
@@ -310,6 +319,43 @@ void orc_met_smsa(const orc_block *b, const orc_turb *tc, const double *sn, cons
         cfke[n] = fabs(dsm1 + dsm2 + dsdif);
       }
   free(s); free(gd); free(gr); free(sdif);
+}
+
+void orc_met_smsa(const orc_block *b, const orc_turb *tc, const double *sn, const double *vol, const double *v,
+                  long ldv, const double *mu, const double *dist, const double *g9, const double *fd5,
+                  double *cfke, double *ts6, double *ts7) {
+  sa_sources(b, tc, sn, vol, v, ldv, mu, dist, g9, fd5, NULL, cfke, ts6, ts7);
+}
+
+/* met_dist.f90:4 (3-D branch, :66-103): delta(m) = the longest of the twelve
+ * edges of real cell m, from the node coordinates x, y, z (stored like the
+ * cells). */
+void orc_met_dist(const orc_block *b, const double *x, const double *y, const double *z, double *delta) {
+  long ci = 1, cj = NID(b), ck = (long)NID(b) * NJD(b);
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - b->npc;
+#define EDGE(p, q) sqrt((x[p] - x[q]) * (x[p] - x[q]) + (y[p] - y[q]) * (y[p] - y[q]) + (z[p] - z[q]) * (z[p] - z[q]))
+        double e[12] = {EDGE(n, n + ci), EDGE(n + cj, n + ci + cj), EDGE(n + ck, n + ci + ck),
+                        EDGE(n + cj + ck, n + ci + cj + ck), EDGE(n, n + cj), EDGE(n + ci, n + ci + cj),
+                        EDGE(n + ck, n + cj + ck), EDGE(n + ci + ck, n + ci + cj + ck), EDGE(n, n + ck),
+                        EDGE(n + ci, n + ci + ck), EDGE(n + cj, n + cj + ck), EDGE(n + ci + cj, n + ci + cj + ck)};
+#undef EDGE
+        double mx = e[0];
+        for (int q = 1; q < 12; q++) mx = e[q] > mx ? e[q] : mx;
+        delta[m] = mx;
+      }
+}
+
+/* met_smdes.f90:4 -- the Spalart-Allmaras sources of met_smsa with the DES
+ * length min(d, 0.65 delta) for the wall distance (delta from met_dist,
+ * written to the caller's delta as the reference does) and |fv2|. */
+void orc_met_smdes(const orc_block *b, const orc_turb *tc, const double *x, const double *y, const double *z,
+                   double *delta, const double *sn, const double *vol, const double *v, long ldv, const double *mu,
+                   const double *dist, const double *g9, const double *fd5, double *cfke, double *ts6, double *ts7) {
+  orc_met_dist(b, x, y, z, delta);
+  sa_sources(b, tc, sn, vol, v, ldv, mu, dist, g9, fd5, delta, cfke, ts6, ts7);
 }
 
 /* met_smko.f90:80-110: compressible Wilcox k-omega sources. */

@@ -536,4 +536,18 @@ contains
          g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), g(1,8), g(1,9), tprod, cfke, frac, ts6, ts7)
   end subroutine ref_met_smmtr
 
+  ! met_smdes.f90:4 (calls met_dist.f90:4, met_difsa.f90:4)
+  subroutine ref_met_smdes(l, x, y, z, delta, sn, vol, v, mu, dist, g, fd, cfke, tt, gx, gy, gz, sdif, ts6, ts7, cmu) &
+       bind(C, name="ref_met_smdes")
+    use mod_met_smdes
+    use chainecarac
+    integer(c_int), value :: l
+    real(c_double) :: x(*), y(*), z(*), delta(*), sn(*), vol(*), v(*), mu(*), dist(*), g(ip00, 9), fd(ip11, 6)
+    real(c_double) :: cfke(*), tt(*), gx(*), gy(*), gz(*), sdif(*), ts6(*), ts7(*), cmu(*)
+    equat = heq
+    call met_smdes(l, x, y, z, delta, sn, vol, v, mu, dist, g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), &
+         g(1,7), g(1,8), g(1,9), fd(1,1), fd(1,2), fd(1,3), cfke, tt, gx, gy, gz, sdif, ts6, ts7, &
+         cmu, cmu, cmu, cmu, cmu, cmu)
+  end subroutine ref_met_smdes
+
 end module codens_ref_harness

@@ -9,6 +9,12 @@ GAM = 1.4
 
 
 def random_mesh(dims, seed, jitter=0.01):
+    lay, sn, vol, _ = random_mesh_xyz(dims, seed, jitter)
+    return lay, sn, vol
+
+
+def random_mesh_xyz(dims, seed, jitter=0.01):
+    """random_mesh and its node coordinates (x, y, z), stored like the cells."""
     lay = L.Layout(dims)
     rng = np.random.Generator(np.random.PCG64(seed))
     x = np.zeros(lay.ip11); y = np.zeros(lay.ip11); z = np.zeros(lay.ip11)
@@ -18,7 +24,7 @@ def random_mesh(dims, seed, jitter=0.01):
         b.view(y)[:] = J * 0.13 + jitter * rng.standard_normal(I.shape)
         b.view(z)[:] = K * 0.07 + jitter * rng.standard_normal(I.shape)
     sn, vol = L.metrics(lay, x, y, z)
-    return lay, sn, vol
+    return lay, sn, vol, (x, y, z)
 
 
 def random_state(n, rng, pinfl=0.0, amp=0.1):

@@ -347,3 +347,30 @@ def test_menter_routines(tsetup, kfracom):
     finally:
         for X in (R, G):
             X.lib.ref_set_walls(0, None)
+
+
+def test_smdes(mesh):
+    """met_smdes through the reference's signature (node coordinates of the
+    mesh, delta returned)."""
+    from randcase import random_mesh_xyz
+    lay, sn, vol, (x, y, z) = random_mesh_xyz([(8, 6, 5), (5, 4, 3)], seed=41)
+    rng = np.random.default_rng(409)
+    n = lay.ip11
+    v = np.zeros((7, n)); v[:5] = random_state(n, rng)
+    v[5] = 1e-3 * (1 + rng.random(n)) * v[0]
+    mu = 1e-3 * (1 + 0.1 * rng.random(n))
+    dist = 0.01 + 0.2 * rng.random(n)
+    R, G = pair(lay, C.orc_phys())
+    tc = C.turb_constants()
+    R.set_turb(tc); G.set_turb(tc)
+    fd = 1e-2 * rng.standard_normal((6, n))
+    cmu = np.ones(n)
+    for l, b in enumerate(lay.blocks):
+        inner = b.interior()
+        g = np.zeros((9, lay.ip00)); g[:, :b.nnn] = rng.standard_normal((9, b.nnn))
+        w5 = [np.zeros(lay.ip00) for _ in range(5)]
+        r_, g_ = _both(R, G, "ref_met_smdes", l + 1, x, y, z, np.zeros(lay.ip00), sn, vol, v, mu, dist, g, fd,
+                       np.zeros(n), *w5, np.zeros(n), np.zeros(n), cmu)
+        close(r_[3][inner - b.npc], g_[3][inner - b.npc])  # delta
+        for q in (11, 17, 18):  # cfke, ts6, ts7
+            close(r_[q][inner], g_[q][inner])

@@ -168,6 +168,40 @@ def test_implimf_3d(ctx, lmx, ityprk):
         assert relerr(hv[e], v_o[e]) <= TOL, f"v[{e}]"
 
 
+@pytest.mark.parametrize("lmx,ityprk", [(1, 0), (4, 0), (10, 1)])
+def test_implimf_3d_f32(ctx, lmx, ityprk):
+    """FP32 defect-correction form against the FP64 oracle: the increment d
+    within 2e-5 relative (FP32 sub-iterations: about 1e-7 per operation,
+    grown over lmx sweeps) and v += d in FP64, so v - v0 carries that same
+    relative error and nothing more."""
+    abi, lay, sn, vol = ctx
+    rng = np.random.default_rng(27 + lmx)
+    n = lay.ip11
+    v = random_state(n, rng)
+    ps, t, c = all_thermo(v)
+    mu = 1e-3 * (1 + 0.1 * rng.random(n)); mut = 1e-4 * rng.random(n)
+    u = 1e-3 * rng.standard_normal((5, n)); ff = 1e-4 * rng.standard_normal((5, n))
+    dt = 0.01 * (1 + rng.random(n))
+    O = C.Orc(lay, C.orc_phys())
+    ph = abi.phys()
+    v_o = v.copy(); d_o = np.zeros((5, n))
+    gv = T(v); gd = T(np.full((5, n), 7.0))
+    for l, b in enumerate(lay.blocks):
+        O.implimf_3d(l, u, dt, v_o, d_o, ff, mu, mut, lmx, sn, vol, ps, c, ityprk=ityprk)
+        abi.implimf_3d_f32(abi.block(b), ph, T(u), T(dt), gv, gd, T(ff), T(mu), T(mut), lmx,
+                           T(sn[9 * b.npc: 9 * (b.npc + b.nnn)]), T(vol), T(ps), T(c), ityprk=ityprk)
+    hv, hd = H(gv), H(gd)
+    inner = lay.interior()
+    for e in range(5):
+        assert relerr(hd[e], d_o[e]) <= 2e-5, f"d[{e}]"
+        assert relerr(hv[e] - v[e], v_o[e] - v[e]) <= 2e-5, f"v[{e}]"
+        assert relerr(hd[e], d_o[e]) > 0  # it is the FP32 path
+    # the block ranges outside the real cells are zeroed, as implimf_3d does
+    owned = np.concatenate([np.arange(b.npc, b.npc + b.nnn) for b in lay.blocks])
+    ghost = np.setdiff1d(owned, inner)
+    assert (hd[:, ghost] == 0).all()
+
+
 @pytest.mark.parametrize("lmx,ityprk,kfmg", [(1, 0, 0), (3, 0, 0), (10, 0, 3), (2, 1, 0)])
 def test_implimf_eu(ctx, lmx, ityprk, kfmg):
     """Euler implicit phase (implimf_eu.f90) against the oracle, 7-plane v and

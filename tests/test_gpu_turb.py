@@ -306,3 +306,37 @@ def test_menter(setup, kfracom):
         gfr = T(fr)
         abi.met_mtcorf1(starts, torch.as_tensor(lines.view(np.uint8), device="cuda"), imx, gmn, gfr)
         assert np.array_equal(H(gfr), fo)
+
+
+def test_smdes():
+    """met_smdes (SA with the DES length from met_dist) against the oracle,
+    on a mesh whose node coordinates the test holds."""
+    import ctypes
+    from oracle import ptr
+    from codens_amd import abi
+    from randcase import random_mesh_xyz
+    lay, sn, vol, (x, y, z) = random_mesh_xyz([(8, 6, 5), (5, 4, 3)], seed=41)
+    rng = np.random.default_rng(383)
+    n = lay.ip11
+    v = np.zeros((7, n)); v[:5] = random_state(n, rng)
+    v[5] = 1e-3 * (1 + rng.random(n)) * v[0]
+    mu = 1e-3 * (1 + 0.1 * rng.random(n))
+    dist = 0.01 + 0.2 * rng.random(n)
+    tc = C.turb_constants()
+    tcl = [getattr(tc, f) for f, _ in C.OrcTurb._fields_]
+    O = C.Orc(lay, C.orc_phys())
+    fd = 1e-2 * rng.standard_normal((6, n))
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        inner = b.interior()
+        g9 = rng.standard_normal((9, b.nnn))
+        do = np.zeros(b.nnn); oo = [np.zeros(n) for _ in range(3)]
+        sb = np.ascontiguousarray(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+        O.lib.orc_met_smdes(ctypes.byref(ob), ctypes.byref(tc), ptr(x), ptr(y), ptr(z), ptr(do), ptr(sb), ptr(vol),
+                            ptr(v), C.L_(n), ptr(mu), ptr(dist), ptr(g9), ptr(fd), *[ptr(a) for a in oo])
+        gd = T(np.zeros(b.nnn)); go = [T(np.zeros(n)) for _ in range(3)]
+        abi.met_smdes(abi.block(b), tcl, T(x), T(y), T(z), gd, snb(sn, b), T(vol), T(v), T(mu), T(dist), T(g9), T(fd),
+                      *go)
+        assert relerr(H(gd)[inner - b.npc], do[inner - b.npc]) <= TOL
+        for a, o in zip(go, oo):
+            assert relerr(H(a)[inner], o[inner]) <= TOL

@@ -224,3 +224,41 @@ def test_sources_kor_chien(setup):
         for a, o in zip(out_r, out_o):
             assert np.array_equal(a, o)
             assert np.isfinite(a[b.interior()]).all()
+
+
+def test_smdes():
+    """met_smdes (SA with the DES length from met_dist's longest cell edge)
+    on a mesh whose node coordinates the test holds: sources, cfke and the
+    delta the reference returns; the DES length is active in some cells and
+    not in others."""
+    from randcase import random_mesh_xyz
+    lay, sn, vol, (x, y, z) = random_mesh_xyz(DIMS, seed=41)
+    rng = np.random.default_rng(83)
+    n = lay.ip11
+    v = np.zeros((7, n)); v[:5] = random_state(n, rng)
+    v[5] = 1e-3 * (1 + rng.random(n)) * v[0]
+    mu = 1e-3 * (1 + 0.1 * rng.random(n))
+    dist = 0.01 + 0.2 * rng.random(n)
+    tc = C.turb_constants()
+    R = C.Ref(lay, C.orc_phys()); R.set_turb(tc)
+    O = oracle.oracle_lib()
+    fd = 1e-2 * rng.standard_normal((6, n))
+    cmu = np.ones(n)
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        g = np.zeros((9, lay.ip00)); g[:, :b.nnn] = rng.standard_normal((9, b.nnn))
+        gb = np.ascontiguousarray(g[:, :b.nnn])
+        w = [np.zeros(lay.ip00) for _ in range(5)]
+        dr = np.full(lay.ip00, 7.0); r = [np.full(n, 7.0) for _ in range(3)]
+        R.lib.ref_met_smdes(l + 1, ptr(x), ptr(y), ptr(z), ptr(dr), ptr(sn), ptr(vol), ptr(v), ptr(mu), ptr(dist),
+                            ptr(g), ptr(fd), ptr(r[0]), *[ptr(a) for a in w], ptr(r[1]), ptr(r[2]), ptr(cmu))
+        do = np.full(lay.ip00, 7.0); o = [np.full(n, 7.0) for _ in range(3)]
+        O.orc_met_smdes(ctypes.byref(ob), ctypes.byref(tc), ptr(x), ptr(y), ptr(z), ptr(do), ptr(snb(sn, b)),
+                        ptr(vol), ptr(v), C.L_(n), ptr(mu), ptr(dist), ptr(gb), ptr(fd), *[ptr(a) for a in o])
+        inner = b.interior()
+        m = inner - b.npc
+        assert np.array_equal(dr[m], do[m])
+        for a, c in zip(r, o):
+            assert np.array_equal(a[inner], c[inner])
+        des = 0.65 * do[m] < dist[inner]
+        assert des.any() and not des.all()

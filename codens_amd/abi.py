@@ -184,6 +184,12 @@ def met_smdes(b, tc, x, y, z, delta, sn_blk, vol, v, mu, dist, g9, fd, cfke, ts6
                               ld(v), _p(mu), _p(dist), _p(g9), _p(fd), _p(cfke), _p(ts6), _p(ts7), _s(stream)))
 
 
+def met_smsasas(b, tc, ischema, sn_blk, vol, v, mu, dist, g9, fd, cfke, ts6, ts7, stream=None):
+    """met_smsasas.f90:4 (ischema = 1: met_laplaciens's plain form)."""
+    _chk(lib.codens_met_smsasas(ctypes.byref(b), _tc(tc), ischema, _p(sn_blk), _p(vol), _p(v), ld(v), _p(mu),
+                                _p(dist), _p(g9), _p(fd), _p(cfke), _p(ts6), _p(ts7), _s(stream)))
+
+
 def _menter(mc):
     """The 10 Menter constants as a host array (betae, okappa, sigme1,
     sigma1, beta1, wsig1, sigme2, sigma2, beta2, wsig2)."""

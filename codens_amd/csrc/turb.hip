@@ -285,8 +285,10 @@ __global__ void __launch_bounds__(TX *TY) k_met_smsa(Blk b, Turb tc, const doubl
                                                      long ldv, const double *__restrict__ mu,
                                                      const double *__restrict__ dist, const double *__restrict__ g9,
                                                      const double *__restrict__ fd,
-                                                     const double *__restrict__ delta, double *__restrict__ cfke,
-                                                     double *__restrict__ ts6, double *__restrict__ ts7) {
+                                                     const double *__restrict__ delta,
+                                                     const double *__restrict__ tn1, const double *__restrict__ tn2,
+                                                     double *__restrict__ cfke, double *__restrict__ ts6,
+                                                     double *__restrict__ ts7) {
   REAL_CELL(b)
   long N = b.nnn;
   double gd[3], gr[3];
@@ -314,9 +316,11 @@ __global__ void __launch_bounds__(TX *TY) k_met_smsa(Blk b, Turb tc, const doubl
   double kapd2 = kappa2 * dist2;
   double fv1 = xkhi3 / (xkhi3 + cv13);
   double fv2 = 1. - (xkhi / (1 + xkhi * fv1));
-  if (delta) fv2 = fabs(fv2);  // met_smdes.f90:128
+  const bool sas = tn1 != nullptr;
+  if (delta || sas) fv2 = fabs(fv2);  // met_smdes.f90:128, met_smsasas.f90:214
   double stilde = vort + nutilde * fv2 / kapd2;
-  double rtilde = fmin(nutilde / (stilde * kapd2), 1.);
+  double rtilde = nutilde / (stilde * kapd2);
+  if (!sas) rtilde = fmin(rtilde, 1.);  // met_smsasas has no clamp
   double rtil6 = pow6(rtilde);
   double ft2 = tc.ct3 * exp(-tc.ct4 * xkhi2);
   double sm1 = tc.cb1 * (1. - ft2) * stilde * v6;
@@ -324,6 +328,19 @@ __global__ void __launch_bounds__(TX *TY) k_met_smsa(Blk b, Turb tc, const doubl
   double xg6 = pow6(xg);
   double fw = xg * pow((1. + cw36) / (xg6 + cw36), uns6);
   double sm2 = -(tc.cw1 * fw - tc.cb1 * ft2 / kappa2) * v6 * nutilde / dist2;
+  if (sas) {  // met_smsasas.f90:218-237: the SAS destruction where fd > 0.8
+    double r8 = 8. * rtilde;
+    double fdd = 1. - tanh((r8 * r8) * r8);
+    if (fdd > 0.8) {
+      double xx = g9[m], yy = g9[4 * N + m], zz = g9[8 * N + m];
+      double s1 = zy + yz, s2 = xz + zx, s3 = yx + xy;
+      double ss = sqrt(4. / 3. * (xx * xx + yy * yy + zz * zz) + s1 * s1 + s2 * s2 + s3 * s3);
+      double t1 = tn1[m], t2 = tn2[m];
+      double xlvk = tc.kappa * ss / sqrt(t1 * t1 + t2 * t2);
+      double xlvk2 = xlvk * xlvk;
+      sm2 = -tc.cw1 * fw * v6 * 1. * nutilde / xlvk2 + tc.cb1 * ft2 * v6 * nutilde / (kappa2 * dist2);
+    }
+  }
   ts6[n] = sm1 + sm2 + sdif;
   ts7[n] = 0.;
   double xkhi4 = xkhi3 * xkhi;
@@ -508,7 +525,8 @@ extern "C" int codens_met_smsa(const codens_block *cb, const double *tc, const d
     return CODENS_EINVAL;
   }
   hipLaunchKernelGGL(k_met_smsa, cell_grid(b), dim3(TX, TY), 0, st, b, make_turb(tc), sn, vol, v, (long)ldv, mu, dist,
-                     g9, fd, (const double *)nullptr, cfke, ts6, ts7);
+                     g9, fd, (const double *)nullptr, (const double *)nullptr, (const double *)nullptr, cfke, ts6,
+                     ts7);
   return codens_check_launch("codens_met_smsa");
 }
 
@@ -545,8 +563,83 @@ extern "C" int codens_met_smdes(const codens_block *cb, const double *tc, const 
   }
   hipLaunchKernelGGL(k_met_dist, cell_grid(b), dim3(TX, TY), 0, st, b, x, y, z, delta);
   hipLaunchKernelGGL(k_met_smsa, cell_grid(b), dim3(TX, TY), 0, st, b, make_turb(tc), sn, vol, v, (long)ldv, mu, dist,
-                     g9, fd, (const double *)delta, cfke, ts6, ts7);
+                     g9, fd, (const double *)delta, (const double *)nullptr, (const double *)nullptr, cfke, ts6,
+                     ts7);
   return codens_check_launch("codens_met_smdes");
+}
+
+// met_laplaciens.f90:4 on the real cells: the face sums of (dsx, dsy, dsz)
+// gathered per cell in the reference's accumulation order (directions k,
+// j, i; in each the low face then the high face, except on the low boundary
+// layer of a line longer than one cell, where the interior loop's high face
+// comes before the boundary loop's low face; interior faces carry both
+// neighbours' values, boundary faces twice the ghost's), scaled by 0.5/vol
+// and summed (:314-325).  ischema = 1 is the plain form, whose j faces read
+// dsy(m-ncj) for the x sum of the cell above (:112); other schemes the
+// weighted form with unit weights.
+__global__ void __launch_bounds__(TX *TY) k_laplacian(Blk b, int ischema, const double *__restrict__ sn,
+                                                      const double *__restrict__ vol, const double *__restrict__ dsx,
+                                                      const double *__restrict__ dsy, const double *__restrict__ dsz,
+                                                      double *__restrict__ out) {
+  REAL_CELL(b)
+  const double *ds[3] = {dsx, dsy, dsz};
+  double acc[3] = {0., 0., 0.};
+#pragma unroll
+  for (int dir = 2; dir >= 0; dir--) {
+    long inc = b.inc(dir);
+    int a = c[dir], a1 = b.lo(dir), a2 = b.hi(dir);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const double *d = ds[q];
+      double sl = snv(sn, b.nnn, m, dir, q), sh = snv(sn, b.nnn, m + inc, dir, q);
+      double lo, hi;
+      if (a > a1) {
+        double below = (ischema == 1 && dir == 1 && q == 0) ? dsy[m - inc] : d[m - inc];
+        lo = ischema == 1 ? sl * (d[m] + below) : (d[m] + d[m - inc]) * sl;
+      } else {
+        lo = sl * 2 * d[m - inc];
+      }
+      if (a + 1 < a2) hi = ischema == 1 ? sh * (d[m + inc] + d[m]) : (d[m + inc] + d[m]) * sh;
+      else hi = sh * 2 * d[m + inc];
+      if (a == a1 && a + 1 < a2) {
+        acc[q] = acc[q] + hi;
+        acc[q] = acc[q] - lo;
+      } else {
+        acc[q] = acc[q] - lo;
+        acc[q] = acc[q] + hi;
+      }
+    }
+  }
+  double vl = vol[n];
+  double ts = (-vl >= 0. && !signbit(-vl)) ? 0.5 : -0.5;  // sign(0.5, -vol)
+  double vols = (0.5 + ts) * 0.000001 + (0.5 - ts) * vl;
+  double c0 = 0.5 / vols;
+  out[m] = acc[0] * c0 + acc[1] * c0 + acc[2] * c0;
+}
+
+extern "C" int codens_met_smsasas(const codens_block *cb, const double *tc, int32_t ischema, const double *sn,
+                                  const double *vol, const double *v, int64_t ldv, const double *mu,
+                                  const double *dist, const double *g9, const double *fd, double *cfke, double *ts6,
+                                  double *ts7, void *stream) {
+  CHECK_BLOCK("codens_met_smsasas")
+  if (!tc || !sn || !vol || !v || !mu || !dist || !g9 || !fd || !cfke || !ts6 || !ts7) {
+    codens_set_error("codens_met_smsasas: null argument");
+    return CODENS_EINVAL;
+  }
+  // the Laplacians of u and v (met_smsasas.f90:173-193)
+  double *tn = (double *)codens_workspace(sizeof(double) * 2 * b.nnn, 7);
+  if (!tn) {
+    codens_set_error("codens_met_smsasas: workspace allocation failed");
+    return CODENS_EHIP;
+  }
+  const long N = b.nnn;
+  hipLaunchKernelGGL(k_laplacian, cell_grid(b), dim3(TX, TY), 0, st, b, (int)ischema, sn, vol, g9, g9 + N, g9 + 2 * N,
+                     tn);
+  hipLaunchKernelGGL(k_laplacian, cell_grid(b), dim3(TX, TY), 0, st, b, (int)ischema, sn, vol, g9 + 3 * N, g9 + 4 * N,
+                     g9 + 5 * N, tn + N);
+  hipLaunchKernelGGL(k_met_smsa, cell_grid(b), dim3(TX, TY), 0, st, b, make_turb(tc), sn, vol, v, (long)ldv, mu, dist,
+                     g9, fd, (const double *)nullptr, (const double *)tn, (const double *)(tn + N), cfke, ts6, ts7);
+  return codens_check_launch("codens_met_smsasas");
 }
 
 extern "C" int codens_met_smko(const codens_block *cb, const double *tc, const double *v, int64_t ldv,

@@ -281,6 +281,15 @@ module codens_c
        type(c_ptr), value :: x, y, z, delta, sn, vol, v, mu, dist, g9, fd, cfke, ts6, ts7, stream
        integer(c_int64_t), value :: ldv
      end function codens_met_smdes
+     integer(c_int32_t) function codens_met_smsasas(b, tc, ischema, sn, vol, v, ldv, mu, dist, g9, fd, cfke, ts6, &
+          ts7, stream) bind(C, name="codens_met_smsasas")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: tc(16)
+       integer(c_int32_t), value :: ischema
+       type(c_ptr), value :: sn, vol, v, mu, dist, g9, fd, cfke, ts6, ts7, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_smsasas
      integer(c_int32_t) function codens_met_mtcorf1(nfaces, face_start, lines, imxclko, mnpar, frac, nfrac, stream) &
           bind(C, name="codens_met_mtcorf1")
        import

@@ -733,3 +733,49 @@ contains
          1_c_int32_t, c_null_ptr), 'met_smdes')
   end subroutine met_smdes
 end module mod_met_smdes
+
+module mod_met_smsasas
+  implicit none
+contains
+  ! met_smsasas.f90:4 -- Spalart-Allmaras with the SAS destruction (the
+  ! scratch arrays t, dtdx..sdif, tn1..tn10 are not written)
+  subroutine met_smsasas(l, sn, vol, v, mu, dist, dvxx, dvxy, dvxz, dvyx, dvyy, dvyz, dvzx, dvzy, dvzz, &
+       txxf5x, txyf5y, txzf5z, cfke, t, dtdx, dtdy, dtdz, sdif, qcxts5, qcyts6, tn1, tn2, tn3, tn10, &
+       cmui1, cmui2, cmuj1, cmuj2, cmuk1, cmuk2)
+    use para_var
+    use para_fige
+    use maillage
+    use schemanum, only: ischema
+    use codens_c
+    implicit none
+    integer :: l
+    double precision, target :: sn(ip31*ndir), vol(ip11), v(ip11,ip60), mu(ip12), dist(ip12)
+    double precision, target :: dvxx(ip00), dvxy(ip00), dvxz(ip00), dvyx(ip00), dvyy(ip00), dvyz(ip00)
+    double precision, target :: dvzx(ip00), dvzy(ip00), dvzz(ip00)
+    double precision, target :: txxf5x(ip12), txyf5y(ip12), txzf5z(ip12), cfke(ip13), qcxts5(ip12), qcyts6(ip12)
+    double precision :: t(ip00), dtdx(ip00), dtdy(ip00), dtdz(ip00), sdif(ip00), tn1(ip00), tn2(ip00), tn3(ip00)
+    double precision :: tn10(ip00)
+    double precision :: cmui1(ip21), cmui2(ip21), cmuj1(ip21), cmuj2(ip21), cmuk1(ip21), cmuk2(ip21)
+    type(codens_block) :: b
+    type(c_ptr) :: dsn, dvol, dv, dmu, ddist, dg, dfd, dcf, d5, d6, g(9), f(3)
+    b = codens_block_of(l)
+    g = [c_loc(dvxx), c_loc(dvxy), c_loc(dvxz), c_loc(dvyx), c_loc(dvyy), c_loc(dvyz), c_loc(dvzx), c_loc(dvzy), &
+         c_loc(dvzz)]
+    f = [c_loc(txxf5x), c_loc(txyf5y), c_loc(txzf5z)]
+    dsn = codens_dev_all(codens_sn_blk(sn, l), 9*nnn(l), codens_geom_mode())
+    dvol = codens_dev(c_loc(vol), ip11, ip11, 1, 1, b, codens_geom_mode())
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 7, b, 1)
+    dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+    ddist = codens_dev(c_loc(dist), ip12, ip12, 1, 1, b, codens_geom_mode())
+    dg = codens_bundle_in(g, 9, nnn(l), 0, nnn(l))
+    dfd = codens_bundle_in(f, 3, ip12, 0, npc(l) + nnn(l))
+    dcf = codens_dev(c_loc(cfke), ip13, ip13, 1, 1, b, 1)
+    d5 = codens_dev(c_loc(qcxts5), ip12, ip12, 1, 1, b, 1)
+    d6 = codens_dev(c_loc(qcyts6), ip12, ip12, 1, 1, b, 1)
+    call codens_check(codens_met_smsasas(b, codens_tc_now(), int(ischema, c_int32_t), dsn, dvol, dv, &
+         int(ip11, c_int64_t), dmu, ddist, dg, dfd, dcf, d5, d6, c_null_ptr), 'met_smsasas')
+    call codens_back(c_loc(cfke), ip13, 1, 1, b)
+    call codens_back(c_loc(qcxts5), ip12, 1, 1, b)
+    call codens_back(c_loc(qcyts6), ip12, 1, 1, b)
+  end subroutine met_smsasas
+end module mod_met_smsasas

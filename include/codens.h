@@ -165,6 +165,16 @@ int codens_met_smdes(const codens_block *b, const double *tc, const double *x, c
                      const double *mu, const double *dist, const double *g9, const double *fd, double *cfke,
                      double *ts6, double *ts7, void *stream);
 
+/* met_smsasas.f90:4 -- Spalart-Allmaras with the SAS destruction: the
+ * sources of codens_met_smsa without the rtilde clamp and with |fv2|, and
+ * where 1 - tanh((8 rtilde)^3) > 0.8 the destruction from the von Karman
+ * length kappa S / sqrt(tn1^2 + tn2^2), tn1, tn2 the Laplacians of u and
+ * v that met_laplaciens.f90:4 forms from grad u and grad v (ischema = 1:
+ * its plain form, else the weighted form; the weights are 1). */
+int codens_met_smsasas(const codens_block *b, const double *tc, int32_t ischema, const double *sn, const double *vol,
+                       const double *v, int64_t ldv, const double *mu, const double *dist, const double *g9,
+                       const double *fd, double *cfke, double *ts6, double *ts7, void *stream);
+
 /* met_smko.f90:4 -- compressible Wilcox k-omega sources. */
 int codens_met_smko(const codens_block *b, const double *tc, const double *v, int64_t ldv, const double *tprod,
                     double *cfke, double *ts6, double *ts7, const double *cson, void *stream);

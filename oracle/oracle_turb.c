@@ -248,9 +248,10 @@ static inline double pow6(double x) { return ((((x * x) * x) * x) * x) * x; }
  * (production, destruction and the cb2/sigma grad(rho nu~).grad(nu~) term),
  * ts7 = 0 and the implicit coefficient cfke.  fd5: grad(nu~) (3 arrays of
  * ldv from met_gradtr); g9: velocity gradients (block-local). */
+/* sas: NULL, or the two Laplacian components of met_smsasas (block-local) */
 static void sa_sources(const orc_block *b, const orc_turb *tc, const double *sn, const double *vol, const double *v,
                        long ldv, const double *mu, const double *dist, const double *g9, const double *fd5,
-                       const double *delta, double *cfke, double *ts6, double *ts7) {
+                       const double *delta, const double *const *sas, double *cfke, double *ts6, double *ts7) {
   long n0 = b->npc, nnn = b->nnn;
   double *s = calloc(nnn, 8), *gd = calloc(3 * nnn, 8), *gr = calloc(3 * nnn, 8), *sdif = calloc(nnn, 8);
   /* met_difsa: gradient of rho nu~, dotted with grad(nu~), times cb2/sigma */
@@ -288,10 +289,10 @@ static void sa_sources(const orc_block *b, const orc_turb *tc, const double *sn,
         double kapd2 = kappa2 * dist2;
         double fv1 = xkhi3 / (xkhi3 + cv13);
         double fv2 = 1. - (xkhi / (1 + xkhi * fv1));
-        if (delta) fv2 = fabs(fv2); /* met_smdes.f90:128 */
+        if (delta || sas) fv2 = fabs(fv2); /* met_smdes.f90:128, met_smsasas.f90:214 */
         double stilde = vort + nutilde * fv2 / kapd2;
         double rtilde = nutilde / (stilde * kapd2);
-        rtilde = fmin(rtilde, 1.);
+        if (!sas) rtilde = fmin(rtilde, 1.); /* met_smsasas has no clamp */
         double rtil6 = pow6(rtilde);
         double ft2 = tc->ct3 * exp(-tc->ct4 * xkhi2);
         double sm1 = tc->cb1 * (1. - ft2) * stilde * v6;
@@ -299,6 +300,20 @@ static void sa_sources(const orc_block *b, const orc_turb *tc, const double *sn,
         double xg6 = pow6(xg);
         double fw = xg * pow((1. + cw36) / (xg6 + cw36), uns6);
         double sm2 = -(tc->cw1 * fw - tc->cb1 * ft2 / kappa2) * v6 * nutilde / dist2;
+        if (sas) { /* met_smsasas.f90:218-237: the SAS destruction where fd = 1 - tanh((8 rtilde)^3) > 0.8 */
+          double r8 = 8. * rtilde;
+          double fdd = 1. - tanh((r8 * r8) * r8);
+          if (fdd > 0.8) {
+            double xx = g9[m], yy = g9[4 * nnn + m], zz = g9[8 * nnn + m];
+            double s1 = zy + yz, s2 = xz + zx, s3 = yx + xy;
+            double ss = sqrt(4. / 3. * (xx * xx + yy * yy + zz * zz) + s1 * s1 + s2 * s2 + s3 * s3);
+            double t1 = sas[0][m], t2 = sas[1][m];
+            double nlaplu = sqrt(t1 * t1 + t2 * t2);
+            double xlvk = tc->kappa * ss / nlaplu;
+            double xlvk2 = xlvk * xlvk;
+            sm2 = -tc->cw1 * fw * v6 * 1. * nutilde / xlvk2 + tc->cb1 * ft2 * v6 * nutilde / (kappa2 * dist2);
+          }
+        }
         ts6[n] = sm1 + sm2 + sdif[m];
         ts7[n] = 0.;
         double xkhi4 = xkhi3 * xkhi;
@@ -324,7 +339,7 @@ static void sa_sources(const orc_block *b, const orc_turb *tc, const double *sn,
 void orc_met_smsa(const orc_block *b, const orc_turb *tc, const double *sn, const double *vol, const double *v,
                   long ldv, const double *mu, const double *dist, const double *g9, const double *fd5,
                   double *cfke, double *ts6, double *ts7) {
-  sa_sources(b, tc, sn, vol, v, ldv, mu, dist, g9, fd5, NULL, cfke, ts6, ts7);
+  sa_sources(b, tc, sn, vol, v, ldv, mu, dist, g9, fd5, NULL, NULL, cfke, ts6, ts7);
 }
 
 /* met_dist.f90:4 (3-D branch, :66-103): delta(m) = the longest of the twelve
@@ -355,7 +370,89 @@ void orc_met_smdes(const orc_block *b, const orc_turb *tc, const double *x, cons
                    double *delta, const double *sn, const double *vol, const double *v, long ldv, const double *mu,
                    const double *dist, const double *g9, const double *fd5, double *cfke, double *ts6, double *ts7) {
   orc_met_dist(b, x, y, z, delta);
-  sa_sources(b, tc, sn, vol, v, ldv, mu, dist, g9, fd5, delta, cfke, ts6, ts7);
+  sa_sources(b, tc, sn, vol, v, ldv, mu, dist, g9, fd5, delta, NULL, cfke, ts6, ts7);
+}
+
+/* met_laplaciens.f90:4 -- face sums of (dsx, dsy, dsz) into (dsd2x, dsd2y,
+ * dsd2z): each zeroed on indc(i1-1,j1-1,k1-1)..indc(i2+1,j2+1,k2+1), then
+ * for the k, j, i faces in that order the interior faces (both neighbours'
+ * values) and the boundary faces (twice the ghost's value), scattered as the
+ * reference loops do.  ischema = 1 is the plain form, whose j faces read
+ * dsy(m-ncj) for dsd2x(m) (:112); other schemes the weighted form with
+ * weights 1 (block-local arrays, sn the block's slice). */
+void orc_met_laplaciens(const orc_block *b, int ischema, const double *sn, const double *vol, const double *dsx,
+                        const double *dsy, const double *dsz, double *dx, double *dy, double *dz) {
+  long n0 = b->npc, nid = NID(b), nijd = (long)NID(b) * NJD(b);
+  const double *ds[3] = {dsx, dsy, dsz};
+  double *dd[3] = {dx, dy, dz};
+  long lo = IDX(b, b->ii1 - 1, b->jj1 - 1, b->kk1 - 1) - n0, hi = IDX(b, b->ii2 + 1, b->jj2 + 1, b->kk2 + 1) - n0;
+  for (long m = lo; m <= hi; m++) dx[m] = dy[m] = dz[m] = 0.;
+  int i1 = b->ii1, i2 = b->ii2, j1 = b->jj1, j2 = b->jj2, k1 = b->kk1, k2 = b->kk2;
+  for (int dir = 2; dir >= 0; dir--) {
+    long inc = dir == 0 ? 1 : (dir == 1 ? nid : nijd);
+    int a1 = dir == 0 ? i1 : (dir == 1 ? j1 : k1), a2 = dir == 0 ? i2 : (dir == 1 ? j2 : k2);
+    /* interior faces a = a1+1 .. a2-1 */
+    for (int k = k1; k <= k2 - 1; k++)
+      for (int j = j1; j <= j2 - 1; j++)
+        for (int i = i1; i <= i2 - 1; i++) {
+          int a = dir == 0 ? i : (dir == 1 ? j : k);
+          if (a < a1 + 1) continue;
+          long m = IDX(b, i, j, k) - n0;
+          for (int c = 0; c < 3; c++) {
+            double s = SN(sn, b, m, dir, c);
+            double own = (ischema == 1 && dir == 1 && c == 0) ? ds[0][m] + ds[1][m - inc] : ds[c][m] + ds[c][m - inc];
+            double t = ischema == 1 ? s * own : (ds[c][m] + ds[c][m - inc]) * s;
+            double t2 = ischema == 1 ? s * (ds[c][m] + ds[c][m - inc]) : t;
+            dd[c][m] = dd[c][m] - t;
+            dd[c][m - inc] = dd[c][m - inc] + t2;
+          }
+        }
+    /* boundary faces a1 (ghost below) then a2 (ghost above) */
+    for (int side = 0; side < 2; side++)
+      for (int k = k1; k <= k2 - 1; k++)
+        for (int j = j1; j <= j2 - 1; j++)
+          for (int i = i1; i <= i2 - 1; i++) {
+            int a = dir == 0 ? i : (dir == 1 ? j : k);
+            if (a != a1) continue;
+            long m0 = IDX(b, i, j, k) - n0, m = side == 0 ? m0 : m0 + (long)(a2 - a1) * inc;
+            for (int c = 0; c < 3; c++) {
+              double s = SN(sn, b, m, dir, c);
+              if (side == 0) dd[c][m] = dd[c][m] - s * 2 * ds[c][m - inc];
+              else dd[c][m - inc] = dd[c][m - inc] + s * 2 * ds[c][m];
+            }
+          }
+  }
+  /* 0.5/vol (vol <= 0 guarded by 1e-6) and dsd2x = the sum of the three
+   * (:314-325), over indc(i1,j1,k1)..indc(i2-1,j2-1,k2-1) */
+  for (long n = IDX(b, i1, j1, k1); n <= IDX(b, i2 - 1, j2 - 1, k2 - 1); n++) {
+    long m = n - n0;
+    double ts = -vol[n] >= 0. && !signbit(-vol[n]) ? 0.5 : -0.5;
+    double vols = (0.5 + ts) * 0.000001 + (0.5 - ts) * vol[n];
+    double c0 = 0.5 / vols;
+    dx[m] = dx[m] * c0;
+    dy[m] = dy[m] * c0;
+    dz[m] = dz[m] * c0;
+    dx[m] = dx[m] + dy[m] + dz[m];
+  }
+}
+
+/* met_smsasas.f90:4 -- Spalart-Allmaras with the SAS destruction: the
+ * sources of met_smsa without the rtilde clamp and with |fv2|, and where
+ * fd = 1 - tanh((8 rtilde)^3) > 0.8 the destruction from the von Karman
+ * length kappa S / sqrt(tn1^2 + tn2^2), tn1 and tn2 the Laplacians of u
+ * and v from met_laplaciens of (du/dx, du/dy, du/dz) and (dv/dx, dv/dy,
+ * dv/dz) (:173-193; the one of w is formed but not used).  g9 holds the velocity gradients (block-local). */
+void orc_met_smsasas(const orc_block *b, const orc_turb *tc, int ischema, const double *sn, const double *vol,
+                     const double *v, long ldv, const double *mu, const double *dist, const double *g9,
+                     const double *fd5, double *cfke, double *ts6, double *ts7) {
+  long nnn = b->nnn;
+  double *w = calloc(6 * nnn, sizeof(double));
+  orc_met_laplaciens(b, ischema, sn, vol, g9, g9 + nnn, g9 + 2 * nnn, w, w + 2 * nnn, w + 3 * nnn);
+  orc_met_laplaciens(b, ischema, sn, vol, g9 + 3 * nnn, g9 + 4 * nnn, g9 + 5 * nnn, w + nnn, w + 4 * nnn,
+                     w + 5 * nnn);
+  const double *sas[2] = {w, w + nnn};
+  sa_sources(b, tc, sn, vol, v, ldv, mu, dist, g9, fd5, NULL, sas, cfke, ts6, ts7);
+  free(w);
 }
 
 /* met_smko.f90:80-110: compressible Wilcox k-omega sources. */

@@ -550,4 +550,23 @@ contains
          cmu, cmu, cmu, cmu, cmu, cmu)
   end subroutine ref_met_smdes
 
+  ! met_smsasas.f90:4 (calls met_difsa.f90:4, met_laplaciens.f90:4); isch: the
+  ! module's ischema for this call (1: met_laplaciens's plain form)
+  subroutine ref_met_smsasas(l, isch, sn, vol, v, mu, dist, g, fd, cfke, tt, gx, gy, gz, sdif, ts6, ts7, tn, cmu) &
+       bind(C, name="ref_met_smsasas")
+    use mod_met_smsasas
+    use chainecarac
+    integer(c_int), value :: l, isch
+    real(c_double) :: sn(*), vol(*), v(*), mu(*), dist(*), g(ip00, 9), fd(ip11, 6), cfke(*), tt(*), gx(*), gy(*)
+    real(c_double) :: gz(*), sdif(*), ts6(*), ts7(*), tn(ip00, 4), cmu(*)
+    integer :: keep
+    equat = heq
+    keep = ischema
+    ischema = isch
+    call met_smsasas(l, sn, vol, v, mu, dist, g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), g(1,8), &
+         g(1,9), fd(1,1), fd(1,2), fd(1,3), cfke, tt, gx, gy, gz, sdif, ts6, ts7, tn(1,1), tn(1,2), tn(1,3), &
+         tn(1,4), cmu, cmu, cmu, cmu, cmu, cmu)
+    ischema = keep
+  end subroutine ref_met_smsasas
+
 end module codens_ref_harness

@@ -374,3 +374,22 @@ def test_smdes(mesh):
         close(r_[3][inner - b.npc], g_[3][inner - b.npc])  # delta
         for q in (11, 17, 18):  # cfke, ts6, ts7
             close(r_[q][inner], g_[q][inner])
+
+
+@pytest.mark.parametrize("ischema", [1, 7])
+def test_smsasas(tsetup, ischema):
+    """met_smsasas through the reference's signature (the module's ischema
+    selects met_laplaciens's form)."""
+    lay, sn, vol, v, mu, mut, dist, c, R, G = tsetup
+    n = lay.ip11
+    rng = np.random.default_rng(419 + ischema)
+    fd = 1e-2 * rng.standard_normal((6, n))
+    cmu = np.ones(n)
+    for l, b in enumerate(lay.blocks):
+        inner = b.interior()
+        g = np.zeros((9, lay.ip00)); g[:, :b.nnn] = 3 * rng.standard_normal((9, b.nnn))
+        w5 = [np.zeros(lay.ip00) for _ in range(5)]
+        r_, g_ = _both(R, G, "ref_met_smsasas", l + 1, ischema, sn, vol, v, mu, dist, g, fd, np.zeros(n), *w5,
+                       np.zeros(n), np.zeros(n), np.zeros((4, lay.ip00)), cmu)
+        for q in (7, 13, 14):  # cfke, ts6, ts7
+            close(r_[q][inner], g_[q][inner])

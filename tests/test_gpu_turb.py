@@ -340,3 +340,27 @@ def test_smdes():
         assert relerr(H(gd)[inner - b.npc], do[inner - b.npc]) <= TOL
         for a, o in zip(go, oo):
             assert relerr(H(a)[inner], o[inner]) <= TOL
+
+
+@pytest.mark.parametrize("ischema", [1, 7])
+def test_smsasas(setup, ischema):
+    """met_smsasas (SA with the SAS destruction, both met_laplaciens forms)
+    against the oracle."""
+    import ctypes
+    from oracle import ptr
+    abi, lay, sn, vol, v, mu, mut, dist, cson, tc, tcl, O = setup
+    n = lay.ip11
+    rng = np.random.default_rng(489 + ischema)
+    fd = 1e-2 * rng.standard_normal((6, n))
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        inner = b.interior()
+        g9 = 3 * rng.standard_normal((9, b.nnn))
+        sb = np.ascontiguousarray(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+        oo = [np.zeros(n) for _ in range(3)]
+        O.lib.orc_met_smsasas(ctypes.byref(ob), ctypes.byref(tc), ischema, ptr(sb), ptr(vol), ptr(v), C.L_(n),
+                              ptr(mu), ptr(dist), ptr(g9), ptr(fd), *[ptr(a) for a in oo])
+        go = [T(np.zeros(n)) for _ in range(3)]
+        abi.met_smsasas(abi.block(b), tcl, ischema, snb(sn, b), T(vol), T(v), T(mu), T(dist), T(g9), T(fd), *go)
+        for a, o in zip(go, oo):
+            assert relerr(H(a)[inner], o[inner]) <= TOL

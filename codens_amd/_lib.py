@@ -84,6 +84,9 @@ SIGNATURES = {
                          c_vp, c_vp, c_vp, c_vp, c_vp],
     "codens_met_smsasas": [P(CodensBlock), P(c_dbl), c_int, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                            c_vp, c_vp],
+    "codens_met_smkl": [P(CodensBlock), c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                        c_vp, c_vp],
+    "codens_met_klmut": [P(CodensBlock), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
     "codens_met_mtcorf1": [c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_i64, c_vp],
     "codens_met_smmt": [P(CodensBlock), c_vp, c_int, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                         c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],

@@ -190,6 +190,22 @@ def met_smsasas(b, tc, ischema, sn_blk, vol, v, mu, dist, g9, fd, cfke, ts6, ts7
                                 _p(dist), _p(g9), _p(fd), _p(cfke), _p(ts6), _p(ts7), _s(stream)))
 
 
+def _kl(kc):
+    """k-l constants cklb1, ckle2, sigmal, xkappa as a host array."""
+    kc = [float(x) for x in kc]
+    assert len(kc) == 4
+    return (ctypes.c_double * 4)(*kc)
+
+
+def met_smkl(b, kc, sn_blk, vol, v, mu, mut, dist, fd, g9, tprod, cfke, ts6, ts7, stream=None):
+    _chk(lib.codens_met_smkl(ctypes.byref(b), _kl(kc), _p(sn_blk), _p(vol), _p(v), ld(v), _p(mu), _p(mut), _p(dist),
+                             _p(fd), _p(g9), _p(tprod), _p(cfke), _p(ts6), _p(ts7), _s(stream)))
+
+
+def met_klmut(b, kc, v, mu, mut, dist, stream=None):
+    _chk(lib.codens_met_klmut(ctypes.byref(b), _kl(kc), _p(v), ld(v), _p(mu), _p(mut), _p(dist), _s(stream)))
+
+
 def _menter(mc):
     """The 10 Menter constants as a host array (betae, okappa, sigme1,
     sigma1, beta1, wsig1, sigme2, sigma2, beta2, wsig2)."""

@@ -359,6 +359,88 @@ __global__ void __launch_bounds__(TX *TY) k_met_smsa(Blk b, Turb tc, const doubl
   cfke[n] = fabs(dsm1 + dsm2 + dsdif);
 }
 
+// ---- Smith k-l model: constants kc = cklb1, ckle2, sigmal, xkappa
+// (atlecdon.f90:226-231) ----
+
+// sqrt(v6/v1) over the block's storage (met_bark.f90:79-87 fills the box
+// i1-1..i2, j1-1..j2, k1-1..k2 of it), block-local
+__global__ void __launch_bounds__(256) k_sqrt_k(long npc, long nnn, const double *__restrict__ v, long ldv,
+                                                double *__restrict__ tk) {
+  long m = blockIdx.x * 256L + threadIdx.x;
+  if (m >= nnn) return;
+  long n = npc + m;
+  tk[m] = sqrt(v[5 * ldv + n] / v[n]);
+}
+
+// met_smkl.f90:163-226 with met_bark.f90:88-116: ts6 = P - csk2 sqrt(k) rho
+// k / l + bark, ts7 = csl1 rho sqrt(k)(1 - l^2/(kappa d)^2) + rho l div u -
+// mut |grad l|^2 l^2/(kappa d)^2 / (sigmal l) + (2/sigmal) mut grad l .
+// grad k / k, cfke the larger modulus of the eigenvalues of the source
+// Jacobian; bark = -2 mu |grad sqrt(k/rho)|^2, grad rho, both Green-Gauss
+__global__ void __launch_bounds__(TX *TY) k_met_smkl(Blk b, double cklb1, double ckle2, double sigmal, double xkappa,
+                                                     const double *__restrict__ sn, const double *__restrict__ vol,
+                                                     const double *__restrict__ v, long ldv,
+                                                     const double *__restrict__ mu, const double *__restrict__ mut,
+                                                     const double *__restrict__ dist, const double *__restrict__ fd,
+                                                     const double *__restrict__ g9, const double *__restrict__ tprod,
+                                                     const double *__restrict__ tk, double *__restrict__ cfke,
+                                                     double *__restrict__ ts6, double *__restrict__ ts7) {
+  REAL_CELL(b)
+  const long N = b.nnn;
+  double gt[3], gr[3];
+  grad_scalar(b, sn, vol, tk - b.npc, nullptr, n, c, gt);
+  grad_scalar(b, sn, vol, v, nullptr, n, c, gr);
+  double bark = -2. * mu[n] * (gt[0] * gt[0] + gt[1] * gt[1] + gt[2] * gt[2]);
+  double csk2 = 2. * sqrt(2.) / cklb1, csl1 = sqrt(2.) * (2. - ckle2) / cklb1;
+  double dssigl = 2. / sigmal, c132 = 1.5 * csk2, csl12 = 2. * csl1;
+  double v1 = v[n], v6 = v[5 * ldv + n], v7 = v[6 * ldv + n];
+  double f5x = fd[n], f5y = fd[ldv + n], f5z = fd[2 * ldv + n];
+  double f6x = fd[3 * ldv + n], f6y = fd[4 * ldv + n], f6z = fd[5 * ldv + n];
+  double xk = v6 / v1, xl = v7 / v1;
+  double kd = xkappa * dist[n];
+  double xkapad2 = kd * kd;
+  double xlskap2 = xl * xl / xkapad2;
+  double rack = sqrt(xk), racrk = sqrt(v6), racro = sqrt(v1);
+  double divv = g9[m] + g9[4 * N + m] + g9[8 * N + m];
+  double glgl = f6x * f6x + f6y * f6y + f6z * f6z;
+  double glgk = (f6x * f5x + f6y * f5y + f6z * f5z) / xk;
+  double sk2 = csk2 * rack * v6 / xl;
+  double sl1 = csl1 * v1 * rack * (1. - xlskap2);
+  double sl2 = v7 * divv;
+  double sl3 = mut[n] * glgl * xlskap2 / (sigmal * xl);
+  double sl4 = dssigl * mut[n] * glgk;
+  ts6[n] = tprod[m] - sk2 + bark;
+  ts7[n] = sl1 + sl2 - sl3 + sl4;
+  double gkgro = (f5x * gr[0] + f5y * gr[1] + f5z * gr[2]) / v6;
+  double glgro = (f6x * gr[0] + f6y * gr[1] + f6z * gr[2]) / v1;
+  double dqkdk = -c132 * racro * racrk / v7 - bark / v6 + mu[n] * gkgro / v1;
+  double dqkdl = sk2 / v7;
+  double dqldl = -csl12 * rack * xl / xkapad2 + divv - mut[n] * (glgl - 2 * xl * glgro) / (sigmal * v1 * xkapad2);
+  double dqldk = dssigl * mut[n] * (-glgk + glgro) / v6;
+  double xdelta = dqkdk * dqkdk + dqldl * dqldl + fabs(-2. * dqkdk * dqldl + 4. * dqkdl * dqldk);
+  double rdelta = sqrt(xdelta);
+  double a1 = fabs((dqkdk + dqldl + rdelta) * 0.5), a2 = fabs((dqkdk + dqldl - rdelta) * 0.5);
+  cfke[n] = a1 > a2 ? a1 : a2;
+}
+
+// met_klmut.f90:38-62: mut = fmu mu chi
+__global__ void __launch_bounds__(TX *TY) k_met_klmut(Blk b, double cklb1, double xkappa, const double *__restrict__ v,
+                                                      long ldv, const double *__restrict__ mu,
+                                                      double *__restrict__ mut, const double *__restrict__ dist) {
+  REAL_CELL(b)
+  const double c1 = 25.5, c2 = 2.;
+  double r2sb1 = sqrt(2.) / pow(cklb1, 1. / 3.);
+  double c14 = ((c1 * c1) * c1) * c1, c22 = c2 * c2;
+  double v1 = v[n], v6 = v[5 * ldv + n], v7 = v[6 * ldv + n];
+  double xxi = r2sb1 * sqrt(v6 / v1) * v7 / mu[n];
+  double xxi2 = xxi * xxi;
+  double q = v7 / (xkappa * v1 * dist[n]);
+  double f1 = exp(-50. * (q * q));
+  double ss2 = (c22 + xxi2) * xxi2;
+  double fmu = pow((c14 * f1 + ss2) / (c14 + ss2), 0.25);
+  mut[n] = fmu * mu[n] * xxi;
+}
+
 // met_smko.f90:80-110 (compressible Wilcox k-omega)
 __global__ void __launch_bounds__(TX *TY) k_met_smko(Blk b, Turb tc, const double *__restrict__ v, long ldv,
                                                      const double *__restrict__ tprod, double *__restrict__ cfke,
@@ -684,4 +766,36 @@ extern "C" int codens_sch_turb(const codens_block *cb, double *u, const double *
                        ts6, ts7, sn, vol);
   }
   return codens_check_launch("codens_sch_turb");
+}
+
+extern "C" int codens_met_smkl(const codens_block *cb, const double *kc, const double *sn, const double *vol,
+                               const double *v, int64_t ldv, const double *mu, const double *mut, const double *dist,
+                               const double *fd, const double *g9, const double *tprod, double *cfke, double *ts6,
+                               double *ts7, void *stream) {
+  CHECK_BLOCK("codens_met_smkl")
+  if (!kc || !sn || !vol || !v || !mu || !mut || !dist || !fd || !g9 || !tprod || !cfke || !ts6 || !ts7) {
+    codens_set_error("codens_met_smkl: null argument");
+    return CODENS_EINVAL;
+  }
+  double *tk = (double *)codens_workspace(sizeof(double) * b.nnn, 7);
+  if (!tk) {
+    codens_set_error("codens_met_smkl: workspace allocation failed");
+    return CODENS_EHIP;
+  }
+  hipLaunchKernelGGL(k_sqrt_k, dim3((unsigned)((b.nnn + 255) / 256)), dim3(256), 0, st, (long)b.npc, (long)b.nnn, v,
+                     (long)ldv, tk);
+  hipLaunchKernelGGL(k_met_smkl, cell_grid(b), dim3(TX, TY), 0, st, b, kc[0], kc[1], kc[2], kc[3], sn, vol, v,
+                     (long)ldv, mu, mut, dist, fd, g9, tprod, (const double *)tk, cfke, ts6, ts7);
+  return codens_check_launch("codens_met_smkl");
+}
+
+extern "C" int codens_met_klmut(const codens_block *cb, const double *kc, const double *v, int64_t ldv,
+                                const double *mu, double *mut, const double *dist, void *stream) {
+  CHECK_BLOCK("codens_met_klmut")
+  if (!kc || !v || !mu || !mut || !dist) {
+    codens_set_error("codens_met_klmut: null argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_met_klmut, cell_grid(b), dim3(TX, TY), 0, st, b, kc[0], kc[3], v, (long)ldv, mu, mut, dist);
+  return codens_check_launch("codens_met_klmut");
 }

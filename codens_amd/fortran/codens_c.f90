@@ -290,6 +290,21 @@ module codens_c
        type(c_ptr), value :: sn, vol, v, mu, dist, g9, fd, cfke, ts6, ts7, stream
        integer(c_int64_t), value :: ldv
      end function codens_met_smsasas
+     integer(c_int32_t) function codens_met_smkl(b, kc, sn, vol, v, ldv, mu, mut, dist, fd, g9, tprod, cfke, ts6, &
+          ts7, stream) bind(C, name="codens_met_smkl")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: kc(4)
+       type(c_ptr), value :: sn, vol, v, mu, mut, dist, fd, g9, tprod, cfke, ts6, ts7, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_smkl
+     integer(c_int32_t) function codens_met_klmut(b, kc, v, ldv, mu, mut, dist, stream) bind(C, name="codens_met_klmut")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: kc(4)
+       type(c_ptr), value :: v, mu, mut, dist, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_klmut
      integer(c_int32_t) function codens_met_mtcorf1(nfaces, face_start, lines, imxclko, mnpar, frac, nfrac, stream) &
           bind(C, name="codens_met_mtcorf1")
        import

@@ -175,6 +175,21 @@ int codens_met_smsasas(const codens_block *b, const double *tc, int32_t ischema,
                        const double *v, int64_t ldv, const double *mu, const double *dist, const double *g9,
                        const double *fd, double *cfke, double *ts6, double *ts7, void *stream);
 
+/* ---- Smith k-l model: kc = cklb1, ckle2, sigmal, xkappa (module
+ * modeleturb, atlecdon.f90:226-231) ---- */
+
+/* met_smkl.f90:4 (with met_bark.f90:4) -- k-l sources ts6, ts7 and the
+ * implicit coefficient cfke (the larger modulus of the source Jacobian's
+ * eigenvalues); fd: gradients of rho k and rho l (6 planes of ldv), g9 and
+ * tprod block-local. */
+int codens_met_smkl(const codens_block *b, const double *kc, const double *sn, const double *vol, const double *v,
+                    int64_t ldv, const double *mu, const double *mut, const double *dist, const double *fd,
+                    const double *g9, const double *tprod, double *cfke, double *ts6, double *ts7, void *stream);
+
+/* met_klmut.f90:4 -- k-l eddy viscosity. */
+int codens_met_klmut(const codens_block *b, const double *kc, const double *v, int64_t ldv, const double *mu,
+                     double *mut, const double *dist, void *stream);
+
 /* met_smko.f90:4 -- compressible Wilcox k-omega sources. */
 int codens_met_smko(const codens_block *b, const double *tc, const double *v, int64_t ldv, const double *tprod,
                     double *cfke, double *ts6, double *ts7, const double *cson, void *stream);

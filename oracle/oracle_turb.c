@@ -455,6 +455,112 @@ void orc_met_smsasas(const orc_block *b, const orc_turb *tc, int ischema, const 
   free(w);
 }
 
+/* k-l model constants (module modeleturb, atlecdon.f90:226-231) */
+typedef struct {
+  double cklb1, ckle2, sigmal, xkappa;
+} orc_kl;
+
+/* met_bark.f90:4 -- bark = -2 mu |grad sqrt(k/rho)|^2 on the storage range
+ * indc(i1,j1,k1)..indc(i2-1,j2-1,k2-1) (block-local), the gradient by
+ * teq_grads of sqrt(v6/v1) filled on the box i1-1..i2 etc. */
+static void bark_of(const orc_block *b, const double *sn, const double *vol, const double *v, long ldv,
+                    const double *mu, double *bark) {
+  long nnn = b->nnn, n0 = b->npc;
+  double *t = calloc(nnn, 8), *g = calloc(3 * nnn, 8);
+  for (int k = b->kk1 - 1; k <= b->kk2; k++)
+    for (int j = b->jj1 - 1; j <= b->jj2; j++)
+      for (int i = b->ii1 - 1; i <= b->ii2; i++) {
+        long n = IDX(b, i, j, k);
+        t[n - n0] = sqrt(v[5 * ldv + n] / v[n]);
+      }
+  orc_teq_grads(b, sn, vol, t, g);
+  for (long n = IDX(b, b->ii1, b->jj1, b->kk1); n <= IDX(b, b->ii2 - 1, b->jj2 - 1, b->kk2 - 1); n++) {
+    long m = n - n0;
+    double x = g[m] * g[m] + g[nnn + m] * g[nnn + m] + g[2 * nnn + m] * g[2 * nnn + m];
+    bark[m] = -2. * mu[n] * x;
+  }
+  free(t); free(g);
+}
+
+/* met_smkl.f90:4 -- Smith k-l sources: ts6 = P - csk2 sqrt(k) rho k / l +
+ * bark, ts7 = csl1 rho sqrt(k) (1 - l^2/(kappa d)^2) + rho l div u - mut
+ * |grad l|^2 l/(kappa d)^2/(sigmal l) + (2/sigmal) mut grad l.grad k / k,
+ * and cfke the larger modulus of the eigenvalues of the 2x2 source Jacobian
+ * (:163-226; bark from met_bark, grad rho by teq_grads).  fd: gradients of
+ * rho k and rho l (six planes of ldv); g9, tprod block-local. */
+void orc_met_smkl(const orc_block *b, const orc_kl *kc, const double *sn, const double *vol, const double *v, long ldv,
+                  const double *mu, const double *mut, const double *dist, const double *fd, const double *g9,
+                  const double *tprod, double *cfke, double *ts6, double *ts7) {
+  long nnn = b->nnn, n0 = b->npc;
+  double *bark = calloc(nnn, 8), *t = calloc(nnn, 8), *gr = calloc(3 * nnn, 8);
+  bark_of(b, sn, vol, v, ldv, mu, bark);
+  fill_box(b, t, v, NULL);
+  orc_teq_grads(b, sn, vol, t, gr);
+  double csk2 = 2. * sqrt(2.) / kc->cklb1, csl1 = sqrt(2.) * (2. - kc->ckle2) / kc->cklb1;
+  double dssigl = 2. / kc->sigmal, c132 = 1.5 * csk2, csl12 = 2. * csl1;
+  const double *f5x = fd, *f5y = fd + ldv, *f5z = fd + 2 * ldv, *f6x = fd + 3 * ldv, *f6y = fd + 4 * ldv,
+               *f6z = fd + 5 * ldv;
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k), m = n - n0;
+        double v1 = v[n], v6 = v[5 * ldv + n], v7 = v[6 * ldv + n];
+        double xk = v6 / v1, xl = v7 / v1;
+        double kd = kc->xkappa * dist[n];
+        double xkapad2 = kd * kd;
+        double xlskap2 = xl * xl / xkapad2;
+        double rack = sqrt(xk), racrk = sqrt(v6), racro = sqrt(v1);
+        double divv = g9[m] + g9[4 * nnn + m] + g9[8 * nnn + m];
+        double glgl = f6x[n] * f6x[n] + f6y[n] * f6y[n] + f6z[n] * f6z[n];
+        double glgk = (f6x[n] * f5x[n] + f6y[n] * f5y[n] + f6z[n] * f5z[n]) / xk;
+        double sk2 = csk2 * rack * v6 / xl;
+        double sl1 = csl1 * v1 * rack * (1. - xlskap2);
+        double sl2 = v7 * divv;
+        double sl3 = mut[n] * glgl * xlskap2 / (kc->sigmal * xl);
+        double sl4 = dssigl * mut[n] * glgk;
+        ts6[n] = tprod[m] - sk2 + bark[m];
+        ts7[n] = sl1 + sl2 - sl3 + sl4;
+        double gkgro = (f5x[n] * gr[m] + f5y[n] * gr[nnn + m] + f5z[n] * gr[2 * nnn + m]) / v6;
+        double glgro = (f6x[n] * gr[m] + f6y[n] * gr[nnn + m] + f6z[n] * gr[2 * nnn + m]) / v1;
+        double dqkdk = -c132 * racro * racrk / v7 - bark[m] / v6 + mu[n] * gkgro / v1;
+        double dqkdl = sk2 / v7;
+        double dqldl = -csl12 * rack * xl / xkapad2 + divv - mut[n] * (glgl - 2 * xl * glgro) / (kc->sigmal * v1 * xkapad2);
+        double dqldk = dssigl * mut[n] * (-glgk + glgro) / v6;
+        double xdelta = dqkdk * dqkdk + dqldl * dqldl + fabs(-2. * dqkdk * dqldl + 4. * dqkdl * dqldk);
+        double rdelta = sqrt(xdelta);
+        double xl1 = (dqkdk + dqldl + rdelta) * 0.5, xl2 = (dqkdk + dqldl - rdelta) * 0.5;
+        double a1 = fabs(xl1), a2 = fabs(xl2);
+        cfke[n] = a1 > a2 ? a1 : a2;
+      }
+  free(bark); free(t); free(gr);
+}
+
+/* met_klmut.f90:4 -- Smith k-l eddy viscosity mut = fmu mu chi, chi =
+ * sqrt(2)/cklb1^(1/3) sqrt(k/rho) rho l / mu, fmu = ((c1^4 f1 + s)/(c1^4 +
+ * s))^(1/4), s = (c2^2 + chi^2) chi^2, f1 = exp(-50 (l/(kappa d))^2),
+ * c1 = 25.5, c2 = 2 (:38-62). */
+void orc_met_klmut(const orc_block *b, const orc_kl *kc, const double *v, long ldv, const double *mu, double *mut,
+                   const double *dist) {
+  double c1 = 25.5, c2 = 2.;
+  double r2sb1 = sqrt(2.) / pow(kc->cklb1, 1. / 3.);
+  double c14 = ((c1 * c1) * c1) * c1, c22 = c2 * c2;
+  for (int k = b->kk1; k <= b->kk2 - 1; k++)
+    for (int j = b->jj1; j <= b->jj2 - 1; j++)
+      for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+        long n = IDX(b, i, j, k);
+        double v1 = v[n], v6 = v[5 * ldv + n], v7 = v[6 * ldv + n];
+        double rack = sqrt(v6 / v1);
+        double xxi = r2sb1 * rack * v7 / mu[n];
+        double xxi2 = xxi * xxi;
+        double q = v7 / (kc->xkappa * v1 * dist[n]);
+        double xl2 = q * q;
+        double f1 = exp(-50. * xl2);
+        double ss2 = (c22 + xxi2) * xxi2;
+        double fmu = pow((c14 * f1 + ss2) / (c14 + ss2), 0.25);
+        mut[n] = fmu * mu[n] * xxi;
+      }
+}
+
 /* met_smko.f90:80-110: compressible Wilcox k-omega sources. */
 void orc_met_smko(const orc_block *b, const orc_turb *tc, const double *v, long ldv, const double *tprod,
                   double *cfke, double *ts6, double *ts7, const double *cson) {

@@ -569,4 +569,33 @@ contains
     ischema = keep
   end subroutine ref_met_smsasas
 
+  ! k-l constants (atlecdon.f90:226-231): cklb1, ckle2, sigmal, xkappa
+  subroutine ref_set_kl(kc) bind(C, name="ref_set_kl")
+    real(c_double) :: kc(4)
+    cklb1 = kc(1); ckle2 = kc(2); sigmal = kc(3); xkappa = kc(4)
+  end subroutine ref_set_kl
+
+  ! met_smkl.f90:4 (calls met_bark.f90:4)
+  subroutine ref_met_smkl(l, sn, vol, v, mu, mut, dist, mnpar, g, fd, tprod, cfke, tt, gx, gy, gz, bark, ts6, ts7, &
+       cmu) bind(C, name="ref_met_smkl")
+    use mod_met_smkl
+    use chainecarac
+    integer(c_int), value :: l
+    integer(c_int) :: mnpar(*)
+    real(c_double) :: sn(*), vol(*), v(*), mu(*), mut(*), dist(*), g(ip00, 9), fd(ip11, 6), tprod(*), cfke(*)
+    real(c_double) :: tt(*), gx(*), gy(*), gz(*), bark(*), ts6(*), ts7(*), cmu(*)
+    equat = heq
+    call met_smkl(l, sn, vol, v, mu, mut, dist, mnpar, g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), &
+         g(1,8), g(1,9), fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6), tprod, cfke, tt, gx, gy, gz, bark, &
+         ts6, ts7, cmu, cmu, cmu, cmu, cmu, cmu)
+  end subroutine ref_met_smkl
+
+  ! met_klmut.f90:4
+  subroutine ref_met_klmut(l, v, mu, mut, dist) bind(C, name="ref_met_klmut")
+    use mod_met_klmut
+    integer(c_int), value :: l
+    real(c_double) :: v(*), mu(*), mut(*), dist(*)
+    call met_klmut(l, v, mu, mut, dist)
+  end subroutine ref_met_klmut
+
 end module codens_ref_harness

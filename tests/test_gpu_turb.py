@@ -364,3 +364,33 @@ def test_smsasas(setup, ischema):
         abi.met_smsasas(abi.block(b), tcl, ischema, snb(sn, b), T(vol), T(v), T(mu), T(dist), T(g9), T(fd), *go)
         for a, o in zip(go, oo):
             assert relerr(H(a)[inner], o[inner]) <= TOL
+
+
+def test_kl_model(setup):
+    """Smith k-l model kernels (met_smkl with met_bark, met_klmut) against
+    the oracle."""
+    import ctypes
+    from oracle import ptr
+    abi, lay, sn, vol, v, mu, mut, dist, cson, tc, tcl, O = setup
+    n = lay.ip11
+    kl = np.array([18.0, 1.2, 1.43, 0.41])
+    rng = np.random.default_rng(497)
+    vk = v.copy(); vk[6] = 0.05 * (1 + rng.random(n)) * v[0]
+    fd = np.zeros((6, n)); fd[:3] = 1e-3 * rng.standard_normal((3, n)); fd[3:] = 1e-2 * rng.standard_normal((3, n))
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        inner = b.interior()
+        g9 = rng.standard_normal((9, b.nnn)); tp = 1e-3 * rng.random(b.nnn)
+        sb = np.ascontiguousarray(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+        oo = [np.zeros(n) for _ in range(3)]
+        O.lib.orc_met_smkl(ctypes.byref(ob), ptr(kl), ptr(sb), ptr(vol), ptr(vk), C.L_(n), ptr(mu), ptr(mut), ptr(dist),
+                           ptr(fd), ptr(g9), ptr(tp), *[ptr(a) for a in oo])
+        go = [T(np.zeros(n)) for _ in range(3)]
+        abi.met_smkl(abi.block(b), kl, snb(sn, b), T(vol), T(vk), T(mu), T(mut), T(dist), T(fd), T(g9), T(tp), *go)
+        for a, o in zip(go, oo):
+            assert relerr(H(a)[inner], o[inner]) <= TOL
+        mo = np.zeros(n)
+        O.lib.orc_met_klmut(ctypes.byref(ob), ptr(kl), ptr(vk), C.L_(n), ptr(mu), ptr(mo), ptr(dist))
+        gm = T(np.zeros(n))
+        abi.met_klmut(abi.block(b), kl, T(vk), T(mu), gm, T(dist))
+        assert relerr(H(gm)[inner], mo[inner]) <= TOL

@@ -295,3 +295,38 @@ def test_smsasas(setup, ischema):
                        ptr(dist), ptr(gb), ptr(fd), *[ptr(a) for a in s])
         differs = r[1][inner] != s[1][inner]
         assert differs.any()
+
+
+KL = np.array([18.0, 1.2, 1.43, 0.41])  # cklb1, ckle2, sigmal, xkappa (atlecdon.f90:226-231)
+
+
+def test_kl_model(setup):
+    """Smith k-l model: met_smkl (with met_bark) and met_klmut, bit for bit."""
+    lay, sn, vol, v, mu, mut, dist, cson, tc, R, O = setup
+    n = lay.ip11
+    rng = np.random.default_rng(97)
+    R.lib.ref_set_kl(ptr(KL))
+    vk = v.copy(); vk[6] = 0.05 * (1 + rng.random(n)) * v[0]  # rho l
+    fd = np.zeros((6, n)); fd[:3] = 1e-3 * rng.standard_normal((3, n)); fd[3:] = 1e-2 * rng.standard_normal((3, n))
+    mnpar = np.zeros(n, dtype=np.int32)
+    cmu = np.ones(n)
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        inner = b.interior()
+        g = np.zeros((9, lay.ip00)); g[:, :b.nnn] = rng.standard_normal((9, b.nnn))
+        gb = np.ascontiguousarray(g[:, :b.nnn])
+        tp = np.zeros(lay.ip00); tp[:b.nnn] = 1e-3 * rng.random(b.nnn)
+        w = [np.zeros(lay.ip00) for _ in range(5)]
+        r = [np.full(n, 7.0) for _ in range(3)]
+        R.lib.ref_met_smkl(l + 1, ptr(sn), ptr(vol), ptr(vk), ptr(mu), ptr(mut), ptr(dist), ptr(mnpar), ptr(g),
+                           ptr(fd), ptr(tp), ptr(r[0]), *[ptr(a) for a in w], ptr(r[1]), ptr(r[2]), ptr(cmu))
+        o = [np.full(n, 7.0) for _ in range(3)]
+        O.orc_met_smkl(ctypes.byref(ob), ptr(KL), ptr(snb(sn, b)), ptr(vol), ptr(vk), C.L_(n), ptr(mu), ptr(mut),
+                       ptr(dist), ptr(fd), ptr(gb), ptr(tp), *[ptr(a) for a in o])
+        for a, c in zip(r, o):
+            assert np.array_equal(a[inner], c[inner])
+            assert np.isfinite(a[inner]).all()
+        mr, mo = np.full(n, 7.0), np.full(n, 7.0)
+        R.lib.ref_met_klmut(l + 1, ptr(vk), ptr(mu), ptr(mr), ptr(dist))
+        O.orc_met_klmut(ctypes.byref(ob), ptr(KL), ptr(vk), C.L_(n), ptr(mu), ptr(mo), ptr(dist))
+        assert np.array_equal(mr, mo)

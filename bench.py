@@ -6,6 +6,12 @@ far-field ghosts, local time step, matrix-free point-Jacobi implicit phase with
 lmax=10 sub-iterations, all FP64 (codens_amd.configs.wall_ns).  One step is one
 implicit iteration (one atsch_num stage); inputs are resident in HBM.
 
+With --workload multiblock: 48 Cartesian blocks of seeded uneven sizes (each
+axis 16..192 cells, 39,991,040 cells in all) joined 1-to-1 along a seeded
+adjacency tree, Spalart-Allmaras RANS at Re=5e6, M=0.7, the same implicit
+phase (codens_amd.configs.multiblock_sa); stages captured in a HIP graph
+(--graph, on by default for this workload).
+
 Prints one JSON line: Mcell-updates/s, the roofline of the dominant kernel
 (implimf_sweep: 264 algorithmic bytes per real cell per launch) measured with
 HIP events on the launch stream, and the reference Fortran build timed on the
@@ -29,9 +35,9 @@ SWEEP_TRAFFIC_BYTES = 632512 * 1024
 HBM_PEAK_GBS = 8000.0
 
 
-def cpu_baseline(case, stages):
+def cpu_baseline(case, stages, sample=None):
     """The reference routines (flang build, 1 core) on `stages` implicit
-    iterations of the same case; falls back to the C restatement."""
+    iterations of `case`; falls back to the C restatement."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle
     import cpu_stage
@@ -43,7 +49,7 @@ def cpu_baseline(case, stages):
     dt = time.perf_counter() - t0
     return {"value": case.layout.ncells * stages / dt / 1e6, "unit": "Mcell-updates/s", "cores": 1,
             "kind": "reference" if use_ref else "port",
-            "sample": f"{stages} implicit iterations of the same 256x128x64 case, lmax=10, "
+            "sample": f"{stages} implicit iterations of {sample or 'the same 256x128x64 case'}, lmax=10, "
                       f"{'flang -O2 build of the reference routines' if use_ref else 'oracle C restatement'}"}
 
 
@@ -61,6 +67,10 @@ def main():
     ap.add_argument("--cpu-stages", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dims", type=str, default="256,128,64")
+    ap.add_argument("--workload", choices=("wall", "multiblock"), default="wall",
+                    help="wall: the 256x128x64 laminar wall block; multiblock: 48 uneven SA blocks, ~40M cells")
+    ap.add_argument("--graph", choices=("auto", "on", "off"), default="auto",
+                    help="replay stages from a captured HIP graph (auto: on for multiblock)")
     ap.add_argument("--implicit", choices=("fp64", "fp32"), default="fp64",
                     help="implicit phase: implimf_3d in FP64 (the reference's), or its FP32 defect-correction form")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -75,10 +85,15 @@ def main():
     from codens_amd import abi, configs
     from codens_amd.solver import Solver
 
+    multi = args.workload == "multiblock"
     ci, cj, ck = (int(x) for x in args.dims.split(","))
-    case = configs.wall_ns(ci, cj, ck)
+    case = configs.multiblock_sa() if multi else configs.wall_ns(ci, cj, ck)
     s = Solver(case, implicit=args.implicit)
     ncell = case.layout.ncells
+    nblk = len(case.layout.blocks)
+    graph = args.graph == "on" or (args.graph == "auto" and multi)
+    if graph:
+        s.capture()  # one eager stage, then the captured one
     for _ in range(args.warmup):
         s.stage()
     torch.cuda.synchronize()
@@ -93,7 +108,9 @@ def main():
         s.stage()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    # per-kernel timing pass with HIP events on the launch stream
+    # per-kernel timing pass with HIP events on the launch stream (eager
+    # stages: the same launches as the graph's)
+    g_saved, s.graph = s.graph, None
     abi.timing_enable(True)
     for _ in range(max(3, args.steps // 4)):
         s.stage()
@@ -104,25 +121,33 @@ def main():
         if cnt:
             kern[name] = {"avg_us": 1e3 * ms / cnt, "launches": cnt}
     abi.timing_enable(False)
+    s.graph = g_saved
     res = s.density_residual()
     ms_step = 1e3 * wall / args.steps
     value = ncell * args.steps / wall / 1e6
     fp32 = args.implicit == "fp32"
     sweep_name = "implimf32_sweep" if fp32 else "implimf_sweep"
     sweep_bytes = SWEEP_BYTES_PER_CELL // 2 if fp32 else SWEEP_BYTES_PER_CELL  # the same 33 values in FP32
-    sw = kern.get(sweep_name, {"avg_us": float("nan")})
-    achieved = sweep_bytes * ncell / (sw["avg_us"] * 1e-6) / 1e9
+    sw = kern.get(sweep_name, {"avg_us": float("nan"), "launches": nblk})
+    # bytes of all sweep launches over their summed time; with uneven blocks
+    # a launch averages sweep_bytes x (cells / blocks)
+    achieved = sweep_bytes * ncell / nblk / (sw["avg_us"] * 1e-6) / 1e9
     out = {
         "metric": "Mcell-updates/s per implicit iter to 1e-8 residual drop; % of HBM roofline",
         "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64 residual/update, f32 implicit" if fp32 else "f64", "data": "synthetic",
-        "config": {"workload": f"wall_ns {ci}x{cj}x{ck} laminar NS, Roe-MUSCL minmod + viscous, implimf_3d lmax=10, "
-                               + ("FP32 implicit phase (defect correction)" if fp32 else "FP64"),
-                   "cells": ncell, "lmax": case.lmax},
+        "config": {"workload": (f"multiblock_sa {nblk} blocks (axes 16..192, seed 20261203), coincident patches, "
+                                "Spalart-Allmaras RANS Re=5e6 M=0.7, Roe-MUSCL minmod + viscous, implimf_3d lmax=10, "
+                                if multi else
+                                f"wall_ns {ci}x{cj}x{ck} laminar NS, Roe-MUSCL minmod + viscous, implimf_3d lmax=10, ")
+                               + ("FP32 implicit phase (defect correction)" if fp32 else "FP64")
+                               + (", stages replayed from a HIP graph" if graph else ""),
+                   "cells": ncell, "blocks": nblk, "lmax": case.lmax},
         "roofline": {"bound": "hbm", "kernel": sweep_name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": SWEEP_TRAFFIC_BYTES if (ci, cj, ck) == (256, 128, 64) and not fp32 else None,
+                     "traffic": SWEEP_TRAFFIC_BYTES if (ci, cj, ck) == (256, 128, 64) and not fp32 and not multi
+                     else None,
                      "bytes_per_cell": sweep_bytes,
                      "avg_us": round(sw["avg_us"], 2)},
         "kernels": kern,
@@ -135,7 +160,15 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(case, args.cpu_stages)
+        if multi:
+            del s
+            torch.cuda.empty_cache()
+            small = configs.multiblock_sa(48, 16, 40, None)
+            out["cpu_baseline"] = cpu_baseline(
+                small, 2, f"a 48-block draw with axes 16..40 ({small.layout.ncells} cells, same seed, "
+                          "same physics)")
+        else:
+            out["cpu_baseline"] = cpu_baseline(case, args.cpu_stages)
     print(json.dumps(out))
 
 

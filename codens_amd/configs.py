@@ -246,3 +246,124 @@ def wall_rans(ci=48, cj=32, ck=16, model="1SA", klroe=2, kwsst=0, nut_ratio=3.0,
     c.name = f"wall_rans_{model}"
     c.turb = TurbCfg(model, turb_constants(model), dist, klroe=klroe, kwsst=kwsst)
     return c
+
+
+def _face_grid(b: L.Block, face: str):
+    """Cell counts of a block face in face_cells' order (outer, inner)."""
+    ci, cj, ck = b.ni - 1, b.nj - 1, b.nk - 1
+    return {"i": (ck, cj), "j": (ck, ci), "k": (cj, ci)}[face[0]]
+
+
+def _opposite(face: str) -> str:
+    return face[0] + ("1" if face[1] == "2" else "2")
+
+
+def multiblock_dims(nblocks=48, lo=16, hi=192, total=(38_000_000, 42_000_000), seed=20261203):
+    """Seeded block dimensions, each axis uniform in lo..hi cells, drawn again
+    (all blocks together) until the total cell count falls in `total`."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    for _ in range(1_000_000):
+        dims = rng.integers(lo, hi + 1, size=(nblocks, 3))
+        tot = int(np.prod(dims, axis=1).sum())
+        if total is None or total[0] <= tot <= total[1]:
+            return [tuple(int(x) for x in d) for d in dims], rng
+    raise RuntimeError("no block dimensions in the requested total")
+
+
+def multiblock_sa(nblocks=48, lo=16, hi=192, total=(38_000_000, 42_000_000), seed=20261203, mach=0.7,
+                  reynz=5e6, nut_ratio=3.0, h=1.0 / 128.0, eta=10.0, lmax=10, ilim=1) -> Case:
+    """Uneven multi-block RANS case: `nblocks` Cartesian blocks of spacing h
+    joined 1-to-1 through coincident face patches (rbvc.f90) along a seeded
+    adjacency tree, Spalart-Allmaras, implicit (kmf=1) with local steps.
+
+    Tree: block b (1..n-1) attaches to a free face of a block drawn among
+    0..b-1 through its opposite face, so every block sits on one integer
+    lattice of spacing h and the two faces' cells coincide.  The patch is the
+    low corner of both faces, min(extent) in each tangential direction; the
+    rest of every face, and every face left free, is far field (clnrd), except
+    the j1 face of block 0, which is an adiabatic no-slip wall (clpara) and
+    the plane the wall distance is measured from.  nu~ = nut_ratio x the
+    laminar kinematic viscosity everywhere."""
+    dims, rng = multiblock_dims(nblocks, lo, hi, total, seed)
+    lay = L.Layout(dims)
+    B = lay.blocks
+    free = [set(L.FACES) for _ in B]
+    free[0].discard("j1")
+    origin = np.zeros((nblocks, 3), dtype=np.int64)
+    links = []
+    for b in range(1, nblocks):
+        cands = [(p, f) for p in range(b) for f in L.FACES if f in free[p]]
+        p, f = cands[int(rng.integers(len(cands)))]
+        d = "ijk".index(f[0])
+        g = _opposite(f)
+        origin[b] = origin[p]
+        origin[b, d] = origin[p, d] + dims[p][d] if f[1] == "2" else origin[p, d] - dims[b][d]
+        free[p].discard(f)
+        free[b].discard(g)
+        links.append((p, f, b, g))
+    n = lay.ip11
+    x = np.zeros(n); y = np.zeros(n); z = np.zeros(n)
+    for l, b in enumerate(B):
+        ci, cj, ck = dims[l]
+        zk = (origin[l, 2] + np.arange(ck + 1)) * h
+        yj = (origin[l, 1] + np.arange(cj + 1)) * h
+        xi = (origin[l, 0] + np.arange(ci + 1)) * h
+        Z, Y, X = np.meshgrid(zk, yj, xi, indexing="ij")
+        sl = (slice(b.kk1 - b.kd1, b.kk2 - b.kd1 + 1), slice(b.jj1 - b.jd1, b.jj2 - b.jd1 + 1),
+              slice(b.ii1 - b.id1, b.ii2 - b.id1 + 1))
+        b.view(x)[sl] = X; b.view(y)[sl] = Y; b.view(z)[sl] = Z
+    sn, vol = L.metrics(lay, x, y, z)
+    # faces: coincident patches first, the rest far field / wall
+    used = {(l, f): np.zeros(_face_grid(B[l], f), dtype=bool) for l in range(nblocks) for f in L.FACES}
+    cells = {}
+    def fc(l, f):
+        if (l, f) not in cells:
+            g, i, fi = L.face_cells(B[l], f)
+            s = _face_grid(B[l], f)
+            cells[(l, f)] = (g.reshape(s), i.reshape(s), fi.reshape(s))
+        return cells[(l, f)]
+    coin = []
+    for p, f, b, g in links:
+        gp, ip_, _ = fc(p, f)
+        gb, ib, _ = fc(b, g)
+        a0, a1 = min(gp.shape[0], gb.shape[0]), min(gp.shape[1], gb.shape[1])
+        P = (slice(0, a0), slice(0, a1))
+        coin.append(FaceSet(gp[P].ravel(), ip_[P].ravel(), ib[P].ravel()))
+        coin.append(FaceSet(gb[P].ravel(), ib[P].ravel(), ip_[P].ravel()))
+        used[(p, f)][P] = True
+        used[(b, g)][P] = True
+    wall_parts, far_parts = [], []
+    for l, b in enumerate(B):
+        for f in L.FACES:
+            g, i, fi = fc(l, f)
+            m = ~used[(l, f)]
+            if not m.any():
+                continue
+            g, i, fi = g[m], i[m], fi[m]
+            if l == 0 and f == "j1":
+                wall_parts.append(FaceSet(g, i))
+            else:
+                nx, ny, nz = L.inward_normals(b, sn, f, fi)
+                far_parts.append(FaceSet(g, i, None, np.stack([nx, ny, nz])))
+    coinc = _cat(coin)
+    coinc.normals = np.zeros((3, len(coinc.ghost)))
+    mu_inf = 1.0 / reynz  # Sutherland at T = 1 (zvismo.f90:90-104)
+    v0 = np.zeros((7, n))
+    v0[:5] = conservative(np.ones(n), np.full(n, mach), np.zeros(n), np.zeros(n), np.full(n, 1.0 / GAM))
+    v0[5] = nut_ratio * mu_inf
+    w = v0[:5, 0].copy()  # far-field state of the mean flow (clnrd)
+    y0 = origin[0, 1] * h
+    dist = np.ones(n)
+    for l, b in enumerate(B):
+        ci, cj, ck = dims[l]
+        yc = (origin[l, 1] + np.arange(cj) + 0.5) * h
+        b.view(dist)[b.kk1 - b.kd1:b.kk2 - b.kd1, b.jj1 - b.jd1:b.jj2 - b.jd1, b.ii1 - b.id1:b.ii2 - b.id1] = \
+            np.abs(yc - y0)[None, :, None]
+    c = Case(f"multiblock_sa_{nblocks}", lay, sn, vol, v0, True,
+             dict(gam=GAM, pinfl=0.0, pr=0.72, prt=1.0, reynz=reynz, tnz=288.15, iflu=1),
+             dict(muscl=1, ilim=ilim, xk=1.0 / 3.0), eta, 1, 0.0, 1, lmax, w,
+             coinc, _cat(wall_parts), _cat(far_parts))
+    c.turb = TurbCfg("1SA", turb_constants("1SA"), dist, klroe=2)
+    c.links = links
+    c.origin = origin
+    return c

@@ -69,6 +69,7 @@ class Solver:
             self.tprod = [f(b.nnn) for b in lay.blocks]
         self.cycle = 0
         self.res0 = None
+        self.graph = None
 
     # -- pieces of one stage ------------------------------------------------
     def thermo(self):
@@ -172,7 +173,19 @@ class Solver:
                    self.vol, self.ps, self.cson, stream=s)
 
     def stage(self):
-        """One residual + update of all blocks (atsch_num.f90:5)."""
+        """One residual + update of all blocks (atsch_num.f90:5); a replay of
+        the captured graph after capture()."""
+        if self.graph is not None:
+            if self.stream is not None:
+                with torch.cuda.stream(self.stream):
+                    self.graph.replay()
+            else:
+                self.graph.replay()
+        else:
+            self._stage_body()
+        self.cycle += 1
+
+    def _stage_body(self):
         self.thermo()
         self.time_step()
         self.ghosts()
@@ -182,7 +195,32 @@ class Solver:
             self.stresses()
         self.residual()
         self.update()
-        self.cycle += 1
+
+    def capture(self):
+        """Record one stage into a HIP graph (torch.cuda.CUDAGraph); later
+        calls of stage() replay it.  The first stage runs eagerly, so that
+        every workspace the stage uses is allocated before the capture (no
+        allocation may happen while a stream is captured); the graph then
+        holds the identical launch sequence with the identical pointers.
+        Only with local time steps (kdtl=1): the global step (kdtl=0) reads a
+        reduction on the host inside the stage.  No launch in the stage
+        records a memset, so replays repeat the eager results exactly."""
+        if self.case.kdtl == 0:
+            raise ValueError("capture needs local time steps (kdtl=1): the global step syncs the host")
+        if self.graph is not None:
+            return
+        self.stage()
+        saved, self.stream = self.stream, None  # launch on the capturing stream
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        try:
+            with torch.cuda.graph(g, stream=side):
+                self._stage_body()
+        finally:
+            self.stream = saved
+        torch.cuda.current_stream().wait_stream(side)
+        self.graph = g
 
     def density_residual(self):
         """||(rho^{n+1}-rho^n)/dt||_2 of the last implicit stage: the increment

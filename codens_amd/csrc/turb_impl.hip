@@ -16,9 +16,11 @@ using namespace codens;
 
 // Workspace of the implicit phase, block-local index m: rv, diag, the half
 // normal velocities u1..u3 and coefficients of the faces, the elimination
-// coefficients alpha, beta6, beta7 and the increments dw6, dw7.
+// coefficients alpha, beta6, beta7 (line-major), the increments dw6, dw7, and
+// the line systems of the current sweep (line-major).
 struct ImpliWs {
   double *rv, *cd, *uf, *ce, *al, *b6, *b7, *dw6, *dw7;
+  double *la, *lb, *lc, *l6, *l7;  // line-major coefficients and right-hand sides of one sweep
 };
 
 // impli2_eqt_3d.f90:113-122: rv and the diagonal's own part
@@ -73,64 +75,182 @@ __global__ void __launch_bounds__(TX *TY) k_impli2_diag(Blk b, ImpliWs w) {
             ce[2 * nnn + m + b.nijd];
 }
 
-// Elimination of one sweep along direction d (impli2_eqt_3d.f90:238-263,
-// 292-317, 346-371): one thread per grid line, from its high end; the
-// recursion values of the previous face stay in registers (zero beyond the
-// last real cell, as the reference's zeroed arrays give).  The transverse
-// increments are read only, so all lines see the previous sweep's values.
-__global__ void __launch_bounds__(64) k_impli2_elim(Blk b, int d, const double *__restrict__ u, long ldv, ImpliWs w) {
-  int t = blockIdx.x * 64 + threadIdx.x;
-  int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
-  int np_ = b.hi(p) - b.lo(p), nq = b.hi(q) - b.lo(q);
-  if (t >= np_ * nq) return;
+// One sweep along direction d (impli2_eqt_3d.f90:238-275, 292-329, 346-383)
+// in two launches.
+//
+// k_impli2_rhs, over all real cells: the line system's coefficients a, b, c
+// and the right-hand sides of both equations, whose transverse terms read
+// the previous sweep's increments only, so every cell is independent.  They
+// go to a line-major layout, entry (a - lo_d) * nl + line with the line index
+// (c_p - lo_p) + (c_q - lo_q) * n_p, so that the recursion below reads
+// consecutive lines at consecutive addresses.  Tiles are 16 x 16 cells of one
+// k plane; for i lines the tile is transposed through LDS, so that both the
+// reads (i fastest) and the writes (j fastest) move 128-byte runs.
+//
+// k_impli2_line, one thread per line: the elimination from the high end of
+// the line (recursion values zero beyond the last real cell, as the
+// reference's zeroed arrays give) and then the substitution from the low
+// end, which writes the increments dw6, dw7.  The coefficients do not depend
+// on the recursion, so eight steps' loads are issued ahead of their
+// arithmetic.
+constexpr int RT = 16;
+
+__global__ void __launch_bounds__(RT *RT) k_impli2_rhs(Blk b, int d, const double *__restrict__ u, long ldv,
+                                                        ImpliWs w) {
+  __shared__ double sh[5][RT][RT + 1];
+  const dim3 xb = xcd_block();
+  const int tx = threadIdx.x % RT, ty = threadIdx.x / RT;
+  const int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
+  const long np_ = b.hi(p) - b.lo(p), nl = np_ * (b.hi(q) - b.lo(q));
   int c[3];
-  c[p] = b.lo(p) + t % np_;
-  c[q] = b.lo(q) + t / np_;
-  long inc = b.inc(d), ip = b.inc(p), iq = b.inc(q), nnn = b.nnn;
-  const double *uD = w.uf + d * nnn, *uP = w.uf + p * nnn, *uQ = w.uf + q * nnn;
-  const double *cD = w.ce + d * nnn, *cP = w.ce + p * nnn, *cQ = w.ce + q * nnn;
-  const double *u6 = u + 5 * ldv, *u7 = u + 6 * ldv;
-  double al1 = 0., b61 = 0., b71 = 0.;  // values at the next face up the line
-  for (int a = b.hi(d) - 1; a >= b.lo(d); a--) {
-    c[d] = a;
-    long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+  c[0] = b.i1 + xb.x * RT + tx;
+  c[1] = b.j1 + xb.y * RT + ty;
+  c[2] = b.k1 + xb.z;
+  const bool in = c[0] < b.i2 && c[1] < b.j2;
+  double val[5] = {0., 0., 0., 0., 0.};
+  if (in) {
+    const long inc = b.inc(d), ip = b.inc(p), iq = b.inc(q), nnn = b.nnn;
+    const long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+    const double *__restrict__ uD = w.uf + d * nnn, *__restrict__ uP = w.uf + p * nnn;
+    const double *__restrict__ uQ = w.uf + q * nnn;
+    const double *__restrict__ cD = w.ce + d * nnn, *__restrict__ cP = w.ce + p * nnn;
+    const double *__restrict__ cQ = w.ce + q * nnn;
+    const double *__restrict__ w6 = w.dw6, *__restrict__ w7 = w.dw7;
     double td = -uP[m + ip] + uP[m] - uQ[m + iq] + uQ[m];
     double tpp = cP[m + ip] - uP[m + ip], tmp_ = cP[m] + uP[m];
     double tpq = cQ[m + iq] - uQ[m + iq], tmq = cQ[m] + uQ[m];
-    const double *w6 = w.dw6, *w7 = w.dw7;
-    double r6 = -u6[n] + td * w6[m] + tmp_ * w6[m - ip] + tpp * w6[m + ip] + tmq * w6[m - iq] + tpq * w6[m + iq];
-    double r7 = -u7[n] + td * w7[m] + tmp_ * w7[m - ip] + tpp * w7[m + ip] + tmq * w7[m - iq] + tpq * w7[m + iq];
-    double ai = cD[m] + uD[m];
-    double bi = w.cd[m] + uD[m + inc] - uD[m];
-    double ci = -cD[m + inc] + uD[m + inc];
-    double den = bi + ci * al1;
-    al1 = ai / den;
-    b61 = (r6 - ci * b61) / den;
-    b71 = (r7 - ci * b71) / den;
-    w.al[m] = al1;
-    w.b6[m] = b61;
-    w.b7[m] = b71;
+    val[3] = -u[5 * ldv + n] + td * w6[m] + tmp_ * w6[m - ip] + tpp * w6[m + ip] + tmq * w6[m - iq] +
+             tpq * w6[m + iq];
+    val[4] = -u[6 * ldv + n] + td * w7[m] + tmp_ * w7[m - ip] + tpp * w7[m + ip] + tmq * w7[m - iq] +
+             tpq * w7[m + iq];
+    val[0] = cD[m] + uD[m];
+    val[1] = w.cd[m] + uD[m + inc] - uD[m];
+    val[2] = -cD[m + inc] + uD[m + inc];
+  }
+  double *const out[5] = {w.la, w.lb, w.lc, w.l6, w.l7};
+  if (d != 0) {
+    if (in) {
+      const long L = (long)(c[d] - b.lo(d)) * nl + (c[p] - b.lo(p)) + (long)(c[q] - b.lo(q)) * np_;
+#pragma unroll
+      for (int e = 0; e < 5; e++) out[e][L] = val[e];
+    }
+    return;
+  }
+#pragma unroll
+  for (int e = 0; e < 5; e++) sh[e][ty][tx] = val[e];
+  __syncthreads();
+  // i lines: p = j, q = k; thread (tx, ty) writes cell (i0 + ty, j0 + tx)
+  const int i = b.i1 + xb.x * RT + ty, j = b.j1 + xb.y * RT + tx;
+  if (i < b.i2 && j < b.j2) {
+    const long L = (long)(i - b.i1) * nl + (j - b.j1) + (long)(c[2] - b.k1) * np_;
+#pragma unroll
+    for (int e = 0; e < 5; e++) out[e][L] = sh[e][tx][ty];
   }
 }
 
-// Substitution from the low end of every line (:265-275 and the like)
-__global__ void __launch_bounds__(64) k_impli2_subst(Blk b, int d, ImpliWs w) {
-  int t = blockIdx.x * 64 + threadIdx.x;
-  int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
-  int np_ = b.hi(p) - b.lo(p), nq = b.hi(q) - b.lo(q);
-  if (t >= np_ * nq) return;
+constexpr int LU = 8;  // recursion steps per batch of loads
+
+// The two recursions of one line, each software-pipelined: the loads of the
+// next LU steps are issued before the arithmetic of the current LU steps, so
+// one memory round trip covers 2 x LU steps of the serial chain.
+__global__ void __launch_bounds__(64) k_impli2_line(Blk b, int d, ImpliWs w) {
+  const int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
+  const long np_ = b.hi(p) - b.lo(p), nl = np_ * (b.hi(q) - b.lo(q));
+  const long t = blockIdx.x * 64L + threadIdx.x;
+  if (t >= nl) return;
+  const int nd = b.hi(d) - b.lo(d), nb = nd / LU;
+  const double *A = w.la + t, *B = w.lb + t, *C = w.lc + t, *R6 = w.l6 + t, *R7 = w.l7 + t;
+  double *AL = w.al + t, *B6 = w.b6 + t, *B7 = w.b7 + t;
+  // elimination from the high end: steps a = nd-1, nd-2, ...
+  double al1 = 0., b61 = 0., b71 = 0.;  // values at the next face up the line
+  double ai[LU], bi[LU], ci[LU], r6[LU], r7[LU];
+  if (nb > 0) {
+#pragma unroll
+    for (int x = 0; x < LU; x++) {
+      const long L = (long)(nd - 1 - x) * nl;
+      ai[x] = A[L]; bi[x] = B[L]; ci[x] = C[L]; r6[x] = R6[L]; r7[x] = R7[L];
+    }
+  }
+  for (int bt = 0; bt < nb; bt++) {
+    const int a = nd - 1 - bt * LU;
+    double an[LU], bn[LU], cn[LU], sn6[LU], sn7[LU];
+    if (bt + 1 < nb) {
+#pragma unroll
+      for (int x = 0; x < LU; x++) {
+        const long L = (long)(a - LU - x) * nl;
+        an[x] = A[L]; bn[x] = B[L]; cn[x] = C[L]; sn6[x] = R6[L]; sn7[x] = R7[L];
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < LU; x++) {
+      const long L = (long)(a - x) * nl;
+      double den = bi[x] + ci[x] * al1;
+      al1 = ai[x] / den;
+      b61 = (r6[x] - ci[x] * b61) / den;
+      b71 = (r7[x] - ci[x] * b71) / den;
+      AL[L] = al1; B6[L] = b61; B7[L] = b71;
+    }
+#pragma unroll
+    for (int x = 0; x < LU; x++) {
+      ai[x] = an[x]; bi[x] = bn[x]; ci[x] = cn[x]; r6[x] = sn6[x]; r7[x] = sn7[x];
+    }
+  }
+  for (int a = nd - 1 - nb * LU; a >= 0; a--) {
+    const long L = (long)a * nl;
+    double c0 = C[L];
+    double den = B[L] + c0 * al1;
+    al1 = A[L] / den;
+    b61 = (R6[L] - c0 * b61) / den;
+    b71 = (R7[L] - c0 * b71) / den;
+    AL[L] = al1; B6[L] = b61; B7[L] = b71;
+  }
+  // substitution from the low end: steps a = 0, 1, ...  (same thread wrote
+  // AL, B6, B7 above, so its own loads see those values)
   int c[3];
-  c[p] = b.lo(p) + t % np_;
-  c[q] = b.lo(q) + t / np_;
+  c[p] = b.lo(p) + (int)(t % np_);
+  c[q] = b.lo(q) + (int)(t / np_);
   c[d] = b.lo(d);
-  long inc = b.inc(d);
-  long m = b.idx(c[0], c[1], c[2]) - b.npc;
+  const long inc = b.inc(d);
+  const long m0 = b.idx(c[0], c[1], c[2]) - b.npc;
+  double *dw6 = w.dw6, *dw7 = w.dw7;
   double w6 = 0., w7 = 0.;  // increments of the ghost cell below the line
-  for (int a = b.lo(d); a < b.hi(d); a++, m += inc) {
-    w6 = w.al[m] * w6 + w.b6[m];
-    w7 = w.al[m] * w7 + w.b7[m];
-    w.dw6[m] = w6;
-    w.dw7[m] = w7;
+  double xa[LU], x6[LU], x7[LU];
+  if (nb > 0) {
+#pragma unroll
+    for (int x = 0; x < LU; x++) {
+      const long L = (long)x * nl;
+      xa[x] = AL[L]; x6[x] = B6[L]; x7[x] = B7[L];
+    }
+  }
+  for (int bt = 0; bt < nb; bt++) {
+    const int a = bt * LU;
+    double ya[LU], y6[LU], y7[LU];
+    if (bt + 1 < nb) {
+#pragma unroll
+      for (int x = 0; x < LU; x++) {
+        const long L = (long)(a + LU + x) * nl;
+        ya[x] = AL[L]; y6[x] = B6[L]; y7[x] = B7[L];
+      }
+    }
+#pragma unroll
+    for (int x = 0; x < LU; x++) {
+      w6 = xa[x] * w6 + x6[x];
+      w7 = xa[x] * w7 + x7[x];
+      const long m = m0 + (long)(a + x) * inc;
+      dw6[m] = w6;
+      dw7[m] = w7;
+    }
+#pragma unroll
+    for (int x = 0; x < LU; x++) {
+      xa[x] = ya[x]; x6[x] = y6[x]; x7[x] = y7[x];
+    }
+  }
+  for (int a = nb * LU; a < nd; a++) {
+    const long L = (long)a * nl;
+    w6 = AL[L] * w6 + B6[L];
+    w7 = AL[L] * w7 + B7[L];
+    dw6[m0 + (long)a * inc] = w6;
+    dw7[m0 + (long)a * inc] = w7;
   }
 }
 
@@ -159,13 +279,13 @@ extern "C" int codens_impli2_eqt_3d(const codens_block *cb, const double *u, int
   Blk b = make_blk(cb);
   hipStream_t st = (hipStream_t)stream;
   long nnn = b.nnn;
-  double *ws = (double *)codens_workspace(sizeof(double) * 13 * nnn, 3);
+  double *ws = (double *)codens_workspace(sizeof(double) * 18 * nnn, 3);
   if (!ws) {
     codens_set_error("codens_impli2_eqt_3d: workspace allocation failed");
     return CODENS_EHIP;
   }
   ImpliWs w{ws, ws + nnn, ws + 2 * nnn, ws + 5 * nnn, ws + 8 * nnn, ws + 9 * nnn, ws + 10 * nnn, ws + 11 * nnn,
-            ws + 12 * nnn};
+            ws + 12 * nnn, ws + 13 * nnn, ws + 14 * nnn, ws + 15 * nnn, ws + 16 * nnn, ws + 17 * nnn};
   CodensTimed _tm("impli2_eqt_3d", st);
   // rv, the face arrays and the increments start at zero (ghost entries stay)
   hipLaunchKernelGGL(k_zero_n, dim3((unsigned)((13 * nnn + 255) / 256)), dim3(256), 0, st, ws, 13 * nnn);
@@ -174,12 +294,13 @@ extern "C" int codens_impli2_eqt_3d(const codens_block *cb, const double *u, int
   dim3 gf((b.i2 - b.i1 + 1 + TX - 1) / TX, (b.j2 - b.j1 + 1 + TY - 1) / TY, b.k2 - b.k1 + 1);
   hipLaunchKernelGGL(k_impli2_faces, gf, t, 0, st, b, v, (long)ldv, sn, vol, w);
   hipLaunchKernelGGL(k_impli2_diag, g, t, 0, st, b, w);
+  dim3 gr((b.i2 - b.i1 + RT - 1) / RT, (b.j2 - b.j1 + RT - 1) / RT, b.k2 - b.k1);
   for (int d = 0; d < 3; d++) {
     int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
     int nl = (b.hi(p) - b.lo(p)) * (b.hi(q) - b.lo(q));
     for (int sweep = 0; sweep < 2; sweep++) {
-      hipLaunchKernelGGL(k_impli2_elim, dim3((nl + 63) / 64), dim3(64), 0, st, b, d, u, (long)ldv, w);
-      hipLaunchKernelGGL(k_impli2_subst, dim3((nl + 63) / 64), dim3(64), 0, st, b, d, w);
+      hipLaunchKernelGGL(k_impli2_rhs, gr, dim3(RT * RT), 0, st, b, d, u, (long)ldv, w);
+      hipLaunchKernelGGL(k_impli2_line, dim3((nl + 63) / 64), dim3(64), 0, st, b, d, w);
     }
   }
   hipLaunchKernelGGL(k_impli2_update, g, t, 0, st, b, v, (long)ldv, w);

@@ -60,6 +60,11 @@ SIGNATURES = {
     "codens_sch_expli": [P(CodensBlock), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
     "codens_implimf_3d": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int,
                           c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp, c_vp],
+    "codens_adj_residual_jvp": [P(CodensBlock), P(CodensPhys), P(CodensScheme), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp,
+                                c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "codens_adj_residual_vjp": [P(CodensBlock), P(CodensPhys), P(CodensScheme), c_vp, c_i64, c_vp, c_vp, c_vp, c_i64,
+                                c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "codens_adj_metric_vjp": [P(CodensBlock), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "codens_implimf_3d_f32": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int,
                           c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp, c_vp],
     "codens_bc_coincident": [c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],

@@ -329,3 +329,32 @@ def timing_read(name, reset=True):
     cnt = ctypes.c_int64(0)
     _chk(lib.codens_timing_read(name.encode(), ctypes.byref(ms), ctypes.byref(cnt), 1 if reset else 0))
     return ms.value, cnt.value
+
+
+# ---- differentiation of the block residual (laminar NS, unlimited MUSCL)
+def _bc(bcs):
+    """(nwall, wg, wi, nfar, fg, fi, nrm) from the (wall ghost, wall inner,
+    far ghost, far inner, far normals) device tensors."""
+    wg, wi, fg, fi, nrm = bcs
+    return (wg.numel(), _p(wg) if wg.numel() else None, _p(wi) if wi.numel() else None, fg.numel(),
+            _p(fg) if fg.numel() else None, _p(fi) if fi.numel() else None, _p(nrm) if fg.numel() else None)
+
+
+def adj_residual_jvp(b, ph, sc, v, vdot, sn_blk, sndot, vol, voldot, winf, bcs, u, udot, stream=None):
+    nw, wg, wi, nf, fg, fi, nrm = _bc(bcs)
+    _chk(lib.codens_adj_residual_jvp(ctypes.byref(b), ctypes.byref(ph), ctypes.byref(sc), _p(v), _p(vdot), ld(v),
+                                     _p(sn_blk), _p(sndot), _p(vol), _p(voldot), _p(winf), nw, wg, wi, nf, fg, fi,
+                                     nrm, _p(u), _p(udot), _s(stream)))
+
+
+def adj_residual_vjp(b, ph, sc, v, sn_blk, vol, winf, bcs, ubar, vbar, snbar, volbar, stream=None):
+    nw, wg, wi, nf, fg, fi, nrm = _bc(bcs)
+    _chk(lib.codens_adj_residual_vjp(ctypes.byref(b), ctypes.byref(ph), ctypes.byref(sc), _p(v), ld(v),
+                                     _p(sn_blk), _p(vol), _p(winf), nw, wg, wi, nf, fg, fi, nrm, _p(ubar), _p(vbar),
+                                     _p(snbar), _p(volbar), _s(stream)))
+
+
+def adj_metric_vjp(b, x, y, z, sg, snbar, volbar, xbar, ybar, zbar, stream=None):
+    sgc = (ctypes.c_double * 3)(*[float(t) for t in sg])
+    _chk(lib.codens_adj_metric_vjp(ctypes.byref(b), _p(x), _p(y), _p(z), sgc, _p(snbar), _p(volbar), _p(xbar),
+                                   _p(ybar), _p(zbar), _s(stream)))

@@ -367,3 +367,76 @@ def multiblock_sa(nblocks=48, lo=16, hi=192, total=(38_000_000, 42_000_000), see
     c.links = links
     c.origin = origin
     return c
+
+
+def adjoint_wall(ci=128, cj=64, ck=32, seed=20261205, mach=0.5, reynz=1e5, alpha=10.0, beta=5.0, amp=0.01,
+                 bump=0.01, tanh_beta=2.5, disp=0.05, eta=1.0, nwall_nodes=4096) -> Case:
+    """Viscous wall block for the discrete adjoint: the curvilinear mesh of
+    wall_ns (seeded 5% node displacement, tanh stretching toward the no-slip
+    adiabatic wall at j=1, far field elsewhere) with a smooth wall bump
+    y += bump sin(2 pi x / Lx) cos(2 pi z / Lz) (1 - eta_j)^2, so that the
+    wall faces have an x component; laminar Navier-Stokes, Roe + unlimited
+    MUSCL (ilim = 0); free stream at Mach `mach` inclined by alpha (x-y) and
+    beta (x-z) degrees, so that no far-field face has a zero normal velocity
+    (the switch of clnrd.f90); initial state = free stream x (1 + amp x 4
+    seeded sinusoidal modes) on every variable.  Local steps at CFL `eta`.
+
+    Extra attributes: nodes (x, y, z storage arrays, ghost nodes
+    extrapolated), wall_nodes (flat indices of `nwall_nodes` seeded wall
+    nodes of the (ci+1) x (ck+1) on j=1)."""
+    lay = L.Layout([(ci, cj, ck)])
+    b = lay.blocks[0]
+    Lx, Ly, Lz = 2.0, 1.0, 0.5
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xi = np.arange(ci + 1) / ci * Lx
+    eta_ = np.arange(cj + 1) / cj
+    yj = Ly * (1.0 - np.tanh(tanh_beta * (1.0 - eta_)) / np.tanh(tanh_beta))
+    zk = np.arange(ck + 1) / ck * Lz
+    Z, Y, X = np.meshgrid(zk, yj, xi, indexing="ij")
+    E = np.meshgrid(zk, eta_, xi, indexing="ij")[1]
+    hx, hz = Lx / ci, Lz / ck
+    hy = np.gradient(yj)[None, :, None] * np.ones_like(Y)
+    kv = rng.integers(1, 4, size=(3, 3))
+    ph = rng.uniform(0, 2 * np.pi, size=3)
+    arg = lambda m: 2 * np.pi * (kv[m, 0] * X / Lx + kv[m, 1] * Y / Ly + kv[m, 2] * Z / Lz) + ph[m]
+    Xd = X + disp * hx * np.sin(arg(0))
+    Yd = Y + disp * hy * np.sin(arg(1)) * (Y > 0)
+    Zd = Z + disp * hz * np.sin(arg(2))
+    Yd = Yd + bump * np.sin(2 * np.pi * X / Lx) * np.cos(2 * np.pi * Z / Lz) * (1.0 - E) ** 2
+    x = np.zeros(lay.ip11); y = np.zeros(lay.ip11); z = np.zeros(lay.ip11)
+    sl = (slice(b.kk1 - b.kd1, b.kk2 - b.kd1 + 1), slice(b.jj1 - b.jd1, b.jj2 - b.jd1 + 1),
+          slice(b.ii1 - b.id1, b.ii2 - b.id1 + 1))
+    b.view(x)[sl] = Xd; b.view(y)[sl] = Yd; b.view(z)[sl] = Zd
+    sn, vol = L.metrics(lay, x, y, z)  # also extrapolates the ghost nodes
+    n = lay.ip11
+    a, bb = np.radians(alpha), np.radians(beta)
+    uu, vv, ww = mach * np.cos(a) * np.cos(bb), mach * np.sin(a) * np.cos(bb), mach * np.sin(bb)
+    winf = conservative(np.array(1.0), np.array(uu), np.array(vv), np.array(ww), np.array(1 / GAM))
+    xc, yc, zc = cell_centres(b, x, y, z)
+    kvs = rng.integers(1, 4, size=(4, 3))
+    phs = rng.uniform(0, 2 * np.pi, size=(4, 5))
+    v0 = np.repeat(winf[:, None], n, axis=1).astype(np.float64)
+    for e in range(5):
+        s = np.zeros_like(xc)
+        for m in range(4):
+            s += np.sin(2 * np.pi * (kvs[m, 0] * xc / Lx + kvs[m, 1] * yc / Ly + kvs[m, 2] * zc / Lz) + phs[m, e])
+        vb = b.view(v0[e])
+        vb[:-1, :-1, :-1] = winf[e] * (1.0 + amp * s)
+    wall_parts, far_parts = [], []
+    for f in L.FACES:
+        g, i, fi = L.face_cells(b, f)
+        if f == "j1":
+            wall_parts.append(FaceSet(g, i))
+        else:
+            nx, ny, nz = L.inward_normals(b, sn, f, fi)
+            far_parts.append(FaceSet(g, i, None, np.stack([nx, ny, nz])))
+    c = Case("adjoint_wall", lay, sn, vol, v0, True,
+             dict(gam=GAM, pinfl=0.0, pr=0.72, prt=1.0, reynz=reynz, tnz=288.15, iflu=1),
+             dict(muscl=1, ilim=0, xk=1.0 / 3.0), eta, 1, 0.0, 1, 10, winf,
+             _empty_faces(), _cat(wall_parts), _cat(far_parts))
+    c.nodes = (x, y, z)
+    ii, kk = np.meshgrid(np.arange(b.ii1, b.ii2 + 1), np.arange(b.kk1, b.kk2 + 1), indexing="ij")
+    allw = b.idx(ii.ravel(), b.jj1, kk.ravel()).astype(np.int64)
+    pick = np.sort(rng.choice(len(allw), size=min(nwall_nodes, len(allw)), replace=False))
+    c.wall_nodes = allw[pick]
+    return c

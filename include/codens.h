@@ -407,6 +407,46 @@ int codens_met_komut(const codens_block *b, double betae, int32_t komsst, const 
                      double *v, int64_t ldv, const double *mu, double *mut, const double *dist, double *g9,
                      void *stream);
 
+/* ---- Differentiation of the block residual (no counterpart in the
+ * reference, which has no adjoint).  The residual is that of
+ * codens_sch_roe (sch_roe.f90:4) for laminar Navier-Stokes with unlimited
+ * MUSCL (scheme ilim = 0; other values are rejected) after codens_zpres,
+ * codens_zvismo, the wall ghosts (codens_bc_wall), the far-field ghosts
+ * (codens_bc_farfield, normals and state winf held fixed), codens_zvisqc and
+ * the ghost stresses (codens_bc_tq_copy), as a function of the real cells of
+ * v (5 x ldv), the block metrics sn (9 x nnn) and vol (ldv).  Only the real
+ * cells of u are defined. */
+
+/* Tangent-linear: u and its directional derivative udot for the direction
+ * (vdot, sndot, voldot). */
+int codens_adj_residual_jvp(const codens_block *b, const codens_phys *ph, const codens_scheme *sc,
+                            const double *v, const double *vdot, int64_t ldv, const double *sn, const double *sndot,
+                            const double *vol, const double *voldot, const double *winf, int64_t nwall,
+                            const int64_t *wall_ghost, const int64_t *wall_inner, int64_t nfar,
+                            const int64_t *far_ghost, const int64_t *far_inner, const double *far_nrm, double *u,
+                            double *udot, void *stream);
+
+/* Adjoint: for the adjoint ubar of u (real cells), vbar = (du/dv)^T ubar on
+ * the real cells (5 x ldv, the block's entries overwritten, ghosts zero),
+ * snbar = (du/dsn)^T ubar (9 x nnn) and volbar = (du/dvol)^T ubar (ldv, the
+ * block's entries). */
+int codens_adj_residual_vjp(const codens_block *b, const codens_phys *ph, const codens_scheme *sc,
+                            const double *v, int64_t ldv, const double *sn, const double *vol, const double *winf,
+                            int64_t nwall, const int64_t *wall_ghost, const int64_t *wall_inner, int64_t nfar,
+                            const int64_t *far_ghost, const int64_t *far_inner, const double *far_nrm,
+                            const double *ubar, double *vbar, double *snbar, double *volbar, void *stream);
+
+/* Adjoint of the mesh metrics (norm.f90 face vectors sg_d (d1 x d2) of the
+ * half-diagonals, svol.f90 volumes) with respect to the node coordinates
+ * x, y, z (storage arrays, nodes at the cells' flat indices): xbar, ybar,
+ * zbar (block-local, nnn) from snbar (9 x nnn) and volbar (the block's
+ * entries of an ldv array, indexed from npc).  sg: the orientation factor
+ * +-0.5 of each direction.  Ghost nodes are treated as independent nodes;
+ * folding them onto the real nodes they copy is left to the caller. */
+int codens_adj_metric_vjp(const codens_block *b, const double *x, const double *y, const double *z,
+                          const double *sg, const double *snbar, const double *volbar, double *xbar, double *ybar,
+                          double *zbar, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

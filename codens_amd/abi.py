@@ -358,3 +358,17 @@ def adj_metric_vjp(b, x, y, z, sg, snbar, volbar, xbar, ybar, zbar, stream=None)
     sgc = (ctypes.c_double * 3)(*[float(t) for t in sg])
     _chk(lib.codens_adj_metric_vjp(ctypes.byref(b), _p(x), _p(y), _p(z), sgc, _p(snbar), _p(volbar), _p(xbar),
                                    _p(ybar), _p(zbar), _s(stream)))
+
+
+def sweep_tile(mode):
+    """Tile shape of the Jacobi sub-iteration: -1 timed choice per block shape
+    (default), 0 = 64 x 8, 1 = 32 x 16.  Returns the previous mode."""
+    prev = ctypes.c_int32(0)
+    _chk(lib.codens_sweep_tile(int(mode), ctypes.byref(prev)))
+    return prev.value
+
+
+def sweep_tile_of(b, eu=False, fp32=False):
+    t = ctypes.c_int32(0)
+    _chk(lib.codens_sweep_tile_of(ctypes.byref(b), int(eu), int(fp32), ctypes.byref(t)))
+    return t.value

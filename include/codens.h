@@ -407,6 +407,18 @@ int codens_met_komut(const codens_block *b, double betae, int32_t komsst, const 
                      double *v, int64_t ldv, const double *mu, double *mut, const double *dist, double *g9,
                      void *stream);
 
+/* Tile shape of the tiled Jacobi sub-iteration of codens_implimf_3d,
+ * codens_implimf_eu and codens_implimf_3d_f32 (no counterpart in the
+ * reference): -1 (default) times the 64 x 8 and 32 x 16 shapes on the first
+ * call for each block shape and keeps the faster, 0 forces 64 x 8, 1 forces
+ * 32 x 16.  Both give identical results.  *previous (may be NULL) receives
+ * the mode in force before the call. */
+int codens_sweep_tile(int32_t mode, int32_t *previous);
+
+/* The tile shape used for a block (eu: implimf_eu, fp32: the FP32 form):
+ * 0 or 1, or -1 if not chosen yet. */
+int codens_sweep_tile_of(const codens_block *b, int32_t eu, int32_t fp32, int32_t *tile);
+
 /* ---- Differentiation of the block residual (no counterpart in the
  * reference, which has no adjoint).  The residual is that of
  * codens_sch_roe (sch_roe.f90:4) for laminar Navier-Stokes with unlimited

@@ -293,3 +293,44 @@ def test_bad_extents_rejected(ctx):
     x = T(np.zeros((5, lay.ip11)))
     with pytest.raises(CodensError):
         abi.zpres(cb, abi.phys(), x, x[0], x[1], x[2])
+
+
+@pytest.mark.parametrize("form", ["fp64", "eu", "fp32"])
+def test_sweep_tile_shapes_identical(form):
+    """The two tile shapes of the Jacobi sub-iteration (64 x 8 and 32 x 16)
+    and the timed choice give bit-identical increments, on a block whose i
+    and j extents are multiples of neither tile (70 x 37 x 9 cells)."""
+    from codens_amd import abi
+    lay, sn, vol = random_mesh([(70, 37, 9)], seed=23)
+    b = lay.blocks[0]
+    rng = np.random.default_rng(31)
+    n = lay.ip11
+    v = random_state(n, rng)
+    ps, t, c = all_thermo(v)
+    mu = 1e-3 * (1 + 0.1 * rng.random(n)); mut = 1e-4 * rng.random(n)
+    u = 1e-3 * rng.standard_normal((5, n)); ff = 1e-4 * rng.standard_normal((5, n))
+    dt = 0.01 * (1 + rng.random(n))
+    ph = abi.phys()
+    snb = T(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+    out = {}
+    prev = abi.sweep_tile(0)
+    try:
+        for mode in (0, 1, -1):
+            abi.sweep_tile(mode)
+            gv, gd = T(v), T(np.zeros((5, n)))
+            if form == "eu":
+                abi.implimf_eu(abi.block(b), ph, T(u), T(dt), gv, gd, T(ff), 6, snb, T(vol), T(ps), T(c))
+            elif form == "fp32":
+                abi.implimf_3d_f32(abi.block(b), ph, T(u), T(dt), gv, gd, T(ff), T(mu), T(mut), 6, snb, T(vol),
+                                   T(ps), T(c), ityprk=1)
+            else:
+                abi.implimf_3d(abi.block(b), ph, T(u), T(dt), gv, gd, T(ff), T(mu), T(mut), 6, snb, T(vol), T(ps),
+                               T(c), ityprk=1)
+            out[mode] = (H(gv), H(gd))
+        chosen = abi.sweep_tile_of(abi.block(b), eu=form == "eu", fp32=form == "fp32")
+    finally:
+        abi.sweep_tile(prev)
+    assert chosen in (0, 1)
+    for mode in (1, -1):
+        assert np.array_equal(out[mode][0], out[0][0]) and np.array_equal(out[mode][1], out[0][1]), mode
+    assert np.abs(out[0][1]).max() > 0

@@ -482,6 +482,12 @@ static void bark_of(const orc_block *b, const double *sn, const double *vol, con
   free(t); free(g);
 }
 
+/* met_bark.f90:4 as a routine of its own (the k-epsilon sources call it) */
+void orc_met_bark(const orc_block *b, const double *sn, const double *vol, const double *v, long ldv,
+                  const double *mu, double *bark) {
+  bark_of(b, sn, vol, v, ldv, mu, bark);
+}
+
 /* met_smkl.f90:4 -- Smith k-l sources: ts6 = P - csk2 sqrt(k) rho k / l +
  * bark, ts7 = csl1 rho sqrt(k) (1 - l^2/(kappa d)^2) + rho l div u - mut
  * |grad l|^2 l/(kappa d)^2/(sigmal l) + (2/sigmal) mut grad l.grad k / k,

@@ -598,4 +598,50 @@ contains
     call met_klmut(l, v, mu, mut, dist)
   end subroutine ref_met_klmut
 
+  ! k-epsilon constants and switches (atlecdon.f90:112-140): cke1, cke2, cmu;
+  ! lparoi; kesst; equatt = '2JL' + the variant letter (code 0 ' ', 1 'S',
+  ! 2 'R', 3 'C', 4 'L'), or '2LS' (code 5)
+  subroutine ref_set_ke(kc, lpar, kes, code) bind(C, name="ref_set_ke")
+    use chainecarac
+    real(c_double) :: kc(3)
+    integer(c_int), value :: lpar, kes, code
+    character(len=1), parameter :: letters(0:4) = (/ ' ', 'S', 'R', 'C', 'L' /)
+    cke1 = kc(1); cke2 = kc(2); cmu = kc(3)
+    lparoi = lpar
+    kesst = kes
+    if (code == 5) then
+       equatt = '2LS   '
+    else
+       equatt = '2JL' // letters(code) // '  '
+    end if
+  end subroutine ref_set_ke
+
+  ! met_smke.f90:4 (met_exgr, met_bark, met_bare, then met_smkes / met_smker
+  ! / met_smkec / met_smkesas by equatt(4:4))
+  subroutine ref_met_smke(l, isch, sn, vol, v, mu, mut, tprod, cfke, g, tt, gx, gy, gz, bark, bare, ts6, ts7, tn, &
+       cson, cmu) bind(C, name="ref_met_smke")
+    use mod_met_smke
+    use chainecarac
+    integer(c_int), value :: l, isch
+    real(c_double) :: sn(*), vol(*), v(*), mu(*), mut(*), tprod(*), cfke(*), g(ip00, 9), tt(*), gx(*), gy(*)
+    real(c_double) :: gz(*), bark(*), bare(*), ts6(*), ts7(*), tn(ip00, 3), cson(*), cmu(*)
+    integer :: keep
+    equat = heq
+    keep = ischema
+    ischema = isch
+    call met_smke(l, sn, vol, v, mu, mut, tprod, cfke, g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), &
+         g(1,8), g(1,9), tt, gx, gy, gz, bark, bare, ts6, ts7, tn(1,1), tn(1,2), tn(1,3), cson, cmu, cmu, cmu, &
+         cmu, cmu, cmu)
+    ischema = keep
+  end subroutine ref_met_smke
+
+  ! met_kemut.f90:4
+  subroutine ref_met_kemut(l, sn, vol, v, mu, mut, dist, g, tt, cmu) bind(C, name="ref_met_kemut")
+    use mod_met_kemut
+    integer(c_int), value :: l
+    real(c_double) :: sn(*), vol(*), v(*), mu(*), mut(*), dist(*), g(ip00, 9), tt(*), cmu(*)
+    call met_kemut(l, sn, vol, tt, g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), g(1,8), g(1,9), &
+         dist, v, mu, mut, cmu, cmu, cmu, cmu, cmu, cmu)
+  end subroutine ref_met_kemut
+
 end module codens_ref_harness

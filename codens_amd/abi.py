@@ -372,3 +372,18 @@ def sweep_tile_of(b, eu=False, fp32=False):
     t = ctypes.c_int32(0)
     _chk(lib.codens_sweep_tile_of(ctypes.byref(b), int(eu), int(fp32), ctypes.byref(t)))
     return t.value
+
+
+def met_smke(b, kc, variant, lparoi, ischema, sn_blk, vol, v, mu, mut, tprod, g9, cson, bark, bare, cfke, ts6, ts7,
+             stream=None):
+    """met_smke.f90:4 (k-epsilon sources; variant 0 smkes, 1 smker, 2 smkec, 3 smkesas)."""
+    k = (ctypes.c_double * 3)(*[float(x) for x in kc])
+    _chk(lib.codens_met_smke(ctypes.byref(b), k, int(variant), int(lparoi), int(ischema), _p(sn_blk), _p(vol), _p(v),
+                             ld(v), _p(mu), _p(mut), _p(tprod), _p(g9), _p(cson), _p(bark), _p(bare), _p(cfke), _p(ts6),
+                             _p(ts7), _s(stream)))
+
+
+def met_kemut(b, cmu, form, lparoi, sn_blk, vol, v, mu, mut, dist, g9, stream=None):
+    """met_kemut.f90:4 (form 0 Jones-Launder, 1 with SST limiter, 2 Launder-Sharma)."""
+    _chk(lib.codens_met_kemut(ctypes.byref(b), float(cmu), int(form), int(lparoi), _p(sn_blk), _p(vol), _p(v), ld(v),
+                              _p(mu), _p(mut), _p(dist), _p(g9), _s(stream)))

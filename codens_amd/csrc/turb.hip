@@ -799,3 +799,221 @@ extern "C" int codens_met_klmut(const codens_block *cb, const double *kc, const 
   hipLaunchKernelGGL(k_met_klmut, cell_grid(b), dim3(TX, TY), 0, st, b, kc[0], kc[3], v, (long)ldv, mu, mut, dist);
   return codens_check_launch("codens_met_klmut");
 }
+
+// ---- k-epsilon family (met_num.f90 ksecmb <= 1): met_smke with met_exgr,
+// met_bark, met_bare and the variants; met_kemut ----
+
+// met_exgr.f90:47-87: ghost layers of the nine block-local velocity
+// gradients copied from the adjacent layer (ex1 = 1, ex2 = 0), in three
+// passes: pass 0 the k faces over the real i, j; pass 1 the i faces over
+// k1-1..k2 and the real j; pass 2 the j faces over k1-1..k2 and i1-1..i2.
+// One thread per (face cell, side).
+__global__ void __launch_bounds__(256) k_exgr(Blk b, int pass, double *__restrict__ g9) {
+  long t = blockIdx.x * 256L + threadIdx.x;
+  int nx, ny;
+  if (pass == 0) { nx = b.i2 - b.i1; ny = b.j2 - b.j1; }
+  else if (pass == 1) { nx = b.j2 - b.j1; ny = b.k2 - b.k1 + 2; }
+  else { nx = b.i2 - b.i1 + 2; ny = b.k2 - b.k1 + 2; }
+  if (t >= 2L * nx * ny) return;
+  int side = (int)(t / ((long)nx * ny));
+  long r = t % ((long)nx * ny);
+  int p = (int)(r % nx), q = (int)(r / nx);
+  int i, j, k;
+  long step;
+  if (pass == 0) { i = b.i1 + p; j = b.j1 + q; k = side == 0 ? b.k1 - 1 : b.k2; step = b.nijd; }
+  else if (pass == 1) { j = b.j1 + p; k = b.k1 - 1 + q; i = side == 0 ? b.i1 - 1 : b.i2; step = 1; }
+  else { i = b.i1 - 1 + p; k = b.k1 - 1 + q; j = side == 0 ? b.j1 - 1 : b.j2; step = b.nid; }
+  long s1 = side == 0 ? step : -step;
+  long m = b.idx(i, j, k) - b.npc;
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    double *d = g9 + (long)c * b.nnn;
+    d[m] = 1. * d[m + s1] + 0. * d[m + 2 * s1];
+  }
+}
+
+// met_bark.f90:88-116: bark = -2 mu |grad sqrt(k/rho)|^2 (block-local)
+__global__ void __launch_bounds__(TX *TY) k_met_bark(Blk b, const double *__restrict__ sn,
+                                                     const double *__restrict__ vol, const double *__restrict__ mu,
+                                                     const double *__restrict__ tk, double *__restrict__ bark) {
+  REAL_CELL(b)
+  double gt[3];
+  grad_scalar(b, sn, vol, tk - b.npc, nullptr, n, c, gt);
+  bark[m] = -2. * mu[n] * (gt[0] * gt[0] + gt[1] * gt[1] + gt[2] * gt[2]);
+}
+
+// met_bare.f90:57-312: bare = 2 mut mu / rho x sum over the nine velocity
+// gradients (dvxx, dvxy, ..., dvzz) of |grad(dv_ab)|^2 (block-local)
+__global__ void __launch_bounds__(TX *TY) k_met_bare(Blk b, const double *__restrict__ sn,
+                                                     const double *__restrict__ vol, const double *__restrict__ v,
+                                                     const double *__restrict__ mu, const double *__restrict__ mut,
+                                                     const double *__restrict__ g9, double *__restrict__ bare) {
+  REAL_CELL(b)
+  double acc = 0.;
+  for (int q = 0; q < 9; q++) {
+    double g[3];
+    grad_scalar(b, sn, vol, g9 + (long)q * b.nnn - b.npc, nullptr, n, c, g);
+    acc = acc + g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+  }
+  bare[m] = 2. * mut[n] * mu[n] * acc / v[n];
+}
+
+__device__ inline double ke_cfke(double c1f1, double c2f2, double tp, double s6, double s7, double bark) {
+  double d = -bark;
+  double x = c1f1 * tp - c2f2 * s7;
+  double g = fmax(0., d * (d - 2. * (x - c2f2 * s7)));
+  double y = x - c2f2 * s7;
+  double rdelta = sqrt(y * y + 4 * x * s7 + g);
+  double bb = x - c2f2 * s7 - d;
+  double xl1 = 0.5 * (bb + rdelta) / s6, xl2 = 0.5 * (bb - rdelta) / s6;
+  return fmax(fabs(xl1), fabs(xl2));
+}
+
+// the variants of met_smke: 0 met_smkes.f90:62-101 (Jones-Launder), 1
+// met_smker.f90:66-84, 2 met_smkec.f90:69-110 (compressible dissipation), 3
+// met_smkesas.f90:126-165 (scale-adaptive; lap: the Laplacians of u and v)
+__global__ void __launch_bounds__(TX *TY) k_met_smke(Blk b, double cke1, double cke2, int variant, int lparoi,
+                                                     const double *__restrict__ v, long ldv,
+                                                     const double *__restrict__ mu, const double *__restrict__ tprod,
+                                                     const double *__restrict__ g9, const double *__restrict__ cson,
+                                                     const double *__restrict__ lap, double *__restrict__ bark,
+                                                     double *__restrict__ bare, double *__restrict__ cfke,
+                                                     double *__restrict__ ts6, double *__restrict__ ts7) {
+  REAL_CELL(b)
+  const long N = b.nnn;
+  double s1 = v[n], s6 = v[5 * ldv + n], s7 = v[6 * ldv + n], tp = tprod[m];
+  double esk = s7 / s6;
+  double roe2sk = (s7 * s7) / s6;
+  double f1 = 1.;
+  if (variant == 1) {
+    ts6[n] = tp - s7;
+    ts7[n] = cke1 * tp * esk - cke2 * roe2sk;
+    cfke[n] = ke_cfke(cke1, cke2, tp, s6, s7, 0.);
+    return;
+  }
+  double retur = (s6 * s6) / (s7 * mu[n]);
+  double arg = -(retur * retur);
+  double f2 = (1. - 0.3 * exp(arg));
+  double epsc = 0., pid = 0.;
+  if (variant == 2) {
+    double xmt = sqrt(2. * s6 / s1) / cson[n];
+    epsc = 0.1 * s7 * (xmt * xmt);
+  }
+  double bk = bark[m], be = bare[m];
+  if (lparoi == 1) {
+    bk = 0.; be = 0.; f2 = 1.;
+    bark[m] = 0.;
+    bare[m] = 0.;
+  }
+  ts6[n] = variant == 2 ? tp + pid - s7 - epsc + bk : tp - s7 + bk;
+  if (variant == 3) {
+    double xx = g9[m], xy = g9[N + m], xz = g9[2 * N + m], yx = g9[3 * N + m], yy = g9[4 * N + m];
+    double yz = g9[5 * N + m], zx = g9[6 * N + m], zy = g9[7 * N + m], zz = g9[8 * N + m];
+    const double qsi2 = 1.47, xkappa = 0.41, cc43 = 4. / 3.;
+    double a = zy + yz, cc = xz + zx, e = yx + xy;
+    double ss = sqrt(cc43 * (xx * xx + yy * yy + zz * zz) + a * a + cc * cc + e * e);
+    double l1 = lap[m], l2 = lap[N + m];
+    double nlaplu = sqrt(l1 * l1 + l2 * l2);
+    double xlvk = xkappa * ss / nlaplu;
+    double xlvk2 = xlvk * xlvk;
+    double xl = pow(s6 / s1, 0.5) / (esk);
+    double xl2 = xl * xl;
+    double psas = xl2 / xlvk2;
+    ts7[n] = cke1 * f1 * tp * esk + qsi2 * psas * tp * esk - cke2 * f2 * roe2sk + be;
+  } else {
+    ts7[n] = cke1 * f1 * tp * esk - cke2 * f2 * roe2sk + be;
+  }
+  cfke[n] = ke_cfke(cke1 * f1, cke2 * f2, tp, s6, s7, bk);
+}
+
+extern "C" int codens_met_smke(const codens_block *cb, const double *kc, int32_t variant, int32_t lparoi,
+                               int32_t ischema, const double *sn, const double *vol, const double *v, int64_t ldv,
+                               const double *mu, const double *mut, const double *tprod, double *g9,
+                               const double *cson, double *bark, double *bare, double *cfke, double *ts6,
+                               double *ts7, void *stream) {
+  CHECK_BLOCK("codens_met_smke")
+  if (!kc || !sn || !vol || !v || !mu || !mut || !tprod || !g9 || !bark || !bare || !cfke || !ts6 || !ts7 ||
+      (variant == 2 && !cson) || variant < 0 || variant > 3) {
+    codens_set_error("codens_met_smke: null or invalid argument");
+    return CODENS_EINVAL;
+  }
+  double *ws = (double *)codens_workspace(sizeof(double) * 3 * b.nnn, 7);
+  if (!ws) {
+    codens_set_error("codens_met_smke: workspace allocation failed");
+    return CODENS_EHIP;
+  }
+  const long N = b.nnn;
+  double *tk = ws, *lap = ws + N;
+  long nf[3] = {2L * (b.i2 - b.i1) * (b.j2 - b.j1), 2L * (b.j2 - b.j1) * (b.k2 - b.k1 + 2),
+                2L * (b.i2 - b.i1 + 2) * (b.k2 - b.k1 + 2)};
+  for (int pass = 0; pass < 3; pass++)
+    hipLaunchKernelGGL(k_exgr, dim3((unsigned)((nf[pass] + 255) / 256)), dim3(256), 0, st, b, pass, g9);
+  hipLaunchKernelGGL(k_sqrt_k, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, (long)b.npc, N, v, (long)ldv, tk);
+  hipLaunchKernelGGL(k_met_bark, cell_grid(b), dim3(TX, TY), 0, st, b, sn, vol, mu, (const double *)tk, bark);
+  hipLaunchKernelGGL(k_met_bare, cell_grid(b), dim3(TX, TY), 0, st, b, sn, vol, v, mu, mut, (const double *)g9,
+                     bare);
+  if (variant == 3) {
+    hipLaunchKernelGGL(k_laplacian, cell_grid(b), dim3(TX, TY), 0, st, b, (int)ischema, sn, vol, g9, g9 + N,
+                       g9 + 2 * N, lap);
+    hipLaunchKernelGGL(k_laplacian, cell_grid(b), dim3(TX, TY), 0, st, b, (int)ischema, sn, vol, g9 + 3 * N,
+                       g9 + 4 * N, g9 + 5 * N, lap + N);
+  }
+  hipLaunchKernelGGL(k_met_smke, cell_grid(b), dim3(TX, TY), 0, st, b, kc[0], kc[1], (int)variant, (int)lparoi, v,
+                     (long)ldv, mu, tprod, (const double *)g9, cson, (const double *)lap, bark, bare, cfke, ts6, ts7);
+  return codens_check_launch("codens_met_smke");
+}
+
+// met_kemut.f90:44-130: form 0 Jones-Launder, 1 with the SST limiter (g9
+// from teq_gradv), 2 Launder-Sharma; fmu = 1 for lparoi 1 or 2
+__global__ void __launch_bounds__(TX *TY) k_met_kemut(Blk b, double cmu, int form, int lparoi,
+                                                      const double *__restrict__ v, long ldv,
+                                                      const double *__restrict__ mu, double *__restrict__ mut,
+                                                      const double *__restrict__ dist,
+                                                      const double *__restrict__ g9) {
+  REAL_CELL(b)
+  const long N = b.nnn;
+  double v1 = v[n], v6 = v[5 * ldv + n], v7 = v[6 * ldv + n];
+  double retur = (v6 * v6) / (v7 * mu[n]);
+  double mut0 = cmu * retur * mu[n];
+  double fmu;
+  if (lparoi == 1 || lparoi == 2) {
+    fmu = 1.;
+  } else if (form == 2) {
+    double q = 1. + retur / 50.;
+    fmu = exp(-3.4 / (q * q));
+  } else {
+    fmu = exp(-2.5 / (1. + retur / 50.));
+  }
+  if (form != 1) {
+    mut[n] = fmu * mut0;
+    return;
+  }
+  double a1 = sqrt(cmu);
+  double coef1 = 2. * pow(v6, 1.5) / (v7 * sqrt(v1) * dist[n]);
+  double coef2 = 500. * mu[n] * cmu * v6 / (v7 * (dist[n] * dist[n]) * v1);
+  double x = g9[7 * N + m] - g9[5 * N + m], y = g9[2 * N + m] - g9[6 * N + m], z = g9[3 * N + m] - g9[N + m];
+  double rota = sqrt(x * x + y * y + z * z);
+  double zeta = coef1 > coef2 ? coef1 : coef2;
+  double z2 = 2. * (zeta * zeta);
+  double exp2x = exp(z2 < 25. ? z2 : 25.);
+  double f2 = (exp2x - 1.) / (exp2x + 1.);
+  double lim = rota * f2 * v6 * a1 * fmu / v7;
+  mut[n] = fmu * mut0 / (1. > lim ? 1. : lim);
+}
+
+extern "C" int codens_met_kemut(const codens_block *cb, double cmu, int32_t form, int32_t lparoi, const double *sn,
+                                const double *vol, double *v, int64_t ldv, const double *mu, double *mut,
+                                const double *dist, double *g9, void *stream) {
+  CHECK_BLOCK("codens_met_kemut")
+  if (!v || !mu || !mut || form < 0 || form > 2 || (form == 1 && (!sn || !vol || !dist || !g9))) {
+    codens_set_error("codens_met_kemut: null or invalid argument");
+    return CODENS_EINVAL;
+  }
+  if (form == 1) {
+    int r = codens_teq_gradv(cb, sn, vol, v, ldv, g9, stream);
+    if (r) return r;
+  }
+  hipLaunchKernelGGL(k_met_kemut, cell_grid(b), dim3(TX, TY), 0, st, b, cmu, (int)form, (int)lparoi, v, (long)ldv, mu,
+                     mut, dist, (const double *)g9);
+  return codens_check_launch("codens_met_kemut");
+}

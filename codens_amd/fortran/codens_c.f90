@@ -298,6 +298,24 @@ module codens_c
        type(c_ptr), value :: sn, vol, v, mu, mut, dist, fd, g9, tprod, cfke, ts6, ts7, stream
        integer(c_int64_t), value :: ldv
      end function codens_met_smkl
+     integer(c_int32_t) function codens_met_smke(b, kc, variant, lparoi, ischema, sn, vol, v, ldv, mu, mut, tprod, &
+          g9, cson, bark, bare, cfke, ts6, ts7, stream) bind(C, name="codens_met_smke")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: kc(3)
+       integer(c_int32_t), value :: variant, lparoi, ischema
+       type(c_ptr), value :: sn, vol, v, mu, mut, tprod, g9, cson, bark, bare, cfke, ts6, ts7, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_smke
+     integer(c_int32_t) function codens_met_kemut(b, cmu, form, lparoi, sn, vol, v, ldv, mu, mut, dist, g9, stream) &
+          bind(C, name="codens_met_kemut")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), value :: cmu
+       integer(c_int32_t), value :: form, lparoi
+       type(c_ptr), value :: sn, vol, v, mu, mut, dist, g9, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_kemut
      integer(c_int32_t) function codens_met_klmut(b, kc, v, ldv, mu, mut, dist, stream) bind(C, name="codens_met_klmut")
        import
        type(codens_block), intent(in) :: b
@@ -362,6 +380,17 @@ module codens_c
   end interface
 
 contains
+
+  ! device address p advanced by `bytes`
+  function codens_ptr_off(p, bytes) result(q)
+    type(c_ptr), intent(in) :: p
+    integer(c_int64_t), intent(in) :: bytes
+    type(c_ptr) :: q
+    integer(c_intptr_t) :: a
+    a = transfer(p, a)
+    a = a + bytes
+    q = transfer(a, q)
+  end function codens_ptr_off
 
   integer function codens_geom_mode()
     codens_geom_mode = 1

@@ -855,3 +855,124 @@ contains
     call codens_back(c_loc(mut), ip12, 1, 1, b)
   end subroutine met_klmut
 end module mod_met_klmut
+
+module mod_met_smke
+  implicit none
+contains
+  ! met_smke.f90:4 -- k-epsilon sources: met_exgr on the velocity gradients
+  ! (their ghost layers come back written, as the reference leaves them),
+  ! met_bark and met_bare into bark and bare, then the variant of
+  ! equatt(4:4) (met_smkes, met_smker, met_smkec, met_smkesas); the scratch
+  ! arrays t, dtdx..dtdz and tn1..tn3 are not written
+  subroutine met_smke(l, sn, vol, v, mu, mut, tprod, cfke, dvxx, dvxy, dvxz, dvyx, dvyy, dvyz, dvzx, dvzy, dvzz, &
+       t, dtdx, dtdy, dtdz, bark, bare, qcxts5, qcyts6, tn1, tn2, tn3, cson, cmui1, cmui2, cmuj1, cmuj2, cmuk1, &
+       cmuk2)
+    use para_var
+    use para_fige
+    use maillage
+    use chainecarac, only: equatt
+    use modeleturb, only: cke1, cke2, cmu, lparoi
+    use schemanum, only: ischema
+    use codens_c
+    implicit none
+    integer :: l
+    double precision, target :: sn(ip31*ndir), vol(ip11), v(ip11,ip60), mu(ip12), mut(ip12), tprod(ip00)
+    double precision, target :: cfke(ip13), cson(ip11), qcxts5(ip12), qcyts6(ip12), bark(ip00), bare(ip00)
+    double precision, target :: dvxx(ip00), dvxy(ip00), dvxz(ip00), dvyx(ip00), dvyy(ip00), dvyz(ip00)
+    double precision, target :: dvzx(ip00), dvzy(ip00), dvzz(ip00)
+    double precision :: t(ip00), dtdx(ip00), dtdy(ip00), dtdz(ip00), tn1(ip00), tn2(ip00), tn3(ip00)
+    double precision :: cmui1(ip21), cmui2(ip21), cmuj1(ip21), cmuj2(ip21), cmuk1(ip21), cmuk2(ip21)
+    type(codens_block) :: b
+    type(c_ptr) :: dsn, dvol, dv, dmu, dmut, dtp, dg, dcs, dbb, dcf, d5, d6, g(9), bb(2)
+    real(c_double) :: kc(3)
+    integer(c_int32_t) :: variant
+    select case (equatt(4:4))
+    case ('R')
+       variant = 1
+    case ('C')
+       variant = 2
+    case ('L')
+       variant = 3
+    case default
+       variant = 0
+    end select
+    b = codens_block_of(l)
+    kc = [cke1, cke2, cmu]
+    g = [c_loc(dvxx), c_loc(dvxy), c_loc(dvxz), c_loc(dvyx), c_loc(dvyy), c_loc(dvyz), c_loc(dvzx), c_loc(dvzy), &
+         c_loc(dvzz)]
+    bb = [c_loc(bark), c_loc(bare)]
+    dsn = codens_dev_all(codens_sn_blk(sn, l), 9*nnn(l), codens_geom_mode())
+    dvol = codens_dev(c_loc(vol), ip11, ip11, 1, 1, b, codens_geom_mode())
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 7, b, 1)
+    dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+    dmut = codens_dev(c_loc(mut), ip12, ip12, 1, 1, b, 1)
+    dcs = codens_dev(c_loc(cson), ip11, ip11, 1, 1, b, 1)
+    dtp = codens_dev_all(c_loc(tprod), nnn(l), 1)
+    dg = codens_bundle_in(g, 9, nnn(l), 0, nnn(l))
+    dbb = codens_bundle_in(bb, 2, nnn(l), 0, nnn(l))
+    dcf = codens_dev(c_loc(cfke), ip13, ip13, 1, 1, b, 1)
+    d5 = codens_dev(c_loc(qcxts5), ip12, ip12, 1, 1, b, 1)
+    d6 = codens_dev(c_loc(qcyts6), ip12, ip12, 1, 1, b, 1)
+    call codens_check(codens_met_smke(b, kc, variant, int(lparoi, c_int32_t), int(ischema, c_int32_t), dsn, dvol, &
+         dv, int(ip11, c_int64_t), dmu, dmut, dtp, dg, dcs, dbb, codens_ptr_off(dbb, 8_c_int64_t*nnn(l)), dcf, d5, &
+         d6, c_null_ptr), 'met_smke')
+    call codens_bundle_back(g, 9, nnn(l), 0, nnn(l))
+    call codens_bundle_back(bb, 2, nnn(l), 0, nnn(l))
+    call codens_back(c_loc(cfke), ip13, 1, 1, b)
+    call codens_back(c_loc(qcxts5), ip12, 1, 1, b)
+    call codens_back(c_loc(qcyts6), ip12, 1, 1, b)
+  end subroutine met_smke
+end module mod_met_smke
+
+module mod_met_kemut
+  implicit none
+contains
+  ! met_kemut.f90:4 -- k-epsilon eddy viscosity ('2JL' with kesst 0 or 1,
+  ! '2LS'); with kesst = 1 the velocity gradients and the rho clamp of
+  ! teq_gradv come back written, as the reference leaves them; the scratch t
+  ! is not written
+  subroutine met_kemut(l, sn, vol, t, dvxx, dvxy, dvxz, dvyx, dvyy, dvyz, dvzx, dvzy, dvzz, dist, v, mu, mut, &
+       cmui1, cmui2, cmuj1, cmuj2, cmuk1, cmuk2)
+    use para_var
+    use para_fige
+    use maillage
+    use chainecarac, only: equatt
+    use modeleturb, only: cmu, lparoi, kesst
+    use codens_c
+    implicit none
+    integer :: l
+    double precision, target :: sn(ip31*ndir), vol(ip11), v(ip11,ip60), mu(ip12), mut(ip12), dist(ip12)
+    double precision, target :: dvxx(ip00), dvxy(ip00), dvxz(ip00), dvyx(ip00), dvyy(ip00), dvyz(ip00)
+    double precision, target :: dvzx(ip00), dvzy(ip00), dvzz(ip00)
+    double precision :: t(ip00)
+    double precision :: cmui1(ip21), cmui2(ip21), cmuj1(ip21), cmuj2(ip21), cmuk1(ip21), cmuk2(ip21)
+    type(codens_block) :: b
+    type(c_ptr) :: dsn, dvol, dv, dmu, dmut, ddist, dg, g(9)
+    integer(c_int32_t) :: form
+    if (equatt(1:3) .eq. '2LS') then
+       form = 2
+    else if (equatt(1:3) .eq. '2JL') then
+       form = merge(1, 0, kesst .eq. 1)
+    else
+       return
+    end if
+    if (equatt(1:3) .eq. '2JL' .and. kesst .ne. 0 .and. kesst .ne. 1) return
+    b = codens_block_of(l)
+    g = [c_loc(dvxx), c_loc(dvxy), c_loc(dvxz), c_loc(dvyx), c_loc(dvyy), c_loc(dvyz), c_loc(dvzx), c_loc(dvzy), &
+         c_loc(dvzz)]
+    dsn = codens_dev_all(codens_sn_blk(sn, l), 9*nnn(l), codens_geom_mode())
+    dvol = codens_dev(c_loc(vol), ip11, ip11, 1, 1, b, codens_geom_mode())
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 7, b, 1)
+    dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+    dmut = codens_dev(c_loc(mut), ip12, ip12, 1, 1, b, 1)
+    ddist = codens_dev(c_loc(dist), ip12, ip12, 1, 1, b, codens_geom_mode())
+    dg = codens_bundle_in(g, 9, nnn(l), 0, nnn(l))
+    call codens_check(codens_met_kemut(b, cmu, form, int(lparoi, c_int32_t), dsn, dvol, dv, int(ip11, c_int64_t), &
+         dmu, dmut, ddist, dg, c_null_ptr), 'met_kemut')
+    call codens_back(c_loc(mut), ip12, 1, 1, b)
+    if (form .eq. 1) then
+       call codens_back(c_loc(v), ip11, 1, 1, b)
+       call codens_bundle_back(g, 9, nnn(l), 0, nnn(l))
+    end if
+  end subroutine met_kemut
+end module mod_met_kemut

@@ -407,6 +407,27 @@ int codens_met_komut(const codens_block *b, double betae, int32_t komsst, const 
                      double *v, int64_t ldv, const double *mu, double *mut, const double *dist, double *g9,
                      void *stream);
 
+/* met_smke.f90:4 -- k-epsilon sources: met_exgr on the three rows of the
+ * velocity gradients g9 (block-local, 9 x nnn: their ghost layers are
+ * written, as the reference does), met_bark and met_bare into bark and bare
+ * (block-local), then the variant selected by equatt(4:4): 0 met_smkes
+ * (' ' or 'S'), 1 met_smker ('R'), 2 met_smkec ('C', reads cson), 3
+ * met_smkesas ('L', Laplacians by met_laplaciens with scheme ischema).
+ * kc = cke1, cke2, cmu; lparoi = 1 zeroes bark, bare and the f2 damping.
+ * tprod block-local; cfke, ts6 (qcxts5), ts7 (qcyts6) storage arrays. */
+int codens_met_smke(const codens_block *b, const double *kc, int32_t variant, int32_t lparoi, int32_t ischema,
+                    const double *sn_blk, const double *vol, const double *v, int64_t ldv, const double *mu,
+                    const double *mut, const double *tprod, double *g9, const double *cson, double *bark,
+                    double *bare, double *cfke, double *ts6, double *ts7, void *stream);
+
+/* met_kemut.f90:4 -- k-epsilon eddy viscosity: form 0 Jones-Launder
+ * ('2JL', kesst = 0), 1 with the SST limiter (kesst = 1: codens_teq_gradv
+ * into g9 first, as the reference calls teq_gradv there; reads dist), 2
+ * Launder-Sharma ('2LS'); fmu = 1 when lparoi is 1 or 2. */
+int codens_met_kemut(const codens_block *b, double cmu, int32_t form, int32_t lparoi, const double *sn_blk,
+                     const double *vol, double *v, int64_t ldv, const double *mu, double *mut, const double *dist,
+                     double *g9, void *stream);
+
 /* Tile shape of the tiled Jacobi sub-iteration of codens_implimf_3d,
  * codens_implimf_eu and codens_implimf_3d_f32 (no counterpart in the
  * reference): -1 (default) times the 64 x 8 and 32 x 16 shapes on the first

@@ -418,3 +418,42 @@ def test_kl_model(tsetup):
             close(r_[q][inner], g_[q][inner])
         r_, g_ = _both(R, G, "ref_met_klmut", l + 1, vk, mu, np.zeros(n), dist)
         close(r_[2][inner], g_[2][inner])
+
+
+@pytest.mark.parametrize("code,lparoi,isch", [(0, 0, 7), (2, 0, 7), (3, 1, 7), (4, 0, 7), (4, 0, 1)])
+def test_ke_model(tsetup, code, lparoi, isch):
+    """met_smke (each variant, by equatt(4:4)) and met_kemut (2JL, 2JL with
+    kesst = 1, 2LS) through the reference's signatures."""
+    from oracle import ptr
+    lay, sn, vol, v, mu, mut, dist, c, R, G = tsetup
+    n = lay.ip11
+    ke = np.array([1.55, 2.0, 0.09])
+    for X in (R, G):
+        X.lib.ref_set_ke(ptr(ke), lparoi, 0, code)
+    rng = np.random.default_rng(611 + code)
+    vk = v.copy(); vk[6] = 5e-2 * (1 + rng.random(n)) * v[0]
+    cmu = np.ones(n)
+    for l, b in enumerate(lay.blocks):
+        inner = b.interior()
+        m = inner - b.npc
+        g = np.zeros((9, lay.ip00)); g[:, :b.nnn] = rng.standard_normal((9, b.nnn))
+        tp = np.zeros(lay.ip00); tp[:b.nnn] = 1e-2 * rng.random(b.nnn)
+        w4 = [np.zeros(lay.ip00) for _ in range(4)]
+        r_, g_ = _both(R, G, "ref_met_smke", l + 1, isch, sn, vol, vk, mu, mut, tp, np.zeros(n), g, *w4,
+                       np.zeros(lay.ip00), np.zeros(lay.ip00), np.zeros(n), np.zeros(n), np.zeros((3, lay.ip00)), c,
+                       cmu)
+        for q in (6, 14, 15):  # cfke, ts6, ts7
+            close(r_[q][inner], g_[q][inner])
+        for q in (12, 13):  # bark, bare
+            close(r_[q][m], g_[q][m])
+        close(r_[7][:, :b.nnn], g_[7][:, :b.nnn])  # velocity gradients with met_exgr's ghost layers
+    for form, kes, ecode in ((0, 0, 0), (1, 1, 1), (2, 0, 5)):
+        for X in (R, G):
+            X.lib.ref_set_ke(ptr(ke), lparoi, kes, ecode)
+        for l, b in enumerate(lay.blocks):
+            inner = b.interior()
+            r_, g_ = _both(R, G, "ref_met_kemut", l + 1, sn, vol, vk, mu, np.zeros(n), dist, np.zeros((9, lay.ip00)),
+                           np.zeros(lay.ip00), cmu)
+            close(r_[4][inner], g_[4][inner])
+            if form == 1:
+                close(r_[2][:, inner], g_[2][:, inner])

@@ -394,3 +394,67 @@ def test_kl_model(setup):
         gm = T(np.zeros(n))
         abi.met_klmut(abi.block(b), kl, T(vk), T(mu), gm, T(dist))
         assert relerr(H(gm)[inner], mo[inner]) <= TOL
+
+
+KE = np.array([1.55, 2.0, 0.09])  # cke1, cke2, cmu (atlecdon.f90:124-128)
+
+
+@pytest.mark.parametrize("variant,lparoi,isch", [(0, 0, 7), (1, 0, 7), (2, 1, 7), (3, 0, 7), (3, 0, 1)])
+def test_ke_sources(setup, variant, lparoi, isch):
+    """k-epsilon sources (met_smke: met_exgr, met_bark, met_bare, then
+    met_smkes / met_smker / met_smkec / met_smkesas) against the oracle:
+    cfke, ts6, ts7, bark and bare within 1e-12, the ghost layers that
+    met_exgr writes into the velocity gradients exactly."""
+    import ctypes
+    from oracle import ptr
+    abi, lay, sn, vol, v, mu, mut, dist, cson, tc, tcl, O = setup
+    n = lay.ip11
+    rng = np.random.default_rng(511 + variant)
+    vk = v.copy(); vk[6] = 5e-2 * (1 + rng.random(n)) * v[0]
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        inner = b.interior()
+        m = inner - b.npc
+        g9 = rng.standard_normal((9, b.nnn)); tp = 1e-2 * rng.random(b.nnn)
+        sb = np.ascontiguousarray(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+        go9 = g9.copy()
+        ob2 = [np.zeros(b.nnn) for _ in range(2)]
+        oo = [np.zeros(n) for _ in range(3)]
+        O.lib.orc_met_smke(ctypes.byref(ob), ptr(KE), variant, lparoi, isch, ptr(sb), ptr(vol), ptr(vk), C.L_(n),
+                           ptr(mu), ptr(mut), ptr(tp), ptr(go9), ptr(cson), ptr(ob2[0]), ptr(ob2[1]),
+                           *[ptr(a) for a in oo])
+        gg9 = T(g9)
+        gb2 = [T(np.zeros(b.nnn)) for _ in range(2)]
+        go = [T(np.zeros(n)) for _ in range(3)]
+        abi.met_smke(abi.block(b), KE, variant, lparoi, isch, snb(sn, b), T(vol), T(vk), T(mu), T(mut), T(tp), gg9,
+                     T(cson), *gb2, *go)
+        for a, o in zip(go, oo):
+            assert relerr(H(a)[inner], o[inner]) <= TOL
+        for a, o in zip(gb2, ob2):
+            assert relerr(H(a)[m], o[m]) <= TOL
+        assert np.array_equal(H(gg9), go9)
+        assert np.abs(oo[2][inner]).max() > 0
+
+
+@pytest.mark.parametrize("form,lparoi", [(0, 0), (1, 0), (2, 0), (2, 1)])
+def test_ke_eddy_viscosity(setup, form, lparoi):
+    """met_kemut (Jones-Launder, with the SST limiter, Launder-Sharma)
+    against the oracle within 1e-12."""
+    import ctypes
+    from oracle import ptr
+    abi, lay, sn, vol, v, mu, mut, dist, cson, tc, tcl, O = setup
+    n = lay.ip11
+    rng = np.random.default_rng(531)
+    vk = v.copy(); vk[6] = 5e-2 * (1 + rng.random(n)) * v[0]
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        inner = b.interior()
+        sb = np.ascontiguousarray(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+        vo = vk.copy(); mo = np.zeros(n); g9o = np.zeros((9, b.nnn))
+        O.lib.orc_met_kemut(ctypes.byref(ob), C.D(KE[2]), form, lparoi, ptr(sb), ptr(vol), ptr(vo), C.L_(n), ptr(mu),
+                            ptr(mo), ptr(dist), ptr(g9o))
+        gv = T(vk); gm = T(np.zeros(n)); gg9 = T(np.zeros((9, b.nnn)))
+        abi.met_kemut(abi.block(b), KE[2], form, lparoi, snb(sn, b), T(vol), gv, T(mu), gm, T(dist), gg9)
+        assert relerr(H(gm)[inner], mo[inner]) <= TOL
+        if form == 1:
+            assert relerr(H(gg9), g9o) <= TOL

@@ -387,3 +387,10 @@ def met_kemut(b, cmu, form, lparoi, sn_blk, vol, v, mu, mut, dist, g9, stream=No
     """met_kemut.f90:4 (form 0 Jones-Launder, 1 with SST limiter, 2 Launder-Sharma)."""
     _chk(lib.codens_met_kemut(ctypes.byref(b), float(cmu), int(form), int(lparoi), _p(sn_blk), _p(vol), _p(v), ld(v),
                               _p(mu), _p(mut), _p(dist), _p(g9), _s(stream)))
+
+
+def met_smklsas(b, kc, ischema, sn_blk, vol, v, mu, mut, dist, fd, g9, tprod, cfke, ts6, ts7, stream=None):
+    """met_smklsas.f90:4 (Smith k-l with the scale-adaptive term)."""
+    k = (ctypes.c_double * 4)(*[float(x) for x in kc])
+    _chk(lib.codens_met_smklsas(ctypes.byref(b), k, int(ischema), _p(sn_blk), _p(vol), _p(v), ld(v), _p(mu), _p(mut),
+                                _p(dist), _p(fd), _p(g9), _p(tprod), _p(cfke), _p(ts6), _p(ts7), _s(stream)))

@@ -383,7 +383,8 @@ __global__ void __launch_bounds__(TX *TY) k_met_smkl(Blk b, double cklb1, double
                                                      const double *__restrict__ mu, const double *__restrict__ mut,
                                                      const double *__restrict__ dist, const double *__restrict__ fd,
                                                      const double *__restrict__ g9, const double *__restrict__ tprod,
-                                                     const double *__restrict__ tk, double *__restrict__ cfke,
+                                                     const double *__restrict__ tk,
+                                                     const double *__restrict__ lap, double *__restrict__ cfke,
                                                      double *__restrict__ ts6, double *__restrict__ ts7) {
   REAL_CELL(b)
   const long N = b.nnn;
@@ -410,7 +411,30 @@ __global__ void __launch_bounds__(TX *TY) k_met_smkl(Blk b, double cklb1, double
   double sl3 = mut[n] * glgl * xlskap2 / (sigmal * xl);
   double sl4 = dssigl * mut[n] * glgk;
   ts6[n] = tprod[m] - sk2 + bark;
-  ts7[n] = sl1 + sl2 - sl3 + sl4;
+  if (lap) {
+    // met_smklsas.f90:238-253: the scale-adaptive term where the shielding
+    // function 1 - tanh((8 rtilde)^3) exceeds 0.8
+    double sl5 = 0.;
+    double xx = g9[m], xy = g9[N + m], xz = g9[2 * N + m], yx = g9[3 * N + m], yy = g9[4 * N + m];
+    double yz = g9[5 * N + m], zx = g9[6 * N + m], zy = g9[7 * N + m], zz = g9[8 * N + m];
+    double nutilde = mut[n] / v1;
+    double dvdv = xx * xx + xy * xy + yx * yx + yy * yy;
+    double rtilde = nutilde / (sqrt(dvdv) * xkapad2);
+    double r8 = 8. * rtilde;
+    double rtil3 = (r8 * r8) * r8;
+    if (1. - tanh(rtil3) > 0.8) {
+      const double qsi2 = 1.47, cc43 = 4. / 3.;
+      double a = zy + yz, cc = xz + zx, e = yx + xy;
+      double ss = sqrt(cc43 * (xx * xx + yy * yy + zz * zz) + a * a + cc * cc + e * e);
+      double l1 = lap[m], l2 = lap[N + m];
+      double nlaplu = sqrt(l1 * l1 + l2 * l2);
+      double xlvk = xkappa * ss / nlaplu;
+      sl5 = qsi2 * tprod[m] * ((xl * xl) * xl) / (xk * (xlvk * xlvk));
+    }
+    ts7[n] = sl1 + sl2 - sl3 + sl4 - sl5;
+  } else {
+    ts7[n] = sl1 + sl2 - sl3 + sl4;
+  }
   double gkgro = (f5x * gr[0] + f5y * gr[1] + f5z * gr[2]) / v6;
   double glgro = (f6x * gr[0] + f6y * gr[1] + f6z * gr[2]) / v1;
   double dqkdk = -c132 * racro * racrk / v7 - bark / v6 + mu[n] * gkgro / v1;
@@ -785,8 +809,35 @@ extern "C" int codens_met_smkl(const codens_block *cb, const double *kc, const d
   hipLaunchKernelGGL(k_sqrt_k, dim3((unsigned)((b.nnn + 255) / 256)), dim3(256), 0, st, (long)b.npc, (long)b.nnn, v,
                      (long)ldv, tk);
   hipLaunchKernelGGL(k_met_smkl, cell_grid(b), dim3(TX, TY), 0, st, b, kc[0], kc[1], kc[2], kc[3], sn, vol, v,
-                     (long)ldv, mu, mut, dist, fd, g9, tprod, (const double *)tk, cfke, ts6, ts7);
+                     (long)ldv, mu, mut, dist, fd, g9, tprod, (const double *)tk, (const double *)nullptr, cfke, ts6,
+                     ts7);
   return codens_check_launch("codens_met_smkl");
+}
+
+extern "C" int codens_met_smklsas(const codens_block *cb, const double *kc, int32_t ischema, const double *sn,
+                                  const double *vol, const double *v, int64_t ldv, const double *mu, const double *mut,
+                                  const double *dist, const double *fd, const double *g9, const double *tprod,
+                                  double *cfke, double *ts6, double *ts7, void *stream) {
+  CHECK_BLOCK("codens_met_smklsas")
+  if (!kc || !sn || !vol || !v || !mu || !mut || !dist || !fd || !g9 || !tprod || !cfke || !ts6 || !ts7) {
+    codens_set_error("codens_met_smklsas: null argument");
+    return CODENS_EINVAL;
+  }
+  double *ws = (double *)codens_workspace(sizeof(double) * 3 * b.nnn, 7);
+  if (!ws) {
+    codens_set_error("codens_met_smklsas: workspace allocation failed");
+    return CODENS_EHIP;
+  }
+  const long N = b.nnn;
+  double *tk = ws, *lap = ws + N;
+  hipLaunchKernelGGL(k_laplacian, cell_grid(b), dim3(TX, TY), 0, st, b, (int)ischema, sn, vol, g9, g9 + N, g9 + 2 * N,
+                     lap);
+  hipLaunchKernelGGL(k_laplacian, cell_grid(b), dim3(TX, TY), 0, st, b, (int)ischema, sn, vol, g9 + 3 * N, g9 + 4 * N,
+                     g9 + 5 * N, lap + N);
+  hipLaunchKernelGGL(k_sqrt_k, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, (long)b.npc, N, v, (long)ldv, tk);
+  hipLaunchKernelGGL(k_met_smkl, cell_grid(b), dim3(TX, TY), 0, st, b, kc[0], kc[1], kc[2], kc[3], sn, vol, v,
+                     (long)ldv, mu, mut, dist, fd, g9, tprod, (const double *)tk, (const double *)lap, cfke, ts6, ts7);
+  return codens_check_launch("codens_met_smklsas");
 }
 
 extern "C" int codens_met_klmut(const codens_block *cb, const double *kc, const double *v, int64_t ldv,

@@ -316,6 +316,15 @@ module codens_c
        type(c_ptr), value :: sn, vol, v, mu, mut, dist, g9, stream
        integer(c_int64_t), value :: ldv
      end function codens_met_kemut
+     integer(c_int32_t) function codens_met_smklsas(b, kc, ischema, sn, vol, v, ldv, mu, mut, dist, fd, g9, tprod, &
+          cfke, ts6, ts7, stream) bind(C, name="codens_met_smklsas")
+       import
+       type(codens_block), intent(in) :: b
+       real(c_double), intent(in) :: kc(4)
+       integer(c_int32_t), value :: ischema
+       type(c_ptr), value :: sn, vol, v, mu, mut, dist, fd, g9, tprod, cfke, ts6, ts7, stream
+       integer(c_int64_t), value :: ldv
+     end function codens_met_smklsas
      integer(c_int32_t) function codens_met_klmut(b, kc, v, ldv, mu, mut, dist, stream) bind(C, name="codens_met_klmut")
        import
        type(codens_block), intent(in) :: b

@@ -976,3 +976,57 @@ contains
     end if
   end subroutine met_kemut
 end module mod_met_kemut
+
+module mod_met_smklsas
+  implicit none
+contains
+  ! met_smklsas.f90:4 -- Smith k-l sources with the scale-adaptive term
+  ! (met_bark's term, met_laplaciens of u and v); the scratch arrays t,
+  ! dtdx..dtdz, bark and tn1..tn9 are not written
+  subroutine met_smklsas(l, sn, vol, v, mu, mut, dist, mnpar, dvxx, dvxy, dvxz, dvyx, dvyy, dvyz, dvzx, dvzy, &
+       dvzz, txxf5x, txyf5y, txzf5z, tyyf6x, tyzf6y, tzzf6z, tprod, cfke, t, dtdx, dtdy, dtdz, bark, qcxts5, &
+       qcyts6, tn1, tn2, tn3, tn9, cmui1, cmui2, cmuj1, cmuj2, cmuk1, cmuk2)
+    use para_var
+    use para_fige
+    use maillage
+    use modeleturb, only: cklb1, ckle2, sigmal, xkappa
+    use schemanum, only: ischema
+    use codens_c
+    implicit none
+    integer :: l
+    integer :: mnpar(ip12)
+    double precision, target :: sn(ip31*ndir), vol(ip11), v(ip11,ip60), mu(ip12), mut(ip12), dist(ip12)
+    double precision, target :: dvxx(ip00), dvxy(ip00), dvxz(ip00), dvyx(ip00), dvyy(ip00), dvyz(ip00)
+    double precision, target :: dvzx(ip00), dvzy(ip00), dvzz(ip00), tprod(ip00)
+    double precision, target :: txxf5x(ip12), txyf5y(ip12), txzf5z(ip12), tyyf6x(ip12), tyzf6y(ip12), tzzf6z(ip12)
+    double precision, target :: cfke(ip13), qcxts5(ip12), qcyts6(ip12)
+    double precision :: t(ip00), dtdx(ip00), dtdy(ip00), dtdz(ip00), bark(ip00)
+    double precision :: tn1(ip00), tn2(ip00), tn3(ip00), tn9(ip00)
+    double precision :: cmui1(ip21), cmui2(ip21), cmuj1(ip21), cmuj2(ip21), cmuk1(ip21), cmuk2(ip21)
+    type(codens_block) :: b
+    type(c_ptr) :: dsn, dvol, dv, dmu, dmut, ddist, dfd, dg, dtp, dcf, d5, d6, f(6), g(9)
+    real(c_double) :: kc(4)
+    b = codens_block_of(l)
+    kc = [cklb1, ckle2, sigmal, xkappa]
+    f = [c_loc(txxf5x), c_loc(txyf5y), c_loc(txzf5z), c_loc(tyyf6x), c_loc(tyzf6y), c_loc(tzzf6z)]
+    g = [c_loc(dvxx), c_loc(dvxy), c_loc(dvxz), c_loc(dvyx), c_loc(dvyy), c_loc(dvyz), c_loc(dvzx), c_loc(dvzy), &
+         c_loc(dvzz)]
+    dsn = codens_dev_all(codens_sn_blk(sn, l), 9*nnn(l), codens_geom_mode())
+    dvol = codens_dev(c_loc(vol), ip11, ip11, 1, 1, b, codens_geom_mode())
+    dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 7, b, 1)
+    dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+    dmut = codens_dev(c_loc(mut), ip12, ip12, 1, 1, b, 1)
+    ddist = codens_dev(c_loc(dist), ip12, ip12, 1, 1, b, codens_geom_mode())
+    dfd = codens_bundle_in(f, 6, ip12, npc(l), npc(l) + nnn(l))
+    dg = codens_bundle_in(g, 9, nnn(l), 0, nnn(l))
+    dtp = codens_dev_all(c_loc(tprod), nnn(l), 1)
+    dcf = codens_dev(c_loc(cfke), ip13, ip13, 1, 1, b, 1)
+    d5 = codens_dev(c_loc(qcxts5), ip12, ip12, 1, 1, b, 1)
+    d6 = codens_dev(c_loc(qcyts6), ip12, ip12, 1, 1, b, 1)
+    call codens_check(codens_met_smklsas(b, kc, int(ischema, c_int32_t), dsn, dvol, dv, int(ip11, c_int64_t), dmu, &
+         dmut, ddist, dfd, dg, dtp, dcf, d5, d6, c_null_ptr), 'met_smklsas')
+    call codens_back(c_loc(cfke), ip13, 1, 1, b)
+    call codens_back(c_loc(qcxts5), ip12, 1, 1, b)
+    call codens_back(c_loc(qcyts6), ip12, 1, 1, b)
+  end subroutine met_smklsas
+end module mod_met_smklsas

@@ -407,6 +407,15 @@ int codens_met_komut(const codens_block *b, double betae, int32_t komsst, const 
                      double *v, int64_t ldv, const double *mu, double *mut, const double *dist, double *g9,
                      void *stream);
 
+/* met_smklsas.f90:4 -- codens_met_smkl with Menter's scale-adaptive term:
+ * ts7 -= qsi2 P l^3 / (k L_vK^2) where 1 - tanh((8 rtilde)^3) > 0.8, L_vK
+ * from the Laplacians of u and v (met_laplaciens, scheme ischema) of the
+ * gradients g9 as given. */
+int codens_met_smklsas(const codens_block *b, const double *kc, int32_t ischema, const double *sn_blk,
+                       const double *vol, const double *v, int64_t ldv, const double *mu, const double *mut,
+                       const double *dist, const double *fd, const double *g9, const double *tprod, double *cfke,
+                       double *ts6, double *ts7, void *stream);
+
 /* met_smke.f90:4 -- k-epsilon sources: met_exgr on the three rows of the
  * velocity gradients g9 (block-local, 9 x nnn: their ghost layers are
  * written, as the reference does), met_bark and met_bare into bark and bare

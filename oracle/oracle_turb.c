@@ -494,9 +494,14 @@ void orc_met_bark(const orc_block *b, const double *sn, const double *vol, const
  * and cfke the larger modulus of the eigenvalues of the 2x2 source Jacobian
  * (:163-226; bark from met_bark, grad rho by teq_grads).  fd: gradients of
  * rho k and rho l (six planes of ldv); g9, tprod block-local. */
-void orc_met_smkl(const orc_block *b, const orc_kl *kc, const double *sn, const double *vol, const double *v, long ldv,
-                  const double *mu, const double *mut, const double *dist, const double *fd, const double *g9,
-                  const double *tprod, double *cfke, double *ts6, double *ts7) {
+/* Shared body of met_smkl and met_smklsas: lap = the Laplacians of u and v
+ * (block-local, two planes) selects the scale-adaptive term sl5 of
+ * met_smklsas.f90:238-251, subtracted from ts7 where the shielding function
+ * fd = 1 - tanh((8 rtilde)^3) exceeds 0.8. */
+static void kl_sources(const orc_block *b, const orc_kl *kc, const double *sn, const double *vol, const double *v,
+                       long ldv, const double *mu, const double *mut, const double *dist, const double *fd,
+                       const double *g9, const double *tprod, const double *lap, double *cfke, double *ts6,
+                       double *ts7) {
   long nnn = b->nnn, n0 = b->npc;
   double *bark = calloc(nnn, 8), *t = calloc(nnn, 8), *gr = calloc(3 * nnn, 8);
   bark_of(b, sn, vol, v, ldv, mu, bark);
@@ -524,8 +529,28 @@ void orc_met_smkl(const orc_block *b, const orc_kl *kc, const double *sn, const 
         double sl2 = v7 * divv;
         double sl3 = mut[n] * glgl * xlskap2 / (kc->sigmal * xl);
         double sl4 = dssigl * mut[n] * glgk;
+        double sl5 = 0.;
+        if (lap) {
+          double nutilde = mut[n] / v1;
+          double xx = g9[m], xy = g9[nnn + m], xz = g9[2 * nnn + m], yx = g9[3 * nnn + m], yy = g9[4 * nnn + m];
+          double yz = g9[5 * nnn + m], zx = g9[6 * nnn + m], zy = g9[7 * nnn + m], zz = g9[8 * nnn + m];
+          double dvdv = xx * xx + xy * xy + yx * yx + yy * yy;
+          double rtilde = nutilde / (sqrt(dvdv) * xkapad2);
+          double r8 = 8. * rtilde;
+          double rtil3 = (r8 * r8) * r8;
+          double fdd = 1. - tanh(rtil3);
+          if (fdd > 0.8) {
+            const double qsi2 = 1.47, cc43 = 4. / 3.;
+            double a = zy + yz, c = xz + zx, e = yx + xy;
+            double ss = sqrt(cc43 * (xx * xx + yy * yy + zz * zz) + a * a + c * c + e * e);
+            double l1 = lap[m], l2 = lap[nnn + m];
+            double nlaplu = sqrt(l1 * l1 + l2 * l2);
+            double xlvk = kc->xkappa * ss / nlaplu;
+            sl5 = qsi2 * tprod[m] * ((xl * xl) * xl) / (xk * (xlvk * xlvk));
+          }
+        }
         ts6[n] = tprod[m] - sk2 + bark[m];
-        ts7[n] = sl1 + sl2 - sl3 + sl4;
+        ts7[n] = lap ? sl1 + sl2 - sl3 + sl4 - sl5 : sl1 + sl2 - sl3 + sl4;
         double gkgro = (f5x[n] * gr[m] + f5y[n] * gr[nnn + m] + f5z[n] * gr[2 * nnn + m]) / v6;
         double glgro = (f6x[n] * gr[m] + f6y[n] * gr[nnn + m] + f6z[n] * gr[2 * nnn + m]) / v1;
         double dqkdk = -c132 * racro * racrk / v7 - bark[m] / v6 + mu[n] * gkgro / v1;
@@ -539,6 +564,28 @@ void orc_met_smkl(const orc_block *b, const orc_kl *kc, const double *sn, const 
         cfke[n] = a1 > a2 ? a1 : a2;
       }
   free(bark); free(t); free(gr);
+}
+
+void orc_met_smkl(const orc_block *b, const orc_kl *kc, const double *sn, const double *vol, const double *v, long ldv,
+                  const double *mu, const double *mut, const double *dist, const double *fd, const double *g9,
+                  const double *tprod, double *cfke, double *ts6, double *ts7) {
+  kl_sources(b, kc, sn, vol, v, ldv, mu, mut, dist, fd, g9, tprod, NULL, cfke, ts6, ts7);
+}
+
+/* met_smklsas.f90:4 -- met_smkl with the scale-adaptive term; the Laplacians
+ * of u and v by met_laplaciens with scheme ischema, of the gradients as given
+ * (their ghost entries read as they are). */
+void orc_met_smklsas(const orc_block *b, const orc_kl *kc, int ischema, const double *sn, const double *vol,
+                     const double *v, long ldv, const double *mu, const double *mut, const double *dist,
+                     const double *fd, const double *g9, const double *tprod, double *cfke, double *ts6,
+                     double *ts7) {
+  long nnn = b->nnn;
+  double *w = calloc(6 * nnn, sizeof(double));
+  orc_met_laplaciens(b, ischema, sn, vol, g9, g9 + nnn, g9 + 2 * nnn, w, w + 2 * nnn, w + 3 * nnn);
+  orc_met_laplaciens(b, ischema, sn, vol, g9 + 3 * nnn, g9 + 4 * nnn, g9 + 5 * nnn, w + nnn, w + 4 * nnn,
+                     w + 5 * nnn);
+  kl_sources(b, kc, sn, vol, v, ldv, mu, mut, dist, fd, g9, tprod, w, cfke, ts6, ts7);
+  free(w);
 }
 
 /* met_klmut.f90:4 -- Smith k-l eddy viscosity mut = fmu mu chi, chi =

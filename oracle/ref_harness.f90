@@ -644,4 +644,23 @@ contains
          dist, v, mu, mut, cmu, cmu, cmu, cmu, cmu, cmu)
   end subroutine ref_met_kemut
 
+  ! met_smklsas.f90:4 (calls met_bark.f90:4 and met_laplaciens.f90:4)
+  subroutine ref_met_smklsas(l, isch, sn, vol, v, mu, mut, dist, mnpar, g, fd, tprod, cfke, tt, gx, gy, gz, bark, &
+       ts6, ts7, tn, cmu) bind(C, name="ref_met_smklsas")
+    use mod_met_smklsas
+    use chainecarac
+    integer(c_int), value :: l, isch
+    integer(c_int) :: mnpar(*)
+    real(c_double) :: sn(*), vol(*), v(*), mu(*), mut(*), dist(*), g(ip00, 9), fd(ip11, 6), tprod(*), cfke(*)
+    real(c_double) :: tt(*), gx(*), gy(*), gz(*), bark(*), ts6(*), ts7(*), tn(ip00, 4), cmu(*)
+    integer :: keep
+    equat = heq
+    keep = ischema
+    ischema = isch
+    call met_smklsas(l, sn, vol, v, mu, mut, dist, mnpar, g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), &
+         g(1,8), g(1,9), fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6), tprod, cfke, tt, gx, gy, gz, bark, &
+         ts6, ts7, tn(1,1), tn(1,2), tn(1,3), tn(1,4), cmu, cmu, cmu, cmu, cmu, cmu)
+    ischema = keep
+  end subroutine ref_met_smklsas
+
 end module codens_ref_harness

@@ -457,3 +457,28 @@ def test_ke_model(tsetup, code, lparoi, isch):
             close(r_[4][inner], g_[4][inner])
             if form == 1:
                 close(r_[2][:, inner], g_[2][:, inner])
+
+
+def test_kl_sas(tsetup):
+    """met_smklsas through the reference's signature."""
+    from oracle import ptr
+    lay, sn, vol, v, mu, mut, dist, c, R, G = tsetup
+    n = lay.ip11
+    kl = np.array([18.0, 1.2, 1.43, 0.41])
+    for X in (R, G):
+        X.lib.ref_set_kl(ptr(kl))
+    rng = np.random.default_rng(433)
+    vk = v.copy(); vk[6] = 0.05 * (1 + rng.random(n)) * v[0]
+    fd = np.zeros((6, n)); fd[:3] = 1e-3 * rng.standard_normal((3, n)); fd[3:] = 1e-2 * rng.standard_normal((3, n))
+    cmu = np.ones(n)
+    for isch in (7, 1):
+        for l, b in enumerate(lay.blocks):
+            inner = b.interior()
+            g = np.zeros((9, lay.ip00)); g[:, :b.nnn] = rng.standard_normal((9, b.nnn))
+            tp = np.zeros(lay.ip00); tp[:b.nnn] = 1e-3 * rng.random(b.nnn)
+            w5 = [np.zeros(lay.ip00) for _ in range(5)]
+            r_, g_ = _both(R, G, "ref_met_smklsas", l + 1, isch, sn, vol, vk, mu, mut, dist,
+                           np.zeros(n, dtype=np.int32), g, fd, tp, np.zeros(n), *w5, np.zeros(n), np.zeros(n),
+                           np.zeros((4, lay.ip00)), cmu)
+            for q in (10, 16, 17):  # cfke, ts6, ts7
+                close(r_[q][inner], g_[q][inner])

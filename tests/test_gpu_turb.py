@@ -458,3 +458,30 @@ def test_ke_eddy_viscosity(setup, form, lparoi):
         assert relerr(H(gm)[inner], mo[inner]) <= TOL
         if form == 1:
             assert relerr(H(gg9), g9o) <= TOL
+
+
+@pytest.mark.parametrize("isch", [7, 1])
+def test_kl_sas(setup, isch):
+    """met_smklsas (Smith k-l with the scale-adaptive term) against the
+    oracle within 1e-12."""
+    import ctypes
+    from oracle import ptr
+    abi, lay, sn, vol, v, mu, mut, dist, cson, tc, tcl, O = setup
+    n = lay.ip11
+    kl = np.array([18.0, 1.2, 1.43, 0.41])
+    rng = np.random.default_rng(557 + isch)
+    vk = v.copy(); vk[6] = 0.05 * (1 + rng.random(n)) * v[0]
+    fd = np.zeros((6, n)); fd[:3] = 1e-3 * rng.standard_normal((3, n)); fd[3:] = 1e-2 * rng.standard_normal((3, n))
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        inner = b.interior()
+        g9 = rng.standard_normal((9, b.nnn)); tp = 1e-3 * rng.random(b.nnn)
+        sb = np.ascontiguousarray(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+        oo = [np.zeros(n) for _ in range(3)]
+        O.lib.orc_met_smklsas(ctypes.byref(ob), ptr(kl), isch, ptr(sb), ptr(vol), ptr(vk), C.L_(n), ptr(mu), ptr(mut),
+                              ptr(dist), ptr(fd), ptr(g9), ptr(tp), *[ptr(a) for a in oo])
+        go = [T(np.zeros(n)) for _ in range(3)]
+        abi.met_smklsas(abi.block(b), kl, isch, snb(sn, b), T(vol), T(vk), T(mu), T(mut), T(dist), T(fd), T(g9),
+                        T(tp), *go)
+        for a, o in zip(go, oo):
+            assert relerr(H(a)[inner], o[inner]) <= TOL

@@ -330,3 +330,43 @@ def test_kl_model(setup):
         R.lib.ref_met_klmut(l + 1, ptr(vk), ptr(mu), ptr(mr), ptr(dist))
         O.orc_met_klmut(ctypes.byref(ob), ptr(KL), ptr(vk), C.L_(n), ptr(mu), ptr(mo), ptr(dist))
         assert np.array_equal(mr, mo)
+
+
+@pytest.mark.parametrize("isch", [7, 1])
+def test_kl_sas(setup, isch):
+    """Smith k-l scale-adaptive sources (met_smklsas with met_bark and
+    met_laplaciens), bit for bit; the shielding function selects the SAS
+    term on part of the cells only."""
+    lay, sn, vol, v, mu, mut, dist, cson, tc, R, O = setup
+    n = lay.ip11
+    rng = np.random.default_rng(99 + isch)
+    R.lib.ref_set_kl(ptr(KL))
+    vk = v.copy(); vk[6] = 0.05 * (1 + rng.random(n)) * v[0]
+    fd = np.zeros((6, n)); fd[:3] = 1e-3 * rng.standard_normal((3, n)); fd[3:] = 1e-2 * rng.standard_normal((3, n))
+    mnpar = np.zeros(n, dtype=np.int32)
+    cmu = np.ones(n)
+    differs = 0
+    for l, b in enumerate(lay.blocks):
+        ob = C.orc_block(b)
+        inner = b.interior()
+        g = np.zeros((9, lay.ip00)); g[:, :b.nnn] = rng.standard_normal((9, b.nnn))
+        gb = np.ascontiguousarray(g[:, :b.nnn])
+        tp = np.zeros(lay.ip00); tp[:b.nnn] = 1e-3 * rng.random(b.nnn)
+        w = [np.zeros(lay.ip00) for _ in range(5)]
+        r = [np.full(n, 7.0) for _ in range(3)]
+        R.lib.ref_met_smklsas(l + 1, isch, ptr(sn), ptr(vol), ptr(vk), ptr(mu), ptr(mut), ptr(dist), ptr(mnpar),
+                              ptr(g), ptr(fd), ptr(tp), ptr(r[0]), *[ptr(a) for a in w], ptr(r[1]), ptr(r[2]),
+                              ptr(np.zeros((4, lay.ip00))), ptr(cmu))
+        o = [np.full(n, 7.0) for _ in range(3)]
+        O.orc_met_smklsas(ctypes.byref(ob), ptr(KL), isch, ptr(snb(sn, b)), ptr(vol), ptr(vk), C.L_(n), ptr(mu),
+                          ptr(mut), ptr(dist), ptr(fd), ptr(gb), ptr(tp), *[ptr(a) for a in o])
+        for a, c in zip(r, o):
+            assert np.array_equal(a[inner], c[inner])
+            assert np.isfinite(a[inner]).all()
+        p = [np.full(n, 7.0) for _ in range(3)]
+        O.orc_met_smkl(ctypes.byref(ob), ptr(KL), ptr(snb(sn, b)), ptr(vol), ptr(vk), C.L_(n), ptr(mu), ptr(mut),
+                       ptr(dist), ptr(fd), ptr(gb), ptr(tp), *[ptr(a) for a in p])
+        d = o[2][inner] != p[2][inner]
+        differs += int(d.sum())
+        assert d.sum() < len(inner)  # the SAS term is off where fd <= 0.8
+    assert differs > 0

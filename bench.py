@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--dims", type=str, default="256,128,64")
     ap.add_argument("--workload", choices=("wall", "multiblock"), default="wall",
                     help="wall: the 256x128x64 laminar wall block; multiblock: 48 uneven SA blocks, ~40M cells")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="side streams for the per-block work (0: 4 for multiblock, 1 otherwise)")
     ap.add_argument("--graph", choices=("auto", "on", "off"), default="auto",
                     help="replay stages from a captured HIP graph (auto: on for multiblock)")
     ap.add_argument("--implicit", choices=("fp64", "fp32"), default="fp64",
@@ -88,7 +90,8 @@ def main():
     multi = args.workload == "multiblock"
     ci, cj, ck = (int(x) for x in args.dims.split(","))
     case = configs.multiblock_sa() if multi else configs.wall_ns(ci, cj, ck)
-    s = Solver(case, implicit=args.implicit)
+    nstreams = args.streams or (4 if multi else 1)
+    s = Solver(case, implicit=args.implicit, streams=nstreams)
     ncell = case.layout.ncells
     nblk = len(case.layout.blocks)
     graph = args.graph == "on" or (args.graph == "auto" and multi)
@@ -111,6 +114,7 @@ def main():
     # per-kernel timing pass with HIP events on the launch stream (eager
     # stages: the same launches as the graph's)
     g_saved, s.graph = s.graph, None
+    side_saved, s.side = s.side, []  # one stream: events bracket one kernel at a time
     abi.timing_enable(True)
     for _ in range(max(3, args.steps // 4)):
         s.stage()
@@ -121,7 +125,7 @@ def main():
         if cnt:
             kern[name] = {"avg_us": 1e3 * ms / cnt, "launches": cnt}
     abi.timing_enable(False)
-    s.graph = g_saved
+    s.graph, s.side = g_saved, side_saved
     res = s.density_residual()
     ms_step = 1e3 * wall / args.steps
     value = ncell * args.steps / wall / 1e6
@@ -142,7 +146,8 @@ def main():
                                 if multi else
                                 f"wall_ns {ci}x{cj}x{ck} laminar NS, Roe-MUSCL minmod + viscous, implimf_3d lmax=10, ")
                                + ("FP32 implicit phase (defect correction)" if fp32 else "FP64")
-                               + (", stages replayed from a HIP graph" if graph else ""),
+                               + (", stages replayed from a HIP graph" if graph else "")
+                               + (f", blocks on {nstreams} concurrent streams" if len(s.side) > 1 else ""),
                    "cells": ncell, "blocks": nblk, "lmax": case.lmax},
         "roofline": {"bound": "hbm", "kernel": sweep_name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

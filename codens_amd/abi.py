@@ -394,3 +394,8 @@ def met_smklsas(b, kc, ischema, sn_blk, vol, v, mu, mut, dist, fd, g9, tprod, cf
     k = (ctypes.c_double * 4)(*[float(x) for x in kc])
     _chk(lib.codens_met_smklsas(ctypes.byref(b), k, int(ischema), _p(sn_blk), _p(vol), _p(v), ld(v), _p(mu), _p(mut),
                                 _p(dist), _p(fd), _p(g9), _p(tprod), _p(cfke), _p(ts6), _p(ts7), _s(stream)))
+
+
+def workspace_set(k):
+    """Select workspace set k (0..15) for the following calls."""
+    _chk(lib.codens_workspace_set(int(k)))

@@ -49,22 +49,38 @@ int codens_check_block(const codens_block *b, const char *what) {
   return CODENS_OK;
 }
 
-// Grow-only device workspaces, one set per device, kept until process exit.
+// Grow-only device workspaces, kept until process exit: per device, up to
+// CODENS_WS_SETS independent sets of slots.  The entry points draw from the
+// set selected by codens_workspace_set (0 by default); a caller that runs
+// blocks concurrently on several streams gives each stream its own set.
 namespace {
+constexpr int CODENS_WS_SETS = 16;
 struct Slot {
   void *p = nullptr;
   size_t n = 0;
 };
 std::mutex g_ws_mu;
-std::vector<std::vector<Slot>> g_ws;  // [device][slot]
+std::vector<std::vector<std::vector<Slot>>> g_ws;  // [device][set][slot]
+int g_ws_set = 0;
 }  // namespace
+
+extern "C" int codens_workspace_set(int32_t set) {
+  if (set < 0 || set >= CODENS_WS_SETS) {
+    codens_set_error("codens_workspace_set: set %d is outside 0..%d", set, CODENS_WS_SETS - 1);
+    return CODENS_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  g_ws_set = set;
+  return CODENS_OK;
+}
 
 void *codens_workspace(size_t bytes, int slot) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   std::lock_guard<std::mutex> lk(g_ws_mu);
   if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
-  auto &v = g_ws[dev];
+  if ((int)g_ws[dev].size() <= g_ws_set) g_ws[dev].resize(g_ws_set + 1);
+  auto &v = g_ws[dev][g_ws_set];
   if ((int)v.size() <= slot) v.resize(slot + 1);
   Slot &s = v[slot];
   if (s.n < bytes) {

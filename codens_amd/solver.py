@@ -25,9 +25,11 @@ def _dev(a, dtype=torch.float64):
 
 
 class Solver:
-    def __init__(self, case: Case, stream=None, implicit="fp64"):
+    def __init__(self, case: Case, stream=None, implicit="fp64", streams=1):
         """implicit: "fp64" (implimf_3d as the reference) or "fp32" (its FP32
-        defect-correction form: FP64 residual and update, FP32 sub-iterations)."""
+        defect-correction form: FP64 residual and update, FP32 sub-iterations).
+        streams > 1: the per-block work of each phase runs on that many side
+        streams (up to 16), each block with the workspace set of its stream."""
         if implicit not in ("fp64", "fp32"):
             raise ValueError(f"implicit must be 'fp64' or 'fp32', not {implicit!r}")
         self.implicit = implicit
@@ -70,23 +72,51 @@ class Solver:
         self.cycle = 0
         self.res0 = None
         self.graph = None
+        if not 1 <= streams <= 16:
+            raise ValueError("streams must be 1..16")
+        self.side = [torch.cuda.Stream() for _ in range(streams)] if streams > 1 and len(self.blocks) > 1 else []
+
+    # -- pieces of one stage ------------------------------------------------
+    # -- per-block work, optionally on several streams -----------------------
+    def _per_block(self, fn):
+        """fn(l, b, stream) for every block.  With concurrent streams the
+        blocks go round robin to the side streams, each with its own
+        workspace set, forked from and joined back to the caller's stream:
+        blocks are independent between two ghost exchanges, so small blocks
+        run side by side instead of leaving most of the chip idle."""
+        if not self.side:
+            for l, b in enumerate(self.blocks):
+                fn(l, b, self.stream)
+            return
+        main = self.stream if self.stream is not None else torch.cuda.current_stream()
+        for q in self.side:
+            q.wait_stream(main)
+        try:
+            for l, b in enumerate(self.blocks):
+                k = l % len(self.side)
+                abi.workspace_set(k)
+                fn(l, b, self.side[k])
+        finally:
+            abi.workspace_set(0)
+        for q in self.side:
+            main.wait_stream(q)
 
     # -- pieces of one stage ------------------------------------------------
     def thermo(self):
-        s = self.stream
-        for b in self.blocks:
+        ns = self.case.ns
+
+        def one(l, b, s):
             abi.zpres(b, self.ph, self.v, self.ps, self.temp, self.cson, stream=s)
-        if self.case.ns:
-            for b in self.blocks:
+            if ns:
                 abi.zvismo(b, self.ph, self.temp, self.mu, self.nbad, stream=s)
+        self._per_block(one)
 
     def time_step(self):
-        s, cs = self.stream, self.case
-        for l, b in enumerate(self.blocks):
-            abi.chronos(b, self.ph, cs.eta, cs.ns, self.v, self.mu, self.mut, self.sn_blk[l], self.vol, self.cson,
-                        self.dt, stream=s)
+        cs = self.case
+        self._per_block(lambda l, b, s: abi.chronos(b, self.ph, cs.eta, cs.ns, self.v, self.mu, self.mut,
+                                                    self.sn_blk[l], self.vol, self.cson, self.dt, stream=s))
         if cs.kdtl == 0:
-            return abi.chrono_global(self.blocks, cs.dt1min, self.dt, stream=s)
+            return abi.chrono_global(self.blocks, cs.dt1min, self.dt, stream=self.stream)
         return None
 
     def ghosts(self):
@@ -110,10 +140,12 @@ class Solver:
         diffusive flux densities; their ghost values (met_rbse, met_rbsc);
         per block the upwind dissipation (met_roe2o or met_roe), the flux
         balance u(6:7) (sch_turb), the line-implicit update (impli2_eqt_3d)
-        and the limiters (at_cutke); then the eddy viscosity."""
-        t, s = self.case.turb, self.stream
+        and the limiters (at_cutke), then the eddy viscosity (a block's eddy
+        viscosity reads that block's state only)."""
+        t = self.case.turb
         sa = t.model == "1SA"
-        for l, b in enumerate(self.blocks):
+
+        def sources(l, b, s):
             abi.teq_gradv(b, self.sn_blk[l], self.vol, self.v, self.g9[l], stream=s)
             abi.met_prod(b, self.mut, self.v, self.g9[l], self.tprod[l], stream=s)
             abi.met_gradtr(b, self.sn_blk[l], self.vol, self.v, self.fd, stream=s)
@@ -123,12 +155,15 @@ class Solver:
             else:
                 abi.met_smko(b, t.tc, self.v, self.tprod[l], self.cfke, self.ts6, self.ts7, self.cson, stream=s)
             abi.met_flud(b, t.kfludis, t.tc, self.v, self.mu, self.mut, self.fd, stream=s)
+        self._per_block(sources)
+        s = self.stream
         if self.tq_copy[0].numel():
             abi.met_rbse(self.tq_copy[0], self.tq_copy[1], self.fd, stream=s)
         g, i, r = self.coin
         if g.numel():
             abi.met_rbsc(g, i, r, self.fd, stream=s)
-        for l, b in enumerate(self.blocks):
+
+        def update(l, b, s):
             if t.klroe == 2:
                 abi.met_roe2o(b, self.v, self.d, self.sn_blk[l], stream=s)
             else:
@@ -137,17 +172,17 @@ class Solver:
             abi.impli2_eqt_3d(b, self.u, self.dt, self.v, self.mu, self.mut, self.cfke, self.sn_blk[l], self.vol,
                               t.cmt, stream=s)
             abi.met_cut(b, 2 if sa else 1, t.epsk, t.epse, self.v, stream=s)
-        for l, b in enumerate(self.blocks):
             if sa:
                 abi.met_samut(b, t.tc, self.v, self.mu, self.mut, stream=s)
             else:
                 abi.met_kocmut(b, t.tc, t.kwsst, self.sn_blk[l], self.vol, self.v, self.mu, self.mut, self.dist,
                                self.g9[l], stream=s)
+        self._per_block(update)
 
     def stresses(self):
+        self._per_block(lambda l, b, s: abi.zvisqc(b, self.ph, self.sn_blk[l], self.vol, self.v, self.temp, self.mu,
+                                                   self.mut, self.tq, stream=s))
         s = self.stream
-        for l, b in enumerate(self.blocks):
-            abi.zvisqc(b, self.ph, self.sn_blk[l], self.vol, self.v, self.temp, self.mu, self.mut, self.tq, stream=s)
         if self.tq_copy[0].numel():
             abi.bc_tq_copy(self.tq_copy[0], self.tq_copy[1], self.tq, stream=s)
         g, i, r = self.coin
@@ -155,22 +190,24 @@ class Solver:
             abi.bc_tq_coincident(g, i, r, self.tq, stream=s)
 
     def residual(self):
-        s = self.stream
-        for l, b in enumerate(self.blocks):
+        def one(l, b, s):
             if self.case.ns:
                 abi.sch_roe(b, self.ph, self.sc, 0, self.u, self.v, self.ff, self.tq, self.sn_blk[l], self.ps, stream=s)
             else:
                 abi.sch_roe_euler(b, self.ph, self.sc, 0, self.u, self.v, self.ff, self.sn_blk[l], self.ps, stream=s)
+        self._per_block(one)
 
     def update(self):
-        s, cs = self.stream, self.case
-        for l, b in enumerate(self.blocks):
+        cs = self.case
+
+        def one(l, b, s):
             if cs.kmf == 0:
                 abi.sch_expli(b, self.u, self.v, self.dt, self.vol, stream=s)
             else:
                 fn = abi.implimf_3d if self.implicit == "fp64" else abi.implimf_3d_f32
                 fn(b, self.ph, self.u, self.dt, self.v, self.d, self.ff, self.mu, self.mut, cs.lmax, self.sn_blk[l],
                    self.vol, self.ps, self.cson, stream=s)
+        self._per_block(one)
 
     def stage(self):
         """One residual + update of all blocks (atsch_num.f90:5); a replay of

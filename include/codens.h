@@ -437,6 +437,13 @@ int codens_met_kemut(const codens_block *b, double cmu, int32_t form, int32_t lp
                      const double *vol, double *v, int64_t ldv, const double *mu, double *mut, const double *dist,
                      double *g9, void *stream);
 
+/* Selects one of 16 independent sets of the library's device workspaces for
+ * the entry points called after it (no counterpart in the reference, which
+ * runs blocks one after another).  For callers that run blocks
+ * concurrently on several streams: each stream its own set, so that no two
+ * concurrent calls share scratch memory.  Set 0 is the default. */
+int codens_workspace_set(int32_t set);
+
 /* Tile shape of the tiled Jacobi sub-iteration of codens_implimf_3d,
  * codens_implimf_eu and codens_implimf_3d_f32 (no counterpart in the
  * reference): -1 (default) times the 64 x 8 and 32 x 16 shapes on the first

@@ -1,17 +1,31 @@
 // Viscous stresses and heat flux (zvisqc -> zgrad2 + zfluto) on gfx950.
-// One thread per real cell gathers the Green-Gauss sums of its six faces
-// (instead of the reference's +/- face scatter), scales them by 1/(2 vol) and
-// forms tau and q in registers: the twelve gradient arrays never reach HBM.
+// The velocities (rho u)/rho of every storage cell are formed once (as
+// zgrad2.f90:161-180 forms vx, vy, vz); then one thread per real cell
+// gathers the Green-Gauss sums of its six faces (instead of the reference's
+// +/- face scatter), scales them by 1/(2 vol) and forms tau and q in
+// registers: the twelve gradient arrays never reach HBM.
 #include "common.hpp"
 
 using namespace codens;
 
+__global__ void __launch_bounds__(256) k_vel(long npc, long nnn, const double *__restrict__ v, long ldv,
+                                             double *__restrict__ uvw) {
+  long m = blockIdx.x * 256L + threadIdx.x;
+  if (m >= nnn) return;
+  long n = npc + m;
+  double r = v[n];
+  uvw[m] = v[ldv + n] / r;
+  uvw[nnn + m] = v[2 * ldv + n] / r;
+  uvw[2 * nnn + m] = v[3 * ldv + n] / r;
+}
+
 __global__ void __launch_bounds__(TX *TY) k_zvisqc(Blk b, Phys ph, const double *__restrict__ sn,
                                                    const double *__restrict__ vol,
-                                                   const double *__restrict__ v, long ldv,
+                                                   const double *__restrict__ uvw,
                                                    const double *__restrict__ temp,
                                                    const double *__restrict__ mu,
-                                                   const double *__restrict__ mut, double *__restrict__ tq) {
+                                                   const double *__restrict__ mut, double *__restrict__ tq,
+                                                   long ldv) {
   const dim3 xb = xcd_block();
   int c[3];
   c[0] = b.i1 + xb.x * TX + threadIdx.x;
@@ -19,10 +33,11 @@ __global__ void __launch_bounds__(TX *TY) k_zvisqc(Blk b, Phys ph, const double 
   c[2] = b.k1 + xb.z;
   if (c[0] >= b.i2 || c[1] >= b.j2) return;
   long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+  const long N = b.nnn;
   double qc[4];
-  qc[0] = v[ldv + n] / v[n];
-  qc[1] = v[2 * ldv + n] / v[n];
-  qc[2] = v[3 * ldv + n] / v[n];
+  qc[0] = uvw[m];
+  qc[1] = uvw[N + m];
+  qc[2] = uvw[2 * N + m];
   qc[3] = temp[n];
   double G[12];
 #pragma unroll
@@ -34,8 +49,8 @@ __global__ void __launch_bounds__(TX *TY) k_zvisqc(Blk b, Phys ph, const double 
     int a = c[d], a1 = b.lo(d), a2 = b.hi(d);
     // low face (S at m): subtract
     {
-      long nn = n - inc;
-      double qn[4] = {v[ldv + nn] / v[nn], v[2 * ldv + nn] / v[nn], v[3 * ldv + nn] / v[nn], temp[nn]};
+      long nn = n - inc, mm = m - inc;
+      double qn[4] = {uvw[mm], uvw[N + mm], uvw[2 * N + mm], temp[nn]};
       bool bnd = (a == a1);
 #pragma unroll
       for (int cc = 0; cc < 3; cc++) {
@@ -49,8 +64,8 @@ __global__ void __launch_bounds__(TX *TY) k_zvisqc(Blk b, Phys ph, const double 
     }
     // high face (S at m+inc): add
     {
-      long nn = n + inc;
-      double qn[4] = {v[ldv + nn] / v[nn], v[2 * ldv + nn] / v[nn], v[3 * ldv + nn] / v[nn], temp[nn]};
+      long nn = n + inc, mm = m + inc;
+      double qn[4] = {uvw[mm], uvw[N + mm], uvw[2 * N + mm], temp[nn]};
       bool bnd = (a + 1 == a2);
 #pragma unroll
       for (int cc = 0; cc < 3; cc++) {
@@ -96,10 +111,17 @@ extern "C" int codens_zvisqc(const codens_block *cb, const codens_phys *cp, cons
     return CODENS_EINVAL;
   }
   Blk b = make_blk(cb);
+  double *uvw = (double *)codens_workspace(sizeof(double) * 3 * b.nnn, 4);
+  if (!uvw) {
+    codens_set_error("codens_zvisqc: workspace allocation failed");
+    return CODENS_EHIP;
+  }
   {
     CodensTimed _tm("zvisqc", (hipStream_t)stream);
-    hipLaunchKernelGGL(k_zvisqc, cell_grid(b), dim3(TX, TY), 0, (hipStream_t)stream, b, make_phys(cp), sn, vol, v,
-                     (long)ldv, temp, mu, mut, tq);
+    hipLaunchKernelGGL(k_vel, dim3((unsigned)((b.nnn + 255) / 256)), dim3(256), 0, (hipStream_t)stream, (long)b.npc,
+                       (long)b.nnn, v, (long)ldv, uvw);
+    hipLaunchKernelGGL(k_zvisqc, cell_grid(b), dim3(TX, TY), 0, (hipStream_t)stream, b, make_phys(cp), sn, vol,
+                       (const double *)uvw, temp, mu, mut, tq, (long)ldv);
   }
   return codens_check_launch("codens_zvisqc");
 }

@@ -75,15 +75,25 @@ class Solver:
         if not 1 <= streams <= 16:
             raise ValueError("streams must be 1..16")
         self.side = [torch.cuda.Stream() for _ in range(streams)] if streams > 1 and len(self.blocks) > 1 else []
+        # blocks to streams: largest first, each to the least-loaded stream
+        # (longest-processing-time order), so that the streams finish together
+        self.lane = [0] * len(self.blocks)
+        if self.side:
+            load = [0] * len(self.side)
+            for l in sorted(range(len(self.blocks)), key=lambda x: -lay.blocks[x].ncells):
+                k = min(range(len(load)), key=lambda x: load[x])
+                self.lane[l] = k
+                load[k] += lay.blocks[l].ncells
 
     # -- pieces of one stage ------------------------------------------------
     # -- per-block work, optionally on several streams -----------------------
     def _per_block(self, fn):
-        """fn(l, b, stream) for every block.  With concurrent streams the
-        blocks go round robin to the side streams, each with its own
-        workspace set, forked from and joined back to the caller's stream:
-        blocks are independent between two ghost exchanges, so small blocks
-        run side by side instead of leaving most of the chip idle."""
+        """fn(l, b, stream) for every block.  With concurrent streams each
+        block goes to its side stream (self.lane), with that stream's
+        workspace set; the side streams fork from and join back to the
+        caller's stream: blocks are independent between two ghost exchanges,
+        so small blocks run side by side instead of leaving most of the chip
+        idle."""
         if not self.side:
             for l, b in enumerate(self.blocks):
                 fn(l, b, self.stream)
@@ -93,7 +103,7 @@ class Solver:
             q.wait_stream(main)
         try:
             for l, b in enumerate(self.blocks):
-                k = l % len(self.side)
+                k = self.lane[l]
                 abi.workspace_set(k)
                 fn(l, b, self.side[k])
         finally:

@@ -12,6 +12,12 @@ adjacency tree, Spalart-Allmaras RANS at Re=5e6, M=0.7, the same implicit
 phase (codens_amd.configs.multiblock_sa); stages captured in a HIP graph
 (--graph, on by default for this workload).
 
+With --workload adjoint: the gradient of the wall force of one 4-stage RK
+step on the 128x64x32 viscous block with respect to the state and 4,096
+wall-node displacements (codens_amd.adjoint): `value` is real cells x RK
+stages / the time of one whole gradient (forward step with its
+checkpoints, reverse sweep, mesh adjoint), next to the forward step alone.
+
 Prints one JSON line: Mcell-updates/s, the roofline of the dominant kernel
 (implimf_sweep: 264 algorithmic bytes per real cell per launch) measured with
 HIP events on the launch stream, and the reference Fortran build timed on the
@@ -67,7 +73,7 @@ def main():
     ap.add_argument("--cpu-stages", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dims", type=str, default="256,128,64")
-    ap.add_argument("--workload", choices=("wall", "multiblock"), default="wall",
+    ap.add_argument("--workload", choices=("wall", "multiblock", "adjoint"), default="wall",
                     help="wall: the 256x128x64 laminar wall block; multiblock: 48 uneven SA blocks, ~40M cells")
     ap.add_argument("--streams", type=int, default=0,
                     help="side streams for the per-block work (0: 4 for multiblock, 1 otherwise)")
@@ -87,6 +93,8 @@ def main():
     from codens_amd import abi, configs
     from codens_amd.solver import Solver
 
+    if args.workload == "adjoint":
+        return adjoint_bench(args)
     multi = args.workload == "multiblock"
     ci, cj, ck = (int(x) for x in args.dims.split(","))
     case = configs.multiblock_sa() if multi else configs.wall_ns(ci, cj, ck)
@@ -174,6 +182,58 @@ def main():
                           "same physics)")
         else:
             out["cpu_baseline"] = cpu_baseline(case, args.cpu_stages)
+    print(json.dumps(out))
+
+
+def adjoint_bench(args):
+    import torch
+    from codens_amd import abi, configs
+    from codens_amd.adjoint import AdjointProblem, gradient, RK_ALPHA
+    case = configs.adjoint_wall(128, 64, 32)
+    P = AdjointProblem(case)
+    v0 = torch.as_tensor(case.v0, device="cuda")
+    delta = torch.zeros(len(case.wall_nodes), dtype=torch.float64, device="cuda")
+    ncell = case.layout.ncells
+    for _ in range(args.warmup):
+        gradient(P, v0, delta)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gradient(P, v0, delta)
+    torch.cuda.synchronize()
+    tg = (time.perf_counter() - t0) / args.steps
+    b = case.layout.blocks[0]
+    sn = torch.as_tensor(case.sn[9 * b.npc: 9 * (b.npc + b.nnn)], device="cuda")
+    vol = torch.as_tensor(case.vol, device="cuda")
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        for _ in range(args.steps):
+            P.objective(P.rk_step(v0, sn, vol), sn)
+    torch.cuda.synchronize()
+    tf = (time.perf_counter() - t0) / args.steps
+    # per-entry-point times with HIP events around single calls
+    ub = torch.ones((5, case.layout.ip11), dtype=torch.float64, device="cuda") * P.real
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+    ev[0].record(); P.residual(v0, sn, vol); ev[1].record()
+    P.jvp(v0, sn, vol, v0, sn, vol); ev[2].record()
+    P.vjp(v0, sn, vol, ub); ev[3].record()
+    torch.cuda.synchronize()
+    nst = len(RK_ALPHA)
+    out = {
+        "metric": "Mcell-updates/s per implicit iter to 1e-8 residual drop; % of HBM roofline",
+        "value": round(ncell * nst / tg / 1e6, 3), "unit": "Mcell-updates/s (RK stages of the gradient)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * tg, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "adjoint_wall 128x64x32 laminar NS, Roe + unlimited MUSCL + viscous, 4-stage RK step, "
+                               "gradient of the wall x force w.r.t. the state and 4096 wall-node displacements",
+                   "cells": ncell, "rk_stages": nst},
+        "forward_step_ms": round(1e3 * tf, 3),
+        "note": "the gradient's time includes the mesh metrics of the displaced nodes on the host (numpy)",
+        "entry_points_ms": {"residual (production)": round(ev[0].elapsed_time(ev[1]), 3),
+                            "codens_adj_residual_jvp": round(ev[1].elapsed_time(ev[2]), 3),
+                            "codens_adj_residual_vjp": round(ev[2].elapsed_time(ev[3]), 3)},
+        "roofline": None,
+    }
     print(json.dumps(out))
 
 

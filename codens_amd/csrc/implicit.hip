@@ -70,6 +70,55 @@ __device__ inline DFluxT<T> dflux(const Phys &ph, const T V[5], const T D[5]) {
   return r;
 }
 
+// The same increment dF = F(v + d) - F(v) in difference form, for the FP32
+// sub-iterations: every term is a product with a change (dm - q drho)/(rho +
+// drho) of a velocity, of the kinetic energy or of the pressure, so that its
+// rounding is relative to |d|.  The form above subtracts two fluxes of size
+// |F|, whose rounding in FP32 (6e-8 |F|) does not shrink with d and would
+// hold the FP64 residual of the defect correction at about 1e-7 of its
+// start.  Algebraically the same increment.
+template <typename T>
+__device__ inline DFluxT<T> dflux_delta(const Phys &ph, const T V[5], const T D[5]) {
+  // no FMA contraction: the same rounding in every tile shape's kernel
+#pragma clang fp contract(off)
+  const T gam1 = (T)ph.gam1, pinfl = (T)ph.pinfl;
+  T r0 = T(1) / V[0], rn = T(1) / (V[0] + D[0]);
+  T u = V[1] * r0, v = V[2] * r0, w = V[3] * r0;
+  T du = (D[1] - u * D[0]) * rn, dv = (D[2] - v * D[0]) * rn, dw = (D[3] - w * D[0]) * rn;
+  T rho1 = V[0] + D[0];
+  T q2 = u * u + v * v + w * w;
+  T dq2 = T(2) * (u * du + v * dv + w * dw) + (du * du + dv * dv + dw * dw);
+  T dk = T(0.5) * (D[0] * q2 + rho1 * dq2);                // change of rho |u|^2 / 2
+  T p = gam1 * (V[4] - T(0.5) * V[0] * q2 - pinfl);
+  T dp = gam1 * (D[4] - dk);
+  // change of rho q_a q_b
+  auto dprod = [&](T qa, T qb, T dqa, T dqb) {
+#pragma clang fp contract(off)
+    return D[0] * qa * qb + rho1 * (qa * dqb + dqa * qb + dqa * dqb);
+  };
+  T h0 = V[4] + p - pinfl, dh = D[4] + dp;                  // E + p - pinfl and its change
+  DFluxT<T> r;
+  r.mx = D[1]; r.my = D[2]; r.mz = D[3];
+  r.xx = dprod(u, u, du, du) + dp;
+  r.xy = dprod(u, v, du, dv);
+  r.xz = dprod(u, w, du, dw);
+  r.yy = dprod(v, v, dv, dv) + dp;
+  r.yz = dprod(v, w, dv, dw);
+  r.zz = dprod(w, w, dw, dw) + dp;
+  r.ex = dh * (u + du) + h0 * du;
+  r.ey = dh * (v + dv) + h0 * dv;
+  r.ez = dh * (w + dw) + h0 * dw;
+  return r;
+}
+
+// dF of the sub-iterations: the reference's form in FP64, the difference
+// form in FP32
+template <typename T>
+__device__ inline DFluxT<T> dflux_sweep(const Phys &ph, const T V[5], const T D[5]) {
+  if constexpr (sizeof(T) == sizeof(float)) return dflux_delta<T>(ph, V, D);
+  else return dflux<T>(ph, V, D);
+}
+
 // implimf_eu.f90:289-312 for one cell: dF from the momenta with
 // wi1 = 1/(rho + d1), the base flux with the caller's pressure p, and the z
 // energy increment without its w factor, as the reference has it.
@@ -242,7 +291,7 @@ __device__ inline void q_of(const Phys &ph, const T *__restrict__ v, long ldv, c
     V[e] = v[e * ldv + n];
     D[e] = d.at(e, n);
   }
-  DFluxT<T> F = dflux<T>(ph, V, D);
+  DFluxT<T> F = dflux_sweep<T>(ph, V, D);
   Q.q[0] = F.mx; Q.q[1] = F.my; Q.q[2] = F.mz;
   Q.q[3] = F.xx; Q.q[4] = F.xy; Q.q[5] = F.xz; Q.q[6] = F.yy; Q.q[7] = F.yz; Q.q[8] = F.zz;
   Q.q[9] = F.ex; Q.q[10] = F.ey; Q.q[11] = F.ez;
@@ -306,7 +355,7 @@ __device__ inline void vd_q(const Phys &ph, const VD<T> &x, bool real, Q14T<T> &
   }
   DFluxT<T> F;
   if constexpr (EU) F = dflux_eu(ph, x.V, x.D, x.p);
-  else F = dflux<T>(ph, x.V, x.D);
+  else F = dflux_sweep<T>(ph, x.V, x.D);
   Q.q[0] = F.mx; Q.q[1] = F.my; Q.q[2] = F.mz;
   Q.q[3] = F.xx; Q.q[4] = F.xy; Q.q[5] = F.xz; Q.q[6] = F.yy; Q.q[7] = F.yz; Q.q[8] = F.zz;
   Q.q[9] = F.ex; Q.q[10] = F.ey; Q.q[11] = F.ez;

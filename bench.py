@@ -18,6 +18,11 @@ wall-node displacements (codens_amd.adjoint): `value` is real cells x RK
 stages / the time of one whole gradient (forward step with its
 checkpoints, reverse sweep, mesh adjoint), next to the forward step alone.
 
+With --workload converge: iterations and wall time to a 1e-8 density-
+residual drop on the inclined-free-stream wall block 32x16x8 (the case of
+tests/golden/converged_wall_32x16x8.npz), FP64 implicit phase and FP32
+defect correction; `value` is the FP32 run's cell-updates/s to the drop.
+
 Prints one JSON line: Mcell-updates/s, the roofline of the dominant kernel
 (implimf_sweep: 264 algorithmic bytes per real cell per launch) measured with
 HIP events on the launch stream, and the reference Fortran build timed on the
@@ -73,7 +78,7 @@ def main():
     ap.add_argument("--cpu-stages", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dims", type=str, default="256,128,64")
-    ap.add_argument("--workload", choices=("wall", "multiblock", "adjoint"), default="wall",
+    ap.add_argument("--workload", choices=("wall", "multiblock", "adjoint", "converge"), default="wall",
                     help="wall: the 256x128x64 laminar wall block; multiblock: 48 uneven SA blocks, ~40M cells")
     ap.add_argument("--streams", type=int, default=0,
                     help="side streams for the per-block work (0: 4 for multiblock, 1 otherwise)")
@@ -95,6 +100,8 @@ def main():
 
     if args.workload == "adjoint":
         return adjoint_bench(args)
+    if args.workload == "converge":
+        return converge_bench(args)
     multi = args.workload == "multiblock"
     ci, cj, ck = (int(x) for x in args.dims.split(","))
     case = configs.multiblock_sa() if multi else configs.wall_ns(ci, cj, ck)
@@ -182,6 +189,37 @@ def main():
                           "same physics)")
         else:
             out["cpu_baseline"] = cpu_baseline(case, args.cpu_stages)
+    print(json.dumps(out))
+
+
+def converge_bench(args):
+    import torch
+    from codens_amd import configs
+    from codens_amd.solver import Solver
+    case = configs.adjoint_wall(32, 16, 8, alpha=10.0, beta=5.0, eta=10.0, amp=0.0, bump=0.0)
+    ncell = case.layout.ncells
+    runs = {}
+    for mode in ("fp64", "fp32"):
+        s = Solver(case, implicit=mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it, hist = s.run_to(drop=1e-8, max_iter=60000, check_every=250)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        runs[mode] = {"iterations": it, "seconds": round(dt, 3), "drop": hist[-1][1],
+                      "Mcell_updates_per_s": round(ncell * it / dt / 1e6, 3)}
+    out = {
+        "metric": "Mcell-updates/s per implicit iter to 1e-8 residual drop; % of HBM roofline",
+        "value": runs["fp32"]["Mcell_updates_per_s"], "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": runs["fp32"]["iterations"], "warmup": 0,
+        "ms_per_step": round(1e3 * runs["fp32"]["seconds"] / runs["fp32"]["iterations"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 residual/update, f32 implicit", "data": "synthetic",
+        "config": {"workload": "wall block 32x16x8, free stream inclined 10 and 5 degrees, laminar NS, unlimited MUSCL, "
+                               "implimf_3d lmax=10, CFL 10, run to a 1e-8 density-residual drop (the residual check "
+                               "every 250 iterations is inside the timing)", "cells": ncell},
+        "to_1e-8": runs, "roofline": None,
+    }
     print(json.dumps(out))
 
 

@@ -399,3 +399,17 @@ def met_smklsas(b, kc, ischema, sn_blk, vol, v, mu, mut, dist, fd, g9, tprod, cf
 def workspace_set(k):
     """Select workspace set k (0..15) for the following calls."""
     _chk(lib.codens_workspace_set(int(k)))
+
+
+def block_array(blocks):
+    """ctypes array of codens_block records (for the batched entry points)."""
+    arr = (CodensBlock * len(blocks))()
+    for k, b in enumerate(blocks):
+        arr[k] = b
+    return arr
+
+
+def sch_roe_batch(blocks_arr, nblk, ph, sc, ityprk, u, v, ff, tq, sn_all, ps, stream=None):
+    """sch_roe.f90:4 for all blocks in one launch per pass."""
+    _chk(lib.codens_sch_roe_batch(int(nblk), blocks_arr, ctypes.byref(ph), ctypes.byref(sc), int(ityprk), _p(u), _p(v),
+                                  ld(v), _p(ff), _p(tq), _p(sn_all), _p(ps), _s(stream)))

@@ -20,6 +20,11 @@
 // operation).
 #include "common.hpp"
 
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
 using namespace codens;
 
 struct Prim {
@@ -41,15 +46,20 @@ __device__ inline Prim prim_at(const double *__restrict__ v, const double *__res
 }
 
 // pass 0: uvw(e-1, m) = v(e, n) / v(1, n), e = 2..4, over the block's storage
-__global__ void __launch_bounds__(256) k_prim_uvw(Blk b, const double *__restrict__ v, long ldv,
-                                                  double *__restrict__ uvw) {
-  long m = blockIdx.x * 256L + threadIdx.x;
+__device__ __forceinline__ void prim_uvw_body(const Blk &b, unsigned bx, const double *__restrict__ v, long ldv,
+                                              double *__restrict__ uvw) {
+  long m = bx * 256L + threadIdx.x;
   if (m >= b.nnn) return;
   long n = b.npc + m;
   double r = v[n];
   uvw[m] = v[ldv + n] / r;
   uvw[b.nnn + m] = v[2 * ldv + n] / r;
   uvw[2 * b.nnn + m] = v[3 * ldv + n] / r;
+}
+
+__global__ void __launch_bounds__(256) k_prim_uvw(Blk b, const double *__restrict__ v, long ldv,
+                                                  double *__restrict__ uvw) {
+  prim_uvw_body(b, blockIdx.x, v, ldv, uvw);
 }
 
 __device__ inline double pget(const Prim &q, int e) {
@@ -186,13 +196,13 @@ __device__ inline long chain_line_base(const Blk &b, int d, long line) {
 // blockIdx.y: variable, blockIdx.z: direction (1 + z).
 constexpr int CH_B = 8;
 
-__global__ void __launch_bounds__(64) k_limiter_chain_jk(Blk b, const double *__restrict__ v,
-                                                         const double *__restrict__ uvw,
-                                                         const double *__restrict__ ps, double *__restrict__ R) {
-  const int d = 1 + blockIdx.z, e = blockIdx.y;
+__device__ __forceinline__ void chain_jk_body(const Blk &b, dim3 bi, const double *__restrict__ v,
+                                              const double *__restrict__ uvw, const double *__restrict__ ps,
+                                              double *__restrict__ R) {
+  const int d = 1 + bi.z, e = bi.y;
   int ta = 0, tb = d == 2 ? 1 : 2;
   long nl = (long)(b.hi(ta) - b.lo(ta)) * (b.hi(tb) - b.lo(tb));
-  long line = (long)blockIdx.x * CH_LINES + threadIdx.x;
+  long line = (long)bi.x * CH_LINES + threadIdx.x;
   if (line >= nl) return;
   int a1 = b.lo(d), a2 = b.hi(d);
   long inc = b.inc(d);
@@ -232,6 +242,12 @@ __global__ void __launch_bounds__(64) k_limiter_chain_jk(Blk b, const double *__
   }
 }
 
+__global__ void __launch_bounds__(64) k_limiter_chain_jk(Blk b, const double *__restrict__ v,
+                                                         const double *__restrict__ uvw,
+                                                         const double *__restrict__ ps, double *__restrict__ R) {
+  chain_jk_body(b, dim3(blockIdx.x, blockIdx.y, blockIdx.z), v, uvw, ps, R);
+}
+
 // Direction i: a lane's line is contiguous in memory, so lanes walking their
 // own lines would touch 64 cache lines per access.  The wave instead moves
 // chunks of CH_C cells of its 64 lines through LDS: coalesced loads (each
@@ -241,14 +257,14 @@ __global__ void __launch_bounds__(64) k_limiter_chain_jk(Blk b, const double *__
 // blockIdx.y: variable.
 constexpr int CH_C = 32;
 
-__global__ void __launch_bounds__(64) k_limiter_chain_i(Blk b, const double *__restrict__ v,
-                                                        const double *__restrict__ uvw,
-                                                        const double *__restrict__ ps, double *__restrict__ R) {
+__device__ __forceinline__ void chain_i_body(const Blk &b, dim3 bi, const double *__restrict__ v,
+                                             const double *__restrict__ uvw, const double *__restrict__ ps,
+                                             double *__restrict__ R) {
   __shared__ double qs[CH_LINES * (CH_C + 1)];
   __shared__ long base[CH_LINES];
-  const int e = blockIdx.y, lane = threadIdx.x;
+  const int e = bi.y, lane = threadIdx.x;
   long nl = (long)(b.j2 - b.j1) * (b.k2 - b.k1);
-  long line0 = (long)blockIdx.x * CH_LINES;
+  long line0 = (long)bi.x * CH_LINES;
   int nlines = (int)min((long)CH_LINES, nl - line0);
   int a1 = b.i1, a2 = b.i2;
   const double *q = chain_q(b, v, uvw, ps, e);
@@ -301,6 +317,12 @@ __global__ void __launch_bounds__(64) k_limiter_chain_i(Blk b, const double *__r
   }
 }
 
+__global__ void __launch_bounds__(64) k_limiter_chain_i(Blk b, const double *__restrict__ v,
+                                                        const double *__restrict__ uvw,
+                                                        const double *__restrict__ ps, double *__restrict__ R) {
+  chain_i_body(b, dim3(blockIdx.x, blockIdx.y, 1), v, uvw, ps, R);
+}
+
 // physical flux . S of a ghost state (sch_roe.f90:497-578)
 __device__ inline void ghost_flux(const Phys &ph, const double w[5], double p, const double T[9], double rdiv,
                                   int euler, double s0, double s1, double s2, double f[5]) {
@@ -340,14 +362,11 @@ __device__ inline void ghost_flux(const Phys &ph, const double w[5], double p, c
 // (sch_roe.f90:497-578), else 0.5 x the Roe flux of the MUSCL states
 // (sch_roe.f90:196-474).  H[e*nnn + m] holds the face whose storage index is
 // m (cells m-inc | m).
-__global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(CODENS_FF_WAVES))) k_face_flux(Blk b, Phys ph, int d, int ns, int ilim, double muscl,
-                                                      double xk, const double *__restrict__ v, long ldv,
-                                                      const double *__restrict__ ps,
-                                                      const double *__restrict__ tq,
-                                                      const double *__restrict__ sn,
-                                                      const double *__restrict__ uvw,
-                                                      const double *__restrict__ rr, double *__restrict__ H) {
-  const dim3 xb = xcd_block();
+__device__ __forceinline__ void face_flux_body(const Blk &b, dim3 xb, const Phys &ph, int d, int ns, int ilim,
+                                               double muscl, double xk, const double *__restrict__ v, long ldv,
+                                               const double *__restrict__ ps, const double *__restrict__ tq,
+                                               const double *__restrict__ sn, const double *__restrict__ uvw,
+                                               const double *__restrict__ rr, double *__restrict__ H) {
   int c[3];
   c[0] = b.i1 + xb.x * TX + threadIdx.x;
   c[1] = b.j1 + xb.y * TY + threadIdx.y;
@@ -404,14 +423,24 @@ __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(COD
   for (int e = 0; e < 5; e++) H[e * b.nnn + m] = f[e];
 }
 
+__global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(CODENS_FF_WAVES))) k_face_flux(Blk b, Phys ph, int d, int ns, int ilim, double muscl,
+                                                      double xk, const double *__restrict__ v, long ldv,
+                                                      const double *__restrict__ ps,
+                                                      const double *__restrict__ tq,
+                                                      const double *__restrict__ sn,
+                                                      const double *__restrict__ uvw,
+                                                      const double *__restrict__ rr, double *__restrict__ H) {
+  face_flux_body(b, xcd_block(), ph, d, ns, ilim, muscl, xk, v, ldv, ps, tq, sn, uvw, rr, H);
+}
+
 // ---------------------------------------------------------------------------
 // pass 3: gather over the zeroed range indc(i1-1,j1-1,k1-1)..indc(i2,j2,k2)
 // (sch_roe.f90:113-125, 448-474 regrouped per cell):
 // u = (H_i(hi) - H_i(lo)) + (H_j(hi) - H_j(lo)) + (H_k(hi) - H_k(lo)) on
 // real cells, 0 elsewhere; ff -= u when ityprk != 0.
-__global__ void __launch_bounds__(256) k_gather(Blk b, long n0, long n1, const double *__restrict__ H, double *u,
-                                                long ldv, double *ff, int ityprk) {
-  long n = n0 + blockIdx.x * 256L + threadIdx.x;
+__device__ __forceinline__ void gather_body(const Blk &b, unsigned bx, long n0, long n1, const double *__restrict__ H,
+                                            double *u, long ldv, double *ff, int ityprk) {
+  long n = n0 + bx * 256L + threadIdx.x;
   if (n > n1) return;
   long m = n - b.npc;
   int i = b.id1 + (int)(m % b.nid);
@@ -432,6 +461,11 @@ __global__ void __launch_bounds__(256) k_gather(Blk b, long n0, long n1, const d
     u[e * ldv + n] = r;
     if (ityprk) ff[e * ldv + n] = ff[e * ldv + n] - r;
   }
+}
+
+__global__ void __launch_bounds__(256) k_gather(Blk b, long n0, long n1, const double *__restrict__ H, double *u,
+                                                long ldv, double *ff, int ityprk) {
+  gather_body(b, blockIdx.x, n0, n1, H, u, ldv, ff, ityprk);
 }
 
 static int roe_common(const codens_block *cb, const codens_phys *cp, const codens_scheme *sc, int ns,
@@ -494,4 +528,199 @@ extern "C" int codens_sch_roe(const codens_block *b, const codens_phys *ph, cons
                               const double *tq, const double *sn, const double *ps, void *stream) {
   return roe_common(b, ph, sc, 1, ityprk, u, v, (long)ldv, ff, tq, sn, ps, (hipStream_t)stream,
                     "codens_sch_roe");
+}
+
+// ---------------------------------------------------------------------------
+// All blocks of a layout in one launch per pass (the residual of a
+// multi-block stage in seven launches instead of seven per block).  A device
+// table gives every block its extents and, for each pass, the first
+// workgroup of the block in the flattened launch and the block's own grid
+// (x, y; z follows); a workgroup finds its block by bisection over the
+// first-workgroup column and runs the pass's body on the block's tile.
+// The passes' workspaces are the per-block ones laid end to end: block l's
+// part starts at its npc (times the planes).
+enum { BP_PRIM, BP_CHAIN_I, BP_CHAIN_JK, BP_FACE0, BP_FACE1, BP_FACE2, BP_GATHER, BP_N };
+
+struct BatchEnt {
+  Blk b;
+  long n0, n1;                  // gather range
+  unsigned start[BP_N];         // first workgroup of this block, per pass
+  unsigned gx[BP_N], gy[BP_N];  // this block's grid, per pass
+};
+
+__device__ __forceinline__ int batch_find(const BatchEnt *__restrict__ tab, int nb, int pass, unsigned wg) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].start[pass] <= wg) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ dim3 batch_tile(const BatchEnt &E, int pass, unsigned wg) {
+  unsigned loc = wg - E.start[pass], gx = E.gx[pass], gy = E.gy[pass];
+  return dim3(loc % gx, (loc / gx) % gy, loc / (gx * gy));
+}
+
+// workgroup order remapped so that each XCD owns a contiguous run of the
+// flattened launch (neighbouring tiles of a block share its L2)
+__device__ __forceinline__ unsigned xcd_linear() {
+  unsigned nwg = gridDim.x, hw = blockIdx.x, x = hw % 8u, q = nwg / 8u, r = nwg % 8u;
+  return x * q + (x < r ? x : r) + hw / 8u;
+}
+
+__global__ void __launch_bounds__(256) k_prim_uvw_batch(const BatchEnt *__restrict__ tab, int nb,
+                                                        const double *__restrict__ v, long ldv, double *uvw) {
+  const BatchEnt &E = tab[batch_find(tab, nb, BP_PRIM, blockIdx.x)];
+  prim_uvw_body(E.b, blockIdx.x - E.start[BP_PRIM], v, ldv, uvw + 3 * E.b.npc);
+}
+
+__global__ void __launch_bounds__(64) k_chain_i_batch(const BatchEnt *__restrict__ tab, int nb,
+                                                      const double *__restrict__ v, const double *__restrict__ uvw,
+                                                      const double *__restrict__ ps, double *R) {
+  const BatchEnt &E = tab[batch_find(tab, nb, BP_CHAIN_I, blockIdx.x)];
+  chain_i_body(E.b, batch_tile(E, BP_CHAIN_I, blockIdx.x), v, uvw + 3 * E.b.npc, ps, R + 15 * E.b.npc);
+}
+
+__global__ void __launch_bounds__(64) k_chain_jk_batch(const BatchEnt *__restrict__ tab, int nb,
+                                                       const double *__restrict__ v, const double *__restrict__ uvw,
+                                                       const double *__restrict__ ps, double *R) {
+  const BatchEnt &E = tab[batch_find(tab, nb, BP_CHAIN_JK, blockIdx.x)];
+  chain_jk_body(E.b, batch_tile(E, BP_CHAIN_JK, blockIdx.x), v, uvw + 3 * E.b.npc, ps, R + 15 * E.b.npc);
+}
+
+__global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(CODENS_FF_WAVES)))
+k_face_flux_batch(const BatchEnt *__restrict__ tab, int nb, Phys ph, int d, int ns, int ilim, double muscl, double xk,
+                  const double *__restrict__ v, long ldv, const double *__restrict__ ps,
+                  const double *__restrict__ tq, const double *__restrict__ sn_all, const double *__restrict__ uvw,
+                  const double *__restrict__ rr, double *H) {
+  const unsigned wg = xcd_linear();
+  const int pass = BP_FACE0 + d;
+  const BatchEnt &E = tab[batch_find(tab, nb, pass, wg)];
+  const long np = E.b.npc;
+  face_flux_body(E.b, batch_tile(E, pass, wg), ph, d, ns, ilim, muscl, xk, v, ldv, ps, tq, sn_all + 9 * np,
+                 uvw + 3 * np, ilim == 1 ? rr + 15 * np + 5 * E.b.nnn * d : nullptr, H + 15 * np + 5 * E.b.nnn * d);
+}
+
+__global__ void __launch_bounds__(256) k_gather_batch(const BatchEnt *__restrict__ tab, int nb,
+                                                      const double *__restrict__ H, double *u, long ldv, double *ff,
+                                                      int ityprk) {
+  const BatchEnt &E = tab[batch_find(tab, nb, BP_GATHER, blockIdx.x)];
+  gather_body(E.b, blockIdx.x - E.start[BP_GATHER], E.n0, E.n1, H + 15 * E.b.npc, u, ldv, ff, ityprk);
+}
+
+namespace {
+std::mutex g_batch_mu;
+// device tables by (device, block extents); made once, kept to the end
+std::map<std::pair<int, std::vector<int32_t>>, BatchEnt *> g_batch_tab;
+}  // namespace
+
+extern "C" int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, const codens_phys *cp,
+                                    const codens_scheme *sc, int32_t ityprk, double *u, const double *v,
+                                    int64_t ldv, double *ff, const double *tq, const double *sn_all,
+                                    const double *ps, void *stream) {
+  const char *what = "codens_sch_roe_batch";
+  if (nblk <= 0 || !blocks || !cp || !sc || !u || !v || !tq || !sn_all || !ps || (ityprk && !ff)) {
+    codens_set_error("%s: null or invalid argument", what);
+    return CODENS_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<BatchEnt> ent(nblk);
+  std::vector<int32_t> key;
+  unsigned tot[BP_N] = {0};
+  long lo = 0, hi = 0;
+  const int ilim = sc->ilim;
+  for (int l = 0; l < nblk; l++) {
+    int rc = codens_check_block(&blocks[l], what);
+    if (rc) return rc;
+    const int32_t *w = (const int32_t *)&blocks[l];
+    key.insert(key.end(), w, w + sizeof(codens_block) / sizeof(int32_t));
+    Blk b = make_blk(&blocks[l]);
+    if (l == 0 || b.npc < lo) lo = b.npc;
+    if (l == 0 || b.npc + b.nnn > hi) hi = b.npc + b.nnn;
+    BatchEnt &E = ent[l];
+    E.b = b;
+    E.n0 = b.idx(b.i1 - 1, b.j1 - 1, b.k1 - 1);
+    E.n1 = b.idx(b.i2, b.j2, b.k2);
+    long nl0 = (long)(b.j2 - b.j1) * (b.k2 - b.k1);
+    long nl12 = (long)(b.i2 - b.i1) * max(b.k2 - b.k1, b.j2 - b.j1);
+    unsigned g[BP_N][3] = {{(unsigned)((b.nnn + 255) / 256), 1, 1},
+                           {(unsigned)((nl0 + CH_LINES - 1) / CH_LINES), 5, 1},
+                           {(unsigned)((nl12 + CH_LINES - 1) / CH_LINES), 5, 2},
+                           {0, 0, 0}, {0, 0, 0}, {0, 0, 0},
+                           {(unsigned)((E.n1 - E.n0 + 256) / 256), 1, 1}};
+    for (int d = 0; d < 3; d++) {
+      int nx = b.i2 - b.i1 + (d == 0), ny = b.j2 - b.j1 + (d == 1), nz = b.k2 - b.k1 + (d == 2);
+      g[BP_FACE0 + d][0] = (nx + TX - 1) / TX;
+      g[BP_FACE0 + d][1] = (ny + TY - 1) / TY;
+      g[BP_FACE0 + d][2] = nz;
+    }
+    for (int p = 0; p < BP_N; p++) {
+      E.start[p] = tot[p];
+      E.gx[p] = g[p][0];
+      E.gy[p] = g[p][1];
+      tot[p] += g[p][0] * g[p][1] * g[p][2];
+    }
+  }
+  if (lo != 0 || hi > ldv) {
+    codens_set_error("%s: the blocks must start at entry 0 and lie within ldv", what);
+    return CODENS_EINVAL;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return CODENS_EHIP;
+  BatchEnt *tab = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_batch_mu);
+    auto k = std::make_pair(dev, key);
+    auto it = g_batch_tab.find(k);
+    if (it != g_batch_tab.end()) {
+      tab = it->second;
+    } else {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+        codens_set_error("%s: first call for these blocks while the stream is captured; call it once before", what);
+        return CODENS_EINVAL;
+      }
+      if (hipMalloc(&tab, sizeof(BatchEnt) * nblk) != hipSuccess ||
+          hipMemcpy(tab, ent.data(), sizeof(BatchEnt) * nblk, hipMemcpyHostToDevice) != hipSuccess) {
+        codens_set_error("%s: block table upload failed", what);
+        return CODENS_EHIP;
+      }
+      g_batch_tab[k] = tab;
+    }
+  }
+  Phys ph = make_phys(cp);
+  const long tot_n = hi;
+  double *H = (double *)codens_workspace(sizeof(double) * 15 * tot_n, 0);
+  double *R = ilim == 1 ? (double *)codens_workspace(sizeof(double) * 15 * tot_n, 1) : nullptr;
+  double *uvw = (double *)codens_workspace(sizeof(double) * 3 * tot_n, 4);
+  if (!H || !uvw || (ilim == 1 && !R)) {
+    codens_set_error("%s: workspace allocation failed", what);
+    return CODENS_EHIP;
+  }
+  {
+    CodensTimed _tm("roe_prim", st);
+    hipLaunchKernelGGL(k_prim_uvw_batch, dim3(tot[BP_PRIM]), dim3(256), 0, st, (const BatchEnt *)tab, (int)nblk, v,
+                       (long)ldv, uvw);
+  }
+  if (ilim == 1) {
+    CodensTimed _tm("roe_limiter_chain", st);
+    hipLaunchKernelGGL(k_chain_i_batch, dim3(tot[BP_CHAIN_I]), dim3(64), 0, st, (const BatchEnt *)tab, (int)nblk, v,
+                       (const double *)uvw, ps, R);
+    hipLaunchKernelGGL(k_chain_jk_batch, dim3(tot[BP_CHAIN_JK]), dim3(64), 0, st, (const BatchEnt *)tab, (int)nblk,
+                       v, (const double *)uvw, ps, R);
+  }
+  for (int d = 0; d < 3; d++) {
+    CodensTimed _tm("roe_face_flux", st);
+    hipLaunchKernelGGL(k_face_flux_batch, dim3(tot[BP_FACE0 + d]), dim3(TX, TY), 0, st, (const BatchEnt *)tab,
+                       (int)nblk, ph, d, 1, ilim, (double)sc->muscl, sc->xk, v, (long)ldv, ps, tq, sn_all,
+                       (const double *)uvw, (const double *)R, H);
+  }
+  {
+    CodensTimed _tm("roe_gather", st);
+    hipLaunchKernelGGL(k_gather_batch, dim3(tot[BP_GATHER]), dim3(256), 0, st, (const BatchEnt *)tab, (int)nblk,
+                       (const double *)H, u, (long)ldv, ff, (int)ityprk);
+  }
+  return codens_check_launch(what);
 }

@@ -25,11 +25,13 @@ def _dev(a, dtype=torch.float64):
 
 
 class Solver:
-    def __init__(self, case: Case, stream=None, implicit="fp64", streams=1):
+    def __init__(self, case: Case, stream=None, implicit="fp64", streams=1, batched=False):
         """implicit: "fp64" (implimf_3d as the reference) or "fp32" (its FP32
         defect-correction form: FP64 residual and update, FP32 sub-iterations).
         streams > 1: the per-block work of each phase runs on that many side
-        streams (up to 16), each block with the workspace set of its stream."""
+        streams (up to 16), each block with the workspace set of its stream.
+        batched: the Navier-Stokes residual of all blocks in one launch per
+        pass (codens_sch_roe_batch) instead of block by block."""
         if implicit not in ("fp64", "fp32"):
             raise ValueError(f"implicit must be 'fp64' or 'fp32', not {implicit!r}")
         self.implicit = implicit
@@ -72,6 +74,8 @@ class Solver:
         self.cycle = 0
         self.res0 = None
         self.graph = None
+        self.batched = batched and case.ns
+        self.blocks_arr = abi.block_array(self.blocks)
         if not 1 <= streams <= 16:
             raise ValueError("streams must be 1..16")
         self.side = [torch.cuda.Stream() for _ in range(streams)] if streams > 1 and len(self.blocks) > 1 else []
@@ -200,6 +204,11 @@ class Solver:
             abi.bc_tq_coincident(g, i, r, self.tq, stream=s)
 
     def residual(self):
+        if self.batched:
+            abi.sch_roe_batch(self.blocks_arr, len(self.blocks), self.ph, self.sc, 0, self.u, self.v, self.ff, self.tq,
+                              self.sn, self.ps, stream=self.stream)
+            return
+
         def one(l, b, s):
             if self.case.ns:
                 abi.sch_roe(b, self.ph, self.sc, 0, self.u, self.v, self.ff, self.tq, self.sn_blk[l], self.ps, stream=s)

@@ -437,6 +437,17 @@ int codens_met_kemut(const codens_block *b, double cmu, int32_t form, int32_t lp
                      const double *vol, double *v, int64_t ldv, const double *mu, double *mut, const double *dist,
                      double *g9, void *stream);
 
+/* sch_roe.f90:4 for every block of a layout in one launch per pass (no
+ * counterpart in the reference, which loops over blocks, atsch_num.f90:621):
+ * the same results as codens_sch_roe called block by block.  blocks: nblk
+ * records of one layout (the first at entry 0, all within ldv); sn_all: the
+ * metric array of the whole layout (block l's slice at 9 npc).  The table of
+ * blocks goes to the device on the first call for a given set of blocks,
+ * which must not happen while the stream is captured into a graph. */
+int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, const codens_phys *ph, const codens_scheme *sc,
+                         int32_t ityprk, double *u, const double *v, int64_t ldv, double *ff, const double *tq,
+                         const double *sn_all, const double *ps, void *stream);
+
 /* Selects one of 16 independent sets of the library's device workspaces for
  * the entry points called after it (no counterpart in the reference, which
  * runs blocks one after another).  For callers that run blocks

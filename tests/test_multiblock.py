@@ -70,3 +70,34 @@ def test_wall_distance_positive():
     inner = case.layout.interior()
     d = case.turb.dist[inner]
     assert d.min() >= 0.5 / 128.0 - 1e-15
+
+
+def test_ghost_node_fold_is_the_transpose_of_extrapolation():
+    """The adjoint harness folds adjoints of ghost nodes back onto the nodes
+    they copy; that must be the exact transpose of the extrapolation the
+    mesh metrics use (layout.extrapolate_ghost_nodes): <E a, w> = <a, F w>
+    for random real-node values a and random weights w on all nodes."""
+    import numpy as np
+    from codens_amd import layout as L
+    from codens_amd.adjoint import _fold_ghost_nodes
+    lay = L.Layout([(5, 4, 3)])
+    b = lay.blocks[0]
+    rng = np.random.default_rng(3)
+    sl = (slice(b.kk1 - b.kd1, b.kk2 - b.kd1 + 1), slice(b.jj1 - b.jd1, b.jj2 - b.jd1 + 1),
+          slice(b.ii1 - b.id1, b.ii2 - b.id1 + 1))
+    for _ in range(3):
+        real = rng.standard_normal(b.view(np.zeros(lay.ip11))[sl].shape)
+        x = np.zeros(lay.ip11); y = np.zeros(lay.ip11); z = np.zeros(lay.ip11)
+        b.view(x)[sl] = real
+        L.extrapolate_ghost_nodes(b, x, y, z)
+        w = rng.standard_normal((b.nkd, b.njd, b.nid))
+        lhs = float((b.view(x) * w).sum())
+        fw = _fold_ghost_nodes(b, w)
+        rhs = float((real * fw[sl]).sum())
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+        # the fold leaves nothing on ghost nodes that the extrapolation fills
+        mask = np.ones_like(w, dtype=bool); mask[sl] = False
+        e = np.zeros(lay.ip11); b.view(e)[sl] = 1.0
+        L.extrapolate_ghost_nodes(b, e, np.zeros(lay.ip11), np.zeros(lay.ip11))
+        filled = (b.view(e) != 0) & mask
+        assert np.all(fw[filled] == 0)

@@ -134,7 +134,7 @@ def main():
     g_saved, s.graph = s.graph, None
     side_saved, s.side = s.side, []  # one stream: events bracket one kernel at a time
     abi.timing_enable(True)
-    for _ in range(max(3, args.steps // 4)):
+    for _ in range(max(3, args.steps // 4)):  # n_timing stages
         s.stage()
     torch.cuda.synchronize()
     kern = {}
@@ -150,10 +150,13 @@ def main():
     fp32 = args.implicit == "fp32"
     sweep_name = "implimf32_sweep" if fp32 else "implimf_sweep"
     sweep_bytes = SWEEP_BYTES_PER_CELL // 2 if fp32 else SWEEP_BYTES_PER_CELL  # the same 33 values in FP32
-    sw = kern.get(sweep_name, {"avg_us": float("nan"), "launches": nblk})
-    # bytes of all sweep launches over their summed time; with uneven blocks
-    # a launch averages sweep_bytes x (cells / blocks)
-    achieved = sweep_bytes * ncell / nblk / (sw["avg_us"] * 1e-6) / 1e9
+    n_timing = max(3, args.steps // 4)
+    sw = kern.get(sweep_name, {"avg_us": float("nan"), "launches": nblk * (case.lmax - 1) * n_timing})
+    # bytes of all sweep launches over their summed time: the timing stages
+    # sweep every cell lmax-1 times, in one launch per block and
+    # sub-iteration, or one per sub-iteration when the blocks are batched
+    cells_per_launch = ncell * (case.lmax - 1) * n_timing / sw["launches"]
+    achieved = sweep_bytes * cells_per_launch / (sw["avg_us"] * 1e-6) / 1e9
     out = {
         "metric": "Mcell-updates/s per implicit iter to 1e-8 residual drop; % of HBM roofline",
         "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -166,14 +169,16 @@ def main():
                                + ("FP32 implicit phase (defect correction)" if fp32 else "FP64")
                                + (", stages replayed from a HIP graph" if graph else "")
                                + (f", blocks on {nstreams} concurrent streams" if len(s.side) > 1 else "")
-                               + (", residual of all blocks in one launch per pass" if s.batched else ""),
+                               + (", residual and implicit phase of all blocks in one launch per pass" if s.batched
+                                  and not fp32 else
+                                  ", residual of all blocks in one launch per pass" if s.batched else ""),
                    "cells": ncell, "blocks": nblk, "lmax": case.lmax},
         "roofline": {"bound": "hbm", "kernel": sweep_name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": SWEEP_TRAFFIC_BYTES if (ci, cj, ck) == (256, 128, 64) and not fp32 and not multi
                      else None,
                      "bytes_per_cell": sweep_bytes,
-                     "avg_us": round(sw["avg_us"], 2)},
+                     "avg_us": round(sw["avg_us"], 2), "cells_per_launch": round(cells_per_launch)},
         "kernels": kern,
         "density_residual": res,
     }

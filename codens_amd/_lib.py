@@ -75,6 +75,8 @@ SIGNATURES = {
     "codens_workspace_set": [c_int],
     "codens_sch_roe_batch": [c_int, c_vp, P(CodensPhys), P(CodensScheme), c_int, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp,
                              c_vp, c_vp],
+    "codens_implimf_3d_batch": [c_int, c_vp, P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int,
+                                c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp],
     "codens_implimf_3d_f32": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int,
                           c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp, c_vp],
     "codens_bc_coincident": [c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],

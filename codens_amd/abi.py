@@ -409,6 +409,14 @@ def block_array(blocks):
     return arr
 
 
+def implimf_3d_batch(blocks_arr, nblk, ph, u, dt, v, d, ff, mu, mut, lmx, sn_all, vol, cson, dtpas=1.0, kfmg=0,
+                     ityprk=0, stream=None):
+    """implimf_3d.f90:4 for all blocks in one launch per pass."""
+    _chk(lib.codens_implimf_3d_batch(int(nblk), blocks_arr, ctypes.byref(ph), _p(u), _p(dt), _p(v), ld(v), _p(d),
+                                     _p(ff), _p(mu), _p(mut), int(lmx), _p(sn_all), _p(vol), dtpas, kfmg, ityprk,
+                                     _p(cson), _s(stream)))
+
+
 def sch_roe_batch(blocks_arr, nblk, ph, sc, ityprk, u, v, ff, tq, sn_all, ps, stream=None):
     """sch_roe.f90:4 for all blocks in one launch per pass."""
     _chk(lib.codens_sch_roe_batch(int(nblk), blocks_arr, ctypes.byref(ph), ctypes.byref(sc), int(ityprk), _p(u), _p(v),

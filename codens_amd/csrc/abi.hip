@@ -6,6 +6,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -97,6 +98,38 @@ void *codens_workspace(size_t bytes, int slot) {
     s.n = bytes;
   }
   return s.p;
+}
+
+namespace {
+std::mutex g_tab_mu;
+// device tables by (device, kind, key); made once, kept to the end
+std::map<std::tuple<int, int, std::vector<int32_t>>, void *> g_tab;
+}  // namespace
+
+void *codens_device_table(int kind, const std::vector<int32_t> &key, const void *data, size_t bytes,
+                          hipStream_t st, const char *what) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    codens_set_error("%s: no current device", what);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  auto k = std::make_tuple(dev, kind, key);
+  auto it = g_tab.find(k);
+  if (it != g_tab.end()) return it->second;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+    codens_set_error("%s: first call for these blocks while the stream is captured; call it once before", what);
+    return nullptr;
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess || hipMemcpy(p, data, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    if (p) (void)hipFree(p);
+    codens_set_error("%s: block table upload failed", what);
+    return nullptr;
+  }
+  g_tab[k] = p;
+  return p;
 }
 
 // ---------------------------------------------------------------------------

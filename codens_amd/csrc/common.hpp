@@ -1,6 +1,7 @@
 // Shared device-side definitions of the gfx950 residual-and-update kernels.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <vector>
 #include <float.h>
 #include <stdint.h>
 
@@ -75,6 +76,35 @@ inline dim3 cell_grid(const Blk &b) {
   return dim3((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz);
 }
 
+// Several blocks in one flattened 1-D launch per pass.  A device table holds
+// one entry per block (type E: members start[pass], gx[pass], gy[pass]): the
+// block's first workgroup in the launch and its own grid (x, y; z follows).
+// Linear workgroup index, remapped so that each XCD owns a contiguous run
+// of the launch (neighbouring tiles of a block share its L2).
+__device__ __forceinline__ unsigned xcd_linear() {
+  unsigned nwg = gridDim.x, hw = blockIdx.x, x = hw % 8u, q = nwg / 8u, r = nwg % 8u;
+  return x * q + (x < r ? x : r) + hw / 8u;
+}
+
+// entry of the block that owns workgroup wg of a pass (bisection over start)
+template <typename E>
+__device__ __forceinline__ int tab_find(const E *__restrict__ tab, int nb, int pass, unsigned wg) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].start[pass] <= wg) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// workgroup wg's tile within its block's own grid
+template <typename E>
+__device__ __forceinline__ dim3 tab_tile(const E &e, int pass, unsigned wg) {
+  unsigned loc = wg - e.start[pass], gx = e.gx[pass], gy = e.gy[pass];
+  return dim3(loc % gx, (loc / gx) % gy, loc / (gx * gy));
+}
+
 }  // namespace codens
 
 // error reporting shared by the entry points (abi.cpp)
@@ -82,6 +112,11 @@ extern "C" void codens_set_error(const char *fmt, ...);
 int codens_check_launch(const char *what);
 int codens_check_block(const codens_block *b, const char *what);
 void *codens_workspace(size_t bytes, int slot);
+// Device copy of a per-layout table, made on the first call for a key and
+// kept for the process on the current device; nullptr (with the error set)
+// if the upload fails or the first call for a key comes while st is captured.
+void *codens_device_table(int kind, const std::vector<int32_t> &key, const void *data, size_t bytes,
+                          hipStream_t st, const char *what);
 bool codens_timing_on();
 void *codens_timing_begin(const char *name, hipStream_t st);
 void codens_timing_end(const char *name, hipStream_t st);

@@ -10,6 +10,8 @@
 // and any traversal order gives the reference's result.
 #include "common.hpp"
 
+#include <algorithm>
+
 #include <array>
 #include <map>
 #include <mutex>
@@ -157,9 +159,11 @@ __device__ inline void face_tn(const DFlux &A, const DFlux &B, double s1, double
   tn[4] = 0.5 * (A.ex + B.ex) * s1 + 0.5 * (A.ey + B.ey) * s2 + 0.5 * (A.ez + B.ez) * s3;
 }
 
-#define CELL3(b)                                     \
+#define CELL3(b) CELL3_AT(b, xcd_block())
+// the same, for the tile XB of a block's own grid
+#define CELL3_AT(b, XB)                              \
   int c[3];                                          \
-  const dim3 xb = xcd_block();                       \
+  const dim3 xb = (XB);                              \
   c[0] = (b).i1 + xb.x * TX + threadIdx.x;     \
   c[1] = (b).j1 + xb.y * TY + threadIdx.y;     \
   c[2] = (b).k1 + xb.z;                        \
@@ -173,15 +177,13 @@ __device__ inline double rv_of(const Phys &ph, const double *mu, const double *m
 // diagonal.  coefe[d] holds the low face of every real cell.  Velocities and
 // rv use one reciprocal of each density (the reference divides each time;
 // the coefficients move by ~1e-16 relative).
-__global__ void __launch_bounds__(TX *TY) k_impl_coef(Blk b, Phys ph, const double *__restrict__ v, long ldv,
-                                                      const double *__restrict__ mu,
-                                                      const double *__restrict__ mut,
-                                                      const double *__restrict__ dt,
-                                                      const double *__restrict__ vol,
-                                                      const double *__restrict__ cson,
-                                                      const double *__restrict__ sn, double dtpas, int kfmg,
-                                                      double *__restrict__ coefe, double *__restrict__ diag) {
-  CELL3(b)
+__device__ __forceinline__ void coef_body(const Blk &b, dim3 tile, const Phys &ph, const double *__restrict__ v,
+                                          long ldv, const double *__restrict__ mu, const double *__restrict__ mut,
+                                          const double *__restrict__ dt, const double *__restrict__ vol,
+                                          const double *__restrict__ cson, const double *__restrict__ sn,
+                                          double dtpas, int kfmg, double *__restrict__ coefe,
+                                          double *__restrict__ diag) {
+  CELL3_AT(b, tile)
   long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
   double rc = 1.0 / v[n];
   double rvc = rv_of(ph, mu, mut, rc, n);
@@ -218,6 +220,17 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef(Blk b, Phys ph, const doub
   dg = dg + cf[0] + cf[1] + cf[2] + cf[3] + cf[4] + cf[5];
   if (kfmg == 3) dg = dg + 1.5 * vol[n] / dtpas;
   diag[m] = dg;
+}
+
+__global__ void __launch_bounds__(TX *TY) k_impl_coef(Blk b, Phys ph, const double *__restrict__ v, long ldv,
+                                                      const double *__restrict__ mu,
+                                                      const double *__restrict__ mut,
+                                                      const double *__restrict__ dt,
+                                                      const double *__restrict__ vol,
+                                                      const double *__restrict__ cson,
+                                                      const double *__restrict__ sn, double dtpas, int kfmg,
+                                                      double *__restrict__ coefe, double *__restrict__ diag) {
+  coef_body(b, xcd_block(), ph, v, ldv, mu, mut, dt, vol, cson, sn, dtpas, kfmg, coefe, diag);
 }
 
 // implimf_eu.f90:124-170: coefficients of the Euler implicit phase,
@@ -382,10 +395,10 @@ __device__ inline void q_tn(const T *A, const T *B, T s1, T s2, T s3, T tn[5]) {
 
 // First sub-iteration: d and dF are zero, so d = -(u (+ff)) / diag.
 template <typename T>
-__global__ void __launch_bounds__(TX *TY) k_impl_sweep1(Blk b, int ityprk, const T *__restrict__ u,
-                                                        const T *__restrict__ ff, long ldv,
-                                                        const T *__restrict__ diag, TView<T> dnew) {
-  CELL3(b)
+__device__ __forceinline__ void sweep1_body(const Blk &b, dim3 tile, int ityprk, const T *__restrict__ u,
+                                            const T *__restrict__ ff, long ldv, const T *__restrict__ diag,
+                                            const TView<T> &dnew) {
+  CELL3_AT(b, tile)
   long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
   T dg = diag[m];
 #pragma unroll
@@ -393,6 +406,13 @@ __global__ void __launch_bounds__(TX *TY) k_impl_sweep1(Blk b, int ityprk, const
     T r = ityprk == 0 ? -u[e * ldv + n] : -u[e * ldv + n] - ff[e * ldv + n];
     dnew.at(e, n) = r / dg;
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TX *TY) k_impl_sweep1(Blk b, int ityprk, const T *__restrict__ u,
+                                                        const T *__restrict__ ff, long ldv,
+                                                        const T *__restrict__ diag, TView<T> dnew) {
+  sweep1_body<T>(b, xcd_block(), ityprk, u, ff, ldv, diag, dnew);
 }
 
 constexpr int SW_PL = (SW_TY + 2) * (SW_TX + 2);  // LDS plane stride between the 14 components
@@ -429,6 +449,18 @@ __device__ inline void lds_sync() {
 #endif
 // EU selects the flux increments of implimf_eu (which read ps); the face
 // terms, coefficients and update are common to both implicit phases.
+template <bool EU, typename T, int TXS, int TYS>
+__device__ __forceinline__ void sweep_body(const Blk b, int lg, int ntx, int nty, const Phys ph, int ityprk,
+                                           const T *__restrict__ u, const T *__restrict__ ff,
+                                           const T *__restrict__ v, long ldv, const double *__restrict__ ps,
+                                           const T *__restrict__ sn, const T *__restrict__ coefe,
+                                           const T *__restrict__ diag, const TView<T> dold, const TView<T> dnew) {
+  constexpr int PL = (TYS + 2) * (TXS + 2), HALO = 2 * TXS + 2 * TYS;
+  static_assert(HALO <= TXS * TYS, "one halo cell per thread at most");
+  __shared__ T sq[14 * PL];
+#include "sweep_tile.inc"
+}
+
 template <bool EU, typename T, int TXS = SW_TX, int TYS = SW_TY>
 __global__ void __launch_bounds__(TXS *TYS) SW_ATTR k_impl_sweep_tiled(Blk b, Phys ph, int ityprk, int ntx, int nty,
                                                                    int nkz, const T *__restrict__ u,
@@ -439,158 +471,16 @@ __global__ void __launch_bounds__(TXS *TYS) SW_ATTR k_impl_sweep_tiled(Blk b, Ph
                                                                    const T *__restrict__ coefe,
                                                                    const T *__restrict__ diag, TView<T> dold,
                                                                    TView<T> dnew) {
-  constexpr int PL = (TYS + 2) * (TXS + 2), HALO = 2 * TXS + 2 * TYS;
-  static_assert(HALO <= TXS * TYS, "one halo cell per thread at most");
-  __shared__ T sq[14 * PL];
   // XCD-aware tile order: the hardware deals workgroups to the 8 XCDs round
   // robin; give each XCD a contiguous run of tiles so that j/k neighbours
   // share its L2.
+  constexpr int PL = (TYS + 2) * (TXS + 2), HALO = 2 * TXS + 2 * TYS;
+  static_assert(HALO <= TXS * TYS, "one halo cell per thread at most");
+  __shared__ T sq[14 * PL];
   int nwg = ntx * nty * nkz;
   int hw = blockIdx.x;
   int lg = (nwg % 8 == 0) ? (hw % 8) * (nwg / 8) + hw / 8 : hw;
-  int bx = lg % ntx, by = (lg / ntx) % nty, bz = lg / (ntx * nty);
-  int tx = threadIdx.x, ty = threadIdx.y, t = ty * TXS + tx;
-  int i0 = b.i1 + bx * TXS, j0 = b.j1 + by * TYS;
-  int i = i0 + tx, j = j0 + ty;
-  int kb = b.k1 + bz * SW_KS, ke = min(kb + SW_KS, b.k2);
-  bool own = i < b.i2 && j < b.j2;
-  // halo cell of this thread (t < HALO): rows j0-1, j0+TYS, columns i0-1, i0+TXS
-  int hi_ = 0, hj = 0, hslot = 0;
-  bool halo = t < HALO;
-  if (halo) {
-    int hx, hy;
-    if (t < TXS) { hi_ = i0 + t; hj = j0 - 1; hx = t + 1; hy = 0; }
-    else if (t < 2 * TXS) { hi_ = i0 + t - TXS; hj = j0 + TYS; hx = t - TXS + 1; hy = TYS + 1; }
-    else if (t < 2 * TXS + TYS) { hi_ = i0 - 1; hj = j0 + t - 2 * TXS; hx = 0; hy = t - 2 * TXS + 1; }
-    else { hi_ = i0 + TXS; hj = j0 + t - 2 * TXS - TYS; hx = TXS + 1; hy = t - 2 * TXS - TYS + 1; }
-    hslot = hy * (TXS + 2) + hx;
-  }
-  bool hreal = halo && hi_ >= b.i1 && hi_ < b.i2 && hj >= b.j1 && hj < b.j2;
-  const long nnn = b.nnn, nijd = b.nijd;
-  const int slot = (ty + 1) * (TXS + 2) + (tx + 1);
-  Q14T<T> Qc;
-  T tlo_k[5], dlo_k[5];
-  if (own) {
-    long n = b.idx(i, j, kb), m = n - b.npc;
-    Q14T<T> Qb;
-    if (kb > b.k1) q_model<EU, T>(ph, v, ldv, dold, ps, n - nijd, Qb); else q_zero(Qb);
-    q_model<EU, T>(ph, v, ldv, dold, ps, n, Qc);
-    q_tn(Qc.q, Qb.q, snt<T>(sn, nnn, m, 2, 0), snt<T>(sn, nnn, m, 2, 1), snt<T>(sn, nnn, m, 2, 2), tlo_k);
-#pragma unroll
-    for (int e = 0; e < 5; e++) dlo_k[e] = q_d(Qb.q, e);
-  } else {
-    q_zero(Qc);
-#pragma unroll
-    for (int e = 0; e < 5; e++) tlo_k[e] = dlo_k[e] = T(0);
-  }
-  // d of the previous plane, stored once the next plane's loads are issued
-  // (the kernel never reads dnew, so the stores can trail by one plane)
-  T dpend[5];
-  long npend = -1;
-  for (int k = kb; k < ke; k++) {
-    long n = own ? b.idx(i, j, k) : 0, m = n - b.npc;
-    bool kreal = k + 1 < b.k2;
-    // this plane's operator terms, loaded as one batch before the next
-    // plane's flux increments are formed, so their latency overlaps that work:
-    // rhs -u (-ff), the i/j face metrics of both faces, coefficients, diagonal
-    // (-u - 0 is -u bit for bit, so ityprk = 0 takes the same form)
-    T uu[5], fv[5], sl[2][3], sh[2][3], cl[3], ch[3], dgm;
-    if (own) {
-#pragma unroll
-      for (int e = 0; e < 5; e++) {
-        uu[e] = u[e * ldv + n];
-        fv[e] = ityprk ? ff[e * ldv + n] : T(0);
-      }
-#pragma unroll
-      for (int d = 0; d < 2; d++) {
-        long inc = d == 0 ? 1 : b.nid;
-        bool rreal = d == 0 ? (i + 1 < b.i2) : (j + 1 < b.j2);
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          sl[d][c] = snt<T>(sn, nnn, m, d, c);
-          sh[d][c] = snt<T>(sn, nnn, m + inc, d, c);
-        }
-        cl[d] = coefe[d * nnn + m];
-        ch[d] = rreal ? coefe[d * nnn + m + inc] : T(0);
-      }
-      cl[2] = coefe[2 * nnn + m];
-      ch[2] = kreal ? coefe[2 * nnn + m + nijd] : T(0);
-      dgm = diag[m];
-    }
-    // inputs of this thread's halo cell (plane k) and of its own cell on
-    // plane k+1, loaded together
-    VD<T> xh, xn;
-    bool hq = halo && hreal, nq = own && kreal;
-    vd_load<EU, T>(v, ldv, dold, ps, hq ? b.idx(hi_, hj, k) : n, hq, xh);
-    vd_load<EU, T>(v, ldv, dold, ps, n + nijd, nq, xn);
-    T skn[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) skn[c] = own ? snt<T>(sn, nnn, m + nijd, 2, c) : T(0);
-    if (npend >= 0) {
-#pragma unroll
-      for (int e = 0; e < 5; e++) dnew.at(e, npend) = dpend[e];
-    }
-    // current plane into LDS (own cell and this thread's halo cell)
-#pragma unroll
-    for (int c = 0; c < 14; c++) sq[c * PL + slot] = Qc.q[c];
-    if (halo) {
-      Q14T<T> Qh;
-      vd_q<EU, T>(ph, xh, hq, Qh);
-#pragma unroll
-      for (int c = 0; c < 14; c++) sq[c * PL + hslot] = Qh.q[c];
-    }
-    // k face above: q of the next plane and its face term
-    T thi_k[5], dhi_k[5];
-    if (own) {
-      Q14T<T> Qn;
-      vd_q<EU, T>(ph, xn, kreal, Qn);
-      q_tn(Qn.q, Qc.q, skn[0], skn[1], skn[2], thi_k);
-#pragma unroll
-      for (int e = 0; e < 5; e++) dhi_k[e] = q_d(Qn.q, e);
-      Qc = Qn;  // plane k+1; the plane-k values are read back from LDS below
-    }
-    lds_sync();
-    if (own) {
-      T d2w[5];
-#pragma unroll
-      for (int e = 0; e < 5; e++) d2w[e] = -uu[e] - fv[e];
-      const T *A = sq + slot;
-      // i then j directions from LDS (implimf_3d.f90:268-362)
-#pragma unroll
-      for (int d = 0; d < 2; d++) {
-        const T *Lp = d == 0 ? A - 1 : A - (TXS + 2);
-        const T *Rp = d == 0 ? A + 1 : A + (TXS + 2);
-        bool rreal = d == 0 ? (i + 1 < b.i2) : (j + 1 < b.j2);
-#pragma unroll
-        for (int e = 0; e < 5; e++) {
-          T tt = d2w[e] + lds_tn<PL>(A, Lp, e, sl[d][0], sl[d][1], sl[d][2]) + cl[d] * lds_d<PL>(Lp, e);
-          if (rreal) tt = tt + ch[d] * lds_d<PL>(Rp, e);
-          d2w[e] = tt - lds_tn<PL>(Rp, A, e, sh[d][0], sh[d][1], sh[d][2]);
-        }
-      }
-      // k direction from registers
-#pragma unroll
-      for (int e = 0; e < 5; e++) {
-        T tt = d2w[e] + tlo_k[e] + cl[2] * dlo_k[e];
-        if (kreal) tt = tt + ch[2] * dhi_k[e];
-        d2w[e] = tt - thi_k[e];
-      }
-      T rdg = T(1) / dgm;
-#pragma unroll
-      for (int e = 0; e < 5; e++) dpend[e] = d2w[e] * rdg;
-      npend = n;
-#pragma unroll
-      for (int e = 0; e < 5; e++) {
-        tlo_k[e] = thi_k[e];
-        dlo_k[e] = lds_d<PL>(A, e);
-      }
-    }
-    lds_sync();
-  }
-  if (npend >= 0) {
-#pragma unroll
-    for (int e = 0; e < 5; e++) dnew.at(e, npend) = dpend[e];
-  }
+#include "sweep_tile.inc"
 }
 
 // ---------------------------------------------------------------------------
@@ -711,11 +601,16 @@ extern "C" int codens_sweep_tile_of(const codens_block *cb, int32_t eu, int32_t 
 }
 
 // implimf_3d.f90:473-486
-__global__ void __launch_bounds__(TX *TY) k_impl_update(Blk b, double *v, long ldv, const double *__restrict__ d,
-                                                        int nvar) {
-  CELL3(b)
+__device__ __forceinline__ void update_body(const Blk &b, dim3 tile, double *v, long ldv,
+                                            const double *__restrict__ d, int nvar) {
+  CELL3_AT(b, tile)
   long n = b.idx(c[0], c[1], c[2]);
   for (int e = 0; e < nvar; e++) v[e * ldv + n] = v[e * ldv + n] + d[e * ldv + n];
+}
+
+__global__ void __launch_bounds__(TX *TY) k_impl_update(Blk b, double *v, long ldv, const double *__restrict__ d,
+                                                        int nvar) {
+  update_body(b, xcd_block(), v, ldv, d, nvar);
 }
 
 __global__ void k_zero5(double *d, long ldv, long n0, long n1) {
@@ -792,6 +687,175 @@ static int implimf_common(bool eu, const codens_block *cb, const codens_phys *cp
   {
     CodensTimed _tm("implimf_update", st);
     hipLaunchKernelGGL(k_impl_update, g, t, 0, st, b, v, (long)ldv, d, nvar);
+  }
+  return codens_check_launch(what);
+}
+
+// ---------------------------------------------------------------------------
+// implimf_3d for all blocks of a layout.  Each pass (zero d, coefficients,
+// first sub-iteration, the lmx-1 tiled sub-iterations, update) is one
+// flattened launch over every block; a device table gives each block its
+// extents and its first workgroup per pass, and a workgroup runs the
+// per-block kernel's own body on its block's tile.  Block l's coefficients,
+// diagonal and ping-pong increment (9 planes of nnn) start at 9 npc of the
+// workspace.  The sub-iterations of small blocks then share the chip instead
+// of each filling a few CUs.
+enum { IB_ZERO, IB_CELL, IB_SWEEP, IB_N };
+
+struct ImplEnt {
+  Blk b;
+  unsigned start[IB_N];         // first workgroup of this block, per pass
+  unsigned gx[IB_N], gy[IB_N];  // this block's grid, per pass
+  int tile;                     // sub-iteration tile shape of this block (0: 64 x 8, 1: 32 x 16)
+};
+
+__global__ void __launch_bounds__(256) k_zero5_batch(const ImplEnt *__restrict__ tab, int nb, double *d, long ldv) {
+  const ImplEnt &E = tab[tab_find(tab, nb, IB_ZERO, blockIdx.x)];
+  const long n0 = E.b.npc, n = n0 + (blockIdx.x - E.start[IB_ZERO]) * 256L + threadIdx.x;
+  if (n >= n0 + E.b.nnn) return;
+#pragma unroll
+  for (int e = 0; e < 5; e++) d[e * ldv + n] = 0.;
+}
+
+__global__ void __launch_bounds__(TX *TY) k_impl_coef_batch(const ImplEnt *__restrict__ tab, int nb, Phys ph,
+                                                            const double *__restrict__ v, long ldv,
+                                                            const double *__restrict__ mu,
+                                                            const double *__restrict__ mut,
+                                                            const double *__restrict__ dt,
+                                                            const double *__restrict__ vol,
+                                                            const double *__restrict__ cson,
+                                                            const double *__restrict__ sn_all, double dtpas, int kfmg,
+                                                            double *ws) {
+  const unsigned wg = xcd_linear();
+  const ImplEnt &E = tab[tab_find(tab, nb, IB_CELL, wg)];
+  const Blk b = E.b;
+  double *c = ws + 9 * b.npc;
+  coef_body(b, tab_tile(E, IB_CELL, wg), ph, v, ldv, mu, mut, dt, vol, cson, sn_all + 9 * b.npc, dtpas, kfmg, c,
+            c + 3 * b.nnn);
+}
+
+__global__ void __launch_bounds__(TX *TY) k_impl_sweep1_batch(const ImplEnt *__restrict__ tab, int nb, int ityprk,
+                                                              const double *__restrict__ u,
+                                                              const double *__restrict__ ff, long ldv, double *ws,
+                                                              double *d, int out_d) {
+  const unsigned wg = xcd_linear();
+  const ImplEnt &E = tab[tab_find(tab, nb, IB_CELL, wg)];
+  const Blk b = E.b;
+  double *c = ws + 9 * b.npc;
+  DView out = out_d ? DView{d, ldv, 0} : DView{c + 4 * b.nnn, b.nnn, b.npc};
+  sweep1_body<double>(b, tab_tile(E, IB_CELL, wg), ityprk, u, ff, ldv, c + 3 * b.nnn, out);
+}
+
+__global__ void __launch_bounds__(SW_TX *SW_TY) k_impl_sweep_batch(const ImplEnt *__restrict__ tab, int nb, Phys ph,
+                                                                   int ityprk, const double *__restrict__ u,
+                                                                   const double *__restrict__ ff,
+                                                                   const double *__restrict__ v, long ldv,
+                                                                   const double *__restrict__ sn_all, double *ws,
+                                                                   double *d, int out_d) {
+  const unsigned wg = xcd_linear();
+  const ImplEnt &E = tab[tab_find(tab, nb, IB_SWEEP, wg)];
+  const Blk b = E.b;
+  double *c = ws + 9 * b.npc;
+  DView A = {d, ldv, 0}, B = {c + 4 * b.nnn, b.nnn, b.npc};
+  const int lg = (int)(wg - E.start[IB_SWEEP]), ntx = (int)E.gx[IB_SWEEP], nty = (int)E.gy[IB_SWEEP];
+  if (E.tile == 1)
+    sweep_body<false, double, 32, 16>(b, lg, ntx, nty, ph, ityprk, u, ff, v, ldv, v, sn_all + 9 * b.npc, c,
+                                      c + 3 * b.nnn, out_d ? B : A, out_d ? A : B);
+  else
+    sweep_body<false, double, SW_TX, SW_TY>(b, lg, ntx, nty, ph, ityprk, u, ff, v, ldv, v, sn_all + 9 * b.npc, c,
+                                            c + 3 * b.nnn, out_d ? B : A, out_d ? A : B);
+}
+
+__global__ void __launch_bounds__(TX *TY) k_impl_update_batch(const ImplEnt *__restrict__ tab, int nb, double *v,
+                                                              long ldv, const double *__restrict__ d, int nvar) {
+  const unsigned wg = xcd_linear();
+  const ImplEnt &E = tab[tab_find(tab, nb, IB_CELL, wg)];
+  update_body(E.b, tab_tile(E, IB_CELL, wg), v, ldv, d, nvar);
+}
+
+extern "C" int codens_implimf_3d_batch(int32_t nblk, const codens_block *blocks, const codens_phys *cp,
+                                       const double *u, const double *dt, double *v, int64_t ldv, double *d,
+                                       const double *ff, const double *mu, const double *mut, int32_t lmx,
+                                       const double *sn_all, const double *vol, double dtpas, int32_t kfmg,
+                                       int32_t ityprk, const double *cson, void *stream) {
+  const char *what = "codens_implimf_3d_batch";
+  if (nblk <= 0 || !blocks || !cp || !u || !dt || !v || !d || !mu || !mut || !sn_all || !vol || !cson ||
+      (ityprk && !ff) || lmx < 0) {
+    codens_set_error("%s: null or invalid argument", what);
+    return CODENS_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<ImplEnt> ent(nblk);
+  std::vector<int32_t> key;
+  std::vector<std::pair<long, long>> span(nblk);
+  unsigned tot[IB_N] = {0};
+  for (int l = 0; l < nblk; l++) {
+    int rc = codens_check_block(&blocks[l], what);
+    if (rc) return rc;
+    const int32_t *w = (const int32_t *)&blocks[l];
+    key.insert(key.end(), w, w + sizeof(codens_block) / sizeof(int32_t));
+    Blk b = make_blk(&blocks[l]);
+    span[l] = {b.npc, b.npc + b.nnn};
+    ImplEnt &E = ent[l];
+    E.b = b;
+    dim3 cg = cell_grid(b);
+    int nkz = (b.k2 - b.k1 + SW_KS - 1) / SW_KS;
+    {
+      std::lock_guard<std::mutex> lk(g_tile_mu);
+      E.tile = g_tile_mode >= 0 ? g_tile_mode : tile_model(b);
+    }
+    const int tx = tile_tx(E.tile), ty = tile_ty(E.tile);
+    unsigned ntx = (b.i2 - b.i1 + tx - 1) / tx, nty = (b.j2 - b.j1 + ty - 1) / ty;
+    unsigned g[IB_N][3] = {{(unsigned)((b.nnn + 255) / 256), 1, 1}, {cg.x, cg.y, cg.z}, {ntx, nty, (unsigned)nkz}};
+    for (int p = 0; p < IB_N; p++) {
+      E.start[p] = tot[p];
+      E.gx[p] = g[p][0];
+      E.gy[p] = g[p][1];
+      tot[p] += g[p][0] * g[p][1] * g[p][2];
+    }
+  }
+  std::sort(span.begin(), span.end());
+  for (int l = 0; l < nblk; l++) {
+    if ((l == 0 && span[0].first != 0) || (l > 0 && span[l].first < span[l - 1].second) || span[l].second > ldv) {
+      codens_set_error("%s: the blocks must start at entry 0, not overlap, and lie within ldv", what);
+      return CODENS_EINVAL;
+    }
+  }
+  const long tot_n = span[nblk - 1].second;
+  for (int l = 0; l < nblk; l++) key.push_back(ent[l].tile);
+  ImplEnt *tab = (ImplEnt *)codens_device_table(1, key, ent.data(), sizeof(ImplEnt) * nblk, st, what);
+  if (!tab) return CODENS_EHIP;
+  double *ws = (double *)codens_workspace(sizeof(double) * 9 * tot_n, 2);
+  if (!ws) {
+    codens_set_error("%s: workspace allocation failed", what);
+    return CODENS_EHIP;
+  }
+  Phys ph = make_phys(cp);
+  const ImplEnt *ct = tab;
+  const int nb = nblk;
+  const double *fx = ityprk ? ff : u;
+  hipLaunchKernelGGL(k_zero5_batch, dim3(tot[IB_ZERO]), dim3(256), 0, st, ct, nb, d, (long)ldv);
+  {
+    CodensTimed _tm("implimf_coef", st);
+    hipLaunchKernelGGL(k_impl_coef_batch, dim3(tot[IB_CELL]), dim3(TX, TY), 0, st, ct, nb, ph, (const double *)v,
+                       (long)ldv, mu, mut, dt, vol, cson, sn_all, dtpas, (int)kfmg, ws);
+  }
+  for (int s = 1; s <= lmx; s++) {
+    int out_d = (lmx - s) % 2 == 0;
+    if (s == 1) {
+      CodensTimed _tm("implimf_sweep1", st);
+      hipLaunchKernelGGL(k_impl_sweep1_batch, dim3(tot[IB_CELL]), dim3(TX, TY), 0, st, ct, nb, (int)ityprk, u, fx,
+                         (long)ldv, ws, d, out_d);
+    } else {
+      CodensTimed _tm("implimf_sweep", st);
+      hipLaunchKernelGGL(k_impl_sweep_batch, dim3(tot[IB_SWEEP]), dim3(SW_TX * SW_TY), 0, st, ct, nb, ph,
+                         (int)ityprk, u, fx, (const double *)v, (long)ldv, sn_all, ws, d, out_d);
+    }
+  }
+  {
+    CodensTimed _tm("implimf_update", st);
+    hipLaunchKernelGGL(k_impl_update_batch, dim3(tot[IB_CELL]), dim3(TX, TY), 0, st, ct, nb, v, (long)ldv,
+                       (const double *)d, 5);
   }
   return codens_check_launch(what);
 }

@@ -20,9 +20,6 @@
 // operation).
 #include "common.hpp"
 
-#include <map>
-#include <mutex>
-#include <utility>
 #include <vector>
 
 using namespace codens;
@@ -548,46 +545,24 @@ struct BatchEnt {
   unsigned gx[BP_N], gy[BP_N];  // this block's grid, per pass
 };
 
-__device__ __forceinline__ int batch_find(const BatchEnt *__restrict__ tab, int nb, int pass, unsigned wg) {
-  int lo = 0, hi = nb - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (tab[mid].start[pass] <= wg) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ dim3 batch_tile(const BatchEnt &E, int pass, unsigned wg) {
-  unsigned loc = wg - E.start[pass], gx = E.gx[pass], gy = E.gy[pass];
-  return dim3(loc % gx, (loc / gx) % gy, loc / (gx * gy));
-}
-
-// workgroup order remapped so that each XCD owns a contiguous run of the
-// flattened launch (neighbouring tiles of a block share its L2)
-__device__ __forceinline__ unsigned xcd_linear() {
-  unsigned nwg = gridDim.x, hw = blockIdx.x, x = hw % 8u, q = nwg / 8u, r = nwg % 8u;
-  return x * q + (x < r ? x : r) + hw / 8u;
-}
-
 __global__ void __launch_bounds__(256) k_prim_uvw_batch(const BatchEnt *__restrict__ tab, int nb,
                                                         const double *__restrict__ v, long ldv, double *uvw) {
-  const BatchEnt &E = tab[batch_find(tab, nb, BP_PRIM, blockIdx.x)];
+  const BatchEnt &E = tab[tab_find(tab, nb, BP_PRIM, blockIdx.x)];
   prim_uvw_body(E.b, blockIdx.x - E.start[BP_PRIM], v, ldv, uvw + 3 * E.b.npc);
 }
 
 __global__ void __launch_bounds__(64) k_chain_i_batch(const BatchEnt *__restrict__ tab, int nb,
                                                       const double *__restrict__ v, const double *__restrict__ uvw,
                                                       const double *__restrict__ ps, double *R) {
-  const BatchEnt &E = tab[batch_find(tab, nb, BP_CHAIN_I, blockIdx.x)];
-  chain_i_body(E.b, batch_tile(E, BP_CHAIN_I, blockIdx.x), v, uvw + 3 * E.b.npc, ps, R + 15 * E.b.npc);
+  const BatchEnt &E = tab[tab_find(tab, nb, BP_CHAIN_I, blockIdx.x)];
+  chain_i_body(E.b, tab_tile(E, BP_CHAIN_I, blockIdx.x), v, uvw + 3 * E.b.npc, ps, R + 15 * E.b.npc);
 }
 
 __global__ void __launch_bounds__(64) k_chain_jk_batch(const BatchEnt *__restrict__ tab, int nb,
                                                        const double *__restrict__ v, const double *__restrict__ uvw,
                                                        const double *__restrict__ ps, double *R) {
-  const BatchEnt &E = tab[batch_find(tab, nb, BP_CHAIN_JK, blockIdx.x)];
-  chain_jk_body(E.b, batch_tile(E, BP_CHAIN_JK, blockIdx.x), v, uvw + 3 * E.b.npc, ps, R + 15 * E.b.npc);
+  const BatchEnt &E = tab[tab_find(tab, nb, BP_CHAIN_JK, blockIdx.x)];
+  chain_jk_body(E.b, tab_tile(E, BP_CHAIN_JK, blockIdx.x), v, uvw + 3 * E.b.npc, ps, R + 15 * E.b.npc);
 }
 
 __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(CODENS_FF_WAVES)))
@@ -597,24 +572,18 @@ k_face_flux_batch(const BatchEnt *__restrict__ tab, int nb, Phys ph, int d, int 
                   const double *__restrict__ rr, double *H) {
   const unsigned wg = xcd_linear();
   const int pass = BP_FACE0 + d;
-  const BatchEnt &E = tab[batch_find(tab, nb, pass, wg)];
+  const BatchEnt &E = tab[tab_find(tab, nb, pass, wg)];
   const long np = E.b.npc;
-  face_flux_body(E.b, batch_tile(E, pass, wg), ph, d, ns, ilim, muscl, xk, v, ldv, ps, tq, sn_all + 9 * np,
+  face_flux_body(E.b, tab_tile(E, pass, wg), ph, d, ns, ilim, muscl, xk, v, ldv, ps, tq, sn_all + 9 * np,
                  uvw + 3 * np, ilim == 1 ? rr + 15 * np + 5 * E.b.nnn * d : nullptr, H + 15 * np + 5 * E.b.nnn * d);
 }
 
 __global__ void __launch_bounds__(256) k_gather_batch(const BatchEnt *__restrict__ tab, int nb,
                                                       const double *__restrict__ H, double *u, long ldv, double *ff,
                                                       int ityprk) {
-  const BatchEnt &E = tab[batch_find(tab, nb, BP_GATHER, blockIdx.x)];
+  const BatchEnt &E = tab[tab_find(tab, nb, BP_GATHER, blockIdx.x)];
   gather_body(E.b, blockIdx.x - E.start[BP_GATHER], E.n0, E.n1, H + 15 * E.b.npc, u, ldv, ff, ityprk);
 }
-
-namespace {
-std::mutex g_batch_mu;
-// device tables by (device, block extents); made once, kept to the end
-std::map<std::pair<int, std::vector<int32_t>>, BatchEnt *> g_batch_tab;
-}  // namespace
 
 extern "C" int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, const codens_phys *cp,
                                     const codens_scheme *sc, int32_t ityprk, double *u, const double *v,
@@ -667,29 +636,8 @@ extern "C" int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, co
     codens_set_error("%s: the blocks must start at entry 0 and lie within ldv", what);
     return CODENS_EINVAL;
   }
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return CODENS_EHIP;
-  BatchEnt *tab = nullptr;
-  {
-    std::lock_guard<std::mutex> lk(g_batch_mu);
-    auto k = std::make_pair(dev, key);
-    auto it = g_batch_tab.find(k);
-    if (it != g_batch_tab.end()) {
-      tab = it->second;
-    } else {
-      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-      if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
-        codens_set_error("%s: first call for these blocks while the stream is captured; call it once before", what);
-        return CODENS_EINVAL;
-      }
-      if (hipMalloc(&tab, sizeof(BatchEnt) * nblk) != hipSuccess ||
-          hipMemcpy(tab, ent.data(), sizeof(BatchEnt) * nblk, hipMemcpyHostToDevice) != hipSuccess) {
-        codens_set_error("%s: block table upload failed", what);
-        return CODENS_EHIP;
-      }
-      g_batch_tab[k] = tab;
-    }
-  }
+  BatchEnt *tab = (BatchEnt *)codens_device_table(0, key, ent.data(), sizeof(BatchEnt) * nblk, st, what);
+  if (!tab) return CODENS_EHIP;
   Phys ph = make_phys(cp);
   const long tot_n = hi;
   double *H = (double *)codens_workspace(sizeof(double) * 15 * tot_n, 0);

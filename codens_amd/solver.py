@@ -31,7 +31,8 @@ class Solver:
         streams > 1: the per-block work of each phase runs on that many side
         streams (up to 16), each block with the workspace set of its stream.
         batched: the Navier-Stokes residual of all blocks in one launch per
-        pass (codens_sch_roe_batch) instead of block by block."""
+        pass (codens_sch_roe_batch) instead of block by block, and so does the
+        FP64 implicit phase (codens_implimf_3d_batch)."""
         if implicit not in ("fp64", "fp32"):
             raise ValueError(f"implicit must be 'fp64' or 'fp32', not {implicit!r}")
         self.implicit = implicit
@@ -218,6 +219,10 @@ class Solver:
 
     def update(self):
         cs = self.case
+        if self.batched and cs.kmf == 1 and self.implicit == "fp64":
+            abi.implimf_3d_batch(self.blocks_arr, len(self.blocks), self.ph, self.u, self.dt, self.v, self.d, self.ff,
+                                 self.mu, self.mut, cs.lmax, self.sn, self.vol, self.cson, stream=self.stream)
+            return
 
         def one(l, b, s):
             if cs.kmf == 0:

@@ -448,6 +448,19 @@ int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, const codens_
                          int32_t ityprk, double *u, const double *v, int64_t ldv, double *ff, const double *tq,
                          const double *sn_all, const double *ps, void *stream);
 
+/* implimf_3d.f90:4 for every block of a layout in one launch per pass (no
+ * counterpart in the reference, which calls implimf_3d block by block,
+ * atsch_num.f90:1040): the same results as codens_implimf_3d called block by
+ * block, bit for bit.  blocks: nblk records of one layout (the first at
+ * entry 0, none overlapping, all within ldv); sn_all: the metric array of the
+ * whole layout (block l's slice at 9 npc).  The block table goes to the
+ * device on the first call for a given set of blocks, which must not happen
+ * while the stream is captured into a graph. */
+int codens_implimf_3d_batch(int32_t nblk, const codens_block *blocks, const codens_phys *ph, const double *u,
+                            const double *dt, double *v, int64_t ldv, double *d, const double *ff, const double *mu,
+                            const double *mut, int32_t lmx, const double *sn_all, const double *vol, double dtpas,
+                            int32_t kfmg, int32_t ityprk, const double *cson, void *stream);
+
 /* Selects one of 16 independent sets of the library's device workspaces for
  * the entry points called after it (no counterpart in the reference, which
  * runs blocks one after another).  For callers that run blocks

@@ -265,7 +265,8 @@ def test_converges_to_1e8_fp64_and_fp32():
 def test_batched_residual_matches_blockwise():
     """The residual of all blocks in one launch per pass
     (codens_sch_roe_batch) equals the block-by-block residual bit for bit,
-    alone and over 3 stages, on the multi-block SA case with minmod."""
+    alone and over 3 stages (batched residual and implicit phase, concurrent
+    streams, graph replay), on the multi-block SA case with minmod."""
     from codens_amd import configs
     from codens_amd.solver import Solver
     case = configs.multiblock_sa(9, 4, 11, None, seed=7, ilim=1, lmax=4)
@@ -283,6 +284,27 @@ def test_batched_residual_matches_blockwise():
     torch.cuda.synchronize()
     for x, y in ((a2.v, b2.v), (a2.mut, b2.mut)):
         assert torch.equal(x.cpu(), y.cpu())
+
+
+@pytest.mark.parametrize("lmax", [1, 4, 7])
+def test_batched_implicit_matches_blockwise(lmax):
+    """implimf_3d of all blocks in one launch per pass
+    (codens_implimf_3d_batch) equals implimf_3d called block by block, bit
+    for bit: the increment d and the updated state, for an odd and an even
+    number of sub-iterations (the ping-pong lands in d either way)."""
+    from codens_amd import abi, configs
+    from codens_amd.solver import Solver
+    case = configs.multiblock_sa(9, 4, 11, None, seed=11, ilim=1, lmax=lmax)
+    a, b = Solver(case), Solver(case)
+    for s in (a, b):
+        s.thermo(); s.time_step(); s.ghosts(); s.stresses(); s.residual()
+    a.update()
+    abi.implimf_3d_batch(b.blocks_arr, len(b.blocks), b.ph, b.u, b.dt, b.v, b.d, b.ff, b.mu, b.mut, lmax, b.sn, b.vol,
+                         b.cson, stream=b.stream)
+    torch.cuda.synchronize()
+    assert torch.equal(a.d[:5].cpu(), b.d[:5].cpu())
+    assert torch.equal(a.v.cpu(), b.v.cpu())
+    assert float(a.d[:5].abs().max()) > 0
 
 
 def test_fp32_defect_correction_spalart_allmaras():

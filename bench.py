@@ -82,8 +82,10 @@ def main():
                     help="wall: the 256x128x64 laminar wall block; multiblock: 48 uneven SA blocks, ~40M cells")
     ap.add_argument("--streams", type=int, default=0,
                     help="side streams for the per-block work (0: 4 for multiblock, 1 otherwise)")
-    ap.add_argument("--batched", choices=("auto", "on", "off"), default="auto",
-                    help="residual of all blocks in one launch per pass (auto: on for multiblock)")
+    ap.add_argument("--batched", default="auto",
+                    help="phases of all blocks in one launch per pass: auto (none: the per-block work on concurrent "
+                         "streams is faster on the multiblock workload), "
+                         "on (residual,implicit,turbulence), off, or a comma-separated list of those names")
     ap.add_argument("--graph", choices=("auto", "on", "off"), default="auto",
                     help="replay stages from a captured HIP graph (auto: on for multiblock)")
     ap.add_argument("--implicit", choices=("fp64", "fp32"), default="fp64",
@@ -108,7 +110,7 @@ def main():
     ci, cj, ck = (int(x) for x in args.dims.split(","))
     case = configs.multiblock_sa() if multi else configs.wall_ns(ci, cj, ck)
     nstreams = args.streams or (4 if multi else 1)
-    batched = args.batched == "on" or (args.batched == "auto" and multi)
+    batched = {"auto": "", "on": True, "off": ""}.get(args.batched, args.batched)
     s = Solver(case, implicit=args.implicit, streams=nstreams, batched=batched)
     ncell = case.layout.ncells
     nblk = len(case.layout.blocks)
@@ -169,9 +171,9 @@ def main():
                                + ("FP32 implicit phase (defect correction)" if fp32 else "FP64")
                                + (", stages replayed from a HIP graph" if graph else "")
                                + (f", blocks on {nstreams} concurrent streams" if len(s.side) > 1 else "")
-                               + (", residual and implicit phase of all blocks in one launch per pass" if s.batched
-                                  and not fp32 else
-                                  ", residual of all blocks in one launch per pass" if s.batched else ""),
+                               + (", " + ", ".join(p for p in ("residual", "implicit", "turbulence")
+                                                    if p in s.batch)
+                                  + " of all blocks in one launch per pass" if s.batch else ""),
                    "cells": ncell, "blocks": nblk, "lmax": case.lmax},
         "roofline": {"bound": "hbm", "kernel": sweep_name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

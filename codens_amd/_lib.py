@@ -91,6 +91,7 @@ SIGNATURES = {
     "codens_implimf_eu": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_int, c_vp, c_vp,
                           c_dbl, c_int, c_int, c_vp, c_vp, c_vp],
     "codens_impli2_eqt_3d": [P(CodensBlock), c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp],
+    "codens_impli2_eqt_3d_batch": [c_int, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp],
     "codens_met_samut": [P(CodensBlock), P(c_dbl), c_vp, c_i64, c_vp, c_vp, c_vp],
     "codens_met_kocmut": [P(CodensBlock), P(c_dbl), c_int, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp],
     "codens_met_smkor": [P(CodensBlock), P(c_dbl), c_dbl, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],

@@ -153,6 +153,13 @@ def impli2_eqt_3d(b, u, dt, v, mu, mut, cfke, sn_blk, vol, cmt, stream=None):
                                   _p(sn_blk), _p(vol), cmt, _s(stream)))
 
 
+def impli2_eqt_3d_batch(blocks_arr, nblk, u, dt, v, mu, mut, cfke, sn_all, vol, cmt, stream=None):
+    """impli2_eqt_3d.f90:4 for all blocks in one launch per pass."""
+    assert u.shape[0] >= 7 and v.shape[0] >= 7 and ld(u) == ld(v)
+    _chk(lib.codens_impli2_eqt_3d_batch(int(nblk), blocks_arr, _p(u), ld(v), _p(dt), _p(v), _p(mu), _p(mut), _p(cfke),
+                                        _p(sn_all), _p(vol), cmt, _s(stream)))
+
+
 def met_samut(b, tc, v, mu, mut, stream=None):
     _chk(lib.codens_met_samut(ctypes.byref(b), _tc(tc), _p(v), ld(v), _p(mu), _p(mut), _s(stream)))
 

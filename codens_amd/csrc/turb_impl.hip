@@ -4,10 +4,16 @@
 // eddy viscosity of the SA and k-omega models (met_samut, met_kocmut).
 #include "common.hpp"
 
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 using namespace codens;
 
-#define REAL3(b)                                     \
-  const dim3 xb = xcd_block();                       \
+#define REAL3(b) REAL3_AT(b, xcd_block())
+// the same, for the tile XB of a block's own grid
+#define REAL3_AT(b, XB)                              \
+  const dim3 xb = (XB);                              \
   int c[3];                                          \
   c[0] = (b).i1 + xb.x * TX + threadIdx.x;           \
   c[1] = (b).j1 + xb.y * TY + threadIdx.y;           \
@@ -23,25 +29,38 @@ struct ImpliWs {
   double *la, *lb, *lc, *l6, *l7;  // line-major coefficients and right-hand sides of one sweep
 };
 
+// the 18 planes of nnn of a block's workspace starting at ws
+__host__ __device__ inline ImpliWs impli_ws(double *ws, long nnn) {
+  return ImpliWs{ws,           ws + nnn,      ws + 2 * nnn,  ws + 5 * nnn,  ws + 8 * nnn,
+                 ws + 9 * nnn, ws + 10 * nnn, ws + 11 * nnn, ws + 12 * nnn, ws + 13 * nnn,
+                 ws + 14 * nnn, ws + 15 * nnn, ws + 16 * nnn, ws + 17 * nnn};
+}
+
 // impli2_eqt_3d.f90:113-122: rv and the diagonal's own part
+__device__ __forceinline__ void cells_body(const Blk &b, dim3 tile, const double *__restrict__ v,
+                                           const double *__restrict__ mu, const double *__restrict__ mut,
+                                           const double *__restrict__ dt, const double *__restrict__ vol,
+                                           const double *__restrict__ cfke, double cmt, const ImpliWs &w) {
+  REAL3_AT(b, tile)
+  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
+  w.rv[m] = (mu[n] + mut[n] / cmt) / v[n];
+  w.cd[m] = vol[n] / dt[n] + cfke[n] * vol[n];
+}
+
 __global__ void __launch_bounds__(TX *TY) k_impli2_cells(Blk b, const double *__restrict__ v, long ldv,
                                                          const double *__restrict__ mu,
                                                          const double *__restrict__ mut,
                                                          const double *__restrict__ dt,
                                                          const double *__restrict__ vol,
                                                          const double *__restrict__ cfke, double cmt, ImpliWs w) {
-  REAL3(b)
-  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
-  w.rv[m] = (mu[n] + mut[n] / cmt) / v[n];
-  w.cd[m] = vol[n] / dt[n] + cfke[n] * vol[n];
+  cells_body(b, xcd_block(), v, mu, mut, dt, vol, cfke, cmt, w);
 }
 
 // impli2_eqt_3d.f90:126-208: coefficients of every face i1..i2 (resp. j, k)
 // of the real transverse range; the launch covers the cells up to i2, j2, k2.
-__global__ void __launch_bounds__(TX *TY) k_impli2_faces(Blk b, const double *__restrict__ v, long ldv,
-                                                         const double *__restrict__ sn,
-                                                         const double *__restrict__ vol, ImpliWs w) {
-  const dim3 xb = xcd_block();
+__device__ __forceinline__ void faces_body(const Blk &b, dim3 xb, const double *__restrict__ v, long ldv,
+                                           const double *__restrict__ sn, const double *__restrict__ vol,
+                                           const ImpliWs &w) {
   int c[3];
   c[0] = b.i1 + xb.x * TX + threadIdx.x;
   c[1] = b.j1 + xb.y * TY + threadIdx.y;
@@ -66,14 +85,22 @@ __global__ void __launch_bounds__(TX *TY) k_impli2_faces(Blk b, const double *__
   }
 }
 
+__global__ void __launch_bounds__(TX *TY) k_impli2_faces(Blk b, const double *__restrict__ v, long ldv,
+                                                         const double *__restrict__ sn,
+                                                         const double *__restrict__ vol, ImpliWs w) {
+  faces_body(b, xcd_block(), v, ldv, sn, vol, w);
+}
+
 // impli2_eqt_3d.f90:210-220
-__global__ void __launch_bounds__(TX *TY) k_impli2_diag(Blk b, ImpliWs w) {
-  REAL3(b)
+__device__ __forceinline__ void diag_body(const Blk &b, dim3 tile, const ImpliWs &w) {
+  REAL3_AT(b, tile)
   long m = b.idx(c[0], c[1], c[2]) - b.npc, nnn = b.nnn;
   const double *ce = w.ce;
   w.cd[m] = w.cd[m] + ce[m] + ce[m + 1] + ce[nnn + m] + ce[nnn + m + b.nid] + ce[2 * nnn + m] +
             ce[2 * nnn + m + b.nijd];
 }
+
+__global__ void __launch_bounds__(TX *TY) k_impli2_diag(Blk b, ImpliWs w) { diag_body(b, xcd_block(), w); }
 
 // One sweep along direction d (impli2_eqt_3d.f90:238-275, 292-329, 346-383)
 // in two launches.
@@ -95,10 +122,9 @@ __global__ void __launch_bounds__(TX *TY) k_impli2_diag(Blk b, ImpliWs w) {
 // arithmetic.
 constexpr int RT = 16;
 
-__global__ void __launch_bounds__(RT *RT) k_impli2_rhs(Blk b, int d, const double *__restrict__ u, long ldv,
-                                                        ImpliWs w) {
+__device__ __forceinline__ void rhs_body(const Blk &b, dim3 xb, int d, const double *__restrict__ u, long ldv,
+                                         const ImpliWs &w) {
   __shared__ double sh[5][RT][RT + 1];
-  const dim3 xb = xcd_block();
   const int tx = threadIdx.x % RT, ty = threadIdx.x / RT;
   const int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
   const long np_ = b.hi(p) - b.lo(p), nl = np_ * (b.hi(q) - b.lo(q));
@@ -148,15 +174,19 @@ __global__ void __launch_bounds__(RT *RT) k_impli2_rhs(Blk b, int d, const doubl
   }
 }
 
+__global__ void __launch_bounds__(RT *RT) k_impli2_rhs(Blk b, int d, const double *__restrict__ u, long ldv,
+                                                        ImpliWs w) {
+  rhs_body(b, xcd_block(), d, u, ldv, w);
+}
+
 constexpr int LU = 8;  // recursion steps per batch of loads
 
 // The two recursions of one line, each software-pipelined: the loads of the
 // next LU steps are issued before the arithmetic of the current LU steps, so
 // one memory round trip covers 2 x LU steps of the serial chain.
-__global__ void __launch_bounds__(64) k_impli2_line(Blk b, int d, ImpliWs w) {
+__device__ __forceinline__ void line_body(const Blk &b, int d, long t, const ImpliWs &w) {
   const int p = d == 0 ? 1 : 0, q = d == 2 ? 1 : 2;
   const long np_ = b.hi(p) - b.lo(p), nl = np_ * (b.hi(q) - b.lo(q));
-  const long t = blockIdx.x * 64L + threadIdx.x;
   if (t >= nl) return;
   const int nd = b.hi(d) - b.lo(d), nb = nd / LU;
   const double *A = w.la + t, *B = w.lb + t, *C = w.lc + t, *R6 = w.l6 + t, *R7 = w.l7 + t;
@@ -254,12 +284,20 @@ __global__ void __launch_bounds__(64) k_impli2_line(Blk b, int d, ImpliWs w) {
   }
 }
 
+__global__ void __launch_bounds__(64) k_impli2_line(Blk b, int d, ImpliWs w) {
+  line_body(b, d, blockIdx.x * 64L + threadIdx.x, w);
+}
+
 // :406-416
-__global__ void __launch_bounds__(TX *TY) k_impli2_update(Blk b, double *v, long ldv, ImpliWs w) {
-  REAL3(b)
+__device__ __forceinline__ void impli2_update_body(const Blk &b, dim3 tile, double *v, long ldv, const ImpliWs &w) {
+  REAL3_AT(b, tile)
   long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
   v[5 * ldv + n] = v[5 * ldv + n] + w.dw6[m];
   v[6 * ldv + n] = v[6 * ldv + n] + w.dw7[m];
+}
+
+__global__ void __launch_bounds__(TX *TY) k_impli2_update(Blk b, double *v, long ldv, ImpliWs w) {
+  impli2_update_body(b, xcd_block(), v, ldv, w);
 }
 
 __global__ void k_zero_n(double *p, long cnt) {
@@ -284,8 +322,7 @@ extern "C" int codens_impli2_eqt_3d(const codens_block *cb, const double *u, int
     codens_set_error("codens_impli2_eqt_3d: workspace allocation failed");
     return CODENS_EHIP;
   }
-  ImpliWs w{ws, ws + nnn, ws + 2 * nnn, ws + 5 * nnn, ws + 8 * nnn, ws + 9 * nnn, ws + 10 * nnn, ws + 11 * nnn,
-            ws + 12 * nnn, ws + 13 * nnn, ws + 14 * nnn, ws + 15 * nnn, ws + 16 * nnn, ws + 17 * nnn};
+  ImpliWs w = impli_ws(ws, nnn);
   CodensTimed _tm("impli2_eqt_3d", st);
   // rv, the face arrays and the increments start at zero (ghost entries stay)
   hipLaunchKernelGGL(k_zero_n, dim3((unsigned)((13 * nnn + 255) / 256)), dim3(256), 0, st, ws, 13 * nnn);
@@ -305,6 +342,156 @@ extern "C" int codens_impli2_eqt_3d(const codens_block *cb, const double *u, int
   }
   hipLaunchKernelGGL(k_impli2_update, g, t, 0, st, b, v, (long)ldv, w);
   return codens_check_launch("codens_impli2_eqt_3d");
+}
+
+// ---------------------------------------------------------------------------
+// impli2_eqt_3d for all blocks of a layout: each pass of the phase (zero,
+// cells, faces, diagonal, and per sweep the line systems and the line
+// recursions, then the update) is one flattened launch over every block,
+// with a device table of the blocks' extents and first workgroups per pass.
+// Block l's 18-plane workspace starts at 18 npc.  The line recursion is a
+// serial chain per line, so a launch takes about the time of its longest
+// line: one launch over the lines of all blocks replaces one per block.
+enum { TB_ZERO, TB_CELL, TB_FACE, TB_RHS, TB_LINE0, TB_LINE1, TB_LINE2, TB_N };
+
+struct TurbEnt {
+  Blk b;
+  unsigned start[TB_N];         // first workgroup of this block, per pass
+  unsigned gx[TB_N], gy[TB_N];  // this block's grid, per pass
+};
+
+__global__ void __launch_bounds__(256) k_impli2_zero_batch(const TurbEnt *__restrict__ tab, int nb, double *ws) {
+  const TurbEnt &E = tab[tab_find(tab, nb, TB_ZERO, blockIdx.x)];
+  const long i = (blockIdx.x - E.start[TB_ZERO]) * 256L + threadIdx.x;
+  if (i < 13 * E.b.nnn) ws[18 * E.b.npc + i] = 0.;
+}
+
+__global__ void __launch_bounds__(TX *TY) k_impli2_cells_batch(const TurbEnt *__restrict__ tab, int nb,
+                                                               const double *__restrict__ v,
+                                                               const double *__restrict__ mu,
+                                                               const double *__restrict__ mut,
+                                                               const double *__restrict__ dt,
+                                                               const double *__restrict__ vol,
+                                                               const double *__restrict__ cfke, double cmt,
+                                                               double *ws) {
+  const unsigned wg = xcd_linear();
+  const TurbEnt &E = tab[tab_find(tab, nb, TB_CELL, wg)];
+  const Blk b = E.b;
+  cells_body(b, tab_tile(E, TB_CELL, wg), v, mu, mut, dt, vol, cfke, cmt, impli_ws(ws + 18 * b.npc, b.nnn));
+}
+
+__global__ void __launch_bounds__(TX *TY) k_impli2_faces_batch(const TurbEnt *__restrict__ tab, int nb,
+                                                               const double *__restrict__ v, long ldv,
+                                                               const double *__restrict__ sn_all,
+                                                               const double *__restrict__ vol, double *ws) {
+  const unsigned wg = xcd_linear();
+  const TurbEnt &E = tab[tab_find(tab, nb, TB_FACE, wg)];
+  const Blk b = E.b;
+  faces_body(b, tab_tile(E, TB_FACE, wg), v, ldv, sn_all + 9 * b.npc, vol, impli_ws(ws + 18 * b.npc, b.nnn));
+}
+
+__global__ void __launch_bounds__(TX *TY) k_impli2_diag_batch(const TurbEnt *__restrict__ tab, int nb, double *ws) {
+  const unsigned wg = xcd_linear();
+  const TurbEnt &E = tab[tab_find(tab, nb, TB_CELL, wg)];
+  const Blk b = E.b;
+  diag_body(b, tab_tile(E, TB_CELL, wg), impli_ws(ws + 18 * b.npc, b.nnn));
+}
+
+__global__ void __launch_bounds__(RT *RT) k_impli2_rhs_batch(const TurbEnt *__restrict__ tab, int nb, int d,
+                                                             const double *__restrict__ u, long ldv, double *ws) {
+  const unsigned wg = xcd_linear();
+  const TurbEnt &E = tab[tab_find(tab, nb, TB_RHS, wg)];
+  const Blk b = E.b;
+  rhs_body(b, tab_tile(E, TB_RHS, wg), d, u, ldv, impli_ws(ws + 18 * b.npc, b.nnn));
+}
+
+__global__ void __launch_bounds__(64) k_impli2_line_batch(const TurbEnt *__restrict__ tab, int nb, int d,
+                                                          double *ws) {
+  const TurbEnt &E = tab[tab_find(tab, nb, TB_LINE0 + d, blockIdx.x)];
+  const Blk b = E.b;
+  line_body(b, d, (blockIdx.x - E.start[TB_LINE0 + d]) * 64L + threadIdx.x, impli_ws(ws + 18 * b.npc, b.nnn));
+}
+
+__global__ void __launch_bounds__(TX *TY) k_impli2_update_batch(const TurbEnt *__restrict__ tab, int nb, double *v,
+                                                                long ldv, double *ws) {
+  const unsigned wg = xcd_linear();
+  const TurbEnt &E = tab[tab_find(tab, nb, TB_CELL, wg)];
+  const Blk b = E.b;
+  impli2_update_body(b, tab_tile(E, TB_CELL, wg), v, ldv, impli_ws(ws + 18 * b.npc, b.nnn));
+}
+
+extern "C" int codens_impli2_eqt_3d_batch(int32_t nblk, const codens_block *blocks, const double *u, int64_t ldv,
+                                          const double *dt, double *v, const double *mu, const double *mut,
+                                          const double *cfke, const double *sn_all, const double *vol, double cmt,
+                                          void *stream) {
+  const char *what = "codens_impli2_eqt_3d_batch";
+  if (nblk <= 0 || !blocks || !u || !dt || !v || !mu || !mut || !cfke || !sn_all || !vol || !(cmt > 0.)) {
+    codens_set_error("%s: null or invalid argument", what);
+    return CODENS_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<TurbEnt> ent(nblk);
+  std::vector<int32_t> key;
+  std::vector<std::pair<long, long>> span(nblk);
+  unsigned tot[TB_N] = {0};
+  for (int l = 0; l < nblk; l++) {
+    int rc = codens_check_block(&blocks[l], what);
+    if (rc) return rc;
+    const int32_t *wd = (const int32_t *)&blocks[l];
+    key.insert(key.end(), wd, wd + sizeof(codens_block) / sizeof(int32_t));
+    Blk b = make_blk(&blocks[l]);
+    span[l] = {b.npc, b.npc + b.nnn};
+    TurbEnt &E = ent[l];
+    E.b = b;
+    const unsigned ni = b.i2 - b.i1, nj = b.j2 - b.j1, nk = b.k2 - b.k1;
+    dim3 cg = cell_grid(b);
+    unsigned g[TB_N][3] = {{(unsigned)((13 * b.nnn + 255) / 256), 1, 1},
+                           {cg.x, cg.y, cg.z},
+                           {(ni + 1 + TX - 1) / TX, (nj + 1 + TY - 1) / TY, nk + 1},
+                           {(ni + RT - 1) / RT, (nj + RT - 1) / RT, nk},
+                           {(nj * nk + 63) / 64, 1, 1},
+                           {(ni * nk + 63) / 64, 1, 1},
+                           {(ni * nj + 63) / 64, 1, 1}};
+    for (int p = 0; p < TB_N; p++) {
+      E.start[p] = tot[p];
+      E.gx[p] = g[p][0];
+      E.gy[p] = g[p][1];
+      tot[p] += g[p][0] * g[p][1] * g[p][2];
+    }
+  }
+  std::sort(span.begin(), span.end());
+  for (int l = 0; l < nblk; l++) {
+    if ((l == 0 && span[0].first != 0) || (l > 0 && span[l].first < span[l - 1].second) || span[l].second > ldv) {
+      codens_set_error("%s: the blocks must start at entry 0, not overlap, and lie within ldv", what);
+      return CODENS_EINVAL;
+    }
+  }
+  const long tot_n = span[nblk - 1].second;
+  TurbEnt *tab = (TurbEnt *)codens_device_table(2, key, ent.data(), sizeof(TurbEnt) * nblk, st, what);
+  if (!tab) return CODENS_EHIP;
+  double *ws = (double *)codens_workspace(sizeof(double) * 18 * tot_n, 3);
+  if (!ws) {
+    codens_set_error("%s: workspace allocation failed", what);
+    return CODENS_EHIP;
+  }
+  const TurbEnt *ct = tab;
+  const int nb = nblk;
+  CodensTimed _tm("impli2_eqt_3d", st);
+  hipLaunchKernelGGL(k_impli2_zero_batch, dim3(tot[TB_ZERO]), dim3(256), 0, st, ct, nb, ws);
+  dim3 t(TX, TY);
+  hipLaunchKernelGGL(k_impli2_cells_batch, dim3(tot[TB_CELL]), t, 0, st, ct, nb, (const double *)v, mu, mut, dt, vol,
+                     cfke, cmt, ws);
+  hipLaunchKernelGGL(k_impli2_faces_batch, dim3(tot[TB_FACE]), t, 0, st, ct, nb, (const double *)v, (long)ldv,
+                     sn_all, vol, ws);
+  hipLaunchKernelGGL(k_impli2_diag_batch, dim3(tot[TB_CELL]), t, 0, st, ct, nb, ws);
+  for (int d = 0; d < 3; d++) {
+    for (int sweep = 0; sweep < 2; sweep++) {
+      hipLaunchKernelGGL(k_impli2_rhs_batch, dim3(tot[TB_RHS]), dim3(RT * RT), 0, st, ct, nb, d, u, (long)ldv, ws);
+      hipLaunchKernelGGL(k_impli2_line_batch, dim3(tot[TB_LINE0 + d]), dim3(64), 0, st, ct, nb, d, ws);
+    }
+  }
+  hipLaunchKernelGGL(k_impli2_update_batch, dim3(tot[TB_CELL]), t, 0, st, ct, nb, v, (long)ldv, ws);
+  return codens_check_launch(what);
 }
 
 // met_samut.f90:46-58

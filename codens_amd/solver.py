@@ -24,15 +24,21 @@ def _dev(a, dtype=torch.float64):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
 
 
+BATCH_PHASES = ("residual", "implicit", "turbulence")
+
+
 class Solver:
     def __init__(self, case: Case, stream=None, implicit="fp64", streams=1, batched=False):
         """implicit: "fp64" (implimf_3d as the reference) or "fp32" (its FP32
         defect-correction form: FP64 residual and update, FP32 sub-iterations).
         streams > 1: the per-block work of each phase runs on that many side
         streams (up to 16), each block with the workspace set of its stream.
-        batched: the Navier-Stokes residual of all blocks in one launch per
-        pass (codens_sch_roe_batch) instead of block by block, and so does the
-        FP64 implicit phase (codens_implimf_3d_batch)."""
+        batched: phases run for all blocks in one launch per pass instead of
+        block by block: "residual" (the Navier-Stokes residual,
+        codens_sch_roe_batch), "implicit" (the FP64 implicit phase,
+        codens_implimf_3d_batch), "turbulence" (the turbulence line-implicit
+        phase, codens_impli2_eqt_3d_batch); True for all three, or a
+        collection (or comma-separated string) of their names."""
         if implicit not in ("fp64", "fp32"):
             raise ValueError(f"implicit must be 'fp64' or 'fp32', not {implicit!r}")
         self.implicit = implicit
@@ -75,7 +81,21 @@ class Solver:
         self.cycle = 0
         self.res0 = None
         self.graph = None
-        self.batched = batched and case.ns
+        if batched is True:
+            phases = set(BATCH_PHASES)
+        elif not batched:
+            phases = set()
+        else:
+            phases = set(batched.split(",") if isinstance(batched, str) else batched)
+        if not phases <= set(BATCH_PHASES):
+            raise ValueError(f"batched phases must be among {BATCH_PHASES}, not {sorted(phases)}")
+        if not case.ns:
+            phases.discard("residual")
+        if case.kmf != 1 or implicit != "fp64":
+            phases.discard("implicit")
+        if case.turb is None:
+            phases.discard("turbulence")
+        self.batch = phases
         self.blocks_arr = abi.block_array(self.blocks)
         if not 1 <= streams <= 16:
             raise ValueError("streams must be 1..16")
@@ -178,21 +198,34 @@ class Solver:
         if g.numel():
             abi.met_rbsc(g, i, r, self.fd, stream=s)
 
-        def update(l, b, s):
+        def balance(l, b, s):
             if t.klroe == 2:
                 abi.met_roe2o(b, self.v, self.d, self.sn_blk[l], stream=s)
             else:
                 abi.met_roe(b, self.v, self.d, self.sn_blk[l], stream=s)
             abi.sch_turb(b, self.u, self.v, self.d, self.fd, self.ts6, self.ts7, self.sn_blk[l], self.vol, stream=s)
-            abi.impli2_eqt_3d(b, self.u, self.dt, self.v, self.mu, self.mut, self.cfke, self.sn_blk[l], self.vol,
-                              t.cmt, stream=s)
+
+        def close(l, b, s):
             abi.met_cut(b, 2 if sa else 1, t.epsk, t.epse, self.v, stream=s)
             if sa:
                 abi.met_samut(b, t.tc, self.v, self.mu, self.mut, stream=s)
             else:
                 abi.met_kocmut(b, t.tc, t.kwsst, self.sn_blk[l], self.vol, self.v, self.mu, self.mut, self.dist,
                                self.g9[l], stream=s)
-        self._per_block(update)
+
+        def update(l, b, s):
+            balance(l, b, s)
+            abi.impli2_eqt_3d(b, self.u, self.dt, self.v, self.mu, self.mut, self.cfke, self.sn_blk[l], self.vol,
+                              t.cmt, stream=s)
+            close(l, b, s)
+        if "turbulence" in self.batch:
+            # the line-implicit phase of all blocks in one launch per pass
+            self._per_block(balance)
+            abi.impli2_eqt_3d_batch(self.blocks_arr, len(self.blocks), self.u, self.dt, self.v, self.mu, self.mut,
+                                    self.cfke, self.sn, self.vol, t.cmt, stream=self.stream)
+            self._per_block(close)
+        else:
+            self._per_block(update)
 
     def stresses(self):
         self._per_block(lambda l, b, s: abi.zvisqc(b, self.ph, self.sn_blk[l], self.vol, self.v, self.temp, self.mu,
@@ -205,7 +238,7 @@ class Solver:
             abi.bc_tq_coincident(g, i, r, self.tq, stream=s)
 
     def residual(self):
-        if self.batched:
+        if "residual" in self.batch:
             abi.sch_roe_batch(self.blocks_arr, len(self.blocks), self.ph, self.sc, 0, self.u, self.v, self.ff, self.tq,
                               self.sn, self.ps, stream=self.stream)
             return
@@ -219,7 +252,7 @@ class Solver:
 
     def update(self):
         cs = self.case
-        if self.batched and cs.kmf == 1 and self.implicit == "fp64":
+        if "implicit" in self.batch:
             abi.implimf_3d_batch(self.blocks_arr, len(self.blocks), self.ph, self.u, self.dt, self.v, self.d, self.ff,
                                  self.mu, self.mut, cs.lmax, self.sn, self.vol, self.cson, stream=self.stream)
             return

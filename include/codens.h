@@ -217,6 +217,18 @@ int codens_impli2_eqt_3d(const codens_block *b, const double *u, int64_t ldv, co
                          const double *mu, const double *mut, const double *cfke, const double *sn_blk,
                          const double *vol, double cmt, void *stream);
 
+/* impli2_eqt_3d.f90:4 for every block of a layout in one launch per pass (no
+ * counterpart in the reference, which calls it block by block from met_num,
+ * met_num.f90:697-930): the same results as codens_impli2_eqt_3d called block
+ * by block, bit for bit.  blocks: nblk records of one layout (the first at
+ * entry 0, none overlapping, all within ldv); sn_all: the metric array of the
+ * whole layout (block l's slice at 9 npc).  The block table goes to the
+ * device on the first call for a given set of blocks, which must not happen
+ * while the stream is captured into a graph. */
+int codens_impli2_eqt_3d_batch(int32_t nblk, const codens_block *blocks, const double *u, int64_t ldv,
+                               const double *dt, double *v, const double *mu, const double *mut, const double *cfke,
+                               const double *sn_all, const double *vol, double cmt, void *stream);
+
 /* met_samut.f90:4 -- SA eddy viscosity mut = rho nu~ fv1 (cv1 = tc[7]). */
 int codens_met_samut(const codens_block *b, const double *tc, const double *v, int64_t ldv, const double *mu,
                      double *mut, void *stream);

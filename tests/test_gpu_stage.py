@@ -307,6 +307,28 @@ def test_batched_implicit_matches_blockwise(lmax):
     assert float(a.d[:5].abs().max()) > 0
 
 
+@pytest.mark.parametrize("streams", [1, 4])
+def test_batched_turbulence_implicit_matches_blockwise(streams):
+    """The Spalart-Allmaras stage with its line-implicit phase for all blocks
+    in one launch per pass (codens_impli2_eqt_3d_batch) equals the stage with
+    impli2_eqt_3d called block by block, bit for bit: the transported
+    variable and the eddy viscosity, after one stage and after three."""
+    from codens_amd import configs
+    from codens_amd.solver import Solver
+    case = configs.multiblock_sa(9, 4, 11, None, seed=13, ilim=1, lmax=3)
+    a, b = Solver(case), Solver(case, batched=True, streams=streams)
+    for s in (a, b):
+        s.thermo(); s.time_step(); s.ghosts(); s.turb_stage()
+    torch.cuda.synchronize()
+    assert torch.equal(a.v[5].cpu(), b.v[5].cpu())
+    assert torch.equal(a.mut.cpu(), b.mut.cpu())
+    for _ in range(3):
+        a.stage(); b.stage()
+    torch.cuda.synchronize()
+    assert torch.equal(a.v.cpu(), b.v.cpu())
+    assert torch.equal(a.mut.cpu(), b.mut.cpu())
+
+
 def test_fp32_defect_correction_spalart_allmaras():
     """FP32 implicit phase with the Spalart-Allmaras turbulence stage (its own
     line-implicit phase stays FP64) on the wall block 32x24x16: 400 stages

@@ -470,16 +470,19 @@ __global__ void __launch_bounds__(TXS *TYS) SW_ATTR k_impl_sweep_tiled(Blk b, Ph
                                                                    const T *__restrict__ sn,
                                                                    const T *__restrict__ coefe,
                                                                    const T *__restrict__ diag, TView<T> dold,
-                                                                   TView<T> dnew) {
+                                                                   TView<T> dnew, int rev) {
   // XCD-aware tile order: the hardware deals workgroups to the 8 XCDs round
   // robin; give each XCD a contiguous run of tiles so that j/k neighbours
-  // share its L2.
+  // share its L2.  rev runs each XCD's tiles in reverse: alternate
+  // sub-iterations then begin where the previous one ended, on the data it
+  // left in the last-level cache (the tile order does not change any value).
   constexpr int PL = (TYS + 2) * (TXS + 2), HALO = 2 * TXS + 2 * TYS;
   static_assert(HALO <= TXS * TYS, "one halo cell per thread at most");
   __shared__ T sq[14 * PL];
   int nwg = ntx * nty * nkz;
   int hw = blockIdx.x;
   int lg = (nwg % 8 == 0) ? (hw % 8) * (nwg / 8) + hw / 8 : hw;
+  if (rev) lg = nwg - 1 - lg;
 #include "sweep_tile.inc"
 }
 
@@ -519,17 +522,17 @@ int tile_model(const Blk &b) {
 template <bool EU, typename T>
 void launch_sweep(int tile, const Blk &b, hipStream_t st, Phys ph, int ityprk, const T *u, const T *ff, const T *v,
                   long ldv, const double *ps, const T *sn, const T *coefe, const T *diag, TView<T> in,
-                  TView<T> out) {
+                  TView<T> out, int rev = 0) {
   int nkz = (b.k2 - b.k1 + SW_KS - 1) / SW_KS;
   if (tile == 1) {
     constexpr int X = 32, Y = 16;
     int ntx = (b.i2 - b.i1 + X - 1) / X, nty = (b.j2 - b.j1 + Y - 1) / Y;
     hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, X, Y>), dim3(ntx * nty * nkz), dim3(X, Y), 0, st, b, ph, ityprk,
-                       ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out);
+                       ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
   } else {
     int ntx = (b.i2 - b.i1 + SW_TX - 1) / SW_TX, nty = (b.j2 - b.j1 + SW_TY - 1) / SW_TY;
     hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, SW_TX, SW_TY>), dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY), 0, st,
-                       b, ph, ityprk, ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out);
+                       b, ph, ityprk, ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
   }
 }
 
@@ -678,10 +681,10 @@ static int implimf_common(bool eu, const codens_block *cb, const codens_phys *cp
       CodensTimed _tm("implimf_sweep", st);
       if (eu)
         launch_sweep<true, double>(tile, b, st, ph, (int)ityprk, u, ityprk ? ff : u, v, (long)ldv, psx, sn, coefe,
-                                   diag, in, out);
+                                   diag, in, out, s % 2 == 0);
       else
         launch_sweep<false, double>(tile, b, st, ph, (int)ityprk, u, ityprk ? ff : u, v, (long)ldv, psx, sn, coefe,
-                                    diag, in, out);
+                                    diag, in, out, s % 2 == 0);
     }
   }
   {
@@ -751,8 +754,8 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) k_impl_sweep_batch(const ImplEnt
                                                                    const double *__restrict__ ff,
                                                                    const double *__restrict__ v, long ldv,
                                                                    const double *__restrict__ sn_all, double *ws,
-                                                                   double *d, int out_d) {
-  const unsigned wg = xcd_linear();
+                                                                   double *d, int out_d, int rev) {
+  const unsigned wg = rev ? gridDim.x - 1 - xcd_linear() : xcd_linear();
   const ImplEnt &E = tab[tab_find(tab, nb, IB_SWEEP, wg)];
   const Blk b = E.b;
   double *c = ws + 9 * b.npc;
@@ -849,7 +852,7 @@ extern "C" int codens_implimf_3d_batch(int32_t nblk, const codens_block *blocks,
     } else {
       CodensTimed _tm("implimf_sweep", st);
       hipLaunchKernelGGL(k_impl_sweep_batch, dim3(tot[IB_SWEEP]), dim3(SW_TX * SW_TY), 0, st, ct, nb, ph,
-                         (int)ityprk, u, fx, (const double *)v, (long)ldv, sn_all, ws, d, out_d);
+                         (int)ityprk, u, fx, (const double *)v, (long)ldv, sn_all, ws, d, out_d, s % 2 == 0);
     }
   }
   {
@@ -937,7 +940,7 @@ extern "C" int codens_implimf_3d_f32(const codens_block *cb, const codens_phys *
                                        snF, cF, dgF, in, out);
       CodensTimed _tm("implimf32_sweep", st);
       launch_sweep<false, float>(tile, b, st, ph, (int)ityprk, uS, ityprk ? fS : uS, vS, nnn, (const double *)nullptr,
-                                 snF, cF, dgF, in, out);
+                                 snF, cF, dgF, in, out, s % 2 == 0);
     }
     last = out;
   }

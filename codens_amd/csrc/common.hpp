@@ -63,11 +63,15 @@ constexpr int TX = 64, TY = 4;
 // Work-group coordinates of a 3-D launch, remapped so that each of the 8 XCDs
 // (which receive work-groups round robin in launch order) owns a contiguous
 // share of the grid: j and k neighbours of a tile then sit in the same L2.
-__device__ inline dim3 xcd_block() {
+// With rev, each XCD runs its share in reverse order: a kernel that follows
+// one over the same arrays then begins on the data that kernel touched last,
+// still in the last-level cache.
+__device__ inline dim3 xcd_block(bool rev = false) {
   unsigned gx = gridDim.x, gy = gridDim.y, nwg = gx * gy * gridDim.z;
   unsigned hw = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
   unsigned x = hw % 8u, q = nwg / 8u, r = nwg % 8u;
   unsigned lg = x * q + (x < r ? x : r) + hw / 8u;
+  if (rev) lg = nwg - 1 - lg;
   return dim3(lg % gx, (lg / gx) % gy, lg / (gx * gy));
 }
 

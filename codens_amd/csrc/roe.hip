@@ -427,7 +427,9 @@ __global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(COD
                                                       const double *__restrict__ sn,
                                                       const double *__restrict__ uvw,
                                                       const double *__restrict__ rr, double *__restrict__ H) {
-  face_flux_body(b, xcd_block(), ph, d, ns, ilim, muscl, xk, v, ldv, ps, tq, sn, uvw, rr, H);
+  // the j pass runs in reverse order, so that each pass begins on the
+  // cells (v, p, uvw, tq) the previous one read last
+  face_flux_body(b, xcd_block(d == 1), ph, d, ns, ilim, muscl, xk, v, ldv, ps, tq, sn, uvw, rr, H);
 }
 
 // ---------------------------------------------------------------------------
@@ -462,7 +464,7 @@ __device__ __forceinline__ void gather_body(const Blk &b, unsigned bx, long n0, 
 
 __global__ void __launch_bounds__(256) k_gather(Blk b, long n0, long n1, const double *__restrict__ H, double *u,
                                                 long ldv, double *ff, int ityprk) {
-  gather_body(b, blockIdx.x, n0, n1, H, u, ldv, ff, ityprk);
+  gather_body(b, gridDim.x - 1 - blockIdx.x, n0, n1, H, u, ldv, ff, ityprk);  // after the k pass, in reverse
 }
 
 static int roe_common(const codens_block *cb, const codens_phys *cp, const codens_scheme *sc, int ns,

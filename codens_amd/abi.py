@@ -369,7 +369,7 @@ def adj_metric_vjp(b, x, y, z, sg, snbar, volbar, xbar, ybar, zbar, stream=None)
 
 def sweep_tile(mode):
     """Tile shape of the Jacobi sub-iteration: -1 timed choice per block shape
-    (default), 0 = 64 x 8, 1 = 32 x 16.  Returns the previous mode."""
+    (default), 0 = 64 x 4, 1 = 32 x 8.  Returns the previous mode."""
     prev = ctypes.c_int32(0)
     _chk(lib.codens_sweep_tile(int(mode), ctypes.byref(prev)))
     return prev.value

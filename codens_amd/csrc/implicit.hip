@@ -279,9 +279,12 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef_eu(Blk b, const double *__
 // increments d2..d4, the nine dF, d1, d5) is formed once per sweep and kept
 // in LDS for the i/j neighbours of its plane and in registers for the k
 // neighbours; the k face term is carried from one plane to the next.
-// LDS: 14 x (SW_TY+2) x (SW_TX+2) doubles = 73,920 bytes.
+// LDS: 14 x (SW_TY+2) x (SW_TX+2) doubles = 44,352 bytes.  A 64 x 4 tile of
+// 256 threads runs two workgroups per CU at its ~237 VGPRs (one workgroup
+// waits at its plane barrier while the other computes): 125.6 us on the
+// 2.1M-cell block against 133.1 for 64 x 8, despite its larger halo.
 #ifndef CODENS_SW_TY
-#define CODENS_SW_TY 8
+#define CODENS_SW_TY 4
 #endif
 #ifndef CODENS_SW_KS
 #define CODENS_SW_KS 8
@@ -454,10 +457,10 @@ __device__ __forceinline__ void sweep_body(const Blk b, int lg, int ntx, int nty
                                            const T *__restrict__ u, const T *__restrict__ ff,
                                            const T *__restrict__ v, long ldv, const double *__restrict__ ps,
                                            const T *__restrict__ sn, const T *__restrict__ coefe,
-                                           const T *__restrict__ diag, const TView<T> dold, const TView<T> dnew) {
+                                           const T *__restrict__ diag, const TView<T> dold, const TView<T> dnew,
+                                           T *sq) {
   constexpr int PL = (TYS + 2) * (TXS + 2), HALO = 2 * TXS + 2 * TYS;
   static_assert(HALO <= TXS * TYS, "one halo cell per thread at most");
-  __shared__ T sq[14 * PL];
 #include "sweep_tile.inc"
 }
 
@@ -487,11 +490,11 @@ __global__ void __launch_bounds__(TXS *TYS) SW_ATTR k_impl_sweep_tiled(Blk b, Ph
 }
 
 // ---------------------------------------------------------------------------
-// Tile shape of the tiled sub-iteration, per block shape.  Two shapes of 512
-// threads: 64 x 8 and 32 x 16.  Every cell's arithmetic is the same in both,
-// so the increments are identical bit for bit; what differs is how many
-// lanes idle on the i and j edges of a block and the LDS halo per cell
-// (1.29 against 1.20).  By default the first call for a block shape times
+// Tile shape of the tiled sub-iteration, per block shape.  Two shapes of the
+// same thread count: SW_TX x SW_TY and 32 x 2 SW_TY (64 x 4 and 32 x 8).
+// Every cell's arithmetic is the same in both, so the increments are
+// identical bit for bit; what differs is how many lanes idle on the i and j
+// edges of a block and the LDS halo per cell.  By default the first call for a block shape times
 // both on the caller's own data (one untimed and two timed launches each,
 // on the caller's stream, writing the same result the real launch writes
 // next) and keeps the faster; while the stream is being captured into a
@@ -504,7 +507,7 @@ int g_tile_mode = -1;
 std::map<std::array<int, 5>, int> g_tile_best;
 
 inline int tile_tx(int tile) { return tile == 1 ? 32 : SW_TX; }
-inline int tile_ty(int tile) { return tile == 1 ? 16 : SW_TY; }
+inline int tile_ty(int tile) { return tile == 1 ? 2 * SW_TY : SW_TY; }
 
 int tile_model(const Blk &b) {
   int nx = b.i2 - b.i1, ny = b.j2 - b.j1;
@@ -525,7 +528,7 @@ void launch_sweep(int tile, const Blk &b, hipStream_t st, Phys ph, int ityprk, c
                   TView<T> out, int rev = 0) {
   int nkz = (b.k2 - b.k1 + SW_KS - 1) / SW_KS;
   if (tile == 1) {
-    constexpr int X = 32, Y = 16;
+    constexpr int X = 32, Y = 2 * SW_TY;
     int ntx = (b.i2 - b.i1 + X - 1) / X, nty = (b.j2 - b.j1 + Y - 1) / Y;
     hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, X, Y>), dim3(ntx * nty * nkz), dim3(X, Y), 0, st, b, ph, ityprk,
                        ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
@@ -579,7 +582,7 @@ int pick_tile(const Blk &b, hipStream_t st, Phys ph, int ityprk, const T *u, con
 
 extern "C" int codens_sweep_tile(int32_t mode, int32_t *previous) {
   if (mode < -1 || mode >= N_SWEEP_TILES) {
-    codens_set_error("codens_sweep_tile: mode %d is not -1 (timed choice), 0 (64 x 8) or 1 (32 x 16)", mode);
+    codens_set_error("codens_sweep_tile: mode %d is not -1 (timed choice), 0 (64 x 4) or 1 (32 x 8)", mode);
     return CODENS_EINVAL;
   }
   std::lock_guard<std::mutex> lk(g_tile_mu);
@@ -709,7 +712,7 @@ struct ImplEnt {
   Blk b;
   unsigned start[IB_N];         // first workgroup of this block, per pass
   unsigned gx[IB_N], gy[IB_N];  // this block's grid, per pass
-  int tile;                     // sub-iteration tile shape of this block (0: 64 x 8, 1: 32 x 16)
+  int tile;                     // sub-iteration tile shape of this block (0: 64 x 4, 1: 32 x 8)
 };
 
 __global__ void __launch_bounds__(256) k_zero5_batch(const ImplEnt *__restrict__ tab, int nb, double *d, long ldv) {
@@ -755,6 +758,8 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) k_impl_sweep_batch(const ImplEnt
                                                                    const double *__restrict__ v, long ldv,
                                                                    const double *__restrict__ sn_all, double *ws,
                                                                    double *d, int out_d, int rev) {
+  constexpr int PL0 = (SW_TY + 2) * (SW_TX + 2), PL1 = (2 * SW_TY + 2) * (32 + 2);
+  __shared__ double sq[14 * (PL0 > PL1 ? PL0 : PL1)];  // one buffer for both shapes
   const unsigned wg = rev ? gridDim.x - 1 - xcd_linear() : xcd_linear();
   const ImplEnt &E = tab[tab_find(tab, nb, IB_SWEEP, wg)];
   const Blk b = E.b;
@@ -762,11 +767,11 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) k_impl_sweep_batch(const ImplEnt
   DView A = {d, ldv, 0}, B = {c + 4 * b.nnn, b.nnn, b.npc};
   const int lg = (int)(wg - E.start[IB_SWEEP]), ntx = (int)E.gx[IB_SWEEP], nty = (int)E.gy[IB_SWEEP];
   if (E.tile == 1)
-    sweep_body<false, double, 32, 16>(b, lg, ntx, nty, ph, ityprk, u, ff, v, ldv, v, sn_all + 9 * b.npc, c,
-                                      c + 3 * b.nnn, out_d ? B : A, out_d ? A : B);
+    sweep_body<false, double, 32, 2 * SW_TY>(b, lg, ntx, nty, ph, ityprk, u, ff, v, ldv, v, sn_all + 9 * b.npc, c,
+                                             c + 3 * b.nnn, out_d ? B : A, out_d ? A : B, sq);
   else
     sweep_body<false, double, SW_TX, SW_TY>(b, lg, ntx, nty, ph, ityprk, u, ff, v, ldv, v, sn_all + 9 * b.npc, c,
-                                            c + 3 * b.nnn, out_d ? B : A, out_d ? A : B);
+                                            c + 3 * b.nnn, out_d ? B : A, out_d ? A : B, sq);
 }
 
 __global__ void __launch_bounds__(TX *TY) k_impl_update_batch(const ImplEnt *__restrict__ tab, int nb, double *v,

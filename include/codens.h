@@ -482,9 +482,9 @@ int codens_workspace_set(int32_t set);
 
 /* Tile shape of the tiled Jacobi sub-iteration of codens_implimf_3d,
  * codens_implimf_eu and codens_implimf_3d_f32 (no counterpart in the
- * reference): -1 (default) times the 64 x 8 and 32 x 16 shapes on the first
- * call for each block shape and keeps the faster, 0 forces 64 x 8, 1 forces
- * 32 x 16.  Both give identical results.  *previous (may be NULL) receives
+ * reference): -1 (default) times the 64 x 4 and 32 x 8 shapes on the first
+ * call for each block shape and keeps the faster, 0 forces 64 x 4, 1 forces
+ * 32 x 8.  Both give identical results.  *previous (may be NULL) receives
  * the mode in force before the call. */
 int codens_sweep_tile(int32_t mode, int32_t *previous);
 

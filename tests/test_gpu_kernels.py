@@ -297,7 +297,7 @@ def test_bad_extents_rejected(ctx):
 
 @pytest.mark.parametrize("form", ["fp64", "eu", "fp32"])
 def test_sweep_tile_shapes_identical(form):
-    """The two tile shapes of the Jacobi sub-iteration (64 x 8 and 32 x 16)
+    """The two tile shapes of the Jacobi sub-iteration (64 x 4 and 32 x 8)
     and the timed choice give bit-identical increments, on a block whose i
     and j extents are multiples of neither tile (70 x 37 x 9 cells)."""
     from codens_amd import abi

@@ -40,9 +40,9 @@ sys.path.insert(0, ROOT)
 SWEEP_BYTES_PER_CELL = 8 * (5 + 5 + 5 + 9 + 1 + 3 + 5)  # u, v, d_in, sn, diag, coefe, d_out
 # Memory traffic of one sweep launch on the 256x128x64 workload: 2.0 x FETCH_SIZE
 # + 1.0 x WRITE_SIZE (factors from a 512 MiB device copy in the same run),
-# 631,669 KiB = 646.8 MB against 553.6 MB algorithmic
+# 665,962 KiB = 681.9 MB against 553.6 MB algorithmic (64x4 tiles)
 # (profiles/traffic_implimf_sweep_wall_ns_256x128x64.md).
-SWEEP_TRAFFIC_BYTES = 631669 * 1024
+SWEEP_TRAFFIC_BYTES = 665962 * 1024
 HBM_PEAK_GBS = 8000.0
 
 

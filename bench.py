@@ -207,7 +207,7 @@ def converge_bench(args):
     import torch
     from codens_amd import configs
     from codens_amd.solver import Solver
-    case = configs.adjoint_wall(32, 16, 8, alpha=10.0, beta=5.0, eta=10.0, amp=0.0, bump=0.0)
+    case = configs.adjoint_wall(32, 16, 8, eta=10.0, amp=0.0, bump=0.0)
     ncell = case.layout.ncells
     runs = {}
     for mode in ("fp64", "fp32"):

@@ -83,6 +83,11 @@ SIGNATURES = {
     "codens_bc_wall": [c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
     "codens_bc_farfield": [c_i64, c_vp, c_vp, c_vp, P(c_dbl), P(CodensPhys), c_vp, c_i64, c_vp, c_vp, c_vp,
                            c_vp],
+    "codens_residu": [c_vp, c_int, c_int, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
+    "codens_check_state": [c_vp, c_int, c_int, P(CodensPhys), c_vp, c_i64, c_vp, c_vp],
+    "codens_bc_sym": [c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp],
+    "codens_rfve": [c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
+    "codens_utnrd": [c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, P(CodensPhys), P(c_dbl)],
     "codens_bc_tq_copy": [c_i64, c_vp, c_vp, c_vp, c_i64, c_vp],
     "codens_bc_tq_coincident": [c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp],
     "codens_sumsq_rate": [P(CodensBlock), c_int, c_vp, c_vp, c_vp, P(c_dbl), c_vp],

@@ -310,6 +310,24 @@ def bc_farfield(g, i, nrm, w0, ph, v, ps, temp, cson, stream=None):
                                 _p(temp), _p(cson), _s(stream)))
 
 
+def rfve(g, i, v, ps, temp, cson, stream=None):
+    """rfve.f90: ghost = adjacent real cell (v, p, T, c)."""
+    _chk(lib.codens_rfve(g.numel(), _p(g), _p(i), _p(v), ld(v), _p(ps), _p(temp), _p(cson), _s(stream)))
+
+
+def bc_sym(g, i, nrm, v, stream=None):
+    """clsym.f90: symmetry-plane ghosts (writes v only)."""
+    _chk(lib.codens_bc_sym(g.numel(), _p(g), _p(i), _p(nrm), _p(v), ld(v), _s(stream)))
+
+
+def utnrd(mach, alpha, beta, roa1, aa1, pa1, ph):
+    """utnrd.f90: the far-field conservative state (5 floats) of Mach number,
+    incidence angles and stagnation state."""
+    w = (ctypes.c_double * 5)()
+    _chk(lib.codens_utnrd(mach, alpha, beta, roa1, aa1, pa1, ctypes.byref(ph), w))
+    return list(w)
+
+
 def bc_tq_copy(g, i, tq, stream=None):
     _chk(lib.codens_bc_tq_copy(g.numel(), _p(g), _p(i), _p(tq), ld(tq), _s(stream)))
 
@@ -323,6 +341,19 @@ def sumsq_rate(blocks, a, bv, dt, stream=None):
     out = ctypes.c_double(0.0)
     _chk(lib.codens_sumsq_rate(arr, len(blocks), _p(a), _p(bv), _p(dt), ctypes.byref(out), _s(stream)))
     return out.value
+
+
+def residu(blocks_arr, nblk, neq, v, u, dt, out, stream=None):
+    """residu.f90 / rscpsv.f90: steadiness norms of (v - u) / dt into the
+    device array out ((nblk + 1) x neq x 7 doubles: L1, L2, Linf, block, i, j,
+    k per block and equation, then the aggregate)."""
+    _chk(lib.codens_residu(blocks_arr, nblk, neq, _p(v), _p(u), ld(v), _p(dt), _p(out), _s(stream)))
+
+
+def check_state(blocks_arr, nblk, neq, ph, v, nbad, stream=None):
+    """Adds to the device int nbad the real cells with rho <= 0, p <= 0 or a
+    non-finite value in the first neq variables."""
+    _chk(lib.codens_check_state(blocks_arr, nblk, neq, ctypes.byref(ph), _p(v), ld(v), _p(nbad), _s(stream)))
 
 
 def timing_enable(on=True):

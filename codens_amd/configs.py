@@ -20,6 +20,7 @@ Seeds: numpy PCG64(20261201 + case index).
 from __future__ import annotations
 
 import dataclasses
+import math
 from typing import Dict, List, Optional, Tuple
 
 import numpy as np
@@ -31,10 +32,29 @@ GAM = 1.4
 
 @dataclasses.dataclass
 class FaceSet:
+    """Boundary faces of one kind, frontier after frontier in the reference's
+    listing order (initis.f90): flat indices of the ghost cell, the adjacent
+    real cell and, for coincident faces, the real cell across the interface.
+    sizes[f] faces belong to frontier f (of block blocks[f]); coincident
+    frontiers come in pairs 2m, 2m+1 that are each other's partner
+    (ndcc, rbvc.f90)."""
     ghost: np.ndarray
     inner: np.ndarray
     remote: Optional[np.ndarray] = None
     normals: Optional[np.ndarray] = None  # (3, nf) inward unit normals
+    sizes: Optional[List[int]] = None
+    blocks: Optional[List[int]] = None
+
+    def __post_init__(self):
+        if self.sizes is None:
+            self.sizes = [len(self.ghost)] if len(self.ghost) else []
+        if self.blocks is None:
+            self.blocks = [0] * len(self.sizes)
+
+    def frontier(self, f):
+        """Slice of frontier f in the face arrays."""
+        o = int(sum(self.sizes[:f]))
+        return slice(o, o + self.sizes[f])
 
 
 @dataclasses.dataclass
@@ -74,11 +94,17 @@ class Case:
     dt1min: float
     kmf: int
     lmax: int
-    winf: np.ndarray  # far-field conservative state
+    winf: np.ndarray  # far-field conservative state (FarField.state of farspec)
     coincident: FaceSet
     wall: FaceSet
     farfield: FaceSet
     turb: Optional[TurbCfg] = None
+    farspec: Optional[FarField] = None
+    sym: Optional[FaceSet] = None  # symmetry planes (clsym.f90)
+
+    def __post_init__(self):
+        if self.sym is None:
+            self.sym = _empty_faces()
 
 
 def conservative(rho, u, v, w, p, gam=GAM, pinfl=0.0):
@@ -87,7 +113,7 @@ def conservative(rho, u, v, w, p, gam=GAM, pinfl=0.0):
 
 def _empty_faces():
     z = np.zeros(0, dtype=np.int64)
-    return FaceSet(z, z, z, np.zeros((3, 0)))
+    return FaceSet(z, z, z, np.zeros((3, 0)), [], [])
 
 
 def _cat(parts: List[FaceSet]) -> FaceSet:
@@ -97,7 +123,49 @@ def _cat(parts: List[FaceSet]) -> FaceSet:
     i = np.concatenate([p.inner for p in parts])
     r = np.concatenate([p.remote for p in parts]) if parts[0].remote is not None else None
     nrm = np.concatenate([p.normals for p in parts], axis=1) if parts[0].normals is not None else None
-    return FaceSet(g, i, r, nrm)
+    sizes = [n for p in parts for n in p.sizes]
+    blocks = [l for p in parts for l in p.blocks]
+    return FaceSet(g, i, r, nrm, sizes, blocks)
+
+
+@dataclasses.dataclass
+class FarField:
+    """A non-reflecting far field as the reference's deck gives it (boundary
+    condition 'nrd', rbord.f90:300-323): Mach number, incidence angles alpha
+    and beta in degrees (bceqt(:,1:3)) and the stagnation state roa1, aa1,
+    pa1 (c_nzst.f90:56-59).  Only a non-rotating frame (omg = 0)."""
+    mach: float
+    alpha: float = 0.0
+    beta: float = 0.0
+    roa1: float = 1.0
+    aa1: float = 1.0
+    pa1: float = 1.0 / GAM
+
+    def state(self, gam=GAM, pinfl=0.0):
+        """The conservative far-field state of utnrd.f90:59-105 with omg = 0:
+        static density, sound speed and pressure from the isentropic
+        relations, velocity (cos a cos b, -sin b, sin a cos b) x M a.  With
+        the same libm calls and operation order as utnrd, so the state is the
+        reference's to the last bit."""
+        gam1 = gam - 1.0
+        gam2, gam4 = 0.5 * gam1, 1.0 / gam1
+        degrad = math.atan2(1.0, 0.0) / 90.0
+        f = 1.0 + gam2 * (self.mach * self.mach)
+        ro = self.roa1 / math.pow(f, gam4)
+        a = self.aa1 / math.pow(f, 0.5)
+        p = self.pa1 / math.pow(f, gam / gam1)
+        q = self.mach * a
+        al, be = self.alpha * degrad, self.beta * degrad
+        return np.array([ro, ro * q * math.cos(al) * math.cos(be), -ro * q * math.sin(be),
+                         ro * q * math.sin(al) * math.cos(be), p / gam1 + pinfl + 0.5 * ro * (q * q)])
+
+
+def static_farfield(mach, alpha=0.0, beta=0.0, gam=GAM):
+    """The far field whose static state is rho = 1, c = 1 (p = 1/gam) at
+    Mach `mach`: the stagnation state of the isentropic relations."""
+    f = 1.0 + 0.5 * (gam - 1.0) * mach * mach
+    return FarField(mach, alpha, beta, roa1=f ** (1.0 / (gam - 1.0)), aa1=math.sqrt(f),
+                    pa1=f ** (gam / (gam - 1.0)) / gam)
 
 
 def cell_centres(b: L.Block, x, y, z):
@@ -143,8 +211,8 @@ def periodic_euler(ci=64, cj=32, ck=4, seed=20261201, mach=0.8, cfl=0.8, amp=0.0
     for d in "ijk":
         g1, i1, _ = L.face_cells(b, d + "1")
         g2, i2, _ = L.face_cells(b, d + "2")
-        parts.append(FaceSet(g1, i1, i2, None))
-        parts.append(FaceSet(g2, i2, i1, None))
+        parts.append(FaceSet(g1, i1, i2, None, [len(g1)], [0]))
+        parts.append(FaceSet(g2, i2, i1, None, [len(g2)], [0]))
     coin = _cat(parts)
     coin.normals = np.zeros((3, len(coin.ghost)))
     # global step (kdtl=0): every stored dt becomes dt1min (chrono.f90:133-152).
@@ -168,7 +236,9 @@ def periodic_euler(ci=64, cj=32, ck=4, seed=20261201, mach=0.8, cfl=0.8, amp=0.0
 
 
 def wall_ns(ci=256, cj=128, ck=64, seed=20261202, mach=0.5, reynz=1e5, eta=10.0, lmax=10, ilim=1,
-            beta=2.5, disp=0.05) -> Case:
+            beta=2.5, disp=0.05, span="farfield") -> Case:
+    """span: the k1 and k2 faces are far field ("farfield") or symmetry
+    planes ("sym", clsym.f90)."""
     lay = L.Layout([(ci, cj, ck)])
     b = lay.blocks[0]
     Lx, Ly, Lz = 2.0, 1.0, 0.5
@@ -192,25 +262,29 @@ def wall_ns(ci=256, cj=128, ck=64, seed=20261202, mach=0.5, reynz=1e5, eta=10.0,
     b.view(x)[sl] = Xd; b.view(y)[sl] = Yd; b.view(z)[sl] = Zd
     sn, vol = L.metrics(lay, x, y, z)
     n = lay.ip11
-    v0 = conservative(np.ones(n), np.full(n, mach), np.zeros(n), np.zeros(n), np.full(n, 1.0 / GAM))
+    far = static_farfield(mach)
+    winf = far.state()
+    v0 = np.repeat(winf[:, None], n, axis=1)
     global _last_wall_distance
     yc = 0.125 * sum(Yd[dk:dk + ck, dj:dj + cj, di:di + ci] for dk in (0, 1) for dj in (0, 1) for di in (0, 1))
     dist = np.ones(n)
     b.view(dist)[b.kk1 - b.kd1:b.kk2 - b.kd1, b.jj1 - b.jd1:b.jj2 - b.jd1, b.ii1 - b.id1:b.ii2 - b.id1] = yc
     _last_wall_distance = dist
-    wall_parts, far_parts = [], []
+    if span not in ("farfield", "sym"):
+        raise ValueError(f"span must be 'farfield' or 'sym', not {span!r}")
+    wall_parts, far_parts, sym_parts = [], [], []
     for f in L.FACES:
         g, i, fi = L.face_cells(b, f)
         if f == "j1":
             wall_parts.append(FaceSet(g, i))
         else:
             nx, ny, nz = L.inward_normals(b, sn, f, fi)
-            far_parts.append(FaceSet(g, i, None, np.stack([nx, ny, nz])))
+            (sym_parts if span == "sym" and f[0] == "k" else far_parts).append(
+                FaceSet(g, i, None, np.stack([nx, ny, nz])))
     return Case("wall_ns", lay, sn, vol, v0, True,
                 dict(gam=GAM, pinfl=0.0, pr=0.72, prt=1.0, reynz=reynz, tnz=288.15, iflu=1),
-                dict(muscl=1, ilim=ilim, xk=1.0 / 3.0), eta, 1, 0.0, 1, lmax,
-                conservative(np.array(1.0), np.array(mach), np.array(0.0), np.array(0.0), np.array(1 / GAM)),
-                _empty_faces(), _cat(wall_parts), _cat(far_parts))
+                dict(muscl=1, ilim=ilim, xk=1.0 / 3.0), eta, 1, 0.0, 1, lmax, winf,
+                _empty_faces(), _cat(wall_parts), _cat(far_parts), farspec=far, sym=_cat(sym_parts))
 
 
 _last_wall_distance = None
@@ -328,8 +402,8 @@ def multiblock_sa(nblocks=48, lo=16, hi=192, total=(38_000_000, 42_000_000), see
         gb, ib, _ = fc(b, g)
         a0, a1 = min(gp.shape[0], gb.shape[0]), min(gp.shape[1], gb.shape[1])
         P = (slice(0, a0), slice(0, a1))
-        coin.append(FaceSet(gp[P].ravel(), ip_[P].ravel(), ib[P].ravel()))
-        coin.append(FaceSet(gb[P].ravel(), ib[P].ravel(), ip_[P].ravel()))
+        coin.append(FaceSet(gp[P].ravel(), ip_[P].ravel(), ib[P].ravel(), None, [a0 * a1], [p]))
+        coin.append(FaceSet(gb[P].ravel(), ib[P].ravel(), ip_[P].ravel(), None, [a0 * a1], [b]))
         used[(p, f)][P] = True
         used[(b, g)][P] = True
     wall_parts, far_parts = [], []
@@ -341,17 +415,18 @@ def multiblock_sa(nblocks=48, lo=16, hi=192, total=(38_000_000, 42_000_000), see
                 continue
             g, i, fi = g[m], i[m], fi[m]
             if l == 0 and f == "j1":
-                wall_parts.append(FaceSet(g, i))
+                wall_parts.append(FaceSet(g, i, None, None, [len(g)], [l]))
             else:
                 nx, ny, nz = L.inward_normals(b, sn, f, fi)
-                far_parts.append(FaceSet(g, i, None, np.stack([nx, ny, nz])))
+                far_parts.append(FaceSet(g, i, None, np.stack([nx, ny, nz]), [len(g)], [l]))
     coinc = _cat(coin)
     coinc.normals = np.zeros((3, len(coinc.ghost)))
     mu_inf = 1.0 / reynz  # Sutherland at T = 1 (zvismo.f90:90-104)
+    far = static_farfield(mach)
+    w = far.state()  # far-field state of the mean flow (clnrd)
     v0 = np.zeros((7, n))
-    v0[:5] = conservative(np.ones(n), np.full(n, mach), np.zeros(n), np.zeros(n), np.full(n, 1.0 / GAM))
+    v0[:5] = w[:, None]
     v0[5] = nut_ratio * mu_inf
-    w = v0[:5, 0].copy()  # far-field state of the mean flow (clnrd)
     y0 = origin[0, 1] * h
     dist = np.ones(n)
     for l, b in enumerate(B):
@@ -362,23 +437,25 @@ def multiblock_sa(nblocks=48, lo=16, hi=192, total=(38_000_000, 42_000_000), see
     c = Case(f"multiblock_sa_{nblocks}", lay, sn, vol, v0, True,
              dict(gam=GAM, pinfl=0.0, pr=0.72, prt=1.0, reynz=reynz, tnz=288.15, iflu=1),
              dict(muscl=1, ilim=ilim, xk=1.0 / 3.0), eta, 1, 0.0, 1, lmax, w,
-             coinc, _cat(wall_parts), _cat(far_parts))
+             coinc, _cat(wall_parts), _cat(far_parts), farspec=far)
     c.turb = TurbCfg("1SA", turb_constants("1SA"), dist, klroe=2)
     c.links = links
     c.origin = origin
     return c
 
 
-def adjoint_wall(ci=128, cj=64, ck=32, seed=20261205, mach=0.5, reynz=1e5, alpha=10.0, beta=5.0, amp=0.01,
+def adjoint_wall(ci=128, cj=64, ck=32, seed=20261205, mach=0.5, reynz=1e5, alpha=5.0, beta=-10.0, amp=0.01,
                  bump=0.01, tanh_beta=2.5, disp=0.05, eta=1.0, nwall_nodes=4096) -> Case:
     """Viscous wall block for the discrete adjoint: the curvilinear mesh of
     wall_ns (seeded 5% node displacement, tanh stretching toward the no-slip
     adiabatic wall at j=1, far field elsewhere) with a smooth wall bump
     y += bump sin(2 pi x / Lx) cos(2 pi z / Lz) (1 - eta_j)^2, so that the
     wall faces have an x component; laminar Navier-Stokes, Roe + unlimited
-    MUSCL (ilim = 0); free stream at Mach `mach` inclined by alpha (x-y) and
-    beta (x-z) degrees, so that no far-field face has a zero normal velocity
-    (the switch of clnrd.f90); initial state = free stream x (1 + amp x 4
+    MUSCL (ilim = 0); free stream at Mach `mach` with the incidence angles
+    alpha and beta of utnrd.f90 (direction (cos a cos b, -sin b, sin a cos b);
+    the defaults point 10 degrees away from the wall and 5 across the span),
+    so that no far-field face has a zero normal velocity (the switch of
+    clnrd.f90); initial state = free stream x (1 + amp x 4
     seeded sinusoidal modes) on every variable.  Local steps at CFL `eta`.
 
     Extra attributes: nodes (x, y, z storage arrays, ghost nodes
@@ -409,9 +486,8 @@ def adjoint_wall(ci=128, cj=64, ck=32, seed=20261205, mach=0.5, reynz=1e5, alpha
     b.view(x)[sl] = Xd; b.view(y)[sl] = Yd; b.view(z)[sl] = Zd
     sn, vol = L.metrics(lay, x, y, z)  # also extrapolates the ghost nodes
     n = lay.ip11
-    a, bb = np.radians(alpha), np.radians(beta)
-    uu, vv, ww = mach * np.cos(a) * np.cos(bb), mach * np.sin(a) * np.cos(bb), mach * np.sin(bb)
-    winf = conservative(np.array(1.0), np.array(uu), np.array(vv), np.array(ww), np.array(1 / GAM))
+    far = static_farfield(mach, alpha, beta)
+    winf = far.state()
     xc, yc, zc = cell_centres(b, x, y, z)
     kvs = rng.integers(1, 4, size=(4, 3))
     phs = rng.uniform(0, 2 * np.pi, size=(4, 5))
@@ -433,7 +509,7 @@ def adjoint_wall(ci=128, cj=64, ck=32, seed=20261205, mach=0.5, reynz=1e5, alpha
     c = Case("adjoint_wall", lay, sn, vol, v0, True,
              dict(gam=GAM, pinfl=0.0, pr=0.72, prt=1.0, reynz=reynz, tnz=288.15, iflu=1),
              dict(muscl=1, ilim=0, xk=1.0 / 3.0), eta, 1, 0.0, 1, 10, winf,
-             _empty_faces(), _cat(wall_parts), _cat(far_parts))
+             _empty_faces(), _cat(wall_parts), _cat(far_parts), farspec=far)
     c.nodes = (x, y, z)
     ii, kk = np.meshgrid(np.arange(b.ii1, b.ii2 + 1), np.arange(b.kk1, b.kk2 + 1), indexing="ij")
     allw = b.idx(ii.ravel(), b.jj1, kk.ravel()).astype(np.int64)

@@ -52,8 +52,14 @@ int codens_check_block(const codens_block *b, const char *what) {
 
 // Grow-only device workspaces, kept until process exit: per device, up to
 // CODENS_WS_SETS independent sets of slots.  The entry points draw from the
-// set selected by codens_workspace_set (0 by default); a caller that runs
-// blocks concurrently on several streams gives each stream its own set.
+// set selected by codens_workspace_set (0 by default, per host thread); a
+// caller that runs blocks concurrently on several streams gives each stream
+// its own set.
+//
+// A slot that must grow gets a new buffer and the old one is retired, not
+// freed: a HIP graph captured earlier holds the old address, and its replays
+// must keep writing memory that nothing else is given.  So growing never
+// synchronises or frees, and is also safe while another stream runs.
 namespace {
 constexpr int CODENS_WS_SETS = 16;
 struct Slot {
@@ -62,7 +68,8 @@ struct Slot {
 };
 std::mutex g_ws_mu;
 std::vector<std::vector<std::vector<Slot>>> g_ws;  // [device][set][slot]
-int g_ws_set = 0;
+std::vector<void *> g_ws_retired;                  // outgrown buffers, live to the end
+thread_local int g_ws_set = 0;
 }  // namespace
 
 extern "C" int codens_workspace_set(int32_t set) {
@@ -85,16 +92,10 @@ void *codens_workspace(size_t bytes, int slot) {
   if ((int)v.size() <= slot) v.resize(slot + 1);
   Slot &s = v[slot];
   if (s.n < bytes) {
-    if (s.p) {
-      if (hipDeviceSynchronize() != hipSuccess) return nullptr;
-      (void)hipFree(s.p);
-      s.p = nullptr;
-      s.n = 0;
-    }
-    if (hipMalloc(&s.p, bytes) != hipSuccess) {
-      s.p = nullptr;
-      return nullptr;
-    }
+    void *np_ = nullptr;
+    if (hipMalloc(&np_, bytes) != hipSuccess) return nullptr;
+    if (s.p) g_ws_retired.push_back(s.p);
+    s.p = np_;
     s.n = bytes;
   }
   return s.p;

@@ -96,6 +96,44 @@ __global__ void __launch_bounds__(BT) k_bc_farfield(long nf, const int64_t *__re
   cson[nl] = sqrt(t);
 }
 
+// rfve.f90:24-41: ghost = adjacent real cell for v (5 planes), p, T and c
+// (atsch_num.f90:450-458 runs it on the non-coincident frontiers in the
+// first cycle of a computation, before rbord)
+__global__ void __launch_bounds__(BT) k_rfve(long nf, const int64_t *__restrict__ g,
+                                             const int64_t *__restrict__ in, double *v, long ldv, double *ps,
+                                             double *temp, double *cson) {
+  long f = blockIdx.x * (long)BT + threadIdx.x;
+  if (f >= nf) return;
+  long nd = g[f], ni = in[f];
+#pragma unroll
+  for (int e = 0; e < 5; e++) v[e * ldv + nd] = v[e * ldv + ni];
+  ps[nd] = ps[ni];
+  temp[nd] = temp[ni];
+  cson[nd] = cson[ni];
+}
+
+// clsym.f90:56-79: symmetry plane.  The ghost takes the adjacent cell's
+// density and energy and its velocity without the normal component; the
+// ghost's p, T and c are left as they are (clsym has no such arguments).
+__global__ void __launch_bounds__(BT) k_bc_sym(long nf, const int64_t *__restrict__ g,
+                                               const int64_t *__restrict__ in, const double *__restrict__ nrm,
+                                               double *v, long ldv) {
+  long f = blockIdx.x * (long)BT + threadIdx.x;
+  if (f >= nf) return;
+  long nl = g[f], ni = in[f];
+  double nx = nrm[f], ny = nrm[nf + f], nz = nrm[2 * nf + f];
+  double ro1 = v[ni];
+  double qx1 = v[ldv + ni] / ro1, qy1 = v[2 * ldv + ni] / ro1, qz1 = v[3 * ldv + ni] / ro1;
+  double roe1 = v[4 * ldv + ni];
+  double qn1 = qx1 * nx + qy1 * ny + qz1 * nz;
+  double qtx = qx1 - qn1 * nx, qty = qy1 - qn1 * ny, qtz = qz1 - qn1 * nz;
+  v[nl] = ro1;
+  v[ldv + nl] = ro1 * qtx;
+  v[2 * ldv + nl] = ro1 * qty;
+  v[3 * ldv + nl] = ro1 * qtz;
+  v[4 * ldv + nl] = roe1;
+}
+
 // rbte.f90:46-59 (9 planes of tau, q) and met_rbse.f90:26-38 (6 planes of
 // the turbulent diffusive flux densities): ghost = adjacent real cell
 template <int NP>
@@ -158,6 +196,54 @@ extern "C" int codens_bc_farfield(int64_t nf, const int64_t *g, const int64_t *i
   hipLaunchKernelGGL(k_bc_farfield, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, nrm, w0[0],
                      w0[1], w0[2], w0[3], w0[4], make_phys(cp), v, (long)ldv, ps, temp, cson);
   return codens_check_launch("codens_bc_farfield");
+}
+
+extern "C" int codens_rfve(int64_t nf, const int64_t *g, const int64_t *in, double *v, int64_t ldv, double *ps,
+                           double *temp, double *cson, void *stream) {
+  if (nf == 0) return CODENS_OK;
+  if (nf < 0 || !g || !in || !v || !ps || !temp || !cson) {
+    codens_set_error("codens_rfve: invalid argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_rfve, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, v, (long)ldv, ps, temp,
+                     cson);
+  return codens_check_launch("codens_rfve");
+}
+
+extern "C" int codens_bc_sym(int64_t nf, const int64_t *g, const int64_t *in, const double *nrm, double *v,
+                             int64_t ldv, void *stream) {
+  if (nf == 0) return CODENS_OK;
+  if (nf < 0 || !g || !in || !nrm || !v) {
+    codens_set_error("codens_bc_sym: invalid argument");
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_bc_sym, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (long)nf, g, in, nrm, v, (long)ldv);
+  return codens_check_launch("codens_bc_sym");
+}
+
+// utnrd.f90:59-105 in a non-rotating frame (omg = 0): the far-field state
+// from the Mach number, the incidence angles (degrees) and the stagnation
+// state, with the reference's libm calls and operation order.  Host only.
+extern "C" int codens_utnrd(double rmach, double alpha0, double beta0, double roa1, double aa1, double pa1,
+                            const codens_phys *cp, double *w0) {
+  if (!cp || !w0 || !(cp->gam > 1.0)) {
+    codens_set_error("codens_utnrd: invalid argument");
+    return CODENS_EINVAL;
+  }
+  const double gam = cp->gam, gam1 = cp->gam - 1.0, gam2 = 0.5 * gam1, gam4 = 1. / gam1;
+  double degrad = atan2(1.0, 0.0) / 90.0;
+  double f = 1. + gam2 * (rmach * rmach);
+  double ro = roa1 / pow(f, gam4);
+  double a = aa1 / pow(f, .5);
+  double p = pa1 / pow(f, gam / gam1);
+  double q = rmach * a;
+  double alphar = alpha0 * degrad, betar = beta0 * degrad;
+  w0[0] = ro;
+  w0[1] = ro * q * cos(alphar) * cos(betar);
+  w0[2] = -ro * q * sin(betar);
+  w0[3] = ro * q * sin(alphar) * cos(betar);
+  w0[4] = p / gam1 + cp->pinfl + 0.5 * ro * (q * q);
+  return CODENS_OK;
 }
 
 extern "C" int codens_bc_tq_copy(int64_t nf, const int64_t *g, const int64_t *in, double *tq, int64_t ldv,

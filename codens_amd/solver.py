@@ -7,9 +7,9 @@ round trip (the global time step of kdtl=0 aside, which is a reduction the
 reference also synchronises on, chrono.f90:132).
 
 Stage order (atsch_num.f90): zpres (:245) -> zvismo (:424, NS) -> chronos
-(:436) -> rbord ghosts (:527; rbvc, walls, far fields, rbvc) -> zvisqc (:563,
-NS; then rbte/rbtc) -> sch_roe / sch_roe_euler (:720-751) -> sch_expli (kmf=0,
-:1021) or implimf_3d (kmf=1, :1040).
+(:436) -> rbord ghosts (:527; rbvc, walls, far fields, symmetry planes,
+rbvc) -> zvisqc (:563, NS; then rbte/rbtc) -> sch_roe / sch_roe_euler
+(:720-751) -> sch_expli (kmf=0, :1021) or implimf_3d (kmf=1, :1040).
 """
 from __future__ import annotations
 
@@ -69,7 +69,11 @@ class Solver:
         self.coin = (i64(c.ghost), i64(c.inner), i64(c.remote if c.remote is not None else c.inner))
         self.wall = (i64(w.ghost), i64(w.inner))
         self.far = (i64(fa.ghost), i64(fa.inner), _dev(fa.normals if fa.normals is not None else np.zeros((3, 0))))
-        self.tq_copy = (torch.cat([self.wall[0], self.far[0]]), torch.cat([self.wall[1], self.far[1]]))
+        sy = case.sym
+        self.sym = (i64(sy.ghost), i64(sy.inner), _dev(sy.normals if sy.normals is not None else np.zeros((3, 0))))
+        # rbte / met_rbse walk every frontier that is not coincident
+        self.tq_copy = (torch.cat([self.wall[0], self.far[0], self.sym[0]]),
+                        torch.cat([self.wall[1], self.far[1], self.sym[1]]))
         self.nbad = torch.zeros(1, dtype=torch.int32, device="cuda")
         t = case.turb
         if t is not None:
@@ -156,6 +160,11 @@ class Solver:
 
     def ghosts(self):
         s = self.stream
+        if self.cycle == 0 and self.tq_copy[0].numel():
+            # first cycle: rfve on the non-coincident frontiers
+            # (atsch_num.f90:450-458); rbord then rewrites every ghost but
+            # the p, T and c of symmetry ghosts
+            abi.rfve(self.tq_copy[0], self.tq_copy[1], self.v, self.ps, self.temp, self.cson, stream=s)
         g, i, r = self.coin
         if g.numel():
             abi.bc_coincident(g, i, r, self.v, self.ps, self.temp, self.cson, stream=s)
@@ -164,6 +173,8 @@ class Solver:
         if self.far[0].numel():
             abi.bc_farfield(self.far[0], self.far[1], self.far[2], self.case.winf, self.ph, self.v, self.ps,
                             self.temp, self.cson, stream=s)
+        if self.sym[0].numel():
+            abi.bc_sym(self.sym[0], self.sym[1], self.sym[2], self.v, stream=s)
         if g.numel():
             abi.bc_coincident(g, i, r, self.v, self.ps, self.temp, self.cson, stream=s)
 
@@ -316,10 +327,41 @@ class Solver:
         torch.cuda.current_stream().wait_stream(side)
         self.graph = g
 
+    def check(self):
+        """Raises CodensError(CODENS_ENONPHYS) if zvismo met T <= 0 in any
+        stage so far (where the reference aborts, zvismo.f90:122-127) or the
+        state now holds a real cell with rho <= 0, p <= 0 or a non-finite
+        value.  A host synchronisation; the solver calls it wherever it reads
+        results anyway (density_residual, norms, host_state)."""
+        nv = 7 if self.case.turb is not None else 5
+        abi.check_state(self.blocks_arr, len(self.blocks), nv, self.ph, self.v, self.nbad, stream=self.stream)
+        bad = int(self.nbad.item())
+        if bad:
+            self.nbad.zero_()
+            raise abi.CodensError(3, f"non-physical state after stage {self.cycle}: {bad} cell(s) with T <= 0 in "
+                                     f"zvismo, rho <= 0, p <= 0 or a non-finite value")
+
     def density_residual(self):
         """||(rho^{n+1}-rho^n)/dt||_2 of the last implicit stage: the increment
         d holds rho^{n+1}-rho^n on the real cells (dua_resro.f90:63-69)."""
+        self.check()
         return float(np.sqrt(abi.sumsq_rate(self.blocks, self.d[0], self.zero, self.dt, stream=self.stream)))
+
+    def norms(self):
+        """The steadiness norms of the last implicit stage (residu.f90 per
+        block, rscpsv.f90 over all blocks) of (v^{n+1} - v^n)/dt: the
+        increment d holds v^{n+1} - v^n on the real cells.  Returns an array
+        (nblocks + 1, neq, 7): L1, L2, Linf, block, i, j, k of Linf; the last
+        row is the aggregate."""
+        self.check()
+        nv = 7 if self.case.turb is not None else 5
+        if getattr(self, "_norm_out", None) is None:
+            self._norm_out = torch.zeros((len(self.blocks) + 1, nv, 7), dtype=torch.float64, device="cuda")
+            self._zero_v = torch.zeros_like(self.d)
+        abi.residu(self.blocks_arr, len(self.blocks), nv, self.d, self._zero_v, self.dt, self._norm_out,
+                   stream=self.stream)
+        torch.cuda.synchronize()
+        return self._norm_out.cpu().numpy()
 
     def run_to(self, drop=1e-8, max_iter=10000, check_every=10):
         """Implicit stages until the density residual falls to `drop` times its
@@ -341,6 +383,8 @@ class Solver:
                 break
         return it, hist
 
-    def host_state(self):
+    def host_state(self, check=True):
         torch.cuda.synchronize()
+        if check:
+            self.check()
         return self.v.cpu().numpy()

@@ -317,6 +317,50 @@ int codens_bc_wall(int64_t nf, const int64_t *ghost, const int64_t *inner, doubl
 int codens_bc_farfield(int64_t nf, const int64_t *ghost, const int64_t *inner, const double *nrm,
                        const double *w0, const codens_phys *ph, double *v, int64_t ldv, double *ps,
                        double *temp, double *cson, void *stream);
+/* utnrd.f90:59-105 -- the far-field state w0[5] (host array) that
+ * codens_bc_farfield takes: Mach number, incidence angles alpha, beta
+ * (degrees; velocity direction (cos a cos b, -sin b, sin a cos b)) and the
+ * stagnation state roa1, aa1, pa1 (c_nzst.f90:56-59), as the 'nrd' branch of
+ * rbord.f90:300-323 computes it.  Non-rotating frame only (omg = 0, so the
+ * node coordinates utnrd reads drop out).  Host computation, no device. */
+int codens_utnrd(double mach, double alpha, double beta, double roa1, double aa1, double pa1,
+                 const codens_phys *ph, double *w0);
+/* residu.f90:131-187 per block and rscpsv.f90:168-192 over all blocks --
+ * steadiness norms of an update: for the first neq (1..7) variables,
+ * r = (v - u) / dt on the real cells (v the new state, u the old one; the
+ * reference passes u = the state before the stage).  out: device array of
+ * (nb + 1) * neq * 7 doubles, record (l, e) at ((l * neq) + e) * 7: L1 =
+ * (1/N) sum |r|, L2 = sqrt((1/N) sum r^2), Linf = max |r|, then the block l
+ * and the (i, j, k) of the last real cell in storage order where |r| =
+ * Linf, as residu locates it; record (nb, e) is the aggregate of all blocks
+ * (sum N_l L1_l / N, sqrt(sum N_l L2_l^2 / N), max Linf_l, and where that
+ * maximum lies).  The reductions run in a fixed order: the result does not
+ * depend on timing.  No host synchronisation (the caller reads out when it
+ * needs it), so the call can be captured in a HIP graph once its workspace
+ * exists.  Uses the library's norms workspace. */
+int codens_residu(const codens_block *blocks, int32_t nb, int32_t neq, const double *v, const double *u,
+                  int64_t ldv, const double *dt, double *out, void *stream);
+
+/* Non-physical states: adds to *nbad (device int) the real cells whose
+ * density or pressure is not positive, or that hold a non-finite value in
+ * the first neq (>= 5) variables -- the states on which the reference
+ * aborts (zvismo.f90:122-127) or produces NaN.  No counterpart routine in
+ * the reference, which checks T only inside zvismo. */
+int codens_check_state(const codens_block *blocks, int32_t nb, int32_t neq, const codens_phys *ph,
+                       const double *v, int64_t ldv, int32_t *nbad, void *stream);
+
+/* rfve.f90:24-41 -- ghost = adjacent real cell for v (5 planes), p, T and c.
+ * atsch_num.f90:450-458 runs it over the non-coincident frontiers in the
+ * first cycle of a computation, before rbord; it is what leaves a symmetry
+ * ghost's p, T and c at the adjacent cell's first-cycle values. */
+int codens_rfve(int64_t nf, const int64_t *ghost, const int64_t *inner, double *v, int64_t ldv, double *ps,
+                double *temp, double *cson, void *stream);
+/* clsym.f90:56-79 -- symmetry plane (rbord.f90:340): ghost density and
+ * energy = adjacent cell's, ghost velocity = adjacent cell's without its
+ * normal component; nrm as for codens_bc_farfield.  Like clsym, it writes
+ * only v: the ghost's p, T and c keep the values they hold. */
+int codens_bc_sym(int64_t nf, const int64_t *ghost, const int64_t *inner, const double *nrm, double *v,
+                  int64_t ldv, void *stream);
 /* rbte.f90:46-59 -- tq ghost = adjacent real cell. */
 int codens_bc_tq_copy(int64_t nf, const int64_t *ghost, const int64_t *inner, double *tq,
                       int64_t ldv, void *stream);

@@ -209,6 +209,59 @@ void orc_bc_farfield(long nf, const long *ghost, const long *inner, const double
   }
 }
 
+/* utnrd.f90:59-105 in a non-rotating frame (omg = 0, so the node
+ * coordinates drop out): the far-field conservative state from the Mach
+ * number, the incidence angles alpha, beta (degrees) and the stagnation
+ * state roa1, aa1, pa1 (c_nzst.f90:56-59). */
+void orc_utnrd(double rmach, double alpha0, double beta0, double roa1, double aa1, double pa1,
+               const orc_phys *ph, double *w0) {
+  const double gam = ph->gam, gam1 = ph->gam1, gam2 = 0.5 * ph->gam1, gam4 = 1. / ph->gam1;
+  double pis2 = atan2(1.0, 0.0), degrad = pis2 / 90.0;
+  double ro = roa1 / pow(1. + gam2 * (rmach * rmach), gam4);
+  double a = aa1 / pow(1. + gam2 * (rmach * rmach), .5);
+  double p = pa1 / pow(1. + gam2 * (rmach * rmach), gam / gam1);
+  double q = rmach * a;
+  double alphar = alpha0 * degrad, betar = beta0 * degrad;
+  w0[0] = ro;
+  w0[1] = ro * q * cos(alphar) * cos(betar);
+  w0[2] = -ro * q * sin(betar);
+  w0[3] = ro * q * sin(alphar) * cos(betar);
+  w0[4] = p / gam1 + ph->pinfl + 0.5 * ro * (q * q);
+}
+
+/* rfve.f90:24-41: ghost = adjacent real cell for v (5 planes), p, T, c. */
+void orc_rfve(long nf, const long *ghost, const long *inner, double *v, long ldv, double *ps, double *temp,
+              double *cson) {
+  for (long f = 0; f < nf; f++) {
+    long nd = ghost[f], ni = inner[f];
+    for (int e = 0; e < 5; e++) v[e * ldv + nd] = v[e * ldv + ni];
+    ps[nd] = ps[ni];
+    temp[nd] = temp[ni];
+    cson[nd] = cson[ni];
+  }
+}
+
+/* clsym.f90:56-79: symmetry plane.  The ghost takes the adjacent cell's
+ * density and energy and its velocity without the normal component; the
+ * ghost's p, T and c are not written (clsym has no such arguments), so they
+ * keep whatever they held. */
+void orc_bc_sym(long nf, const long *ghost, const long *inner, const double *nxn, const double *nyn,
+                const double *nzn, double *v, long ldv) {
+  for (long f = 0; f < nf; f++) {
+    long nl = ghost[f], ni = inner[f];
+    double ro1 = v[ni];
+    double qx1 = v[ldv + ni] / ro1, qy1 = v[2 * ldv + ni] / ro1, qz1 = v[3 * ldv + ni] / ro1;
+    double roe1 = v[4 * ldv + ni];
+    double qn1 = qx1 * nxn[f] + qy1 * nyn[f] + qz1 * nzn[f];
+    double qtx = qx1 - qn1 * nxn[f], qty = qy1 - qn1 * nyn[f], qtz = qz1 - qn1 * nzn[f];
+    v[nl] = ro1;
+    v[ldv + nl] = ro1 * qtx;
+    v[2 * ldv + nl] = ro1 * qty;
+    v[3 * ldv + nl] = ro1 * qtz;
+    v[4 * ldv + nl] = roe1;
+  }
+}
+
 /* rbte.f90:46-59 (tq ghost = adjacent real cell) for non-coincident faces and
  * rbtc.f90:104-117 (average with the remote cell) for coincident faces. */
 void orc_bc_tq_copy(long nf, const long *ghost, const long *inner, double *tq, long ldv) {
@@ -478,4 +531,66 @@ void orc_implimf_eu(const orc_block *b, const orc_phys *ph, const double *u, con
   for (int c = 0; c < 3; c++) free(coefe[c]);
   for (int c = 0; c < 5; c++) free(d2w[c]);
   free(diag);
+}
+
+/* ------------------------------------------------------------------------ */
+/* residu.f90:93-187 for each block, then the aggregate of rscpsv.f90:
+ * 168-192.  r = (v - u) / dt on the real cells for the first neq variables.
+ * out: (nb + 1) records of neq x 7 doubles (L1, L2, Linf, block, i, j, k),
+ * as the device entry point writes them; the per-block sums run in the
+ * reference's loop order, the location is the last cell attaining Linf
+ * (residu's s = 0.5 (sign(1, |r| - dumax) + 1)). */
+void orc_residu(const orc_block *blocks, int nb, int neq, const double *v, const double *u, long ldv,
+                const double *dt, double *out) {
+  double d1g[7] = {0}, d2g[7] = {0}, dmg[7] = {0};
+  long npts = 0;
+  int gl[7] = {0};
+  for (int l = 0; l < nb; l++) {
+    const orc_block *b = &blocks[l];
+    long nmax = (long)(b->ii2 - b->ii1) * (b->jj2 - b->jj1) * (b->kk2 - b->kk1);
+    double coef = 1. / (double)nmax;
+    double d1[7] = {0}, d2[7] = {0}, dm[7] = {0};
+    int im[7] = {0}, jm[7] = {0}, km[7] = {0};
+    for (int k = b->kk1; k <= b->kk2 - 1; k++)
+      for (int j = b->jj1; j <= b->jj2 - 1; j++)
+        for (int i = b->ii1; i <= b->ii2 - 1; i++) {
+          long n = IDX(b, i, j, k);
+          for (int e = 0; e < neq; e++) {
+            double r = (v[e * ldv + n] - u[e * ldv + n]) / dt[n];
+            d1[e] = d1[e] + fabs(r);
+            d2[e] = d2[e] + r * r;
+            dm[e] = fmax(dm[e], fabs(r));
+            double s = 0.5 * (copysign(1.0, fabs(r) - dm[e]) + 1.);
+            im[e] = (int)lround((1. - s) * im[e] + s * i);
+            jm[e] = (int)lround((1. - s) * jm[e] + s * j);
+            km[e] = (int)lround((1. - s) * km[e] + s * k);
+          }
+        }
+    for (int e = 0; e < neq; e++) {
+      double *o = out + ((long)l * neq + e) * 7;
+      o[0] = coef * d1[e];
+      o[1] = sqrt(coef * d2[e]);
+      o[2] = dm[e];
+      o[3] = l;
+      o[4] = im[e];
+      o[5] = jm[e];
+      o[6] = km[e];
+      d1g[e] = d1g[e] + nmax * o[0];
+      d2g[e] = d2g[e] + nmax * (o[1] * o[1]);
+      if (l == 0 || fabs(o[2]) >= dmg[e]) gl[e] = l;
+      dmg[e] = fmax(dmg[e], fabs(o[2]));
+    }
+    npts += nmax;
+  }
+  for (int e = 0; e < neq; e++) {
+    double *o = out + ((long)nb * neq + e) * 7;
+    double *w = out + ((long)gl[e] * neq + e) * 7;
+    o[0] = d1g[e] / npts;
+    o[1] = sqrt(d2g[e] / npts);
+    o[2] = dmg[e];
+    o[3] = w[3];
+    o[4] = w[4];
+    o[5] = w[5];
+    o[6] = w[6];
+  }
 }

@@ -663,4 +663,227 @@ contains
     ischema = keep
   end subroutine ref_met_smklsas
 
+  ! --------------------------------------------------------------------
+  ! Boundary tables (module boundary, module mod_mpi on one rank): nfr
+  ! frontiers listed one after the other, frontier f holding mt(f) faces
+  ! (mpb = mpn = running sum, initbs.f90 / initis.f90); partner(f) is the
+  ! coincident frontier of f (0 when f is not coincident); no periodic
+  ! rotation (mper = 0).  Frontier numbers are global = local on one rank.
+  ! --------------------------------------------------------------------
+  subroutine ref_set_boundaries(nfr, mt, partner) bind(C, name="ref_set_boundaries")
+    use boundary
+    use mod_mpi
+    integer(c_int), value :: nfr
+    integer(c_int) :: mt(*), partner(*)
+    integer :: f, tot
+    if (allocated(mmb)) deallocate(mmb, mpb, mpn, lbd, nba, ndcc, mper, ndlb)
+    if (allocated(bcl_to_bcg)) deallocate(bcl_to_bcg, bcg_to_proc)
+    allocate(mmb(nfr), mpb(nfr), mpn(nfr), lbd(nfr), nba(nfr), ndcc(nfr), mper(nfr), ndlb(nfr))
+    allocate(bcl_to_bcg(nfr), bcg_to_proc(nfr))
+    tot = 0
+    do f = 1, nfr
+       mmb(f) = mt(f)
+       mpb(f) = tot
+       mpn(f) = tot
+       tot = tot + mt(f)
+       nba(f) = f
+       lbd(f) = f
+       ndcc(f) = partner(f)
+       mper(f) = 0
+       ndlb(f) = 1
+       bcl_to_bcg(f) = f
+       bcg_to_proc(f) = 0
+    enddo
+    num_bcg = nfr
+    nbd = nfr
+    ip40 = max(tot, 1)
+    ip41 = max(tot, 1)
+    ip42 = max(tot, 1)
+    ip43 = max(tot, 1)
+    ip44 = 1
+    protat = 0.
+    omg = 0.
+  end subroutine ref_set_boundaries
+
+  ! the frontier list lbd(1:nl) = fr(1:nl) that rbvc / rbte / rbtc /
+  ! met_rbse / met_rbsc walk (zvisqc.f90:182-205, met_num.f90:575-600)
+  subroutine set_lbd(nl, fr)
+    use boundary
+    integer(c_int) :: nl, fr(*)
+    integer :: f
+    nbd = nl
+    do f = 1, nl
+       lbd(f) = fr(f)
+    enddo
+  end subroutine set_lbd
+
+  ! rbvc.f90:4 -- coincident ghosts, face average with the partner frontier
+  subroutine ref_rbvc(nl, fr, v, ncbd, ncin, ps, temp, cson) bind(C, name="ref_rbvc")
+    use mod_rbvc
+    integer(c_int), value :: nl
+    integer(c_int) :: fr(*), ncbd(*), ncin(*)
+    real(c_double) :: v(*), ps(*), temp(*), cson(*)
+    integer :: mnc(1)
+    mnc = 0
+    call set_lbd(nl, fr)
+    call rbvc(v, ncbd, ncin, mnc, ps, temp, cson)
+  end subroutine ref_rbvc
+
+  ! residu.f90:4 -- steadiness norms of block l, 5 equations, or 7 with a
+  ! transport model (equat(6:7) = 'ke'); kimp = 0 keeps it from printing
+  subroutine ref_residu(l, neq, u, v, dt, dumy1, dumy2, dumax, idumx, jdumx, kdumx) bind(C, name="ref_residu")
+    use mod_residu
+    use chainecarac
+    use sortiefichier
+    use constantes
+    integer(c_int), value :: l, neq
+    real(c_double) :: u(*), v(*), dt(*), dumy1(neqt), dumy2(neqt), dumax(neqt)
+    integer(c_int) :: idumx(neqt), jdumx(neqt), kdumx(neqt)
+    real(c_double), allocatable :: res(:, :)
+    integer :: kimp0
+    allocate(res(ip00, 7))
+    if (neq == 7) then
+       equat = 'ns3d ke'
+    else
+       equat = heq
+    endif
+    kimp0 = kimp
+    kimp = 0
+    reelmn = 1.d-300
+    call residu(1, l, u, v, dt, res(1,1), res(1,2), res(1,3), res(1,4), res(1,5), res(1,6), res(1,7), 1, &
+         dumy1, dumy2, dumax, idumx, jdumx, kdumx)
+    kimp = kimp0
+    deallocate(res)
+  end subroutine ref_residu
+
+  ! rfve.f90:4 -- ghost = adjacent cell (v, p, T, c) on the listed frontiers
+  subroutine ref_rfve(nl, fr, v, ps, temp, cson, ncbd, ncin) bind(C, name="ref_rfve")
+    use mod_rfve
+    integer(c_int), value :: nl
+    integer(c_int) :: fr(*), ncbd(*), ncin(*)
+    real(c_double) :: v(*), ps(*), temp(*), cson(*)
+    call set_lbd(nl, fr)
+    call rfve(v, ps, temp, cson, ncbd, ncin)
+  end subroutine ref_rfve
+
+  ! clpara.f90:4 -- adiabatic wall, one frontier
+  subroutine ref_clpara(mfb, ncbd, v, ncin, ps, temp, cson) bind(C, name="ref_clpara")
+    use boundary
+    use mod_clpara
+    integer(c_int), value :: mfb
+    integer(c_int) :: ncbd(*), ncin(*)
+    real(c_double) :: v(*), ps(*), temp(*), cson(*)
+    call clpara(mfb, ncbd, v, mmb, mpb, ncin, ps, temp, cson)
+  end subroutine ref_clpara
+
+  ! rbord.f90:300-323 -- 'nrd' frontier of block l: utnrd.f90:4 (state from
+  ! Mach, incidence angles alpha, beta and the stagnation state roa1, aa1,
+  ! pa1), rbve.f90:4 (ghost = adjacent cell), clnrd.f90:4.  w0(5) returns
+  ! the state of the frontier's first face.
+  subroutine ref_farfield(mfb, l, mach, alpha, beta, roa, aa, pa, nxn, nyn, nzn, ncbd, ncin, v, ps, temp, cson, &
+       y, z, w0) bind(C, name="ref_farfield")
+    use boundary
+    use definition
+    use mod_utnrd
+    use mod_rbve
+    use mod_clnrd
+    integer(c_int), value :: mfb, l
+    real(c_double), value :: mach, alpha, beta, roa, aa, pa
+    integer(c_int) :: ncbd(*), ncin(*)
+    real(c_double) :: nxn(*), nyn(*), nzn(*), v(*), ps(*), temp(*), cson(*), y(*), z(*), w0(5)
+    real(c_double), allocatable :: bceqt(:, :), rod(:), roud(:), rovd(:), rowd(:), roed(:)
+    integer :: nl, fr(1)
+    allocate(bceqt(ip41, neqt), rod(ip40), roud(ip40), rovd(ip40), rowd(ip40), roed(ip40))
+    bceqt = 0.
+    bceqt(mpb(mfb) + 1, 1) = mach
+    bceqt(mpb(mfb) + 1, 2) = alpha
+    bceqt(mpb(mfb) + 1, 3) = beta
+    roa1 = roa
+    aa1 = aa
+    pa1 = pa
+    call utnrd(bceqt, mfb, l, rod, roud, rovd, rowd, roed, ncbd, y, z)
+    nl = 1
+    fr(1) = mfb
+    call set_lbd(nl, fr)
+    call rbve(v, ncin, ncbd)
+    call clnrd(mfb, rod, roud, rovd, rowd, roed, nxn, nyn, nzn, ncbd, v, mmb, mpb, mpn, l, ps, temp, cson)
+    w0(1) = rod(1)
+    w0(2) = roud(1)
+    w0(3) = rovd(1)
+    w0(4) = rowd(1)
+    w0(5) = roed(1)
+    deallocate(bceqt, rod, roud, rovd, rowd, roed)
+  end subroutine ref_farfield
+
+  ! clsym.f90:4 -- symmetry plane, one frontier
+  subroutine ref_clsym(mfb, nxn, nyn, nzn, ncin, ncbd, v) bind(C, name="ref_clsym")
+    use boundary
+    use mod_clsym
+    integer(c_int), value :: mfb
+    integer(c_int) :: ncin(*), ncbd(*)
+    real(c_double) :: nxn(*), nyn(*), nzn(*), v(*)
+    call clsym(mfb, nxn, nyn, nzn, ncin, ncbd, v, mmb, mpb, mpn)
+  end subroutine ref_clsym
+
+  ! rbte.f90:4 -- stress and heat-flux ghosts = adjacent cell (tq: 9 planes)
+  subroutine ref_rbte(nl, fr, tq, ncbd, ncin) bind(C, name="ref_rbte")
+    use mod_rbte
+    integer(c_int), value :: nl
+    integer(c_int) :: fr(*), ncbd(*), ncin(*)
+    real(c_double) :: tq(ip12, 9)
+    call set_lbd(nl, fr)
+    call rbte(tq(1,1), tq(1,2), tq(1,3), tq(1,4), tq(1,5), tq(1,6), tq(1,7), tq(1,8), tq(1,9), ncbd, ncin)
+  end subroutine ref_rbte
+
+  ! rbtc.f90:4 -- coincident stress and heat-flux ghosts
+  subroutine ref_rbtc(nl, fr, tq, ncbd, ncin) bind(C, name="ref_rbtc")
+    use mod_rbtc
+    integer(c_int), value :: nl
+    integer(c_int) :: fr(*), ncbd(*), ncin(*)
+    real(c_double) :: tq(ip12, 9)
+    integer :: mnc(1)
+    mnc = 0
+    call set_lbd(nl, fr)
+    call rbtc(tq(1,1), tq(1,2), tq(1,3), tq(1,4), tq(1,5), tq(1,6), tq(1,7), tq(1,8), tq(1,9), ncbd, ncin, mnc)
+  end subroutine ref_rbtc
+
+  ! met_rbse.f90:4 -- turbulence flux-density ghosts = adjacent cell (fd: 6 planes)
+  subroutine ref_met_rbse(nl, fr, fd, ncbd, ncin) bind(C, name="ref_met_rbse")
+    use mod_met_rbse
+    integer(c_int), value :: nl
+    integer(c_int) :: fr(*), ncbd(*), ncin(*)
+    real(c_double) :: fd(ip12, 6)
+    call set_lbd(nl, fr)
+    call met_rbse(fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6), ncbd, ncin)
+  end subroutine ref_met_rbse
+
+  ! met_rbsc.f90:4 -- coincident turbulence flux-density ghosts
+  subroutine ref_met_rbsc(nl, fr, fd, ncbd, ncin) bind(C, name="ref_met_rbsc")
+    use mod_met_rbsc
+    integer(c_int), value :: nl
+    integer(c_int) :: fr(*), ncbd(*), ncin(*)
+    real(c_double) :: fd(ip12, 6)
+    integer :: mnc(1)
+    mnc = 0
+    call set_lbd(nl, fr)
+    call met_rbsc(fd(1,1), fd(1,2), fd(1,3), fd(1,4), fd(1,5), fd(1,6), ncbd, ncin, mnc)
+  end subroutine ref_met_rbsc
+
+  ! initns.f90:4 -- inward unit normals of frontier mfb, face `iface`
+  ! (1..6: i1 i2 j1 j2 k1 k2) of block l over the node range imin..kmax
+  subroutine ref_initns(mfb, l, iface, imin, imax, jmin, jmax, kmin, kmax, x, y, z, ncbd, nxn, nyn, nzn) &
+       bind(C, name="ref_initns")
+    use mod_initns
+    integer(c_int), value :: mfb, l, iface, imin, imax, jmin, jmax, kmin, kmax
+    integer(c_int) :: ncbd(*)
+    real(c_double) :: x(*), y(*), z(*), nxn(*), nyn(*), nzn(*)
+    real(c_double), allocatable :: tn(:, :)
+    character(len=2), parameter :: fname(6) = (/ 'i1', 'i2', 'j1', 'j2', 'k1', 'k2' /)
+    allocate(tn(ip00, 9))
+    tn = 0.
+    call initns(mfb, l, fname(iface), imin, imax, jmin, jmax, kmin, kmax, heq, x, y, z, ncbd, nxn, nyn, nzn, &
+         tn(1,1), tn(1,2), tn(1,3), tn(1,4), tn(1,5), tn(1,6), tn(1,7), tn(1,8), tn(1,9))
+    deallocate(tn)
+  end subroutine ref_initns
+
 end module codens_ref_harness

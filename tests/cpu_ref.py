@@ -349,3 +349,103 @@ Orc.met_smsa = _o_met_smsa
 Orc.met_smko = _o_met_smko
 Orc.met_flud = _o_met_flud
 Orc.sch_turb = _o_sch_turb
+
+
+def orc_utnrd(lib, ph, mach, alpha, beta, roa1, aa1, pa1):
+    """The far-field state of utnrd.f90 (oracle restatement), 5 doubles."""
+    w0 = np.zeros(5)
+    lib.orc_utnrd(D(mach), D(alpha), D(beta), D(roa1), D(aa1), D(pa1), ctypes.byref(ph), ptr(w0))
+    return w0
+
+
+def orc_rfve(lib, g, i, v, ps, temp, cson):
+    lib.orc_rfve(L_(len(g)), ptr(g), ptr(i), ptr(v), L_(v.shape[1]), ptr(ps), ptr(temp), ptr(cson))
+
+
+def orc_bc_sym(lib, g, i, nx, ny, nz, v):
+    lib.orc_bc_sym(L_(len(g)), ptr(g), ptr(i), ptr(nx), ptr(ny), ptr(nz), ptr(v), L_(v.shape[1]))
+
+
+class RefFrontiers:
+    """The reference's boundary routines (rbvc, clpara, utnrd + rbve + clnrd,
+    clsym, rbte, rbtc, met_rbse, met_rbsc) over a list of frontiers, each a
+    FaceSet-like (ghost, inner) slice in the reference's listing order; the
+    flang build sees them as frontiers 1..nfr, face m of frontier f at
+    mpb(f)+m.  partner[f] (0-based, -1 for none) is the coincident frontier."""
+
+    def __init__(self, ref: Ref, ghosts, inners, partner):
+        self.lib = ref.lib
+        self.nfr = len(ghosts)
+        self.mt = np.array([len(g) for g in ghosts], dtype=np.int32)
+        self.off = np.concatenate([[0], np.cumsum(self.mt)]).astype(np.int64)
+        self.ncbd = np.ascontiguousarray(np.concatenate(ghosts).astype(np.int32) + 1)
+        self.ncin = np.ascontiguousarray(np.concatenate(inners).astype(np.int32) + 1)
+        part = np.array([p + 1 for p in partner], dtype=np.int32)
+        self.lib.ref_set_boundaries(self.nfr, ptr(self.mt), ptr(part))
+
+    def _list(self, frs):
+        return np.ascontiguousarray(np.array([f + 1 for f in frs], dtype=np.int32))
+
+    def rbvc(self, frs, v, ps, temp, cson):
+        fr = self._list(frs)
+        self.lib.ref_rbvc(len(fr), ptr(fr), ptr(v), ptr(self.ncbd), ptr(self.ncin), ptr(ps), ptr(temp), ptr(cson))
+
+    def rfve(self, frs, v, ps, temp, cson):
+        fr = self._list(frs)
+        self.lib.ref_rfve(len(fr), ptr(fr), ptr(v), ptr(ps), ptr(temp), ptr(cson), ptr(self.ncbd), ptr(self.ncin))
+
+    def clpara(self, f, v, ps, temp, cson):
+        self.lib.ref_clpara(f + 1, ptr(self.ncbd), ptr(v), ptr(self.ncin), ptr(ps), ptr(temp), ptr(cson))
+
+    def _normals(self, f, nrm):
+        n = [np.zeros(len(self.ncbd)) for _ in range(3)]
+        for c in range(3):
+            n[c][self.off[f]:self.off[f + 1]] = nrm[c]
+        return n
+
+    def farfield(self, f, l, mach, alpha, beta, roa1, aa1, pa1, nrm, v, ps, temp, cson, y=None, z=None):
+        nx, ny, nz = self._normals(f, nrm)
+        n = v.shape[1]
+        y = np.zeros(n) if y is None else y
+        z = np.zeros(n) if z is None else z
+        w0 = np.zeros(5)
+        self.lib.ref_farfield(f + 1, l + 1, D(mach), D(alpha), D(beta), D(roa1), D(aa1), D(pa1), ptr(nx), ptr(ny),
+                              ptr(nz), ptr(self.ncbd), ptr(self.ncin), ptr(v), ptr(ps), ptr(temp), ptr(cson),
+                              ptr(y), ptr(z), ptr(w0))
+        return w0
+
+    def clsym(self, f, nrm, v):
+        nx, ny, nz = self._normals(f, nrm)
+        self.lib.ref_clsym(f + 1, ptr(nx), ptr(ny), ptr(nz), ptr(self.ncin), ptr(self.ncbd), ptr(v))
+
+    def rbte(self, frs, tq):
+        fr = self._list(frs)
+        self.lib.ref_rbte(len(fr), ptr(fr), ptr(tq), ptr(self.ncbd), ptr(self.ncin))
+
+    def rbtc(self, frs, tq):
+        fr = self._list(frs)
+        self.lib.ref_rbtc(len(fr), ptr(fr), ptr(tq), ptr(self.ncbd), ptr(self.ncin))
+
+    def met_rbse(self, frs, fd):
+        fr = self._list(frs)
+        self.lib.ref_met_rbse(len(fr), ptr(fr), ptr(fd), ptr(self.ncbd), ptr(self.ncin))
+
+    def met_rbsc(self, frs, fd):
+        fr = self._list(frs)
+        self.lib.ref_met_rbsc(len(fr), ptr(fr), ptr(fd), ptr(self.ncbd), ptr(self.ncin))
+
+    def initns(self, f, l, face, b, x, y, z):
+        """Inward unit normals of frontier f = face `face` of block l (initns.f90)."""
+        nx, ny, nz = (np.zeros(len(self.ncbd)) for _ in range(3))
+        rng = {"i": (b.ii1, b.ii2), "j": (b.jj1, b.jj2), "k": (b.kk1, b.kk2)}
+        lim = []
+        for d in "ijk":
+            lo, hi = rng[d]
+            if d == face[0]:
+                lo = hi = (lo if face[1] == "1" else hi)
+            lim += [lo, hi]
+        iface = "ijk".index(face[0]) * 2 + (0 if face[1] == "1" else 1) + 1
+        self.lib.ref_initns(f + 1, l + 1, iface, *lim, ptr(x), ptr(y), ptr(z), ptr(self.ncbd), ptr(nx), ptr(ny),
+                            ptr(nz))
+        s = slice(self.off[f], self.off[f + 1])
+        return nx[s].copy(), ny[s].copy(), nz[s].copy()

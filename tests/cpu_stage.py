@@ -27,6 +27,26 @@ class CpuStage:
         if t is not None:
             self.fd = z(6, n); self.cfke = z(n); self.ts6 = z(n); self.ts7 = z(n)
             self.tc = C.OrcTurb(*t.tc)
+        # with the reference build, its own boundary routines over the
+        # case's frontiers: coincident pairs first, then walls, far fields
+        # and symmetry planes, in listing order
+        self.RF = None
+        self.first = True
+        if use_ref:
+            fr, part = [], []
+            c = case.coincident
+            for f in range(len(c.sizes)):
+                s_ = c.frontier(f)
+                fr.append(("rc", c, s_, c.blocks[f]))
+                part.append(f ^ 1)
+            for kind, fs in (("para", case.wall), ("nrd", case.farfield), ("sym", case.sym)):
+                for f in range(len(fs.sizes)):
+                    fr.append((kind, fs, fs.frontier(f), fs.blocks[f]))
+                    part.append(-1)
+            self.fr = fr
+            self.RF = C.RefFrontiers(self.K, [x[1].ghost[x[2]] for x in fr], [x[1].inner[x[2]] for x in fr], part)
+            self.coin_fr = [k for k, x in enumerate(fr) if x[0] == "rc"]
+            self.other_fr = [k for k, x in enumerate(fr) if x[0] != "rc"]
 
     def stage(self):
         cs, K, O = self.case, self.K, self.O
@@ -41,28 +61,40 @@ class CpuStage:
         if cs.kdtl == 0:
             stop, dtmin = O.chrono_global(cs.dt1min, self.dt)
             assert not stop, f"global step {dtmin} below dt1min {cs.dt1min}"
-        c, w, f = cs.coincident, cs.wall, cs.farfield
-        if len(c.ghost):
-            O.bc_coincident(c.ghost, c.inner, c.remote, self.v, self.ps, self.temp, self.cson)
-        if len(w.ghost):
-            O.bc_wall(w.ghost, w.inner, self.v, self.ps, self.temp, self.cson)
-        if len(f.ghost):
-            nrm = np.ascontiguousarray(f.normals)
-            O.bc_farfield(f.ghost, f.inner, nrm[0].copy(), nrm[1].copy(), nrm[2].copy(),
-                          np.ascontiguousarray(cs.winf, dtype=np.float64), self.v, self.ps, self.temp, self.cson)
-        if len(c.ghost):
-            O.bc_coincident(c.ghost, c.inner, c.remote, self.v, self.ps, self.temp, self.cson)
+        if self.first:
+            # atsch_num.f90:450-458: rfve on the non-coincident frontiers in
+            # the first cycle
+            self.first = False
+            if self.RF is not None:
+                if self.other_fr:
+                    self.RF.rfve(self.other_fr, self.v, self.ps, self.temp, self.cson)
+            else:
+                w, f, y = cs.wall, cs.farfield, cs.sym
+                gg = np.concatenate([w.ghost, f.ghost, y.ghost]); ii = np.concatenate([w.inner, f.inner, y.inner])
+                if len(gg):
+                    C.orc_rfve(O.lib, gg, ii, self.v, self.ps, self.temp, self.cson)
+        if self.RF is not None:
+            self.rbord_ref()
+        else:
+            self.rbord()
+        c, w, f, y = cs.coincident, cs.wall, cs.farfield, cs.sym
         if cs.turb is not None:
             self.turb_stage()
         if cs.ns:
             for l in range(nb):
                 g = K.zgrad2(l, cs.sn, cs.vol, self.v, self.temp)
                 K.zfluto(l, self.mu, self.mut, self.tq, g)
-            gg = np.concatenate([w.ghost, f.ghost]); ii = np.concatenate([w.inner, f.inner])
-            if len(gg):
-                O.bc_tq_copy(gg, ii, self.tq)
-            if len(c.ghost):
-                O.bc_tq_coincident(c.ghost, c.inner, c.remote, self.tq)
+            if self.RF is not None:  # zvisqc.f90:182-205
+                if self.other_fr:
+                    self.RF.rbte(self.other_fr, self.tq)
+                if self.coin_fr:
+                    self.RF.rbtc(self.coin_fr, self.tq)
+            else:
+                gg = np.concatenate([w.ghost, f.ghost, y.ghost]); ii = np.concatenate([w.inner, f.inner, y.inner])
+                if len(gg):
+                    O.bc_tq_copy(gg, ii, self.tq)
+                if len(c.ghost):
+                    O.bc_tq_coincident(c.ghost, c.inner, c.remote, self.tq)
         for l in range(nb):
             if cs.ns:
                 K.sch_roe(l, 0, self.u, self.v, self.ff, self.tq, cs.sn, self.ps)
@@ -74,6 +106,44 @@ class CpuStage:
             else:
                 K.implimf_3d(l, self.u, self.dt, self.v, self.d, self.ff, self.mu, self.mut, cs.lmax, cs.sn,
                              cs.vol, self.ps, self.cson)
+
+    def rbord(self):
+        """rbord.f90:137-429 on the oracle: coincident ghosts, walls, far
+        fields, symmetry planes, coincident ghosts again."""
+        cs, O = self.case, self.O
+        c, w, f, y = cs.coincident, cs.wall, cs.farfield, cs.sym
+        if len(c.ghost):
+            O.bc_coincident(c.ghost, c.inner, c.remote, self.v, self.ps, self.temp, self.cson)
+        if len(w.ghost):
+            O.bc_wall(w.ghost, w.inner, self.v, self.ps, self.temp, self.cson)
+        if len(f.ghost):
+            nrm = np.ascontiguousarray(f.normals)
+            O.bc_farfield(f.ghost, f.inner, nrm[0].copy(), nrm[1].copy(), nrm[2].copy(),
+                          np.ascontiguousarray(cs.winf, dtype=np.float64), self.v, self.ps, self.temp, self.cson)
+        if len(y.ghost):
+            nrm = np.ascontiguousarray(y.normals)
+            C.orc_bc_sym(O.lib, y.ghost, y.inner, nrm[0].copy(), nrm[1].copy(), nrm[2].copy(), self.v)
+        if len(c.ghost):
+            O.bc_coincident(c.ghost, c.inner, c.remote, self.v, self.ps, self.temp, self.cson)
+
+    def rbord_ref(self):
+        """rbord.f90:137-429 with the reference's own routines: rbvc, then
+        every frontier (clpara; utnrd, rbve and clnrd; clsym), then rbvc."""
+        cs, RF = self.case, self.RF
+        a = (self.v, self.ps, self.temp, self.cson)
+        if self.coin_fr:
+            RF.rbvc(self.coin_fr, *a)
+        spec = cs.farspec
+        for k, (kind, fs, s_, l) in enumerate(self.fr):
+            if kind == "para":
+                RF.clpara(k, *a)
+            elif kind == "nrd":
+                RF.farfield(k, l, spec.mach, spec.alpha, spec.beta, spec.roa1, spec.aa1, spec.pa1,
+                            np.ascontiguousarray(fs.normals[:, s_]), *a)
+            elif kind == "sym":
+                RF.clsym(k, np.ascontiguousarray(fs.normals[:, s_]), self.v)
+        if self.coin_fr:
+            RF.rbvc(self.coin_fr, *a)
 
     def turb_stage(self):
         """met_num.f90:162-930 for SA / Wilcox k-omega without wall laws, the
@@ -97,13 +167,20 @@ class CpuStage:
                 O.met_smko(l, self.tc, self.v, tp, self.cfke, self.ts6, self.ts7, self.cson)
             O.met_flud(l, t.kfludis, self.tc, self.v, self.mu, self.mut, self.fd)
             g9s.append(g9); tps.append(tp)
-        w, f, c = cs.wall, cs.farfield, cs.coincident
-        gg = np.ascontiguousarray(np.concatenate([w.ghost, f.ghost]).astype(np.int64))
-        ii = np.ascontiguousarray(np.concatenate([w.inner, f.inner]).astype(np.int64))
-        if len(gg):
-            O.lib.orc_met_rbse(L_(len(gg)), ptr(gg), ptr(ii), ptr(self.fd), L_(n))
-        if len(c.ghost):
-            O.lib.orc_met_rbsc(L_(len(c.ghost)), ptr(c.ghost), ptr(c.inner), ptr(c.remote), ptr(self.fd), L_(n))
+        w, f, c, y = cs.wall, cs.farfield, cs.coincident, cs.sym
+        if self.RF is not None:  # met_num.f90:575-600
+            if self.other_fr:
+                self.RF.met_rbse(self.other_fr, self.fd)
+            if self.coin_fr:
+                self.RF.met_rbsc(self.coin_fr, self.fd)
+        else:
+            gg = np.ascontiguousarray(np.concatenate([w.ghost, f.ghost, y.ghost]).astype(np.int64))
+            ii = np.ascontiguousarray(np.concatenate([w.inner, f.inner, y.inner]).astype(np.int64))
+            if len(gg):
+                O.lib.orc_met_rbse(L_(len(gg)), ptr(gg), ptr(ii), ptr(self.fd), L_(n))
+            if len(c.ghost):
+                O.lib.orc_met_rbsc(L_(len(c.ghost)), ptr(c.ghost), ptr(c.inner), ptr(c.remote), ptr(self.fd),
+                                   L_(n))
         for l, b in enumerate(lay.blocks):
             if t.klroe == 2:
                 O.met_roe2o(l, self.v, self.d, cs.sn)

@@ -1,10 +1,13 @@
 """Writes converged_wall_32x16x8.npz: the reference routines (flang build of
 the reference sources, oracle/_ref) run 40,000 implicit stages of the
-inclined-free-stream wall block 32x16x8 (configs.adjoint_wall with
-alpha = 10, beta = 5, CFL 10, no bump, no state perturbation; unlimited
-MUSCL, implimf_3d with lmax = 10).  Stored: the state on the real cells
-(5 x cells, in layout.interior() order) after 40,000 stages, and the
-density-residual drop every 2,000 stages.
+inclined-free-stream wall block 32x16x8 (configs.adjoint_wall with its
+default far field, utnrd.f90 incidence angles alpha = 5, beta = -10; CFL 10,
+no bump, no state perturbation; unlimited MUSCL, implimf_3d with
+lmax = 10).  Every routine of the stage is the reference's own: zpres,
+zvismo, chronos, rfve in the first cycle, the rbord sequence (clpara; utnrd,
+rbve, clnrd), zgrad2, zfluto, rbte, sch_roe, implimf_3d.  Stored: the state
+on the real cells (5 x cells, in layout.interior() order) after 40,000
+stages, and the density-residual drop every 2,000 stages.
 
 Run from the repository root:  python tests/golden/make_converged_wall.py
 (about 3 minutes on one core)."""
@@ -27,7 +30,7 @@ EVERY = 2000
 
 def main():
     assert oracle.ref_lib() is not None, "the reference build (oracle/_ref) is needed"
-    c = configs.adjoint_wall(32, 16, 8, alpha=10.0, beta=5.0, eta=10.0, amp=0.0, bump=0.0)
+    c = configs.adjoint_wall(32, 16, 8, eta=10.0, amp=0.0, bump=0.0)
     st = cpu_stage.CpuStage(c, use_ref=True)
     inner = c.layout.interior()
 

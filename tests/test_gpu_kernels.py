@@ -285,6 +285,84 @@ def test_boundary_ghosts(ctx):
     assert relerr(H(gtq), tq_o) <= TOL
 
 
+def test_symmetry_ghosts_and_farfield_state(ctx):
+    """codens_bc_sym (clsym.f90) against the oracle on both k faces, bit for
+    bit (contraction off in the ghost kernels): v written, p/T/c untouched;
+    codens_utnrd (utnrd.f90) against the oracle's restatement, bit for bit,
+    and the far field it feeds."""
+    abi, lay, sn, vol = ctx
+    from codens_amd import layout as L
+    rng = np.random.default_rng(17)
+    n = lay.ip11
+    v = random_state(n, rng)
+    v[1:4] *= 3.0 * (1 + rng.random(n))  # strong normal components
+    ps, t, c = all_thermo(v)
+    O = C.Orc(lay, C.orc_phys())
+    vo, vg, psg = v.copy(), T(v), T(ps)
+    for l, b in enumerate(lay.blocks):
+        for face in ("k1", "k2"):
+            g, i, fi = L.face_cells(b, face)
+            nrm = np.stack(L.inward_normals(b, sn, face, fi))
+            C.orc_bc_sym(O.lib, g, i, nrm[0].copy(), nrm[1].copy(), nrm[2].copy(), vo)
+            abi.bc_sym(T(g, torch.int64), T(i, torch.int64), T(nrm), vg)
+    assert np.array_equal(H(vg), vo)
+    assert np.array_equal(H(psg), ps)
+    assert not np.array_equal(vo, v)
+    ph = abi.phys()
+    for mach, a, bb in [(0.5, 0.0, 0.0), (0.8, 10.0, 5.0), (2.2, -30.0, 60.0)]:
+        w = np.array(abi.utnrd(mach, a, bb, 1.3, 1.1, 0.95, ph))
+        wo = C.orc_utnrd(O.lib, C.orc_phys(), mach, a, bb, 1.3, 1.1, 0.95)
+        assert np.array_equal(w, wo)
+
+
+@pytest.mark.parametrize("neq", [5, 7])
+def test_residu_norms(ctx, neq):
+    """codens_residu (residu.f90 per block, rscpsv.f90 aggregate) against the
+    oracle: L1 and L2 within 1e-13 (the sums run in another order), Linf and
+    its location exact; exact ties locate the last real cell; two calls in a
+    row (the second on other data) give the second's norms."""
+    abi, lay, sn, vol = ctx
+    import ctypes
+    from oracle import ptr
+    rng = np.random.default_rng(50 + neq)
+    n = lay.ip11
+    nb = len(lay.blocks)
+    blocks = abi.block_array([abi.block(b) for b in lay.blocks])
+    oblocks = (C.OrcBlock * nb)(*[C.orc_block(b) for b in lay.blocks])
+    for trial in range(3):
+        u = rng.standard_normal((7, n))
+        v = u + 1e-3 * rng.standard_normal((7, n))
+        dt = 0.01 * (1 + rng.random(n))
+        if trial == 2:  # exact ties: r = 0.5 on every cell
+            u = np.zeros((7, n)); v = np.repeat((0.5 * dt)[None, :], 7, axis=0)
+        ref = np.zeros((nb + 1, neq, 7))
+        C.oracle.oracle_lib().orc_residu(oblocks, nb, neq, ptr(v), ptr(u), C.L_(n), ptr(dt), ptr(ref))
+        out = torch.zeros((nb + 1, neq, 7), dtype=torch.float64, device="cuda")
+        abi.residu(blocks, nb, neq, T(v), T(u), T(dt), out)
+        got = H(out)
+        assert np.allclose(got[..., :2], ref[..., :2], rtol=1e-13, atol=0), trial
+        assert np.array_equal(got[..., 2:], ref[..., 2:]), trial
+
+
+def test_check_state_counts_nonphysical(ctx):
+    """codens_check_state counts real cells with rho <= 0, p <= 0 or NaN,
+    injected from the host; ghost cells are not counted."""
+    abi, lay, sn, vol = ctx
+    rng = np.random.default_rng(61)
+    n = lay.ip11
+    v = random_state(n, rng)
+    b = lay.blocks[0]
+    bad = [b.idx(b.ii1, b.jj1, b.kk1), b.idx(b.ii1 + 1, b.jj1 + 2, b.kk1 + 1), b.idx(b.ii2 - 1, b.jj2 - 1, b.kk2 - 1)]
+    v[0, bad[0]] = -1.0                 # rho < 0
+    v[4, bad[1]] = 0.1 * v[4, bad[1]]   # p < 0
+    v[2, bad[2]] = np.nan
+    v[0, b.idx(b.ii1 - 1, b.jj1, b.kk1)] = -5.0  # a ghost: not counted
+    blocks = abi.block_array([abi.block(x) for x in lay.blocks])
+    nbad = torch.zeros(1, dtype=torch.int32, device="cuda")
+    abi.check_state(blocks, len(lay.blocks), 5, abi.phys(), T(v), nbad)
+    assert int(nbad.item()) == 3
+
+
 def test_bad_extents_rejected(ctx):
     abi, lay, sn, vol = ctx
     from codens_amd.abi import CodensError

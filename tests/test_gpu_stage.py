@@ -56,6 +56,24 @@ def test_wall_ns_reduced_10_stages():
     _cmp(case, g.host_state(), c.v, 1e-10)
 
 
+def test_wall_ns_symmetry_span_10_stages():
+    """The wall block with symmetry planes (clsym.f90) on both k faces, 10
+    implicit stages against the oracle, unlimited MUSCL.  The symmetry
+    ghosts' p, T and c keep the adjacent cells' first-cycle values, as in the
+    reference (clsym writes v only; rfve fills them in the first cycle,
+    atsch_num.f90:450-458)."""
+    from codens_amd import configs
+    import cpu_stage
+    case = configs.wall_ns(32, 24, 16, lmax=10, ilim=0, span="sym")
+    g, c = _solver(case), cpu_stage.CpuStage(case)
+    for _ in range(10):
+        g.stage(); c.stage()
+    _cmp(case, g.host_state(), c.v, 1e-10)
+    gh = case.sym.ghost
+    ps = g.ps.cpu().numpy()
+    assert np.array_equal(ps[gh], c.ps[gh]) and ps[gh].min() > 0
+
+
 def test_wall_ns_full_one_stage():
     """Full 256x128x64 block, minmod limiter: one stage from a developed state
     (20 device stages from free stream, then a seeded 1% perturbation of every
@@ -238,7 +256,7 @@ def test_converges_to_1e8_fp64_and_fp32():
     from codens_amd.solver import Solver
     gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "converged_wall_32x16x8.npz"))
     gdrop = {int(i): d for i, d in gold["drop"]}
-    case = configs.adjoint_wall(32, 16, 8, alpha=10.0, beta=5.0, eta=10.0, amp=0.0, bump=0.0)
+    case = configs.adjoint_wall(32, 16, 8, eta=10.0, amp=0.0, bump=0.0)
     inner = case.layout.interior()
     cross, final = {}, {}
     for mode in ("fp64", "fp32"):
@@ -348,3 +366,31 @@ def test_fp32_defect_correction_spalart_allmaras():
         assert relerr(b[e, inner], a[e, inner]) <= 1e-7, e
     assert relerr(mb[inner], ma[inner]) <= 1e-7
     assert relerr(b[0, inner], a[0, inner]) > 0
+
+
+def test_nonphysical_state_raises():
+    """A real cell with T <= 0, written into the state from the host: the
+    stage's kernels run to completion (NaN arithmetic, no fault), zvismo
+    counts the cell, and the solver's next read raises CODENS_ENONPHYS, where
+    the reference aborts (zvismo.f90:122-127).  A clean solver on the same
+    case reads without error, and its norms locate the largest change."""
+    from codens_amd import configs
+    from codens_amd.abi import CodensError
+    from codens_amd.solver import Solver
+    case = configs.wall_ns(24, 16, 8, lmax=3, ilim=0)
+    s = Solver(case)
+    b = case.layout.blocks[0]
+    n = b.idx(b.ii1 + 5, b.jj1 + 3, b.kk1 + 2)
+    s.v[4, n] = 0.2 * s.v[4, n]  # internal energy below zero: p, T < 0
+    s.stage()
+    with pytest.raises(CodensError) as e:
+        s.density_residual()
+    assert e.value.code == 3
+    ok = Solver(case)
+    for _ in range(3):
+        ok.stage()
+    r = ok.density_residual()
+    nr = ok.norms()
+    assert nr.shape == (2, 5, 7) and np.isfinite(r)
+    # the aggregate's density L2 is the dua_resro residual over sqrt(N)
+    assert abs(nr[-1, 0, 1] - r / np.sqrt(case.layout.ncells)) <= 1e-12 * nr[-1, 0, 1]

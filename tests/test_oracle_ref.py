@@ -84,16 +84,36 @@ def test_viscous_and_implicit_bit_exact(mesh):
     assert np.array_equal(vr, vo)
 
 
-@pytest.mark.parametrize("which", ["periodic_euler", "wall_ns"])
-def test_stages_bit_exact(which):
+def _stage_case(which):
     from codens_amd import configs
+    return {"periodic_euler": lambda: configs.periodic_euler(),
+            "wall_ns": lambda: configs.wall_ns(24, 16, 8, lmax=4),
+            "wall_ns_sym": lambda: configs.wall_ns(24, 16, 8, lmax=4, span="sym"),
+            "inclined": lambda: configs.adjoint_wall(16, 8, 8, eta=10.0, amp=0.01),
+            "wall_rans_sa": lambda: configs.wall_rans(16, 12, 8, model="1SA", ilim=0, lmax=3),
+            "wall_rans_ko": lambda: configs.wall_rans(16, 12, 8, model="2KO", ilim=0, lmax=3),
+            "multiblock_sa": lambda: configs.multiblock_sa(6, 4, 9, None, seed=7, ilim=1, lmax=3)}[which]()
+
+
+@pytest.mark.parametrize("which", ["periodic_euler", "wall_ns", "wall_ns_sym", "inclined", "wall_rans_sa",
+                                   "wall_rans_ko", "multiblock_sa"])
+def test_stages_bit_exact(which):
+    """Whole stages of the oracle (C restatement of every kernel and ghost
+    rule) against the reference build (its own kernels and its own boundary
+    routines rbvc, clpara, utnrd + rbve + clnrd, clsym, rbte, rbtc, met_rbse,
+    met_rbsc, in rbord's order), bit for bit, state and ghosts."""
     import cpu_stage
-    case = configs.periodic_euler() if which == "periodic_euler" else configs.wall_ns(24, 16, 8, lmax=4)
+    case = _stage_case(which)
     a, b = cpu_stage.CpuStage(case, use_ref=False), cpu_stage.CpuStage(case, use_ref=True)
     for _ in range(4):
         a.stage(); b.stage()
-    inner = case.layout.interior()
-    assert np.array_equal(a.v[:, inner], b.v[:5, inner])
+    nv = 7 if case.turb is not None else 5
+    assert np.array_equal(a.v[:nv], b.v[:nv], equal_nan=True)
+    assert np.all(np.isfinite(a.v[:nv, case.layout.interior()]))
+    for x, y in ((a.ps, b.ps), (a.temp, b.temp), (a.cson, b.cson), (a.tq, b.tq), (a.dt, b.dt)):
+        assert np.array_equal(x, y, equal_nan=True)
+    if case.turb is not None:
+        assert np.array_equal(a.fd, b.fd, equal_nan=True) and np.array_equal(a.mut, b.mut)
 
 
 def test_metrics_bit_exact():

@@ -304,22 +304,41 @@ def wall_rans(ci=48, cj=32, ck=16, model="1SA", klroe=2, kwsst=0, nut_ratio=3.0,
     omega with mut/mu = nut_ratio (k-omega), uniform; wall distance = y of
     the cell centre."""
     c = wall_ns(ci, cj, ck, **kw)
+    return with_turbulence(c, _last_wall_distance.copy(), model, klroe, kwsst, nut_ratio)
+
+
+def with_turbulence(c: Case, dist, model="1SA", klroe=2, kwsst=0, nut_ratio=3.0) -> Case:
+    """Case c with a turbulence model added: planes 6-7 of the state (SA:
+    nu~ = nut_ratio x the laminar kinematic viscosity; k-omega: k = 1e-6
+    |U|^2 and omega with mut/mu = nut_ratio), uniform, and the wall distance
+    `dist` (storage array)."""
     n = c.layout.ip11
-    dist = _last_wall_distance.copy()
-    mach = float(c.winf[1] / c.winf[0])
+    mach = float(np.sqrt(np.sum(c.winf[1:4] ** 2)) / c.winf[0])
     mu_inf = 1.0 / c.phys["reynz"]  # Sutherland at T = 1 (zvismo.f90:90-104)
     v0 = np.zeros((7, n)); v0[:5] = c.v0
-    w = np.zeros(7); w[:5] = c.winf
     if model == "1SA":
-        v0[5] = nut_ratio * mu_inf; w[5] = v0[5, 0]
+        v0[5] = nut_ratio * mu_inf
     else:
         k = 1e-6 * mach * mach
-        om = k / (nut_ratio * mu_inf)
-        v0[5] = k; v0[6] = om; w[5] = k; w[6] = om
+        v0[5] = k; v0[6] = k / (nut_ratio * mu_inf)
     c.v0 = v0
-    c.name = f"wall_rans_{model}"
+    c.name = f"{c.name}_{model}"
     c.turb = TurbCfg(model, turb_constants(model), dist, klroe=klroe, kwsst=kwsst)
     return c
+
+
+def inclined_rans(ci=32, cj=16, ck=8, model="2KO", klroe=2, kwsst=0, nut_ratio=3.0, eta=10.0, **kw) -> Case:
+    """The inclined-free-stream wall block of adjoint_wall (no bump, no state
+    perturbation: the case that converges laminar) with a turbulence model;
+    wall distance = y of the cell centre."""
+    c = adjoint_wall(ci, cj, ck, eta=eta, amp=0.0, bump=0.0, **kw)
+    b = c.layout.blocks[0]
+    x, y, z = c.nodes
+    _, yc, _ = cell_centres(b, x, y, z)
+    dist = np.ones(c.layout.ip11)
+    r = (slice(b.kk1 - b.kd1, b.kk2 - b.kd1), slice(b.jj1 - b.jd1, b.jj2 - b.jd1), slice(b.ii1 - b.id1, b.ii2 - b.id1))
+    b.view(dist)[r] = yc[r]
+    return with_turbulence(c, dist, model, klroe, kwsst, nut_ratio)
 
 
 def _face_grid(b: L.Block, face: str):

@@ -20,6 +20,8 @@
 // operation).
 #include "common.hpp"
 
+#include <algorithm>
+#include <utility>
 #include <vector>
 
 using namespace codens;
@@ -600,7 +602,7 @@ extern "C" int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, co
   std::vector<BatchEnt> ent(nblk);
   std::vector<int32_t> key;
   unsigned tot[BP_N] = {0};
-  long lo = 0, hi = 0;
+  std::vector<std::pair<long, long>> span(nblk);
   const int ilim = sc->ilim;
   for (int l = 0; l < nblk; l++) {
     int rc = codens_check_block(&blocks[l], what);
@@ -608,8 +610,7 @@ extern "C" int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, co
     const int32_t *w = (const int32_t *)&blocks[l];
     key.insert(key.end(), w, w + sizeof(codens_block) / sizeof(int32_t));
     Blk b = make_blk(&blocks[l]);
-    if (l == 0 || b.npc < lo) lo = b.npc;
-    if (l == 0 || b.npc + b.nnn > hi) hi = b.npc + b.nnn;
+    span[l] = std::make_pair(b.npc, b.npc + b.nnn);
     BatchEnt &E = ent[l];
     E.b = b;
     E.n0 = b.idx(b.i1 - 1, b.j1 - 1, b.k1 - 1);
@@ -634,14 +635,18 @@ extern "C" int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, co
       tot[p] += g[p][0] * g[p][1] * g[p][2];
     }
   }
-  if (lo != 0 || hi > ldv) {
-    codens_set_error("%s: the blocks must start at entry 0 and lie within ldv", what);
-    return CODENS_EINVAL;
+  // blocks overlapping in storage would share workspace ranges and race
+  std::sort(span.begin(), span.end());
+  for (int l = 0; l < nblk; l++) {
+    if ((l == 0 && span[0].first != 0) || (l > 0 && span[l].first < span[l - 1].second) || span[l].second > ldv) {
+      codens_set_error("%s: the blocks must start at entry 0, not overlap, and lie within ldv", what);
+      return CODENS_EINVAL;
+    }
   }
   BatchEnt *tab = (BatchEnt *)codens_device_table(0, key, ent.data(), sizeof(BatchEnt) * nblk, st, what);
   if (!tab) return CODENS_EHIP;
   Phys ph = make_phys(cp);
-  const long tot_n = hi;
+  const long tot_n = span[nblk - 1].second;
   double *H = (double *)codens_workspace(sizeof(double) * 15 * tot_n, 0);
   double *R = ilim == 1 ? (double *)codens_workspace(sizeof(double) * 15 * tot_n, 1) : nullptr;
   double *uvw = (double *)codens_workspace(sizeof(double) * 3 * tot_n, 4);

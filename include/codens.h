@@ -269,7 +269,14 @@ int codens_sch_expli(const codens_block *b, const double *u, double *v, int64_t 
 /* implimf_3d.f90:4 -- matrix-free point-Jacobi implicit phase: coefficients,
  * lmx sub-iterations, then v += d.  d (5 arrays of ldv) receives the final
  * increment, zero on the ghosts.  ff is read only when ityprk != 0; dtpas
- * only when kfmg = 3.  Uses a library-owned device workspace. */
+ * only when kfmg = 3.  Uses a library-owned device workspace.
+ * Contract on ps: the pressure of the base flux F(v) is re-derived from v
+ * with the zpres formula (zpres.f90:72-73), so ps is not read.  The
+ * reference reads ps(n) there (implimf_3d.f90:444-452); the results agree
+ * when ps = zpres(v) on the real cells, which holds in atsch_num's stage
+ * (zpres runs at its start and v is unchanged until the implicit phase), to
+ * rounding (about 3e-14 relative in d).  A caller whose ps differs from
+ * zpres(v) gets the increment of zpres(v), not of its ps. */
 int codens_implimf_3d(const codens_block *b, const codens_phys *ph, const double *u,
                       const double *dt, double *v, int64_t ldv, double *d, const double *ff,
                       const double *mu, const double *mut, int32_t lmx, const double *sn_blk,

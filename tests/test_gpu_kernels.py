@@ -9,7 +9,7 @@ torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
 import cpu_ref as C
-from randcase import all_thermo, random_mesh, random_state, relerr
+from randcase import GAM, all_thermo, random_mesh, random_state, relerr
 
 TOL = 1e-12
 DIMS = [(8, 6, 5), (5, 4, 3)]
@@ -354,7 +354,10 @@ def test_check_state_counts_nonphysical(ctx):
     b = lay.blocks[0]
     bad = [b.idx(b.ii1, b.jj1, b.kk1), b.idx(b.ii1 + 1, b.jj1 + 2, b.kk1 + 1), b.idx(b.ii2 - 1, b.jj2 - 1, b.kk2 - 1)]
     v[0, bad[0]] = -1.0                 # rho < 0
-    v[4, bad[1]] = 0.1 * v[4, bad[1]]   # p < 0
+    # p < 0: total energy set to half the kinetic energy
+    ke = 0.5 * (v[1, bad[1]] ** 2 + v[2, bad[1]] ** 2 + v[3, bad[1]] ** 2) / v[0, bad[1]]
+    v[4, bad[1]] = 0.5 * ke
+    assert (GAM - 1) * (v[4, bad[1]] - ke) < 0
     v[2, bad[2]] = np.nan
     v[0, b.idx(b.ii1 - 1, b.jj1, b.kk1)] = -5.0  # a ghost: not counted
     blocks = abi.block_array([abi.block(x) for x in lay.blocks])

@@ -381,7 +381,8 @@ def test_nonphysical_state_raises():
     s = Solver(case)
     b = case.layout.blocks[0]
     n = b.idx(b.ii1 + 5, b.jj1 + 3, b.kk1 + 2)
-    s.v[4, n] = 0.2 * s.v[4, n]  # internal energy below zero: p, T < 0
+    ke = 0.5 * float((s.v[1, n] ** 2 + s.v[2, n] ** 2 + s.v[3, n] ** 2) / s.v[0, n])
+    s.v[4, n] = 0.5 * ke  # total energy below the kinetic energy: p, T < 0
     s.stage()
     with pytest.raises(CodensError) as e:
         s.density_residual()
@@ -394,3 +395,100 @@ def test_nonphysical_state_raises():
     assert nr.shape == (2, 5, 7) and np.isfinite(r)
     # the aggregate's density L2 is the dua_resro residual over sqrt(N)
     assert abs(nr[-1, 0, 1] - r / np.sqrt(case.layout.ncells)) <= 1e-12 * nr[-1, 0, 1]
+
+
+def _multitile_case():
+    """6 uneven blocks, axes 66..130 cells (4.66M cells): every block spans
+    several i-j tiles of both sub-iteration shapes (64x4, 32x8) and several
+    8-plane k-slabs, with partial tiles on the i and j edges."""
+    from codens_amd import configs
+    case = configs.multiblock_sa(6, 66, 130, None, seed=5, ilim=1, lmax=4)
+    dims = [(b.ii2 - b.ii1, b.jj2 - b.jj1, b.kk2 - b.kk1) for b in case.layout.blocks]
+    assert all(ci > 64 and cj > 8 and ck > 8 for ci, cj, ck in dims), dims
+    assert any(ci % 64 and ci % 32 for ci, _, _ in dims), dims
+    return case
+
+
+def test_batched_multitile_residual_and_turbulence():
+    """codens_sch_roe_batch and codens_impli2_eqt_3d_batch on blocks of several
+    tiles each equal the block-by-block calls bit for bit: the residual of
+    the first stage, and the state and eddy viscosity after two stages with
+    every phase batched."""
+    from codens_amd.solver import Solver
+    case = _multitile_case()
+    a, b = Solver(case), Solver(case, batched=("residual", "turbulence"))
+    for s in (a, b):
+        s.thermo(); s.time_step(); s.ghosts(); s.turb_stage(); s.stresses(); s.residual()
+    torch.cuda.synchronize()
+    assert torch.equal(a.u.cpu(), b.u.cpu())
+    assert float(a.u[:5].abs().max()) > 0
+    a2, b2 = Solver(case), Solver(case, batched=True)
+    for _ in range(2):
+        a2.stage(); b2.stage()
+    torch.cuda.synchronize()
+    for x, y in ((a2.v, b2.v), (a2.mut, b2.mut)):
+        assert torch.equal(x.cpu(), y.cpu())
+
+
+@pytest.mark.parametrize("tile", [0, 1, -1])
+def test_batched_multitile_implicit(tile):
+    """codens_implimf_3d_batch on blocks of several tiles and k-slabs equals
+    implimf_3d called block by block, bit for bit, with the 64x4 shape
+    forced (0), the 32x8 shape forced (1) and the per-block choice (-1)."""
+    from codens_amd import abi
+    from codens_amd.solver import Solver
+    case = _multitile_case()
+    prev = abi.sweep_tile(tile)
+    try:
+        a, b = Solver(case), Solver(case)
+        for s in (a, b):
+            s.thermo(); s.time_step(); s.ghosts(); s.stresses(); s.residual()
+        a.update()
+        abi.implimf_3d_batch(b.blocks_arr, len(b.blocks), b.ph, b.u, b.dt, b.v, b.d, b.ff, b.mu, b.mut, case.lmax,
+                             b.sn, b.vol, b.cson, stream=b.stream)
+        torch.cuda.synchronize()
+    finally:
+        abi.sweep_tile(prev)
+    assert torch.equal(a.d[:5].cpu(), b.d[:5].cpu())
+    assert torch.equal(a.v.cpu(), b.v.cpu())
+    assert float(a.d[:5].abs().max()) > 0
+
+
+def test_sch_roe_batch_rejects_overlapping_blocks():
+    """Blocks that overlap in storage would share workspace ranges:
+    codens_sch_roe_batch refuses them with CODENS_EINVAL before any launch."""
+    from codens_amd import abi, configs
+    from codens_amd.abi import CodensError
+    from codens_amd.solver import Solver
+    case = configs.multiblock_sa(3, 6, 9, None, seed=3, ilim=1, lmax=2)
+    s = Solver(case)
+    bl = [abi.block(b) for b in case.layout.blocks]
+    bl[2].npc = bl[1].npc + 1
+    bl[2].nnn = bl[1].nnn  # same extents as block 1, shifted by one entry
+    for f in ("id1", "id2", "jd1", "jd2", "kd1", "kd2", "ii1", "ii2", "jj1", "jj2", "kk1", "kk2"):
+        setattr(bl[2], f, getattr(bl[1], f))
+    with pytest.raises(CodensError) as e:
+        abi.sch_roe_batch(abi.block_array(bl), 3, s.ph, s.sc, 0, s.u, s.v, s.ff, s.tq, s.sn, s.ps)
+    assert e.value.code == 1 and "overlap" in str(e.value)
+
+
+def test_graph_replay_after_workspace_growth():
+    """A stage captured in a graph keeps its workspace pointers: a later eager
+    solver that needs larger workspaces (the batched phases size them for all
+    blocks) gets new buffers and the old ones stay allocated, so replays of
+    the first graph equal an eager twin bit for bit."""
+    from codens_amd import configs
+    from codens_amd.solver import Solver
+    case = configs.multiblock_sa(9, 4, 11, None, seed=7, ilim=1, lmax=4)
+    a, twin = Solver(case), Solver(case)
+    a.capture()
+    twin.stage()
+    big = Solver(case, batched=True)
+    big.stage()  # grows workspace slots 0, 1, 2, 3 and 4 past the per-block sizes
+    scratch = [torch.full((case.layout.ip11 * 15,), 7.0, dtype=torch.float64, device="cuda") for _ in range(4)]
+    for r in range(3):
+        a.stage(); twin.stage(); big.stage()
+        torch.cuda.synchronize()
+        for x, y in ((a.v, twin.v), (a.mut, twin.mut), (a.d, twin.d)):
+            assert torch.equal(x.cpu(), y.cpu()), r
+    assert all(float(t.min()) == 7.0 == float(t.max()) for t in scratch)

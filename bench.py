@@ -64,10 +64,36 @@ def cpu_baseline(case, stages, sample=None):
                       f"{'flang -O2 build of the reference routines' if use_ref else 'oracle C restatement'}"}
 
 
+def cpu_allcores(case, stages, sample=None):
+    """The oracle's C restatement of the same stage (OpenMP over the block's
+    planes, bit for bit the serial result) on every core OpenMP is given:
+    the all-cores CPU comparator beside the one-core reference build."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import cpu_stage
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
+    st = cpu_stage.CpuStage(case, use_ref=False)
+    st.stage()
+    t0 = time.perf_counter()
+    for _ in range(stages):
+        st.stage()
+    dt = time.perf_counter() - t0
+    return {"value": case.layout.ncells * stages / dt / 1e6, "unit": "Mcell-updates/s", "cores": threads,
+            "kind": "port", "cpu": model, "nproc": os.cpu_count(),
+            "sample": f"{stages} implicit iterations of {sample or 'the same 256x128x64 case'} (after one untimed), "
+                      f"lmax=10, oracle C restatement with OpenMP"}
+
+
 # kernels (or launch groups) timed with HIP events on the launch stream
 TIMED_KERNELS = ("implimf_sweep", "implimf_sweep1", "implimf_coef", "implimf_update", "implimf32_sweep",
-                 "implimf32_sweep1", "implimf32_coef", "implimf32_update", "roe_limiter_chain",
-                 "roe_face_flux", "roe_gather", "zvisqc", "chronos", "zpres", "zvismo")
+                 "implimf32_sweep1", "implimf32_coef", "implimf32_update", "roe_prim", "roe_limiter_chain",
+                 "roe_face_flux", "roe_gather", "roe_fused", "roe_zero_ghost", "zvisqc", "chronos", "zpres",
+                 "zvismo")
 
 
 def main():
@@ -90,6 +116,8 @@ def main():
                     help="replay stages from a captured HIP graph (auto: on for multiblock)")
     ap.add_argument("--implicit", choices=("fp64", "fp32"), default="fp64",
                     help="implicit phase: implimf_3d in FP64 (the reference's), or its FP32 defect-correction form")
+    ap.add_argument("--residual", choices=("fused", "split"), default="fused",
+                    help="sch_roe as one fused kernel (default) or as face passes + gather")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="finish with 3 device-to-device copies of MIB MiB: the known-size kernel that "
                          "calibrates FETCH_SIZE/WRITE_SIZE counter passes")
@@ -102,6 +130,7 @@ def main():
     from codens_amd import abi, configs
     from codens_amd.solver import Solver
 
+    abi.residual_path(1 if args.residual == "fused" else 0)
     if args.workload == "adjoint":
         return adjoint_bench(args)
     if args.workload == "converge":
@@ -200,6 +229,7 @@ def main():
                           "same physics)")
         else:
             out["cpu_baseline"] = cpu_baseline(case, args.cpu_stages)
+            out["cpu_baseline_all_cores"] = cpu_allcores(case, 3)
     print(json.dumps(out))
 
 

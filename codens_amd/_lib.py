@@ -66,6 +66,11 @@ SIGNATURES = {
                                 c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "codens_adj_metric_vjp": [P(CodensBlock), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "codens_sweep_tile": [c_int, c_vp],
+    "codens_residual_path": [c_int, c_vp],
+    "codens_restart_write": [ctypes.c_char_p, c_vp, c_int, c_int, c_int, c_vp, c_i64, c_vp],
+    "codens_restart_read": [ctypes.c_char_p, c_vp, c_int, c_int, c_int, c_vp, c_i64, c_vp],
+    "codens_mesh_write": [ctypes.c_char_p, c_vp, c_int, c_i64, c_vp, c_vp, c_vp],
+    "codens_mesh_read": [ctypes.c_char_p, c_vp, c_int, c_i64, c_vp, c_vp, c_vp],
     "codens_sweep_tile_of": [P(CodensBlock), c_int, c_int, c_vp],
     "codens_met_smke": [P(CodensBlock), c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,
                         c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
@@ -111,6 +116,8 @@ SIGNATURES = {
                         c_vp, c_vp],
     "codens_met_klmut": [P(CodensBlock), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
     "codens_met_mtcorf1": [c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_i64, c_vp],
+    "codens_met_bord": [c_int, c_i64, c_vp, c_vp, c_vp, c_dbl, c_dbl, c_vp, c_i64, c_vp],
+    "codens_met_parko": [c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_dbl, c_vp, c_i64, c_vp],
     "codens_met_smmt": [P(CodensBlock), c_vp, c_int, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                         c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
     "codens_met_smmtr": [P(CodensBlock), c_vp, c_int, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,

@@ -9,7 +9,9 @@ contiguous and on the current HIP device.
 from __future__ import annotations
 
 import ctypes
+import os
 
+import numpy as np
 import torch
 
 from ._lib import CodensBlock, CodensPhys, CodensScheme, lib
@@ -234,6 +236,22 @@ def met_mtcorf1(face_start, lines, imxclko, mnpar, frac, stream=None):
     _chk(lib.codens_met_mtcorf1(nf, fs, _p(lines), imxclko, _p(mnpar), _p(frac), frac.numel(), _s(stream)))
 
 
+def met_bord(kind, g, i, r, rokinf, roeinf, v, stream=None):
+    """met_bord.f90:4 on one frontier list: kind 0 coincident, 1 wall
+    (kparoi 0), 2 wall (kparoi 1), 3 far field, 4 symmetry."""
+    _chk(lib.codens_met_bord(int(kind), g.numel(), _p(g), _p(i), _p(r), float(rokinf), float(roeinf), _p(v), ld(v),
+                             _s(stream)))
+
+
+def met_parko(face_start, lines, ghost, mnpar, mu, dist, beta1, v, stream=None):
+    """met_parko.f90:4: face_start host int64 (walls + 1); lines device uint8
+    (codens_wall_line records); ghost int64, mnpar int32 device tensors."""
+    fs, nf = _starts(face_start)
+    assert mnpar.dtype == torch.int32 and ghost.dtype == torch.int64
+    _chk(lib.codens_met_parko(nf, fs, _p(lines), _p(ghost), _p(mnpar), _p(mu), _p(dist), float(beta1), _p(v), ld(v),
+                              _s(stream)))
+
+
 def met_smmt(b, mc, kfracom, imxclko, v, mu, mut, dist, fd, tprod, cfke, gkgo, frac, ts6, ts7, face_start=None,
              lines=None, mnpar=None, stream=None):
     fs, nf = _starts(face_start)
@@ -400,9 +418,48 @@ def adj_metric_vjp(b, x, y, z, sg, snbar, volbar, xbar, ybar, zbar, stream=None)
 
 def sweep_tile(mode):
     """Tile shape of the Jacobi sub-iteration: -1 timed choice per block shape
-    (default), 0 = 64 x 4, 1 = 32 x 8.  Returns the previous mode."""
+    (default), 0 = 64 x 4, 1 = 32 x 8, 2 = 16 x 16.  Returns the previous mode."""
     prev = ctypes.c_int32(0)
     _chk(lib.codens_sweep_tile(int(mode), ctypes.byref(prev)))
+    return prev.value
+
+
+def _h(a):
+    """Host (numpy float64, C-contiguous) array for the file-format entry points."""
+    if a is None:
+        return None
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+        raise TypeError("expected a C-contiguous float64 numpy array")
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def restart_write(path, blocks_arr, nblk, v, mut=None, nturb=0):
+    """writda.f90:4 records of every block (host arrays: v (nvar, ldv), mut (ldv) or None)."""
+    _chk(lib.codens_restart_write(os.fsencode(path), blocks_arr, int(nblk), int(mut is not None), int(nturb), _h(v),
+                                  v.shape[-1], _h(mut)))
+
+
+def restart_read(path, blocks_arr, nblk, v, mut=None, nturb=0):
+    """readda.f90:4: the records into the real cells of the host arrays v (and mut)."""
+    _chk(lib.codens_restart_read(os.fsencode(path), blocks_arr, int(nblk), int(mut is not None), int(nturb), _h(v),
+                                 v.shape[-1], _h(mut)))
+
+
+def mesh_write(path, blocks_arr, nblk, x, y, z):
+    """writdg.f90:4 records (nodes of every block) from host arrays."""
+    _chk(lib.codens_mesh_write(os.fsencode(path), blocks_arr, int(nblk), x.shape[-1], _h(x), _h(y), _h(z)))
+
+
+def mesh_read(path, blocks_arr, nblk, x, y, z):
+    """readdg.f90:4 records into the nodes of host arrays."""
+    _chk(lib.codens_mesh_read(os.fsencode(path), blocks_arr, int(nblk), x.shape[-1], _h(x), _h(y), _h(z)))
+
+
+def residual_path(fused):
+    """Kernel arrangement of sch_roe / sch_roe_euler: 1 fused (default), 0
+    face passes + gather.  Returns the previous arrangement."""
+    prev = ctypes.c_int32(0)
+    _chk(lib.codens_residual_path(int(fused), ctypes.byref(prev)))
     return prev.value
 
 

@@ -69,6 +69,19 @@ class TurbCfg:
     kwsst: int = 0      # k-omega eddy viscosity: 1 with the SST limiter
     epsk: float = 1e-12
     epse: float = 1e-12
+    rokinf: float = 0.0  # far-field ghost values of planes 6, 7 (met_bord.f90:204-214)
+    roeinf: float = 0.0
+    # met_parko's walls (k-omega): LINE records of the wall points, their
+    # ghosts, the walls' point ranges and every cell's wall point (1-based,
+    # 0 for none); None where the model has no omega wall condition
+    wall_lines: Optional[np.ndarray] = None
+    wall_ghost: Optional[np.ndarray] = None
+    wall_start: Optional[np.ndarray] = None
+    mnpar: Optional[np.ndarray] = None
+
+    @property
+    def kparoi(self):   # atlecdon.f90:191 (1SA), :360 (2KO)
+        return {"1SA": 0, "2KO": 1}[self.model]
 
     @property
     def cmt(self):      # impli2_eqt_3d.f90:82-95
@@ -323,8 +336,37 @@ def with_turbulence(c: Case, dist, model="1SA", klroe=2, kwsst=0, nut_ratio=3.0)
         v0[5] = k; v0[6] = k / (nut_ratio * mu_inf)
     c.v0 = v0
     c.name = f"{c.name}_{model}"
-    c.turb = TurbCfg(model, turb_constants(model), dist, klroe=klroe, kwsst=kwsst)
+    c.turb = TurbCfg(model, turb_constants(model), dist, klroe=klroe, kwsst=kwsst, rokinf=float(v0[5, 0]),
+                     roeinf=float(v0[6, 0]))
+    if c.turb.kparoi == 1:
+        c.turb.wall_lines, c.turb.wall_ghost, c.turb.wall_start, c.turb.mnpar = j1_wall_lines(c)
     return c
+
+
+# record of one wall point for met_parko / met_mtcorf1 (codens_wall_line)
+WALL_LINE = np.dtype([("nmin", "<i8"), ("m3min", "<i4"), ("m3max", "<i4"), ("isens3", "<i4"), ("dm3", "<i4"),
+                      ("mpn0", "<i4"), ("mpn1", "<i4")])
+
+
+def j1_wall_lines(c: Case):
+    """The wall points of a single block's j1 wall (c.wall, in its listing
+    order), as atindnor.f90 describes them: first cell nmin, the j index from
+    jj1 to jj2 upward (dm3 = nid), wall points numbered 1..npts; every real
+    cell's wall point is the one of its (i, k) column (mnpar, 1-based)."""
+    b = c.layout.blocks[0]
+    assert len(c.layout.blocks) == 1
+    npts = len(c.wall.ghost)
+    ln = np.zeros(npts, dtype=WALL_LINE)
+    ln["nmin"] = c.wall.inner
+    ln["m3min"], ln["m3max"], ln["isens3"], ln["dm3"] = b.jj1, b.jj2, 1, b.nid
+    ln["mpn0"], ln["mpn1"] = 0, npts
+    mnpar = np.zeros(c.layout.ip11, dtype=np.int32)
+    ni = b.ii2 - b.ii1
+    for j in range(b.jj1, b.jj2):
+        k, i = np.meshgrid(np.arange(b.kk1, b.kk2), np.arange(b.ii1, b.ii2), indexing="ij")
+        mnpar[b.idx(i, j, k).ravel()] = ((k - b.kk1) * ni + (i - b.ii1)).ravel() + 1
+    assert np.array_equal(mnpar[c.wall.inner], np.arange(1, npts + 1))
+    return ln, np.asarray(c.wall.ghost, dtype=np.int64), np.array([0, npts], dtype=np.int64), mnpar
 
 
 def inclined_rans(ci=32, cj=16, ck=8, model="2KO", klroe=2, kwsst=0, nut_ratio=3.0, eta=10.0, **kw) -> Case:
@@ -457,7 +499,7 @@ def multiblock_sa(nblocks=48, lo=16, hi=192, total=(38_000_000, 42_000_000), see
              dict(gam=GAM, pinfl=0.0, pr=0.72, prt=1.0, reynz=reynz, tnz=288.15, iflu=1),
              dict(muscl=1, ilim=ilim, xk=1.0 / 3.0), eta, 1, 0.0, 1, lmax, w,
              coinc, _cat(wall_parts), _cat(far_parts), farspec=far)
-    c.turb = TurbCfg("1SA", turb_constants("1SA"), dist, klroe=2)
+    c.turb = TurbCfg("1SA", turb_constants("1SA"), dist, klroe=2, rokinf=float(v0[5, 0]), roeinf=float(v0[6, 0]))
     c.links = links
     c.origin = origin
     return c

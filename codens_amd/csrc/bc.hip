@@ -292,3 +292,102 @@ extern "C" int codens_met_rbsc(int64_t nf, const int64_t *g, const int64_t *in, 
                      fd, (long)ldv);
   return codens_check_launch("codens_met_rbsc");
 }
+
+// met_bord.f90:4 with met_rbvc.f90:30-65 and met_rbve.f90:26-36: the two
+// transported turbulence variables (planes 5, 6) of the ghosts of one
+// frontier list, by kind (codens_met_bord's codes).  A coincident ghost
+// reads two real cells, which no ghost list writes, so the reference's
+// buffer of remote values is not needed.
+__global__ void __launch_bounds__(BT) k_met_bord(int kind, long nf, const int64_t *__restrict__ g,
+                                                 const int64_t *__restrict__ in, const int64_t *__restrict__ rm,
+                                                 double a, double b, double *v, long ldv) {
+  long f = blockIdx.x * (long)BT + threadIdx.x;
+  if (f >= nf) return;
+  double *k = v + 5 * ldv, *w = v + 6 * ldv;
+  long nd = g[f], ni = in[f];
+  switch (kind) {
+    case 0: {
+      long nr = rm[f];
+      k[nd] = 0.5 * (k[ni] + k[nr]);
+      w[nd] = 0.5 * (w[ni] + w[nr]);
+      break;
+    }
+    case 1: k[nd] = 0.; w[nd] = 0.; break;
+    case 2: k[nd] = 0.; w[nd] = w[ni]; break;
+    case 3: k[nd] = a; w[nd] = b; break;
+    default: k[nd] = k[ni]; w[nd] = w[ni]; break;
+  }
+}
+
+extern "C" int codens_met_bord(int32_t kind, int64_t nf, const int64_t *g, const int64_t *in, const int64_t *rm,
+                               double rokinf, double roeinf, double *v, int64_t ldv, void *stream) {
+  if (nf == 0) return CODENS_OK;
+  if (nf < 0 || kind < 0 || kind > 4 || !g || (kind != 1 && kind != 3 && !in) || (kind == 0 && !rm) || !v ||
+      ldv <= 0) {
+    codens_set_error("codens_met_bord: invalid argument (kind %d)", kind);
+    return CODENS_EINVAL;
+  }
+  hipLaunchKernelGGL(k_met_bord, fgrid(nf), dim3(BT), 0, (hipStream_t)stream, (int)kind, (long)nf, g,
+                     in ? in : g, rm ? rm : g, rokinf, roeinf, v, (long)ldv);
+  return codens_check_launch("codens_met_bord");
+}
+
+// met_parko.f90:4, one thread per wall point: the wall cell's rho omega =
+// 6 mu / (beta1 d^2); along the wall normal, the next min(10, m3tot) cells
+// whose wall point lies on this wall are blended with it over cells 2..4
+// (imxclko = 3) and floored by it; the ghost takes the wall cell's value.
+__global__ void __launch_bounds__(BT) k_met_parko(long nl, const codens_wall_line *__restrict__ ln,
+                                                  const int64_t *__restrict__ ghost,
+                                                  const int32_t *__restrict__ mnpar,
+                                                  const double *__restrict__ mu, const double *__restrict__ dist,
+                                                  double beta1, double *v, long ldv) {
+  long p = blockIdx.x * (long)BT + threadIdx.x;
+  if (p >= nl) return;
+  double *w = v + 6 * ldv;
+  const codens_wall_line L = ln[p];
+  const int m3clpar = 10, imxclko = 3;
+  const long st = (long)L.isens3 * L.dm3;
+  const long nmin = L.nmin, nminp = nmin + st, nmaxp = nmin + st * (imxclko + 1);
+  const int m3tot = abs(L.m3max - L.m3min) + 1;
+  const int nmore = min(m3tot, m3clpar);
+  double dn = dist[nmin];
+  w[nmin] = 6. * mu[nmin] / (beta1 * (dn * dn));
+  for (int s = 1; s <= nmore; s++) {
+    long n = nmin + s * st;
+    int npar = mnpar[n];
+    if (npar > L.mpn0 && npar <= L.mpn1) {
+      double d = dist[n];
+      double rom = 6. * mu[n] / (beta1 * (d * d));
+      if ((n - nmaxp) * (n - nminp) <= 0) {
+        double pond = (d - dist[nminp]) / (dist[nmaxp] - dist[nminp]);
+        w[n] = rom * (1. - pond) + pond * w[n];
+      }
+      w[n] = fmax(w[n], rom);
+    }
+  }
+  w[ghost[p]] = w[nmin];
+}
+
+extern "C" int codens_met_parko(int32_t nfaces, const int64_t *face_start, const codens_wall_line *lines,
+                                const int64_t *ghost, const int32_t *mnpar, const double *mu, const double *dist,
+                                double beta1, double *v, int64_t ldv, void *stream) {
+  if (nfaces == 0) return CODENS_OK;
+  if (nfaces < 0 || !face_start || !lines || !ghost || !mnpar || !mu || !dist || !v || ldv <= 0) {
+    codens_set_error("codens_met_parko: invalid argument");
+    return CODENS_EINVAL;
+  }
+  // walls in order (a cell reached from two walls gets both, as the
+  // reference's loop over walls gives it); the points of one wall are
+  // distinct cell columns
+  for (int f = 0; f < nfaces; f++) {
+    long a = face_start[f], e = face_start[f + 1];
+    if (e < a) {
+      codens_set_error("codens_met_parko: face_start is not increasing");
+      return CODENS_EINVAL;
+    }
+    if (e > a)
+      hipLaunchKernelGGL(k_met_parko, fgrid(e - a), dim3(BT), 0, (hipStream_t)stream, e - a, lines + a, ghost + a,
+                         mnpar, mu, dist, beta1, v, (long)ldv);
+  }
+  return codens_check_launch("codens_met_parko");
+}

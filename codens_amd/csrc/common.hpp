@@ -122,6 +122,11 @@ void *codens_workspace(size_t bytes, int slot);
 void *codens_device_table(int kind, const std::vector<int32_t> &key, const void *data, size_t bytes,
                           hipStream_t st, const char *what);
 bool codens_timing_on();
+// implimf_3d with the caller's scratch of 9 nnn doubles (nothing allocated)
+int codens_implimf_3d_with(const codens_block *b, const codens_phys *ph, const double *u, const double *dt, double *v,
+                           int64_t ldv, double *d, const double *ff, const double *mu, const double *mut, int32_t lmx,
+                           const double *sn_blk, const double *vol, double dtpas, int32_t kfmg, int32_t ityprk,
+                           const double *cson, void *stream, double *ws, const char *what);
 void *codens_timing_begin(const char *name, hipStream_t st);
 void codens_timing_end(const char *name, hipStream_t st);
 
@@ -129,6 +134,10 @@ void codens_timing_end(const char *name, hipStream_t st);
 struct CodensTimed {
   const char *name;
   hipStream_t st;
-  CodensTimed(const char *n, hipStream_t s) : name(n), st(s) { codens_timing_begin(n, s); }
-  ~CodensTimed() { codens_timing_end(name, st); }
+  CodensTimed(const char *n, hipStream_t s) : name(n), st(s) {
+    if (n) codens_timing_begin(n, s);
+  }
+  ~CodensTimed() {
+    if (name) codens_timing_end(name, st);
+  }
 };

@@ -11,6 +11,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 #include <array>
 #include <map>
@@ -183,6 +184,7 @@ __device__ __forceinline__ void coef_body(const Blk &b, dim3 tile, const Phys &p
                                           const double *__restrict__ cson, const double *__restrict__ sn,
                                           double dtpas, int kfmg, double *__restrict__ coefe,
                                           double *__restrict__ diag) {
+#pragma clang fp contract(off)  // the same roundings as k_impl_coef_tiled
   CELL3_AT(b, tile)
   long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
   double rc = 1.0 / v[n];
@@ -231,6 +233,171 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef(Blk b, Phys ph, const doub
                                                       const double *__restrict__ sn, double dtpas, int kfmg,
                                                       double *__restrict__ coefe, double *__restrict__ diag) {
   coef_body(b, xcd_block(), ph, v, ldv, mu, mut, dt, vol, cson, sn, dtpas, kfmg, coefe, diag);
+}
+
+// The same coefficients with each face evaluated once.  A workgroup owns a
+// TX x TY tile of real cells and marches CK_KS k-planes.  Per plane, the
+// velocity, sound speed, rv and volume of every cell of the tile and of its
+// 1-cell ring (i0-1..i0+TX, j0-1..j0+TY) go to LDS; each thread evaluates the
+// low i and j faces of its cell, the ring's threads the tile's high faces,
+// and the k face below its cell from its own column's previous plane.  A
+// face's coefficient is the same expression of the same two cells whichever
+// cell owns it: 0.5 (|V.S| + |S| c) + (rv_lo + rv_hi) |S|^2 / (vol_lo + vol_hi)
+// with V and c averaged (high + low) / 2, as coef_body forms it from either
+// side, so the values are coef_body's bit for bit.  A cell's diagonal adds
+// its six faces in the order of implimf_3d.f90:206-215 (i low, i high, j low,
+// j high, k low, k high) and is complete once the next plane has evaluated
+// the k face above it.
+constexpr int CK_KS = 8;
+constexpr int CK_PL = (TX + 2) * (TY + 2);
+
+struct CoefCell {  // the per-cell operands of a face coefficient
+  double u, v, w, c, rv, vol;
+};
+
+__device__ __forceinline__ CoefCell coef_cell(const Blk &b, const Phys &ph, const double *__restrict__ v, long ldv,
+                                              const double *__restrict__ mu, const double *__restrict__ mut,
+                                              const double *__restrict__ vol, const double *__restrict__ cson,
+                                              int i, int j, int k) {
+  long n = b.idx(i, j, k);
+  double r = 1.0 / v[n];
+  CoefCell q;
+  q.u = v[ldv + n] * r;
+  q.v = v[2 * ldv + n] * r;
+  q.w = v[3 * ldv + n] * r;
+  q.c = cson[n];
+  q.rv = b.real(i, j, k) ? rv_of(ph, mu, mut, r, n) : 0.;
+  q.vol = vol[n];
+  return q;
+}
+
+// coefficient of the face between cells lo and hi (normal s of the face)
+__device__ __forceinline__ double coef_face(const CoefCell &hi, const CoefCell &lo, double s1, double s2, double s3) {
+#pragma clang fp contract(off)  // the same roundings as coef_body
+  double cnds = s1 * s1 + s2 * s2 + s3 * s3;
+  double uu = 0.5 * (hi.u + lo.u), vv = 0.5 * (hi.v + lo.v), ww = 0.5 * (hi.w + lo.w);
+  double cc = 0.5 * (hi.c + lo.c), rsum = hi.rv + lo.rv, vsum = hi.vol + lo.vol;
+  double vnf = uu * s1 + vv * s2 + ww * s3;
+  return 0.5 * (fabs(vnf) + sqrt(cnds) * cc) + rsum * cnds / vsum;
+}
+
+__global__ void __launch_bounds__(TX *TY) k_impl_coef_tiled(Blk b, Phys ph, const double *__restrict__ v, long ldv,
+                                                            const double *__restrict__ mu,
+                                                            const double *__restrict__ mut,
+                                                            const double *__restrict__ dt,
+                                                            const double *__restrict__ vol,
+                                                            const double *__restrict__ cson,
+                                                            const double *__restrict__ sn, double dtpas, int kfmg,
+                                                            double *__restrict__ coefe, double *__restrict__ diag) {
+#pragma clang fp contract(off)  // the same roundings as coef_body
+  __shared__ double L[6][CK_PL];
+  __shared__ double CI[(TX + 1) * TY], CJ[TX * (TY + 1)];
+  const dim3 xb = xcd_block();
+  const int tx = threadIdx.x, ty = threadIdx.y, t = tx + TX * ty;
+  const int i0 = b.i1 + xb.x * TX, j0 = b.j1 + xb.y * TY;
+  const int kb = b.k1 + xb.z * CK_KS, ke = min(kb + CK_KS, b.k2);
+  const int i = i0 + tx, j = j0 + ty;
+  // own: a real cell; load: a real cell or the ghost past the block's high
+  // i or j end, whose faces with the last real cells this thread evaluates
+  // when the tile is partial (it is then inside the tile)
+  const bool own = i < b.i2 && j < b.j2, load = i <= b.i2 && j <= b.j2;
+  // ring cell of this thread (t < 2 TX + 2 TY): columns i0-1, i0+TX, rows
+  // j0-1, j0+TY, loaded where it lies in the block's storage
+  int ri = 0, rj = 0, rslot = 0;
+  const bool ring = t < 2 * TX + 2 * TY;
+  if (ring) {
+    int rx, ry;
+    if (t < TX) { ri = i0 + t; rj = j0 - 1; rx = t + 1; ry = 0; }
+    else if (t < 2 * TX) { ri = i0 + t - TX; rj = j0 + TY; rx = t - TX + 1; ry = TY + 1; }
+    else if (t < 2 * TX + TY) { ri = i0 - 1; rj = j0 + t - 2 * TX; rx = 0; ry = t - 2 * TX + 1; }
+    else { ri = i0 + TX; rj = j0 + t - 2 * TX - TY; rx = TX + 1; ry = t - 2 * TX - TY + 1; }
+    rslot = ry * (TX + 2) + rx;
+  }
+  const int id2 = b.id1 + (int)b.nid - 1, jd2 = b.jd1 + (int)(b.nijd / b.nid) - 1;
+  const bool rneed = ring && ri >= b.id1 && ri <= id2 && rj >= b.jd1 && rj <= jd2;
+  const int slot = (ty + 1) * (TX + 2) + (tx + 1);
+  CoefCell below{}, cur{};
+  if (own) below = coef_cell(b, ph, v, ldv, mu, mut, vol, cson, i, j, kb - 1);
+  double part = 0.;  // the diagonal of the cell below, all but its k high face
+  long mb = -1;
+  for (int k = kb; k <= ke; k++) {
+    const bool top = k == ke;  // the k face above the slab's last plane only
+    if (load) cur = coef_cell(b, ph, v, ldv, mu, mut, vol, cson, i, j, k);
+    if (!top) {
+      if (load) {
+        L[0][slot] = cur.u; L[1][slot] = cur.v; L[2][slot] = cur.w;
+        L[3][slot] = cur.c; L[4][slot] = cur.rv; L[5][slot] = cur.vol;
+      }
+      if (rneed) {
+        CoefCell q = coef_cell(b, ph, v, ldv, mu, mut, vol, cson, ri, rj, k);
+        L[0][rslot] = q.u; L[1][rslot] = q.v; L[2][rslot] = q.w;
+        L[3][rslot] = q.c; L[4][rslot] = q.rv; L[5][rslot] = q.vol;
+      }
+    }
+    __syncthreads();
+    double ckl = 0.;
+    long m = -1;
+    if (own) {
+      long n = b.idx(i, j, k);
+      m = n - b.npc;
+      ckl = coef_face(cur, below, snv(sn, b.nnn, m, 2, 0), snv(sn, b.nnn, m, 2, 1), snv(sn, b.nnn, m, 2, 2));
+      if (mb >= 0) {  // the cell below is complete
+        double dg = part + ckl;
+        if (kfmg == 3) dg = dg + 1.5 * vol[mb + b.npc] / dtpas;
+        diag[mb] = dg;
+      }
+    }
+    if (!top) {
+      auto lds = [&](int s) {
+        CoefCell q;
+        q.u = L[0][s]; q.v = L[1][s]; q.w = L[2][s]; q.c = L[3][s]; q.rv = L[4][s]; q.vol = L[5][s];
+        return q;
+      };
+      // the low i face of this thread's cell where it bounds a real cell
+      // (rows j < j2, up to the block's high boundary face i = i2), the low
+      // j face likewise
+      const long mo = b.idx(min(i, b.i2), min(j, b.j2), k) - b.npc;
+      if (i <= b.i2 && j < b.j2)
+        CI[ty * (TX + 1) + tx] = coef_face(lds(slot), lds(slot - 1), snv(sn, b.nnn, mo, 0, 0),
+                                           snv(sn, b.nnn, mo, 0, 1), snv(sn, b.nnn, mo, 0, 2));
+      if (i < b.i2 && j <= b.j2)
+        CJ[ty * TX + tx] = coef_face(lds(slot), lds(slot - (TX + 2)), snv(sn, b.nnn, mo, 1, 0),
+                                     snv(sn, b.nnn, mo, 1, 1), snv(sn, b.nnn, mo, 1, 2));
+      // the tile's high faces: i face at i0+TX of row j0+t (t < TY), j face
+      // at j0+TY of column i0+t-TY (TY <= t < TY + TX), where they bound a
+      // real cell of the tile
+      if (t < TY) {
+        int fj = j0 + t, fi = i0 + TX;
+        if (fi <= b.i2 && fj < b.j2) {
+          long mf = b.idx(fi, fj, k) - b.npc;
+          int s = (t + 1) * (TX + 2) + TX + 1;
+          CI[t * (TX + 1) + TX] = coef_face(lds(s), lds(s - 1), snv(sn, b.nnn, mf, 0, 0), snv(sn, b.nnn, mf, 0, 1),
+                                            snv(sn, b.nnn, mf, 0, 2));
+        }
+      } else if (t < TY + TX) {
+        int fi = i0 + t - TY, fj = j0 + TY;
+        if (fj <= b.j2 && fi < b.i2) {
+          long mf = b.idx(fi, fj, k) - b.npc;
+          int s = (TY + 1) * (TX + 2) + (t - TY) + 1;
+          CJ[TY * TX + (t - TY)] = coef_face(lds(s), lds(s - (TX + 2)), snv(sn, b.nnn, mf, 1, 0),
+                                             snv(sn, b.nnn, mf, 1, 1), snv(sn, b.nnn, mf, 1, 2));
+        }
+      }
+    }
+    __syncthreads();
+    if (!top && own) {
+      long n = m + b.npc;
+      double c0 = CI[ty * (TX + 1) + tx], c1 = CI[ty * (TX + 1) + tx + 1];
+      double c2 = CJ[ty * TX + tx], c3 = CJ[(ty + 1) * TX + tx];
+      coefe[m] = c0;
+      coefe[b.nnn + m] = c2;
+      coefe[2 * b.nnn + m] = ckl;
+      part = vol[n] / dt[n];
+      part = part + c0 + c1 + c2 + c3 + ckl;
+      mb = m;
+    }
+    below = cur;
+  }
 }
 
 // implimf_eu.f90:124-170: coefficients of the Euler implicit phase,
@@ -293,14 +460,30 @@ constexpr int SW_TX = 64, SW_TY = CODENS_SW_TY, SW_KS = CODENS_SW_KS;
 static_assert(2 * SW_TX + 2 * SW_TY <= SW_TX * SW_TY, "one halo cell per thread at most");
 constexpr int SW_HALO = 2 * SW_TX + 2 * SW_TY;
 
+// The increment of the first sub-iteration, formed where it is read instead
+// of stored: d = (-u [- ff]) / diag (implimf_3d.f90:230-267 at ls = 1, the
+// first sub-iteration starting from d = 0), the expression of k_impl_sweep1
+// bit for bit.  The second sub-iteration reads it through this view, so the
+// first needs no pass of its own.
+template <typename T>
+struct FirstView {
+  const T *u, *ff, *diag;
+  long ldv, npc;
+  int ityprk;
+  __device__ inline T at(int e, long n) const {
+    T r = ityprk == 0 ? -u[e * ldv + n] : -u[e * ldv + n] - ff[e * ldv + n];
+    return r / diag[n - npc];
+  }
+};
+
 template <typename T>
 struct Q14T {
   T q[14];
 };
 using Q14 = Q14T<double>;
 
-template <typename T>
-__device__ inline void q_of(const Phys &ph, const T *__restrict__ v, long ldv, const TView<T> &d, long n, Q14T<T> &Q) {
+template <typename T, class DIN>
+__device__ inline void q_of(const Phys &ph, const T *__restrict__ v, long ldv, const DIN &d, long n, Q14T<T> &Q) {
   T V[5], D[5];
 #pragma unroll
   for (int e = 0; e < 5; e++) {
@@ -315,7 +498,8 @@ __device__ inline void q_of(const Phys &ph, const T *__restrict__ v, long ldv, c
 }
 
 // the same 14-vector for implimf_eu (its own flux increments, caller's p)
-__device__ inline void q_of_eu(const Phys &ph, const double *__restrict__ v, long ldv, const DView &d,
+template <class DIN>
+__device__ inline void q_of_eu(const Phys &ph, const double *__restrict__ v, long ldv, const DIN &d,
                                const double *__restrict__ ps, long n, Q14 &Q) {
   double V[5], D[5];
 #pragma unroll
@@ -330,11 +514,11 @@ __device__ inline void q_of_eu(const Phys &ph, const double *__restrict__ v, lon
   Q.q[12] = D[0]; Q.q[13] = D[4];
 }
 
-template <bool EU, typename T>
-__device__ inline void q_model(const Phys &ph, const T *__restrict__ v, long ldv, const TView<T> &d,
+template <bool EU, typename T, class DIN>
+__device__ inline void q_model(const Phys &ph, const T *__restrict__ v, long ldv, const DIN &d,
                                const double *__restrict__ ps, long n, Q14T<T> &Q) {
   if constexpr (EU) q_of_eu(ph, v, ldv, d, ps, n, Q);
-  else q_of<T>(ph, v, ldv, d, n, Q);
+  else q_of<T, DIN>(ph, v, ldv, d, n, Q);
 }
 
 template <typename T>
@@ -352,8 +536,8 @@ struct VD {
   double p;
 };
 
-template <bool EU, typename T>
-__device__ inline void vd_load(const T *__restrict__ v, long ldv, const TView<T> &d, const double *__restrict__ ps,
+template <bool EU, typename T, class DIN>
+__device__ inline void vd_load(const T *__restrict__ v, long ldv, const DIN &d, const double *__restrict__ ps,
                                long n, bool real, VD<T> &x) {
 #pragma unroll
   for (int e = 0; e < 5; e++) {
@@ -452,19 +636,21 @@ __device__ inline void lds_sync() {
 #endif
 // EU selects the flux increments of implimf_eu (which read ps); the face
 // terms, coefficients and update are common to both implicit phases.
-template <bool EU, typename T, int TXS, int TYS>
+template <bool EU, typename T, int TXS, int TYS, class DIN>
 __device__ __forceinline__ void sweep_body(const Blk b, int lg, int ntx, int nty, const Phys ph, int ityprk,
                                            const T *__restrict__ u, const T *__restrict__ ff,
                                            const T *__restrict__ v, long ldv, const double *__restrict__ ps,
                                            const T *__restrict__ sn, const T *__restrict__ coefe,
-                                           const T *__restrict__ diag, const TView<T> dold, const TView<T> dnew,
+                                           const T *__restrict__ diag, const DIN dold, const TView<T> dnew,
                                            T *sq) {
   constexpr int PL = (TYS + 2) * (TXS + 2), HALO = 2 * TXS + 2 * TYS;
   static_assert(HALO <= TXS * TYS, "one halo cell per thread at most");
 #include "sweep_tile.inc"
 }
 
-template <bool EU, typename T, int TXS = SW_TX, int TYS = SW_TY>
+// FIRST: the second sub-iteration, reading the first one's increment as
+// FirstView forms it from u, ff and diag (dold_mem unused)
+template <bool EU, typename T, int TXS = SW_TX, int TYS = SW_TY, bool FIRST = false>
 __global__ void __launch_bounds__(TXS *TYS) SW_ATTR k_impl_sweep_tiled(Blk b, Phys ph, int ityprk, int ntx, int nty,
                                                                    int nkz, const T *__restrict__ u,
                                                                    const T *__restrict__ ff,
@@ -472,8 +658,12 @@ __global__ void __launch_bounds__(TXS *TYS) SW_ATTR k_impl_sweep_tiled(Blk b, Ph
                                                                    const double *__restrict__ ps,
                                                                    const T *__restrict__ sn,
                                                                    const T *__restrict__ coefe,
-                                                                   const T *__restrict__ diag, TView<T> dold,
+                                                                   const T *__restrict__ diag, TView<T> dold_mem,
                                                                    TView<T> dnew, int rev) {
+  using DIN = std::conditional_t<FIRST, FirstView<T>, TView<T>>;
+  DIN dold;
+  if constexpr (FIRST) dold = FirstView<T>{u, ff, diag, ldv, b.npc, ityprk};
+  else dold = dold_mem;
   // XCD-aware tile order: the hardware deals workgroups to the 8 XCDs round
   // robin; give each XCD a contiguous run of tiles so that j/k neighbours
   // share its L2.  rev runs each XCD's tiles in reverse: alternate
@@ -501,19 +691,22 @@ __global__ void __launch_bounds__(TXS *TYS) SW_ATTR k_impl_sweep_tiled(Blk b, Ph
 // graph, an unseen shape takes the shape with fewer idle lanes and halo
 // cells instead.  codens_sweep_tile forces one shape.
 namespace {
-constexpr int N_SWEEP_TILES = 2;
+// shapes 0 (SW_TX x SW_TY), 1 (32 x 2 SW_TY) and 2 (16 x 16), all of 256
+// threads; the all-blocks launch holds the first two (N_BATCH_TILES)
+constexpr int N_SWEEP_TILES = 3, N_BATCH_TILES = 2;
 std::mutex g_tile_mu;
 int g_tile_mode = -1;
 std::map<std::array<int, 5>, int> g_tile_best;
 
-inline int tile_tx(int tile) { return tile == 1 ? 32 : SW_TX; }
-inline int tile_ty(int tile) { return tile == 1 ? 2 * SW_TY : SW_TY; }
+inline int tile_tx(int tile) { return tile == 2 ? 16 : (tile == 1 ? 32 : SW_TX); }
+inline int tile_ty(int tile) { return tile == 2 ? 16 : (tile == 1 ? 2 * SW_TY : SW_TY); }
 
-int tile_model(const Blk &b) {
+// the shape with the fewest idle lanes and halo cells among the first n
+int tile_model(const Blk &b, int n = N_BATCH_TILES) {
   int nx = b.i2 - b.i1, ny = b.j2 - b.j1;
   double best = 0.;
   int pick = 0;
-  for (int t = 0; t < N_SWEEP_TILES; t++) {
+  for (int t = 0; t < n; t++) {
     int tx = tile_tx(t), ty = tile_ty(t);
     double lanes = (double)((nx + tx - 1) / tx * tx) * ((ny + ty - 1) / ty * ty);
     double cost = lanes * (tx + 2) * (ty + 2) / (double)(tx * ty);
@@ -522,24 +715,29 @@ int tile_model(const Blk &b) {
   return pick;
 }
 
-template <bool EU, typename T>
+template <bool EU, typename T, bool FIRST = false>
 void launch_sweep(int tile, const Blk &b, hipStream_t st, Phys ph, int ityprk, const T *u, const T *ff, const T *v,
                   long ldv, const double *ps, const T *sn, const T *coefe, const T *diag, TView<T> in,
                   TView<T> out, int rev = 0) {
   int nkz = (b.k2 - b.k1 + SW_KS - 1) / SW_KS;
-  if (tile == 1) {
+  if (tile == 2) {
+    constexpr int X = 16, Y = 16;
+    int ntx = (b.i2 - b.i1 + X - 1) / X, nty = (b.j2 - b.j1 + Y - 1) / Y;
+    hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, X, Y, FIRST>), dim3(ntx * nty * nkz), dim3(X, Y), 0, st, b, ph,
+                       ityprk, ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
+  } else if (tile == 1) {
     constexpr int X = 32, Y = 2 * SW_TY;
     int ntx = (b.i2 - b.i1 + X - 1) / X, nty = (b.j2 - b.j1 + Y - 1) / Y;
-    hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, X, Y>), dim3(ntx * nty * nkz), dim3(X, Y), 0, st, b, ph, ityprk,
-                       ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
+    hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, X, Y, FIRST>), dim3(ntx * nty * nkz), dim3(X, Y), 0, st, b, ph,
+                       ityprk, ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
   } else {
     int ntx = (b.i2 - b.i1 + SW_TX - 1) / SW_TX, nty = (b.j2 - b.j1 + SW_TY - 1) / SW_TY;
-    hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, SW_TX, SW_TY>), dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY), 0, st,
-                       b, ph, ityprk, ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
+    hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, SW_TX, SW_TY, FIRST>), dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY),
+                       0, st, b, ph, ityprk, ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
   }
 }
 
-template <bool EU, typename T>
+template <bool EU, typename T, bool FIRST = false>
 int pick_tile(const Blk &b, hipStream_t st, Phys ph, int ityprk, const T *u, const T *ff, const T *v, long ldv,
               const double *ps, const T *sn, const T *coefe, const T *diag, TView<T> in, TView<T> out) {
   std::array<int, 5> key = {b.i2 - b.i1, b.j2 - b.j1, b.k2 - b.k1, (int)EU, (int)sizeof(T)};
@@ -560,9 +758,10 @@ int pick_tile(const Blk &b, hipStream_t st, Phys ph, int ityprk, const T *u, con
   int pick = tile_model(b);
   float best = -1.f;
   for (int t = 0; t < N_SWEEP_TILES; t++) {
-    launch_sweep<EU, T>(t, b, st, ph, ityprk, u, ff, v, ldv, ps, sn, coefe, diag, in, out);
+    launch_sweep<EU, T, FIRST>(t, b, st, ph, ityprk, u, ff, v, ldv, ps, sn, coefe, diag, in, out);
     (void)hipEventRecord(e0, st);
-    for (int r = 0; r < 2; r++) launch_sweep<EU, T>(t, b, st, ph, ityprk, u, ff, v, ldv, ps, sn, coefe, diag, in, out);
+    for (int r = 0; r < 2; r++)
+      launch_sweep<EU, T, FIRST>(t, b, st, ph, ityprk, u, ff, v, ldv, ps, sn, coefe, diag, in, out);
     (void)hipEventRecord(e1, st);
     float ms = 0.f;
     if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
@@ -580,9 +779,20 @@ int pick_tile(const Blk &b, hipStream_t st, Phys ph, int ityprk, const T *u, con
 }
 }  // namespace
 
+// tile of a block without timing anything: the forced mode, the choice
+// already timed for its shape, or the model's
+static int known_tile(const Blk &b, bool eu, int tsize) {
+  std::array<int, 5> key = {b.i2 - b.i1, b.j2 - b.j1, b.k2 - b.k1, eu ? 1 : 0, tsize};
+  std::lock_guard<std::mutex> lk(g_tile_mu);
+  if (g_tile_mode >= 0) return g_tile_mode;
+  auto it = g_tile_best.find(key);
+  return it != g_tile_best.end() ? it->second : tile_model(b);
+}
+
 extern "C" int codens_sweep_tile(int32_t mode, int32_t *previous) {
   if (mode < -1 || mode >= N_SWEEP_TILES) {
-    codens_set_error("codens_sweep_tile: mode %d is not -1 (timed choice), 0 (64 x 4) or 1 (32 x 8)", mode);
+    codens_set_error("codens_sweep_tile: mode %d is not -1 (timed choice), 0 (64 x 4), 1 (32 x 8) or 2 (16 x 16)",
+                     mode);
     return CODENS_EINVAL;
   }
   std::lock_guard<std::mutex> lk(g_tile_mu);
@@ -629,11 +839,15 @@ __global__ void k_zero5(double *d, long ldv, long n0, long n1) {
 // Shared host sequence of both implicit phases: zero d, coefficients, lmx
 // sub-iterations (ping-pong between the caller's d and the workspace, the
 // last one landing in d), then v += d on nvar planes.
+// ws_ext: scratch of 9 nnn doubles supplied by the caller (then nothing is
+// allocated, no timing event is created and the sub-iteration tile is the
+// forced or already chosen one, else the model's); nullptr: the library's
+// workspace.
 static int implimf_common(bool eu, const codens_block *cb, const codens_phys *cp, const double *u, const double *dt,
                           double *v, int64_t ldv, int nvar, double *d, const double *ff, const double *mu,
                           const double *mut, int32_t lmx, const double *sn, const double *vol, double dtpas,
                           double dt1min, int32_t kfmg, int32_t ityprk, const double *ps, const double *cson,
-                          void *stream, const char *what) {
+                          void *stream, const char *what, double *ws_ext = nullptr) {
   int rc = codens_check_block(cb, what);
   if (rc) return rc;
   if (!cp || !u || !dt || !v || !d || (!eu && (!mu || !mut)) || (eu && !ps) || !sn || !vol || !cson ||
@@ -648,53 +862,77 @@ static int implimf_common(bool eu, const codens_block *cb, const codens_phys *cp
   Blk b = make_blk(cb);
   Phys ph = make_phys(cp);
   hipStream_t st = (hipStream_t)stream;
-  double *ws = (double *)codens_workspace(sizeof(double) * 9 * b.nnn, 2);
+  const bool fixed = ws_ext != nullptr;
+  double *ws = fixed ? ws_ext : (double *)codens_workspace(sizeof(double) * 9 * b.nnn, 2);
   if (!ws) {
     codens_set_error("%s: workspace allocation failed", what);
     return CODENS_EHIP;
   }
   double *coefe = ws, *diag = ws + 3 * b.nnn, *W = ws + 4 * b.nnn;
   long n0 = b.npc, n1 = b.npc + b.nnn - 1;
+  auto tname = [fixed](const char *n) { return fixed ? (const char *)nullptr : n; };
   hipLaunchKernelGGL(k_zero5, dim3((b.nnn + 255) / 256), dim3(256), 0, st, d, (long)ldv, n0, n1);
   dim3 g = cell_grid(b), t(TX, TY);
   {
-    CodensTimed _tm("implimf_coef", st);
-    if (eu)
+    CodensTimed _tm(tname("implimf_coef"), st);
+    if (eu) {
       hipLaunchKernelGGL(k_impl_coef_eu, g, t, 0, st, b, v, (long)ldv, dt, vol, cson, sn,
                          kfmg == 3 ? 1.5 / dt1min : 0., coefe, diag);
-    else
-      hipLaunchKernelGGL(k_impl_coef, g, t, 0, st, b, ph, v, (long)ldv, mu, mut, dt, vol, cson, sn, dtpas,
+    } else {
+      dim3 gc(g.x, g.y, (unsigned)((b.k2 - b.k1 + CK_KS - 1) / CK_KS));
+      hipLaunchKernelGGL(k_impl_coef_tiled, gc, t, 0, st, b, ph, v, (long)ldv, mu, mut, dt, vol, cson, sn, dtpas,
                          (int)kfmg, coefe, diag);
+    }
   }
   DView A = {d, (long)ldv, 0}, B = {W, b.nnn, b.npc};
   int tile = 0;
-  for (int s = 1; s <= lmx; s++) {
+  // sub-iteration 1 (d = -u/diag) runs as a pass of its own only when it is
+  // the last; otherwise sub-iteration 2 forms it where it reads it
+  // (FirstView), bit for bit the same values
+  for (int s = (lmx == 1 ? 1 : 2); s <= lmx; s++) {
     DView out = ((lmx - s) % 2 == 0) ? A : B;
     DView in = ((lmx - s) % 2 == 0) ? B : A;
+    const double *fx = ityprk ? ff : u;
     if (s == 1) {
-      CodensTimed _tm("implimf_sweep1", st);
-      hipLaunchKernelGGL(k_impl_sweep1<double>, g, t, 0, st, b, (int)ityprk, u, ityprk ? ff : u, (long)ldv, diag, out);
+      CodensTimed _tm(tname("implimf_sweep1"), st);
+      hipLaunchKernelGGL(k_impl_sweep1<double>, g, t, 0, st, b, (int)ityprk, u, fx, (long)ldv, diag, out);
     } else {
       const double *psx = eu ? ps : v;
-      if (s == 2)
-        tile = eu ? pick_tile<true, double>(b, st, ph, (int)ityprk, u, ityprk ? ff : u, v, (long)ldv, psx, sn, coefe,
-                                           diag, in, out)
-                  : pick_tile<false, double>(b, st, ph, (int)ityprk, u, ityprk ? ff : u, v, (long)ldv, psx, sn, coefe,
-                                            diag, in, out);
-      CodensTimed _tm("implimf_sweep", st);
-      if (eu)
-        launch_sweep<true, double>(tile, b, st, ph, (int)ityprk, u, ityprk ? ff : u, v, (long)ldv, psx, sn, coefe,
-                                   diag, in, out, s % 2 == 0);
+      if (s == 2 && fixed)
+        tile = known_tile(b, eu, (int)sizeof(double));
+      else if (s == 2)
+        tile = eu ? pick_tile<true, double, true>(b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag,
+                                                  in, out)
+                  : pick_tile<false, double, true>(b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag,
+                                                   in, out);
+      CodensTimed _tm(tname("implimf_sweep"), st);
+      if (eu && s == 2)
+        launch_sweep<true, double, true>(tile, b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag, in,
+                                         out, 1);
+      else if (eu)
+        launch_sweep<true, double>(tile, b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag, in, out,
+                                   s % 2 == 0);
+      else if (s == 2)
+        launch_sweep<false, double, true>(tile, b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag,
+                                          in, out, 1);
       else
-        launch_sweep<false, double>(tile, b, st, ph, (int)ityprk, u, ityprk ? ff : u, v, (long)ldv, psx, sn, coefe,
-                                    diag, in, out, s % 2 == 0);
+        launch_sweep<false, double>(tile, b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag, in, out,
+                                    s % 2 == 0);
     }
   }
   {
-    CodensTimed _tm("implimf_update", st);
+    CodensTimed _tm(tname("implimf_update"), st);
     hipLaunchKernelGGL(k_impl_update, g, t, 0, st, b, v, (long)ldv, d, nvar);
   }
   return codens_check_launch(what);
+}
+
+int codens_implimf_3d_with(const codens_block *b, const codens_phys *ph, const double *u, const double *dt, double *v,
+                           int64_t ldv, double *d, const double *ff, const double *mu, const double *mut, int32_t lmx,
+                           const double *sn_blk, const double *vol, double dtpas, int32_t kfmg, int32_t ityprk,
+                           const double *cson, void *stream, double *ws, const char *what) {
+  return implimf_common(false, b, ph, u, dt, v, ldv, 5, d, ff, mu, mut, lmx, sn_blk, vol, dtpas, 1.0, kfmg, ityprk,
+                        nullptr, cson, stream, what, ws);
 }
 
 // ---------------------------------------------------------------------------
@@ -810,7 +1048,9 @@ extern "C" int codens_implimf_3d_batch(int32_t nblk, const codens_block *blocks,
     int nkz = (b.k2 - b.k1 + SW_KS - 1) / SW_KS;
     {
       std::lock_guard<std::mutex> lk(g_tile_mu);
-      E.tile = g_tile_mode >= 0 ? g_tile_mode : tile_model(b);
+      // the all-blocks launch holds shapes 0 and 1 (a forced 16 x 16 takes
+      // the model's choice between them)
+      E.tile = (g_tile_mode >= 0 && g_tile_mode < N_BATCH_TILES) ? g_tile_mode : tile_model(b);
     }
     const int tx = tile_tx(E.tile), ty = tile_ty(E.tile);
     unsigned ntx = (b.i2 - b.i1 + tx - 1) / tx, nty = (b.j2 - b.j1 + ty - 1) / ty;
@@ -934,18 +1174,23 @@ extern "C" int codens_implimf_3d_f32(const codens_block *cb, const codens_phys *
   TView<float> A = {dA, nnn, b.npc}, B = {dB, nnn, b.npc};
   TView<float> last = A;
   int tile = 0;
-  for (int s = 1; s <= lmx; s++) {
+  // as in the FP64 phase: sub-iteration 2 forms sub-iteration 1's increment
+  for (int s = (lmx == 1 ? 1 : 2); s <= lmx; s++) {
     TView<float> out = (s % 2) ? A : B, in = (s % 2) ? B : A;
     if (s == 1) {
       CodensTimed _tm("implimf32_sweep1", st);
       hipLaunchKernelGGL(k_impl_sweep1<float>, g, t, 0, st, b, (int)ityprk, uS, ityprk ? fS : uS, nnn, dgF, out);
     } else {
       if (s == 2)
-        tile = pick_tile<false, float>(b, st, ph, (int)ityprk, uS, ityprk ? fS : uS, vS, nnn, (const double *)nullptr,
-                                       snF, cF, dgF, in, out);
+        tile = pick_tile<false, float, true>(b, st, ph, (int)ityprk, uS, ityprk ? fS : uS, vS, nnn,
+                                             (const double *)nullptr, snF, cF, dgF, in, out);
       CodensTimed _tm("implimf32_sweep", st);
-      launch_sweep<false, float>(tile, b, st, ph, (int)ityprk, uS, ityprk ? fS : uS, vS, nnn, (const double *)nullptr,
-                                 snF, cF, dgF, in, out, s % 2 == 0);
+      if (s == 2)
+        launch_sweep<false, float, true>(tile, b, st, ph, (int)ityprk, uS, ityprk ? fS : uS, vS, nnn,
+                                         (const double *)nullptr, snF, cF, dgF, in, out, 1);
+      else
+        launch_sweep<false, float>(tile, b, st, ph, (int)ityprk, uS, ityprk ? fS : uS, vS, nnn,
+                                   (const double *)nullptr, snF, cF, dgF, in, out, s % 2 == 0);
     }
     last = out;
   }

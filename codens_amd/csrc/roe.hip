@@ -145,6 +145,35 @@ __device__ void roe_face(const Phys &ph, double s0, double s1, double s2, const 
   f[4] = dfex * s0 + dfey * s1 + dfez * s2 - di4;
 }
 
+// Half the Roe flux of one interior face between cells a-1 and a
+// (sch_roe.f90:196-474): q[s][e], primitive variable e of cell a-2+s (s =
+// 0..3); with lim, the MUSCL slopes are limited by the chained ratios rl of
+// face a and rh of face a+1 (minmod, sch_roe.f90:1500-1504); cm = muscl/4;
+// tl, tr: stresses and heat flux of cells a-1 and a (zero for Euler).
+__device__ __forceinline__ void muscl_roe_half(const Phys &ph, double s0, double s1, double s2, const double q[4][5],
+                                               bool lim, const double rl[5], const double rh[5], double cm, double xk,
+                                               const double tl[9], const double tr[9], double f[5]) {
+  double L[5], R[5];
+  if (lim) {
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+      double qa = q[0][e], qb = q[1][e], qc = q[2][e], qd = q[3][e];
+      L[e] = qb + cm * minmod_phi(rl[e]) * (1. - xk) * (qb - qa) + cm * minmod_phi(1. / rl[e]) * (1. + xk) * (qc - qb);
+      R[e] = qc - cm * minmod_phi(rh[e]) * (1. + xk) * (qc - qb) - cm * minmod_phi(1. / rh[e]) * (1. - xk) * (qd - qc);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 5; e++) {
+      double qa = q[0][e], qb = q[1][e], qc = q[2][e], qd = q[3][e];
+      L[e] = qb + cm * ((1. - xk) * (qb - qa) + (1. + xk) * (qc - qb));
+      R[e] = qc - cm * ((1. + xk) * (qc - qb) + (1. - xk) * (qd - qc));
+    }
+  }
+  roe_face(ph, s0, s1, s2, L, R, tl, tr, f);
+#pragma unroll
+  for (int e = 0; e < 5; e++) f[e] *= 0.5;
+}
+
 // Chained slope ratios of the limiter (sch_roe.f90:146-194).  Along each grid
 // line the reference first stores the differences r(a) = q(a) - q(a-1) of
 // every face, then replaces r(a) by r(a)/r(a-1) in storage order, so the
@@ -387,26 +416,19 @@ __device__ __forceinline__ void face_flux_body(const Blk &b, dim3 xb, const Phys
     double rdiv = (!ns && a == a1 && d > 0) ? v[n] : w[0];  // sch_roe_euler.f90:630-632, 912-914
     ghost_flux(ph, w, ps[ng], T, rdiv, !ns, s0, s1, s2, f);
   } else {
-    Prim q[4];
+    double q[4][5];
 #pragma unroll
-    for (int s = 0; s < 4; s++) q[s] = prim_at(v, uvw, b.nnn, b.npc, ps, n + (s - 2) * inc);
-    double L[5], R[5];
-    double cm = 0.25 * muscl;
-    if (ilim == 1 && a <= a2 - 2) {
+    for (int s = 0; s < 4; s++) {
+      Prim c = prim_at(v, uvw, b.nnn, b.npc, ps, n + (s - 2) * inc);
 #pragma unroll
-      for (int e = 0; e < 5; e++) {
-        double rl = rr[e * b.nnn + m], rh = rr[e * b.nnn + m + inc];
-        double qa = pget(q[0], e), qb = pget(q[1], e), qc = pget(q[2], e), qd = pget(q[3], e);
-        L[e] = qb + cm * minmod_phi(rl) * (1. - xk) * (qb - qa) + cm * minmod_phi(1. / rl) * (1. + xk) * (qc - qb);
-        R[e] = qc - cm * minmod_phi(rh) * (1. + xk) * (qc - qb) - cm * minmod_phi(1. / rh) * (1. - xk) * (qd - qc);
-      }
-    } else {
+      for (int e = 0; e < 5; e++) q[s][e] = pget(c, e);
+    }
+    const bool lim = ilim == 1 && a <= a2 - 2;
+    double rl[5], rh[5];
 #pragma unroll
-      for (int e = 0; e < 5; e++) {
-        double qa = pget(q[0], e), qb = pget(q[1], e), qc = pget(q[2], e), qd = pget(q[3], e);
-        L[e] = qb + cm * ((1. - xk) * (qb - qa) + (1. + xk) * (qc - qb));
-        R[e] = qc - cm * ((1. + xk) * (qc - qb) + (1. - xk) * (qd - qc));
-      }
+    for (int e = 0; e < 5; e++) {
+      rl[e] = lim ? rr[e * b.nnn + m] : 0.;
+      rh[e] = lim ? rr[e * b.nnn + m + inc] : 0.;
     }
     double tl[9], tr[9];
 #pragma unroll
@@ -414,9 +436,7 @@ __device__ __forceinline__ void face_flux_body(const Blk &b, dim3 xb, const Phys
       tl[q2] = ns ? tq[q2 * ldv + n - inc] : 0.;
       tr[q2] = ns ? tq[q2 * ldv + n] : 0.;
     }
-    roe_face(ph, s0, s1, s2, L, R, tl, tr, f);
-#pragma unroll
-    for (int e = 0; e < 5; e++) f[e] *= 0.5;
+    muscl_roe_half(ph, s0, s1, s2, q, lim, rl, rh, 0.25 * muscl, xk, tl, tr, f);
   }
 #pragma unroll
   for (int e = 0; e < 5; e++) H[e * b.nnn + m] = f[e];
@@ -469,6 +489,298 @@ __global__ void __launch_bounds__(256) k_gather(Blk b, long n0, long n1, const d
   gather_body(b, gridDim.x - 1 - blockIdx.x, n0, n1, H, u, ldv, ff, ityprk);  // after the k pass, in reverse
 }
 
+// ---------------------------------------------------------------------------
+// Fused residual: the face fluxes and the gather of passes 2-3 in one kernel,
+// with no half-flux planes H in HBM.  A workgroup owns a tile of FT x FT
+// cells in i-j and marches a slab of up to FS k-planes.  Per plane it stages
+// the primitive variables of the tile with a 2-cell halo in i and j (for the
+// MUSCL stencil) and the stresses/heat flux with a 1-cell halo in LDS,
+// evaluates every i face (FT+1 per row) and j face (FT+1 per column) of the
+// tile once, and keeps each thread's own k column in registers: the face
+// between planes k and k+1 is evaluated once per cell and carried to the
+// next plane as its low face.  A cell's residual then sums the same face
+// values in the same order as the gather,
+//   u = (F_i(hi) - F_i(lo)) + (F_j(hi) - F_j(lo)) + (F_k(hi) - F_k(lo)),
+// so each face value, and u, are those of passes 2-3 (each face is the same
+// function of the same operands).  The k face between two slabs is evaluated
+// by both (one extra face in FS per cell).
+//
+// Faces on the block's boundary (a = a1, a2) are the physical flux of the
+// ghost state (sch_roe.f90:497-578), read from HBM (they are few).  The
+// FT+1-th face of each row and column (32 per plane) is taken by half a wave,
+// a different wave on each plane, so the extra work is spread over the four.
+constexpr int FT = 16;       // tile cells per direction (i, j)
+constexpr int FPP = FT + 4;  // pitch of the primitive plane (halo 2)
+constexpr int FQP = FT + 2;  // pitch of the stress plane (halo 1)
+constexpr int FS = 8;        // k planes per workgroup
+
+struct FusedArgs {
+  Blk b;
+  Phys ph;
+  double cm, xk;
+  const double *v;
+  long ldv;
+  const double *ps, *tq, *sn, *rr;
+  double *u, *ff;
+  int ityprk;
+};
+
+// primitive variables of storage entry n: rho, u, v, w (the divisions of
+// pass 0) and p
+__device__ __forceinline__ void fused_prim(const FusedArgs &A, long n, double q[5]) {
+  double r = A.v[n];
+  q[0] = r;
+  q[1] = A.v[A.ldv + n] / r;
+  q[2] = A.v[2 * A.ldv + n] / r;
+  q[3] = A.v[3 * A.ldv + n] / r;
+  q[4] = A.ps[n];
+}
+
+// physical flux of the ghost state of boundary face (direction d, storage
+// entry n of the face's high-side cell; ghost at n - inc on the low face, n
+// on the high one)
+template <bool NS>
+__device__ __forceinline__ void fused_ghost(const FusedArgs &A, int d, bool low, long n, long inc, double s0,
+                                            double s1, double s2, double f[5]) {
+  long ng = low ? n - inc : n;
+  double w[5], T[9];
+#pragma unroll
+  for (int e = 0; e < 5; e++) w[e] = A.v[e * A.ldv + ng];
+#pragma unroll
+  for (int q = 0; q < 9; q++) T[q] = NS ? A.tq[q * A.ldv + ng] : 0.;
+  double rdiv = (!NS && low && d > 0) ? A.v[n] : w[0];  // sch_roe_euler.f90:630-632, 912-914
+  ghost_flux(A.ph, w, A.ps[ng], T, rdiv, !NS, s0, s1, s2, f);
+}
+
+template <bool NS, bool LIM>
+__device__ __forceinline__ void fused_body(const FusedArgs &A, dim3 t) {
+  const Blk &b = A.b;
+  __shared__ double P[5][FPP * FPP];
+  __shared__ double Q[NS ? 9 : 1][NS ? FQP * FQP : 1];
+  __shared__ double Fi[5][(FT + 1) * FT];
+  __shared__ double Fj[5][FT * (FT + 1)];
+  const int tid = threadIdx.x, tx = tid % FT, ty = tid / FT;
+  const int i0 = b.i1 + (int)t.x * FT, j0 = b.j1 + (int)t.y * FT;
+  const int kb = b.k1 + (int)t.z * FS, ke = min(kb + FS, b.k2);
+  const int njd = (int)(b.nijd / b.nid), nkd = (int)(b.nnn / b.nijd);
+  const int id2 = b.id1 + (int)b.nid - 1, jd2 = b.jd1 + njd - 1, kd2 = b.kd1 + nkd - 1;
+  auto cl = [](int x, int lo, int hi) { return min(max(x, lo), hi); };
+  // own column (clamped into the block's storage: cells past the real ones
+  // are the ghost layer, or beyond it, whose values no needed face uses)
+  const int i = i0 + tx, j = j0 + ty;
+  const bool cell = i < b.i2 && j < b.j2;
+  const long col = b.idx(cl(i, b.id1, id2), cl(j, b.jd1, jd2), b.kd1);  // entry at k = kd1
+  auto at = [&](int k) { return col + (long)(cl(k, b.kd1, kd2) - b.kd1) * b.nijd; };
+  // halo element of this thread: prims (tid < 128) or stresses (128..191)
+  int hii = 0, hjj = 0, hkind = 0;  // kind 1: prim, 2: stress
+  if (tid < 64) {
+    int side = tid / 32, r = tid % 32;
+    hjj = r / 2; hii = side ? FT + (r % 2) : -2 + (r % 2); hkind = 1;
+  } else if (tid < 128) {
+    int h = tid - 64, side = h / 32, r = h % 32;
+    hii = r % FT; hjj = side ? FT + r / FT : -2 + r / FT; hkind = 1;
+  } else if (NS && tid < 192) {
+    int g = tid - 128;
+    if (g < 32) { hjj = g % FT; hii = (g / FT) ? FT : -1; }
+    else { g -= 32; hii = g % FT; hjj = (g / FT) ? FT : -1; }
+    hkind = 2;
+  }
+  const long hcol = b.idx(cl(i0 + hii, b.id1, id2), cl(j0 + hjj, b.jd1, jd2), b.kd1);
+  const int so = (ty + 2) * FPP + tx + 2, sq = (ty + 1) * FQP + tx + 1;
+  const double z9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // own column registers: c0 = q(k-1), cq = q(k), c2 = q(k+1); Hlo = the
+  // low k face of plane k
+  double c0[5], cq[5], c2[5], Hlo[5];
+  fused_prim(A, at(kb - 1), c0);
+  fused_prim(A, at(kb), cq);
+  fused_prim(A, at(kb + 1), c2);
+  // the low face of the slab's first plane
+  if (cell) {
+    const long n = b.idx(i, j, kb), m = n - b.npc;
+    double s0 = snv(A.sn, b.nnn, m, 2, 0), s1 = snv(A.sn, b.nnn, m, 2, 1), s2 = snv(A.sn, b.nnn, m, 2, 2);
+    if (kb == b.k1) {
+      fused_ghost<NS>(A, 2, true, n, b.nijd, s0, s1, s2, Hlo);
+    } else {
+      double q[4][5], rl[5], rh[5], tl[9], tr[9];
+      fused_prim(A, at(kb - 2), q[0]);
+#pragma unroll
+      for (int e = 0; e < 5; e++) { q[1][e] = c0[e]; q[2][e] = cq[e]; q[3][e] = c2[e]; }
+      const long nl = at(kb - 1), nr = at(kb);
+#pragma unroll
+      for (int x = 0; x < 9; x++) {
+        tl[x] = NS ? A.tq[x * A.ldv + nl] : 0.;
+        tr[x] = NS ? A.tq[x * A.ldv + nr] : 0.;
+      }
+      const bool lim = LIM && kb <= b.k2 - 2;
+#pragma unroll
+      for (int e = 0; e < 5; e++) {
+        rl[e] = lim ? A.rr[(10 + e) * b.nnn + m] : 0.;
+        rh[e] = lim ? A.rr[(10 + e) * b.nnn + m + b.nijd] : 0.;
+      }
+      muscl_roe_half(A.ph, s0, s1, s2, q, lim, rl, rh, A.cm, A.xk, tl, tr, Hlo);
+    }
+  }
+  for (int k = kb; k < ke; k++) {
+    // plane k into LDS: own cell, own stresses, halo element
+    {
+      const long nk = at(k);
+#pragma unroll
+      for (int e = 0; e < 5; e++) P[e][so] = cq[e];
+      if (NS) {
+#pragma unroll
+        for (int x = 0; x < 9; x++) Q[x][sq] = A.tq[x * A.ldv + nk];
+      }
+      const long nh = hcol + (long)(cl(k, b.kd1, kd2) - b.kd1) * b.nijd;
+      if (hkind == 1) {
+        double hv[5];
+        fused_prim(A, nh, hv);
+        const int sh = (hjj + 2) * FPP + hii + 2;
+#pragma unroll
+        for (int e = 0; e < 5; e++) P[e][sh] = hv[e];
+      } else if (NS && hkind == 2) {
+        const int sh = (hjj + 1) * FQP + hii + 1;
+#pragma unroll
+        for (int x = 0; x < 9; x++) Q[x][sh] = A.tq[x * A.ldv + nh];
+      }
+    }
+    __syncthreads();
+    // i and j faces of plane k; the extra (FT+1-th) faces by half of wave
+    // (k - kb) % 4
+    const int wave = tid / 64, lane = tid % 64;
+    const int njob = (wave == (k - kb) % 4 && lane < 2 * FT) ? 3 : 2;
+    for (int job = 0; job < njob; job++) {
+      int d, fx, fy;
+      if (job == 0) { d = 0; fx = tx; fy = ty; }
+      else if (job == 1) { d = 1; fx = tx; fy = ty; }
+      else if (lane < FT) { d = 0; fx = FT; fy = lane; }
+      else { d = 1; fx = lane - FT; fy = FT; }
+      const int fi = i0 + fx, fj = j0 + fy;
+      const bool need = d == 0 ? (fi <= b.i2 && fj < b.j2) : (fi < b.i2 && fj <= b.j2);
+      if (!need) continue;
+      const int a = d == 0 ? fi : fj, a1 = d == 0 ? b.i1 : b.j1, a2 = d == 0 ? b.i2 : b.j2;
+      const long inc = d == 0 ? 1 : b.nid;
+      const long n = b.idx(fi, fj, k), m = n - b.npc;
+      double s0 = snv(A.sn, b.nnn, m, d, 0), s1 = snv(A.sn, b.nnn, m, d, 1), s2 = snv(A.sn, b.nnn, m, d, 2);
+      double f[5];
+      if (a == a1 || a == a2) {
+        fused_ghost<NS>(A, d, a == a1, n, inc, s0, s1, s2, f);
+      } else {
+        double q[4][5], rl[5], rh[5], tl[9], tr[9];
+        const int dx = d == 0, dy = d == 1;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          const int sl = (fy + (s - 2) * dy + 2) * FPP + fx + (s - 2) * dx + 2;
+#pragma unroll
+          for (int e = 0; e < 5; e++) q[s][e] = P[e][sl];
+        }
+        const int ql = (fy - dy + 1) * FQP + fx - dx + 1, qr = (fy + 1) * FQP + fx + 1;
+#pragma unroll
+        for (int x = 0; x < 9; x++) {
+          tl[x] = NS ? Q[x][ql] : 0.;
+          tr[x] = NS ? Q[x][qr] : 0.;
+        }
+        const bool lim = LIM && a <= a2 - 2;
+#pragma unroll
+        for (int e = 0; e < 5; e++) {
+          rl[e] = lim ? A.rr[(5 * d + e) * b.nnn + m] : 0.;
+          rh[e] = lim ? A.rr[(5 * d + e) * b.nnn + m + inc] : 0.;
+        }
+        muscl_roe_half(A.ph, s0, s1, s2, q, lim, rl, rh, A.cm, A.xk, tl, tr, f);
+      }
+      if (d == 0) {
+#pragma unroll
+        for (int e = 0; e < 5; e++) Fi[e][fy * (FT + 1) + fx] = f[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 5; e++) Fj[e][fy * FT + fx] = f[e];
+      }
+    }
+    // the k face between planes k and k+1 of the own column
+    double Hhi[5], c3[5];
+    fused_prim(A, at(k + 2), c3);
+    if (cell) {
+      const long n = b.idx(i, j, k + 1), m = n - b.npc;
+      double s0 = snv(A.sn, b.nnn, m, 2, 0), s1 = snv(A.sn, b.nnn, m, 2, 1), s2 = snv(A.sn, b.nnn, m, 2, 2);
+      if (k + 1 == b.k2) {
+        fused_ghost<NS>(A, 2, false, n, b.nijd, s0, s1, s2, Hhi);
+      } else {
+        double q[4][5], rl[5], rh[5], tl[9], tr[9];
+#pragma unroll
+        for (int e = 0; e < 5; e++) { q[0][e] = c0[e]; q[1][e] = cq[e]; q[2][e] = c2[e]; q[3][e] = c3[e]; }
+        const long nr = at(k + 1);
+#pragma unroll
+        for (int x = 0; x < 9; x++) {
+          tl[x] = NS ? Q[x][sq] : 0.;
+          tr[x] = NS ? A.tq[x * A.ldv + nr] : 0.;
+        }
+        const bool lim = LIM && k + 1 <= b.k2 - 2;
+#pragma unroll
+        for (int e = 0; e < 5; e++) {
+          rl[e] = lim ? A.rr[(10 + e) * b.nnn + m] : 0.;
+          rh[e] = lim ? A.rr[(10 + e) * b.nnn + m + b.nijd] : 0.;
+        }
+        muscl_roe_half(A.ph, s0, s1, s2, q, lim, rl, rh, A.cm, A.xk, tl, tr, Hhi);
+      }
+    }
+    __syncthreads();
+    if (cell) {
+      const long n = b.idx(i, j, k);
+#pragma unroll
+      for (int e = 0; e < 5; e++) {
+        double r = Fi[e][ty * (FT + 1) + tx + 1] - Fi[e][ty * (FT + 1) + tx];
+        r = r + (Fj[e][(ty + 1) * FT + tx] - Fj[e][ty * FT + tx]);
+        r = r + (Hhi[e] - Hlo[e]);
+        A.u[e * A.ldv + n] = r;
+        if (A.ityprk) A.ff[e * A.ldv + n] = A.ff[e * A.ldv + n] - r;
+        Hlo[e] = Hhi[e];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 5; e++) { c0[e] = cq[e]; cq[e] = c2[e]; c2[e] = c3[e]; }
+  }
+}
+
+template <bool NS, bool LIM>
+__global__ void __launch_bounds__(FT *FT) __attribute__((amdgpu_waves_per_eu(2))) k_residual_fused(FusedArgs A) {
+  fused_body<NS, LIM>(A, xcd_block());
+}
+
+// u = 0 on the entries of indc(i1-1,j1-1,k1-1)..indc(i2,j2,k2) that are not
+// real cells (sch_roe.f90:110-125; the fused kernel writes the real ones)
+__device__ __forceinline__ void zero_ghost_body(const Blk &b, unsigned bx, long n0, long n1, double *u, long ldv) {
+  long n = n0 + bx * 256L + threadIdx.x;
+  if (n > n1) return;
+  long m = n - b.npc;
+  int i = b.id1 + (int)(m % b.nid);
+  int j = b.jd1 + (int)((m / b.nid) % (b.nijd / b.nid));
+  int k = b.kd1 + (int)(m / b.nijd);
+  if (b.real(i, j, k)) return;
+#pragma unroll
+  for (int e = 0; e < 5; e++) u[e * ldv + n] = 0.;
+}
+
+__global__ void __launch_bounds__(256) k_zero_ghost(Blk b, long n0, long n1, double *u, long ldv) {
+  zero_ghost_body(b, blockIdx.x, n0, n1, u, ldv);
+}
+
+inline dim3 fused_grid(const Blk &b) {
+  int nx = b.i2 - b.i1, ny = b.j2 - b.j1, nz = b.k2 - b.k1;
+  return dim3((nx + FT - 1) / FT, (ny + FT - 1) / FT, (nz + FS - 1) / FS);
+}
+
+// Residual path: 1 (default) the fused kernel, 0 the face passes + gather.
+static int g_residual_fused = 1;
+
+extern "C" int codens_residual_path(int32_t fused, int32_t *previous) {
+  if (fused != 0 && fused != 1) {
+    codens_set_error("codens_residual_path: 0 (face passes + gather) or 1 (fused), not %d", fused);
+    return CODENS_EINVAL;
+  }
+  if (previous) *previous = g_residual_fused;
+  g_residual_fused = fused;
+  return CODENS_OK;
+}
+
 static int roe_common(const codens_block *cb, const codens_phys *cp, const codens_scheme *sc, int ns,
                       int ityprk, double *u, const double *v, long ldv, double *ff, const double *tq,
                       const double *sn, const double *ps, hipStream_t st, const char *what) {
@@ -481,14 +793,16 @@ static int roe_common(const codens_block *cb, const codens_phys *cp, const coden
   Blk b = make_blk(cb);
   Phys ph = make_phys(cp);
   int ilim = ns ? sc->ilim : 0;
-  double *H = (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 0);
+  const bool fused = g_residual_fused == 1;
+  double *H = fused ? nullptr : (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 0);
   double *R = ilim == 1 ? (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 1) : nullptr;
-  double *uvw = (double *)codens_workspace(sizeof(double) * 3 * b.nnn, 4);
-  if (!H || !uvw || (ilim == 1 && !R)) {
+  const bool need_uvw = !fused || ilim == 1;  // the chains read u, v, w planes
+  double *uvw = need_uvw ? (double *)codens_workspace(sizeof(double) * 3 * b.nnn, 4) : nullptr;
+  if ((!fused && !H) || (need_uvw && !uvw) || (ilim == 1 && !R)) {
     codens_set_error("%s: workspace allocation failed", what);
     return CODENS_EHIP;
   }
-  {
+  if (need_uvw) {
     CodensTimed _tm("roe_prim", st);
     hipLaunchKernelGGL(k_prim_uvw, dim3((unsigned)((b.nnn + 255) / 256)), dim3(256), 0, st, b, v, ldv, uvw);
   }
@@ -501,6 +815,20 @@ static int roe_common(const codens_block *cb, const codens_phys *cp, const coden
     hipLaunchKernelGGL(k_limiter_chain_jk, dim3((unsigned)((nl12 + CH_LINES - 1) / CH_LINES), 5, 2), dim3(64), 0, st,
                        b, v, uvw, ps, R);
   }
+  long n0 = b.idx(b.i1 - 1, b.j1 - 1, b.k1 - 1), n1 = b.idx(b.i2, b.j2, b.k2);
+  if (fused) {
+    FusedArgs A{b, ph, 0.25 * sc->muscl, sc->xk, v, ldv, ps, ns ? tq : v, sn, R, u, ff, ityprk};
+    {
+      CodensTimed _tm("roe_fused", st);
+      dim3 g = fused_grid(b);
+      if (ns && ilim == 1) hipLaunchKernelGGL((k_residual_fused<true, true>), g, dim3(FT * FT), 0, st, A);
+      else if (ns) hipLaunchKernelGGL((k_residual_fused<true, false>), g, dim3(FT * FT), 0, st, A);
+      else hipLaunchKernelGGL((k_residual_fused<false, false>), g, dim3(FT * FT), 0, st, A);
+    }
+    CodensTimed _tm("roe_zero_ghost", st);
+    hipLaunchKernelGGL(k_zero_ghost, dim3((unsigned)((n1 - n0 + 256) / 256)), dim3(256), 0, st, b, n0, n1, u, ldv);
+    return codens_check_launch(what);
+  }
   for (int d = 0; d < 3; d++) {
     int nx = b.i2 - b.i1 + (d == 0), ny = b.j2 - b.j1 + (d == 1), nz = b.k2 - b.k1 + (d == 2);
     dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz);
@@ -508,7 +836,6 @@ static int roe_common(const codens_block *cb, const codens_phys *cp, const coden
     hipLaunchKernelGGL(k_face_flux, g, dim3(TX, TY), 0, st, b, ph, d, ns, ilim, (double)sc->muscl, sc->xk, v, ldv,
                        ps, ns ? tq : v, sn, uvw, ilim == 1 ? R + 5 * b.nnn * d : nullptr, H + 5 * b.nnn * d);
   }
-  long n0 = b.idx(b.i1 - 1, b.j1 - 1, b.k1 - 1), n1 = b.idx(b.i2, b.j2, b.k2);
   {
     CodensTimed _tm("roe_gather", st);
     hipLaunchKernelGGL(k_gather, dim3((unsigned)((n1 - n0 + 256) / 256)), dim3(256), 0, st, b, n0, n1, H, u, ldv,
@@ -540,7 +867,7 @@ extern "C" int codens_sch_roe(const codens_block *b, const codens_phys *ph, cons
 // first-workgroup column and runs the pass's body on the block's tile.
 // The passes' workspaces are the per-block ones laid end to end: block l's
 // part starts at its npc (times the planes).
-enum { BP_PRIM, BP_CHAIN_I, BP_CHAIN_JK, BP_FACE0, BP_FACE1, BP_FACE2, BP_GATHER, BP_N };
+enum { BP_PRIM, BP_CHAIN_I, BP_CHAIN_JK, BP_FACE0, BP_FACE1, BP_FACE2, BP_GATHER, BP_FUSED, BP_N };
 
 struct BatchEnt {
   Blk b;
@@ -582,6 +909,27 @@ k_face_flux_batch(const BatchEnt *__restrict__ tab, int nb, Phys ph, int d, int 
                  uvw + 3 * np, ilim == 1 ? rr + 15 * np + 5 * E.b.nnn * d : nullptr, H + 15 * np + 5 * E.b.nnn * d);
 }
 
+// the fused residual of every block: workgroup -> block (bisection) -> tile
+template <bool NS, bool LIM>
+__global__ void __launch_bounds__(FT *FT) __attribute__((amdgpu_waves_per_eu(2)))
+k_residual_fused_batch(const BatchEnt *__restrict__ tab, int nb, Phys ph, double cm, double xk,
+                       const double *__restrict__ v, long ldv, const double *__restrict__ ps,
+                       const double *__restrict__ tq, const double *__restrict__ sn_all,
+                       const double *__restrict__ rr, double *u, double *ff, int ityprk) {
+  const unsigned wg = xcd_linear();
+  const BatchEnt &E = tab[tab_find(tab, nb, BP_FUSED, wg)];
+  const long np = E.b.npc;
+  FusedArgs A{E.b, ph, cm, xk, v, ldv, ps, tq, sn_all + 9 * np, LIM ? rr + 15 * np : nullptr, u, ff, ityprk};
+  fused_body<NS, LIM>(A, tab_tile(E, BP_FUSED, wg));
+}
+
+// u = 0 on the non-real entries of every block's gather range
+__global__ void __launch_bounds__(256) k_zero_ghost_batch(const BatchEnt *__restrict__ tab, int nb, double *u,
+                                                          long ldv) {
+  const BatchEnt &E = tab[tab_find(tab, nb, BP_GATHER, blockIdx.x)];
+  zero_ghost_body(E.b, blockIdx.x - E.start[BP_GATHER], E.n0, E.n1, u, ldv);
+}
+
 __global__ void __launch_bounds__(256) k_gather_batch(const BatchEnt *__restrict__ tab, int nb,
                                                       const double *__restrict__ H, double *u, long ldv, double *ff,
                                                       int ityprk) {
@@ -621,7 +969,12 @@ extern "C" int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, co
                            {(unsigned)((nl0 + CH_LINES - 1) / CH_LINES), 5, 1},
                            {(unsigned)((nl12 + CH_LINES - 1) / CH_LINES), 5, 2},
                            {0, 0, 0}, {0, 0, 0}, {0, 0, 0},
-                           {(unsigned)((E.n1 - E.n0 + 256) / 256), 1, 1}};
+                           {(unsigned)((E.n1 - E.n0 + 256) / 256), 1, 1},
+                           {0, 0, 0}};
+    {
+      dim3 fg = fused_grid(b);
+      g[BP_FUSED][0] = fg.x; g[BP_FUSED][1] = fg.y; g[BP_FUSED][2] = fg.z;
+    }
     for (int d = 0; d < 3; d++) {
       int nx = b.i2 - b.i1 + (d == 0), ny = b.j2 - b.j1 + (d == 1), nz = b.k2 - b.k1 + (d == 2);
       g[BP_FACE0 + d][0] = (nx + TX - 1) / TX;
@@ -647,14 +1000,16 @@ extern "C" int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, co
   if (!tab) return CODENS_EHIP;
   Phys ph = make_phys(cp);
   const long tot_n = span[nblk - 1].second;
-  double *H = (double *)codens_workspace(sizeof(double) * 15 * tot_n, 0);
+  const bool fused = g_residual_fused == 1;
+  const bool need_uvw = !fused || ilim == 1;
+  double *H = fused ? nullptr : (double *)codens_workspace(sizeof(double) * 15 * tot_n, 0);
   double *R = ilim == 1 ? (double *)codens_workspace(sizeof(double) * 15 * tot_n, 1) : nullptr;
-  double *uvw = (double *)codens_workspace(sizeof(double) * 3 * tot_n, 4);
-  if (!H || !uvw || (ilim == 1 && !R)) {
+  double *uvw = need_uvw ? (double *)codens_workspace(sizeof(double) * 3 * tot_n, 4) : nullptr;
+  if ((!fused && !H) || (need_uvw && !uvw) || (ilim == 1 && !R)) {
     codens_set_error("%s: workspace allocation failed", what);
     return CODENS_EHIP;
   }
-  {
+  if (need_uvw) {
     CodensTimed _tm("roe_prim", st);
     hipLaunchKernelGGL(k_prim_uvw_batch, dim3(tot[BP_PRIM]), dim3(256), 0, st, (const BatchEnt *)tab, (int)nblk, v,
                        (long)ldv, uvw);
@@ -665,6 +1020,24 @@ extern "C" int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, co
                        (const double *)uvw, ps, R);
     hipLaunchKernelGGL(k_chain_jk_batch, dim3(tot[BP_CHAIN_JK]), dim3(64), 0, st, (const BatchEnt *)tab, (int)nblk,
                        v, (const double *)uvw, ps, R);
+  }
+  if (fused) {
+    {
+      CodensTimed _tm("roe_fused", st);
+      const double cm = 0.25 * sc->muscl;
+      if (ilim == 1)
+        hipLaunchKernelGGL((k_residual_fused_batch<true, true>), dim3(tot[BP_FUSED]), dim3(FT * FT), 0, st,
+                           (const BatchEnt *)tab, (int)nblk, ph, cm, sc->xk, v, (long)ldv, ps, tq, sn_all,
+                           (const double *)R, u, ff, (int)ityprk);
+      else
+        hipLaunchKernelGGL((k_residual_fused_batch<true, false>), dim3(tot[BP_FUSED]), dim3(FT * FT), 0, st,
+                           (const BatchEnt *)tab, (int)nblk, ph, cm, sc->xk, v, (long)ldv, ps, tq, sn_all,
+                           (const double *)R, u, ff, (int)ityprk);
+    }
+    CodensTimed _tm("roe_zero_ghost", st);
+    hipLaunchKernelGGL(k_zero_ghost_batch, dim3(tot[BP_GATHER]), dim3(256), 0, st, (const BatchEnt *)tab, (int)nblk,
+                       u, (long)ldv);
+    return codens_check_launch(what);
   }
   for (int d = 0; d < 3; d++) {
     CodensTimed _tm("roe_face_flux", st);

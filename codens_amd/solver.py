@@ -78,6 +78,12 @@ class Solver:
         t = case.turb
         if t is not None:
             self.dist = _dev(t.dist)
+            if t.kparoi == 1:
+                if t.wall_lines is None:
+                    raise ValueError("a k-omega case needs its wall points (met_parko, kparoi = 1)")
+                self.parko = (np.asarray(t.wall_start, dtype=np.int64),
+                              torch.as_tensor(t.wall_lines.view(np.uint8).copy(), device="cuda"),
+                              _dev(t.wall_ghost, torch.int64), _dev(t.mnpar, torch.int32))
             self.fd = f(6, n)
             self.cfke = f(n); self.ts6 = f(n); self.ts7 = f(n)
             self.g9 = [f(9, b.nnn) for b in lay.blocks]
@@ -190,6 +196,22 @@ class Solver:
         viscosity reads that block's state only)."""
         t = self.case.turb
         sa = t.model == "1SA"
+        s = self.stream
+        # met_num.f90:145-159: the turbulence ghosts of every frontier
+        # (met_bord: coincident averages first), then the omega wall
+        # condition of the k-omega models (met_parko)
+        g, i, r = self.coin
+        if g.numel():
+            abi.met_bord(0, g, i, r, 0.0, 0.0, self.v, stream=s)
+        if self.wall[0].numel():
+            abi.met_bord(1 + t.kparoi, self.wall[0], self.wall[1], None, 0.0, 0.0, self.v, stream=s)
+        if self.far[0].numel():
+            abi.met_bord(3, self.far[0], self.far[1], None, t.rokinf, t.roeinf, self.v, stream=s)
+        if self.sym[0].numel():
+            abi.met_bord(4, self.sym[0], self.sym[1], None, 0.0, 0.0, self.v, stream=s)
+        if t.kparoi == 1:
+            fs, lines, ghost, mnpar = self.parko
+            abi.met_parko(fs, lines, ghost, mnpar, self.mu, self.dist, 0.075, self.v, stream=s)
 
         def sources(l, b, s):
             abi.teq_gradv(b, self.sn_blk[l], self.vol, self.v, self.g9[l], stream=s)
@@ -388,3 +410,34 @@ class Solver:
         if check:
             self.check()
         return self.v.cpu().numpy()
+
+    def _restart_args(self):
+        ns = self.case.ns
+        nturb = 2 if self.case.turb is not None else 0  # equat(6:7) = 'ke' or a 7-variable RANS model
+        return ns, nturb
+
+    def save_restart(self, path):
+        """Writes the restart file of writda.f90 (as svfw.f90:58 writes it):
+        per block rho, rho u, rho v, rho w, rho E, then mut (Navier-Stokes)
+        and the two turbulence variables (with a turbulence model), over the
+        real cells.  A copy of the state to the host, then host file I/O."""
+        ns, nturb = self._restart_args()
+        v = np.ascontiguousarray(self.host_state())
+        mut = np.ascontiguousarray(self.mut.cpu().numpy()) if ns else None
+        abi.restart_write(path, self.blocks_arr, len(self.blocks), v, mut, nturb)
+
+    def load_restart(self, path):
+        """Reads a writda.f90 restart (readda.f90:4) into the real cells of
+        the state (and mut); ghost entries keep their values and the next
+        stage's boundary routines set them.  Must not be called while a
+        captured graph is in use with other pointers (the arrays are updated
+        in place, so a captured graph stays valid)."""
+        ns, nturb = self._restart_args()
+        torch.cuda.synchronize()
+        v = np.ascontiguousarray(self.v.cpu().numpy())
+        mut = np.ascontiguousarray(self.mut.cpu().numpy()) if ns else None
+        abi.restart_read(path, self.blocks_arr, len(self.blocks), v, mut, nturb)
+        self.v.copy_(torch.as_tensor(v))
+        if ns:
+            self.mut.copy_(torch.as_tensor(mut))
+        torch.cuda.synchronize()

@@ -65,7 +65,8 @@ enum {
   CODENS_EINVAL = 1,     /* bad extents, null pointer, size over a limit */
   CODENS_EHIP = 2,       /* a HIP call or launch failed */
   CODENS_ENONPHYS = 3,   /* non-physical state (T <= 0), where the reference aborts */
-  CODENS_ESTOP = 4       /* global time step below dt1min, where the reference stops */
+  CODENS_ESTOP = 4,      /* global time step below dt1min, where the reference stops */
+  CODENS_EIO = 5         /* a file could not be opened, written or read as expected */
 };
 
 const char *codens_last_error(void);
@@ -433,6 +434,35 @@ int codens_host_bundle_fetch(const void *const *hosts, int32_t count, int64_t by
  * (they are distinct cell columns).  Cells a line would reach outside
  * [0, nfrac) are skipped. */
 
+/* met_bord.f90:4 (with met_rbvc.f90:30-65 and met_rbve.f90:26-36) -- the
+ * ghost values of the two transported turbulence variables (v planes 5 and
+ * 6, 0-based) on one frontier list (ghost g, adjacent real cell in, and for
+ * coincident frontiers the real cell across the interface rm), by kind:
+ *   0 coincident ('rc'): 0.5 (in + rm), the met_rbvc average;
+ *   1 wall ('para') with kparoi = 0: 0 and 0;
+ *   2 wall ('para') with kparoi = 1: 0 and the adjacent cell's v7;
+ *   3 far field ('nrd'): rokinf and roeinf;
+ *   4 symmetry ('sym'): the adjacent cell's values.
+ * met_num.f90:145-151 calls it first in the turbulence stage: coincident
+ * frontiers, then every frontier in listing order.  in may be NULL for
+ * kinds 1 and 3, rm for kinds other than 0. */
+int codens_met_bord(int32_t kind, int64_t nf, const int64_t *g, const int64_t *in, const int64_t *rm,
+                    double rokinf, double roeinf, double *v, int64_t ldv, void *stream);
+
+/* met_parko.f90:4 -- omega wall condition of the k-omega models (kparoi = 1,
+ * met_num.f90:153-159): per wall point (lines[face_start[w]..face_start[w+1])
+ * of wall w, as for codens_met_mtcorf1; ghost[p] the point's ghost cell),
+ * rho omega = 6 mu / (beta1 d^2) in the wall cell; in the next min(10,
+ * m3tot) cells of its line whose wall point (mnpar, 1-based) lies in the
+ * wall's window, the value blended with the current one over cells 2..4
+ * (pond = (d - d_2) / (d_4 - d_2)) and floored by 6 mu / (beta1 d^2); the
+ * ghost takes the wall cell's value.  beta1 = 0.075 for '2KO'.  face_start
+ * is host memory; walls run in order.  The reference stops when a line has
+ * fewer than 10 cells (met_parko.f90:45-51); the caller checks that. */
+int codens_met_parko(int32_t nfaces, const int64_t *face_start, const codens_wall_line *lines, const int64_t *ghost,
+                     const int32_t *mnpar, const double *mu, const double *dist, double beta1, double *v, int64_t ldv,
+                     void *stream);
+
 /* met_mtcorf1.f90:4 -- wall correction of the blending function frac. */
 int codens_met_mtcorf1(int32_t nfaces, const int64_t *face_start, const codens_wall_line *lines, int32_t imxclko,
                        const int32_t *mnpar, double *frac, int64_t nfrac, void *stream);
@@ -533,11 +563,22 @@ int codens_workspace_set(int32_t set);
 
 /* Tile shape of the tiled Jacobi sub-iteration of codens_implimf_3d,
  * codens_implimf_eu and codens_implimf_3d_f32 (no counterpart in the
- * reference): -1 (default) times the 64 x 4 and 32 x 8 shapes on the first
- * call for each block shape and keeps the faster, 0 forces 64 x 4, 1 forces
- * 32 x 8.  Both give identical results.  *previous (may be NULL) receives
- * the mode in force before the call. */
+ * reference): -1 (default) times the 64 x 4, 32 x 8 and 16 x 16 shapes on
+ * the first call for each block shape and keeps the fastest, 0 forces
+ * 64 x 4, 1 forces 32 x 8, 2 forces 16 x 16 (the all-blocks entry points
+ * hold the first two: there a forced 16 x 16 takes the lane-and-halo model's
+ * choice between them).  All give identical results.  *previous (may be
+ * NULL) receives the mode in force before the call. */
 int codens_sweep_tile(int32_t mode, int32_t *previous);
+
+/* Kernel arrangement of codens_sch_roe and codens_sch_roe_euler (no
+ * counterpart in the reference): 1 (default) one fused kernel that stages
+ * an i-j tile of the state in LDS, marches k, evaluates every face once and
+ * accumulates the residual of its cells; 0 one face-flux pass per direction
+ * into a face array, then a gather.  Both evaluate the same face values and
+ * sum them in the same order.  *previous (may be NULL) receives the
+ * arrangement in force before the call. */
+int codens_residual_path(int32_t fused, int32_t *previous);
 
 /* The tile shape used for a block (eu: implimf_eu, fp32: the FP32 form):
  * 0 or 1, or -1 if not chosen yet. */
@@ -582,6 +623,40 @@ int codens_adj_residual_vjp(const codens_block *b, const codens_phys *ph, const 
 int codens_adj_metric_vjp(const codens_block *b, const double *x, const double *y, const double *z,
                           const double *sg, const double *snbar, const double *volbar, double *xbar, double *ybar,
                           double *zbar, void *stream);
+
+/* ---- On-disk formats of the reference.  Host memory and file I/O only, no
+ * GPU work: a device-resident driver copies its arrays out before writing
+ * and in after reading.  Files are Fortran unformatted sequential (each
+ * record: its byte length as a 4-byte integer, the data, the length again,
+ * little endian), one record per variable and block, i fastest, then j, k.
+ * A record is limited to 2 GiB (CODENS_EINVAL beyond). */
+
+/* writda.f90:4 (restart, written by svfw.f90:58 after chdacc 'cccc'): for
+ * each block in order, records over its real cells ii1..ii2-1, jj1..jj2-1,
+ * kk1..kk2-1: rho, rho u, rho v, rho w, rho E (v planes 0..4), then mut when
+ * ns != 0 (equat(1:2) = 'ns'), then v planes 5 and 6 when nturb = 2
+ * (equat(6:7) = 'ke', or equat(2:4) = 'res' with ip60 = 7).  The file is
+ * created or truncated (the reference rewinds for its first block). */
+int codens_restart_write(const char *path, const codens_block *blocks, int32_t nb, int32_t ns, int32_t nturb,
+                         const double *v, int64_t ldv, const double *mut);
+
+/* readda.f90:4 (read by infw.f90:69): the records of codens_restart_write
+ * into the real cells of v (and mut); every other entry is left as it was.
+ * CODENS_EIO when the file ends early or a record's length does not match
+ * its block. */
+int codens_restart_read(const char *path, const codens_block *blocks, int32_t nb, int32_t ns, int32_t nturb,
+                        double *v, int64_t ldv, double *mut);
+
+/* writdg.f90:4: for each block, records x, y, z over its nodes ii1..ii2,
+ * jj1..jj2, kk1..kk2; node (i, j, k) of block l at the block's storage entry
+ * of (i, j, k) (npn = npc), arrays of ld entries. */
+int codens_mesh_write(const char *path, const codens_block *blocks, int32_t nb, int64_t ld, const double *x,
+                      const double *y, const double *z);
+
+/* readdg.f90:4: the records of codens_mesh_write into the nodes of x, y, z;
+ * other entries untouched. */
+int codens_mesh_read(const char *path, const codens_block *blocks, int32_t nb, int64_t ld, double *x, double *y,
+                     double *z);
 
 #ifdef __cplusplus
 }

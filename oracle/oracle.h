@@ -10,6 +10,29 @@
  */
 #ifndef CODENS_ORACLE_H
 #define CODENS_ORACLE_H
+
+/* A loop over the faces of direction d (index ranges lo..hi) whose body
+ * writes the cells on both sides of each face, or runs a recurrence along
+ * d: the OpenMP threads share a direction other than d (k for d = 0, 1; j
+ * for d = 2), so no two threads touch one cell or one grid line, and each
+ * cell still receives its faces in increasing order along d, as in the
+ * serial loop: the result is the serial one bit for bit, for any thread
+ * count.  Declares i, j, k for the body. */
+/* calloc(n, 8) with the zeroing shared among the OpenMP threads (the pages
+ * are first touched by the threads that use them) */
+#include <stdlib.h>
+static inline double *orc_zeros(long n) {
+  double *p = (double *)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (long q = 0; q < n; q++) p[q] = 0.;
+  return p;
+}
+
+#define FOR3D(lo, hi, d) \
+  _Pragma("omp parallel for schedule(static)") \
+  for (int o_ = (d) == 2 ? (lo)[1] : (lo)[2]; o_ <= ((d) == 2 ? (hi)[1] : (hi)[2]); o_++) \
+    for (int p_ = (d) == 2 ? (lo)[2] : (lo)[1]; p_ <= ((d) == 2 ? (hi)[2] : (hi)[1]); p_++) \
+      for (int i = (lo)[0], j = (d) == 2 ? o_ : p_, k = (d) == 2 ? p_ : o_; i <= (hi)[0]; i++)
 #include <stdint.h>
 
 typedef struct {

@@ -38,13 +38,20 @@ static inline int d2of(const orc_block *b, int d) { return d == 0 ? b->ii2 : (d 
     for (int j = (lo)[1]; j <= (hi)[1]; j++) \
       for (int i = (lo)[0]; i <= (hi)[0]; i++)
 
+/* The same loop with the k planes shared among OpenMP threads: for bodies
+ * that write only their own cell (or face). */
+#define FOR3P(lo, hi) \
+  _Pragma("omp parallel for schedule(static)") \
+  FOR3(lo, hi)
+
+
 /* zpres.f90:66-79 */
 void orc_zpres(const orc_block *b, const orc_phys *ph, const double *v, long ldv,
                double *ps, double *temp, double *cson) {
   const double *v1 = v, *v2 = v + ldv, *v3 = v + 2 * ldv, *v4 = v + 3 * ldv, *v5 = v + 4 * ldv;
   int lo[3], hi[3];
   ranges(b, 0, b->ii1, b->ii2 - 1, lo, hi);
-  FOR3(lo, hi) {
+  FOR3P(lo, hi) {
     long n = IDX(b, i, j, k);
     double rhoe = v5[n] - 0.5 * (v2[n] * v2[n] + v3[n] * v3[n] + v4[n] * v4[n]) / v1[n];
     ps[n] = ph->gam1 * (rhoe - ph->pinfl);
@@ -60,6 +67,7 @@ int orc_zvismo(const orc_block *b, const orc_phys *ph, const double *temp, doubl
   double a = 110.4 / ph->tnz, usrey = 1. / ph->reynz;
   int lo[3], hi[3];
   ranges(b, 0, b->ii1, b->ii2 - 1, lo, hi);
+  _Pragma("omp parallel for schedule(static) reduction(+ : bad)")
   FOR3(lo, hi) {
     long n = IDX(b, i, j, k);
     double t = temp[n];
@@ -76,9 +84,10 @@ void orc_chronos(const orc_block *b, const orc_phys *ph, double etal, int ns,
                  const double *sn, const double *vol, const double *cson, double *dt) {
   int i1 = b->ii1, i2 = b->ii2, j1 = b->jj1, j2 = b->jj2, k1 = b->kk1, k2 = b->kk2;
   long nnn = b->nnn, n0 = b->npc, nci = INC(b, 0), ncj = INC(b, 1), nck = INC(b, 2);
-  double *sfsi = calloc(nnn, sizeof(double)), *sfsj = calloc(nnn, sizeof(double));
-  double *sfsk = calloc(nnn, sizeof(double)), *dism = calloc(nnn, sizeof(double));
+  double *sfsi = orc_zeros(nnn), *sfsj = orc_zeros(nnn);
+  double *sfsk = orc_zeros(nnn), *dism = orc_zeros(nnn);
   for (long n = IDX(b, i1, j1, k1); n <= IDX(b, i2, j2, k2); n++) dt[n] = 0.;
+#pragma omp parallel for schedule(static)
   for (int k = k1; k <= k2 - 1; k++) {
     for (int j = j1; j <= j2 - 1; j++)
       for (int i = i1; i <= i2; i++) {
@@ -93,6 +102,7 @@ void orc_chronos(const orc_block *b, const orc_phys *ph, double etal, int ns,
                   SN(sn, b, m, 1, 2) * SN(sn, b, m, 1, 2);
       }
   }
+#pragma omp parallel for schedule(static)
   for (int k = k1; k <= k2; k++)
     for (int j = j1; j <= j2 - 1; j++)
       for (int i = i1; i <= i2 - 1; i++) {
@@ -100,6 +110,7 @@ void orc_chronos(const orc_block *b, const orc_phys *ph, double etal, int ns,
         sfsk[m] = SN(sn, b, m, 2, 0) * SN(sn, b, m, 2, 0) + SN(sn, b, m, 2, 1) * SN(sn, b, m, 2, 1) +
                   SN(sn, b, m, 2, 2) * SN(sn, b, m, 2, 2);
       }
+#pragma omp parallel for schedule(static)
   for (int k = k1; k <= k2 - 1; k++)
     for (int j = j1; j <= j2 - 1; j++)
       for (int i = i1; i <= i2 - 1; i++) {
@@ -111,6 +122,7 @@ void orc_chronos(const orc_block *b, const orc_phys *ph, double etal, int ns,
         dism[mc] = vol[mc + n0] / smoy;
       }
   const double *t1 = v, *t2 = v + ldv, *t3 = v + 2 * ldv, *t4 = v + 3 * ldv;
+#pragma omp parallel for schedule(static)
   for (int k = k1; k <= k2 - 1; k++)
     for (int j = j1; j <= j2 - 1; j++)
       for (int i = i1; i <= i2 - 1; i++) {
@@ -122,6 +134,7 @@ void orc_chronos(const orc_block *b, const orc_phys *ph, double etal, int ns,
         dt[nc] = etal * dte;
       }
   if (ns) {
+#pragma omp parallel for schedule(static)
     for (int k = k1; k <= k2 - 1; k++)
       for (int j = j1; j <= j2 - 1; j++)
         for (int i = i1; i <= i2 - 1; i++) {
@@ -163,7 +176,7 @@ void orc_sch_expli(const orc_block *b, const double *u, double *v, long ldv, con
                    const double *vol) {
   int lo[3], hi[3];
   ranges(b, 0, b->ii1, b->ii2 - 1, lo, hi);
-  FOR3(lo, hi) {
+  FOR3P(lo, hi) {
     long n = IDX(b, i, j, k);
     double dtv = dt[n] / vol[n];
     for (int e = 0; e < 5; e++) v[e * ldv + n] = v[e * ldv + n] + dtv * u[e * ldv + n];
@@ -370,7 +383,7 @@ static void roe_residual(const orc_block *b, const orc_phys *ph, const orc_sch *
   for (long n = IDX(b, b->ii1 - 1, b->jj1 - 1, b->kk1 - 1); n <= IDX(b, b->ii2, b->jj2, b->kk2); n++)
     for (int e = 0; e < 5; e++) u[e * ldv + n] = 0.;
   double *r[5];
-  for (int e = 0; e < 5; e++) r[e] = calloc(nnn, sizeof(double));
+  for (int e = 0; e < 5; e++) r[e] = orc_zeros(nnn);
   for (int d = 0; d < 3; d++) {
     long inc = INC(b, d);
     int a1 = d1of(b, d), a2 = d2of(b, d);
@@ -378,7 +391,7 @@ static void roe_residual(const orc_block *b, const orc_phys *ph, const orc_sch *
     if (ilim == 1) {
       /* slope differences on faces a1..a2 (sch_roe.f90:146-158) */
       ranges(b, d, a1, a2, lo, hi);
-      FOR3(lo, hi) {
+      FOR3P(lo, hi) {
         long n = IDX(b, i, j, k), m = n - n0;
         r[0][m] = v[n] - v[n - inc];
         for (int e = 1; e < 4; e++) r[e][m] = v[e * ldv + n] / v[n] - v[e * ldv + n - inc] / v[n - inc];
@@ -387,7 +400,7 @@ static void roe_residual(const orc_block *b, const orc_phys *ph, const orc_sch *
       /* in-place ratios on faces a1+1..a2-1 in storage order: the value read
        * at m-inc has already been replaced by its own ratio (:161-194) */
       ranges(b, d, a1 + 1, a2 - 1, lo, hi);
-      FOR3(lo, hi) {
+      FOR3D(lo, hi, d) {
         long m = IDX(b, i, j, k) - n0;
         for (int e = 0; e < 5; e++) {
           if (fabs(r[e][m - inc] * r[e][m] * r[e][m + inc]) <= DBL_MIN) r[e][m] = 1.;
@@ -397,7 +410,7 @@ static void roe_residual(const orc_block *b, const orc_phys *ph, const orc_sch *
     }
     /* interior faces a1+1..a2-1 (:322-495) */
     ranges(b, d, a1 + 1, a2 - 1, lo, hi);
-    FOR3(lo, hi) {
+    FOR3D(lo, hi, d) {
       long n = IDX(b, i, j, k), m = n - n0;
       int along = d == 0 ? i : (d == 1 ? j : k);
       double L[5], R[5];

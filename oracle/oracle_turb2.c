@@ -216,6 +216,78 @@ void orc_met_rbsc(long nf, const long *ghost, const long *inner, const long *rem
   free(buf);
 }
 
+/* met_bord.f90:4 -- ghost values of the two transported variables (v
+ * planes 6, 7; 0-based 5, 6) on one frontier list, by its kind:
+ *   0 coincident ('rc'): 0.5 (inner + remote), met_rbvc.f90:30-65 (all
+ *     remote values read before any ghost is written);
+ *   1 wall ('para') with kparoi = 0: 0, 0 (met_bord.f90:118-125);
+ *   2 wall ('para') with kparoi = 1: 0, the inner cell's v7 (:127-134);
+ *   3 far field ('nrd'): rokinf = a, roeinf = b (met_rbve's copy is
+ *     overwritten, :204-214);
+ *   4 symmetry ('sym'): the inner cell's values (:228-236). */
+void orc_met_bord(int kind, long nf, const long *ghost, const long *inner, const long *remote, double a, double b,
+                  double *v, long ldv) {
+  double *k = v + 5 * ldv, *w = v + 6 * ldv;
+  if (kind == 0) {
+    double *buf = malloc(sizeof(double) * 2 * (nf > 0 ? nf : 1));
+    for (long f = 0; f < nf; f++) { buf[2 * f] = k[remote[f]]; buf[2 * f + 1] = w[remote[f]]; }
+    for (long f = 0; f < nf; f++) {
+      k[ghost[f]] = 0.5 * (k[inner[f]] + buf[2 * f]);
+      w[ghost[f]] = 0.5 * (w[inner[f]] + buf[2 * f + 1]);
+    }
+    free(buf);
+    return;
+  }
+  for (long f = 0; f < nf; f++) {
+    long g = ghost[f], i = inner[f];
+    switch (kind) {
+      case 1: k[g] = 0.; w[g] = 0.; break;
+      case 2: k[g] = 0.; w[g] = w[i]; break;
+      case 3: k[g] = a; w[g] = b; break;
+      default: k[g] = k[i]; w[g] = w[i]; break;
+    }
+  }
+}
+
+/* met_parko.f90:4 -- omega condition of the k-omega models at a wall (kparoi
+ * = 1): per wall point (line as met_mtcorf1's: first cell nmin, cells
+ * nmin + s isens3 dm3 for s = 0.. along the wall normal, wall point numbers
+ * (mpn0, mpn1] in mnpar), rho omega = 6 mu / (beta1 d^2) in the wall cell;
+ * in the next min(10, m3tot) cells whose wall point lies on this wall, the
+ * value blended with the current one over cells 2..4 (pond = (d - d2) / (d4
+ * - d2), the reference's imxclko = 3) and floored by 6 mu / (beta1 d^2); the
+ * ghost takes the wall cell's value.  Returns 1 (and writes nothing) when a
+ * line is shorter than 10 cells, where the reference stops. */
+int orc_met_parko(long nl, const orc_wall_line *ln, const long *ghost, const int *mnpar, const double *mu,
+                  const double *dist, double beta1, double *v, long ldv) {
+  double *w = v + 6 * ldv;
+  const int m3clpar = 10, imxclko = 3;
+  for (long p = 0; p < nl; p++)
+    if (abs(ln[p].m3max - ln[p].m3min) + 1 < m3clpar) return 1;
+  for (long p = 0; p < nl; p++) {
+    const orc_wall_line *L = &ln[p];
+    long st = (long)L->isens3 * L->dm3;
+    long nmin = L->nmin, nminp = nmin + st, nmaxp = nmin + st * (imxclko + 1);
+    int m3tot = abs(L->m3max - L->m3min) + 1;
+    int nmore = m3tot < m3clpar ? m3tot : m3clpar;
+    w[nmin] = 6. * mu[nmin] / (beta1 * (dist[nmin] * dist[nmin]));
+    for (int s = 1; s <= nmore; s++) {
+      long n = nmin + s * st;
+      int npar = mnpar[n];
+      if (npar > L->mpn0 && npar <= L->mpn1) {
+        double rom = 6. * mu[n] / (beta1 * (dist[n] * dist[n]));
+        if ((n - nmaxp) * (n - nminp) <= 0) {
+          double pond = (dist[n] - dist[nminp]) / (dist[nmaxp] - dist[nminp]);
+          w[n] = rom * (1. - pond) + pond * w[n];
+        }
+        w[n] = fmax(w[n], rom);
+      }
+    }
+    w[ghost[p]] = w[nmin];
+  }
+  return 0;
+}
+
 /* met_smkor.f90:4 -- Wilcox k-omega sources with cross diffusion and the
  * strain-rate limiter (equatt '2KOR', met_num.f90:509): fd holds the
  * gradients of k and omega (met_gradtr), g9 the velocity gradients

@@ -27,7 +27,8 @@ void orc_zgrad2(const orc_block *b, const double *sn, const double *vol, const d
                 const double *temp, double *g) {
   long n0 = b->npc, nnn = b->nnn;
   int i1 = b->ii1, i2 = b->ii2, j1 = b->jj1, j2 = b->jj2, k1 = b->kk1, k2 = b->kk2;
-  double *vx = calloc(nnn, 8), *vy = calloc(nnn, 8), *vz = calloc(nnn, 8);
+  double *vx = orc_zeros(nnn), *vy = orc_zeros(nnn), *vz = orc_zeros(nnn);
+#pragma omp parallel for schedule(static)
   for (int k = k1 - 1; k <= k2; k++)
     for (int j = j1 - 1; j <= j2; j++)
       for (int i = i1 - 1; i <= i2; i++) {
@@ -45,14 +46,11 @@ void orc_zgrad2(const orc_block *b, const double *sn, const double *vol, const d
     int lo[3] = {i1, j1, k1}, hi[3] = {i2 - 1, j2 - 1, k2 - 1};
     int a1 = d == 0 ? i1 : (d == 1 ? j1 : k1), a2 = d == 0 ? i2 : (d == 1 ? j2 : k2);
     lo[d] = a1 + 1; hi[d] = a2 - 1;
-    /* interior faces; the j sweep runs j outer, k inner (:277-278) */
-    int order_jk = (d == 1);
-    int outer_lo = order_jk ? lo[1] : lo[2], outer_hi = order_jk ? hi[1] : hi[2];
-    int mid_lo = order_jk ? lo[2] : lo[1], mid_hi = order_jk ? hi[2] : hi[1];
-    for (int o = outer_lo; o <= outer_hi; o++)
-      for (int q = mid_lo; q <= mid_hi; q++)
-        for (int i = lo[0]; i <= hi[0]; i++) {
-          int j = order_jk ? o : q, k = order_jk ? q : o;
+    /* interior faces (the reference's j sweep runs j outer, k inner,
+     * :277-278; each cell receives its two j faces in increasing j either
+     * way, so the threads share k there): FOR3D splits a direction other
+     * than d, the result is the serial one bit for bit */
+    FOR3D(lo, hi, d) {
           long n = IDX(b, i, j, k), m = n - n0;
           double sx = SN(sn, b, m, d, 0), sy = SN(sn, b, m, d, 1), sz = SN(sn, b, m, d, 2);
           double fv[4] = {vx[m] + vx[m - inc], vy[m] + vy[m - inc], vz[m] + vz[m - inc],
@@ -71,6 +69,7 @@ void orc_zgrad2(const orc_block *b, const double *sn, const double *vol, const d
     for (int side = 0; side < 2; side++) {
       int l2[3] = {i1, j1, k1}, h2[3] = {i2 - 1, j2 - 1, k2 - 1};
       l2[d] = h2[d] = side == 0 ? a1 : a2;
+#pragma omp parallel for schedule(static)
       for (int k = l2[2]; k <= h2[2]; k++)
         for (int j = l2[1]; j <= h2[1]; j++)
           for (int i = l2[0]; i <= h2[0]; i++) {
@@ -88,6 +87,7 @@ void orc_zgrad2(const orc_block *b, const double *sn, const double *vol, const d
   }
   /* scaling by 1/(2 vol), guarded for vol <= 0 (:463-485), over the
    * contiguous storage range first..last real cell */
+#pragma omp parallel for schedule(static)
   for (long n = IDX(b, i1, j1, k1); n <= IDX(b, i2 - 1, j2 - 1, k2 - 1); n++) {
     long m = n - n0;
     double ts = copysign(0.5, -vol[n]);
@@ -103,6 +103,7 @@ void orc_zfluto(const orc_block *b, const orc_phys *ph, const double *mu, const 
                 double *tq, long ldv, const double *g) {
   long n0 = b->npc, nnn = b->nnn;
   const double ds3 = 2. / 3.;
+#pragma omp parallel for schedule(static)
   for (int k = b->kk1; k <= b->kk2 - 1; k++)
     for (int j = b->jj1; j <= b->jj2 - 1; j++)
       for (int i = b->ii1; i <= b->ii2 - 1; i++) {
@@ -287,13 +288,14 @@ void orc_implimf_3d(const orc_block *b, const orc_phys *ph, int ns, const double
   long n0 = b->npc, nnn = b->nnn;
   int i1 = b->ii1, i2 = b->ii2, j1 = b->jj1, j2 = b->jj2, k1 = b->kk1, k2 = b->kk2;
   long nci = INC(b, 0), ncj = INC(b, 1), nck = INC(b, 2);
-  double *df[9], *coefe[3], *rv = calloc(nnn, 8), *diag = calloc(nnn, 8), *d2w[5];
-  for (int c = 0; c < 9; c++) df[c] = calloc(nnn, 8);
-  for (int c = 0; c < 3; c++) coefe[c] = calloc(nnn, 8);
-  for (int c = 0; c < 5; c++) d2w[c] = calloc(nnn, 8);
+  double *df[9], *coefe[3], *rv = orc_zeros(nnn), *diag = orc_zeros(nnn), *d2w[5];
+  for (int c = 0; c < 9; c++) df[c] = orc_zeros(nnn);
+  for (int c = 0; c < 3; c++) coefe[c] = orc_zeros(nnn);
+  for (int c = 0; c < 5; c++) d2w[c] = orc_zeros(nnn);
   const double fact = 1.5;
   for (long n = IDX(b, i1 - 1, j1 - 1, k1 - 1); n <= IDX(b, i2 + 1, j2 + 1, k2 + 1); n++)
     for (int e = 0; e < 5; e++) d[e * ldv + n] = 0.;
+#pragma omp parallel for schedule(static)
   for (int k = k1; k <= k2 - 1; k++)
     for (int j = j1; j <= j2 - 1; j++)
       for (int i = i1; i <= i2 - 1; i++) {
@@ -305,6 +307,7 @@ void orc_implimf_3d(const orc_block *b, const orc_phys *ph, int ns, const double
     long inc = INC(b, dd);
     int lo[3] = {i1, j1, k1}, hi[3] = {i2 - 1, j2 - 1, k2 - 1};
     hi[dd] += 1; /* faces lo..hi of the direction, incl. the two boundaries */
+#pragma omp parallel for schedule(static)
     for (int k = lo[2]; k <= hi[2]; k++)
       for (int j = lo[1]; j <= hi[1]; j++)
         for (int i = lo[0]; i <= hi[0]; i++) {
@@ -319,6 +322,7 @@ void orc_implimf_3d(const orc_block *b, const orc_phys *ph, int ns, const double
           coefe[dd][m] = 0.5 * (fabs(vn) + sqrt(cnds) * cc) + (rv[m] + rv[m - inc]) * cnds / (vol[n] + vol[n - inc]);
         }
   }
+#pragma omp parallel for schedule(static)
   for (int k = k1; k <= k2 - 1; k++)
     for (int j = j1; j <= j2 - 1; j++)
       for (int i = i1; i <= i2 - 1; i++) {
@@ -328,6 +332,7 @@ void orc_implimf_3d(const orc_block *b, const orc_phys *ph, int ns, const double
         if (kfmg == 3) diag[m] = diag[m] + fact * vol[m + n0] / dtpas;
       }
   for (int ls = 1; ls <= lmx; ls++) {
+#pragma omp parallel for schedule(static)
     for (int k = k1; k <= k2 - 1; k++)
       for (int j = j1; j <= j2 - 1; j++)
         for (int i = i1; i <= i2 - 1; i++) {
@@ -339,9 +344,7 @@ void orc_implimf_3d(const orc_block *b, const orc_phys *ph, int ns, const double
       long inc = INC(b, dd);
       int lo[3] = {i1, j1, k1}, hi[3] = {i2 - 1, j2 - 1, k2 - 1};
       hi[dd] += 1;
-      for (int k = lo[2]; k <= hi[2]; k++)
-        for (int j = lo[1]; j <= hi[1]; j++)
-          for (int i = lo[0]; i <= hi[0]; i++) {
+      FOR3D(lo, hi, dd) {
             long n = IDX(b, i, j, k), m = n - n0;
             double s1 = SN(sn, b, m, dd, 0), s2 = SN(sn, b, m, dd, 1), s3 = SN(sn, b, m, dd, 2);
             /* df order: xx xy xz ex yy yz ey zz ez (implimf_3d.f90:97-105) */
@@ -365,6 +368,7 @@ void orc_implimf_3d(const orc_block *b, const orc_phys *ph, int ns, const double
             for (int e = 0; e < 5; e++) d2w[e][m - inc] = d2w[e][m - inc] - tn[e];
           }
     }
+#pragma omp parallel for schedule(static)
     for (int k = k1; k <= k2 - 1; k++)
       for (int j = j1; j <= j2 - 1; j++)
         for (int i = i1; i <= i2 - 1; i++) {
@@ -391,6 +395,7 @@ void orc_implimf_3d(const orc_block *b, const orc_phys *ph, int ns, const double
           df[8][m] = wi * (wi5 + pres - pinfl) - fez;
         }
   }
+#pragma omp parallel for schedule(static)
   for (int k = k1; k <= k2 - 1; k++)
     for (int j = j1; j <= j2 - 1; j++)
       for (int i = i1; i <= i2 - 1; i++) {
@@ -420,10 +425,10 @@ void orc_implimf_eu(const orc_block *b, const orc_phys *ph, const double *u, con
   const double gam1 = ph->gam1, pinfl = ph->pinfl;
   long n0 = b->npc, nnn = b->nnn;
   int i1 = b->ii1, i2 = b->ii2, j1 = b->jj1, j2 = b->jj2, k1 = b->kk1, k2 = b->kk2;
-  double *df[9], *coefe[3], *diag = calloc(nnn, 8), *d2w[5];
-  for (int c = 0; c < 9; c++) df[c] = calloc(nnn, 8);
-  for (int c = 0; c < 3; c++) coefe[c] = calloc(nnn, 8);
-  for (int c = 0; c < 5; c++) d2w[c] = calloc(nnn, 8);
+  double *df[9], *coefe[3], *diag = orc_zeros(nnn), *d2w[5];
+  for (int c = 0; c < 9; c++) df[c] = orc_zeros(nnn);
+  for (int c = 0; c < 3; c++) coefe[c] = orc_zeros(nnn);
+  for (int c = 0; c < 5; c++) d2w[c] = orc_zeros(nnn);
   double fact = kfmg == 3 ? 1.5 / dt1min : 0.;
   for (int e = 0; e < 5; e++)
     for (long n = IDX(b, i1 - 1, j1 - 1, k1 - 1); n <= IDX(b, i2 + 1, j2 + 1, k2 + 1); n++) d[e * ldv + n] = 0.;

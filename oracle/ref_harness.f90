@@ -705,6 +705,57 @@ contains
     omg = 0.
   end subroutine ref_set_boundaries
 
+  ! met_bord.f90:4 -- the turbulence ghosts (v(:,6:7)) of every frontier
+  ! set by ref_set_boundaries, in frontier order: kinds(f) = 0 coincident
+  ! ('rc'), 1 wall ('para'), 2 far field ('nrd'), 3 symmetry ('sym'); kpar:
+  ! kparoi; rk, re: rokinf, roeinf.  Coincident frontiers first go through
+  ! met_rbvc (the partner frontiers set by ref_set_boundaries).
+  subroutine ref_met_bord(nfr, kinds, kpar, rk, re, v, ncbd, ncin) bind(C, name="ref_met_bord")
+    use boundary
+    use modeleturb
+    use mod_met_bord
+    integer(c_int), value :: nfr, kpar
+    integer(c_int) :: kinds(*), ncbd(*), ncin(*)
+    real(c_double), value :: rk, re
+    real(c_double) :: v(*)
+    character(len=4), parameter :: names(0:3) = ['rc  ', 'para', 'nrd ', 'sym ']
+    real(8), allocatable :: bceq(:, :)
+    real(8) :: utau(1), mu(1), xnr(1), ynr(1), znr(1)
+    integer :: mnc(1), mnr(1), f, nc
+    if (allocated(cl)) deallocate(cl)
+    if (allocated(nfbc)) deallocate(nfbc)
+    allocate(cl(nfr), nfbc(nfr), bceq(ip41, neqt))
+    bceq = 0.
+    nc = 0
+    do f = 1, nfr
+       cl(f) = names(kinds(f))
+       if (kinds(f) == 0) then
+          nc = nc + 1
+          nfbc(nc) = f
+       endif
+    enddo
+    mtcx = nc
+    mtbx = nfr
+    kparoi = kpar
+    rokinf = rk
+    roeinf = re
+    mnc = 0; mnr = 0; utau = 0.; mu = 0.; xnr = 0.; ynr = 0.; znr = 0.
+    call met_bord(0, bceq, mnc, ncin, mnr, xnr, ynr, znr, ncbd, v, utau, mu)
+    deallocate(bceq)
+  end subroutine ref_met_bord
+
+  ! met_parko.f90:4 -- omega wall condition over the walls of ref_set_walls
+  ! (equatt '2KO': imxclko = 3, beta1 = 0.075)
+  subroutine ref_met_parko(ncin, ncbd, v, mut, mu, dist, mnpar) bind(C, name="ref_met_parko")
+    use modeleturb
+    use chainecarac
+    use mod_met_parko
+    integer(c_int) :: ncin(*), ncbd(*), mnpar(*)
+    real(c_double) :: v(*), mut(*), mu(*), dist(*)
+    equatt = '2KO    '
+    call met_parko(0, ncin, ncbd, v, mut, mu, dist, mnpar)
+  end subroutine ref_met_parko
+
   ! the frontier list lbd(1:nl) = fr(1:nl) that rbvc / rbte / rbtc /
   ! met_rbse / met_rbsc walk (zvisqc.f90:182-205, met_num.f90:575-600)
   subroutine set_lbd(nl, fr)
@@ -885,5 +936,109 @@ contains
          tn(1,1), tn(1,2), tn(1,3), tn(1,4), tn(1,5), tn(1,6), tn(1,7), tn(1,8), tn(1,9))
     deallocate(tn)
   end subroutine ref_initns
+
+  ! --------------------------------------------------------------------
+  ! On-disk formats: the reference's writda / readda (restart) and writdg /
+  ! readdg (mesh), every block in order on one sequential unformatted unit,
+  ! as svfw.f90:58 and infw.f90:69 call them block after block.  The
+  ! ordering helpers of mod_mpi take block l as its own global block on
+  ! rank 0 (the serial build).
+  ! --------------------------------------------------------------------
+  subroutine serial_blocks(nb)
+    use mod_mpi
+    integer, intent(in) :: nb
+    integer :: l
+    if (allocated(bl_to_bg)) deallocate(bl_to_bg)
+    if (allocated(bg_to_proc)) deallocate(bg_to_proc)
+    allocate(bl_to_bg(nb), bg_to_proc(nb))
+    do l = 1, nb
+       bl_to_bg(l) = l
+       bg_to_proc(l) = 0
+    enddo
+  end subroutine serial_blocks
+
+  function cstr(p, n) result(fn)
+    integer(c_int), intent(in) :: n
+    character(kind=c_char), intent(in) :: p(n)
+    character(len=n) :: fn
+    integer :: i
+    do i = 1, n
+       fn(i:i) = p(i)
+    enddo
+  end function cstr
+
+  ! writda.f90:4 with chdacc 'cccc' extents (the real cells); eq: equat
+  subroutine ref_writda_all(path, plen, nb, eq, v, mut) bind(C, name="ref_writda_all")
+    use chainecarac
+    use mod_writda
+    integer(c_int), value :: plen, nb
+    character(kind=c_char) :: path(plen), eq(7)
+    real(c_double) :: v(ip11, *), mut(*)
+    real(8) :: utau(1)
+    integer :: l, n0
+    call serial_blocks(nb)
+    utau = 0.
+    open(unit=31, file=cstr(path, plen), form='unformatted', access='sequential', status='replace')
+    do l = 1, nb
+       n0 = npc(l)
+       call writda(l, 31, cstr(eq, 7), utau, ii1(l), ii2(l) - 1, jj1(l), jj2(l) - 1, kk1(l), kk2(l) - 1, &
+            v(n0 + 1, 1), v(n0 + 1, 2), v(n0 + 1, 3), v(n0 + 1, 4), v(n0 + 1, 5), v(n0 + 1, 6), v(n0 + 1, 7), &
+            ip12, mut(n0 + 1))
+    enddo
+    close(31)
+  end subroutine ref_writda_all
+
+  ! readda.f90:4 (kfmg = 0, kutau = 0: no dual-time copies, no utau)
+  subroutine ref_readda_all(path, plen, nb, eq, v, mut) bind(C, name="ref_readda_all")
+    use chainecarac
+    use mod_readda
+    integer(c_int), value :: plen, nb
+    character(kind=c_char) :: path(plen), eq(7)
+    real(c_double) :: v(ip11, ip60), mut(ip12)
+    real(8), allocatable :: utau(:)
+    integer :: l
+    call serial_blocks(nb)
+    equat = cstr(eq, 7)
+    kfmg = 0
+    kutau = 0
+    keinit = 0
+    allocate(utau(1))
+    open(unit=32, file=cstr(path, plen), form='unformatted', access='sequential', status='old')
+    do l = 1, nb
+       call readda(l, 32, v, mut, utau, v, v, v)
+    enddo
+    close(32)
+  end subroutine ref_readda_all
+
+  ! writdg.f90:4 over the nodes ii1..ii2, jj1..jj2, kk1..kk2
+  subroutine ref_writdg_all(path, plen, nb, x, y, z) bind(C, name="ref_writdg_all")
+    use mod_writdg
+    integer(c_int), value :: plen, nb
+    character(kind=c_char) :: path(plen)
+    real(c_double) :: x(*), y(*), z(*)
+    integer :: l, n0
+    call serial_blocks(nb)
+    open(unit=33, file=cstr(path, plen), form='unformatted', access='sequential', status='replace')
+    do l = 1, nb
+       n0 = npn(l)
+       call writdg(l, 33, ii1(l), ii2(l), jj1(l), jj2(l), kk1(l), kk2(l), x(n0 + 1), y(n0 + 1), z(n0 + 1))
+    enddo
+    close(33)
+  end subroutine ref_writdg_all
+
+  ! readdg.f90:4
+  subroutine ref_readdg_all(path, plen, nb, x, y, z) bind(C, name="ref_readdg_all")
+    use mod_readdg
+    integer(c_int), value :: plen, nb
+    character(kind=c_char) :: path(plen)
+    real(c_double) :: x(ip21), y(ip21), z(ip21)
+    integer :: l
+    call serial_blocks(nb)
+    open(unit=34, file=cstr(path, plen), form='unformatted', access='sequential', status='old')
+    do l = 1, nb
+       call readdg(l, 34, x, y, z)
+    enddo
+    close(34)
+  end subroutine ref_readdg_all
 
 end module codens_ref_harness

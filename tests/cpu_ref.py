@@ -7,6 +7,7 @@ Every function takes the ``Layout`` and numpy arrays in the reference storage
 from __future__ import annotations
 
 import ctypes
+import os
 
 import numpy as np
 
@@ -61,6 +62,25 @@ class Ref:
 
     def zpres(self, l, v, ps, temp, cson):
         self.lib.ref_zpres(l + 1, ptr(v), ptr(ps), ptr(temp), ptr(cson))
+
+    # on-disk formats (writda / readda / writdg / readdg over all blocks)
+    def writda_all(self, path, equat, v, mut):
+        p = os.fsencode(path)
+        self.lib.ref_writda_all(ctypes.c_char_p(p), len(p), len(self.lay.blocks), equat.encode().ljust(7)[:7],
+                                ptr(v), ptr(mut))
+
+    def readda_all(self, path, equat, v, mut):
+        p = os.fsencode(path)
+        self.lib.ref_readda_all(ctypes.c_char_p(p), len(p), len(self.lay.blocks), equat.encode().ljust(7)[:7],
+                                ptr(v), ptr(mut))
+
+    def writdg_all(self, path, x, y, z):
+        p = os.fsencode(path)
+        self.lib.ref_writdg_all(ctypes.c_char_p(p), len(p), len(self.lay.blocks), ptr(x), ptr(y), ptr(z))
+
+    def readdg_all(self, path, x, y, z):
+        p = os.fsencode(path)
+        self.lib.ref_readdg_all(ctypes.c_char_p(p), len(p), len(self.lay.blocks), ptr(x), ptr(y), ptr(z))
 
     def zvismo(self, l, mu, v, temp):
         self.lib.ref_zvismo(l + 1, ptr(mu), ptr(v), ptr(temp))
@@ -417,6 +437,12 @@ class RefFrontiers:
     def clsym(self, f, nrm, v):
         nx, ny, nz = self._normals(f, nrm)
         self.lib.ref_clsym(f + 1, ptr(nx), ptr(ny), ptr(nz), ptr(self.ncin), ptr(self.ncbd), ptr(v))
+
+    def met_bord(self, kinds, kparoi, rokinf, roeinf, v):
+        """met_bord.f90:4 over every frontier: kinds[f] in 'rc', 'para', 'nrd', 'sym'."""
+        code = np.ascontiguousarray(np.array([["rc", "para", "nrd", "sym"].index(k) for k in kinds], dtype=np.int32))
+        self.lib.ref_met_bord(self.nfr, ptr(code), int(kparoi), D(rokinf), D(roeinf), ptr(v), ptr(self.ncbd),
+                              ptr(self.ncin))
 
     def rbte(self, frs, tq):
         fr = self._list(frs)

@@ -155,6 +155,27 @@ class CpuStage:
         n = lay.ip11
         sa = t.model == "1SA"
         L_, D = C.L_, C.D
+        # met_num.f90:145-159: met_bord (the reference's own with its build),
+        # then met_parko for k-omega (the oracle's, pinned bit for bit against
+        # the reference's in tests/test_oracle_metbord.py: the reference's
+        # needs the wall tables of atindnor, which this driver does not set)
+        w, f, c, y = cs.wall, cs.farfield, cs.coincident, cs.sym
+        if self.RF is not None:
+            self.RF.met_bord([x[0] for x in self.fr], t.kparoi, t.rokinf, t.roeinf, self.v)
+        else:
+            lists = [(0, c.ghost, c.inner, c.remote), (1 + t.kparoi, w.ghost, w.inner, w.inner),
+                     (3, f.ghost, f.inner, f.inner), (4, y.ghost, y.inner, y.inner)]
+            for kind, gg, ii, rr in lists:
+                if len(gg):
+                    gg, ii, rr = (np.ascontiguousarray(np.asarray(a, dtype=np.int64)) for a in (gg, ii, rr))
+                    O.lib.orc_met_bord(kind, L_(len(gg)), ptr(gg), ptr(ii), ptr(rr), D(t.rokinf), D(t.roeinf),
+                                       ptr(self.v), L_(n))
+        if t.kparoi == 1:
+            ln = t.wall_lines
+            rc = O.lib.orc_met_parko(L_(len(ln)), ln.ctypes.data_as(ctypes.c_void_p),
+                                     ptr(np.ascontiguousarray(t.wall_ghost)), ptr(t.mnpar), ptr(self.mu),
+                                     ptr(t.dist), D(0.075), ptr(self.v), L_(n))
+            assert rc == 0, "met_parko: a wall line shorter than 10 cells (the reference stops)"
         g9s, tps = [], []
         for l, b in enumerate(lay.blocks):
             g9 = O.teq_gradv(l, cs.sn, cs.vol, self.v)
@@ -167,7 +188,6 @@ class CpuStage:
                 O.met_smko(l, self.tc, self.v, tp, self.cfke, self.ts6, self.ts7, self.cson)
             O.met_flud(l, t.kfludis, self.tc, self.v, self.mu, self.mut, self.fd)
             g9s.append(g9); tps.append(tp)
-        w, f, c, y = cs.wall, cs.farfield, cs.coincident, cs.sym
         if self.RF is not None:  # met_num.f90:575-600
             if self.other_fr:
                 self.RF.met_rbse(self.other_fr, self.fd)

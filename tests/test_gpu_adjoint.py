@@ -131,17 +131,24 @@ def test_finite_differences_64_directions(small):
 
 
 def test_full_size_adjoint():
-    """128x64x32 block, 4096 wall nodes: dot-product test of the residual,
-    tangent against reverse and central FD on 4 directions."""
+    """The specified size: 128x64x32 block, 4096 seeded wall nodes.  On 64
+    seeded unit directions of (state, wall displacement): the tangent-linear
+    sweep against the reverse-sweep gradient within 1e-10, and central finite
+    differences (eps 1e-7) against it within 1e-6."""
     from codens_amd.adjoint import gradient, directional
     P, case, v0, delta = _setup(128, 64, 32, 4096)
     _, gv, gd = gradient(P, v0, delta)
     d0 = delta.cpu().numpy()
-    for vdir, ddir in _dirs(P, case, np.random.default_rng(11), 4):
+    eps = 1e-7
+    e_fd, e_tl = [], []
+    for vdir, ddir in _dirs(P, case, np.random.default_rng(11), 64):
         t = directional(P, v0, delta, vdir, ddir)
         r = float((gv * vdir).sum() + (gd * ddir).sum())
-        assert abs(t - r) <= 1e-10 * abs(r), (t, r)
-        eps = 1e-7
-        fd = (P.J_value(v0 + eps * vdir, d0 + eps * ddir.cpu().numpy())
-              - P.J_value(v0 - eps * vdir, d0 - eps * ddir.cpu().numpy())) / (2 * eps)
-        assert abs(fd - r) <= 1e-6 * abs(r), (fd, r)
+        dd = ddir.cpu().numpy()
+        fd = (P.J_value(v0 + eps * vdir, d0 + eps * dd) - P.J_value(v0 - eps * vdir, d0 - eps * dd)) / (2 * eps)
+        e_tl.append(abs(t - r) / abs(r))
+        e_fd.append(abs(fd - r) / abs(r))
+    print("128x64x32, 64 directions: FD max %.3e median %.3e; tangent max %.3e"
+          % (max(e_fd), float(np.median(e_fd)), max(e_tl)))
+    assert max(e_tl) <= 1e-10, e_tl
+    assert max(e_fd) <= 1e-6, e_fd

@@ -61,8 +61,16 @@ def test_thermo_dt(ctx, pinfl):
         assert relerr(H(gdt), dt_o) <= TOL, f"chronos ns={ns}"
 
 
+@pytest.fixture(params=[1, 0], ids=["fused", "split"])
+def path(request):
+    from codens_amd import abi
+    prev = abi.residual_path(request.param)
+    yield request.param
+    abi.residual_path(prev)
+
+
 @pytest.mark.parametrize("ns,ilim,pinfl", [(0, 0, 0.0), (0, 0, 0.3), (1, 0, 0.0), (1, 1, 0.0), (1, 1, 0.2)])
-def test_roe_residual(ctx, ns, ilim, pinfl):
+def test_roe_residual(ctx, path, ns, ilim, pinfl):
     abi, lay, sn, vol = ctx
     rng = np.random.default_rng(2 + ilim)
     n = lay.ip11
@@ -85,6 +93,58 @@ def test_roe_residual(ctx, ns, ilim, pinfl):
     for e in range(5):
         assert relerr(gu_h[e], u_o[e]) <= TOL, f"u[{e}]"
         assert relerr(gff_h[e], ff_o[e]) <= TOL, f"ff[{e}]"
+
+
+@pytest.mark.parametrize("ns,ilim", [(0, 0), (1, 0), (1, 1)])
+def test_roe_residual_multitile(ns, ilim):
+    """Blocks spanning several 16x16 tiles and 8-plane k slabs of the fused
+    residual, with partial tiles and slabs on every high edge (37x21x19,
+    16x16x8, 17x33x9): the fused kernel against the oracle within 1e-12, and
+    against the face passes + gather within 1e-14 (the same face values,
+    summed in the same order)."""
+    from codens_amd import abi
+    lay, sn, vol = random_mesh([(37, 21, 19), (16, 16, 8), (17, 33, 9)], seed=23)
+    rng = np.random.default_rng(31 + ilim)
+    n = lay.ip11
+    v = random_state(n, rng)
+    ps, _, _ = all_thermo(v)
+    tq = 0.01 * rng.standard_normal((9, n)) if ns else np.zeros((9, n))
+    O = C.Orc(lay, C.orc_phys(), muscl=1, xk=1 / 3, ilim=ilim, ns=bool(ns))
+    ph, sc = abi.phys(), abi.scheme(1, ilim, 1 / 3)
+    u_o = np.zeros((5, n)); ff_o = np.ones((5, n))
+    for l in range(len(lay.blocks)):
+        if ns:
+            O.sch_roe(l, 1, u_o, v, ff_o, tq, sn, ps)
+        else:
+            O.sch_roe_euler(l, 1, u_o, v, ff_o, sn, ps)
+    res = {}
+    for fused in (1, 0):
+        prev = abi.residual_path(fused)
+        try:
+            gu = T(np.full((5, n), 3.0)); gff = T(np.ones((5, n))); gv = T(v); gps = T(ps); gtq = T(tq)
+            for l, b in enumerate(lay.blocks):
+                snb = T(sn[9 * b.npc: 9 * (b.npc + b.nnn)])
+                if ns:
+                    abi.sch_roe(abi.block(b), ph, sc, 1, gu, gv, gff, gtq, snb, gps)
+                else:
+                    abi.sch_roe_euler(abi.block(b), ph, sc, 1, gu, gv, gff, snb, gps)
+            res[fused] = (H(gu), H(gff))
+        finally:
+            abi.residual_path(prev)
+    w = np.zeros(n, dtype=bool)  # the entries sch_roe writes
+    for b in lay.blocks:
+        w[b.idx(b.ii1 - 1, b.jj1 - 1, b.kk1 - 1):b.idx(b.ii2, b.jj2, b.kk2) + 1] = True
+    for e in range(5):
+        assert relerr(res[1][0][e, w], u_o[e, w]) <= TOL, f"u[{e}]"
+        assert relerr(res[1][1][e], ff_o[e]) <= TOL, f"ff[{e}]"
+        assert relerr(res[1][0][e, w], res[0][0][e, w]) <= 1e-14, f"fused vs split u[{e}]"
+    # both write exactly indc(i1-1,j1-1,k1-1)..indc(i2,j2,k2), ghosts zero
+    for b in lay.blocks:
+        n0, n1 = b.idx(b.ii1 - 1, b.jj1 - 1, b.kk1 - 1), b.idx(b.ii2, b.jj2, b.kk2)
+        for f in (1, 0):
+            u = res[f][0]
+            assert np.all(u[:, b.npc:n0] == 3.0) and np.all(u[:, n1 + 1:b.npc + b.nnn] == 3.0), f
+            assert np.array_equal(u[:, n0:n1 + 1] == 0.0, u_o[:, n0:n1 + 1] == 0.0), f
 
 
 @pytest.mark.parametrize("ki2,ki4", [(0.5, 1.0 / 32), (1.0, 0.05)])
@@ -378,9 +438,9 @@ def test_bad_extents_rejected(ctx):
 
 @pytest.mark.parametrize("form", ["fp64", "eu", "fp32"])
 def test_sweep_tile_shapes_identical(form):
-    """The two tile shapes of the Jacobi sub-iteration (64 x 4 and 32 x 8)
-    and the timed choice give bit-identical increments, on a block whose i
-    and j extents are multiples of neither tile (70 x 37 x 9 cells)."""
+    """The three tile shapes of the Jacobi sub-iteration (64 x 4, 32 x 8 and
+    16 x 16) and the timed choice give bit-identical increments, on a block
+    whose i and j extents are multiples of no tile (70 x 37 x 9 cells)."""
     from codens_amd import abi
     lay, sn, vol = random_mesh([(70, 37, 9)], seed=23)
     b = lay.blocks[0]
@@ -396,7 +456,7 @@ def test_sweep_tile_shapes_identical(form):
     out = {}
     prev = abi.sweep_tile(0)
     try:
-        for mode in (0, 1, -1):
+        for mode in (0, 1, 2, -1):
             abi.sweep_tile(mode)
             gv, gd = T(v), T(np.zeros((5, n)))
             if form == "eu":
@@ -411,7 +471,54 @@ def test_sweep_tile_shapes_identical(form):
         chosen = abi.sweep_tile_of(abi.block(b), eu=form == "eu", fp32=form == "fp32")
     finally:
         abi.sweep_tile(prev)
-    assert chosen in (0, 1)
-    for mode in (1, -1):
+    assert chosen in (0, 1, 2)
+    for mode in (1, 2, -1):
         assert np.array_equal(out[mode][0], out[0][0]) and np.array_equal(out[mode][1], out[0][1]), mode
     assert np.abs(out[0][1]).max() > 0
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3, 4])
+def test_met_bord(ctx, kind):
+    """codens_met_bord against the oracle, bit for bit, for each frontier
+    kind: coincident average, wall (kparoi 0 and 1), far field, symmetry."""
+    from codens_amd import layout as L
+    abi, lay, sn, vol = ctx
+    rng = np.random.default_rng(71 + kind)
+    n = lay.ip11
+    v = rng.standard_normal((7, n))
+    g, i, _ = L.face_cells(lay.blocks[0], "j1")
+    g2, i2, _ = L.face_cells(lay.blocks[0], "j2")
+    r = i2  # any real cells of the same count stand for the partner's
+    O = C.Orc(lay, C.orc_phys())
+    vo = v.copy()
+    ga, ia, ra = (np.ascontiguousarray(a.astype(np.int64)) for a in (g, i, r))
+    O.lib.orc_met_bord(kind, C.L_(len(ga)), C.ptr(ga), C.ptr(ia), C.ptr(ra), C.D(2.5e-4), C.D(7.0), C.ptr(vo),
+                       C.L_(n))
+    gv = T(v)
+    abi.met_bord(kind, T(ga, torch.int64), T(ia, torch.int64), T(ra, torch.int64), 2.5e-4, 7.0, gv)
+    assert np.array_equal(H(gv), vo)
+    assert not np.array_equal(vo, v)
+
+
+def test_met_parko():
+    """codens_met_parko against the oracle on the j1 wall of a 9x13x5 block
+    (13 cells from the wall: the 10-cell reach stays inside), bit for bit."""
+    from codens_amd import abi, configs
+    case = configs.wall_rans(9, 13, 5, model="2KO")
+    t = case.turb
+    n = case.layout.ip11
+    rng = np.random.default_rng(77)
+    v = rng.standard_normal((7, n)); v[6] = 40 * np.abs(v[6])
+    mu = 1e-3 * (1 + rng.random(n))
+    O = C.Orc(case.layout, C.orc_phys())
+    vo = v.copy()
+    import ctypes
+    rc = O.lib.orc_met_parko(C.L_(len(t.wall_lines)), t.wall_lines.ctypes.data_as(ctypes.c_void_p),
+                             C.ptr(np.ascontiguousarray(t.wall_ghost)), C.ptr(t.mnpar), C.ptr(mu), C.ptr(t.dist),
+                             C.D(0.075), C.ptr(vo), C.L_(n))
+    assert rc == 0
+    gv = T(v)
+    abi.met_parko(t.wall_start, torch.as_tensor(t.wall_lines.view(np.uint8).copy(), device="cuda"),
+                  T(t.wall_ghost, torch.int64), T(t.mnpar, torch.int32), T(mu), T(t.dist), 0.075, gv)
+    assert np.array_equal(H(gv), vo)
+    assert not np.array_equal(vo[6], v[6])

@@ -100,8 +100,9 @@ def _stage_case(which):
 def test_stages_bit_exact(which):
     """Whole stages of the oracle (C restatement of every kernel and ghost
     rule) against the reference build (its own kernels and its own boundary
-    routines rbvc, clpara, utnrd + rbve + clnrd, clsym, rbte, rbtc, met_rbse,
-    met_rbsc, in rbord's order), bit for bit, state and ghosts."""
+    routines rfve in the first cycle, rbvc, clpara, utnrd + rbve + clnrd,
+    clsym, rbte, rbtc, met_bord, met_rbse, met_rbsc, in rbord's and
+    met_num's order), bit for bit, state and ghosts."""
     import cpu_stage
     case = _stage_case(which)
     a, b = cpu_stage.CpuStage(case, use_ref=False), cpu_stage.CpuStage(case, use_ref=True)

@@ -84,6 +84,8 @@ SIGNATURES = {
                                 c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp],
     "codens_implimf_3d_f32": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int,
                           c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp, c_vp],
+    "codens_implimf_3d_noalloc": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,
+                                  c_int, c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp, c_vp],
     "codens_bc_coincident": [c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
     "codens_bc_wall": [c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
     "codens_bc_farfield": [c_i64, c_vp, c_vp, c_vp, P(c_dbl), P(CodensPhys), c_vp, c_i64, c_vp, c_vp, c_vp,

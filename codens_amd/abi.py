@@ -306,6 +306,15 @@ def implimf_3d_f32(b, ph, u, dt, v, d, ff, mu, mut, lmx, sn_blk, vol, ps, cson, 
                                    _s(stream)))
 
 
+def implimf_3d_noalloc(b, ph, u, dt, v, d, ff, mu, mut, lmx, sn_blk, vol, ps, cson, dtpas=1.0, kfmg=0, ityprk=0,
+                       stream=None):
+    """implimf_3d with the library's static scratch: allocates nothing (for
+    callers under graph capture); blocks of up to 2,400,000 storage entries."""
+    _chk(lib.codens_implimf_3d_noalloc(ctypes.byref(b), ctypes.byref(ph), _p(u), _p(dt), _p(v), ld(v), _p(d), _p(ff),
+                                       _p(mu), _p(mut), lmx, _p(sn_blk), _p(vol), dtpas, kfmg, ityprk, _p(ps),
+                                       _p(cson), _s(stream)))
+
+
 def implimf_eu(b, ph, u, dt, v, d, ff, lmx, sn_blk, vol, ps, cson, dt1min=1.0, kfmg=0, ityprk=0, stream=None):
     """implimf_eu.f90:6; v and d hold nvar = v.shape[0] planes, all updated."""
     assert d.shape[0] >= v.shape[0]

@@ -284,6 +284,22 @@ int codens_implimf_3d(const codens_block *b, const codens_phys *ph, const double
                       const double *vol, double dtpas, int32_t kfmg, int32_t ityprk,
                       const double *ps, const double *cson, void *stream);
 
+/* codens_implimf_3d with no allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as codens_implimf_3d; its scratch (9 doubles per
+ * storage entry of the block) is a static device array of the library,
+ * sized for blocks of up to 2,400,000 storage entries (172.8 MB per device,
+ * held until the process ends); a larger block is refused (CODENS_EINVAL).
+ * Every call on a device shares that array: calls that overlap on one device
+ * race.  The sub-iteration tile is the forced one (codens_sweep_tile), the
+ * one already timed for the block's shape, or the lane-and-halo model's; no
+ * timing launch is made. */
+int codens_implimf_3d_noalloc(const codens_block *b, const codens_phys *ph, const double *u, const double *dt,
+                              double *v, int64_t ldv, double *d, const double *ff, const double *mu, const double *mut,
+                              int32_t lmx, const double *sn_blk, const double *vol, double dtpas, int32_t kfmg,
+                              int32_t ityprk, const double *ps, const double *cson, void *stream);
+
 /* FP32 defect-correction form of codens_implimf_3d, with the same arguments
  * and results: the coefficients are computed in FP64 and rounded, and the
  * lmx Jacobi sub-iterations run in FP32 on FP32 copies of v, u, ff and sn;

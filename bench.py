@@ -118,6 +118,9 @@ def main():
                     help="implicit phase: implimf_3d in FP64 (the reference's), or its FP32 defect-correction form")
     ap.add_argument("--residual", choices=("fused", "split"), default="fused",
                     help="sch_roe as one fused kernel (default) or as face passes + gather")
+    ap.add_argument("--sweep-tile", type=int, default=-1, choices=(-1, 0, 1, 2),
+                    help="tile of the implicit sub-iteration: -1 timed choice per block shape (default), "
+                         "0 64x4, 1 32x8, 2 16x16")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="finish with 3 device-to-device copies of MIB MiB: the known-size kernel that "
                          "calibrates FETCH_SIZE/WRITE_SIZE counter passes")
@@ -131,6 +134,7 @@ def main():
     from codens_amd.solver import Solver
 
     abi.residual_path(1 if args.residual == "fused" else 0)
+    abi.sweep_tile(args.sweep_tile)
     if args.workload == "adjoint":
         return adjoint_bench(args)
     if args.workload == "converge":
@@ -203,7 +207,10 @@ def main():
                                + (", " + ", ".join(p for p in ("residual", "implicit", "turbulence")
                                                     if p in s.batch)
                                   + " of all blocks in one launch per pass" if s.batch else ""),
-                   "cells": ncell, "blocks": nblk, "lmax": case.lmax},
+                   "cells": ncell, "blocks": nblk, "lmax": case.lmax,
+                   "sweep_tile": ({0: "64x4", 1: "32x8", 2: "16x16"}.get(
+                       abi.sweep_tile_of(abi.block(case.layout.blocks[0]), fp32=fp32), "none")
+                       if nblk == 1 else "per block")},
         "roofline": {"bound": "hbm", "kernel": sweep_name, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": SWEEP_TRAFFIC_BYTES if (ci, cj, ck) == (256, 128, 64) and not fp32 and not multi

@@ -58,6 +58,8 @@ SIGNATURES = {
     "codens_met_flud": [P(CodensBlock), c_int, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp],
     "codens_sch_turb": [P(CodensBlock), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "codens_sch_expli": [P(CodensBlock), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp],
+    "codens_mesh_metrics": [P(CodensBlock), c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "codens_rk_stage": [P(CodensBlock), c_dbl, c_vp, c_vp, c_vp, c_i64, c_int, c_vp, c_vp, c_vp],
     "codens_implimf_3d": [P(CodensBlock), P(CodensPhys), c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_int,
                           c_vp, c_vp, c_dbl, c_int, c_int, c_vp, c_vp, c_vp],
     "codens_adj_residual_jvp": [P(CodensBlock), P(CodensPhys), P(CodensScheme), c_vp, c_vp, c_i64, c_vp, c_vp, c_vp,

@@ -290,6 +290,21 @@ def sch_expli(b, u, v, dt, vol, stream=None):
     _chk(lib.codens_sch_expli(ctypes.byref(b), _p(u), _p(v), ld(v), _p(dt), _p(vol), _s(stream)))
 
 
+def mesh_metrics(b, x, y, z, sn_blk, vol, stream=None):
+    """Ghost nodes (overwritten in x, y, z), sn_blk (9 nnn) and vol of one
+    block from its node coordinates (extmhg.f90, norm.f90, svol.f90)."""
+    _chk(lib.codens_mesh_metrics(ctypes.byref(b), _p(x), _p(y), _p(z), _p(sn_blk), _p(vol), _s(stream)))
+
+
+def rk_stage(b, alpha, v0, u, v, dt, vol, nvar=5, stream=None):
+    """v = v0 - alpha dt/vol u on the real cells (one stage of the 4-stage
+    explicit scheme; v0, u and v share one leading dimension)."""
+    if ld(v0) != ld(v) or ld(u) != ld(v):
+        raise ValueError("rk_stage: v0, u and v must share one leading dimension")
+    _chk(lib.codens_rk_stage(ctypes.byref(b), float(alpha), _p(v0), _p(u), _p(v), ld(v), int(nvar), _p(dt), _p(vol),
+                             _s(stream)))
+
+
 def implimf_3d(b, ph, u, dt, v, d, ff, mu, mut, lmx, sn_blk, vol, ps, cson, dtpas=1.0, kfmg=0, ityprk=0,
                stream=None):
     _chk(lib.codens_implimf_3d(ctypes.byref(b), ctypes.byref(ph), _p(u), _p(dt), _p(v), ld(v), _p(d), _p(ff),

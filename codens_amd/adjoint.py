@@ -22,12 +22,12 @@ Pieces, each a torch.autograd.Function over the C-ABI:
   codens_bc_wall, codens_bc_farfield, codens_zvisqc, codens_bc_tq_copy,
   codens_sch_roe); backward = codens_adj_residual_vjp; jvp =
   codens_adj_residual_jvp.
-* Metrics: forward = the mesh metrics of the displaced nodes (norm.f90,
-  svol.f90, on the host); backward = codens_adj_metric_vjp plus the
-  transposed ghost-node extrapolation; jvp = exact polynomial differences of
-  the metrics (sn is bilinear and vol cubic in the nodes, so a central
-  difference is exact for sn and a Richardson pair of central differences is
-  exact for vol).
+* Metrics: forward = the mesh metrics of the displaced nodes on the device
+  (codens_mesh_metrics: extmhg ghost nodes, norm.f90, svol.f90); backward =
+  codens_adj_metric_vjp plus the transposed ghost-node extrapolation; jvp =
+  exact polynomial differences of the metrics (sn is bilinear and vol cubic
+  in the nodes, so a central difference is exact for sn and a Richardson
+  pair of central differences is exact for vol).
 
 torch.autograd keeps the four stage states v_0..v_3 (and the mesh metrics)
 as the checkpoints of the reverse sweep.
@@ -51,8 +51,9 @@ def _dev(a, dtype=torch.float64):
 def _fold_ghost_nodes(b: L.Block, bar: np.ndarray) -> np.ndarray:
     """Transpose of layout.extrapolate_ghost_nodes on a [k, j, i] view:
     adjoints of the ghost nodes added to the nodes they copy, in the reverse
-    order of the copies (j layers, then i layers, then k layers)."""
-    a = bar.copy()
+    order of the copies (j layers, then i layers, then k layers).  A numpy
+    array or a torch tensor."""
+    a = bar.clone() if isinstance(bar, torch.Tensor) else bar.copy()
     i1, i2, j1, j2 = b.ii1 - b.id1, b.ii2 - b.id1, b.jj1 - b.jd1, b.jj2 - b.jd1
     k1, k2 = b.kk1 - b.kd1, b.kk2 - b.kd1
     a[:, j1, :] += a[:, j1 - 1, :]; a[:, j1 - 1, :] = 0.
@@ -89,7 +90,10 @@ class AdjointProblem:
         self.S.time_step()
         self.dt = self.S.dt.clone()
         self.x0, self.y0, self.z0 = (np.ascontiguousarray(a).copy() for a in case.nodes)
+        self.xyz0 = tuple(_dev(a) for a in (self.x0, self.y0, self.z0))
         self.wall_nodes = np.asarray(case.wall_nodes)
+        self.wall_nodes_d = i64(self.wall_nodes)
+        self.wall_local = i64(self.wall_nodes - self.b.npc)
         self.wall_cells = i64(w.inner)
         self.wall_sx = i64(self.b.nnn + (w.inner - self.b.npc))  # sn(m, d = j, c = x) of the wall faces
         self.sg = self._orientation()
@@ -112,38 +116,41 @@ class AdjointProblem:
         return out
 
     # -- mesh ----------------------------------------------------------------
-    def nodes(self, delta: np.ndarray):
-        y = self.y0.copy()
-        y[self.wall_nodes] += delta
-        return self.x0.copy(), y, self.z0.copy()
+    def nodes(self, delta):
+        """Node coordinates with the wall nodes moved by delta (y only), as
+        fresh device arrays."""
+        x, y, z = (a.clone() for a in self.xyz0)
+        y.index_add_(0, self.wall_nodes_d, torch.as_tensor(delta, dtype=torch.float64, device="cuda"))
+        return x, y, z
 
-    def metrics_np(self, delta: np.ndarray):
+    def metrics(self, delta):
+        """(sn of the block (9 nnn), vol (ip11), the nodes with their ghost
+        layer) of the mesh moved by delta, on the device."""
         x, y, z = self.nodes(delta)
-        sn, vol = L.metrics(self.lay, x, y, z)
-        b = self.b
-        return sn[9 * b.npc: 9 * (b.npc + b.nnn)], vol
+        sn = torch.empty(9 * self.b.nnn, dtype=torch.float64, device="cuda")
+        vol = torch.zeros(self.lay.ip11, dtype=torch.float64, device="cuda")
+        abi.mesh_metrics(self.blk, x, y, z, sn, vol)
+        return sn, vol, (x, y, z)
 
-    def metric_jvp_np(self, delta: np.ndarray, ddot: np.ndarray):
+    def metric_jvp(self, delta, ddot):
         """Exact derivative of (sn, vol) along the wall displacement ddot."""
-        def at(t):
-            return self.metrics_np(delta + t * ddot)
-        sp, vp = at(1.0); sm, vm = at(-1.0)
-        sh, vh = at(0.5); sl, vl = at(-0.5)
+        sp, vp, _ = self.metrics(delta + ddot)
+        sm, vm, _ = self.metrics(delta - ddot)
+        _, vh, _ = self.metrics(delta + 0.5 * ddot)
+        _, vl, _ = self.metrics(delta - 0.5 * ddot)
         snd = 0.5 * (sp - sm)
         d1 = 0.5 * (vp - vm)
         d2 = vh - vl
         return snd, (4.0 * d2 - d1) / 3.0
 
-    def metric_vjp(self, snbar, volbar):
-        """(snbar, volbar) -> adjoint of the wall displacements."""
-        n = self.lay.ip11
-        xs = [_dev(a) for a in (self.x0, self._ycur, self.z0)]
+    def metric_vjp(self, snbar, volbar, xyz):
+        """(snbar, volbar) -> adjoint of the wall displacements, at the nodes
+        xyz (with their ghost layer) the metrics were formed from."""
         nnn = self.b.nnn
         bars = [torch.zeros(nnn, dtype=torch.float64, device="cuda") for _ in range(3)]
-        abi.adj_metric_vjp(self.blk, *xs, self.sg, snbar.contiguous(), volbar.contiguous(), *bars)
-        yb = bars[1].cpu().numpy().reshape(self.b.nkd, self.b.njd, self.b.nid)
-        yb = _fold_ghost_nodes(self.b, yb).ravel()
-        return yb[self.wall_nodes - self.b.npc]
+        abi.adj_metric_vjp(self.blk, *xyz, self.sg, snbar.contiguous(), volbar.contiguous(), *bars)
+        yb = _fold_ghost_nodes(self.b, bars[1].view(self.b.nkd, self.b.njd, self.b.nid))
+        return yb.reshape(-1)[self.wall_local]
 
     # -- residual ------------------------------------------------------------
     def residual(self, v, sn_blk, vol):
@@ -196,11 +203,11 @@ class AdjointProblem:
         sn_blk, vol = Metrics.apply(delta, self)
         return self.objective(self.rk_step(v0, sn_blk, vol), sn_blk)
 
-    def J_value(self, v0, delta_np):
-        """Forward only, for finite differences (production kernels, host metrics)."""
+    def J_value(self, v0, delta):
+        """Forward only, for finite differences (production kernels, device
+        metrics); delta a numpy array or a device tensor."""
         with torch.no_grad():
-            sn, vol = self.metrics_np(delta_np)
-            sn_blk, vol = _dev(sn), _dev(vol)
+            sn_blk, vol, _ = self.metrics(delta)
             return float(self.objective(self.rk_step(v0, sn_blk, vol), sn_blk))
 
 
@@ -228,27 +235,25 @@ class Residual(torch.autograd.Function):
 class Metrics(torch.autograd.Function):
     @staticmethod
     def forward(ctx, delta, prob):
-        d = delta.detach().cpu().numpy()
-        sn, vol = prob.metrics_np(d)
-        prob._ycur = prob.nodes(d)[1]
+        d = delta.detach()
+        sn, vol, xyz = prob.metrics(d)
         ctx.prob = prob
         ctx.delta = d
-        return _dev(sn), _dev(vol)
+        ctx.xyz = xyz
+        return sn, vol
 
     @staticmethod
     def backward(ctx, snbar, volbar):
         prob = ctx.prob
-        prob._ycur = prob.nodes(ctx.delta)[1]
         if snbar is None:
             snbar = torch.zeros(9 * prob.b.nnn, dtype=torch.float64, device="cuda")
         if volbar is None:
             volbar = torch.zeros(prob.lay.ip11, dtype=torch.float64, device="cuda")
-        return _dev(prob.metric_vjp(snbar, volbar)), None
+        return prob.metric_vjp(snbar, volbar, ctx.xyz), None
 
     @staticmethod
     def jvp(ctx, ddot, _):
-        snd, vold = ctx.prob.metric_jvp_np(ctx.delta, ddot.detach().cpu().numpy())
-        return _dev(snd), _dev(vold)
+        return ctx.prob.metric_jvp(ctx.delta, ddot.detach())
 
 
 def gradient(prob: AdjointProblem, v0, delta):

@@ -101,6 +101,23 @@ __global__ void __launch_bounds__(TX *TY) k_sch_expli(Blk b, const double *__res
   for (int e = 0; e < 5; e++) v[e * ldv + n] = v[e * ldv + n] + dtv * u[e * ldv + n];
 }
 
+// One stage of the 4-stage explicit scheme composed from the single-stage
+// residual: v = v0 - alpha (dt / vol) u on the real cells, nvar planes.
+// The reference's only explicit update is forward Euler, v += dt/vol u with
+// u the net outgoing flux (sch_expli.f90:53-66); the stage takes the
+// physical sign.  No contraction: the product and the difference round
+// separately, as the C restatement does.
+__global__ void __launch_bounds__(TX *TY) k_rk_stage(Blk b, double alpha, const double *__restrict__ v0,
+                                                     const double *__restrict__ u, double *__restrict__ v, long ldv,
+                                                     const double *__restrict__ dt, const double *__restrict__ vol,
+                                                     int nvar) {
+#pragma clang fp contract(off)
+  CELL_IJK(b)
+  long n = b.idx(i, j, k);
+  double c = alpha * (dt[n] / vol[n]);
+  for (int e = 0; e < nvar; e++) v[e * ldv + n] = v0[e * ldv + n] - c * u[e * ldv + n];
+}
+
 extern "C" int codens_zpres(const codens_block *cb, const codens_phys *cp, const double *v, int64_t ldv,
                             double *ps, double *temp, double *cson, void *stream) {
   int rc = codens_check_block(cb, "codens_zpres");
@@ -172,4 +189,21 @@ extern "C" int codens_sch_expli(const codens_block *cb, const double *u, double 
                      dt, vol);
   }
   return codens_check_launch("codens_sch_expli");
+}
+
+extern "C" int codens_rk_stage(const codens_block *cb, double alpha, const double *v0, const double *u, double *v,
+                               int64_t ldv, int32_t nvar, const double *dt, const double *vol, void *stream) {
+  int rc = codens_check_block(cb, "codens_rk_stage");
+  if (rc) return rc;
+  if (!v0 || !u || !v || !dt || !vol || nvar < 1 || nvar > 7) {
+    codens_set_error("codens_rk_stage: invalid argument (null pointer or nvar %d outside 1..7)", nvar);
+    return CODENS_EINVAL;
+  }
+  Blk b = make_blk(cb);
+  {
+    CodensTimed _tm("rk_stage", (hipStream_t)stream);
+    hipLaunchKernelGGL(k_rk_stage, cell_grid(b), dim3(TX, TY), 0, (hipStream_t)stream, b, alpha, v0, u, v,
+                       (long)ldv, dt, vol, (int)nvar);
+  }
+  return codens_check_launch("codens_rk_stage");
 }

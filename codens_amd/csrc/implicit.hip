@@ -11,7 +11,6 @@
 #include "common.hpp"
 
 #include <algorithm>
-#include <type_traits>
 
 #include <array>
 #include <map>
@@ -175,79 +174,21 @@ __device__ inline double rv_of(const Phys &ph, const double *mu, const double *m
 }
 
 // implimf_3d.f90:113-224: spectral-radius coefficients of the faces and the
-// diagonal.  coefe[d] holds the low face of every real cell.  Velocities and
-// rv use one reciprocal of each density (the reference divides each time;
-// the coefficients move by ~1e-16 relative).
-__device__ __forceinline__ void coef_body(const Blk &b, dim3 tile, const Phys &ph, const double *__restrict__ v,
-                                          long ldv, const double *__restrict__ mu, const double *__restrict__ mut,
-                                          const double *__restrict__ dt, const double *__restrict__ vol,
-                                          const double *__restrict__ cson, const double *__restrict__ sn,
-                                          double dtpas, int kfmg, double *__restrict__ coefe,
-                                          double *__restrict__ diag) {
-#pragma clang fp contract(off)  // the same roundings as k_impl_coef_tiled
-  CELL3_AT(b, tile)
-  long n = b.idx(c[0], c[1], c[2]), m = n - b.npc;
-  double rc = 1.0 / v[n];
-  double rvc = rv_of(ph, mu, mut, rc, n);
-  double dg = vol[n] / dt[n];
-  double u0 = v[ldv + n] * rc, v0 = v[2 * ldv + n] * rc, w0 = v[3 * ldv + n] * rc;
-  double cf[6];
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    long inc = b.inc(d);
-    int a = c[d], a1 = b.lo(d), a2 = b.hi(d);
-#pragma unroll
-    for (int side = 0; side < 2; side++) {
-      long nn = side == 0 ? n - inc : n + inc;
-      long mf = side == 0 ? m : m + inc;
-      bool nreal = side == 0 ? (a > a1) : (a + 1 < a2);
-      double s1 = snv(sn, b.nnn, mf, d, 0), s2 = snv(sn, b.nnn, mf, d, 1), s3 = snv(sn, b.nnn, mf, d, 2);
-      double cnds = s1 * s1 + s2 * s2 + s3 * s3;
-      double rn = 1.0 / v[nn];
-      double un = v[ldv + nn] * rn, vn_ = v[2 * ldv + nn] * rn, wn = v[3 * ldv + nn] * rn;
-      double rvn = nreal ? rv_of(ph, mu, mut, rn, nn) : 0.;
-      double uu, vv, ww, cc, rsum, vsum;
-      if (side == 0) {  // face m: cells n (own) and n-inc
-        uu = 0.5 * (u0 + un); vv = 0.5 * (v0 + vn_); ww = 0.5 * (w0 + wn);
-        cc = 0.5 * (cson[n] + cson[nn]); rsum = rvc + rvn; vsum = vol[n] + vol[nn];
-      } else {          // face m+inc: cells n+inc and n (own)
-        uu = 0.5 * (un + u0); vv = 0.5 * (vn_ + v0); ww = 0.5 * (wn + w0);
-        cc = 0.5 * (cson[nn] + cson[n]); rsum = rvn + rvc; vsum = vol[nn] + vol[n];
-      }
-      double vnf = uu * s1 + vv * s2 + ww * s3;
-      cf[2 * d + side] = 0.5 * (fabs(vnf) + sqrt(cnds) * cc) + rsum * cnds / vsum;
-    }
-    coefe[d * b.nnn + m] = cf[2 * d];
-  }
-  dg = dg + cf[0] + cf[1] + cf[2] + cf[3] + cf[4] + cf[5];
-  if (kfmg == 3) dg = dg + 1.5 * vol[n] / dtpas;
-  diag[m] = dg;
-}
-
-__global__ void __launch_bounds__(TX *TY) k_impl_coef(Blk b, Phys ph, const double *__restrict__ v, long ldv,
-                                                      const double *__restrict__ mu,
-                                                      const double *__restrict__ mut,
-                                                      const double *__restrict__ dt,
-                                                      const double *__restrict__ vol,
-                                                      const double *__restrict__ cson,
-                                                      const double *__restrict__ sn, double dtpas, int kfmg,
-                                                      double *__restrict__ coefe, double *__restrict__ diag) {
-  coef_body(b, xcd_block(), ph, v, ldv, mu, mut, dt, vol, cson, sn, dtpas, kfmg, coefe, diag);
-}
-
-// The same coefficients with each face evaluated once.  A workgroup owns a
-// TX x TY tile of real cells and marches CK_KS k-planes.  Per plane, the
-// velocity, sound speed, rv and volume of every cell of the tile and of its
-// 1-cell ring (i0-1..i0+TX, j0-1..j0+TY) go to LDS; each thread evaluates the
-// low i and j faces of its cell, the ring's threads the tile's high faces,
-// and the k face below its cell from its own column's previous plane.  A
-// face's coefficient is the same expression of the same two cells whichever
-// cell owns it: 0.5 (|V.S| + |S| c) + (rv_lo + rv_hi) |S|^2 / (vol_lo + vol_hi)
-// with V and c averaged (high + low) / 2, as coef_body forms it from either
-// side, so the values are coef_body's bit for bit.  A cell's diagonal adds
-// its six faces in the order of implimf_3d.f90:206-215 (i low, i high, j low,
-// j high, k low, k high) and is complete once the next plane has evaluated
-// the k face above it.
+// diagonal; coefe[d] holds the low face of every real cell.  A workgroup
+// owns a TX x TY tile of real cells and marches CK_KS k-planes, so that each
+// face is evaluated once.  Per plane, the velocity, sound speed, rv and
+// volume of every cell of the tile and of its 1-cell ring (i0-1..i0+TX,
+// j0-1..j0+TY) go to LDS; each thread evaluates the low i and j faces of its
+// cell, the ring's threads the tile's high faces, and the k face below its
+// cell from its own column's previous plane.  A face's coefficient is one
+// expression of its two cells whichever tile evaluates it:
+// 0.5 (|V.S| + |S| c) + (rv_lo + rv_hi) |S|^2 / (vol_lo + vol_hi), with V
+// and c averaged (high + low) / 2 and no contraction.  Velocities and rv use
+// one reciprocal of each density (the reference divides each time; the
+// coefficients move by ~1e-16 relative).  A cell's diagonal adds its six
+// faces in the order of implimf_3d.f90:206-215 (i low, i high, j low, j high,
+// k low, k high) and is complete once the next plane has evaluated the k face
+// above it.  Every global load of a plane is issued before its first barrier.
 constexpr int CK_KS = 8;
 constexpr int CK_PL = (TX + 2) * (TY + 2);
 
@@ -273,7 +214,7 @@ __device__ __forceinline__ CoefCell coef_cell(const Blk &b, const Phys &ph, cons
 
 // coefficient of the face between cells lo and hi (normal s of the face)
 __device__ __forceinline__ double coef_face(const CoefCell &hi, const CoefCell &lo, double s1, double s2, double s3) {
-#pragma clang fp contract(off)  // the same roundings as coef_body
+#pragma clang fp contract(off)
   double cnds = s1 * s1 + s2 * s2 + s3 * s3;
   double uu = 0.5 * (hi.u + lo.u), vv = 0.5 * (hi.v + lo.v), ww = 0.5 * (hi.w + lo.w);
   double cc = 0.5 * (hi.c + lo.c), rsum = hi.rv + lo.rv, vsum = hi.vol + lo.vol;
@@ -281,18 +222,23 @@ __device__ __forceinline__ double coef_face(const CoefCell &hi, const CoefCell &
   return 0.5 * (fabs(vnf) + sqrt(cnds) * cc) + rsum * cnds / vsum;
 }
 
-__global__ void __launch_bounds__(TX *TY) k_impl_coef_tiled(Blk b, Phys ph, const double *__restrict__ v, long ldv,
-                                                            const double *__restrict__ mu,
-                                                            const double *__restrict__ mut,
-                                                            const double *__restrict__ dt,
-                                                            const double *__restrict__ vol,
-                                                            const double *__restrict__ cson,
-                                                            const double *__restrict__ sn, double dtpas, int kfmg,
-                                                            double *__restrict__ coefe, double *__restrict__ diag) {
-#pragma clang fp contract(off)  // the same roundings as coef_body
-  __shared__ double L[6][CK_PL];
-  __shared__ double CI[(TX + 1) * TY], CJ[TX * (TY + 1)];
-  const dim3 xb = xcd_block();
+// LDS of one workgroup: the plane's cell operands and the i and j face
+// coefficients of the tile
+struct CoefLds {
+  double L[6][CK_PL];
+  double CI[(TX + 1) * TY], CJ[TX * (TY + 1)];
+};
+
+// the tile xb (x, y: TX x TY columns, z: CK_KS-plane slab) of block b
+__device__ __forceinline__ void coef_tiled_body(const Blk &b, dim3 xb, const Phys &ph, const double *__restrict__ v,
+                                                long ldv, const double *__restrict__ mu,
+                                                const double *__restrict__ mut, const double *__restrict__ dt,
+                                                const double *__restrict__ vol, const double *__restrict__ cson,
+                                                const double *__restrict__ sn, double dtpas, int kfmg,
+                                                double *__restrict__ coefe, double *__restrict__ diag, CoefLds &S) {
+#pragma clang fp contract(off)
+  auto &L = S.L;
+  double *CI = S.CI, *CJ = S.CJ;
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + TX * ty;
   const int i0 = b.i1 + xb.x * TX, j0 = b.j1 + xb.y * TY;
   const int kb = b.k1 + xb.z * CK_KS, ke = min(kb + CK_KS, b.k2);
@@ -320,30 +266,61 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef_tiled(Blk b, Phys ph, cons
   if (own) below = coef_cell(b, ph, v, ldv, mu, mut, vol, cson, i, j, kb - 1);
   double part = 0.;  // the diagonal of the cell below, all but its k high face
   long mb = -1;
+  // the tile's high face of a ring thread: i face at i0+TX of row j0+t
+  // (t < TY), j face at j0+TY of column i0+t-TY (TY <= t < TY + TX), where it
+  // bounds a real cell of the tile; its LDS slot and CI/CJ entry
+  int hdir = -1, hs = 0, hc = 0, hfi = 0, hfj = 0;
+  if (t < TY) {
+    hfj = j0 + t; hfi = i0 + TX;
+    if (hfi <= b.i2 && hfj < b.j2) { hdir = 0; hs = (t + 1) * (TX + 2) + TX + 1; hc = t * (TX + 1) + TX; }
+  } else if (t < TY + TX) {
+    hfi = i0 + t - TY; hfj = j0 + TY;
+    if (hfj <= b.j2 && hfi < b.i2) { hdir = 1; hs = (TY + 1) * (TX + 2) + (t - TY) + 1; hc = TY * TX + (t - TY); }
+  }
+  // the low i face of this thread's cell where it bounds a real cell (rows
+  // j < j2, up to the block's high boundary face i = i2), the low j face
+  // likewise
+  const bool fi = i <= b.i2 && j < b.j2, fj = i < b.i2 && j <= b.j2;
   for (int k = kb; k <= ke; k++) {
     const bool top = k == ke;  // the k face above the slab's last plane only
+    // every global load of the plane, issued before its first barrier so
+    // that their latencies overlap
     if (load) cur = coef_cell(b, ph, v, ldv, mu, mut, vol, cson, i, j, k);
+    CoefCell rq{};
+    const bool rl = !top && rneed;
+    if (rl) rq = coef_cell(b, ph, v, ldv, mu, mut, vol, cson, ri, rj, k);
+    long m = -1;
+    double sk0 = 0., sk1 = 0., sk2 = 0., dtn = 1.;
+    if (own) {
+      m = b.idx(i, j, k) - b.npc;
+      sk0 = snv(sn, b.nnn, m, 2, 0); sk1 = snv(sn, b.nnn, m, 2, 1); sk2 = snv(sn, b.nnn, m, 2, 2);
+      if (!top) dtn = dt[m + b.npc];
+    }
+    const long mo = b.idx(min(i, b.i2), min(j, b.j2), k) - b.npc;
+    double si0 = 0., si1 = 0., si2 = 0., sj0 = 0., sj1 = 0., sj2 = 0., sh0 = 0., sh1 = 0., sh2 = 0.;
+    if (!top && fi) { si0 = snv(sn, b.nnn, mo, 0, 0); si1 = snv(sn, b.nnn, mo, 0, 1); si2 = snv(sn, b.nnn, mo, 0, 2); }
+    if (!top && fj) { sj0 = snv(sn, b.nnn, mo, 1, 0); sj1 = snv(sn, b.nnn, mo, 1, 1); sj2 = snv(sn, b.nnn, mo, 1, 2); }
+    if (!top && hdir >= 0) {
+      long mf = b.idx(hfi, hfj, k) - b.npc;
+      sh0 = snv(sn, b.nnn, mf, hdir, 0); sh1 = snv(sn, b.nnn, mf, hdir, 1); sh2 = snv(sn, b.nnn, mf, hdir, 2);
+    }
     if (!top) {
       if (load) {
         L[0][slot] = cur.u; L[1][slot] = cur.v; L[2][slot] = cur.w;
         L[3][slot] = cur.c; L[4][slot] = cur.rv; L[5][slot] = cur.vol;
       }
-      if (rneed) {
-        CoefCell q = coef_cell(b, ph, v, ldv, mu, mut, vol, cson, ri, rj, k);
-        L[0][rslot] = q.u; L[1][rslot] = q.v; L[2][rslot] = q.w;
-        L[3][rslot] = q.c; L[4][rslot] = q.rv; L[5][rslot] = q.vol;
+      if (rl) {
+        L[0][rslot] = rq.u; L[1][rslot] = rq.v; L[2][rslot] = rq.w;
+        L[3][rslot] = rq.c; L[4][rslot] = rq.rv; L[5][rslot] = rq.vol;
       }
     }
     __syncthreads();
     double ckl = 0.;
-    long m = -1;
     if (own) {
-      long n = b.idx(i, j, k);
-      m = n - b.npc;
-      ckl = coef_face(cur, below, snv(sn, b.nnn, m, 2, 0), snv(sn, b.nnn, m, 2, 1), snv(sn, b.nnn, m, 2, 2));
+      ckl = coef_face(cur, below, sk0, sk1, sk2);
       if (mb >= 0) {  // the cell below is complete
         double dg = part + ckl;
-        if (kfmg == 3) dg = dg + 1.5 * vol[mb + b.npc] / dtpas;
+        if (kfmg == 3) dg = dg + 1.5 * below.vol / dtpas;
         diag[mb] = dg;
       }
     }
@@ -353,51 +330,36 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef_tiled(Blk b, Phys ph, cons
         q.u = L[0][s]; q.v = L[1][s]; q.w = L[2][s]; q.c = L[3][s]; q.rv = L[4][s]; q.vol = L[5][s];
         return q;
       };
-      // the low i face of this thread's cell where it bounds a real cell
-      // (rows j < j2, up to the block's high boundary face i = i2), the low
-      // j face likewise
-      const long mo = b.idx(min(i, b.i2), min(j, b.j2), k) - b.npc;
-      if (i <= b.i2 && j < b.j2)
-        CI[ty * (TX + 1) + tx] = coef_face(lds(slot), lds(slot - 1), snv(sn, b.nnn, mo, 0, 0),
-                                           snv(sn, b.nnn, mo, 0, 1), snv(sn, b.nnn, mo, 0, 2));
-      if (i < b.i2 && j <= b.j2)
-        CJ[ty * TX + tx] = coef_face(lds(slot), lds(slot - (TX + 2)), snv(sn, b.nnn, mo, 1, 0),
-                                     snv(sn, b.nnn, mo, 1, 1), snv(sn, b.nnn, mo, 1, 2));
-      // the tile's high faces: i face at i0+TX of row j0+t (t < TY), j face
-      // at j0+TY of column i0+t-TY (TY <= t < TY + TX), where they bound a
-      // real cell of the tile
-      if (t < TY) {
-        int fj = j0 + t, fi = i0 + TX;
-        if (fi <= b.i2 && fj < b.j2) {
-          long mf = b.idx(fi, fj, k) - b.npc;
-          int s = (t + 1) * (TX + 2) + TX + 1;
-          CI[t * (TX + 1) + TX] = coef_face(lds(s), lds(s - 1), snv(sn, b.nnn, mf, 0, 0), snv(sn, b.nnn, mf, 0, 1),
-                                            snv(sn, b.nnn, mf, 0, 2));
-        }
-      } else if (t < TY + TX) {
-        int fi = i0 + t - TY, fj = j0 + TY;
-        if (fj <= b.j2 && fi < b.i2) {
-          long mf = b.idx(fi, fj, k) - b.npc;
-          int s = (TY + 1) * (TX + 2) + (t - TY) + 1;
-          CJ[TY * TX + (t - TY)] = coef_face(lds(s), lds(s - (TX + 2)), snv(sn, b.nnn, mf, 1, 0),
-                                             snv(sn, b.nnn, mf, 1, 1), snv(sn, b.nnn, mf, 1, 2));
-        }
-      }
+      if (fi) CI[ty * (TX + 1) + tx] = coef_face(lds(slot), lds(slot - 1), si0, si1, si2);
+      if (fj) CJ[ty * TX + tx] = coef_face(lds(slot), lds(slot - (TX + 2)), sj0, sj1, sj2);
+      if (hdir == 0) CI[hc] = coef_face(lds(hs), lds(hs - 1), sh0, sh1, sh2);
+      else if (hdir == 1) CJ[hc] = coef_face(lds(hs), lds(hs - (TX + 2)), sh0, sh1, sh2);
     }
     __syncthreads();
     if (!top && own) {
-      long n = m + b.npc;
       double c0 = CI[ty * (TX + 1) + tx], c1 = CI[ty * (TX + 1) + tx + 1];
       double c2 = CJ[ty * TX + tx], c3 = CJ[(ty + 1) * TX + tx];
       coefe[m] = c0;
       coefe[b.nnn + m] = c2;
       coefe[2 * b.nnn + m] = ckl;
-      part = vol[n] / dt[n];
+      part = cur.vol / dtn;
       part = part + c0 + c1 + c2 + c3 + ckl;
       mb = m;
     }
     below = cur;
   }
+}
+
+__global__ void __launch_bounds__(TX *TY) k_impl_coef_tiled(Blk b, Phys ph, const double *__restrict__ v, long ldv,
+                                                            const double *__restrict__ mu,
+                                                            const double *__restrict__ mut,
+                                                            const double *__restrict__ dt,
+                                                            const double *__restrict__ vol,
+                                                            const double *__restrict__ cson,
+                                                            const double *__restrict__ sn, double dtpas, int kfmg,
+                                                            double *__restrict__ coefe, double *__restrict__ diag) {
+  __shared__ CoefLds S;
+  coef_tiled_body(b, xcd_block(), ph, v, ldv, mu, mut, dt, vol, cson, sn, dtpas, kfmg, coefe, diag, S);
 }
 
 // implimf_eu.f90:124-170: coefficients of the Euler implicit phase,
@@ -459,22 +421,6 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef_eu(Blk b, const double *__
 constexpr int SW_TX = 64, SW_TY = CODENS_SW_TY, SW_KS = CODENS_SW_KS;
 static_assert(2 * SW_TX + 2 * SW_TY <= SW_TX * SW_TY, "one halo cell per thread at most");
 constexpr int SW_HALO = 2 * SW_TX + 2 * SW_TY;
-
-// The increment of the first sub-iteration, formed where it is read instead
-// of stored: d = (-u [- ff]) / diag (implimf_3d.f90:230-267 at ls = 1, the
-// first sub-iteration starting from d = 0), the expression of k_impl_sweep1
-// bit for bit.  The second sub-iteration reads it through this view, so the
-// first needs no pass of its own.
-template <typename T>
-struct FirstView {
-  const T *u, *ff, *diag;
-  long ldv, npc;
-  int ityprk;
-  __device__ inline T at(int e, long n) const {
-    T r = ityprk == 0 ? -u[e * ldv + n] : -u[e * ldv + n] - ff[e * ldv + n];
-    return r / diag[n - npc];
-  }
-};
 
 template <typename T>
 struct Q14T {
@@ -648,9 +594,7 @@ __device__ __forceinline__ void sweep_body(const Blk b, int lg, int ntx, int nty
 #include "sweep_tile.inc"
 }
 
-// FIRST: the second sub-iteration, reading the first one's increment as
-// FirstView forms it from u, ff and diag (dold_mem unused)
-template <bool EU, typename T, int TXS = SW_TX, int TYS = SW_TY, bool FIRST = false>
+template <bool EU, typename T, int TXS = SW_TX, int TYS = SW_TY>
 __global__ void __launch_bounds__(TXS *TYS) SW_ATTR k_impl_sweep_tiled(Blk b, Phys ph, int ityprk, int ntx, int nty,
                                                                    int nkz, const T *__restrict__ u,
                                                                    const T *__restrict__ ff,
@@ -658,12 +602,8 @@ __global__ void __launch_bounds__(TXS *TYS) SW_ATTR k_impl_sweep_tiled(Blk b, Ph
                                                                    const double *__restrict__ ps,
                                                                    const T *__restrict__ sn,
                                                                    const T *__restrict__ coefe,
-                                                                   const T *__restrict__ diag, TView<T> dold_mem,
+                                                                   const T *__restrict__ diag, TView<T> dold,
                                                                    TView<T> dnew, int rev) {
-  using DIN = std::conditional_t<FIRST, FirstView<T>, TView<T>>;
-  DIN dold;
-  if constexpr (FIRST) dold = FirstView<T>{u, ff, diag, ldv, b.npc, ityprk};
-  else dold = dold_mem;
   // XCD-aware tile order: the hardware deals workgroups to the 8 XCDs round
   // robin; give each XCD a contiguous run of tiles so that j/k neighbours
   // share its L2.  rev runs each XCD's tiles in reverse: alternate
@@ -680,16 +620,16 @@ __global__ void __launch_bounds__(TXS *TYS) SW_ATTR k_impl_sweep_tiled(Blk b, Ph
 }
 
 // ---------------------------------------------------------------------------
-// Tile shape of the tiled sub-iteration, per block shape.  Two shapes of the
-// same thread count: SW_TX x SW_TY and 32 x 2 SW_TY (64 x 4 and 32 x 8).
-// Every cell's arithmetic is the same in both, so the increments are
-// identical bit for bit; what differs is how many lanes idle on the i and j
-// edges of a block and the LDS halo per cell.  By default the first call for a block shape times
-// both on the caller's own data (one untimed and two timed launches each,
-// on the caller's stream, writing the same result the real launch writes
-// next) and keeps the faster; while the stream is being captured into a
-// graph, an unseen shape takes the shape with fewer idle lanes and halo
-// cells instead.  codens_sweep_tile forces one shape.
+// Tile shape of the tiled sub-iteration, per block shape: 64 x 4, 32 x 8 or
+// 16 x 16, all of 256 threads.  Every cell's arithmetic is the same in all,
+// so the increments are identical bit for bit; what differs is how many
+// lanes idle on the i and j edges of a block, the LDS halo per cell and the
+// length of the rows each wave reads.  By default the first call for a
+// block shape times each on the caller's own data (one untimed and two
+// timed launches each, on the caller's stream, writing the same result the
+// real launch writes next) and keeps the fastest; while the stream is being
+// captured into a graph, an unseen block shape takes tile_model's choice.
+// codens_sweep_tile forces one shape.
 namespace {
 // shapes 0 (SW_TX x SW_TY), 1 (32 x 2 SW_TY) and 2 (16 x 16), all of 256
 // threads; the all-blocks launch holds the first two (N_BATCH_TILES)
@@ -701,21 +641,27 @@ std::map<std::array<int, 5>, int> g_tile_best;
 inline int tile_tx(int tile) { return tile == 2 ? 16 : (tile == 1 ? 32 : SW_TX); }
 inline int tile_ty(int tile) { return tile == 2 ? 16 : (tile == 1 ? 2 * SW_TY : SW_TY); }
 
-// the shape with the fewest idle lanes and halo cells among the first n
+// The shape with the least modelled time among the first n: the lanes the
+// shape launches over the block's i-j extent (idle lanes past its edges
+// included) times the shape's measured time per lane on a block that none
+// of them leaves idle lanes in (256 x 128 x 64 on MI355X: 64 x 4 134.7 us,
+// 32 x 8 139.2 us, 16 x 16 154.3 us).  The narrower shapes read more halo
+// cells per cell and shorter runs of each row.
 int tile_model(const Blk &b, int n = N_BATCH_TILES) {
+  static const double per_lane[N_SWEEP_TILES] = {1.0, 139.2 / 134.7, 154.3 / 134.7};
   int nx = b.i2 - b.i1, ny = b.j2 - b.j1;
   double best = 0.;
   int pick = 0;
   for (int t = 0; t < n; t++) {
     int tx = tile_tx(t), ty = tile_ty(t);
     double lanes = (double)((nx + tx - 1) / tx * tx) * ((ny + ty - 1) / ty * ty);
-    double cost = lanes * (tx + 2) * (ty + 2) / (double)(tx * ty);
+    double cost = lanes * per_lane[t];
     if (t == 0 || cost < best) { best = cost; pick = t; }
   }
   return pick;
 }
 
-template <bool EU, typename T, bool FIRST = false>
+template <bool EU, typename T>
 void launch_sweep(int tile, const Blk &b, hipStream_t st, Phys ph, int ityprk, const T *u, const T *ff, const T *v,
                   long ldv, const double *ps, const T *sn, const T *coefe, const T *diag, TView<T> in,
                   TView<T> out, int rev = 0) {
@@ -723,21 +669,21 @@ void launch_sweep(int tile, const Blk &b, hipStream_t st, Phys ph, int ityprk, c
   if (tile == 2) {
     constexpr int X = 16, Y = 16;
     int ntx = (b.i2 - b.i1 + X - 1) / X, nty = (b.j2 - b.j1 + Y - 1) / Y;
-    hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, X, Y, FIRST>), dim3(ntx * nty * nkz), dim3(X, Y), 0, st, b, ph,
+    hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, X, Y>), dim3(ntx * nty * nkz), dim3(X, Y), 0, st, b, ph,
                        ityprk, ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
   } else if (tile == 1) {
     constexpr int X = 32, Y = 2 * SW_TY;
     int ntx = (b.i2 - b.i1 + X - 1) / X, nty = (b.j2 - b.j1 + Y - 1) / Y;
-    hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, X, Y, FIRST>), dim3(ntx * nty * nkz), dim3(X, Y), 0, st, b, ph,
+    hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, X, Y>), dim3(ntx * nty * nkz), dim3(X, Y), 0, st, b, ph,
                        ityprk, ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
   } else {
     int ntx = (b.i2 - b.i1 + SW_TX - 1) / SW_TX, nty = (b.j2 - b.j1 + SW_TY - 1) / SW_TY;
-    hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, SW_TX, SW_TY, FIRST>), dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY),
+    hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, SW_TX, SW_TY>), dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY),
                        0, st, b, ph, ityprk, ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
   }
 }
 
-template <bool EU, typename T, bool FIRST = false>
+template <bool EU, typename T>
 int pick_tile(const Blk &b, hipStream_t st, Phys ph, int ityprk, const T *u, const T *ff, const T *v, long ldv,
               const double *ps, const T *sn, const T *coefe, const T *diag, TView<T> in, TView<T> out) {
   std::array<int, 5> key = {b.i2 - b.i1, b.j2 - b.j1, b.k2 - b.k1, (int)EU, (int)sizeof(T)};
@@ -758,10 +704,10 @@ int pick_tile(const Blk &b, hipStream_t st, Phys ph, int ityprk, const T *u, con
   int pick = tile_model(b);
   float best = -1.f;
   for (int t = 0; t < N_SWEEP_TILES; t++) {
-    launch_sweep<EU, T, FIRST>(t, b, st, ph, ityprk, u, ff, v, ldv, ps, sn, coefe, diag, in, out);
+    launch_sweep<EU, T>(t, b, st, ph, ityprk, u, ff, v, ldv, ps, sn, coefe, diag, in, out);
     (void)hipEventRecord(e0, st);
     for (int r = 0; r < 2; r++)
-      launch_sweep<EU, T, FIRST>(t, b, st, ph, ityprk, u, ff, v, ldv, ps, sn, coefe, diag, in, out);
+      launch_sweep<EU, T>(t, b, st, ph, ityprk, u, ff, v, ldv, ps, sn, coefe, diag, in, out);
     (void)hipEventRecord(e1, st);
     float ms = 0.f;
     if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
@@ -886,10 +832,9 @@ static int implimf_common(bool eu, const codens_block *cb, const codens_phys *cp
   }
   DView A = {d, (long)ldv, 0}, B = {W, b.nnn, b.npc};
   int tile = 0;
-  // sub-iteration 1 (d = -u/diag) runs as a pass of its own only when it is
-  // the last; otherwise sub-iteration 2 forms it where it reads it
-  // (FirstView), bit for bit the same values
-  for (int s = (lmx == 1 ? 1 : 2); s <= lmx; s++) {
+  // sub-iteration 1 starts from d = 0: d = -u/diag (k_impl_sweep1); the
+  // others are tiled sweeps, alternate ones in reverse tile order
+  for (int s = 1; s <= lmx; s++) {
     DView out = ((lmx - s) % 2 == 0) ? A : B;
     DView in = ((lmx - s) % 2 == 0) ? B : A;
     const double *fx = ityprk ? ff : u;
@@ -901,20 +846,13 @@ static int implimf_common(bool eu, const codens_block *cb, const codens_phys *cp
       if (s == 2 && fixed)
         tile = known_tile(b, eu, (int)sizeof(double));
       else if (s == 2)
-        tile = eu ? pick_tile<true, double, true>(b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag,
-                                                  in, out)
-                  : pick_tile<false, double, true>(b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag,
-                                                   in, out);
+        tile = eu ? pick_tile<true, double>(b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag, in, out)
+                  : pick_tile<false, double>(b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag, in,
+                                             out);
       CodensTimed _tm(tname("implimf_sweep"), st);
-      if (eu && s == 2)
-        launch_sweep<true, double, true>(tile, b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag, in,
-                                         out, 1);
-      else if (eu)
+      if (eu)
         launch_sweep<true, double>(tile, b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag, in, out,
                                    s % 2 == 0);
-      else if (s == 2)
-        launch_sweep<false, double, true>(tile, b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag,
-                                          in, out, 1);
       else
         launch_sweep<false, double>(tile, b, st, ph, (int)ityprk, u, fx, v, (long)ldv, psx, sn, coefe, diag, in, out,
                                     s % 2 == 0);
@@ -944,7 +882,7 @@ int codens_implimf_3d_with(const codens_block *b, const codens_phys *ph, const d
 // diagonal and ping-pong increment (9 planes of nnn) start at 9 npc of the
 // workspace.  The sub-iterations of small blocks then share the chip instead
 // of each filling a few CUs.
-enum { IB_ZERO, IB_CELL, IB_SWEEP, IB_N };
+enum { IB_ZERO, IB_CELL, IB_SWEEP, IB_COEF, IB_N };
 
 struct ImplEnt {
   Blk b;
@@ -970,12 +908,13 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef_batch(const ImplEnt *__res
                                                             const double *__restrict__ cson,
                                                             const double *__restrict__ sn_all, double dtpas, int kfmg,
                                                             double *ws) {
+  __shared__ CoefLds S;
   const unsigned wg = xcd_linear();
-  const ImplEnt &E = tab[tab_find(tab, nb, IB_CELL, wg)];
+  const ImplEnt &E = tab[tab_find(tab, nb, IB_COEF, wg)];
   const Blk b = E.b;
   double *c = ws + 9 * b.npc;
-  coef_body(b, tab_tile(E, IB_CELL, wg), ph, v, ldv, mu, mut, dt, vol, cson, sn_all + 9 * b.npc, dtpas, kfmg, c,
-            c + 3 * b.nnn);
+  coef_tiled_body(b, tab_tile(E, IB_COEF, wg), ph, v, ldv, mu, mut, dt, vol, cson, sn_all + 9 * b.npc, dtpas, kfmg,
+                  c, c + 3 * b.nnn, S);
 }
 
 __global__ void __launch_bounds__(TX *TY) k_impl_sweep1_batch(const ImplEnt *__restrict__ tab, int nb, int ityprk,
@@ -1054,7 +993,10 @@ extern "C" int codens_implimf_3d_batch(int32_t nblk, const codens_block *blocks,
     }
     const int tx = tile_tx(E.tile), ty = tile_ty(E.tile);
     unsigned ntx = (b.i2 - b.i1 + tx - 1) / tx, nty = (b.j2 - b.j1 + ty - 1) / ty;
-    unsigned g[IB_N][3] = {{(unsigned)((b.nnn + 255) / 256), 1, 1}, {cg.x, cg.y, cg.z}, {ntx, nty, (unsigned)nkz}};
+    unsigned g[IB_N][3] = {{(unsigned)((b.nnn + 255) / 256), 1, 1},
+                           {cg.x, cg.y, cg.z},
+                           {ntx, nty, (unsigned)nkz},
+                           {cg.x, cg.y, (unsigned)((b.k2 - b.k1 + CK_KS - 1) / CK_KS)}};
     for (int p = 0; p < IB_N; p++) {
       E.start[p] = tot[p];
       E.gx[p] = g[p][0];
@@ -1085,7 +1027,7 @@ extern "C" int codens_implimf_3d_batch(int32_t nblk, const codens_block *blocks,
   hipLaunchKernelGGL(k_zero5_batch, dim3(tot[IB_ZERO]), dim3(256), 0, st, ct, nb, d, (long)ldv);
   {
     CodensTimed _tm("implimf_coef", st);
-    hipLaunchKernelGGL(k_impl_coef_batch, dim3(tot[IB_CELL]), dim3(TX, TY), 0, st, ct, nb, ph, (const double *)v,
+    hipLaunchKernelGGL(k_impl_coef_batch, dim3(tot[IB_COEF]), dim3(TX, TY), 0, st, ct, nb, ph, (const double *)v,
                        (long)ldv, mu, mut, dt, vol, cson, sn_all, dtpas, (int)kfmg, ws);
   }
   for (int s = 1; s <= lmx; s++) {
@@ -1161,8 +1103,9 @@ extern "C" int codens_implimf_3d_f32(const codens_block *cb, const codens_phys *
   dim3 g = cell_grid(b), t(TX, TY);
   {
     CodensTimed _tm("implimf32_coef", st);
-    hipLaunchKernelGGL(k_impl_coef, g, t, 0, st, b, ph, v, (long)ldv, mu, mut, dt, vol, cson, sn, dtpas, (int)kfmg,
-                       ws, ws + 3 * nnn);
+    dim3 gc(g.x, g.y, (unsigned)((b.k2 - b.k1 + CK_KS - 1) / CK_KS));
+    hipLaunchKernelGGL(k_impl_coef_tiled, gc, t, 0, st, b, ph, v, (long)ldv, mu, mut, dt, vol, cson, sn, dtpas,
+                       (int)kfmg, ws, ws + 3 * nnn);
     hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, 0L, 4, (const double *)ws, nnn, cF);
     hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, (long)b.npc, 5, v, (long)ldv, vF);
     hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, (long)b.npc, 5, u, (long)ldv, uF);
@@ -1174,23 +1117,19 @@ extern "C" int codens_implimf_3d_f32(const codens_block *cb, const codens_phys *
   TView<float> A = {dA, nnn, b.npc}, B = {dB, nnn, b.npc};
   TView<float> last = A;
   int tile = 0;
-  // as in the FP64 phase: sub-iteration 2 forms sub-iteration 1's increment
-  for (int s = (lmx == 1 ? 1 : 2); s <= lmx; s++) {
+  // as in the FP64 phase: sub-iteration 1 from d = 0, then the tiled sweeps
+  for (int s = 1; s <= lmx; s++) {
     TView<float> out = (s % 2) ? A : B, in = (s % 2) ? B : A;
     if (s == 1) {
       CodensTimed _tm("implimf32_sweep1", st);
       hipLaunchKernelGGL(k_impl_sweep1<float>, g, t, 0, st, b, (int)ityprk, uS, ityprk ? fS : uS, nnn, dgF, out);
     } else {
       if (s == 2)
-        tile = pick_tile<false, float, true>(b, st, ph, (int)ityprk, uS, ityprk ? fS : uS, vS, nnn,
-                                             (const double *)nullptr, snF, cF, dgF, in, out);
+        tile = pick_tile<false, float>(b, st, ph, (int)ityprk, uS, ityprk ? fS : uS, vS, nnn, (const double *)nullptr,
+                                       snF, cF, dgF, in, out);
       CodensTimed _tm("implimf32_sweep", st);
-      if (s == 2)
-        launch_sweep<false, float, true>(tile, b, st, ph, (int)ityprk, uS, ityprk ? fS : uS, vS, nnn,
-                                         (const double *)nullptr, snF, cF, dgF, in, out, 1);
-      else
-        launch_sweep<false, float>(tile, b, st, ph, (int)ityprk, uS, ityprk ? fS : uS, vS, nnn,
-                                   (const double *)nullptr, snF, cF, dgF, in, out, s % 2 == 0);
+      launch_sweep<false, float>(tile, b, st, ph, (int)ityprk, uS, ityprk ? fS : uS, vS, nnn, (const double *)nullptr,
+                                 snF, cF, dgF, in, out, s % 2 == 0);
     }
     last = out;
   }

@@ -89,6 +89,8 @@ class Solver:
             self.g9 = [f(9, b.nnn) for b in lay.blocks]
             self.tprod = [f(b.nnn) for b in lay.blocks]
         self.cycle = 0
+        self.first_ghosts = True  # rfve before the first rbord only
+        self.v0 = None  # the step's initial state (4-stage explicit scheme)
         self.res0 = None
         self.graph = None
         if batched is True:
@@ -166,11 +168,12 @@ class Solver:
 
     def ghosts(self):
         s = self.stream
-        if self.cycle == 0 and self.tq_copy[0].numel():
+        if self.first_ghosts and self.tq_copy[0].numel():
             # first cycle: rfve on the non-coincident frontiers
             # (atsch_num.f90:450-458); rbord then rewrites every ghost but
             # the p, T and c of symmetry ghosts
             abi.rfve(self.tq_copy[0], self.tq_copy[1], self.v, self.ps, self.temp, self.cson, stream=s)
+        self.first_ghosts = False
         g, i, r = self.coin
         if g.numel():
             abi.bc_coincident(g, i, r, self.v, self.ps, self.temp, self.cson, stream=s)
@@ -298,6 +301,40 @@ class Solver:
                 fn(b, self.ph, self.u, self.dt, self.v, self.d, self.ff, self.mu, self.mut, cs.lmax, self.sn_blk[l],
                    self.vol, self.ps, self.cson, stream=s)
         self._per_block(one)
+
+    RK_ALPHA = (0.25, 1.0 / 3.0, 0.5, 1.0)
+
+    def rk4_step(self):
+        """One step of the 4-stage explicit scheme composed from the
+        single-stage residual u (net outgoing flux; the reference's explicit
+        update is forward Euler only, sch_expli.f90:53-66):
+        v_s = v_0 - alpha_s dt/vol u(v_{s-1}), alpha = 1/4, 1/3, 1/2, 1, with
+        the physical sign and the step's dt from the initial state (chronos;
+        the global step when kdtl = 0).  Each stage recomputes the
+        thermodynamics, ghosts, stresses and residual of its state.  Mean
+        flow only."""
+        cs = self.case
+        if cs.turb is not None:
+            raise ValueError("the 4-stage explicit scheme covers the mean flow only")
+        if self.v0 is None:
+            self.v0 = torch.empty_like(self.v)
+        self.thermo()
+        self.time_step()
+        if self.stream is not None:
+            with torch.cuda.stream(self.stream):
+                self.v0.copy_(self.v)
+        else:
+            self.v0.copy_(self.v)
+        for st, a in enumerate(self.RK_ALPHA):
+            if st:
+                self.thermo()
+            self.ghosts()
+            if cs.ns:
+                self.stresses()
+            self.residual()
+            self._per_block(lambda l, b, s: abi.rk_stage(b, a, self.v0, self.u, self.v, self.dt, self.vol, nvar=5,
+                                                         stream=s))
+        self.cycle += 1
 
     def stage(self):
         """One residual + update of all blocks (atsch_num.f90:5); a replay of

@@ -267,6 +267,16 @@ int codens_met_cut(const codens_block *b, int32_t which, double epsk, double eps
 int codens_sch_expli(const codens_block *b, const double *u, double *v, int64_t ldv,
                      const double *dt, const double *vol, void *stream);
 
+/* One stage of the 4-stage explicit scheme built on the single-stage
+ * residual (the reference has forward Euler only, sch_expli.f90:53-66):
+ * v = v0 - alpha * dt/vol * u on the real cells, planes 0..nvar-1 (u, v0
+ * and v share ldv; v may not alias v0 or u).  A step is four calls with
+ * alpha = 1/4, 1/3, 1/2, 1, each on the residual of the previous stage's v
+ * and the step's fixed dt.  The product and the difference round
+ * separately (no fused multiply-add). */
+int codens_rk_stage(const codens_block *b, double alpha, const double *v0, const double *u, double *v,
+                    int64_t ldv, int32_t nvar, const double *dt, const double *vol, void *stream);
+
 /* implimf_3d.f90:4 -- matrix-free point-Jacobi implicit phase: coefficients,
  * lmx sub-iterations, then v += d.  d (5 arrays of ldv) receives the final
  * increment, zero on the ghosts.  ff is read only when ityprk != 0; dtpas
@@ -293,7 +303,7 @@ int codens_implimf_3d(const codens_block *b, const codens_phys *ph, const double
  * held until the process ends); a larger block is refused (CODENS_EINVAL).
  * Every call on a device shares that array: calls that overlap on one device
  * race.  The sub-iteration tile is the forced one (codens_sweep_tile), the
- * one already timed for the block's shape, or the lane-and-halo model's; no
+ * one already timed for the block's shape, or the lane model's; no
  * timing launch is made. */
 int codens_implimf_3d_noalloc(const codens_block *b, const codens_phys *ph, const double *u, const double *dt,
                               double *v, int64_t ldv, double *d, const double *ff, const double *mu, const double *mut,
@@ -410,7 +420,7 @@ int codens_sumsq_rate(const codens_block *blocks, int32_t nb, const double *a, c
  * solve.f90:487-515): when enabled, every launch of the main kernels is
  * bracketed by HIP events on its own stream.  Names: "zpres", "chronos",
  * "zvisqc", "roe_limiter_chain", "roe_face_flux", "roe_gather",
- * "implimf_coef", "implimf_sweep", "implimf_update", "sch_expli".
+ * "implimf_coef", "implimf_sweep", "implimf_update", "sch_expli", "rk_stage".
  * codens_timing_read synchronises on the recorded events and returns the
  * summed duration and launch count of one kernel; reset != 0 clears it. */
 int codens_timing_enable(int32_t on);
@@ -582,8 +592,8 @@ int codens_workspace_set(int32_t set);
  * reference): -1 (default) times the 64 x 4, 32 x 8 and 16 x 16 shapes on
  * the first call for each block shape and keeps the fastest, 0 forces
  * 64 x 4, 1 forces 32 x 8, 2 forces 16 x 16 (the all-blocks entry points
- * hold the first two: there a forced 16 x 16 takes the lane-and-halo model's
- * choice between them).  All give identical results.  *previous (may be
+ * hold the first two: there a forced 16 x 16 takes the lane model's choice
+ * between them: launched lanes times each shape's measured time per lane).  All give identical results.  *previous (may be
  * NULL) receives the mode in force before the call. */
 int codens_sweep_tile(int32_t mode, int32_t *previous);
 
@@ -628,6 +638,18 @@ int codens_adj_residual_vjp(const codens_block *b, const codens_phys *ph, const 
                             int64_t nwall, const int64_t *wall_ghost, const int64_t *wall_inner, int64_t nfar,
                             const int64_t *far_ghost, const int64_t *far_inner, const double *far_nrm,
                             const double *ubar, double *vbar, double *snbar, double *volbar, void *stream);
+
+/* Mesh metrics of one block from its node coordinates x, y, z (storage
+ * arrays, nodes at the cells' flat indices, one ghost node layer on every
+ * side): extmhg.f90:88-125 (ex1 = 1, ex2 = 0) first overwrites the ghost
+ * nodes with copies of the nearest real nodes (k layers, then i, then j);
+ * then norm.f90:102-286 gives sn_blk (9 x nnn, the face area vectors of
+ * every node's low i, j, k faces, zero where a face leaves the storage) and
+ * svol.f90:90-149 the volumes vol (indexed from npc; the last storage layer
+ * of each direction gets 0).  Operations in the reference's order without
+ * contraction. */
+int codens_mesh_metrics(const codens_block *b, double *x, double *y, double *z, double *sn_blk, double *vol,
+                        void *stream);
 
 /* Adjoint of the mesh metrics (norm.f90 face vectors sg_d (d1 x d2) of the
  * half-diagonals, svol.f90 volumes) with respect to the node coordinates

@@ -183,6 +183,20 @@ void orc_sch_expli(const orc_block *b, const double *u, double *v, long ldv, con
   }
 }
 
+/* One stage of the 4-stage explicit scheme composed on the residual (the
+ * reference has the single-stage update above only): v = v0 - alpha dt/vol u
+ * on the real cells, nvar planes, with the physical sign. */
+void orc_rk_stage(const orc_block *b, double alpha, const double *v0, const double *u, double *v, long ldv,
+                  int nvar, const double *dt, const double *vol) {
+  int lo[3], hi[3];
+  ranges(b, 0, b->ii1, b->ii2 - 1, lo, hi);
+  FOR3P(lo, hi) {
+    long n = IDX(b, i, j, k);
+    double c = alpha * (dt[n] / vol[n]);
+    for (int e = 0; e < nvar; e++) v[e * ldv + n] = v0[e * ldv + n] - c * u[e * ldv + n];
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Roe flux of one face from the left/right primitive states, in the explicit
  * P |Lambda| Q form of sch_roe.f90:327-473 / sch_roe_euler.f90:181-320.

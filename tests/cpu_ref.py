@@ -193,6 +193,10 @@ class Orc:
     def sch_expli(self, l, u, v, dt, vol):
         self.lib.orc_sch_expli(self.B(l), ptr(u), ptr(v), self.ld, ptr(dt), ptr(vol))
 
+    def rk_stage(self, l, alpha, v0, u, v, dt, vol):
+        nv = min(v.shape[0], u.shape[0])
+        self.lib.orc_rk_stage(self.B(l), D(alpha), ptr(v0), ptr(u), ptr(v), self.ld, nv, ptr(dt), ptr(vol))
+
     def implimf_3d(self, l, u, dt, v, d, ff, mu, mut, lmx, sn, vol, ps, cson, dtpas=1.0, ityprk=0):
         b = self.lay.blocks[l]
         snb = np.ascontiguousarray(sn_block(sn, b))

@@ -49,18 +49,59 @@ class CpuStage:
             self.other_fr = [k for k, x in enumerate(fr) if x[0] != "rc"]
 
     def stage(self):
-        cs, K, O = self.case, self.K, self.O
+        """One stage of Solver: thermodynamics, time step, ghosts, [turbulence],
+        stresses, residual, then the explicit or the implicit update."""
+        self.thermo()
+        self.time_step()
+        self.residual()
+        cs, K = self.case, self.K
+        for l in range(len(cs.layout.blocks)):
+            if cs.kmf == 0:
+                K.sch_expli(l, self.u, self.v, self.dt, cs.vol)
+            else:
+                K.implimf_3d(l, self.u, self.dt, self.v, self.d, self.ff, self.mu, self.mut, cs.lmax, cs.sn,
+                             cs.vol, self.ps, self.cson)
+
+    RK_ALPHA = (0.25, 1.0 / 3.0, 0.5, 1.0)
+
+    def rk4_step(self):
+        """One step of the 4-stage explicit scheme, in Solver.rk4_step's order:
+        the step's dt from the initial state, then per stage the residual of
+        the previous stage's state and v = v0 - alpha dt/vol u (oracle
+        orc_rk_stage, real cells)."""
+        cs = self.case
+        self.thermo()
+        self.time_step()
+        v0 = self.v.copy()
+        for s, a in enumerate(self.RK_ALPHA):
+            if s:
+                self.thermo()
+            self.residual()
+            for l in range(len(cs.layout.blocks)):
+                self.O.rk_stage(l, a, v0, self.u, self.v, self.dt, cs.vol)
+
+    def thermo(self):
+        cs, K = self.case, self.K
         nb = len(cs.layout.blocks)
         for l in range(nb):
             K.zpres(l, self.v, self.ps, self.temp, self.cson)
         if cs.ns:
             for l in range(nb):
                 K.zvismo(l, self.mu, self.v, self.temp)
-        for l in range(nb):
+
+    def time_step(self):
+        cs, K, O = self.case, self.K, self.O
+        for l in range(len(cs.layout.blocks)):
             K.chronos(l, cs.eta, self.mu, self.mut, self.v, self.dt, cs.sn, cs.vol, self.cson)
         if cs.kdtl == 0:
             stop, dtmin = O.chrono_global(cs.dt1min, self.dt)
             assert not stop, f"global step {dtmin} below dt1min {cs.dt1min}"
+
+    def residual(self):
+        """Ghosts (rfve in the first cycle), turbulence, stresses and the
+        residual u of the current state."""
+        cs, K, O = self.case, self.K, self.O
+        nb = len(cs.layout.blocks)
         if self.first:
             # atsch_num.f90:450-458: rfve on the non-coincident frontiers in
             # the first cycle
@@ -100,12 +141,6 @@ class CpuStage:
                 K.sch_roe(l, 0, self.u, self.v, self.ff, self.tq, cs.sn, self.ps)
             else:
                 K.sch_roe_euler(l, 0, self.u, self.v, self.ff, cs.sn, self.ps)
-        for l in range(nb):
-            if cs.kmf == 0:
-                K.sch_expli(l, self.u, self.v, self.dt, cs.vol)
-            else:
-                K.implimf_3d(l, self.u, self.dt, self.v, self.d, self.ff, self.mu, self.mut, cs.lmax, cs.sn,
-                             cs.vol, self.ps, self.cson)
 
     def rbord(self):
         """rbord.f90:137-429 on the oracle: coincident ghosts, walls, far

@@ -513,3 +513,48 @@ def test_kl_sas(tsetup):
                            np.zeros((4, lay.ip00)), cmu)
             for q in (10, 16, 17):  # cfke, ts6, ts7
                 close(r_[q][inner], g_[q][inner])
+
+
+def test_dropin_stage_rate():
+    """PCIe-inclusive rate of the drop-in path on the 256x128x64 wall block
+    (2,097,152 cells): the routines of an implicit stage that the drop-ins
+    replace -- zpres, zvismo, chronos, sch_roe (minmod), implimf_3d
+    (lmax = 10) -- called with the reference's host arrays, so that every
+    call moves its block windows host -> device and its outputs back.  The
+    rest of the stage (ghost cells, zgrad2/zfluto, which have no drop-in)
+    runs on the oracle between them and is not timed.  Prints the rate over
+    three timed stages (after one untimed) and checks that the state stays
+    physical."""
+    import time
+    import cpu_stage
+    from codens_amd import configs
+    case = configs.wall_ns()
+    st = cpu_stage.CpuStage(case)
+    sc = case.scheme
+    G = C.Ref(case.layout, C.orc_phys(**case.phys), lib=oracle.dropin_lib(), muscl=sc["muscl"], xk=sc["xk"],
+              ilim=sc["ilim"], ns=True)
+    spent = [0.0]
+
+    class Timed:
+        """The stage's kernel calls: the drop-in ones timed, the others the oracle's."""
+        def __getattr__(self, name):
+            if name in ("zpres", "zvismo", "chronos", "sch_roe", "implimf_3d"):
+                fn = getattr(G, name)
+
+                def call(*a, **k):
+                    t0 = time.perf_counter()
+                    fn(*a, **k)
+                    spent[0] += time.perf_counter() - t0
+                return call
+            return getattr(st.O, name)
+    st.K = Timed()
+    st.stage()  # first call: device buffers made, metrics and volumes uploaded
+    spent[0] = 0.0
+    for _ in range(3):
+        st.stage()
+    sec = spent[0] / 3
+    rate = case.layout.ncells / sec / 1e6
+    print("drop-in path, 256x128x64: %.2f ms per stage in the drop-in routines, %.1f M cell-updates/s "
+          "(PCIe-inclusive)" % (1e3 * sec, rate))
+    inner = case.layout.interior()
+    assert np.isfinite(st.v[:, inner]).all() and (st.v[0, inner] > 0).all()

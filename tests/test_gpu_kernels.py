@@ -314,6 +314,58 @@ def test_sch_expli_and_reductions(ctx):
         abi.chrono_global(blocks, 2.0, T(dt))
 
 
+def test_rk_stage(ctx):
+    """codens_rk_stage against orc_rk_stage bit for bit (both round the
+    product and the difference separately), on 5 and 7 planes; ghosts of v
+    untouched."""
+    abi, lay, sn, vol = ctx
+    rng = np.random.default_rng(19)
+    n = lay.ip11
+    O = C.Orc(lay, C.orc_phys())
+    for nv in (5, 7):
+        v0 = rng.standard_normal((nv, n)); u = rng.standard_normal((nv, n)); dt = 0.01 * (1 + rng.random(n))
+        v_o = np.full((nv, n), 3.0); gv = T(v_o)
+        for l, b in enumerate(lay.blocks):
+            O.rk_stage(l, 1.0 / 3.0, v0, u, v_o, dt, vol)
+            abi.rk_stage(abi.block(b), 1.0 / 3.0, T(v0), T(u), gv, T(dt), T(vol), nvar=nv)
+        np.testing.assert_array_equal(H(gv), v_o)
+
+
+def test_mesh_metrics_match_host():
+    """codens_mesh_metrics (ghost nodes, sn, vol on the device) against the
+    host restatement layout.metrics, which is bit-exact against the
+    reference's extmhg/norm/svol (test_oracle_ref.py): equal bit for bit on
+    two jittered blocks, the second mirrored so that every orientation
+    factor is -0.5; the ghost nodes start as garbage and both sides
+    overwrite them."""
+    from codens_amd import abi
+    from codens_amd import layout as L
+    from randcase import random_mesh_xyz
+    lay, _, _, (x, y, z) = random_mesh_xyz([(13, 9, 7), (6, 5, 4)], seed=31, jitter=0.02)
+    b1 = lay.blocks[1]
+    b1.view(x)[:] *= -1.0
+    rng = np.random.default_rng(8)
+    xs = []
+    for a in (x, y, z):
+        a = a.copy()
+        for b in lay.blocks:
+            A = b.view(a)
+            ghost = np.ones(A.shape, bool)
+            ghost[b.kk1 - b.kd1:b.kk2 - b.kd1 + 1, b.jj1 - b.jd1:b.jj2 - b.jd1 + 1, b.ii1 - b.id1:b.ii2 - b.id1 + 1] = False
+            A[ghost] = rng.standard_normal(int(ghost.sum()))
+        xs.append(a)
+    hx, hy, hz = (a.copy() for a in xs)
+    sn_h, vol_h = L.metrics(lay, hx, hy, hz)
+    gx, gy, gz = (T(a) for a in xs)
+    sn_d = torch.zeros(9 * lay.ip11, dtype=torch.float64, device="cuda")
+    vol_d = torch.zeros(lay.ip11, dtype=torch.float64, device="cuda")
+    for b in lay.blocks:
+        abi.mesh_metrics(abi.block(b), gx, gy, gz, sn_d[9 * b.npc: 9 * (b.npc + b.nnn)], vol_d)
+    for g, h in ((gx, hx), (gy, hy), (gz, hz), (sn_d, sn_h), (vol_d, vol_h)):
+        np.testing.assert_array_equal(H(g), h)
+    assert (vol_h[lay.interior()] > 0).all()
+
+
 def test_boundary_ghosts(ctx):
     abi, lay, sn, vol = ctx
     from codens_amd import layout as L

@@ -40,9 +40,9 @@ sys.path.insert(0, ROOT)
 SWEEP_BYTES_PER_CELL = 8 * (5 + 5 + 5 + 9 + 1 + 3 + 5)  # u, v, d_in, sn, diag, coefe, d_out
 # Memory traffic of one sweep launch on the 256x128x64 workload: 2.0 x FETCH_SIZE
 # + 1.0 x WRITE_SIZE (factors from a 512 MiB device copy in the same run),
-# 665,962 KiB = 681.9 MB against 553.6 MB algorithmic (64x4 tiles)
-# (profiles/traffic_implimf_sweep_wall_ns_256x128x64.md).
-SWEEP_TRAFFIC_BYTES = 665962 * 1024
+# 665,124 KiB = 681.1 MB against 553.6 MB algorithmic (64x4 tiles)
+# (profiles/traffic_wall_ns_256x128x64.md).
+SWEEP_TRAFFIC_BYTES = 665124 * 1024
 HBM_PEAK_GBS = 8000.0
 
 
@@ -109,15 +109,17 @@ def main():
     ap.add_argument("--streams", type=int, default=0,
                     help="side streams for the per-block work (0: 4 for multiblock, 1 otherwise)")
     ap.add_argument("--batched", default="auto",
-                    help="phases of all blocks in one launch per pass: auto (none: the per-block work on concurrent "
-                         "streams is faster on the multiblock workload), "
-                         "on (residual,implicit,turbulence), off, or a comma-separated list of those names")
+                    help="phases of all blocks in one launch per pass: auto (the implicit phase on the multiblock "
+                         "workload, the fastest combination there; the residual and turbulence phases stay per block "
+                         "on the concurrent streams), on (residual,implicit,turbulence), off, or a comma-separated list "
+                         "of those names")
     ap.add_argument("--graph", choices=("auto", "on", "off"), default="auto",
                     help="replay stages from a captured HIP graph (auto: on for multiblock)")
     ap.add_argument("--implicit", choices=("fp64", "fp32"), default="fp64",
                     help="implicit phase: implimf_3d in FP64 (the reference's), or its FP32 defect-correction form")
-    ap.add_argument("--residual", choices=("fused", "split"), default="fused",
-                    help="sch_roe as one fused kernel (default) or as face passes + gather")
+    ap.add_argument("--residual", choices=("fused", "split", "persistent"), default="fused",
+                    help="sch_roe as one fused kernel (default), as face passes + gather, or fused with the "
+                         "all-blocks launch persistent over a tile queue")
     ap.add_argument("--sweep-tile", type=int, default=-1, choices=(-1, 0, 1, 2),
                     help="tile of the implicit sub-iteration: -1 timed choice per block shape (default), "
                          "0 64x4, 1 32x8, 2 16x16")
@@ -133,7 +135,7 @@ def main():
     from codens_amd import abi, configs
     from codens_amd.solver import Solver
 
-    abi.residual_path(1 if args.residual == "fused" else 0)
+    abi.residual_path({"split": 0, "fused": 1, "persistent": 2}[args.residual])
     abi.sweep_tile(args.sweep_tile)
     if args.workload == "adjoint":
         return adjoint_bench(args)
@@ -143,7 +145,7 @@ def main():
     ci, cj, ck = (int(x) for x in args.dims.split(","))
     case = configs.multiblock_sa() if multi else configs.wall_ns(ci, cj, ck)
     nstreams = args.streams or (4 if multi else 1)
-    batched = {"auto": "", "on": True, "off": ""}.get(args.batched, args.batched)
+    batched = {"auto": "implicit" if multi else "", "on": True, "off": ""}.get(args.batched, args.batched)
     s = Solver(case, implicit=args.implicit, streams=nstreams, batched=batched)
     ncell = case.layout.ncells
     nblk = len(case.layout.blocks)
@@ -314,7 +316,7 @@ def adjoint_bench(args):
                                "gradient of the wall x force w.r.t. the state and 4096 wall-node displacements",
                    "cells": ncell, "rk_stages": nst},
         "forward_step_ms": round(1e3 * tf, 3),
-        "note": "the gradient's time includes the mesh metrics of the displaced nodes on the host (numpy)",
+        "note": "the gradient's time includes the mesh metrics of the displaced nodes, their JVP and adjoint, all on the device",
         "entry_points_ms": {"residual (production)": round(ev[0].elapsed_time(ev[1]), 3),
                             "codens_adj_residual_jvp": round(ev[1].elapsed_time(ev[2]), 3),
                             "codens_adj_residual_vjp": round(ev[2].elapsed_time(ev[3]), 3)},

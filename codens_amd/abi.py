@@ -481,7 +481,8 @@ def mesh_read(path, blocks_arr, nblk, x, y, z):
 
 def residual_path(fused):
     """Kernel arrangement of sch_roe / sch_roe_euler: 1 fused (default), 0
-    face passes + gather.  Returns the previous arrangement."""
+    face passes + gather, 2 fused with the all-blocks launch persistent over
+    a tile queue.  Returns the previous arrangement."""
     prev = ctypes.c_int32(0)
     _chk(lib.codens_residual_path(int(fused), ctypes.byref(prev)))
     return prev.value

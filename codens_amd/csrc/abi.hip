@@ -101,6 +101,59 @@ void *codens_workspace(size_t bytes, int slot) {
   return s.p;
 }
 
+__global__ void __launch_bounds__(256) k_zero_shell(codens::Blk b, long n0, long n1, double *p, long ld, int np) {
+  const codens::Shell s = codens::make_shell(b);
+  long t = blockIdx.x * 256L + threadIdx.x;
+  if (t >= s.na + s.nb + s.nc) return;
+  long n = codens::shell_entry(b, s, t);
+  if (n < n0 || n > n1) return;
+  for (int e = 0; e < np; e++) p[e * ld + n] = 0.;
+}
+
+void codens_zero_shell(const codens::Blk &b, long n0, long n1, double *p, long ld, int np, hipStream_t st) {
+  long cnt = codens::shell_count(b);
+  if (cnt > 0)
+    hipLaunchKernelGGL(k_zero_shell, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, b, n0, n1, p, ld, np);
+}
+
+namespace {
+std::map<std::pair<int, int>, unsigned *> g_queue;  // (device, workspace set) -> 2 counters
+}  // namespace
+
+unsigned *codens_queue_counters(hipStream_t st, const char *what) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    codens_set_error("%s: no current device", what);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  auto k = std::make_pair(dev, g_ws_set);
+  auto it = g_queue.find(k);
+  if (it != g_queue.end()) return it->second;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+    codens_set_error("%s: first persistent launch while the stream is captured; call it once before", what);
+    return nullptr;
+  }
+  unsigned *p = nullptr;
+  if (hipMalloc(&p, 2 * sizeof(unsigned)) != hipSuccess || hipMemset(p, 0, 2 * sizeof(unsigned)) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess) {
+    if (p) (void)hipFree(p);
+    codens_set_error("%s: tile queue allocation failed", what);
+    return nullptr;
+  }
+  g_queue[k] = p;
+  return p;
+}
+
+int codens_resident_workgroups(int per_cu) {
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+    ncu = 256;
+  return per_cu * ncu;
+}
+
 namespace {
 std::mutex g_tab_mu;
 // device tables by (device, kind, key); made once, kept to the end

@@ -80,6 +80,75 @@ inline dim3 cell_grid(const Blk &b) {
   return dim3((nx + TX - 1) / TX, (ny + TY - 1) / TY, nz);
 }
 
+// The non-real (ghost) entries of a block's storage, in this order: the k
+// planes outside k1..k2-1; the j rows outside j1..j2-1 of the real planes;
+// the i cells outside i1..i2-1 of the real rows.  shell_count(b) is their
+// number, shell_entry(b, t) the storage index of the t-th.  A kernel that
+// clears the ghosts of an array whose real cells another kernel writes
+// then touches only these (an eighth of the block's storage for 64^3
+// cells, more for small blocks).
+struct Shell {
+  long nid, nijd, njd, nkd;
+  int li0, li1, lj0, lj1, lk0, lk1;  // real cells, storage-local, half-open
+  long na, nb, nc;                    // entries of the three parts
+};
+
+__host__ __device__ inline Shell make_shell(const Blk &b) {
+  Shell s;
+  s.nid = b.nid; s.nijd = b.nijd; s.njd = b.nijd / b.nid; s.nkd = b.nnn / b.nijd;
+  s.li0 = b.i1 - b.id1; s.li1 = b.i2 - b.id1;
+  s.lj0 = b.j1 - b.jd1; s.lj1 = b.j2 - b.jd1;
+  s.lk0 = b.k1 - b.kd1; s.lk1 = b.k2 - b.kd1;
+  s.na = (s.lk0 + (s.nkd - s.lk1)) * s.nijd;
+  s.nb = (long)(s.lk1 - s.lk0) * (s.lj0 + (s.njd - s.lj1)) * s.nid;
+  s.nc = (long)(s.lk1 - s.lk0) * (s.lj1 - s.lj0) * (s.li0 + (s.nid - s.li1));
+  return s;
+}
+
+__host__ __device__ inline long shell_count(const Blk &b) {
+  Shell s = make_shell(b);
+  return s.na + s.nb + s.nc;
+}
+
+__host__ __device__ inline long shell_entry(const Blk &b, const Shell &s, long t) {
+  long lk, lj, li;
+  if (t < s.na) {
+    long p = t / s.nijd;
+    lk = p < s.lk0 ? p : s.lk1 + (p - s.lk0);
+    return b.npc + lk * s.nijd + t % s.nijd;
+  }
+  t -= s.na;
+  if (t < s.nb) {
+    long rb = s.lj0 + (s.njd - s.lj1), row = t / s.nid, r = row % rb;
+    lk = s.lk0 + row / rb;
+    lj = r < s.lj0 ? r : s.lj1 + (r - s.lj0);
+    li = t % s.nid;
+  } else {
+    t -= s.nb;
+    long cc = s.li0 + (s.nid - s.li1), q = t / cc, c = t % cc, nj = s.lj1 - s.lj0;
+    lj = s.lj0 + q % nj;
+    lk = s.lk0 + q / nj;
+    li = c < s.li0 ? c : s.li1 + (c - s.li0);
+  }
+  return b.npc + lk * s.nijd + lj * s.nid + li;
+}
+
+// k planes per workgroup of a k-marching kernel: kmax, halved until the
+// grid of tiles_xy i-j tiles has at least 512 workgroups (two per compute
+// unit of MI355X's 256) or the slab is one plane.  A deeper slab re-reads
+// fewer planes below its first; a small block needs more workgroups to fill
+// the chip.  The coefficient and sub-iteration kernels form a slab's first
+// plane with the functions that form the planes they carry, so their values
+// do not depend on the depth (their all-blocks launches keep full slabs).
+// The fused residual forms the k face below a slab in other code than the
+// faces it carries, which may round it differently (a few ulp): its
+// all-blocks launch gives each block the per-block depth.
+inline int slab_depth(long tiles_xy, int nz, int kmax) {
+  int ks = kmax;
+  while (ks > 1 && tiles_xy * ((nz + ks - 1) / ks) < 512) ks /= 2;
+  return ks;
+}
+
 // Several blocks in one flattened 1-D launch per pass.  A device table holds
 // one entry per block (type E: members start[pass], gx[pass], gy[pass]): the
 // block's first workgroup in the launch and its own grid (x, y; z follows).
@@ -111,6 +180,11 @@ __device__ __forceinline__ dim3 tab_tile(const E &e, int pass, unsigned wg) {
 
 }  // namespace codens
 
+// p[e ld + n] = 0 for the ghost entries n of block b within n0..n1, planes
+// e < np (abi.hip)
+__global__ void k_zero_shell(codens::Blk b, long n0, long n1, double *p, long ld, int np);
+void codens_zero_shell(const codens::Blk &b, long n0, long n1, double *p, long ld, int np, hipStream_t st);
+
 // error reporting shared by the entry points (abi.cpp)
 extern "C" void codens_set_error(const char *fmt, ...);
 int codens_check_launch(const char *what);
@@ -122,6 +196,12 @@ void *codens_workspace(size_t bytes, int slot);
 void *codens_device_table(int kind, const std::vector<int32_t> &key, const void *data, size_t bytes,
                           hipStream_t st, const char *what);
 bool codens_timing_on();
+// Two device counters of a persistent launch's tile queue, per device and
+// workspace set, zeroed when made (on the first call, which may not come
+// while st is captured); the launch leaves them at zero again.
+unsigned *codens_queue_counters(hipStream_t st, const char *what);
+// workgroups resident on the current device at per_cu per compute unit
+int codens_resident_workgroups(int per_cu);
 // implimf_3d with the caller's scratch of 9 nnn doubles (nothing allocated)
 int codens_implimf_3d_with(const codens_block *b, const codens_phys *ph, const double *u, const double *dt, double *v,
                            int64_t ldv, double *d, const double *ff, const double *mu, const double *mut, int32_t lmx,

@@ -84,12 +84,6 @@ __global__ void __launch_bounds__(TX *TY) k_chronos(Blk b, Phys ph, double etal,
   dt[n] = d;
 }
 
-// chronos.f90:110-114 zeroes dt over indc(i1,j1,k1)..indc(i2,j2,k2)
-__global__ void k_zero_range(double *p, long n0, long n1) {
-  long n = n0 + blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (n <= n1) p[n] = 0.;
-}
-
 // sch_expli.f90:53-66
 __global__ void __launch_bounds__(TX *TY) k_sch_expli(Blk b, const double *__restrict__ u, double *v, long ldv,
                                                       const double *__restrict__ dt,
@@ -165,7 +159,9 @@ extern "C" int codens_chronos(const codens_block *cb, const codens_phys *cp, dou
   Blk b = make_blk(cb);
   hipStream_t st = (hipStream_t)stream;
   long n0 = b.idx(b.i1, b.j1, b.k1), n1 = b.idx(b.i2, b.j2, b.k2);
-  hipLaunchKernelGGL(k_zero_range, dim3((n1 - n0 + 256) / 256), dim3(256), 0, st, dt, n0, n1);
+  // chronos.f90:110-114 zeroes dt over indc(i1,j1,k1)..indc(i2,j2,k2): the
+  // ghost entries of that range here, the real cells are written below
+  codens_zero_shell(b, n0, n1, dt, 0, 1, st);
   {
     CodensTimed _tm("chronos", st);
     hipLaunchKernelGGL(k_chronos, cell_grid(b), dim3(TX, TY), 0, st, b, make_phys(cp), eta, (int)ns, v,

@@ -175,8 +175,9 @@ __device__ inline double rv_of(const Phys &ph, const double *mu, const double *m
 
 // implimf_3d.f90:113-224: spectral-radius coefficients of the faces and the
 // diagonal; coefe[d] holds the low face of every real cell.  A workgroup
-// owns a TX x TY tile of real cells and marches CK_KS k-planes, so that each
-// face is evaluated once.  Per plane, the velocity, sound speed, rv and
+// owns a TX x TY tile of real cells and marches a slab of up to CK_KS
+// k-planes (fewer on small blocks, slab_depth), so that each face is
+// evaluated once.  Per plane, the velocity, sound speed, rv and
 // volume of every cell of the tile and of its 1-cell ring (i0-1..i0+TX,
 // j0-1..j0+TY) go to LDS; each thread evaluates the low i and j faces of its
 // cell, the ring's threads the tile's high faces, and the k face below its
@@ -235,13 +236,14 @@ __device__ __forceinline__ void coef_tiled_body(const Blk &b, dim3 xb, const Phy
                                                 const double *__restrict__ mut, const double *__restrict__ dt,
                                                 const double *__restrict__ vol, const double *__restrict__ cson,
                                                 const double *__restrict__ sn, double dtpas, int kfmg,
-                                                double *__restrict__ coefe, double *__restrict__ diag, CoefLds &S) {
+                                                double *__restrict__ coefe, double *__restrict__ diag, int ks,
+                                                CoefLds &S) {
 #pragma clang fp contract(off)
   auto &L = S.L;
   double *CI = S.CI, *CJ = S.CJ;
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + TX * ty;
   const int i0 = b.i1 + xb.x * TX, j0 = b.j1 + xb.y * TY;
-  const int kb = b.k1 + xb.z * CK_KS, ke = min(kb + CK_KS, b.k2);
+  const int kb = b.k1 + xb.z * ks, ke = min(kb + ks, b.k2);
   const int i = i0 + tx, j = j0 + ty;
   // own: a real cell; load: a real cell or the ghost past the block's high
   // i or j end, whose faces with the last real cells this thread evaluates
@@ -357,9 +359,10 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef_tiled(Blk b, Phys ph, cons
                                                             const double *__restrict__ vol,
                                                             const double *__restrict__ cson,
                                                             const double *__restrict__ sn, double dtpas, int kfmg,
-                                                            double *__restrict__ coefe, double *__restrict__ diag) {
+                                                            double *__restrict__ coefe, double *__restrict__ diag,
+                                                            int ks) {
   __shared__ CoefLds S;
-  coef_tiled_body(b, xcd_block(), ph, v, ldv, mu, mut, dt, vol, cson, sn, dtpas, kfmg, coefe, diag, S);
+  coef_tiled_body(b, xcd_block(), ph, v, ldv, mu, mut, dt, vol, cson, sn, dtpas, kfmg, coefe, diag, ks, S);
 }
 
 // implimf_eu.f90:124-170: coefficients of the Euler implicit phase,
@@ -583,7 +586,7 @@ __device__ inline void lds_sync() {
 // EU selects the flux increments of implimf_eu (which read ps); the face
 // terms, coefficients and update are common to both implicit phases.
 template <bool EU, typename T, int TXS, int TYS, class DIN>
-__device__ __forceinline__ void sweep_body(const Blk b, int lg, int ntx, int nty, const Phys ph, int ityprk,
+__device__ __forceinline__ void sweep_body(const Blk b, int lg, int ntx, int nty, int ks, const Phys ph, int ityprk,
                                            const T *__restrict__ u, const T *__restrict__ ff,
                                            const T *__restrict__ v, long ldv, const double *__restrict__ ps,
                                            const T *__restrict__ sn, const T *__restrict__ coefe,
@@ -596,7 +599,7 @@ __device__ __forceinline__ void sweep_body(const Blk b, int lg, int ntx, int nty
 
 template <bool EU, typename T, int TXS = SW_TX, int TYS = SW_TY>
 __global__ void __launch_bounds__(TXS *TYS) SW_ATTR k_impl_sweep_tiled(Blk b, Phys ph, int ityprk, int ntx, int nty,
-                                                                   int nkz, const T *__restrict__ u,
+                                                                   int nkz, int ks, const T *__restrict__ u,
                                                                    const T *__restrict__ ff,
                                                                    const T *__restrict__ v, long ldv,
                                                                    const double *__restrict__ ps,
@@ -665,21 +668,24 @@ template <bool EU, typename T>
 void launch_sweep(int tile, const Blk &b, hipStream_t st, Phys ph, int ityprk, const T *u, const T *ff, const T *v,
                   long ldv, const double *ps, const T *sn, const T *coefe, const T *diag, TView<T> in,
                   TView<T> out, int rev = 0) {
-  int nkz = (b.k2 - b.k1 + SW_KS - 1) / SW_KS;
+  const int tx = tile_tx(tile), ty = tile_ty(tile);
+  const int ntx0 = (b.i2 - b.i1 + tx - 1) / tx, nty0 = (b.j2 - b.j1 + ty - 1) / ty;
+  const int ks = slab_depth((long)ntx0 * nty0, b.k2 - b.k1, SW_KS);
+  const int nkz = (b.k2 - b.k1 + ks - 1) / ks;
   if (tile == 2) {
     constexpr int X = 16, Y = 16;
     int ntx = (b.i2 - b.i1 + X - 1) / X, nty = (b.j2 - b.j1 + Y - 1) / Y;
     hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, X, Y>), dim3(ntx * nty * nkz), dim3(X, Y), 0, st, b, ph,
-                       ityprk, ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
+                       ityprk, ntx, nty, nkz, ks, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
   } else if (tile == 1) {
     constexpr int X = 32, Y = 2 * SW_TY;
     int ntx = (b.i2 - b.i1 + X - 1) / X, nty = (b.j2 - b.j1 + Y - 1) / Y;
     hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, X, Y>), dim3(ntx * nty * nkz), dim3(X, Y), 0, st, b, ph,
-                       ityprk, ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
+                       ityprk, ntx, nty, nkz, ks, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
   } else {
     int ntx = (b.i2 - b.i1 + SW_TX - 1) / SW_TX, nty = (b.j2 - b.j1 + SW_TY - 1) / SW_TY;
     hipLaunchKernelGGL((k_impl_sweep_tiled<EU, T, SW_TX, SW_TY>), dim3(ntx * nty * nkz), dim3(SW_TX, SW_TY),
-                       0, st, b, ph, ityprk, ntx, nty, nkz, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
+                       0, st, b, ph, ityprk, ntx, nty, nkz, ks, u, ff, v, ldv, ps, sn, coefe, diag, in, out, rev);
   }
 }
 
@@ -817,7 +823,9 @@ static int implimf_common(bool eu, const codens_block *cb, const codens_phys *cp
   double *coefe = ws, *diag = ws + 3 * b.nnn, *W = ws + 4 * b.nnn;
   long n0 = b.npc, n1 = b.npc + b.nnn - 1;
   auto tname = [fixed](const char *n) { return fixed ? (const char *)nullptr : n; };
-  hipLaunchKernelGGL(k_zero5, dim3((b.nnn + 255) / 256), dim3(256), 0, st, d, (long)ldv, n0, n1);
+  // d = 0 on the ghosts; the last sub-iteration writes every real cell
+  if (lmx >= 1) codens_zero_shell(b, n0, n1, d, (long)ldv, 5, st);
+  else hipLaunchKernelGGL(k_zero5, dim3((b.nnn + 255) / 256), dim3(256), 0, st, d, (long)ldv, n0, n1);
   dim3 g = cell_grid(b), t(TX, TY);
   {
     CodensTimed _tm(tname("implimf_coef"), st);
@@ -825,9 +833,10 @@ static int implimf_common(bool eu, const codens_block *cb, const codens_phys *cp
       hipLaunchKernelGGL(k_impl_coef_eu, g, t, 0, st, b, v, (long)ldv, dt, vol, cson, sn,
                          kfmg == 3 ? 1.5 / dt1min : 0., coefe, diag);
     } else {
-      dim3 gc(g.x, g.y, (unsigned)((b.k2 - b.k1 + CK_KS - 1) / CK_KS));
+      const int ks = slab_depth((long)g.x * g.y, b.k2 - b.k1, CK_KS);
+      dim3 gc(g.x, g.y, (unsigned)((b.k2 - b.k1 + ks - 1) / ks));
       hipLaunchKernelGGL(k_impl_coef_tiled, gc, t, 0, st, b, ph, v, (long)ldv, mu, mut, dt, vol, cson, sn, dtpas,
-                         (int)kfmg, coefe, diag);
+                         (int)kfmg, coefe, diag, ks);
     }
   }
   DView A = {d, (long)ldv, 0}, B = {W, b.nnn, b.npc};
@@ -889,6 +898,7 @@ struct ImplEnt {
   unsigned start[IB_N];         // first workgroup of this block, per pass
   unsigned gx[IB_N], gy[IB_N];  // this block's grid, per pass
   int tile;                     // sub-iteration tile shape of this block (0: 64 x 4, 1: 32 x 8)
+  int ks_coef, ks_sweep;        // k planes per workgroup
 };
 
 __global__ void __launch_bounds__(256) k_zero5_batch(const ImplEnt *__restrict__ tab, int nb, double *d, long ldv) {
@@ -914,7 +924,7 @@ __global__ void __launch_bounds__(TX *TY) k_impl_coef_batch(const ImplEnt *__res
   const Blk b = E.b;
   double *c = ws + 9 * b.npc;
   coef_tiled_body(b, tab_tile(E, IB_COEF, wg), ph, v, ldv, mu, mut, dt, vol, cson, sn_all + 9 * b.npc, dtpas, kfmg,
-                  c, c + 3 * b.nnn, S);
+                  c, c + 3 * b.nnn, E.ks_coef, S);
 }
 
 __global__ void __launch_bounds__(TX *TY) k_impl_sweep1_batch(const ImplEnt *__restrict__ tab, int nb, int ityprk,
@@ -943,11 +953,12 @@ __global__ void __launch_bounds__(SW_TX *SW_TY) k_impl_sweep_batch(const ImplEnt
   double *c = ws + 9 * b.npc;
   DView A = {d, ldv, 0}, B = {c + 4 * b.nnn, b.nnn, b.npc};
   const int lg = (int)(wg - E.start[IB_SWEEP]), ntx = (int)E.gx[IB_SWEEP], nty = (int)E.gy[IB_SWEEP];
+  const double *sn = sn_all + 9 * b.npc;
   if (E.tile == 1)
-    sweep_body<false, double, 32, 2 * SW_TY>(b, lg, ntx, nty, ph, ityprk, u, ff, v, ldv, v, sn_all + 9 * b.npc, c,
+    sweep_body<false, double, 32, 2 * SW_TY>(b, lg, ntx, nty, E.ks_sweep, ph, ityprk, u, ff, v, ldv, v, sn, c,
                                              c + 3 * b.nnn, out_d ? B : A, out_d ? A : B, sq);
   else
-    sweep_body<false, double, SW_TX, SW_TY>(b, lg, ntx, nty, ph, ityprk, u, ff, v, ldv, v, sn_all + 9 * b.npc, c,
+    sweep_body<false, double, SW_TX, SW_TY>(b, lg, ntx, nty, E.ks_sweep, ph, ityprk, u, ff, v, ldv, v, sn, c,
                                             c + 3 * b.nnn, out_d ? B : A, out_d ? A : B, sq);
 }
 
@@ -984,7 +995,6 @@ extern "C" int codens_implimf_3d_batch(int32_t nblk, const codens_block *blocks,
     ImplEnt &E = ent[l];
     E.b = b;
     dim3 cg = cell_grid(b);
-    int nkz = (b.k2 - b.k1 + SW_KS - 1) / SW_KS;
     {
       std::lock_guard<std::mutex> lk(g_tile_mu);
       // the all-blocks launch holds shapes 0 and 1 (a forced 16 x 16 takes
@@ -993,10 +1003,15 @@ extern "C" int codens_implimf_3d_batch(int32_t nblk, const codens_block *blocks,
     }
     const int tx = tile_tx(E.tile), ty = tile_ty(E.tile);
     unsigned ntx = (b.i2 - b.i1 + tx - 1) / tx, nty = (b.j2 - b.j1 + ty - 1) / ty;
+    // full slabs: the launch has workgroups to spare, and these kernels give
+    // the same values at any slab depth
+    E.ks_coef = CK_KS;
+    E.ks_sweep = SW_KS;
+    const unsigned nkz = (unsigned)((b.k2 - b.k1 + E.ks_sweep - 1) / E.ks_sweep);
     unsigned g[IB_N][3] = {{(unsigned)((b.nnn + 255) / 256), 1, 1},
                            {cg.x, cg.y, cg.z},
-                           {ntx, nty, (unsigned)nkz},
-                           {cg.x, cg.y, (unsigned)((b.k2 - b.k1 + CK_KS - 1) / CK_KS)}};
+                           {ntx, nty, nkz},
+                           {cg.x, cg.y, (unsigned)((b.k2 - b.k1 + E.ks_coef - 1) / E.ks_coef)}};
     for (int p = 0; p < IB_N; p++) {
       E.start[p] = tot[p];
       E.gx[p] = g[p][0];
@@ -1099,13 +1114,16 @@ extern "C" int codens_implimf_3d_f32(const codens_block *cb, const codens_phys *
   float *vF = wf, *uF = wf + 5 * nnn, *fF = wf + 10 * nnn, *snF = wf + 15 * nnn, *cF = wf + 24 * nnn;
   float *dgF = cF + 3 * nnn, *dA = wf + 28 * nnn, *dB = wf + 33 * nnn;
   dim3 g1((unsigned)((nnn + 255) / 256)), t1(256);
-  hipLaunchKernelGGL(k_zero5, g1, t1, 0, st, d, (long)ldv, (long)b.npc, (long)(b.npc + nnn - 1));
+  // d = 0 on the ghosts; the update writes every real cell
+  if (lmx >= 1) codens_zero_shell(b, (long)b.npc, (long)(b.npc + nnn - 1), d, (long)ldv, 5, st);
+  else hipLaunchKernelGGL(k_zero5, g1, t1, 0, st, d, (long)ldv, (long)b.npc, (long)(b.npc + nnn - 1));
   dim3 g = cell_grid(b), t(TX, TY);
   {
     CodensTimed _tm("implimf32_coef", st);
-    dim3 gc(g.x, g.y, (unsigned)((b.k2 - b.k1 + CK_KS - 1) / CK_KS));
+    const int ks = slab_depth((long)g.x * g.y, b.k2 - b.k1, CK_KS);
+    dim3 gc(g.x, g.y, (unsigned)((b.k2 - b.k1 + ks - 1) / ks));
     hipLaunchKernelGGL(k_impl_coef_tiled, gc, t, 0, st, b, ph, v, (long)ldv, mu, mut, dt, vol, cson, sn, dtpas,
-                       (int)kfmg, ws, ws + 3 * nnn);
+                       (int)kfmg, ws, ws + 3 * nnn, ks);
     hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, 0L, 4, (const double *)ws, nnn, cF);
     hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, (long)b.npc, 5, v, (long)ldv, vF);
     hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, (long)b.npc, 5, u, (long)ldv, uF);

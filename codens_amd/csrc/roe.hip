@@ -310,23 +310,27 @@ __device__ __forceinline__ void chain_i_body(const Blk &b, dim3 bi, const double
   }
   double *Re = R + (long)e * b.nnn - b.npc;
   __syncthreads();
-  for (int a0 = a1 + 1; a0 <= a2 - 1; a0 += CH_C) {
-    int nc = min(CH_C, a2 - a0);  // faces a0 .. a0+nc-1 of this chunk
-    // q at cells a0+1 .. a0+nc of every line; pass it covers lines
-    // it*(64/CH_C) + lane/CH_C, cell lane%CH_C.  All passes' loads are issued
-    // before the first LDS write (unrolled), clamped to the chunk.
-    constexpr int LPP = 64 / CH_C, NPASS = CH_LINES / LPP;
-    const int c = lane % CH_C, lsub = lane / CH_C;
-    const int cc = min(c, nc - 1);
-    double qv[NPASS];
+  // chunk a0 (faces a0 .. a0+nc-1) needs q at cells a0+1 .. a0+nc of every
+  // line; load pass it covers lines it*(64/CH_C) + lane/CH_C, cell
+  // lane%CH_C, clamped to the chunk.  The next chunk's loads are issued as
+  // soon as the current chunk is in LDS, so they are in flight while its
+  // chains run and its ratios are stored.
+  constexpr int LPP = 64 / CH_C, NPASS = CH_LINES / LPP;
+  const int c = lane % CH_C, lsub = lane / CH_C;
+  auto load_chunk = [&](int a0, double *qv) {
+    const int nc = min(CH_C, a2 - a0), cc = min(c, nc - 1);
 #pragma unroll
     for (int it = 0; it < NPASS; it++) {
       int l = min(it * LPP + lsub, nlines - 1);
       qv[it] = q[base[l] + (a0 + 1 + cc - a1)];
     }
+  };
+  auto run_chunk = [&](int a0, const double *qv, double *qn) {
+    const int nc = min(CH_C, a2 - a0);
 #pragma unroll
     for (int it = 0; it < NPASS; it++) qs[(it * LPP + lsub) * (CH_C + 1) + c] = qv[it];
     __syncthreads();
+    if (a0 + CH_C <= a2 - 1) load_chunk(a0 + CH_C, qn);
     if (own) {
 #pragma unroll
       for (int k = 0; k < CH_C; k++) {
@@ -342,6 +346,16 @@ __device__ __forceinline__ void chain_i_body(const Blk &b, dim3 bi, const double
       if (l < nlines && c < nc) Re[base[l] + (a0 + c - a1)] = qs[l * (CH_C + 1) + c];
     }
     __syncthreads();
+  };
+  double qa[NPASS], qb[NPASS];
+  int a0 = a1 + 1;
+  if (a0 <= a2 - 1) load_chunk(a0, qa);
+  while (a0 <= a2 - 1) {
+    run_chunk(a0, qa, qb);
+    a0 += CH_C;
+    if (a0 > a2 - 1) break;
+    run_chunk(a0, qb, qa);
+    a0 += CH_C;
   }
 }
 
@@ -442,7 +456,8 @@ __device__ __forceinline__ void face_flux_body(const Blk &b, dim3 xb, const Phys
   for (int e = 0; e < 5; e++) H[e * b.nnn + m] = f[e];
 }
 
-__global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(CODENS_FF_WAVES))) k_face_flux(Blk b, Phys ph, int d, int ns, int ilim, double muscl,
+__global__ void __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(CODENS_FF_WAVES)))
+k_face_flux(Blk b, Phys ph, int d, int ns, int ilim, double muscl,
                                                       double xk, const double *__restrict__ v, long ldv,
                                                       const double *__restrict__ ps,
                                                       const double *__restrict__ tq,
@@ -503,7 +518,9 @@ __global__ void __launch_bounds__(256) k_gather(Blk b, long n0, long n1, const d
 //   u = (F_i(hi) - F_i(lo)) + (F_j(hi) - F_j(lo)) + (F_k(hi) - F_k(lo)),
 // so each face value, and u, are those of passes 2-3 (each face is the same
 // function of the same operands).  The k face between two slabs is evaluated
-// by both (one extra face in FS per cell).
+// by both (one extra face per slab and cell).  A slab is FS planes deep on
+// large blocks, shallower on small ones (slab_depth), in the per-block and
+// the all-blocks launch alike.
 //
 // Faces on the block's boundary (a = a1, a2) are the physical flux of the
 // ghost state (sch_roe.f90:497-578), read from HBM (they are few).  The
@@ -512,7 +529,7 @@ __global__ void __launch_bounds__(256) k_gather(Blk b, long n0, long n1, const d
 constexpr int FT = 16;       // tile cells per direction (i, j)
 constexpr int FPP = FT + 4;  // pitch of the primitive plane (halo 2)
 constexpr int FQP = FT + 2;  // pitch of the stress plane (halo 1)
-constexpr int FS = 8;        // k planes per workgroup
+constexpr int FS = 8;        // k planes per workgroup, at most
 
 struct FusedArgs {
   Blk b;
@@ -523,6 +540,7 @@ struct FusedArgs {
   const double *ps, *tq, *sn, *rr;
   double *u, *ff;
   int ityprk;
+  int ks;  // k planes per workgroup
 };
 
 // primitive variables of storage entry n: rho, u, v, w (the divisions of
@@ -561,7 +579,7 @@ __device__ __forceinline__ void fused_body(const FusedArgs &A, dim3 t) {
   __shared__ double Fj[5][FT * (FT + 1)];
   const int tid = threadIdx.x, tx = tid % FT, ty = tid / FT;
   const int i0 = b.i1 + (int)t.x * FT, j0 = b.j1 + (int)t.y * FT;
-  const int kb = b.k1 + (int)t.z * FS, ke = min(kb + FS, b.k2);
+  const int kb = b.k1 + (int)t.z * A.ks, ke = min(kb + A.ks, b.k2);
   const int njd = (int)(b.nijd / b.nid), nkd = (int)(b.nnn / b.nijd);
   const int id2 = b.id1 + (int)b.nid - 1, jd2 = b.jd1 + njd - 1, kd2 = b.kd1 + nkd - 1;
   auto cl = [](int x, int lo, int hi) { return min(max(x, lo), hi); };
@@ -759,21 +777,20 @@ __device__ __forceinline__ void zero_ghost_body(const Blk &b, unsigned bx, long 
   for (int e = 0; e < 5; e++) u[e * ldv + n] = 0.;
 }
 
-__global__ void __launch_bounds__(256) k_zero_ghost(Blk b, long n0, long n1, double *u, long ldv) {
-  zero_ghost_body(b, blockIdx.x, n0, n1, u, ldv);
-}
 
-inline dim3 fused_grid(const Blk &b) {
+inline dim3 fused_grid(const Blk &b, int ks) {
   int nx = b.i2 - b.i1, ny = b.j2 - b.j1, nz = b.k2 - b.k1;
-  return dim3((nx + FT - 1) / FT, (ny + FT - 1) / FT, (nz + FS - 1) / FS);
+  return dim3((nx + FT - 1) / FT, (ny + FT - 1) / FT, (nz + ks - 1) / ks);
 }
 
-// Residual path: 1 (default) the fused kernel, 0 the face passes + gather.
+// Residual path: 1 (default) the fused kernel, 0 the face passes + gather,
+// 2 the fused kernel with the all-blocks launch persistent over a tile queue.
 static int g_residual_fused = 1;
 
 extern "C" int codens_residual_path(int32_t fused, int32_t *previous) {
-  if (fused != 0 && fused != 1) {
-    codens_set_error("codens_residual_path: 0 (face passes + gather) or 1 (fused), not %d", fused);
+  if (fused < 0 || fused > 2) {
+    codens_set_error("codens_residual_path: 0 (face passes + gather), 1 (fused) or 2 (fused, persistent all-blocks "
+                     "launch), not %d", fused);
     return CODENS_EINVAL;
   }
   if (previous) *previous = g_residual_fused;
@@ -793,7 +810,7 @@ static int roe_common(const codens_block *cb, const codens_phys *cp, const coden
   Blk b = make_blk(cb);
   Phys ph = make_phys(cp);
   int ilim = ns ? sc->ilim : 0;
-  const bool fused = g_residual_fused == 1;
+  const bool fused = g_residual_fused >= 1;
   double *H = fused ? nullptr : (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 0);
   double *R = ilim == 1 ? (double *)codens_workspace(sizeof(double) * 15 * b.nnn, 1) : nullptr;
   const bool need_uvw = !fused || ilim == 1;  // the chains read u, v, w planes
@@ -817,16 +834,18 @@ static int roe_common(const codens_block *cb, const codens_phys *cp, const coden
   }
   long n0 = b.idx(b.i1 - 1, b.j1 - 1, b.k1 - 1), n1 = b.idx(b.i2, b.j2, b.k2);
   if (fused) {
-    FusedArgs A{b, ph, 0.25 * sc->muscl, sc->xk, v, ldv, ps, ns ? tq : v, sn, R, u, ff, ityprk};
+    dim3 g = fused_grid(b, FS);
+    const int ks = slab_depth((long)g.x * g.y, b.k2 - b.k1, FS);
+    g = fused_grid(b, ks);
+    FusedArgs A{b, ph, 0.25 * sc->muscl, sc->xk, v, ldv, ps, ns ? tq : v, sn, R, u, ff, ityprk, ks};
     {
       CodensTimed _tm("roe_fused", st);
-      dim3 g = fused_grid(b);
       if (ns && ilim == 1) hipLaunchKernelGGL((k_residual_fused<true, true>), g, dim3(FT * FT), 0, st, A);
       else if (ns) hipLaunchKernelGGL((k_residual_fused<true, false>), g, dim3(FT * FT), 0, st, A);
       else hipLaunchKernelGGL((k_residual_fused<false, false>), g, dim3(FT * FT), 0, st, A);
     }
     CodensTimed _tm("roe_zero_ghost", st);
-    hipLaunchKernelGGL(k_zero_ghost, dim3((unsigned)((n1 - n0 + 256) / 256)), dim3(256), 0, st, b, n0, n1, u, ldv);
+    codens_zero_shell(b, n0, n1, u, ldv, 5, st);
     return codens_check_launch(what);
   }
   for (int d = 0; d < 3; d++) {
@@ -874,6 +893,7 @@ struct BatchEnt {
   long n0, n1;                  // gather range
   unsigned start[BP_N];         // first workgroup of this block, per pass
   unsigned gx[BP_N], gy[BP_N];  // this block's grid, per pass
+  int ks;                       // k planes per workgroup of the fused pass
 };
 
 __global__ void __launch_bounds__(256) k_prim_uvw_batch(const BatchEnt *__restrict__ tab, int nb,
@@ -919,8 +939,45 @@ k_residual_fused_batch(const BatchEnt *__restrict__ tab, int nb, Phys ph, double
   const unsigned wg = xcd_linear();
   const BatchEnt &E = tab[tab_find(tab, nb, BP_FUSED, wg)];
   const long np = E.b.npc;
-  FusedArgs A{E.b, ph, cm, xk, v, ldv, ps, tq, sn_all + 9 * np, LIM ? rr + 15 * np : nullptr, u, ff, ityprk};
+  FusedArgs A{E.b, ph, cm, xk, v, ldv, ps, tq, sn_all + 9 * np, LIM ? rr + 15 * np : nullptr, u, ff, ityprk, E.ks};
   fused_body<NS, LIM>(A, tab_tile(E, BP_FUSED, wg));
+}
+
+// The same all-blocks residual as one persistent launch: about two
+// workgroups per CU take the tiles of the flattened grid (block by block, in
+// the table's order) from a queue, a device counter, until it runs dry.  The
+// last workgroup to leave resets the counters, so every launch finds them at
+// zero and records no memset (graph replays see the same start).  A tile's
+// work is k_residual_fused_batch's for the same workgroup index, so the
+// residual is the same bit for bit.  Calls that overlap on one device must
+// use different workspace sets (each set has its own counters).
+template <bool NS, bool LIM>
+__global__ void __launch_bounds__(FT *FT) __attribute__((amdgpu_waves_per_eu(2)))
+k_residual_fused_queue(const BatchEnt *__restrict__ tab, int nb, unsigned ntiles, unsigned *queue, Phys ph,
+                       double cm, double xk, const double *__restrict__ v, long ldv, const double *__restrict__ ps,
+                       const double *__restrict__ tq, const double *__restrict__ sn_all,
+                       const double *__restrict__ rr, double *u, double *ff, int ityprk) {
+  __shared__ unsigned next;
+  for (;;) {
+    if (threadIdx.x == 0) next = atomicAdd(&queue[0], 1u);
+    __syncthreads();
+    const unsigned wg = next;
+    __syncthreads();  // read by every thread before thread 0 takes the next tile
+    if (wg >= ntiles) break;
+    const BatchEnt &E = tab[tab_find(tab, nb, BP_FUSED, wg)];
+    const long np = E.b.npc;
+    FusedArgs A{E.b, ph, cm, xk, v, ldv, ps, tq, sn_all + 9 * np, LIM ? rr + 15 * np : nullptr, u, ff, ityprk, E.ks};
+    fused_body<NS, LIM>(A, tab_tile(E, BP_FUSED, wg));
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    // every workgroup has left its loop once the count reaches the grid:
+    // nothing takes from the queue any more
+    if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+      atomicExch(&queue[0], 0u);
+      atomicExch(&queue[1], 0u);
+    }
+  }
 }
 
 // u = 0 on the non-real entries of every block's gather range
@@ -972,7 +1029,12 @@ extern "C" int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, co
                            {(unsigned)((E.n1 - E.n0 + 256) / 256), 1, 1},
                            {0, 0, 0}};
     {
-      dim3 fg = fused_grid(b);
+      // the slab depth of the per-block launch: the block's residual is then
+      // the per-block one bit for bit (a slab's first k face is formed by
+      // other code than the faces it carries, which may round differently)
+      dim3 fg = fused_grid(b, FS);
+      E.ks = slab_depth((long)fg.x * fg.y, b.k2 - b.k1, FS);
+      fg = fused_grid(b, E.ks);
       g[BP_FUSED][0] = fg.x; g[BP_FUSED][1] = fg.y; g[BP_FUSED][2] = fg.z;
     }
     for (int d = 0; d < 3; d++) {
@@ -1000,7 +1062,7 @@ extern "C" int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, co
   if (!tab) return CODENS_EHIP;
   Phys ph = make_phys(cp);
   const long tot_n = span[nblk - 1].second;
-  const bool fused = g_residual_fused == 1;
+  const bool fused = g_residual_fused >= 1;
   const bool need_uvw = !fused || ilim == 1;
   double *H = fused ? nullptr : (double *)codens_workspace(sizeof(double) * 15 * tot_n, 0);
   double *R = ilim == 1 ? (double *)codens_workspace(sizeof(double) * 15 * tot_n, 1) : nullptr;
@@ -1022,9 +1084,23 @@ extern "C" int codens_sch_roe_batch(int32_t nblk, const codens_block *blocks, co
                        v, (const double *)uvw, ps, R);
   }
   if (fused) {
-    {
+    const double cm = 0.25 * sc->muscl;
+    if (g_residual_fused == 2) {
+      unsigned *queue = codens_queue_counters(st, what);
+      if (!queue) return CODENS_EHIP;
+      const unsigned ntiles = tot[BP_FUSED];
+      const unsigned nwg = std::min<unsigned>(ntiles, (unsigned)codens_resident_workgroups(2));
       CodensTimed _tm("roe_fused", st);
-      const double cm = 0.25 * sc->muscl;
+      if (ilim == 1)
+        hipLaunchKernelGGL((k_residual_fused_queue<true, true>), dim3(nwg), dim3(FT * FT), 0, st,
+                           (const BatchEnt *)tab, (int)nblk, ntiles, queue, ph, cm, sc->xk, v, (long)ldv, ps, tq,
+                           sn_all, (const double *)R, u, ff, (int)ityprk);
+      else
+        hipLaunchKernelGGL((k_residual_fused_queue<true, false>), dim3(nwg), dim3(FT * FT), 0, st,
+                           (const BatchEnt *)tab, (int)nblk, ntiles, queue, ph, cm, sc->xk, v, (long)ldv, ps, tq,
+                           sn_all, (const double *)R, u, ff, (int)ityprk);
+    } else {
+      CodensTimed _tm("roe_fused", st);
       if (ilim == 1)
         hipLaunchKernelGGL((k_residual_fused_batch<true, true>), dim3(tot[BP_FUSED]), dim3(FT * FT), 0, st,
                            (const BatchEnt *)tab, (int)nblk, ph, cm, sc->xk, v, (long)ldv, ps, tq, sn_all,

@@ -1,5 +1,6 @@
 // Viscous stresses and heat flux (zvisqc -> zgrad2 + zfluto) on gfx950.
-// A workgroup owns a TX x TY tile of real cells and marches ZV_KS k-planes.
+// A workgroup owns a TX x TY tile of real cells and marches a slab of up to
+// ZV_KS k-planes (fewer on small blocks, slab_depth).
 // Per plane, the velocities (rho u)/rho (one division each, as
 // zgrad2.f90:161-180 forms vx, vy, vz) and T of the tile and of its 1-cell
 // ring go to LDS for the i and j neighbours; the own column's planes k-1
@@ -29,12 +30,13 @@ __global__ void __launch_bounds__(TX *TY) k_zvisqc_tiled(Blk b, Phys ph, const d
                                                          const double *__restrict__ v, long ldv,
                                                          const double *__restrict__ temp,
                                                          const double *__restrict__ mu,
-                                                         const double *__restrict__ mut, double *__restrict__ tq) {
+                                                         const double *__restrict__ mut, double *__restrict__ tq,
+                                                         int ks) {
   __shared__ double Lq[4][ZV_PL];
   const dim3 xb = xcd_block();
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + TX * ty;
   const int i0 = b.i1 + xb.x * TX, j0 = b.j1 + xb.y * TY;
-  const int kb = b.k1 + xb.z * ZV_KS, ke = min(kb + ZV_KS, b.k2);
+  const int kb = b.k1 + xb.z * ks, ke = min(kb + ks, b.k2);
   const int i = i0 + tx, j = j0 + ty;
   const bool own = i < b.i2 && j < b.j2;
   // ring cell of this thread (t < 2 TX + 2 TY), loaded where it lies in the
@@ -176,8 +178,9 @@ extern "C" int codens_zvisqc(const codens_block *cb, const codens_phys *cp, cons
   Blk b = make_blk(cb);
   CodensTimed _tm("zvisqc", (hipStream_t)stream);
   dim3 g = cell_grid(b);
-  g.z = (unsigned)((b.k2 - b.k1 + ZV_KS - 1) / ZV_KS);
+  const int ks = slab_depth((long)g.x * g.y, b.k2 - b.k1, ZV_KS);
+  g.z = (unsigned)((b.k2 - b.k1 + ks - 1) / ks);
   hipLaunchKernelGGL(k_zvisqc_tiled, g, dim3(TX, TY), 0, (hipStream_t)stream, b, make_phys(cp), sn, vol, v, (long)ldv,
-                     temp, mu, mut, tq);
+                     temp, mu, mut, tq, ks);
   return codens_check_launch("codens_zvisqc");
 }

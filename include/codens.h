@@ -593,17 +593,23 @@ int codens_workspace_set(int32_t set);
  * the first call for each block shape and keeps the fastest, 0 forces
  * 64 x 4, 1 forces 32 x 8, 2 forces 16 x 16 (the all-blocks entry points
  * hold the first two: there a forced 16 x 16 takes the lane model's choice
- * between them: launched lanes times each shape's measured time per lane).  All give identical results.  *previous (may be
- * NULL) receives the mode in force before the call. */
+ * between them: launched lanes times each shape's measured time per lane).
+ * All give identical results.  *previous (may be NULL) receives the mode in
+ * force before the call. */
 int codens_sweep_tile(int32_t mode, int32_t *previous);
 
 /* Kernel arrangement of codens_sch_roe and codens_sch_roe_euler (no
  * counterpart in the reference): 1 (default) one fused kernel that stages
  * an i-j tile of the state in LDS, marches k, evaluates every face once and
  * accumulates the residual of its cells; 0 one face-flux pass per direction
- * into a face array, then a gather.  Both evaluate the same face values and
- * sum them in the same order.  *previous (may be NULL) receives the
- * arrangement in force before the call. */
+ * into a face array, then a gather; 2 as 1, with codens_sch_roe_batch's
+ * fused pass launched persistent: about two workgroups per compute unit
+ * take the tiles of every block from a queue (a device counter per
+ * workspace set, zeroed when made and left at zero by each launch, so the
+ * launch can be captured into a graph after a first eager call).  All
+ * evaluate the same face values and sum them in the same order.
+ * *previous (may be NULL) receives the arrangement in force before the
+ * call. */
 int codens_residual_path(int32_t fused, int32_t *previous);
 
 /* The tile shape used for a block (eu: implimf_eu, fp32: the FP32 form):

@@ -101,7 +101,9 @@ def test_roe_residual_multitile(ns, ilim):
     residual, with partial tiles and slabs on every high edge (37x21x19,
     16x16x8, 17x33x9): the fused kernel against the oracle within 1e-12, and
     against the face passes + gather within 1e-14 (the same face values,
-    summed in the same order)."""
+    summed in the same order); with viscous terms, the all-blocks launch
+    (one grid, and the persistent tile queue) against the per-block fused
+    kernel bit for bit."""
     from codens_amd import abi
     lay, sn, vol = random_mesh([(37, 21, 19), (16, 16, 8), (17, 33, 9)], seed=23)
     rng = np.random.default_rng(31 + ilim)
@@ -138,6 +140,21 @@ def test_roe_residual_multitile(ns, ilim):
         assert relerr(res[1][0][e, w], u_o[e, w]) <= TOL, f"u[{e}]"
         assert relerr(res[1][1][e], ff_o[e]) <= TOL, f"ff[{e}]"
         assert relerr(res[1][0][e, w], res[0][0][e, w]) <= 1e-14, f"fused vs split u[{e}]"
+    if ns:
+        # all blocks in one launch, per tile grid and as the persistent
+        # tile-queue launch (twice: the second finds the queue the first left):
+        # the per-block fused residual bit for bit
+        arr = abi.block_array([abi.block(b) for b in lay.blocks])
+        for path in (1, 2):
+            prev = abi.residual_path(path)
+            try:
+                for rep in range(2):
+                    gu = T(np.full((5, n), 3.0)); gff = T(np.ones((5, n)))
+                    abi.sch_roe_batch(arr, len(lay.blocks), ph, sc, 1, gu, T(v), gff, T(tq), T(sn), T(ps))
+                    np.testing.assert_array_equal(H(gu), res[1][0])
+                    np.testing.assert_array_equal(H(gff), res[1][1])
+            finally:
+                abi.residual_path(prev)
     # both write exactly indc(i1-1,j1-1,k1-1)..indc(i2,j2,k2), ghosts zero
     for b in lay.blocks:
         n0, n1 = b.idx(b.ii1 - 1, b.jj1 - 1, b.kk1 - 1), b.idx(b.ii2, b.jj2, b.kk2)

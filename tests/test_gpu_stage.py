@@ -346,11 +346,23 @@ def test_residual_floor_is_the_references():
     assert 3e-6 < min(ref.values()) < 5e-6
 
 
-def test_batched_residual_matches_blockwise():
+@pytest.mark.parametrize("path", [1, 2])
+def test_batched_residual_matches_blockwise(path):
     """The residual of all blocks in one launch per pass
-    (codens_sch_roe_batch) equals the block-by-block residual bit for bit,
-    alone and over 3 stages (batched residual and implicit phase, concurrent
+    (codens_sch_roe_batch; path 2: the fused pass as the persistent
+    tile-queue launch) equals the block-by-block residual bit for bit, alone
+    and over 3 stages (batched residual and implicit phase, concurrent
     streams, graph replay), on the multi-block SA case with minmod."""
+    from codens_amd import abi, configs
+    from codens_amd.solver import Solver
+    prev = abi.residual_path(path)
+    try:
+        _batched_residual_case()
+    finally:
+        abi.residual_path(prev)
+
+
+def _batched_residual_case():
     from codens_amd import configs
     from codens_amd.solver import Solver
     case = configs.multiblock_sa(9, 4, 11, None, seed=7, ilim=1, lmax=4)

@@ -243,12 +243,14 @@ def main():
 
 
 def converge_bench(args):
+    import numpy as np
     import torch
     from codens_amd import configs
     from codens_amd.solver import Solver
     case = configs.adjoint_wall(32, 16, 8, eta=10.0, amp=0.0, bump=0.0)
     ncell = case.layout.ncells
-    runs = {}
+    runs, fields = {}, {}
+    inner = case.layout.interior()
     for mode in ("fp64", "fp32"):
         s = Solver(case, implicit=mode)
         torch.cuda.synchronize()
@@ -258,6 +260,15 @@ def converge_bench(args):
         dt = time.perf_counter() - t0
         runs[mode] = {"iterations": it, "seconds": round(dt, 3), "drop": hist[-1][1],
                       "Mcell_updates_per_s": round(ncell * it / dt / 1e6, 3)}
+        w = s.host_state()[:5, inner]
+        g1 = case.phys["gam"] - 1.0
+        fields[mode] = {"rho": w[0], "u": w[1] / w[0],
+                        "p": g1 * (w[4] - 0.5 * (w[1] ** 2 + w[2] ** 2 + w[3] ** 2) / w[0])}
+    # the converged FP32 defect-correction fields against the FP64 solve
+    runs["fp32_vs_fp64"] = {
+        q: {"L2": float(np.linalg.norm(fields["fp32"][q] - fields["fp64"][q]) / np.linalg.norm(fields["fp64"][q])),
+            "Linf": float(np.abs(fields["fp32"][q] - fields["fp64"][q]).max() / np.abs(fields["fp64"][q]).max())}
+        for q in ("rho", "u", "p")}
     out = {
         "metric": "Mcell-updates/s per implicit iter to 1e-8 residual drop; % of HBM roofline",
         "value": runs["fp32"]["Mcell_updates_per_s"], "unit": "Mcell-updates/s", "n_gpus": 1,

@@ -313,6 +313,18 @@ def test_converges_to_1e8_fp64_and_fp32():
         assert relerr(final["fp32"][e], gold["v"][e]) <= 1e-10, e
     assert not np.array_equal(final["fp32"], final["fp64"])  # it is the FP32 path
 
+    def prim(w):
+        g1 = case.phys["gam"] - 1.0
+        return {"rho": w[0], "u": w[1] / w[0], "p": g1 * (w[4] - 0.5 * (w[1] ** 2 + w[2] ** 2 + w[3] ** 2) / w[0])}
+    ref = prim(gold["v"])
+    for mode, tol in (("fp64", 1e-12), ("fp32", 1e-10)):
+        got = prim(final[mode])
+        for q in ("rho", "u", "p"):
+            l2 = np.linalg.norm(got[q] - ref[q]) / np.linalg.norm(ref[q])
+            linf = np.abs(got[q] - ref[q]).max() / np.abs(ref[q]).max()
+            print("%s vs the reference build, %s: L2 %.2e, Linf %.2e" % (mode, q, l2, linf))
+            assert l2 <= tol and linf <= tol, (mode, q, l2, linf)
+
 
 FLOOR_HISTORY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "floor_history_64x32x16.npz")
 

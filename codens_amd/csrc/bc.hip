@@ -142,8 +142,11 @@ __global__ void __launch_bounds__(BT) k_bc_planes_copy(long nf, const int64_t *_
   long f = blockIdx.x * (long)BT + threadIdx.x;
   if (f >= nf) return;
   long nd = g[f], ni = in[f];
+  double x[NP];  // every load before the first store (a ghost is never a source)
 #pragma unroll
-  for (int q = 0; q < NP; q++) tq[q * ldv + nd] = tq[q * ldv + ni];
+  for (int q = 0; q < NP; q++) x[q] = tq[q * ldv + ni];
+#pragma unroll
+  for (int q = 0; q < NP; q++) tq[q * ldv + nd] = x[q];
 }
 
 // rbtc.f90:104-117 and met_rbsc.f90:84-95 without periodic rotation:
@@ -155,8 +158,11 @@ __global__ void __launch_bounds__(BT) k_bc_planes_coincident(long nf, const int6
   long f = blockIdx.x * (long)BT + threadIdx.x;
   if (f >= nf) return;
   long nd = g[f], ni = in[f], nr = rm[f];
+  double x[NP];  // every load before the first store (a ghost is never a source)
 #pragma unroll
-  for (int q = 0; q < NP; q++) tq[q * ldv + nd] = 0.5 * (tq[q * ldv + ni] + tq[q * ldv + nr]);
+  for (int q = 0; q < NP; q++) x[q] = 0.5 * (tq[q * ldv + ni] + tq[q * ldv + nr]);
+#pragma unroll
+  for (int q = 0; q < NP; q++) tq[q * ldv + nd] = x[q];
 }
 
 static inline dim3 fgrid(int64_t nf) { return dim3((unsigned)((nf + BT - 1) / BT)); }

@@ -1068,11 +1068,26 @@ extern "C" int codens_implimf_3d_batch(int32_t nblk, const codens_block *blocks,
 // ---------------------------------------------------------------------------
 // FP32 defect-correction form.  np planes of a source (stride lds, storage
 // entries npc .. npc+nnn-1) rounded to float planes of stride nnn.
-__global__ void __launch_bounds__(256) k_to_f32(long nnn, long npc, int np, const double *__restrict__ src, long lds,
-                                                float *__restrict__ dst) {
+struct F32Src {
+  const double *src;  // entry npc + m of plane e at src[e * lds + npc + m]
+  long lds, npc;
+  int np;
+  float *dst;  // plane e at dst[e * nnn + m]
+};
+struct F32Set {
+  F32Src s[5];
+  int n;
+};
+
+// up to five such conversions in one launch (one pass over the block's
+// entries)
+__global__ void __launch_bounds__(256) k_to_f32_set(long nnn, F32Set set) {
   long m = blockIdx.x * 256L + threadIdx.x;
   if (m >= nnn) return;
-  for (int e = 0; e < np; e++) dst[e * nnn + m] = (float)src[e * lds + npc + m];
+  for (int q = 0; q < set.n; q++) {
+    const F32Src &c = set.s[q];
+    for (int e = 0; e < c.np; e++) c.dst[e * nnn + m] = (float)c.src[e * c.lds + c.npc + m];
+  }
 }
 
 // d = the FP32 increment widened to FP64, v += d (real cells)
@@ -1124,11 +1139,13 @@ extern "C" int codens_implimf_3d_f32(const codens_block *cb, const codens_phys *
     dim3 gc(g.x, g.y, (unsigned)((b.k2 - b.k1 + ks - 1) / ks));
     hipLaunchKernelGGL(k_impl_coef_tiled, gc, t, 0, st, b, ph, v, (long)ldv, mu, mut, dt, vol, cson, sn, dtpas,
                        (int)kfmg, ws, ws + 3 * nnn, ks);
-    hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, 0L, 4, (const double *)ws, nnn, cF);
-    hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, (long)b.npc, 5, v, (long)ldv, vF);
-    hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, (long)b.npc, 5, u, (long)ldv, uF);
-    if (ityprk) hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, (long)b.npc, 5, ff, (long)ldv, fF);
-    hipLaunchKernelGGL(k_to_f32, g1, t1, 0, st, nnn, 0L, 9, sn, nnn, snF);
+    F32Set set{};
+    set.s[set.n++] = F32Src{(const double *)ws, nnn, 0L, 4, cF};
+    set.s[set.n++] = F32Src{v, (long)ldv, (long)b.npc, 5, vF};
+    set.s[set.n++] = F32Src{u, (long)ldv, (long)b.npc, 5, uF};
+    if (ityprk) set.s[set.n++] = F32Src{ff, (long)ldv, (long)b.npc, 5, fF};
+    set.s[set.n++] = F32Src{sn, nnn, 0L, 9, snF};
+    hipLaunchKernelGGL(k_to_f32_set, g1, t1, 0, st, nnn, set);
   }
   // storage-indexed views of the block-local float planes (entry n at n - npc)
   const float *vS = vF - b.npc, *uS = uF - b.npc, *fS = fF - b.npc;

@@ -40,9 +40,9 @@ sys.path.insert(0, ROOT)
 SWEEP_BYTES_PER_CELL = 8 * (5 + 5 + 5 + 9 + 1 + 3 + 5)  # u, v, d_in, sn, diag, coefe, d_out
 # Memory traffic of one sweep launch on the 256x128x64 workload: 2.0 x FETCH_SIZE
 # + 1.0 x WRITE_SIZE (factors from a 512 MiB device copy in the same run),
-# 665,124 KiB = 681.1 MB against 553.6 MB algorithmic (64x4 tiles)
+# 658,503 KiB = 674.3 MB against 553.6 MB algorithmic (64x4 tiles)
 # (profiles/traffic_wall_ns_256x128x64.md).
-SWEEP_TRAFFIC_BYTES = 665124 * 1024
+SWEEP_TRAFFIC_BYTES = 658503 * 1024
 HBM_PEAK_GBS = 8000.0
 
 

@@ -15,9 +15,12 @@
 //     reference does (sch_roe.f90:110-125).
 // The dissipation |A| dW is evaluated as P (|Lambda| (Q dW)), the same
 // matrices as the reference's P |Lambda| Q product with a quarter of the work.
-// The Roe average multiplies by reciprocals of rho_L, rho_R, |S|, c and
-// 1/(gamma-1) where the reference divides (at most one rounding apart per
-// operation).
+// The Roe average forms sqrt(rho_L rho_R) as rho_L sqrt(rho_R/rho_L) and
+// multiplies by reciprocals of rho_L, rho_R, |S|, c, 1 + sqrt(rho_R/rho_L),
+// sqrt(rho_L rho_R) and gamma-1 where the reference divides; the limiter
+// takes 1/r of its slope ratio and the fused kernel 1/rho of its cells the
+// same way (rcp64; a product by it is at most two roundings from the
+// reference's quotient).
 #include "common.hpp"
 
 #include <algorithm>
@@ -67,6 +70,14 @@ __device__ inline double pget(const Prim &q, int e) {
 
 __device__ inline double minmod_phi(double a) { return fmax(0., fmin(1., a)); }
 
+// minmod_phi(1 / r) without the division where its value is 0 or 1: 1/r
+// lies in (0, 1) only for r > 1 (clamped below the overflow of 1/r, where
+// the reciprocal of the clamp stands in for 0); r <= 1 gives 1 when r is
+// +0 or positive, 0 when r is negative or -0, as the division would.
+__device__ inline double minmod_phi_inv(double r) {
+  return r > 1. ? rcp64(fmin(r, 1e300)) : (signbit(r) ? 0. : 1.);
+}
+
 // Roe flux (2x the numerical flux . S) of one face, sch_roe.f90:327-473.
 // tl, tr: stresses and heat flux of the left/right cells (9 values each) or
 // all zero for the Euler form.
@@ -76,20 +87,20 @@ __device__ void roe_face(const Phys &ph, double s0, double s1, double s2, const 
   double rhol = L[0], ul = L[1], vl = L[2], wl = L[3], pl = L[4];
   double rhor = R[0], ur = R[1], vr = R[2], wr = R[3], prr = R[4];
   const double igam1 = ph.igam1;
-  double cnds = sqrt(s0 * s0 + s1 * s1 + s2 * s2), icn = 1. / cnds;
+  double cnds = sqrt(s0 * s0 + s1 * s1 + s2 * s2), icn = rcp64(cnds);
   double nx = s0 * icn, ny = s1 * icn, nz = s2 * icn;
-  double irl = 1. / rhol, irr = 1. / rhor;
+  double irl = rcp64(rhol), irr = rcp64(rhor);
   double al2 = gam * pl * irl, ar2 = gam * prr * irr;
   double q2l = ul * ul + vl * vl + wl * wl, q2r = ur * ur + vr * vr + wr * wr;
   double hl = al2 * igam1 + 0.5 * q2l, hr = ar2 * igam1 + 0.5 * q2r;
   double el = (pl * igam1 + pinfl) * irl + 0.5 * q2l;
   double er = (prr * igam1 + pinfl) * irr + 0.5 * q2r;
-  double gd = sqrt(rhor * irl), gd1 = 1. / (1. + gd), gd2 = gd * gd1;
-  double rhom = sqrt(rhol * rhor), rhomi = 1. / rhom;
+  double gd = sqrt(rhor * irl), gd1 = rcp64(1. + gd), gd2 = gd * gd1;
+  double rhom = rhol * gd, rhomi = rcp64(rhom);  // sqrt(rho_L rho_R)
   double um = gd1 * ul + gd2 * ur, vm = gd1 * vl + gd2 * vr, wm = gd1 * wl + gd2 * wr;
   double hm = gd1 * hl + gd2 * hr;
   double vitm2 = 0.5 * (um * um + vm * vm + wm * wm);
-  double am = sqrt(gam1 * (hm - vitm2)), ami = 1. / am;
+  double am = sqrt(gam1 * (hm - vitm2)), ami = rcp64(am);
   double am2i = ami * ami;
   double vn = um * nx + vm * ny + wm * nz;
   double rhoiam = rhom * ami, rhoami = ami * rhomi;
@@ -158,8 +169,8 @@ __device__ __forceinline__ void muscl_roe_half(const Phys &ph, double s0, double
 #pragma unroll
     for (int e = 0; e < 5; e++) {
       double qa = q[0][e], qb = q[1][e], qc = q[2][e], qd = q[3][e];
-      L[e] = qb + cm * minmod_phi(rl[e]) * (1. - xk) * (qb - qa) + cm * minmod_phi(1. / rl[e]) * (1. + xk) * (qc - qb);
-      R[e] = qc - cm * minmod_phi(rh[e]) * (1. + xk) * (qc - qb) - cm * minmod_phi(1. / rh[e]) * (1. - xk) * (qd - qc);
+      L[e] = qb + cm * minmod_phi(rl[e]) * (1. - xk) * (qb - qa) + cm * minmod_phi_inv(rl[e]) * (1. + xk) * (qc - qb);
+      R[e] = qc - cm * minmod_phi(rh[e]) * (1. + xk) * (qc - qb) - cm * minmod_phi_inv(rh[e]) * (1. - xk) * (qd - qc);
     }
   } else {
 #pragma unroll
@@ -546,11 +557,11 @@ struct FusedArgs {
 // primitive variables of storage entry n: rho, u, v, w (the divisions of
 // pass 0) and p
 __device__ __forceinline__ void fused_prim(const FusedArgs &A, long n, double q[5]) {
-  double r = A.v[n];
+  double r = A.v[n], ir = rcp64(r);
   q[0] = r;
-  q[1] = A.v[A.ldv + n] / r;
-  q[2] = A.v[2 * A.ldv + n] / r;
-  q[3] = A.v[3 * A.ldv + n] / r;
+  q[1] = A.v[A.ldv + n] * ir;
+  q[2] = A.v[2 * A.ldv + n] * ir;
+  q[3] = A.v[3 * A.ldv + n] * ir;
   q[4] = A.ps[n];
 }
 

@@ -73,6 +73,94 @@ contains
   end subroutine zvismo
 end module mod_zvismo
 
+module mod_zvisqc
+  implicit none
+contains
+  ! zvisqc.f90:4 -- stresses and heat flux toxx..qcz of every block of grid
+  ! level img: Green-Gauss gradients (zgrad2) and the stress and heat-flux
+  ! closure (zfluto) in one device pass per block, then the ghosts by the
+  ! reference's own rbte and rbtc over the frontier lists nfbn, nfba, nfbc
+  ! (zvisqc.f90:182-205).  The transition and multigrid updates of mut
+  ! (atctranske, smg_fcs) run on the host first, as they read neither the
+  ! gradients nor the stresses.  The gradient scratch dvxx..dvzz is not
+  ! written.  Stops for ischema = 1 (zgrad's gradients) and for gradient
+  ! weights cmui1..cmuk2 other than 1 (inivec.f90:107-115).
+  subroutine zvisqc(img, s, mu, ro, x, y, z, mut, dist, mnpar, fgam, toxx, toxy, toxz, toyy, toyz, tozz, &
+       qcx, qcy, qcz, icyc, mcyturb, ncbd, ncin, mnc, sn, vol, dvxx, dvxy, dvxz, dvyx, dvyy, dvyz, dvzx, &
+       dvzy, dvzz, cmui1, cmui2, cmuj1, cmuj2, cmuk1, cmuk2, ztemp)
+    use para_var
+    use para_fige
+    use boundary
+    use maillage
+    use chainecarac
+    use schemanum
+    use modeleturb
+    use mod_smg_fcs
+    use mod_atctranske
+    use mod_rbte
+    use mod_rbtc
+    use codens_c
+    implicit none
+    integer :: icyc, img, mcyturb, mnc(ip43), mnpar(ip12), ncbd(ip41), ncin(ip41)
+    double precision, target :: s(ip11,ip60), mu(ip12), mut(ip12), sn(ip31*ndir), vol(ip11), ztemp(ip11)
+    double precision, target :: toxx(ip12), toxy(ip12), toxz(ip12), toyy(ip12), toyz(ip12), tozz(ip12)
+    double precision, target :: qcx(ip12), qcy(ip12), qcz(ip12)
+    double precision :: ro(ip11), x(ip21), y(ip21), z(ip21), dist(ip12), fgam(ip12)
+    double precision :: dvxx(ip00), dvxy(ip00), dvxz(ip00), dvyx(ip00), dvyy(ip00), dvyz(ip00)
+    double precision :: dvzx(ip00), dvzy(ip00), dvzz(ip00)
+    double precision :: cmui1(ip21), cmui2(ip21), cmuj1(ip21), cmuj2(ip21), cmuk1(ip21), cmuk2(ip21)
+    type(codens_block) :: b
+    type(c_ptr) :: dv, dtm, dmu, dmut, dsn, dvol, dtq, tq(9)
+    integer :: l, lm, mf, npsn, n0, n1
+    call codens_need_3d(equat, 'zvisqc')
+    if (ischema .eq. 1) error stop 'zvisqc: ischema = 1 (zgrad) has no device form'
+    tq = [c_loc(toxx), c_loc(toxy), c_loc(toxz), c_loc(toyy), c_loc(toyz), c_loc(tozz), c_loc(qcx), &
+         c_loc(qcy), c_loc(qcz)]
+    do l = 1, lzx
+       lm = l + (img-1)*lz
+       n0 = npc(lm) + 1
+       n1 = npc(lm) + nnn(lm)
+       if (any(cmui1(n0:n1) .ne. 1.) .or. any(cmui2(n0:n1) .ne. 1.) .or. any(cmuj1(n0:n1) .ne. 1.) .or. &
+           any(cmuj2(n0:n1) .ne. 1.) .or. any(cmuk1(n0:n1) .ne. 1.) .or. any(cmuk2(n0:n1) .ne. 1.)) &
+           error stop 'zvisqc: gradient weights other than 1 have no device form'
+       if (icyc .gt. icytur0 .and. mcyturb .eq. 1) then
+          if (img .eq. mgl) then
+             if (ktransi .eq. 1) call atctranske(l, s, mu, mut, mnpar, fgam)
+          else
+             call smg_fcs(img-1, img, vol, mut)
+          endif
+       endif
+       b = codens_block_of(lm)
+       npsn = ndir*npfb(lm) + 1
+       dv = codens_dev(c_loc(s), ip11*ip60, ip11, 1, 5, b, 1)
+       dtm = codens_dev(c_loc(ztemp), ip11, ip11, 1, 1, b, 1)
+       dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
+       dmut = codens_dev(c_loc(mut), ip12, ip12, 1, 1, b, 1)
+       dsn = codens_dev_all(c_loc(sn(npsn)), 9*nnn(lm), codens_geom_mode())
+       dvol = codens_dev(c_loc(vol), ip11, ip11, 1, 1, b, codens_geom_mode())
+       dtq = codens_tq_in(tq, ip12, b)
+       call codens_check(codens_zvisqc(b, codens_phys_now(), dsn, dvol, dv, int(ip11, c_int64_t), dtm, dmu, &
+            dmut, dtq, c_null_ptr), 'zvisqc')
+       call codens_tq_back(tq, ip12, b)
+    enddo
+    do mf = 1, mtnx
+       lbd(mf) = nfbn(mf) + (img-1)*mtb
+    enddo
+    nbd = mtnx
+    call rbte(toxx, toxy, toxz, toyy, toyz, tozz, qcx, qcy, qcz, ncbd, ncin)
+    do mf = 1, mtax
+       lbd(mf) = nfba(mf) + (img-1)*mtb
+    enddo
+    nbd = mtax
+    call rbte(toxx, toxy, toxz, toyy, toyz, tozz, qcx, qcy, qcz, ncbd, ncin)
+    do mf = 1, mtcx
+       lbd(mf) = nfbc(mf) + (img-1)*mtb
+    enddo
+    nbd = mtcx
+    call rbtc(toxx, toxy, toxz, toyy, toyz, tozz, qcx, qcy, qcz, ncbd, ncin, mnc)
+  end subroutine zvisqc
+end module mod_zvisqc
+
 module mod_chronos
   implicit none
 contains

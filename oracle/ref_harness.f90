@@ -898,6 +898,47 @@ contains
     call rbtc(tq(1,1), tq(1,2), tq(1,3), tq(1,4), tq(1,5), tq(1,6), tq(1,7), tq(1,8), tq(1,9), ncbd, ncin, mnc)
   end subroutine ref_rbtc
 
+  ! zvisqc.f90:4 -- stresses and heat flux of every block (zgrad2, then
+  ! zfluto), then their ghosts: rbte over the non-coincident frontiers
+  ! frn(1:nn) (the list nfbn), rbtc over the coincident ones frc(1:nc) (the
+  ! list nfbc); no adjacent-frontier list (mtax = 0).  icyc = 0 keeps the
+  ! transition and multigrid branches off; the gradient weights cmui1..cmuk2
+  ! are 1 (inivec.f90:107-115); ischema = 7 selects zgrad2; equat (module
+  ! chainecarac, which zvisqc reads) is the 3-D Navier-Stokes set.
+  subroutine ref_zvisqc(nn, frn, nc, frc, s, mu, mut, tq, ncbd, ncin, sn, vol, temp) bind(C, name="ref_zvisqc")
+    use boundary
+    use chainecarac
+    use mod_zvisqc
+    integer(c_int), value :: nn, nc
+    integer(c_int) :: frn(*), frc(*), ncbd(*), ncin(*)
+    real(c_double) :: s(*), mu(*), mut(*), tq(ip12, 9), sn(*), vol(*), temp(*)
+    real(c_double), allocatable :: xyz(:), cmu(:), g(:, :), dist(:), fgam(:)
+    integer, allocatable :: mnpar(:)
+    integer :: mnc(1), f, isch0
+    allocate(xyz(ip21), cmu(ip21), g(ip00, 9), dist(ip12), fgam(ip12), mnpar(ip12))
+    xyz = 0.; cmu = 1.; g = 0.; dist = 1.; fgam = 0.; mnpar = 0; mnc = 0
+    if (allocated(nfbn)) deallocate(nfbn)
+    if (allocated(nfbc)) deallocate(nfbc)
+    if (allocated(nfba)) deallocate(nfba)
+    allocate(nfbn(max(nn, 1)), nfbc(max(nc, 1)), nfba(1))
+    do f = 1, nn
+       nfbn(f) = frn(f)
+    enddo
+    do f = 1, nc
+       nfbc(f) = frc(f)
+    enddo
+    mtnx = nn; mtcx = nc; mtax = 0; mtb = 0
+    isch0 = ischema
+    ischema = 7
+    equat = heq
+    call zvisqc(1, s, mu, s, xyz, xyz, xyz, mut, dist, mnpar, fgam, &
+         tq(1,1), tq(1,2), tq(1,3), tq(1,4), tq(1,5), tq(1,6), tq(1,7), tq(1,8), tq(1,9), &
+         0, 0, ncbd, ncin, mnc, sn, vol, &
+         g(1,1), g(1,2), g(1,3), g(1,4), g(1,5), g(1,6), g(1,7), g(1,8), g(1,9), &
+         cmu, cmu, cmu, cmu, cmu, cmu, temp)
+    ischema = isch0
+  end subroutine ref_zvisqc
+
   ! met_rbse.f90:4 -- turbulence flux-density ghosts = adjacent cell (fd: 6 planes)
   subroutine ref_met_rbse(nl, fr, fd, ncbd, ncin) bind(C, name="ref_met_rbse")
     use mod_met_rbse

@@ -456,6 +456,13 @@ class RefFrontiers:
         fr = self._list(frs)
         self.lib.ref_rbtc(len(fr), ptr(fr), ptr(tq), ptr(self.ncbd), ptr(self.ncin))
 
+    def zvisqc(self, frn, frc, v, mu, mut, tq, sn, vol, temp):
+        """zvisqc.f90:4 over every block, then rbte over the frontiers frn and
+        rbtc over the coincident frontiers frc; sn: the whole metric array."""
+        a, c = self._list(frn), self._list(frc)
+        self.lib.ref_zvisqc(len(a), ptr(a), len(c), ptr(c), ptr(v), ptr(mu), ptr(mut), ptr(tq), ptr(self.ncbd),
+                            ptr(self.ncin), ptr(sn), ptr(vol), ptr(temp))
+
     def met_rbse(self, frs, fd):
         fr = self._list(frs)
         self.lib.ref_met_rbse(len(fr), ptr(fr), ptr(fd), ptr(self.ncbd), ptr(self.ncin))

@@ -32,21 +32,30 @@ class CpuStage:
         # and symmetry planes, in listing order
         self.RF = None
         self.first = True
+        # zvisqc in one call (stresses and their ghosts), if set: replaces the
+        # per-block zgrad2 + zfluto and the stress ghosts below
+        self.zvisqc = None
         if use_ref:
-            fr, part = [], []
-            c = case.coincident
-            for f in range(len(c.sizes)):
-                s_ = c.frontier(f)
-                fr.append(("rc", c, s_, c.blocks[f]))
-                part.append(f ^ 1)
-            for kind, fs in (("para", case.wall), ("nrd", case.farfield), ("sym", case.sym)):
-                for f in range(len(fs.sizes)):
-                    fr.append((kind, fs, fs.frontier(f), fs.blocks[f]))
-                    part.append(-1)
-            self.fr = fr
-            self.RF = C.RefFrontiers(self.K, [x[1].ghost[x[2]] for x in fr], [x[1].inner[x[2]] for x in fr], part)
-            self.coin_fr = [k for k, x in enumerate(fr) if x[0] == "rc"]
-            self.other_fr = [k for k, x in enumerate(fr) if x[0] != "rc"]
+            self.RF, self.coin_fr, self.other_fr = self.frontiers(self.K)
+
+    def frontiers(self, K):
+        """The case's frontiers as the reference's boundary routines of the
+        build behind K see them: (RefFrontiers, coincident frontier numbers,
+        the others)."""
+        case = self.case
+        fr, part = [], []
+        c = case.coincident
+        for f in range(len(c.sizes)):
+            s_ = c.frontier(f)
+            fr.append(("rc", c, s_, c.blocks[f]))
+            part.append(f ^ 1)
+        for kind, fs in (("para", case.wall), ("nrd", case.farfield), ("sym", case.sym)):
+            for f in range(len(fs.sizes)):
+                fr.append((kind, fs, fs.frontier(f), fs.blocks[f]))
+                part.append(-1)
+        self.fr = fr
+        RF = C.RefFrontiers(K, [x[1].ghost[x[2]] for x in fr], [x[1].inner[x[2]] for x in fr], part)
+        return (RF, [k for k, x in enumerate(fr) if x[0] == "rc"], [k for k, x in enumerate(fr) if x[0] != "rc"])
 
     def stage(self):
         """One stage of Solver: thermodynamics, time step, ghosts, [turbulence],
@@ -121,7 +130,9 @@ class CpuStage:
         c, w, f, y = cs.coincident, cs.wall, cs.farfield, cs.sym
         if cs.turb is not None:
             self.turb_stage()
-        if cs.ns:
+        if cs.ns and self.zvisqc is not None:
+            self.zvisqc()
+        elif cs.ns:
             for l in range(nb):
                 g = K.zgrad2(l, cs.sn, cs.vol, self.v, self.temp)
                 K.zfluto(l, self.mu, self.mut, self.tq, g)

@@ -5,7 +5,7 @@ curvilinear layout (so every host<->device window starts inside the arrays).
 
 The drop-in library is oracle/_ref/libcodens_dropin.so: the reference harness
 and reference objects with the drop-in modules linked in place of the
-reference's zpres, zvismo, chronos, sch_roe_euler, sch_roe, sch_expli,
+reference's zpres, zvismo, zvisqc, chronos, sch_roe_euler, sch_roe, sch_expli,
 implimf_3d, implimf_eu, dissip_jameson, met_roe2o and met_roe."""
 import numpy as np
 import pytest
@@ -81,6 +81,37 @@ def test_roe_residual(mesh, ns, ilim):
     close(res["r"][0][:5, inner], res["g"][0][:5, inner])
     close(res["r"][1][:5], res["g"][1][:5])
     assert np.array_equal(res["r"][0][5:], res["g"][0][5:])
+
+
+def test_zvisqc(mesh):
+    """zvisqc over both blocks with its ghosts: rbte over the non-coincident
+    frontiers, rbtc over two coincident pairs (block 0's i faces, block 1's j
+    faces); the drop-in computes the real cells on the GPU and calls the
+    reference's rbte and rbtc for the ghosts."""
+    from codens_amd import layout as L
+    lay, sn, vol = mesh
+    rng = np.random.default_rng(53)
+    n = lay.ip11
+    ph = C.orc_phys()
+    R, G = pair(lay, ph)
+    v = random_state(n, rng)
+    _, t, _ = all_thermo(v)
+    mu = 1e-3 * (1 + 0.1 * rng.random(n)); mut = 1e-3 * rng.random(n)
+    faces = [(0, "i1"), (0, "i2"), (1, "j1"), (1, "j2"), (0, "j1"), (0, "j2"), (0, "k1"), (0, "k2"),
+             (1, "i1"), (1, "i2"), (1, "k1"), (1, "k2")]
+    gh, inn = [], []
+    for l, f in faces:
+        g, i, _ = L.face_cells(lay.blocks[l], f)
+        gh.append(g); inn.append(i)
+    partner = [1, 0, 3, 2] + [-1] * 8
+    out = {}
+    for X, k in ((R, "r"), (G, "g")):
+        RF = C.RefFrontiers(X, gh, inn, partner)
+        tq = np.full((9, n), -7.0)
+        RF.zvisqc(list(range(4, 12)), [0, 1, 2, 3], v, mu, mut, tq, sn, vol, t)
+        out[k] = tq
+    close(out["r"], out["g"])
+    assert np.array_equal(out["r"] == -7.0, out["g"] == -7.0)  # the same entries written
 
 
 @pytest.mark.parametrize("lmx,ityprk", [(1, 0), (4, 0), (3, 1)])
@@ -518,10 +549,10 @@ def test_kl_sas(tsetup):
 def test_dropin_stage_rate():
     """PCIe-inclusive rate of the drop-in path on the 256x128x64 wall block
     (2,097,152 cells): the routines of an implicit stage that the drop-ins
-    replace -- zpres, zvismo, chronos, sch_roe (minmod), implimf_3d
-    (lmax = 10) -- called with the reference's host arrays, so that every
-    call moves its block windows host -> device and its outputs back.  The
-    rest of the stage (ghost cells, zgrad2/zfluto, which have no drop-in)
+    replace -- zpres, zvismo, chronos, zvisqc (with its stress ghosts),
+    sch_roe (minmod), implimf_3d (lmax = 10) -- called with the reference's
+    host arrays, so that every call moves its block windows host -> device
+    and its outputs back.  The rest of the stage (the state's ghost cells)
     runs on the oracle between them and is not timed.  Prints the rate over
     three timed stages (after one untimed) and checks that the state stays
     physical."""
@@ -548,6 +579,13 @@ def test_dropin_stage_rate():
                 return call
             return getattr(st.O, name)
     st.K = Timed()
+    RFG, coin, other = st.frontiers(G)
+
+    def zvisqc():
+        t0 = time.perf_counter()
+        RFG.zvisqc(other, coin, st.v, st.mu, st.mut, st.tq, case.sn, case.vol, st.temp)
+        spent[0] += time.perf_counter() - t0
+    st.zvisqc = zvisqc
     st.stage()  # first call: device buffers made, metrics and volumes uploaded
     spent[0] = 0.0
     for _ in range(3):

@@ -201,6 +201,35 @@ def test_stress_and_turbulence_ghosts(setup):
     assert np.array_equal(fa, fb)
 
 
+def test_zvisqc_is_gradients_closure_then_ghosts(setup):
+    """zvisqc.f90:4 as a whole (the reference's block loop of zgrad2 and
+    zfluto, then rbte over the non-coincident and rbtc over the coincident
+    frontiers, zvisqc.f90:134-205) against the oracle's zgrad2 and zfluto per
+    block followed by its two stress-ghost restatements, bit for bit."""
+    lay, sn, xyz, fr, RF, O = setup
+    _, _, vol, _ = random_mesh_xyz(DIMS, seed=11)
+    rng = np.random.default_rng(9)
+    n = lay.ip11
+    v = state(n, rng, amp=0.3)
+    _, t, _ = all_thermo(v)
+    mu = 1e-3 * (1 + 0.1 * rng.random(n)); mut = 1e-3 * rng.random(n)
+    noncoin = [k for k, f in enumerate(fr) if f["kind"] != "rc"]
+    coin = [0, 1, 2, 3]
+    ta = np.full((9, n), -7.0)
+    RF.zvisqc(noncoin, coin, v, mu, mut, ta, sn, vol, t)
+    tb = np.full((9, n), -7.0)
+    for l in range(len(lay.blocks)):
+        O.zfluto(l, mu, mut, tb, O.zgrad2(l, sn, vol, v, t))
+    gg = np.concatenate([fr[k]["g"] for k in noncoin]); ii = np.concatenate([fr[k]["i"] for k in noncoin])
+    g = np.concatenate([fr[k]["g"] for k in coin])
+    i = np.concatenate([fr[k]["i"] for k in coin])
+    r = np.concatenate([fr[k]["r"] for k in coin])
+    O.lib.orc_bc_tq_copy(C.L_(len(gg)), C.ptr(gg), C.ptr(ii), C.ptr(tb), C.L_(n))
+    O.lib.orc_bc_tq_coincident(C.L_(len(g)), C.ptr(g), C.ptr(i), C.ptr(r), C.ptr(tb), C.L_(n))
+    assert np.array_equal(ta, tb)
+    assert np.all(ta[:, np.concatenate([gg, g])] != -7.0)  # every listed ghost written
+
+
 def test_inward_normals_match_initns(setup):
     lay, sn, xyz, fr, RF, O = setup
     x, y, z = xyz

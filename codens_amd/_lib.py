@@ -134,6 +134,11 @@ SIGNATURES = {
     "codens_host_fetch": [c_vp, c_i64, c_i64, c_i64, c_int, c_vp],
     "codens_host_bundle": [P(c_vp), c_int, c_i64, c_i64, c_i64, c_int, P(c_vp), c_vp],
     "codens_host_bundle_fetch": [P(c_vp), c_int, c_i64, c_i64, c_i64, c_vp],
+    "codens_host_residency": [c_int, P(c_int), c_vp],
+    "codens_host_sync": [c_vp, c_vp],
+    "codens_host_sync_all": [c_vp],
+    "codens_host_modified": [c_vp],
+    "codens_host_traffic": [P(c_i64), P(c_i64)],
 }
 
 

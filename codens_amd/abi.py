@@ -488,6 +488,53 @@ def residual_path(fused):
     return prev.value
 
 
+# ---- host-array mirrors (the drop-in boundary) -----------------------------
+def _host_ptr(a):
+    """Address of a C-contiguous numpy array's first element."""
+    if not (a.flags["C_CONTIGUOUS"] and a.flags["WRITEABLE"]):
+        raise ValueError("host arrays must be C-contiguous and writeable numpy arrays")
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def host_mirror(a, offset, width, pitch=0, rows=1, to_device=1):
+    """Device mirror of host array a (window in bytes); returns its address."""
+    d = ctypes.c_void_p()
+    _chk(lib.codens_host_mirror(_host_ptr(a), a.nbytes, offset, width, pitch or width, rows, to_device,
+                                ctypes.byref(d), None))
+    return d.value
+
+
+def host_fetch(a, offset, width, pitch=0, rows=1):
+    _chk(lib.codens_host_fetch(_host_ptr(a), offset, width, pitch or width, rows, None))
+
+
+def host_residency(on):
+    """1: the mirrors keep device-newer windows until host_sync; 0: every
+    window moves both ways.  Returns the previous setting."""
+    prev = ctypes.c_int32(0)
+    _chk(lib.codens_host_residency(int(on), ctypes.byref(prev), None))
+    return prev.value
+
+
+def host_sync(a):
+    _chk(lib.codens_host_sync(_host_ptr(a), None))
+
+
+def host_sync_all():
+    _chk(lib.codens_host_sync_all(None))
+
+
+def host_modified(a):
+    _chk(lib.codens_host_modified(_host_ptr(a)))
+
+
+def host_traffic():
+    """(bytes host -> device, bytes device -> host) moved by the mirrors so far."""
+    h, d = ctypes.c_int64(0), ctypes.c_int64(0)
+    _chk(lib.codens_host_traffic(ctypes.byref(h), ctypes.byref(d)))
+    return h.value, d.value
+
+
 def sweep_tile_of(b, eu=False, fp32=False):
     t = ctypes.c_int32(0)
     _chk(lib.codens_sweep_tile_of(ctypes.byref(b), int(eu), int(fp32), ctypes.byref(t)))

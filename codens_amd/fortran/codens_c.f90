@@ -75,6 +75,24 @@ module codens_c
        integer(c_int64_t), value :: bytes, offset, width
        type(c_ptr), value :: stream
      end function codens_host_bundle_fetch
+     integer(c_int32_t) function codens_host_residency(on, previous, stream) bind(C, name="codens_host_residency")
+       import
+       integer(c_int32_t), value :: on
+       integer(c_int32_t) :: previous
+       type(c_ptr), value :: stream
+     end function codens_host_residency
+     integer(c_int32_t) function codens_host_sync(host, stream) bind(C, name="codens_host_sync")
+       import
+       type(c_ptr), value :: host, stream
+     end function codens_host_sync
+     integer(c_int32_t) function codens_host_sync_all(stream) bind(C, name="codens_host_sync_all")
+       import
+       type(c_ptr), value :: stream
+     end function codens_host_sync_all
+     integer(c_int32_t) function codens_host_modified(host) bind(C, name="codens_host_modified")
+       import
+       type(c_ptr), value :: host
+     end function codens_host_modified
      integer(c_int32_t) function codens_zvisqc(b, ph, sn, vol, v, ldv, temp, mu, mut, tq, stream) &
           bind(C, name="codens_zvisqc")
        import
@@ -628,6 +646,19 @@ contains
     call codens_check(codens_host_fetch(a, 8_c_int64_t*(b%npc + int(row0-1,c_int64_t)*ld), 8_c_int64_t*b%nnn, &
          8_c_int64_t*ld, int(rows,c_int32_t), c_null_ptr), 'codens_back')
   end subroutine codens_back
+
+  ! Residency hooks (codens_host_residency): host code about to read array
+  ! a, and host code that has written a.  With residency off they cost a
+  ! lookup.
+  subroutine codens_sync(a)
+    type(c_ptr), intent(in) :: a
+    call codens_check(codens_host_sync(a, c_null_ptr), 'codens_sync')
+  end subroutine codens_sync
+
+  subroutine codens_modified(a)
+    type(c_ptr), intent(in) :: a
+    call codens_check(codens_host_modified(a), 'codens_modified')
+  end subroutine codens_modified
 
   ! The 9 viscous arrays toxx..qcz (each n doubles) as one device bundle
   ! with slot stride n, block b's slice copied in.

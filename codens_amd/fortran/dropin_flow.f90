@@ -60,12 +60,14 @@ contains
     dtm = codens_dev(c_loc(temp), ip11, ip11, 1, 1, b, 1)
     dmu = codens_dev(c_loc(mu), ip12, ip12, 1, 1, b, 1)
     nbad = 0
+    call codens_modified(c_loc(nbad))
     call codens_check(codens_host_mirror(c_loc(nbad), 4_c_int64_t, 0_c_int64_t, 4_c_int64_t, 4_c_int64_t, &
          1_c_int32_t, 1_c_int32_t, dn, c_null_ptr), 'zvismo')
     call codens_check(codens_zvismo(b, codens_phys_now(), dtm, dmu, dn, c_null_ptr), 'zvismo')
     call codens_back(c_loc(mu), ip12, 1, 1, b)
     call codens_check(codens_host_fetch(c_loc(nbad), 0_c_int64_t, 4_c_int64_t, 4_c_int64_t, 1_c_int32_t, &
          c_null_ptr), 'zvismo')
+    call codens_sync(c_loc(nbad))
     if (nbad .ne. 0) then
        write(imp,'(/,"!!!zvismo: temperature negative en",i8," cellules domaine= ",i4)') nbad, l
        error stop 'zvismo: non-positive temperature'
@@ -83,8 +85,10 @@ contains
   ! (zvisqc.f90:182-205).  The transition and multigrid updates of mut
   ! (atctranske, smg_fcs) run on the host first, as they read neither the
   ! gradients nor the stresses.  The gradient scratch dvxx..dvzz is not
-  ! written.  Stops for ischema = 1 (zgrad's gradients) and for gradient
-  ! weights cmui1..cmuk2 other than 1 (inivec.f90:107-115).
+  ! written.  Under residency, the host routines' arrays are synchronised
+  ! before them and marked written after them.  Stops for ischema = 1
+  ! (zgrad's gradients) and for gradient weights cmui1..cmuk2 other than 1
+  ! (inivec.f90:107-115).
   subroutine zvisqc(img, s, mu, ro, x, y, z, mut, dist, mnpar, fgam, toxx, toxy, toxz, toyy, toyz, tozz, &
        qcx, qcy, qcz, icyc, mcyturb, ncbd, ncin, mnc, sn, vol, dvxx, dvxy, dvxz, dvyx, dvyy, dvyz, dvzx, &
        dvzy, dvzz, cmui1, cmui2, cmuj1, cmuj2, cmuk1, cmuk2, ztemp)
@@ -111,7 +115,7 @@ contains
     double precision :: cmui1(ip21), cmui2(ip21), cmuj1(ip21), cmuj2(ip21), cmuk1(ip21), cmuk2(ip21)
     type(codens_block) :: b
     type(c_ptr) :: dv, dtm, dmu, dmut, dsn, dvol, dtq, tq(9)
-    integer :: l, lm, mf, npsn, n0, n1
+    integer :: k, l, lm, mf, npsn, n0, n1
     call codens_need_3d(equat, 'zvisqc')
     if (ischema .eq. 1) error stop 'zvisqc: ischema = 1 (zgrad) has no device form'
     tq = [c_loc(toxx), c_loc(toxy), c_loc(toxz), c_loc(toyy), c_loc(toyz), c_loc(tozz), c_loc(qcx), &
@@ -124,11 +128,14 @@ contains
            any(cmuj2(n0:n1) .ne. 1.) .or. any(cmuk1(n0:n1) .ne. 1.) .or. any(cmuk2(n0:n1) .ne. 1.)) &
            error stop 'zvisqc: gradient weights other than 1 have no device form'
        if (icyc .gt. icytur0 .and. mcyturb .eq. 1) then
+          call codens_sync(c_loc(s)); call codens_sync(c_loc(mu)); call codens_sync(c_loc(mut))
+          call codens_sync(c_loc(vol))
           if (img .eq. mgl) then
              if (ktransi .eq. 1) call atctranske(l, s, mu, mut, mnpar, fgam)
           else
              call smg_fcs(img-1, img, vol, mut)
           endif
+          call codens_modified(c_loc(mut))
        endif
        b = codens_block_of(lm)
        npsn = ndir*npfb(lm) + 1
@@ -142,6 +149,9 @@ contains
        call codens_check(codens_zvisqc(b, codens_phys_now(), dsn, dvol, dv, int(ip11, c_int64_t), dtm, dmu, &
             dmut, dtq, c_null_ptr), 'zvisqc')
        call codens_tq_back(tq, ip12, b)
+    enddo
+    do k = 1, 9
+       call codens_sync(tq(k))
     enddo
     do mf = 1, mtnx
        lbd(mf) = nfbn(mf) + (img-1)*mtb
@@ -158,6 +168,9 @@ contains
     enddo
     nbd = mtcx
     call rbtc(toxx, toxy, toxz, toyy, toyz, tozz, qcx, qcy, qcz, ncbd, ncin, mnc)
+    do k = 1, 9
+       call codens_modified(tq(k))
+    enddo
   end subroutine zvisqc
 end module mod_zvisqc
 

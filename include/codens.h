@@ -451,6 +451,29 @@ int codens_host_bundle(const void *const *hosts, int32_t count, int64_t bytes, i
 int codens_host_bundle_fetch(const void *const *hosts, int32_t count, int64_t bytes, int64_t offset,
                              int64_t width, void *stream);
 
+/* Residency of the host-array mirrors (no reference counterpart: the
+ * drop-in boundary's host/device protocol).  codens_host_residency(0), the
+ * default: every window moves in before a call's kernels and back after
+ * them, as above.  codens_host_residency(1): the library records, per host
+ * array, the windows whose device copy is current and those whose device
+ * copy is newer than the host's; a window moving in whose device copy is
+ * current is not copied, and a window moving back is not copied but becomes
+ * device-newer.  The host then calls codens_host_sync(a) before its own code
+ * reads array a (device-newer windows of a come back) and
+ * codens_host_modified(a) after its own code writes a (a's device copies are
+ * forgotten; CODENS_EINVAL while a has device-newer windows).
+ * codens_host_sync_all brings back every device-newer window; turning
+ * residency off does the same and forgets every window.  A stage whose host
+ * code reads and writes none of its arrays moves nothing over PCIe.
+ * codens_host_traffic: bytes moved host -> device and device -> host by the
+ * mirrors since the library was loaded.  The arrays are keyed by the address
+ * of their first element. */
+int codens_host_residency(int32_t on, int32_t *previous, void *stream);
+int codens_host_sync(const void *host, void *stream);
+int codens_host_sync_all(void *stream);
+int codens_host_modified(const void *host);
+int codens_host_traffic(int64_t *h2d, int64_t *d2h);
+
 /* ---- Menter k-omega model (met_num.f90:449-456, 542-546, 815-850) ----
  * mc: the model constants betae, okappa, sigme1, sigma1, beta1, wsig1,
  * sigme2, sigma2, beta2, wsig2 (module modeleturb, atlecdon.f90:255-278).

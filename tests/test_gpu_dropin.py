@@ -114,6 +114,66 @@ def test_zvisqc(mesh):
     assert np.array_equal(out["r"] == -7.0, out["g"] == -7.0)  # the same entries written
 
 
+def _dropin_stage(G, lay, A, sn, vol):
+    """zpres, zvismo, chronos, sch_roe (minmod, viscous) and implimf_3d
+    (lmax = 3) of every block, through the drop-ins, on the host arrays of A."""
+    for l in range(len(lay.blocks)):
+        G.zpres(l, A["v"], A["p"], A["t"], A["c"])
+        G.zvismo(l, A["mu"], A["v"], A["t"])
+        G.chronos(l, 0.8, A["mu"], A["mut"], A["v"], A["dt"], sn, vol, A["c"])
+        G.sch_roe(l, 0, A["u"], A["v"], A["ff"], A["tq"], sn, A["p"])
+        G.implimf_3d(l, A["u"], A["dt"], A["v"], A["d"], A["ff"], A["mu"], A["mut"], 3, sn, vol, A["p"], A["c"])
+
+
+def test_residency(mesh):
+    """The mirrors' residency protocol under the drop-ins: a resident stage
+    leaves the host arrays as they were until codens_host_sync_all, which
+    brings back exactly the copy-mode results; a second stage with no host
+    access in between moves only zvismo's 4-byte counter each way per block;
+    codens_host_modified refuses an array with device-newer windows; turning
+    residency off brings everything back."""
+    from codens_amd import abi
+    lay, sn, vol = mesh
+    rng = np.random.default_rng(59)
+    n, nb = lay.ip11, len(lay.blocks)
+    G = C.Ref(lay, C.orc_phys(), lib=oracle.dropin_lib(), ilim=1, ns=True)
+    v = random_state(n, rng)
+    p, t, c = all_thermo(v)
+    base = dict(v=np.vstack([v, np.zeros((2, n))]), p=p, t=t, c=c, mu=np.zeros(n), mut=1e-4 * rng.random(n),
+                dt=np.zeros(n), u=np.zeros((7, n)), ff=np.zeros((7, n)), d=np.zeros((7, n)),
+                tq=0.01 * rng.standard_normal((9, n)))
+
+    def fresh():
+        return {k: np.ascontiguousarray(a.copy()) for k, a in base.items()}
+    assert abi.host_residency(0) == 0
+    A = fresh()
+    _dropin_stage(G, lay, A, sn, vol)
+    S1 = {k: a.copy() for k, a in A.items()}
+    _dropin_stage(G, lay, A, sn, vol)
+    S2 = A
+    B = fresh()
+    abi.host_residency(1)
+    try:
+        _dropin_stage(G, lay, B, sn, vol)
+        for k in ("v", "p", "u", "d", "dt"):
+            assert np.array_equal(B[k], base[k]), k  # nothing came back yet
+        abi.host_sync_all()
+        for k in S1:
+            assert np.array_equal(B[k], S1[k]), k
+        h0, d0 = abi.host_traffic()
+        _dropin_stage(G, lay, B, sn, vol)
+        h1, d1 = abi.host_traffic()
+        assert (h1 - h0, d1 - d0) == (4 * nb, 4 * nb)
+        with pytest.raises(abi.CodensError):
+            abi.host_modified(B["v"])
+        abi.host_sync(B["v"])
+        abi.host_modified(B["v"])  # now current on the host: allowed
+    finally:
+        abi.host_residency(0)
+    for k in S2:
+        assert np.array_equal(B[k], S2[k]), k
+
+
 @pytest.mark.parametrize("lmx,ityprk", [(1, 0), (4, 0), (3, 1)])
 def test_implicit_and_explicit_update(mesh, lmx, ityprk):
     lay, sn, vol = mesh
@@ -555,7 +615,8 @@ def test_dropin_stage_rate():
     and its outputs back.  The rest of the stage (the state's ghost cells)
     runs on the oracle between them and is not timed.  Prints the rate over
     three timed stages (after one untimed) and checks that the state stays
-    physical."""
+    physical; then the same with the mirrors resident (codens_host_residency),
+    with the bytes each stage moves."""
     import time
     import cpu_stage
     from codens_amd import configs
@@ -595,4 +656,39 @@ def test_dropin_stage_rate():
     print("drop-in path, 256x128x64: %.2f ms per stage in the drop-in routines, %.1f M cell-updates/s "
           "(PCIe-inclusive)" % (1e3 * sec, rate))
     inner = case.layout.interior()
+    assert np.isfinite(st.v[:, inner]).all() and (st.v[0, inner] > 0).all()
+
+    # The same with the mirrors resident: the stage's only host code, the
+    # oracle's rbord, gets v, p, T, c synchronised before it and marked
+    # written after it (timed with the drop-ins: it is PCIe traffic).
+    from codens_amd import abi
+    host_arrays = (st.v, st.ps, st.temp, st.cson)
+    orig_rbord = st.rbord
+
+    def rbord():
+        t0 = time.perf_counter()
+        for a in host_arrays:
+            abi.host_sync(a)
+        spent[0] += time.perf_counter() - t0
+        orig_rbord()
+        t0 = time.perf_counter()
+        for a in host_arrays:
+            abi.host_modified(a)
+        spent[0] += time.perf_counter() - t0
+    st.rbord = rbord
+    abi.host_residency(1)
+    try:
+        st.stage()
+        spent[0] = 0.0
+        h0, d0 = abi.host_traffic()
+        for _ in range(3):
+            st.stage()
+        h1, d1 = abi.host_traffic()
+    finally:
+        abi.host_residency(0)
+    sec_r = spent[0] / 3
+    cells = case.layout.ncells
+    print("drop-in path, resident mirrors: %.2f ms per stage, %.1f M cell-updates/s; %.0f B/cell host->device, "
+          "%.0f B/cell device->host per stage (rbord's v, p, T, c and zvisqc's stress ghosts on the host)"
+          % (1e3 * sec_r, cells / sec_r / 1e6, (h1 - h0) / 3 / cells, (d1 - d0) / 3 / cells))
     assert np.isfinite(st.v[:, inner]).all() and (st.v[0, inner] > 0).all()

@@ -75,6 +75,18 @@ module codens_c
        integer(c_int64_t), value :: bytes, offset, width
        type(c_ptr), value :: stream
      end function codens_host_bundle_fetch
+     integer(c_int32_t) function codens_bc_tq_copy(nf, ghost, inner, tq, ldv, stream) &
+          bind(C, name="codens_bc_tq_copy")
+       import
+       integer(c_int64_t), value :: nf, ldv
+       type(c_ptr), value :: ghost, inner, tq, stream
+     end function codens_bc_tq_copy
+     integer(c_int32_t) function codens_bc_tq_coincident(nf, ghost, inner, remote, tq, ldv, stream) &
+          bind(C, name="codens_bc_tq_coincident")
+       import
+       integer(c_int64_t), value :: nf, ldv
+       type(c_ptr), value :: ghost, inner, remote, tq, stream
+     end function codens_bc_tq_coincident
      integer(c_int32_t) function codens_host_residency(on, previous, stream) bind(C, name="codens_host_residency")
        import
        integer(c_int32_t), value :: on

@@ -76,19 +76,29 @@ contains
 end module mod_zvismo
 
 module mod_zvisqc
+  use iso_c_binding, only: c_int64_t, c_ptr, c_null_ptr
   implicit none
+  ! device copies of the stress-ghost lists (0-based storage indices): rbte's
+  ! (ghost, adjacent cell) over nfbn and nfba, rbtc's (ghost, adjacent cell,
+  ! cell across) over nfbc, kept while they stay the same
+  integer(c_int64_t), allocatable, target, save :: zq_g(:), zq_i(:), zq_cg(:), zq_ci(:), zq_cr(:)
+  ! the gradient-weight array last found all ones (checked once per array)
+  type(c_ptr), save :: zq_cmu = c_null_ptr
 contains
   ! zvisqc.f90:4 -- stresses and heat flux toxx..qcz of every block of grid
   ! level img: Green-Gauss gradients (zgrad2) and the stress and heat-flux
-  ! closure (zfluto) in one device pass per block, then the ghosts by the
-  ! reference's own rbte and rbtc over the frontier lists nfbn, nfba, nfbc
-  ! (zvisqc.f90:182-205).  The transition and multigrid updates of mut
-  ! (atctranske, smg_fcs) run on the host first, as they read neither the
-  ! gradients nor the stresses.  The gradient scratch dvxx..dvzz is not
-  ! written.  Under residency, the host routines' arrays are synchronised
-  ! before them and marked written after them.  Stops for ischema = 1
-  ! (zgrad's gradients) and for gradient weights cmui1..cmuk2 other than 1
-  ! (inivec.f90:107-115).
+  ! closure (zfluto) in one device pass per block, then the ghosts of
+  ! zvisqc.f90:182-205 on the device: rbte's copies over the frontier lists
+  ! nfbn and nfba, rbtc's averages over nfbc.  Where rbtc has no device form
+  ! (a rotated periodic frontier, a partner frontier on another process or
+  ! outside the list) the reference's own rbte and rbtc run on the host.  The
+  ! transition and multigrid updates of mut (atctranske, smg_fcs) run on the
+  ! host first, as they read neither the gradients nor the stresses.  The
+  ! gradient scratch dvxx..dvzz is not written.  Under residency, host
+  ! routines' arrays are synchronised before them and marked written after
+  ! them.  Stops for ischema = 1 (zgrad's gradients) and for gradient weights
+  ! cmui1..cmuk2 other than 1 (inivec.f90:107-115; checked on the first call
+  ! with a given cmui1, as the reference sets them once).
   subroutine zvisqc(img, s, mu, ro, x, y, z, mut, dist, mnpar, fgam, toxx, toxy, toxz, toyy, toyz, tozz, &
        qcx, qcy, qcz, icyc, mcyturb, ncbd, ncin, mnc, sn, vol, dvxx, dvxy, dvxz, dvyx, dvyy, dvyz, dvzx, &
        dvzy, dvzz, cmui1, cmui2, cmuj1, cmuj2, cmuk1, cmuk2, ztemp)
@@ -112,21 +122,26 @@ contains
     double precision :: ro(ip11), x(ip21), y(ip21), z(ip21), dist(ip12), fgam(ip12)
     double precision :: dvxx(ip00), dvxy(ip00), dvxz(ip00), dvyx(ip00), dvyy(ip00), dvyz(ip00)
     double precision :: dvzx(ip00), dvzy(ip00), dvzz(ip00)
-    double precision :: cmui1(ip21), cmui2(ip21), cmuj1(ip21), cmuj2(ip21), cmuk1(ip21), cmuk2(ip21)
+    double precision, target :: cmui1(ip21), cmui2(ip21), cmuj1(ip21), cmuj2(ip21), cmuk1(ip21), cmuk2(ip21)
     type(codens_block) :: b
     type(c_ptr) :: dv, dtm, dmu, dmut, dsn, dvol, dtq, tq(9)
     integer :: k, l, lm, mf, npsn, n0, n1
+    logical :: check_cmu
     call codens_need_3d(equat, 'zvisqc')
     if (ischema .eq. 1) error stop 'zvisqc: ischema = 1 (zgrad) has no device form'
     tq = [c_loc(toxx), c_loc(toxy), c_loc(toxz), c_loc(toyy), c_loc(toyz), c_loc(tozz), c_loc(qcx), &
          c_loc(qcy), c_loc(qcz)]
+    dtq = c_null_ptr
+    check_cmu = .not. c_associated(zq_cmu, c_loc(cmui1))
     do l = 1, lzx
        lm = l + (img-1)*lz
        n0 = npc(lm) + 1
        n1 = npc(lm) + nnn(lm)
-       if (any(cmui1(n0:n1) .ne. 1.) .or. any(cmui2(n0:n1) .ne. 1.) .or. any(cmuj1(n0:n1) .ne. 1.) .or. &
-           any(cmuj2(n0:n1) .ne. 1.) .or. any(cmuk1(n0:n1) .ne. 1.) .or. any(cmuk2(n0:n1) .ne. 1.)) &
-           error stop 'zvisqc: gradient weights other than 1 have no device form'
+       if (check_cmu) then
+          if (any(cmui1(n0:n1) .ne. 1.) .or. any(cmui2(n0:n1) .ne. 1.) .or. any(cmuj1(n0:n1) .ne. 1.) .or. &
+              any(cmuj2(n0:n1) .ne. 1.) .or. any(cmuk1(n0:n1) .ne. 1.) .or. any(cmuk2(n0:n1) .ne. 1.)) &
+              error stop 'zvisqc: gradient weights other than 1 have no device form'
+       endif
        if (icyc .gt. icytur0 .and. mcyturb .eq. 1) then
           call codens_sync(c_loc(s)); call codens_sync(c_loc(mu)); call codens_sync(c_loc(mut))
           call codens_sync(c_loc(vol))
@@ -148,7 +163,22 @@ contains
        dtq = codens_tq_in(tq, ip12, b)
        call codens_check(codens_zvisqc(b, codens_phys_now(), dsn, dvol, dv, int(ip11, c_int64_t), dtm, dmu, &
             dmut, dtq, c_null_ptr), 'zvisqc')
-       call codens_tq_back(tq, ip12, b)
+    enddo
+    zq_cmu = c_loc(cmui1)
+    ! the ghosts: on the device when every listed frontier has a device form
+    ! (the bundle of all blocks' stresses is dtq), else on the host
+    if (c_associated(dtq) .and. ghost_lists(img, ncbd, ncin)) then
+       if (size(zq_g) > 0) call codens_check(codens_bc_tq_copy(int(size(zq_g), c_int64_t), &
+            list_dev(zq_g), list_dev(zq_i), dtq, int(ip12, c_int64_t), c_null_ptr), 'zvisqc')
+       if (size(zq_cg) > 0) call codens_check(codens_bc_tq_coincident(int(size(zq_cg), c_int64_t), &
+            list_dev(zq_cg), list_dev(zq_ci), list_dev(zq_cr), dtq, int(ip12, c_int64_t), c_null_ptr), 'zvisqc')
+       do l = 1, lzx
+          call codens_tq_back(tq, ip12, codens_block_of(l + (img-1)*lz))
+       enddo
+       return
+    endif
+    do l = 1, lzx
+       call codens_tq_back(tq, ip12, codens_block_of(l + (img-1)*lz))
     enddo
     do k = 1, 9
        call codens_sync(tq(k))
@@ -172,6 +202,97 @@ contains
        call codens_modified(tq(k))
     enddo
   end subroutine zvisqc
+
+  ! device address of a ghost list (copied when first made or rebuilt)
+  function list_dev(a) result(dev)
+    use codens_c
+    integer(c_int64_t), target, intent(in) :: a(:)
+    type(c_ptr) :: dev
+    integer(c_int64_t) :: nb
+    nb = 8_c_int64_t * size(a)
+    call codens_check(codens_host_mirror(c_loc(a), nb, 0_c_int64_t, nb, nb, 1_c_int32_t, 1_c_int32_t, dev, &
+         c_null_ptr), 'zvisqc')
+  end function list_dev
+
+  ! The stress-ghost lists of rbte over nfbn, nfba and of rbtc over nfbc
+  ! (zvisqc.f90:182-205), face m of frontier f at mpb(f)+m: ghost ncbd,
+  ! adjacent cell ncin, and for rbtc the partner frontier's adjacent cell of
+  ! the same face number (rbtc.f90:54-117).  Built on every call and kept
+  ! (with their device copies) while they come out the same.  False where
+  ! rbtc has no device form: a rotated periodic frontier (mper /= 0), a
+  ! partner on another process, outside nfbc, or of another face count.
+  logical function ghost_lists(img, ncbd, ncin)
+    use para_var
+    use para_fige
+    use boundary
+    use mod_mpi
+    use codens_c
+    integer, intent(in) :: img, ncbd(ip41), ncin(ip41)
+    integer(c_int64_t), allocatable :: g(:), i(:), cg(:), ci(:), cr(:)
+    integer, allocatable :: fr(:), part(:)
+    integer :: mf2, mfb, m, nn, nc, q, f, nlist
+    ghost_lists = .false.
+    nlist = mtnx + mtax + mtcx
+    allocate(fr(nlist), part(nlist))
+    fr(1:mtnx) = nfbn(1:mtnx) + (img-1)*mtb
+    fr(mtnx+1:mtnx+mtax) = nfba(1:mtax) + (img-1)*mtb
+    fr(mtnx+mtax+1:) = nfbc(1:mtcx) + (img-1)*mtb
+    part = 0
+    nn = 0
+    do q = 1, mtnx + mtax
+       nn = nn + mmb(fr(q))
+    enddo
+    nc = 0
+    do q = mtnx + mtax + 1, nlist
+       mfb = fr(q)
+       if (mper(mfb) /= 0) return
+       do mf2 = mtnx + mtax + 1, nlist
+          if (bcl_to_bcg(fr(mf2)) == ndcc(mfb)) part(q) = fr(mf2)
+       enddo
+       if (part(q) == 0) return
+       if (bcg_to_proc(bcl_to_bcg(mfb)) /= bcg_to_proc(ndcc(mfb)) .or. mmb(part(q)) /= mmb(mfb)) return
+       nc = nc + mmb(mfb)
+    enddo
+    allocate(g(nn), i(nn), cg(nc), ci(nc), cr(nc))
+    f = 0
+    do q = 1, mtnx + mtax
+       do m = 1, mmb(fr(q))
+          f = f + 1
+          g(f) = ncbd(mpb(fr(q)) + m) - 1
+          i(f) = ncin(mpb(fr(q)) + m) - 1
+       enddo
+    enddo
+    f = 0
+    do q = mtnx + mtax + 1, nlist
+       do m = 1, mmb(fr(q))
+          f = f + 1
+          cg(f) = ncbd(mpb(fr(q)) + m) - 1
+          ci(f) = ncin(mpb(fr(q)) + m) - 1
+          cr(f) = ncin(mpb(part(q)) + m) - 1
+       enddo
+    enddo
+    ghost_lists = .true.
+    if (allocated(zq_g)) then
+       if (same(zq_g, g) .and. same(zq_i, i) .and. same(zq_cg, cg) .and. same(zq_ci, ci) .and. same(zq_cr, cr)) &
+            return
+       ! other lists: the device copies of the kept ones are stale
+       call forget(zq_g); call forget(zq_i); call forget(zq_cg); call forget(zq_ci); call forget(zq_cr)
+       deallocate(zq_g, zq_i, zq_cg, zq_ci, zq_cr)
+    endif
+    call move_alloc(g, zq_g); call move_alloc(i, zq_i)
+    call move_alloc(cg, zq_cg); call move_alloc(ci, zq_ci); call move_alloc(cr, zq_cr)
+  contains
+    subroutine forget(a)
+      integer(c_int64_t), target, intent(in) :: a(:)
+      if (size(a) > 0) call codens_modified(c_loc(a))
+    end subroutine forget
+
+    logical function same(a, b)
+      integer(c_int64_t), intent(in) :: a(:), b(:)
+      same = size(a) == size(b)
+      if (same) same = all(a == b)
+    end function same
+  end function ghost_lists
 end module mod_zvisqc
 
 module mod_chronos

@@ -912,11 +912,20 @@ contains
     integer(c_int), value :: nn, nc
     integer(c_int) :: frn(*), frc(*), ncbd(*), ncin(*)
     real(c_double) :: s(*), mu(*), mut(*), tq(ip12, 9), sn(*), vol(*), temp(*)
-    real(c_double), allocatable :: xyz(:), cmu(:), g(:, :), dist(:), fgam(:)
-    integer, allocatable :: mnpar(:)
+    ! the routine's scratch and unused inputs, kept between calls (as a
+    ! driver keeps its arrays) and remade when the layout grows
+    real(c_double), allocatable, save :: xyz(:), cmu(:), g(:, :), dist(:), fgam(:)
+    integer, allocatable, save :: mnpar(:)
     integer :: mnc(1), f, isch0
-    allocate(xyz(ip21), cmu(ip21), g(ip00, 9), dist(ip12), fgam(ip12), mnpar(ip12))
-    xyz = 0.; cmu = 1.; g = 0.; dist = 1.; fgam = 0.; mnpar = 0; mnc = 0
+    if (allocated(xyz)) then
+       if (size(xyz) < ip21 .or. size(g, 1) < ip00 .or. size(dist) < ip12) &
+            deallocate(xyz, cmu, g, dist, fgam, mnpar)
+    endif
+    if (.not. allocated(xyz)) then
+       allocate(xyz(ip21), cmu(ip21), g(ip00, 9), dist(ip12), fgam(ip12), mnpar(ip12))
+       xyz = 0.; cmu = 1.; g = 0.; dist = 1.; fgam = 0.; mnpar = 0
+    endif
+    mnc = 0
     if (allocated(nfbn)) deallocate(nfbn)
     if (allocated(nfbc)) deallocate(nfbc)
     if (allocated(nfba)) deallocate(nfba)

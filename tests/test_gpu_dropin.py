@@ -616,7 +616,8 @@ def test_dropin_stage_rate():
     runs on the oracle between them and is not timed.  Prints the rate over
     three timed stages (after one untimed) and checks that the state stays
     physical; then the same with the mirrors resident (codens_host_residency),
-    with the bytes each stage moves."""
+    first with the oracle's rbord on the host between the drop-ins, then with
+    the ghost step on the device, with the bytes each stage moves."""
     import time
     import cpu_stage
     from codens_amd import configs
@@ -689,6 +690,70 @@ def test_dropin_stage_rate():
     sec_r = spent[0] / 3
     cells = case.layout.ncells
     print("drop-in path, resident mirrors: %.2f ms per stage, %.1f M cell-updates/s; %.0f B/cell host->device, "
-          "%.0f B/cell device->host per stage (rbord's v, p, T, c and zvisqc's stress ghosts on the host)"
+          "%.0f B/cell device->host per stage (rbord's v, p, T, c on the host)"
           % (1e3 * sec_r, cells / sec_r / 1e6, (h1 - h0) / 3 / cells, (d1 - d0) / 3 / cells))
+    assert np.isfinite(st.v[:, inner]).all() and (st.v[0, inner] > 0).all()
+
+    # A driver whose ghost step runs on the device as well: the C-ABI boundary
+    # routines (rbord's sequence) on the drop-ins' own device copies of v, p,
+    # T, c, whose windows it then declares device-newer.  No host code touches
+    # the stage's arrays, so a resident stage moves only zvismo's counter.
+    import ctypes
+    import torch
+    from codens_amd._lib import lib
+    lay = case.layout
+    n = lay.ip11
+    i64 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int64), device="cuda")
+    f64 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device="cuda")
+    cg, w, fa, sy = case.coincident, case.wall, case.farfield, case.sym
+    L = dict(cg=i64(cg.ghost), ci=i64(cg.inner), cr=i64(cg.remote if cg.remote is not None else cg.inner),
+             wg=i64(w.ghost), wi=i64(w.inner), fg=i64(fa.ghost), fi=i64(fa.inner),
+             fn=f64(fa.normals if fa.normals is not None else np.zeros((3, 0))),
+             yg=i64(sy.ghost), yi=i64(sy.inner), yn=f64(sy.normals if sy.normals is not None else np.zeros((3, 0))))
+    P = lambda k: ctypes.c_void_p(L[k].data_ptr())
+    ph = abi.phys(**case.phys)
+    w0 = (ctypes.c_double * 5)(*[float(x) for x in case.winf])
+    torch.cuda.synchronize()
+    arrays = ((st.v, 5), (st.ps, 1), (st.temp, 1), (st.cson, 1))
+
+    def rbord_dev():
+        t0 = time.perf_counter()
+        d = []
+        for a, rows in arrays:
+            for b in lay.blocks:
+                base = abi.host_mirror(a, 8 * b.npc, 8 * b.nnn, 8 * n, rows)
+            d.append(ctypes.c_void_p(base))
+        V, Pp, T, Cc = d
+        chk = lambda rc: rc == 0 or pytest.fail(lib.codens_last_error().decode())
+        if L["cg"].numel():
+            chk(lib.codens_bc_coincident(L["cg"].numel(), P("cg"), P("ci"), P("cr"), V, n, Pp, T, Cc, None))
+        if L["wg"].numel():
+            chk(lib.codens_bc_wall(L["wg"].numel(), P("wg"), P("wi"), V, n, Pp, T, Cc, None))
+        if L["fg"].numel():
+            chk(lib.codens_bc_farfield(L["fg"].numel(), P("fg"), P("fi"), P("fn"), w0, ctypes.byref(ph), V, n, Pp,
+                                       T, Cc, None))
+        if L["yg"].numel():
+            chk(lib.codens_bc_sym(L["yg"].numel(), P("yg"), P("yi"), P("yn"), V, n, None))
+        if L["cg"].numel():
+            chk(lib.codens_bc_coincident(L["cg"].numel(), P("cg"), P("ci"), P("cr"), V, n, Pp, T, Cc, None))
+        for a, rows in arrays:
+            for b in lay.blocks:
+                abi.host_fetch(a, 8 * b.npc, 8 * b.nnn, 8 * n, rows)
+        spent[0] += time.perf_counter() - t0
+    st.rbord = rbord_dev
+    abi.host_residency(1)
+    try:
+        st.stage()
+        spent[0] = 0.0
+        h0, d0 = abi.host_traffic()
+        for _ in range(3):
+            st.stage()
+        h1, d1 = abi.host_traffic()
+    finally:
+        abi.host_residency(0)
+    sec_d = spent[0] / 3
+    print("drop-in path, resident mirrors and device ghosts: %.2f ms per stage, %.1f M cell-updates/s; "
+          "%d B host->device, %d B device->host per stage" % (1e3 * sec_d, cells / sec_d / 1e6, (h1 - h0) // 3,
+                                                             (d1 - d0) // 3))
+    assert (h1 - h0, d1 - d0) == (3 * 4 * len(lay.blocks), 3 * 4 * len(lay.blocks))
     assert np.isfinite(st.v[:, inner]).all() and (st.v[0, inner] > 0).all()

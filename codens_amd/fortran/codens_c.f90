@@ -75,6 +75,31 @@ module codens_c
        integer(c_int64_t), value :: bytes, offset, width
        type(c_ptr), value :: stream
      end function codens_host_bundle_fetch
+     integer(c_int32_t) function codens_bc_coincident(nf, ghost, inner, remote, v, ldv, ps, temp, cson, stream) &
+          bind(C, name="codens_bc_coincident")
+       import
+       integer(c_int64_t), value :: nf, ldv
+       type(c_ptr), value :: ghost, inner, remote, v, ps, temp, cson, stream
+     end function codens_bc_coincident
+     integer(c_int32_t) function codens_bc_wall(nf, ghost, inner, v, ldv, ps, temp, cson, stream) &
+          bind(C, name="codens_bc_wall")
+       import
+       integer(c_int64_t), value :: nf, ldv
+       type(c_ptr), value :: ghost, inner, v, ps, temp, cson, stream
+     end function codens_bc_wall
+     integer(c_int32_t) function codens_bc_farfield(nf, ghost, inner, nrm, w0, ph, v, ldv, ps, temp, cson, stream) &
+          bind(C, name="codens_bc_farfield")
+       import
+       integer(c_int64_t), value :: nf, ldv
+       type(c_ptr), value :: ghost, inner, nrm, v, ps, temp, cson, stream
+       real(c_double) :: w0(5)
+       type(codens_phys) :: ph
+     end function codens_bc_farfield
+     integer(c_int32_t) function codens_bc_sym(nf, ghost, inner, nrm, v, ldv, stream) bind(C, name="codens_bc_sym")
+       import
+       integer(c_int64_t), value :: nf, ldv
+       type(c_ptr), value :: ghost, inner, nrm, v, stream
+     end function codens_bc_sym
      integer(c_int32_t) function codens_bc_tq_copy(nf, ghost, inner, tq, ldv, stream) &
           bind(C, name="codens_bc_tq_copy")
        import

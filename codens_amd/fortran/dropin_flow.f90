@@ -589,3 +589,249 @@ contains
     call codens_back(c_loc(d), ip11, 6, 2, b)
   end subroutine met_roe
 end module mod_met_roe
+
+module mod_rbord
+  use iso_c_binding, only: c_int64_t, c_double
+  implicit none
+  ! ghost lists (0-based storage indices) and unit normals, kept with their
+  ! device copies while they come out the same: coincident (ghost, adjacent
+  ! cell, cell across), walls, far fields and symmetry planes (ghost,
+  ! adjacent cell); normals as one (3, faces) block per frontier
+  integer(c_int64_t), allocatable, target, save :: rb_cg(:), rb_ci(:), rb_cr(:), rb_wg(:), rb_wi(:)
+  integer(c_int64_t), allocatable, target, save :: rb_fg(:), rb_fi(:), rb_yg(:), rb_yi(:)
+  real(c_double), allocatable, target, save :: rb_fn(:), rb_yn(:)
+contains
+  ! rbord.f90:4 -- the ghost cells of v, p, T, c on every frontier of grid
+  ! level img, on the device: rbvc's averages over the coincident frontiers
+  ! nfbc, the frontiers of nba by kind ('para': clpara; 'nrd': utnrd, rbve,
+  ! clnrd; 'sym': clsym; 'rc': nothing), then rbvc again (rbord.f90:137-429);
+  ! p, T, c of the symmetry ghosts keep their values, as clsym leaves them.
+  ! Each kind writes its own ghost cells from real cells, so the kinds run
+  ! grouped, one launch per kind (per far-field frontier: each has its state,
+  ! from the reference's utnrd on the host).  Other kinds, preconditioned
+  ! far fields (kprec >= 1), a far-field state that varies along the frontier
+  ! (a rotating frame) and rotated or remote coincident partners stop: they
+  ! have no device form.
+  subroutine rbord(kpst, img, mnc, ncin, mnr, xnr, ynr, znr, nxn, nyn, nzn, ncbd, sn, vol, v, dt, bceqt, &
+       rpi, rti, d0x, d0y, d0z, x, y, z, pres, tp, rod, roud, rovd, rowd, roed, icyc, tm1, tm2, tm3, tm4, &
+       tm5, tm6, tm7, tm8, tm9, tm10, tm11, tm12, tm13, pression, ztemp, cson)
+    use para_var
+    use para_fige
+    use maillage
+    use boundary
+    use schemanum
+    use definition
+    use mod_mpi
+    use mod_utnrd
+    use codens_c
+    implicit none
+    integer :: icyc, img, kpst, mnc(ip43), mnr(ip44), ncbd(ip41), ncin(ip41)
+    double precision :: bceqt(ip41,neqt), cson(ip11), d0x(ip40), d0y(ip40), d0z(ip40), dt(ip11)
+    double precision :: nxn(ip42), nyn(ip42), nzn(ip42), pres(ip40), pression(ip11), rod(ip40), roed(ip40)
+    double precision :: roud(ip40), rovd(ip40), rowd(ip40), rpi(ip40), rti(ip40), sn(ip31*ndir), tm1(ip40)
+    double precision :: tm10(ip40), tm11(ip40), tm12(ip40), tm13(ip40), tm2(ip40), tm3(ip40), tm4(ip40)
+    double precision :: tm5(ip40), tm6(ip40), tm7(ip40), tm8(ip40), tm9(ip40), tp(ip40), v(ip11,ip60)
+    double precision :: vol(ip11), x(ip21), xnr(ip44), y(ip21), ynr(ip44), z(ip21), znr(ip44), ztemp(ip11)
+    target :: v, pression, ztemp, cson
+    integer(c_int64_t), allocatable :: cg(:), ci(:), cr(:), wg(:), wi(:), fg(:), fi(:), yg(:), yi(:)
+    real(c_double), allocatable :: fn(:), yn(:), w0(:, :)
+    integer, allocatable :: nff(:)
+    integer :: no, mfb, mfbm, l, lm, m, q, mf2, part, nc, nw, nf, ny, kf, nfr
+    type(c_ptr) :: dv, dp, dtm, dc
+    type(codens_block) :: b
+    integer(c_int64_t) :: ld
+    ! lists of this call
+    nc = 0; nw = 0; nf = 0; ny = 0; nfr = 0
+    do q = 1, mtcx
+       nc = nc + mmb(nfbc(q))
+    enddo
+    do no = 1, mtbx
+       mfb = nba(no)
+       mfbm = mfb + (img-1)*mtb
+       select case (kind_of(cl(mfb)))
+       case (1)
+          nw = nw + mmb(mfbm)
+       case (2)
+          if (kprec .ne. 0) error stop 'rbord: preconditioned far field (kprec >= 1) has no device form'
+          nf = nf + mmb(mfbm); nfr = nfr + 1
+       case (3)
+          ny = ny + mmb(mfbm)
+       case (4)
+       case default
+          write(*, '("rbord: frontier kind ",a," has no device form")') cl(mfb)
+          error stop 'rbord: frontier kind without a device form'
+       end select
+    enddo
+    allocate(cg(nc), ci(nc), cr(nc), wg(nw), wi(nw), fg(nf), fi(nf), yg(ny), yi(ny), fn(3*nf), yn(3*ny))
+    allocate(w0(5, nfr), nff(nfr))
+    ! coincident (rbvc.f90): face m of frontier f takes the partner's
+    ! adjacent cell of face m
+    nc = 0
+    do q = 1, mtcx
+       mfbm = nfbc(q)
+       part = 0
+       do mf2 = 1, mtcx
+          if (bcl_to_bcg(nfbc(mf2)) == ndcc(mfbm)) part = nfbc(mf2)
+       enddo
+       if (part == 0 .or. mper(mfbm) /= 0 .or. bcg_to_proc(bcl_to_bcg(mfbm)) /= bcg_to_proc(ndcc(mfbm))) &
+            error stop 'rbord: a coincident frontier without a device form (rotated, or its partner remote)'
+       do m = 1, mmb(mfbm)
+          nc = nc + 1
+          cg(nc) = ncbd(mpb(mfbm) + m) - 1
+          ci(nc) = ncin(mpb(mfbm) + m) - 1
+          cr(nc) = ncin(mpb(part) + m) - 1
+       enddo
+    enddo
+    nw = 0; nf = 0; ny = 0; kf = 0
+    do no = 1, mtbx
+       mfb = nba(no)
+       mfbm = mfb + (img-1)*mtb
+       l = ndlb(mfb)
+       lm = l + (img-1)*lz
+       select case (kind_of(cl(mfb)))
+       case (1)
+          do m = 1, mmb(mfbm)
+             nw = nw + 1
+             wg(nw) = ncbd(mpb(mfbm) + m) - 1
+             wi(nw) = ncin(mpb(mfbm) + m) - 1
+          enddo
+       case (2)
+          call utnrd(bceqt, mfbm, lm, rod, roud, rovd, rowd, roed, ncbd, y, z)
+          kf = kf + 1
+          nff(kf) = mmb(mfbm)
+          w0(:, kf) = [rod(1), roud(1), rovd(1), rowd(1), roed(1)]
+          do m = 1, mmb(mfbm)
+             if (rod(m) /= rod(1) .or. roud(m) /= roud(1) .or. rovd(m) /= rovd(1) .or. rowd(m) /= rowd(1) .or. &
+                 roed(m) /= roed(1)) error stop 'rbord: a far-field state varying along the frontier (omg /= 0)'
+             fg(nf + m) = ncbd(mpb(mfbm) + m) - 1
+             fi(nf + m) = ncin(mpb(mfbm) + m) - 1
+             fn(3*nf + m) = nxn(mpn(mfbm) + m)
+             fn(3*nf + mmb(mfbm) + m) = nyn(mpn(mfbm) + m)
+             fn(3*nf + 2*mmb(mfbm) + m) = nzn(mpn(mfbm) + m)
+          enddo
+          nf = nf + mmb(mfbm)
+       case (3)
+          do m = 1, mmb(mfbm)
+             yg(ny + m) = ncbd(mpb(mfbm) + m) - 1
+             yi(ny + m) = ncin(mpb(mfbm) + m) - 1
+          enddo
+          ! clsym takes the normals of each face; one (3, faces) block for all
+          ny = ny + mmb(mfbm)
+       end select
+    enddo
+    ny = 0
+    do no = 1, mtbx
+       mfb = nba(no)
+       mfbm = mfb + (img-1)*mtb
+       if (kind_of(cl(mfb)) /= 3) cycle
+       do m = 1, mmb(mfbm)
+          yn(ny + m) = nxn(mpn(mfbm) + m)
+          yn(size(yg) + ny + m) = nyn(mpn(mfbm) + m)
+          yn(2*size(yg) + ny + m) = nzn(mpn(mfbm) + m)
+       enddo
+       ny = ny + mmb(mfbm)
+    enddo
+    call keep_i(rb_cg, cg); call keep_i(rb_ci, ci); call keep_i(rb_cr, cr)
+    call keep_i(rb_wg, wg); call keep_i(rb_wi, wi); call keep_i(rb_fg, fg); call keep_i(rb_fi, fi)
+    call keep_i(rb_yg, yg); call keep_i(rb_yi, yi); call keep_d(rb_fn, fn); call keep_d(rb_yn, yn)
+    ! every block's window of v, p, T, c on the device (frontiers reach
+    ! across blocks); the last call gives the buffers' base addresses
+    do l = 1, lzx
+       b = codens_block_of(l + (img-1)*lz)
+       dv = codens_dev(c_loc(v), ip11*ip60, ip11, 1, 5, b, 1)
+       dp = codens_dev(c_loc(pression), ip11, ip11, 1, 1, b, 1)
+       dtm = codens_dev(c_loc(ztemp), ip11, ip11, 1, 1, b, 1)
+       dc = codens_dev(c_loc(cson), ip11, ip11, 1, 1, b, 1)
+    enddo
+    ld = int(ip11, c_int64_t)
+    if (size(rb_cg) > 0) call codens_check(codens_bc_coincident(int(size(rb_cg), c_int64_t), ldev(rb_cg), &
+         ldev(rb_ci), ldev(rb_cr), dv, ld, dp, dtm, dc, c_null_ptr), 'rbord')
+    if (size(rb_wg) > 0) call codens_check(codens_bc_wall(int(size(rb_wg), c_int64_t), ldev(rb_wg), ldev(rb_wi), &
+         dv, ld, dp, dtm, dc, c_null_ptr), 'rbord')
+    nf = 0
+    do kf = 1, nfr
+       call codens_check(codens_bc_farfield(int(nff(kf), c_int64_t), codens_ptr_off(ldev(rb_fg), 8_c_int64_t*nf), &
+            codens_ptr_off(ldev(rb_fi), 8_c_int64_t*nf), codens_ptr_off(ddev(rb_fn), 24_c_int64_t*nf), w0(:, kf), &
+            codens_phys_now(), dv, ld, dp, dtm, dc, c_null_ptr), 'rbord')
+       nf = nf + nff(kf)
+    enddo
+    if (size(rb_yg) > 0) call codens_check(codens_bc_sym(int(size(rb_yg), c_int64_t), ldev(rb_yg), ldev(rb_yi), &
+         ddev(rb_yn), dv, ld, c_null_ptr), 'rbord')
+    if (size(rb_cg) > 0) call codens_check(codens_bc_coincident(int(size(rb_cg), c_int64_t), ldev(rb_cg), &
+         ldev(rb_ci), ldev(rb_cr), dv, ld, dp, dtm, dc, c_null_ptr), 'rbord')
+    do l = 1, lzx
+       b = codens_block_of(l + (img-1)*lz)
+       call codens_back(c_loc(v), ip11, 1, 5, b)
+       call codens_back(c_loc(pression), ip11, 1, 1, b)
+       call codens_back(c_loc(ztemp), ip11, 1, 1, b)
+       call codens_back(c_loc(cson), ip11, 1, 1, b)
+    enddo
+  end subroutine rbord
+
+  ! rbord.f90:139-280's tests of a frontier's kind: 1 'para', 2 'nrd*',
+  ! 3 'sym*', 4 'rc*', 0 any other
+  integer function kind_of(c)
+    character(len=4), intent(in) :: c
+    kind_of = 0
+    if (c .eq. 'para') then
+       kind_of = 1
+    else if (c(1:3) .eq. 'nrd') then
+       kind_of = 2
+    else if (c(1:3) .eq. 'sym') then
+       kind_of = 3
+    else if (c(1:3) .ne. 'rec' .and. c(1:3) .ne. 'rnc' .and. c(1:2) .eq. 'rc') then
+       kind_of = 4
+    endif
+  end function kind_of
+
+  ! device addresses of the kept lists (copied when new or changed)
+  function ldev(a) result(dev)
+    use codens_c
+    integer(c_int64_t), target, intent(in) :: a(:)
+    type(c_ptr) :: dev
+    integer(c_int64_t) :: nb
+    nb = 8_c_int64_t * size(a)
+    call codens_check(codens_host_mirror(c_loc(a), nb, 0_c_int64_t, nb, nb, 1_c_int32_t, 1_c_int32_t, dev, &
+         c_null_ptr), 'rbord')
+  end function ldev
+
+  function ddev(a) result(dev)
+    use codens_c
+    real(c_double), target, intent(in) :: a(:)
+    type(c_ptr) :: dev
+    integer(c_int64_t) :: nb
+    nb = 8_c_int64_t * size(a)
+    call codens_check(codens_host_mirror(c_loc(a), nb, 0_c_int64_t, nb, nb, 1_c_int32_t, 1_c_int32_t, dev, &
+         c_null_ptr), 'rbord')
+  end function ddev
+
+  ! keep: the new list in place of the kept one unless they are equal (a
+  ! replaced list's device copy is forgotten first)
+  subroutine keep_i(kept, new)
+    use codens_c
+    integer(c_int64_t), allocatable, target, intent(inout) :: kept(:)
+    integer(c_int64_t), allocatable, intent(inout) :: new(:)
+    if (allocated(kept)) then
+       if (size(kept) == size(new)) then
+          if (all(kept == new)) return
+       endif
+       if (size(kept) > 0) call codens_modified(c_loc(kept))
+       deallocate(kept)
+    endif
+    call move_alloc(new, kept)
+  end subroutine keep_i
+
+  subroutine keep_d(kept, new)
+    use codens_c
+    real(c_double), allocatable, target, intent(inout) :: kept(:)
+    real(c_double), allocatable, intent(inout) :: new(:)
+    if (allocated(kept)) then
+       if (size(kept) == size(new)) then
+          if (all(kept == new)) return
+       endif
+       if (size(kept) > 0) call codens_modified(c_loc(kept))
+       deallocate(kept)
+    endif
+    call move_alloc(new, kept)
+  end subroutine keep_d
+end module mod_rbord

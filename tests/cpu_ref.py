@@ -463,6 +463,28 @@ class RefFrontiers:
         self.lib.ref_zvisqc(len(a), ptr(a), len(c), ptr(c), ptr(v), ptr(mu), ptr(mut), ptr(tq), ptr(self.ncbd),
                             ptr(self.ncin), ptr(sn), ptr(vol), ptr(temp))
 
+    def normals(self, nrm):
+        """nxn, nyn, nzn over every face: nrm[f], the (3, faces) inward normals
+        of frontier f, or None (zeros)."""
+        n = [np.zeros(len(self.ncbd)) for _ in range(3)]
+        for f, x in enumerate(nrm):
+            if x is not None:
+                for q in range(3):
+                    n[q][self.off[f]:self.off[f + 1]] = x[q]
+        return n
+
+    def rbord(self, kinds, blocks, coin, n, v, ps, temp, cson, mach, alpha, beta, roa1, aa1, pa1, y, z):
+        """rbord.f90:4 over every frontier (drop-in build only): kinds[f] in
+        'rc', 'para', 'nrd', 'sym', blocks[f] 0-based, coin the coincident
+        frontiers (nfbc), n = normals(...), one far-field state for every
+        'nrd' frontier."""
+        code = np.ascontiguousarray(np.array([["rc", "para", "nrd", "sym"].index(k) for k in kinds], dtype=np.int32))
+        blk = np.ascontiguousarray(np.array([b_ + 1 for b_ in blocks], dtype=np.int32))
+        c = self._list(coin)
+        self.lib.ref_rbord(self.nfr, ptr(code), ptr(blk), len(c), ptr(c), ptr(self.ncbd), ptr(self.ncin), ptr(n[0]),
+                           ptr(n[1]), ptr(n[2]), ptr(v), ptr(ps), ptr(temp), ptr(cson), D(mach), D(alpha), D(beta),
+                           D(roa1), D(aa1), D(pa1), ptr(y), ptr(z))
+
     def met_rbse(self, frs, fd):
         fr = self._list(frs)
         self.lib.ref_met_rbse(len(fr), ptr(fr), ptr(fd), ptr(self.ncbd), ptr(self.ncin))

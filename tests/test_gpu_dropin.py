@@ -114,6 +114,49 @@ def test_zvisqc(mesh):
     assert np.array_equal(out["r"] == -7.0, out["g"] == -7.0)  # the same entries written
 
 
+def test_rbord():
+    """rbord through the drop-in (the ghost kernels on the GPU, grouped by
+    kind) against the reference's own sequence -- rbvc, each frontier's
+    routine in listing order (clpara, utnrd + rbve + clnrd, clsym), rbvc again
+    -- on the two-block layout of test_oracle_bc.py: two coincident pairs,
+    walls, far fields whose normal Mach numbers take both signs on both sides
+    of 1, and symmetry planes."""
+    import test_oracle_bc as B
+    from codens_amd import layout as L
+    from randcase import random_mesh_xyz
+    lay, sn, vol, xyz = random_mesh_xyz(B.DIMS, seed=11)
+    fr = []
+    for l, f, kind in B.FRONTIERS:
+        g, i, fi = L.face_cells(lay.blocks[l], f)
+        fr.append(dict(l=l, kind=kind, g=g, i=i, nrm=np.stack(L.inward_normals(lay.blocks[l], sn, f, fi))))
+    partner = [1, 0, 3, 2] + [-1] * (len(fr) - 4)
+    gh, inn = [f["g"] for f in fr], [f["i"] for f in fr]
+    R, G = C.Ref(lay, C.orc_phys()), C.Ref(lay, C.orc_phys(), lib=oracle.dropin_lib())
+    RR, RG = C.RefFrontiers(R, gh, inn, partner), C.RefFrontiers(G, gh, inn, partner)
+    v = B.state(lay.ip11, np.random.default_rng(7))
+    a, b = B.arrays(v), B.arrays(v)
+    mach, alpha, beta = 0.8, 10.0, 5.0
+    st = (B.STAG["roa1"], B.STAG["aa1"], B.STAG["pa1"])
+    coin = [0, 1, 2, 3]
+    RR.rbvc(coin, *a)
+    for k, f in enumerate(fr):
+        if f["kind"] == "para":
+            RR.clpara(k, *a)
+        elif f["kind"] == "nrd":
+            RR.farfield(k, f["l"], mach, alpha, beta, *st, f["nrm"], *a)
+        elif f["kind"] == "sym":
+            RR.clsym(k, f["nrm"], a[0])
+    RR.rbvc(coin, *a)
+    x, y, z = xyz
+    RG.rbord([f["kind"] for f in fr], [f["l"] for f in fr], coin,
+             RG.normals([f["nrm"] if f["kind"] in ("nrd", "sym") else None for f in fr]), *b, mach, alpha, beta,
+             *st, y, z)
+    for x_, y_ in zip(a, b):
+        assert np.array_equal(np.isnan(x_), np.isnan(y_))
+        close(np.nan_to_num(x_), np.nan_to_num(y_))
+    assert not np.array_equal(a[0], v)  # the ghosts were written
+
+
 def _dropin_stage(G, lay, A, sn, vol):
     """zpres, zvismo, chronos, sch_roe (minmod, viscous) and implimf_3d
     (lmax = 3) of every block, through the drop-ins, on the host arrays of A."""
@@ -754,6 +797,36 @@ def test_dropin_stage_rate():
     sec_d = spent[0] / 3
     print("drop-in path, resident mirrors and device ghosts: %.2f ms per stage, %.1f M cell-updates/s; "
           "%d B host->device, %d B device->host per stage" % (1e3 * sec_d, cells / sec_d / 1e6, (h1 - h0) // 3,
+                                                             (d1 - d0) // 3))
+    assert (h1 - h0, d1 - d0) == (3 * 4 * len(lay.blocks), 3 * 4 * len(lay.blocks))
+    assert np.isfinite(st.v[:, inner]).all() and (st.v[0, inner] > 0).all()
+
+    # The same ghost step through the rbord drop-in: a stage made only of
+    # drop-in calls, resident.
+    fs = case.farspec
+    kinds = [x[0] for x in st.fr]
+    nrm = RFG.normals([x[1].normals[:, x[2]] if x[0] in ("nrd", "sym") else None for x in st.fr])
+    yz = np.zeros(n)
+
+    def rbord_dropin():
+        t0 = time.perf_counter()
+        RFG.rbord(kinds, [x[3] for x in st.fr], coin, nrm, st.v, st.ps, st.temp, st.cson, fs.mach, fs.alpha,
+                  fs.beta, fs.roa1, fs.aa1, fs.pa1, yz, yz)
+        spent[0] += time.perf_counter() - t0
+    st.rbord = rbord_dropin
+    abi.host_residency(1)
+    try:
+        st.stage()
+        spent[0] = 0.0
+        h0, d0 = abi.host_traffic()
+        for _ in range(3):
+            st.stage()
+        h1, d1 = abi.host_traffic()
+    finally:
+        abi.host_residency(0)
+    sec_e = spent[0] / 3
+    print("drop-in path, resident mirrors and the rbord drop-in: %.2f ms per stage, %.1f M cell-updates/s; "
+          "%d B host->device, %d B device->host per stage" % (1e3 * sec_e, cells / sec_e / 1e6, (h1 - h0) // 3,
                                                              (d1 - d0) // 3))
     assert (h1 - h0, d1 - d0) == (3 * 4 * len(lay.blocks), 3 * 4 * len(lay.blocks))
     assert np.isfinite(st.v[:, inner]).all() and (st.v[0, inner] > 0).all()

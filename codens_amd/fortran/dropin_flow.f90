@@ -14,7 +14,11 @@
 ! (equat(3:4) = '3d'); other forms stop.
 !
 ! Set codens_geom_once = .true. (module codens_c) once all metrics sn and
-! volumes vol are final to upload them only on first use.
+! volumes vol are final to upload them only on first use.  With
+! codens_host_residency(1) the copies back are deferred and the copies in
+! skipped while the device holds the current values: the driver then calls
+! codens_sync before its own code reads an array and codens_modified after
+! its own code writes one (include/codens.h).
 
 module mod_zpres
   implicit none

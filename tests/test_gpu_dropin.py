@@ -5,7 +5,7 @@ curvilinear layout (so every host<->device window starts inside the arrays).
 
 The drop-in library is oracle/_ref/libcodens_dropin.so: the reference harness
 and reference objects with the drop-in modules linked in place of the
-reference's zpres, zvismo, zvisqc, chronos, sch_roe_euler, sch_roe, sch_expli,
+reference's zpres, zvismo, zvisqc, rbord, chronos, sch_roe_euler, sch_roe, sch_expli,
 implimf_3d, implimf_eu, dissip_jameson, met_roe2o and met_roe."""
 import numpy as np
 import pytest

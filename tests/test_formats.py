@@ -1,4 +1,4 @@
-"""On-disk formats (SURVEY row f4): the restart records of writda.f90 /
+"""On-disk formats: the restart records of writda.f90 /
 readda.f90 and the mesh records of writdg.f90 / readdg.f90, written and read
 by the library's host code (codens_restart_*, codens_mesh_*), against the
 reference's own routines (flang build, oracle/_ref): the files are identical

@@ -353,14 +353,28 @@ def tsetup(mesh):
     return lay, sn, vol, v, mu, mut, dist, c, R, G
 
 
+_RESIDENT = [False]
+
+
 def _both(R, G, name, *arrays_and_scalars, outs=()):
     """Call wrapper `name` on the reference and on the drop-in library with
-    copies of the listed numpy arrays; returns the two lists of arrays."""
+    copies of the listed numpy arrays; returns the two lists of arrays.  With
+    the mirrors resident, the drop-in's results are brought back and its
+    arrays forgotten, rows of 2-D arrays included (they are freed, and another
+    array may take an address the library keeps)."""
     from oracle import ptr
     res = []
     for X in (R, G):
         args = [a.copy() if isinstance(a, np.ndarray) else a for a in arrays_and_scalars]
         getattr(X.lib, name)(*[ptr(a) if isinstance(a, np.ndarray) else a for a in args])
+        if _RESIDENT[0] and X is G:
+            from codens_amd import abi
+            abi.host_sync_all()
+            for a in args:  # every array the drop-in may have keyed: a 2-D array's rows are the
+                if isinstance(a, np.ndarray):  # fd5x..fd6z, toxx..qcz arrays of a bundle
+                    abi.host_modified(a)
+                    for row in (a if a.ndim == 2 else ()):
+                        abi.host_modified(row)
         res.append([a for a in args if isinstance(a, np.ndarray)])
     return res
 
@@ -429,6 +443,20 @@ def test_turbulence_routines(tsetup):
         for which in (1, 2):
             r_, g_ = _both(R, G, "ref_met_cut", l + 1, which, D(1e-6), D(1e-4), vv)
             assert np.array_equal(r_[0], g_[0])
+
+
+def test_turbulence_routines_resident(tsetup):
+    """The same routines with the mirrors resident: the turbulence drop-ins'
+    bundles (one bundle key named with 6 and with 3 arrays), plain mirrors and
+    whole-array windows give the copy mode's results."""
+    from codens_amd import abi
+    _RESIDENT[0] = True
+    abi.host_residency(1)
+    try:
+        test_turbulence_routines(tsetup)
+    finally:
+        _RESIDENT[0] = False
+        abi.host_residency(0)
 
 
 @pytest.mark.parametrize("model", [0, 1])

@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(RT *RT) k_impli2_rhs(Blk b, int d, const doubl
   rhs_body(b, xcd_block(), d, u, ldv, w);
 }
 
-constexpr int LU = 8;  // recursion steps per batch of loads
+constexpr int LU = 16;  // recursion steps per batch of loads
 
 // The two recursions of one line, each software-pipelined: the loads of the
 // next LU steps are issued before the arithmetic of the current LU steps, so

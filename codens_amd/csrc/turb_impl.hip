@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(TX *TY) k_impli2_diag(Blk b, ImpliWs w) { diag
 // the line (recursion values zero beyond the last real cell, as the
 // reference's zeroed arrays give) and then the substitution from the low
 // end, which writes the increments dw6, dw7.  The coefficients do not depend
-// on the recursion, so sixteen steps' loads are issued ahead of their
+// on the recursion, so eight steps' loads are issued ahead of their
 // arithmetic.
 constexpr int RT = 16;
 
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(RT *RT) k_impli2_rhs(Blk b, int d, const doubl
   rhs_body(b, xcd_block(), d, u, ldv, w);
 }
 
-constexpr int LU = 16;  // recursion steps per batch of loads
+constexpr int LU = 8;  // recursion steps per batch of loads
 
 // The two recursions of one line, each software-pipelined: the loads of the
 // next LU steps are issued before the arithmetic of the current LU steps, so
